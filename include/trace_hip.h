@@ -1,0 +1,137 @@
+/*
+ * trace_hip.h -- C-ABI of the MI355X (gfx950) rendering hot path.
+ *
+ * The reference (ChSchmidt99/trace, pure Go) has no FFI; its seams are Go
+ * functions whose bodies a cgo shim replaces with the calls below.  Each entry
+ * point names the reference function it stands in for (paths relative to the
+ * reference checkout).
+ *
+ * Conventions
+ *   - Every call returns an int status: TRACE_OK (0) or a negative TRACE_E_*;
+ *     on failure trace_last_error(ctx) holds a message.  (The reference panics
+ *     on errors; the Go shim turns a non-zero status into a panic.)
+ *   - Plain pointers and sizes only.  Pointers named h_* are host memory, d_*
+ *     are device memory.  Host inputs are copied during the call and never
+ *     retained.  Host-pointer calls are synchronous; *_device calls are
+ *     asynchronous on the context's stream (see trace_ctx_set_stream).
+ *   - A context is not thread-safe; distinct contexts may be used concurrently.
+ *   - Triangles are 9 doubles each: v0.xyz, v1.xyz, v2.xyz.
+ *   - Rays are 8 floats each: origin.xyz, tmin, direction.xyz, tmax.  The
+ *     direction is not normalised (the reference never normalises it).
+ */
+#ifndef TRACE_HIP_H
+#define TRACE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct trace_ctx trace_ctx;
+typedef struct trace_mesh trace_mesh;
+typedef struct trace_bvh trace_bvh;
+
+enum {
+  TRACE_OK = 0,
+  TRACE_E_INVALID = -1,     /* bad argument (null handle, n < 0, size mismatch) */
+  TRACE_E_NODEVICE = -2,    /* no HIP device / device index out of range */
+  TRACE_E_HIP = -3,         /* a HIP runtime call failed */
+  TRACE_E_OOM = -4,         /* device allocation failed */
+  TRACE_E_UNSUPPORTED = -5, /* input outside what this build supports */
+  TRACE_E_DEVICE = -6       /* a kernel reported an internal error (bounded wait expired, stack overflow) */
+};
+
+/* Morton key modes for the LBVH build.
+ * REF63: the reference's 63-bit code (21 bits per axis, lbvh.go:64-73,
+ *        morton.go:6-20), sorted in full; topology equals the reference's.
+ * M30:   code63 >> 33 (10 bits per axis); the reference tree truncated at
+ *        bit 33, half the radix passes. */
+enum { TRACE_KEY_REF63 = 0, TRACE_KEY_M30 = 1 };
+
+/* Miss shaders (render.go:228-246). */
+enum { TRACE_MISS_DEFAULT = 0, TRACE_MISS_WHITE = 1, TRACE_MISS_SKY = 2, TRACE_MISS_SUN = 3 };
+
+/* Material kinds (material.go:13-42). */
+enum { TRACE_MAT_DIFFUSE = 0, TRACE_MAT_LIGHT = 1 };
+
+int trace_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+int trace_ctx_create(int device, trace_ctx** out);
+int trace_ctx_destroy(trace_ctx* ctx);
+/* Message of the last failed call on ctx ("" if none).  Valid until the next call. */
+const char* trace_last_error(const trace_ctx* ctx);
+/* Use an external hipStream_t for all work of ctx (NULL restores the ctx's own stream). */
+int trace_ctx_set_stream(trace_ctx* ctx, void* hip_stream);
+int trace_ctx_synchronize(trace_ctx* ctx);
+
+/* ---- geometry ----------------------------------------------------------- */
+/* Device-resident triangle soup in caller order.  Replaces the marshalling of
+ * Scene.collectTracables (scene.go:100-114) + NewTriangle (primitive.go:102-114). */
+int trace_mesh_create(trace_ctx* ctx, const double* h_pos, int64_t ntri, trace_mesh** out);
+/* Same from device memory (9 doubles per triangle), asynchronous. */
+int trace_mesh_create_device(trace_ctx* ctx, const double* d_pos, int64_t ntri, trace_mesh** out);
+/* Replace the positions of an existing mesh (same ntri); used for per-frame rebuilds
+ * (cmd/dynamic/main.go:113-117). */
+int trace_mesh_update(trace_ctx* ctx, trace_mesh* mesh, const double* h_pos);
+int trace_mesh_destroy(trace_mesh* mesh);
+
+/* ---- LBVH build --------------------------------------------------------- */
+/* LBVH(prims, enclosing, threads) (lbvh.go:17-25); DefaultLBVH (lbvh.go:13-15)
+ * when h_enclosing is NULL.  h_enclosing = {minx,miny,minz,maxx,maxy,maxz}
+ * is used only for the Morton quantisation, as in the reference. */
+int trace_lbvh_build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing,
+                     int key_mode, trace_bvh** out);
+/* Rebuild into an existing LBVH handle, reusing its device buffers (grow-only);
+ * asynchronous on the context stream, no device allocation once the buffers fit.
+ * The per-frame rebuild of cmd/dynamic/main.go:113-117. */
+int trace_lbvh_rebuild(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing,
+                       int key_mode, trace_bvh* bvh);
+int trace_bvh_destroy(trace_bvh* bvh);
+
+typedef struct {
+  int64_t n_prims;   /* triangles */
+  int64_t n_leaves;  /* leaves (LBVH: unique Morton keys) */
+  int64_t n_inner;   /* inner nodes */
+  int32_t depth;     /* inner-node levels on the longest root-to-leaf path */
+  int32_t key_mode;  /* TRACE_KEY_* or -1 for a PHR tree */
+  int32_t arity;     /* maximum children per inner node */
+  int32_t pad;
+  double root_box[6];
+} trace_bvh_info;
+int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out);
+
+/* Sorted Morton keys and the sort permutation (sorted position -> triangle index). */
+int trace_bvh_export_sorted(trace_ctx* ctx, const trace_bvh* bvh, uint64_t* h_keys, int32_t* h_perm);
+
+/* Canonical preorder serialisation (children in stored order, children[0] first),
+ * 4 int32 per node: kind (0 inner, 1 leaf), first, last (leaf: sorted-position range
+ * of its triangles, inclusive; inner: covered leaf range), n_children; and 6 floats per
+ * node: the node's AABB.  Returns the node count in *n_out; fails with TRACE_E_INVALID
+ * if cap is too small. */
+int trace_bvh_export_canonical(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap,
+                               int32_t* h_rec, float* h_box, int64_t* n_out);
+
+/* BVH.Cost() (bvh.go:46-48, 226-239) and BVH.Size() (bvh.go:50-52, 159-167). */
+int trace_bvh_cost(trace_ctx* ctx, const trace_bvh* bvh, double* cost);
+int trace_bvh_size(trace_ctx* ctx, const trace_bvh* bvh, int64_t* size);
+
+/* ---- closest hit -------------------------------------------------------- */
+/* Bulk BVH.intersected (bvh.go:74-77, 179-199 + primitive.go:172-208): per ray the
+ * closest triangle with t in [tmin, tmax]; hit = triangle index or -1, t = hit
+ * distance (tmax on a miss).  Equal-t ties resolve to the triangle the reference's
+ * depth-first order visits last. */
+int trace_closest_hit(trace_ctx* ctx, const trace_bvh* bvh, const float* h_rays, int64_t nrays,
+                      int32_t* h_hit, float* h_t);
+int trace_closest_hit_device(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays,
+                             int64_t nrays, int32_t* d_hit, float* d_t);
+/* Counting mode: per ray, inner nodes visited and triangles tested (for byte accounting). */
+int trace_closest_hit_counts(trace_ctx* ctx, const trace_bvh* bvh, const float* h_rays,
+                             int64_t nrays, int32_t* h_nodes, int32_t* h_tris);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRACE_HIP_H */

@@ -1,0 +1,434 @@
+// ORACLE -- test infrastructure only.
+//
+// A CPU, FP64 restatement of the reference's hot path (ChSchmidt99/trace,
+// pkg/pt).  It exists to check the HIP library: only tests/, the smoke check
+// and bench.py's CPU-baseline leg load it, and never as the thing measured or
+// shipped.  Every function cites the reference lines it restates.
+//
+// Deliberate, documented differences from the reference (all are places where
+// the reference's own result is not defined by its source):
+//   - the Morton sort is stable (ties keep triangle index order); the
+//     reference's sort.Slice is unstable (lbvh.go:298), and equal keys land in
+//     one leaf anyway (lbvh.go:134-145);
+//   - a zero-extent axis quantises to 0 (Go's uint64(NaN) is platform defined);
+//   - a one-triangle scene yields a root leaf (the reference dereferences a nil
+//     parent, bvh.go:202-206);
+//   - the closest-hit result carries the triangle index, which the reference's
+//     hit record does not have (primitive.go:7-13).
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off), output oracle/_build/.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct V3 {
+  double x, y, z;
+};
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }  // math.go Dot
+inline V3 cross(V3 a, V3 b) {                                                // math.go Cross
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// util.go:47-66
+inline double min3(double a, double b, double c) {
+  if (a <= b && a <= c) return a;
+  if (b <= a && b <= c) return b;
+  return c;
+}
+inline double max3(double a, double b, double c) {
+  if (a >= b && a >= c) return a;
+  if (b >= a && b >= c) return b;
+  return c;
+}
+
+struct Box {
+  double lo[3], hi[3];
+};
+
+// primitive.go:102-114 (box of a triangle)
+inline Box tri_box(const double* p) {
+  Box b;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = min3(p[a], p[3 + a], p[6 + a]);
+    b.hi[a] = max3(p[a], p[3 + a], p[6 + a]);
+  }
+  return b;
+}
+
+// aabb.add via MinVec/MaxVec (bounding.go:49-51, util.go:69-100)
+inline Box box_union(const Box& a, const Box& b) {
+  Box r;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = a.lo[k] > b.lo[k] ? b.lo[k] : a.lo[k];
+    r.hi[k] = a.hi[k] < b.hi[k] ? b.hi[k] : a.hi[k];
+  }
+  return r;
+}
+
+// bounding.go:45-47
+inline double surface(const Box& b) {
+  const double w = b.hi[0] - b.lo[0], h = b.hi[1] - b.lo[1], d = b.hi[2] - b.lo[2];
+  return 2 * w * h + 2 * w * d + 2 * h * d;
+}
+
+// morton.go:12-20
+inline uint64_t split_by3(uint64_t a) {
+  uint64_t x = a & 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffULL;
+  x = (x | x << 16) & 0x1f0000ff0000ffULL;
+  x = (x | x << 8) & 0x100f00f00f00f00fULL;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+  x = (x | x << 2) & 0x1249249249249249ULL;
+  return x;
+}
+
+// lbvh.go:64-73 with the centroid of bounding.go:36-43
+inline uint64_t morton(const double* p, const Box& enc) {
+  const Box b = tri_box(p);
+  uint64_t q[3];
+  for (int a = 0; a < 3; ++a) {
+    const double c = (b.lo[a] + b.hi[a]) * (1.0 / 2.0);
+    const double delta = std::fabs(enc.lo[a] - c);
+    const double extent = enc.hi[a] - enc.lo[a];
+    const double f = delta / (extent / double(2097152 - 1));
+    q[a] = (f >= 0.0) ? uint64_t(f) : 0;  // NaN (zero extent) -> 0
+  }
+  return split_by3(q[0]) | split_by3(q[1]) << 1 | split_by3(q[2]) << 2;  // morton.go:6-10
+}
+
+struct Node {
+  bool leaf;
+  int32_t first, last;  // sorted-position range (inclusive)
+  int32_t child[2];
+  Box box;
+};
+
+struct Tree {
+  int64_t n = 0;
+  std::vector<double> pos;
+  std::vector<uint64_t> keys;  // sorted
+  std::vector<int32_t> perm;   // sorted position -> triangle index
+  std::vector<Node> nodes;
+  int32_t root = -1;
+};
+
+// lbvh.go:184-203 findSplit (binary search for the first key with mask set)
+int64_t find_split(const uint64_t* k, int64_t len, uint64_t mask) {
+  if ((k[0] & mask) > 0) return 0;
+  int64_t l = 0, r = len - 1;
+  while (l <= r) {
+    const int64_t m = (l + r) / 2;
+    if ((k[m] & mask) == 0) {
+      l = m + 1;
+    } else if ((k[m] & mask) > 0 && (k[m - 1] & mask) > 0) {
+      r = m - 1;
+    } else {
+      return m;
+    }
+  }
+  return len;
+}
+
+// lbvh.go:133-177 lbvhJob.process (recursive form of the job queue)
+int32_t build_range(Tree& t, int64_t b, int64_t e, uint64_t mask) {
+  const uint64_t* k = t.keys.data();
+  while (true) {
+    if (k[b] == k[e - 1]) {  // isLeaf, lbvh.go:179-181
+      Node nd{};
+      nd.leaf = true;
+      nd.first = int32_t(b);
+      nd.last = int32_t(e - 1);
+      nd.child[0] = nd.child[1] = -1;
+      t.nodes.push_back(nd);
+      return int32_t(t.nodes.size() - 1);
+    }
+    const int64_t s = find_split(k + b, e - b, mask);
+    if (s == 0 || s == e - b) {
+      mask >>= 1;  // no node for an empty split (lbvh.go:150-159)
+      continue;
+    }
+    Node nd{};
+    nd.leaf = false;
+    nd.first = int32_t(b);
+    nd.last = int32_t(e - 1);
+    t.nodes.push_back(nd);
+    const int32_t id = int32_t(t.nodes.size() - 1);
+    const int32_t c0 = build_range(t, b, b + s, mask >> 1);
+    const int32_t c1 = build_range(t, b + s, e, mask >> 1);
+    t.nodes[id].child[0] = c0;
+    t.nodes[id].child[1] = c1;
+    return id;
+  }
+}
+
+// bvh.go:201-222 (values only; the union is exact so order does not matter)
+Box refit(Tree& t, int32_t id) {
+  Node& nd = t.nodes[id];
+  if (nd.leaf) {
+    Box b = tri_box(&t.pos[9 * size_t(t.perm[nd.first])]);
+    for (int32_t i = nd.first + 1; i <= nd.last; ++i)
+      b = box_union(b, tri_box(&t.pos[9 * size_t(t.perm[i])]));  // enclosingSlice
+    nd.box = b;
+    return b;
+  }
+  const Box b0 = refit(t, nd.child[0]);
+  const Box b1 = refit(t, nd.child[1]);
+  t.nodes[id].box = box_union(b0, b1);
+  return t.nodes[id].box;
+}
+
+// bvh.go:226-239
+double cost_of(const Tree& t, int32_t id) {
+  const Node& nd = t.nodes[id];
+  if (nd.leaf) return 1.0 * double(nd.last - nd.first + 1);
+  double acc = 0.0;
+  const double sa = surface(nd.box);
+  for (int c = 0; c < 2; ++c) {
+    if (sa != 0) acc += surface(t.nodes[nd.child[c]].box) / sa * cost_of(t, nd.child[c]);
+  }
+  return 2.0 + acc;
+}
+
+// ---- traversal ------------------------------------------------------------
+struct Ray {
+  V3 o, d, inv;
+  int sign[3];
+};
+
+// camera.go:113-135 newRay
+Ray make_ray(const float* r) {
+  Ray ray;
+  ray.o = {r[0], r[1], r[2]};
+  ray.d = {r[4], r[5], r[6]};
+  ray.inv = {1.0 / ray.d.x, 1.0 / ray.d.y, 1.0 / ray.d.z};
+  ray.sign[0] = ray.inv.x < 0;
+  ray.sign[1] = ray.inv.y < 0;
+  ray.sign[2] = ray.inv.z < 0;
+  return ray;
+}
+
+// bounding.go:53-91 (Williams et al. slab test, NaN never rejects)
+bool box_hit(const Box& b, const Ray& r, double tmin, double tmax) {
+  const double* bx[2] = {b.lo, b.hi};
+  double txmin = (bx[r.sign[0]][0] - r.o.x) * r.inv.x;
+  double txmax = (bx[1 - r.sign[0]][0] - r.o.x) * r.inv.x;
+  const double tymin = (bx[r.sign[1]][1] - r.o.y) * r.inv.y;
+  const double tymax = (bx[1 - r.sign[1]][1] - r.o.y) * r.inv.y;
+  if (txmin > tymax || tymin > txmax) return false;
+  if (tymin > txmin) txmin = tymin;
+  if (tymax < txmax) txmax = tymax;
+  const double tzmin = (bx[r.sign[2]][2] - r.o.z) * r.inv.z;
+  const double tzmax = (bx[1 - r.sign[2]][2] - r.o.z) * r.inv.z;
+  if (txmin > tzmax || tzmin > txmax) return false;
+  if (tzmin > txmin) txmin = tzmin;
+  if (tzmax < txmax) txmax = tzmax;
+  if (txmax < tmin || txmin > tmax) return false;
+  return true;
+}
+
+struct HitRec {
+  double t;
+  int32_t prim;  // triangle index
+  double u, v;
+  bool front;
+};
+
+// primitive.go:172-208 Moeller-Trumbore, util.go:10-12 ApproxZero
+bool tri_hit(const double* p, const Ray& r, double tmin, double tmax, HitRec& h, int32_t prim) {
+  const V3 v0{p[0], p[1], p[2]};
+  const V3 e1 = sub(V3{p[3], p[4], p[5]}, v0);
+  const V3 e2 = sub(V3{p[6], p[7], p[8]}, v0);
+  const V3 pv = cross(r.d, e2);
+  const double det = dot(e1, pv);
+  if (std::fabs(det) <= 1e-7) return false;
+  const double inv = 1 / det;
+  const V3 tv = sub(r.o, v0);
+  const double u = dot(tv, pv) * inv;
+  if (u < 0 || u > 1) return false;
+  const V3 q = cross(tv, e1);
+  const double v = dot(r.d, q) * inv;
+  if (v < 0 || u + v > 1) return false;
+  const double t = dot(e2, q) * inv;
+  if (t < tmin || t > tmax) return false;
+  h.t = t;
+  h.prim = prim;
+  h.u = u;
+  h.v = v;
+  h.front = det > 0;
+  return true;
+}
+
+// bvh.go:179-199 (recursive, children in stored order; each accepted hit
+// tightens tMax for the siblings that follow)
+bool node_hit(const Tree& t, int32_t id, const Ray& r, double tmin, double tmax, HitRec& h) {
+  const Node& nd = t.nodes[id];
+  if (!box_hit(nd.box, r, tmin, tmax)) return false;
+  bool did = false;
+  if (nd.leaf) {
+    for (int32_t i = nd.first; i <= nd.last; ++i) {
+      const int32_t prim = t.perm[i];
+      if (tri_hit(&t.pos[9 * size_t(prim)], r, tmin, h.t, h, prim)) did = true;
+    }
+  } else {
+    for (int c = 0; c < 2; ++c)
+      if (node_hit(t, nd.child[c], r, tmin, h.t, h)) did = true;
+  }
+  return did;
+}
+
+template <typename F>
+void parallel_for(int64_t n, int threads, F f) {
+  if (threads <= 1 || n < 1024) {
+    f(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> ts;
+  const int64_t chunk = (n + threads - 1) / threads;
+  for (int i = 0; i < threads; ++i) {
+    const int64_t b = i * chunk, e = std::min<int64_t>(n, b + chunk);
+    if (b >= e) break;
+    ts.emplace_back([=] { f(b, e); });
+  }
+  for (auto& th : ts) th.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct Tree oracle_tree;
+
+uint64_t oracle_split_by3(uint64_t a) { return split_by3(a); }
+
+// bounding.go:19-25
+void oracle_enclosing(const double* pos, int64_t n, double* out6) {
+  Box b = tri_box(pos);
+  for (int64_t i = 1; i < n; ++i) b = box_union(b, tri_box(pos + 9 * i));
+  for (int a = 0; a < 3; ++a) {
+    out6[a] = b.lo[a];
+    out6[3 + a] = b.hi[a];
+  }
+}
+
+// lbvh.go:33-73: code per triangle; key_mode 1 keeps code >> 33 (30 bits)
+void oracle_morton(const double* pos, int64_t n, const double* qbox6, int key_mode,
+                   uint64_t* keys) {
+  Box enc;
+  for (int a = 0; a < 3; ++a) {
+    enc.lo[a] = qbox6[a];
+    enc.hi[a] = qbox6[3 + a];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t c = morton(pos + 9 * i, enc);
+    keys[i] = key_mode == 1 ? (c >> 33) : c;
+  }
+}
+
+// lbvh.go:216-301 result: pairs ascending by code (stable)
+void oracle_sort(const uint64_t* keys, int64_t n, uint64_t* sorted, int32_t* perm) {
+  std::vector<int32_t> idx(n);
+  for (int64_t i = 0; i < n; ++i) idx[i] = int32_t(i);
+  std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return keys[a] < keys[b]; });
+  for (int64_t i = 0; i < n; ++i) {
+    perm[i] = idx[i];
+    sorted[i] = keys[idx[i]];
+  }
+}
+
+// LBVH (lbvh.go:17-25): Morton codes, sort, top-down split from bit 62, refit.
+oracle_tree* oracle_lbvh_build(const double* pos, int64_t n, const double* enclosing6,
+                               int key_mode) {
+  if (n <= 0) return nullptr;
+  Tree* t = new Tree();
+  t->n = n;
+  t->pos.assign(pos, pos + 9 * n);
+  double box[6];
+  if (enclosing6) std::memcpy(box, enclosing6, sizeof(box));
+  else oracle_enclosing(pos, n, box);
+  std::vector<uint64_t> codes(n);
+  oracle_morton(pos, n, box, key_mode, codes.data());
+  t->keys.resize(n);
+  t->perm.resize(n);
+  oracle_sort(codes.data(), n, t->keys.data(), t->perm.data());
+  t->nodes.reserve(2 * n);
+  t->root = build_range(*t, 0, n, uint64_t(1) << 62);
+  refit(*t, t->root);
+  return t;
+}
+
+void oracle_tree_free(oracle_tree* t) { delete t; }
+
+int64_t oracle_tree_node_count(const oracle_tree* t) { return int64_t(t->nodes.size()); }
+
+void oracle_tree_sorted(const oracle_tree* t, uint64_t* keys, int32_t* perm) {
+  std::memcpy(keys, t->keys.data(), t->n * sizeof(uint64_t));
+  std::memcpy(perm, t->perm.data(), t->n * sizeof(int32_t));
+}
+
+// Preorder, children[0] first: rec = kind, first, last, n_children; box FP64.
+void oracle_tree_canonical(const oracle_tree* t, int32_t* rec, double* box) {
+  std::vector<int32_t> st{t->root};
+  int64_t k = 0;
+  while (!st.empty()) {
+    const int32_t id = st.back();
+    st.pop_back();
+    const Node& nd = t->nodes[id];
+    rec[4 * k + 0] = nd.leaf ? 1 : 0;
+    rec[4 * k + 1] = nd.first;
+    rec[4 * k + 2] = nd.last;
+    rec[4 * k + 3] = nd.leaf ? 0 : 2;
+    for (int a = 0; a < 3; ++a) {
+      box[6 * k + a] = nd.box.lo[a];
+      box[6 * k + 3 + a] = nd.box.hi[a];
+    }
+    if (!nd.leaf) {
+      st.push_back(nd.child[1]);
+      st.push_back(nd.child[0]);
+    }
+    ++k;
+  }
+}
+
+double oracle_tree_cost(const oracle_tree* t) { return cost_of(*t, t->root); }
+
+// BVH.intersected (bvh.go:74-77) for a batch of rays (8 floats each); t is FP64.
+void oracle_closest_hit(const oracle_tree* t, const float* rays, int64_t nrays, int32_t* hit,
+                        double* tout, int threads) {
+  parallel_for(nrays, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const float* rr = rays + 8 * i;
+      const Ray r = make_ray(rr);
+      HitRec h{};
+      h.t = rr[7];
+      h.prim = -1;
+      node_hit(*t, t->root, r, rr[3], rr[7], h);
+      hit[i] = h.prim;
+      tout[i] = h.t;
+    }
+  });
+}
+
+// All triangles in index order, no tree (pins the traversal on small inputs).
+void oracle_closest_hit_brute(const double* pos, int64_t n, const float* rays, int64_t nrays,
+                              int32_t* hit, double* tout, int threads) {
+  parallel_for(nrays, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const float* rr = rays + 8 * i;
+      const Ray r = make_ray(rr);
+      HitRec h{};
+      h.t = rr[7];
+      h.prim = -1;
+      for (int64_t k = 0; k < n; ++k) tri_hit(pos + 9 * k, r, rr[3], h.t, h, int32_t(k));
+      hit[i] = h.prim;
+      tout[i] = h.t;
+    }
+  });
+}
+
+}  // extern "C"

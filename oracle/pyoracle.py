@@ -1,0 +1,144 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes access to the FP64 CPU restatement in oracle.cpp.  Only tests/, the
+smoke check and bench.py's CPU-baseline leg import this module; the product
+library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_P = C.c_void_p
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "oracle_split_by3": ([C.c_uint64], C.c_uint64),
+        "oracle_enclosing": ([_P, C.c_int64, _P], None),
+        "oracle_morton": ([_P, C.c_int64, _P, C.c_int, _P], None),
+        "oracle_sort": ([_P, C.c_int64, _P, _P], None),
+        "oracle_lbvh_build": ([_P, C.c_int64, _P, C.c_int], _P),
+        "oracle_tree_free": ([_P], None),
+        "oracle_tree_node_count": ([_P], C.c_int64),
+        "oracle_tree_sorted": ([_P, _P, _P], None),
+        "oracle_tree_canonical": ([_P, _P, _P], None),
+        "oracle_tree_cost": ([_P], C.c_double),
+        "oracle_closest_hit": ([_P, _P, C.c_int64, _P, _P, C.c_int], None),
+        "oracle_closest_hit_brute": ([_P, C.c_int64, _P, C.c_int64, _P, _P, C.c_int], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def _threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def split_by3(a: int) -> int:
+    return int(load().oracle_split_by3(a))
+
+
+def enclosing(pos: np.ndarray) -> np.ndarray:
+    p = np.ascontiguousarray(pos, np.float64)
+    out = np.empty(6, np.float64)
+    load().oracle_enclosing(_p(p), p.shape[0], _p(out))
+    return out
+
+
+def morton(pos: np.ndarray, qbox: np.ndarray, key_mode: int = 0) -> np.ndarray:
+    p = np.ascontiguousarray(pos, np.float64)
+    q = np.ascontiguousarray(qbox, np.float64)
+    keys = np.empty(p.shape[0], np.uint64)
+    load().oracle_morton(_p(p), p.shape[0], _p(q), key_mode, _p(keys))
+    return keys
+
+
+def stable_sort(keys: np.ndarray):
+    k = np.ascontiguousarray(keys, np.uint64)
+    s = np.empty_like(k)
+    perm = np.empty(k.shape[0], np.int32)
+    load().oracle_sort(_p(k), k.shape[0], _p(s), _p(perm))
+    return s, perm
+
+
+class Tree:
+    """Reference LBVH (top-down split, FP64 boxes) built by the oracle."""
+
+    def __init__(self, pos: np.ndarray, enclosing_box=None, key_mode: int = 0):
+        self.lib = load()
+        self.pos = np.ascontiguousarray(pos, np.float64)
+        enc = None if enclosing_box is None else np.ascontiguousarray(enclosing_box, np.float64)
+        self.h = self.lib.oracle_lbvh_build(_p(self.pos), self.pos.shape[0], None if enc is None else _p(enc),
+                                            key_mode)
+        if not self.h:
+            raise ValueError("oracle LBVH needs at least one triangle")
+        self.n = self.pos.shape[0]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_tree_free(self.h)
+            self.h = None
+
+    def node_count(self) -> int:
+        return int(self.lib.oracle_tree_node_count(self.h))
+
+    def sorted(self):
+        keys = np.empty(self.n, np.uint64)
+        perm = np.empty(self.n, np.int32)
+        self.lib.oracle_tree_sorted(self.h, _p(keys), _p(perm))
+        return keys, perm
+
+    def canonical(self):
+        n = self.node_count()
+        rec = np.empty((n, 4), np.int32)
+        box = np.empty((n, 6), np.float64)
+        self.lib.oracle_tree_canonical(self.h, _p(rec), _p(box))
+        return rec, box
+
+    def cost(self) -> float:
+        return float(self.lib.oracle_tree_cost(self.h))
+
+    def closest_hit(self, rays: np.ndarray, threads: int | None = None):
+        r = np.ascontiguousarray(rays, np.float32)
+        n = r.shape[0]
+        hit = np.empty(n, np.int32)
+        t = np.empty(n, np.float64)
+        self.lib.oracle_closest_hit(self.h, _p(r), n, _p(hit), _p(t), threads or _threads())
+        return hit, t
+
+
+def closest_hit_brute(pos: np.ndarray, rays: np.ndarray, threads: int | None = None):
+    p = np.ascontiguousarray(pos, np.float64)
+    r = np.ascontiguousarray(rays, np.float32)
+    n = r.shape[0]
+    hit = np.empty(n, np.int32)
+    t = np.empty(n, np.float64)
+    load().oracle_closest_hit_brute(_p(p), p.shape[0], _p(r), n, _p(hit), _p(t), threads or _threads())
+    return hit, t

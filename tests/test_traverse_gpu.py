@@ -1,0 +1,131 @@
+"""GPU parity of closest-hit traversal against the FP64 oracle.
+
+Bar: hit triangle IDs agree on >= 99.99 % of rays; where both hit the same
+triangle, t agrees within 1e-5 relative.  Mismatches can only come from FP32
+vs FP64 rounding at triangle edges or the |det| <= 1e-7 boundary."""
+import math
+
+import numpy as np
+import pytest
+
+from trace_amd import KEY_M30, KEY_REF63, scenes
+from trace_amd.device import Bvh, Mesh
+
+pytestmark = pytest.mark.gpu
+
+HIT_AGREE = 0.9999
+T_REL = 1e-5
+
+
+def _check(hit, t, ohit, ot, min_hits=0.1):
+    agree = np.mean(hit == ohit)
+    assert agree >= HIT_AGREE, f"hit agreement {agree:.6f}"
+    both = (hit == ohit) & (hit >= 0)
+    rel = np.abs(t[both].astype(np.float64) - ot[both]) / np.maximum(np.abs(ot[both]), 1e-30)
+    assert rel.size == 0 or rel.max() <= T_REL, f"max rel t error {rel.max():.3g}"
+    miss = hit < 0
+    assert np.all(np.isinf(t[miss]) | (t[miss] > 0))
+    assert np.mean(hit >= 0) >= min_hits
+
+
+@pytest.mark.parametrize("key_mode", [KEY_REF63, KEY_M30])
+def test_incoherent_small(ctx, oracle, key_mode):
+    print(f"traverse incoherent 20k tris mode={key_mode}", flush=True)
+    pos = scenes.random_triangles(20_000, seed=21, mean_edge=0.02)
+    rays = scenes.incoherent_rays(50_000, seed=22)
+    mesh = Mesh(ctx, pos)
+    bvh = Bvh.lbvh(ctx, mesh, key_mode=key_mode)
+    hit, t = bvh.closest_hit(rays)
+    ohit, ot = oracle.Tree(pos, key_mode=key_mode).closest_hit(rays)
+    _check(hit, t, ohit, ot)
+
+
+def test_against_brute_force(ctx, oracle):
+    print("traverse vs brute force", flush=True)
+    pos = scenes.random_triangles(2000, seed=23, mean_edge=0.05)
+    rays = scenes.incoherent_rays(4000, seed=24)
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    hit, t = bvh.closest_hit(rays)
+    ohit, ot = oracle.closest_hit_brute(pos, rays)
+    _check(hit, t, ohit, ot)
+
+
+def test_edge_cases(ctx):
+    print("traverse edge cases", flush=True)
+    tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0], [0, 0, -1, 1, 0, -1, 0, 1, -1]], np.float64)
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, tri))
+    inf = math.inf
+    rays = np.array([
+        [0.25, 0.25, 2.0, 0.001, 0, 0, -1, inf],    # hits z=0 at t=2
+        [0.25, 0.25, 2.0, 0.001, 0, 0, -4, inf],    # unnormalised: t=0.5
+        [0.75, 0.75, 2.0, 0.001, 0, 0, -1, inf],    # outside both
+        [0.25, 0.25, 2.0, 0.001, 0, 0, -1, 2.0],    # t == tmax accepted
+        [0.25, 0.25, 2.0, 0.001, 0, 0, -1, 1.999],  # t > tmax rejected
+        [0.25, 0.25, -2.0, 0.001, 0, 0, -1, inf],   # behind
+        [0.25, 0.25, -2.0, 0.001, 0, 0, 1, inf],    # from below: z=-1 first, t=1
+        [0.25, 0.25, 2.0, 0.001, 1, 0, 0, inf],     # parallel to the plane
+    ], np.float32)
+    hit, t = bvh.closest_hit(rays)
+    assert hit.tolist() == [0, 0, -1, 0, -1, -1, 1, -1]
+    assert t[0] == 2.0 and t[1] == 0.5 and t[6] == 1.0
+    assert np.isinf(t[2]) and t[4] == np.float32(1.999)
+
+
+def test_tie_goes_to_reference_winner(ctx, oracle):
+    print("traverse tie rule", flush=True)
+    tri = [0, 0, 0, 1, 0, 0, 0, 1, 0]
+    pos = np.array([tri, tri, tri], np.float64)
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    ray = np.array([[0.25, 0.25, 1.0, 0.001, 0, 0, -1, math.inf]], np.float32)
+    hit, _ = bvh.closest_hit(ray)
+    ohit, _ = oracle.Tree(pos).closest_hit(ray)
+    assert hit[0] == ohit[0] == 2  # last in sorted (stable) order
+
+
+def test_empty_ray_batch(ctx):
+    pos = scenes.random_triangles(10, seed=1)
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    hit, t = bvh.closest_hit(np.zeros((0, 8), np.float32))
+    assert hit.size == 0 and t.size == 0
+
+
+@pytest.mark.parametrize("key_mode", [KEY_REF63, KEY_M30])
+def test_config2_sampled(ctx, oracle, key_mode):
+    """1M triangles; 200k incoherent + 200k coherent rays checked ray by ray."""
+    print(f"traverse 1M sampled mode={key_mode}", flush=True)
+    pos = scenes.random_triangles(1_000_000, seed=1)
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, pos), key_mode=key_mode)
+    tree = oracle.Tree(pos, key_mode=key_mode)
+    inc = scenes.incoherent_rays(200_000, seed=2)
+    hit, t = bvh.closest_hit(inc)
+    ohit, ot = tree.closest_hit(inc)
+    _check(hit, t, ohit, ot, min_hits=0.9)
+    coh = scenes.coherent_rays_cfg2(4096, 2048)[::41]
+    hit, t = bvh.closest_hit(coh)
+    ohit, ot = tree.closest_hit(coh)
+    _check(hit, t, ohit, ot, min_hits=0.3)
+
+
+def test_config2_full_batch_properties(ctx):
+    """All 8M incoherent rays: every reported hit re-intersects its triangle at
+    the reported t in FP64, and t >= tmin."""
+    print("traverse 8M properties", flush=True)
+    pos = scenes.random_triangles(1_000_000, seed=1)
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    rays = scenes.incoherent_rays(8_000_000, seed=2)
+    hit, t = bvh.closest_hit(rays)
+    h = hit >= 0
+    assert h.mean() > 0.9
+    assert np.all(t[h] >= rays[h, 3])
+    sel = np.flatnonzero(h)[::97]
+    tri = pos[hit[sel]].reshape(-1, 3, 3)
+    o = rays[sel, 0:3].astype(np.float64)
+    d = rays[sel, 4:7].astype(np.float64)
+    e1 = tri[:, 1] - tri[:, 0]
+    e2 = tri[:, 2] - tri[:, 0]
+    p = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, p)
+    tv = o - tri[:, 0]
+    q = np.cross(tv, e1)
+    tt = np.einsum("ij,ij->i", e2, q) / det
+    assert np.max(np.abs(tt - t[sel]) / np.abs(tt)) < T_REL

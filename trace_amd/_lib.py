@@ -1,0 +1,106 @@
+"""ctypes binding of ``libtrace_hip.so`` (the C-ABI in include/trace_hip.h).
+
+Loading fails loudly if the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtrace_hip.so")
+
+TRACE_OK = 0
+TRACE_E_INVALID = -1
+TRACE_E_NODEVICE = -2
+TRACE_E_HIP = -3
+TRACE_E_OOM = -4
+TRACE_E_UNSUPPORTED = -5
+TRACE_E_DEVICE = -6
+
+KEY_REF63 = 0
+KEY_M30 = 1
+
+MISS_DEFAULT, MISS_WHITE, MISS_SKY, MISS_SUN = 0, 1, 2, 3
+MAT_DIFFUSE, MAT_LIGHT = 0, 1
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_I32 = C.c_int32
+_DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+
+
+class TraceError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"trace error {code}: {msg}")
+        self.code = code
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [
+        ("n_prims", C.c_int64),
+        ("n_leaves", C.c_int64),
+        ("n_inner", C.c_int64),
+        ("depth", C.c_int32),
+        ("key_mode", C.c_int32),
+        ("arity", C.c_int32),
+        ("pad", C.c_int32),
+        ("root_box", C.c_double * 6),
+    ]
+
+
+# name -> argtypes; every entry returns int unless listed in _RESTYPES
+SIGNATURES = {
+    "trace_version": [],
+    "trace_ctx_create": [C.c_int, C.POINTER(_P)],
+    "trace_ctx_destroy": [_P],
+    "trace_last_error": [_P],
+    "trace_ctx_set_stream": [_P, _P],
+    "trace_ctx_synchronize": [_P],
+    "trace_mesh_create": [_P, _P, _I64, C.POINTER(_P)],
+    "trace_mesh_create_device": [_P, _P, _I64, C.POINTER(_P)],
+    "trace_mesh_update": [_P, _P, _P],
+    "trace_mesh_destroy": [_P],
+    "trace_lbvh_build": [_P, _P, _P, C.c_int, C.POINTER(_P)],
+    "trace_lbvh_rebuild": [_P, _P, _P, C.c_int, _P],
+    "trace_bvh_destroy": [_P],
+    "trace_bvh_get_info": [_P, _P, C.POINTER(BvhInfo)],
+    "trace_bvh_export_sorted": [_P, _P, _P, _P],
+    "trace_bvh_export_canonical": [_P, _P, _I64, _P, _P, C.POINTER(_I64)],
+    "trace_bvh_cost": [_P, _P, _DP],
+    "trace_bvh_size": [_P, _P, C.POINTER(_I64)],
+    "trace_closest_hit": [_P, _P, _P, _I64, _P, _P],
+    "trace_closest_hit_device": [_P, _P, _P, _I64, _P, _P],
+    "trace_closest_hit_counts": [_P, _P, _P, _I64, _P, _P],
+}
+_RESTYPES = {"trace_last_error": C.c_char_p}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (once).  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"trace_amd: {LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != TRACE_OK:
+        msg = ""
+        if ctx is not None:
+            m = load().trace_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise TraceError(rc, msg)

@@ -1,0 +1,434 @@
+// C-ABI entry points (include/trace_hip.h).
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "lbvh.hpp"
+#include "traverse.hpp"
+
+namespace {
+
+bool valid_key_mode(int m) { return m == TRACE_KEY_REF63 || m == TRACE_KEY_M30; }
+
+}  // namespace
+
+int ctx_reserve_scratch(trace_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->scratch_bytes) return TRACE_OK;
+  if (ctx->scratch) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  const size_t want = std::max(bytes, size_t(1) << 20);
+  HIP_TRY(ctx, hipMalloc(&ctx->scratch, want));
+  ctx->scratch_bytes = want;
+  return TRACE_OK;
+}
+
+int ctx_check_device_error(trace_ctx* ctx, const char* what) {
+  uint32_t e = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&e, ctx->d_err, sizeof(e), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (e) {
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    TRACE_FAIL(ctx, TRACE_E_DEVICE, "%s: device reported error word 0x%x (%s%s%s)", what, e,
+               (e & TRACE_ERR_SPIN) ? "bounded wait expired; " : "",
+               (e & TRACE_ERR_STACK) ? "traversal stack overflow; " : "",
+               (e & TRACE_ERR_TREE) ? "malformed tree reference" : "");
+  }
+  return TRACE_OK;
+}
+
+extern "C" {
+
+int trace_version(void) { return 1; }
+
+int trace_ctx_create(int device, trace_ctx** out) {
+  if (!out) return TRACE_E_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return TRACE_E_NODEVICE;
+  if (device < 0 || device >= count) return TRACE_E_NODEVICE;
+  trace_ctx* ctx = new (std::nothrow) trace_ctx();
+  if (!ctx) return TRACE_E_OOM;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&ctx->d_err), sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(ctx->d_err, 0, sizeof(uint32_t)) != hipSuccess) {
+    delete ctx;
+    return TRACE_E_HIP;
+  }
+  ctx->stream = ctx->own_stream;
+  *out = ctx;
+  return TRACE_OK;
+}
+
+int trace_ctx_destroy(trace_ctx* ctx) {
+  if (!ctx) return TRACE_E_INVALID;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) hipFree(ctx->scratch);
+  if (ctx->d_err) hipFree(ctx->d_err);
+  if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+  return TRACE_OK;
+}
+
+const char* trace_last_error(const trace_ctx* ctx) {
+  if (!ctx) return "null context";
+  return ctx->err.c_str();
+}
+
+int trace_ctx_set_stream(trace_ctx* ctx, void* hip_stream) {
+  if (!ctx) return TRACE_E_INVALID;
+  ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+  return TRACE_OK;
+}
+
+int trace_ctx_synchronize(trace_ctx* ctx) {
+  if (!ctx) return TRACE_E_INVALID;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return ctx_check_device_error(ctx, "trace_ctx_synchronize");
+}
+
+// ---- mesh ------------------------------------------------------------------
+static int mesh_create_impl(trace_ctx* ctx, const double* pos, int64_t ntri, trace_mesh** out,
+                            hipMemcpyKind kind) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!out || !pos || ntri <= 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_create: need positions and ntri > 0 (got %lld)", (long long)ntri);
+  if (ntri >= (int64_t(1) << 30)) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_mesh_create: ntri %lld exceeds 2^30 - 1", (long long)ntri);
+  *out = nullptr;
+  trace_mesh* m = new (std::nothrow) trace_mesh();
+  if (!m) TRACE_FAIL(ctx, TRACE_E_OOM, "trace_mesh_create: host allocation failed");
+  m->ntri = ntri;
+  hipError_t e = m->pos.alloc(9 * ntri);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->pos.p, pos, 9 * ntri * sizeof(double), kind, ctx->stream);
+  if (e == hipSuccess && kind == hipMemcpyHostToDevice) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    delete m;
+    TRACE_FAIL(ctx, e == hipErrorOutOfMemory ? TRACE_E_OOM : TRACE_E_HIP, "trace_mesh_create: %s", hipGetErrorString(e));
+  }
+  *out = m;
+  return TRACE_OK;
+}
+
+int trace_mesh_create(trace_ctx* ctx, const double* h_pos, int64_t ntri, trace_mesh** out) {
+  return mesh_create_impl(ctx, h_pos, ntri, out, hipMemcpyHostToDevice);
+}
+
+int trace_mesh_create_device(trace_ctx* ctx, const double* d_pos, int64_t ntri, trace_mesh** out) {
+  return mesh_create_impl(ctx, d_pos, ntri, out, hipMemcpyDeviceToDevice);
+}
+
+int trace_mesh_update(trace_ctx* ctx, trace_mesh* mesh, const double* h_pos) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !h_pos) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_update: null argument");
+  HIP_TRY(ctx, hipMemcpyAsync(mesh->pos.p, h_pos, 9 * mesh->ntri * sizeof(double),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return TRACE_OK;
+}
+
+int trace_mesh_destroy(trace_mesh* mesh) {
+  if (!mesh) return TRACE_E_INVALID;
+  delete mesh;
+  return TRACE_OK;
+}
+
+// ---- LBVH ------------------------------------------------------------------
+int trace_lbvh_build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing,
+                     int key_mode, trace_bvh** out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_build: null argument");
+  if (!valid_key_mode(key_mode)) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_build: unknown key mode %d", key_mode);
+  *out = nullptr;
+  trace_bvh* bvh = new (std::nothrow) trace_bvh();
+  if (!bvh) TRACE_FAIL(ctx, TRACE_E_OOM, "trace_lbvh_build: host allocation failed");
+  int rc = lbvh::build(ctx, mesh, h_enclosing, key_mode, bvh);
+  if (rc == TRACE_OK) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      ctx->err = std::string("trace_lbvh_build: ") + hipGetErrorString(e);
+      rc = TRACE_E_HIP;
+    } else {
+      rc = ctx_check_device_error(ctx, "trace_lbvh_build");
+    }
+  }
+  if (rc != TRACE_OK) {
+    delete bvh;
+    return rc;
+  }
+  *out = bvh;
+  return TRACE_OK;
+}
+
+int trace_lbvh_rebuild(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing,
+                       int key_mode, trace_bvh* bvh) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !bvh) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_rebuild: null argument");
+  if (!valid_key_mode(key_mode)) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_rebuild: unknown key mode %d", key_mode);
+  if (bvh->key_mode < 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_rebuild: not an LBVH handle");
+  return lbvh::build(ctx, mesh, h_enclosing, key_mode, bvh);
+}
+
+int trace_bvh_destroy(trace_bvh* bvh) {
+  if (!bvh) return TRACE_E_INVALID;
+  delete bvh;
+  return TRACE_OK;
+}
+
+static int read_header(trace_ctx* ctx, const trace_bvh* bvh, BvhHeader* h) {
+  HIP_TRY(ctx, hipMemcpyAsync(h, bvh->header.p, sizeof(BvhHeader), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return TRACE_OK;
+}
+
+int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!bvh || !out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_get_info: null argument");
+  BvhHeader h;
+  int rc = read_header(ctx, bvh, &h);
+  if (rc) return rc;
+  std::memset(out, 0, sizeof(*out));
+  out->n_prims = bvh->n_prims;
+  out->n_leaves = h.n_leaves;
+  out->n_inner = h.root_cnt > 0 ? 0 : h.n_inner;
+  out->depth = h.depth;
+  out->key_mode = bvh->key_mode;
+  out->arity = bvh->arity;
+  for (int a = 0; a < 3; ++a) {
+    out->root_box[a] = h.root_lo[a];
+    out->root_box[3 + a] = h.root_hi[a];
+  }
+  return TRACE_OK;
+}
+
+int trace_bvh_export_sorted(trace_ctx* ctx, const trace_bvh* bvh, uint64_t* h_keys, int32_t* h_perm) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!bvh) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_sorted: null bvh");
+  if (h_keys) {
+    if (!bvh->keys.p) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_sorted: tree has no Morton keys");
+    HIP_TRY(ctx, hipMemcpyAsync(h_keys, bvh->keys.p, bvh->n_prims * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if (h_perm)
+    HIP_TRY(ctx, hipMemcpyAsync(h_perm, bvh->perm.p, bvh->n_prims * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return TRACE_OK;
+}
+
+// Host copy of the binary traversal tree.
+struct HostTree {
+  BvhHeader h;
+  std::vector<BvhNode> nodes;
+};
+
+static int fetch_tree(trace_ctx* ctx, const trace_bvh* bvh, HostTree* t) {
+  int rc = read_header(ctx, bvh, &t->h);
+  if (rc) return rc;
+  const int64_t ninner = t->h.root_cnt > 0 ? 0 : t->h.n_inner;
+  t->nodes.resize(ninner);
+  if (ninner > 0) {
+    HIP_TRY(ctx, hipMemcpyAsync(t->nodes.data(), bvh->nodes.p, ninner * sizeof(BvhNode), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return TRACE_OK;
+}
+
+namespace {
+struct Item {
+  int32_t ref, cnt;  // cnt > 0: leaf
+  float box[6];
+};
+}  // namespace
+
+int trace_bvh_export_canonical(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap, int32_t* h_rec,
+                               float* h_box, int64_t* n_out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!bvh || !n_out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_canonical: null argument");
+  HostTree t;
+  int rc = fetch_tree(ctx, bvh, &t);
+  if (rc) return rc;
+  const int64_t total = t.h.root_cnt > 0 ? 1 : 2 * int64_t(t.h.n_inner) + 1;
+  *n_out = total;
+  if (!h_rec || !h_box) return TRACE_OK;  // size query
+  if (cap < total) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_canonical: capacity %lld < %lld nodes", (long long)cap, (long long)total);
+  // Preorder with children[0] first.  Each emitted record's [first, last]
+  // is filled once its subtree is complete (post-visit), so keep a second
+  // stack of open inner records.
+  std::vector<Item> stack;
+  Item root;
+  root.ref = t.h.root_ref;
+  root.cnt = t.h.root_cnt;
+  for (int a = 0; a < 3; ++a) {
+    root.box[a] = t.h.root_lo[a];
+    root.box[3 + a] = t.h.root_hi[a];
+  }
+  stack.push_back(root);
+  int64_t k = 0;
+  // first/last of inner nodes = first of leftmost leaf, last of rightmost leaf
+  std::vector<int64_t> open;  // record index of inner nodes awaiting their last leaf
+  std::vector<int32_t> pending_children;
+  while (!stack.empty()) {
+    Item it = stack.back();
+    stack.pop_back();
+    int32_t* r = h_rec + 4 * k;
+    std::memcpy(h_box + 6 * k, it.box, sizeof(it.box));
+    if (it.cnt > 0) {
+      r[0] = 1;
+      r[1] = it.ref;
+      r[2] = it.ref + it.cnt - 1;
+      r[3] = 0;
+    } else {
+      const BvhNode& nd = t.nodes[it.ref];
+      r[0] = 0;
+      r[1] = -1;
+      r[2] = -1;
+      r[3] = 2;
+      Item c0, c1;
+      c0.ref = nd.ref[0];
+      c0.cnt = nd.cnt[0];
+      c1.ref = nd.ref[1];
+      c1.cnt = nd.cnt[1];
+      for (int a = 0; a < 3; ++a) {
+        c0.box[a] = nd.lo0[a];
+        c0.box[3 + a] = nd.hi0[a];
+        c1.box[a] = nd.lo1[a];
+        c1.box[3 + a] = nd.hi1[a];
+      }
+      stack.push_back(c1);
+      stack.push_back(c0);
+    }
+    ++k;
+  }
+  // Fill inner ranges bottom-up: in preorder, an inner node's range is from the
+  // first leaf after it to the last leaf of its subtree.  Walk backwards with a
+  // stack of subtree spans.
+  std::vector<std::pair<int32_t, int32_t>> spans;  // (first, last) of completed subtrees
+  for (int64_t i = total - 1; i >= 0; --i) {
+    int32_t* r = h_rec + 4 * i;
+    if (r[0] == 1) {
+      spans.push_back({r[1], r[2]});
+    } else {
+      auto c0 = spans.back();
+      spans.pop_back();
+      auto c1 = spans.back();
+      spans.pop_back();
+      r[1] = c0.first;
+      r[2] = c1.second;
+      spans.push_back({r[1], r[2]});
+    }
+  }
+  return TRACE_OK;
+}
+
+static double surface(const float* b) {
+  const double w = double(b[3]) - double(b[0]);
+  const double h = double(b[4]) - double(b[1]);
+  const double d = double(b[5]) - double(b[2]);
+  return 2 * w * h + 2 * w * d + 2 * h * d;  // bounding.go:45-47
+}
+
+int trace_bvh_cost(trace_ctx* ctx, const trace_bvh* bvh, double* cost) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!bvh || !cost) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_cost: null argument");
+  int64_t n = 0;
+  int rc = trace_bvh_export_canonical(ctx, bvh, 0, nullptr, nullptr, &n);
+  if (rc) return rc;
+  std::vector<int32_t> rec(4 * n);
+  std::vector<float> box(6 * n);
+  rc = trace_bvh_export_canonical(ctx, bvh, n, rec.data(), box.data(), &n);
+  if (rc) return rc;
+  // cost(leaf) = INTERSECTION_COST * prims; cost(inner) = TRAVERSAL_COST +
+  // sum_c SA(c)/SA(node) * cost(c), children skipped when SA(node) == 0
+  // (bvh.go:226-239).  Evaluated in reverse preorder: the children of an
+  // inner record are the top entries of the stack of finished subtrees,
+  // children[0] on top.
+  struct Done {
+    double cost, sa;
+  };
+  std::vector<Done> st;
+  for (int64_t i = n - 1; i >= 0; --i) {
+    const int32_t* r = rec.data() + 4 * i;
+    const double sa = surface(box.data() + 6 * i);
+    if (r[0] == 1) {
+      st.push_back({1.0 * double(r[2] - r[1] + 1), sa});
+    } else {
+      double acc = 0.0;
+      for (int c = 0; c < r[3]; ++c) {
+        const Done d = st.back();
+        st.pop_back();
+        if (sa != 0) acc += d.sa / sa * d.cost;
+      }
+      st.push_back({2.0 + acc, sa});
+    }
+  }
+  *cost = st.empty() ? 0.0 : st.back().cost;
+  return TRACE_OK;
+}
+
+int trace_bvh_size(trace_ctx* ctx, const trace_bvh* bvh, int64_t* size) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!bvh || !size) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_size: null argument");
+  return trace_bvh_export_canonical(ctx, bvh, 0, nullptr, nullptr, size);
+}
+
+// ---- closest hit -------------------------------------------------------------
+int trace_closest_hit_device(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays,
+                             int64_t nrays, int32_t* d_hit, float* d_t) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!bvh || nrays < 0 || (nrays > 0 && (!d_rays || !d_hit || !d_t)))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_closest_hit_device: bad arguments");
+  return traverse::closest_hit(ctx, bvh, d_rays, nrays, d_hit, d_t, nullptr, nullptr);
+}
+
+static int closest_hit_host(trace_ctx* ctx, const trace_bvh* bvh, const float* h_rays,
+                            int64_t nrays, int32_t* h_hit, float* h_t, int32_t* h_nodes,
+                            int32_t* h_tris, const char* what) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!bvh || nrays < 0 || (nrays > 0 && (!h_rays || !h_hit || !h_t)))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: bad arguments", what);
+  if (nrays == 0) return TRACE_OK;
+  const bool counting = h_nodes != nullptr;
+  DevArray<char> buf;
+  const size_t ray_b = size_t(nrays) * 8 * sizeof(float);
+  const size_t out_b = size_t(nrays) * sizeof(int32_t);
+  HIP_TRY(ctx, buf.alloc(ray_b + out_b * (counting ? 4 : 2)));
+  float* d_rays = reinterpret_cast<float*>(buf.p);
+  int32_t* d_hit = reinterpret_cast<int32_t*>(buf.p + ray_b);
+  float* d_t = reinterpret_cast<float*>(buf.p + ray_b + out_b);
+  int32_t* d_nodes = counting ? reinterpret_cast<int32_t*>(buf.p + ray_b + 2 * out_b) : nullptr;
+  int32_t* d_tris = counting ? reinterpret_cast<int32_t*>(buf.p + ray_b + 3 * out_b) : nullptr;
+  HIP_TRY(ctx, hipMemcpyAsync(d_rays, h_rays, ray_b, hipMemcpyHostToDevice, ctx->stream));
+  int rc = traverse::closest_hit(ctx, bvh, d_rays, nrays, d_hit, d_t, d_nodes, d_tris);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(h_hit, d_hit, out_b, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(h_t, d_t, out_b, hipMemcpyDeviceToHost, ctx->stream));
+  if (counting) {
+    HIP_TRY(ctx, hipMemcpyAsync(h_nodes, d_nodes, out_b, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h_tris, d_tris, out_b, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return ctx_check_device_error(ctx, what);
+}
+
+int trace_closest_hit(trace_ctx* ctx, const trace_bvh* bvh, const float* h_rays, int64_t nrays,
+                      int32_t* h_hit, float* h_t) {
+  return closest_hit_host(ctx, bvh, h_rays, nrays, h_hit, h_t, nullptr, nullptr,
+                          "trace_closest_hit");
+}
+
+int trace_closest_hit_counts(trace_ctx* ctx, const trace_bvh* bvh, const float* h_rays,
+                             int64_t nrays, int32_t* h_nodes, int32_t* h_tris) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!h_nodes || !h_tris) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_closest_hit_counts: null output");
+  std::vector<int32_t> hit(nrays > 0 ? nrays : 0);
+  std::vector<float> t(nrays > 0 ? nrays : 0);
+  return closest_hit_host(ctx, bvh, h_rays, nrays, hit.data(), t.data(), h_nodes, h_tris,
+                          "trace_closest_hit_counts");
+}
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+// Shared host/device definitions of the gfx950 hot path: context, device
+// buffers, the BVH node record and small wave64 helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "trace_hip.h"
+
+// ---------------------------------------------------------------------------
+// host-side error plumbing
+// ---------------------------------------------------------------------------
+struct trace_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  uint32_t* d_err = nullptr;  // device error word: bit 0 spin bound, bit 1 stack overflow
+  // grow-only scratch arena, reused across builds (never shrinks)
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+#define TRACE_FAIL(ctx, code, ...)                                \
+  do {                                                            \
+    char _b[512];                                                 \
+    snprintf(_b, sizeof(_b), __VA_ARGS__);                        \
+    (ctx)->err = _b;                                              \
+    return (code);                                                \
+  } while (0)
+
+#define HIP_TRY(ctx, call)                                                         \
+  do {                                                                             \
+    hipError_t _e = (call);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      TRACE_FAIL(ctx, _e == hipErrorOutOfMemory ? TRACE_E_OOM : TRACE_E_HIP,       \
+                 "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, \
+                 __LINE__);                                                        \
+    }                                                                              \
+  } while (0)
+
+// Device allocation owned by a handle.  Freed with hipFree in the destructor.
+template <typename T>
+struct DevArray {
+  T* p = nullptr;
+  size_t n = 0;
+  DevArray() = default;
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  ~DevArray() { release(); }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), bytes);
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+};
+
+// Bump allocator over the context scratch arena.  Sizes are rounded up to 256 B.
+struct ScratchPlan {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  }
+};
+int ctx_reserve_scratch(trace_ctx* ctx, size_t bytes);
+int ctx_check_device_error(trace_ctx* ctx, const char* what);
+
+// ---------------------------------------------------------------------------
+// device-resident data
+// ---------------------------------------------------------------------------
+struct trace_mesh {
+  int64_t ntri = 0;
+  DevArray<double> pos;  // 9 doubles per triangle, caller order
+};
+
+// Inner node of the binary traversal tree: both child boxes in one 64-byte
+// record so an inner-node visit is one 64 B load.  cnt[c] == 0 means child c is
+// an inner node with index ref[c]; cnt[c] > 0 means child c is a leaf holding
+// triangles [ref[c], ref[c] + cnt[c]) of the sorted triangle array.
+struct __align__(16) BvhNode {
+  float lo0[3];
+  float hi0[3];
+  float lo1[3];
+  float hi1[3];
+  int32_t ref[2];
+  int32_t cnt[2];
+};
+static_assert(sizeof(BvhNode) == 64, "node record must be 64 bytes");
+
+// Small header that kernels read from device memory so that no host round
+// trip is needed between the build and the traversal.
+struct BvhHeader {
+  float root_lo[3];
+  float root_hi[3];
+  int32_t root_ref;   // inner node index, or first triangle if root_cnt > 0
+  int32_t root_cnt;   // > 0: the root is a leaf
+  int32_t n_leaves;   // unique keys (LBVH) or leaves
+  int32_t n_inner;
+  int32_t depth;
+  int32_t n_prims;
+  double enc_lo[3];   // FP64 scene box (union of triangle boxes)
+  double enc_hi[3];
+};
+
+// A tree as the reference stores it (for Cost/Size/export and as PHR input):
+// node i has n_children children listed at child_list[child_off[i] ...].
+// Leaves hold a sorted-position range.  Used for n-ary PHR trees; the LBVH
+// keeps its binary form only.
+struct trace_bvh {
+  int64_t n_prims = 0;
+  int32_t key_mode = 0;  // -1 for PHR
+  int32_t arity = 2;
+  DevArray<uint64_t> keys;     // sorted Morton keys (LBVH only)
+  DevArray<int32_t> perm;      // sorted position -> caller triangle index
+  DevArray<float> tris;        // sorted order: v0, v1-v0, v2-v0 (9 floats)
+  DevArray<float> prim_box;    // sorted order: lo.xyz hi.xyz (6 floats, outward-rounded)
+  DevArray<BvhNode> nodes;     // binary traversal tree
+  DevArray<int32_t> parent;    // inner parents then leaf parents ((p << 1) | slot, -1 = root)
+  DevArray<uint32_t> run_start;// leaf i covers [run_start[i], run_start[i+1])
+  DevArray<BvhHeader> header;
+  // host mirror of the n-ary structure for PHR trees (empty for LBVH)
+  std::vector<int32_t> nary_first;   // per node: leaf -> first prim pos, inner -> -1
+  std::vector<int32_t> nary_count;   // per node: leaf -> prim count, inner -> #children
+  std::vector<int32_t> nary_child;   // per inner node: offset into nary_children
+  std::vector<int32_t> nary_children;
+  std::vector<double> nary_box;      // 6 per node, FP64
+  int32_t nary_root = -1;
+  std::vector<int32_t> leaf_prims;   // PHR leaves list prims explicitly (sorted positions)
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+#define TRACE_ERR_SPIN 1u
+#define TRACE_ERR_STACK 2u
+#define TRACE_ERR_TREE 4u  // a kernel met an out-of-range node or triangle reference
+
+__device__ inline uint64_t lanemask_lt() {
+  const uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
+
+// Round a double toward -inf / +inf when narrowing to float so that float
+// boxes always contain the double boxes they stand for.
+__device__ __host__ inline float f32_down(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = nextafterf(f, -INFINITY);
+  return f;
+}
+__device__ __host__ inline float f32_up(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = nextafterf(f, INFINITY);
+  return f;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+template <typename T>
+__global__ void fill_kernel(T* p, size_t n, T v) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// Clears device words with a kernel (not hipMemsetAsync) so the work stays
+// valid inside captured HIP graphs.
+template <typename T>
+inline void fill_async(hipStream_t st, T* p, size_t n, T v) {
+  if (n == 0) return;
+  const int64_t blocks = std::min<int64_t>(ceil_div(int64_t(n), 256), 2048);
+  fill_kernel<T><<<int(blocks), 256, 0, st>>>(p, n, v);
+}

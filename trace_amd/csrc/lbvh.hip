@@ -1,0 +1,490 @@
+// Morton LBVH build for gfx950.
+//
+// Pipeline (all on the context stream, no host round trip):
+//   enclosing_kernel  FP64 scene box = union of triangle boxes       bounding.go:19-25
+//   header_kernel     root box, quantisation box                      lbvh.go:13-15
+//   morton_kernel     63-bit FP64 Morton code per triangle             lbvh.go:64-73, morton.go:6-20
+//   onesweep sort     stable radix sort of (key, triangle index)       lbvh.go:216-301 (bucket sort)
+//   unique kernels    equal-key runs -> one leaf each                  lbvh.go:134-145, 179-181
+//   karras_kernel     binary radix tree over the unique keys           lbvh.go:147-203 (top-down split)
+//   reorder_kernel    FP32 triangles and boxes in sorted order
+//   refit_kernel      bottom-up AABBs with per-node arrival counters   bvh.go:104-121, 201-222
+//
+// The reference builds the tree top-down: a key range whose first and last
+// codes are equal becomes one leaf; otherwise the range splits at the first
+// key that has the highest differing bit set.  That is exactly the binary
+// radix tree of Karras (2012) over the distinct keys, which every inner node
+// can find independently; the leaf of distinct key u is the run of triangles
+// sharing it.
+#include <algorithm>
+
+#include "lbvh.hpp"
+#include "onesweep.hpp"
+
+namespace lbvh {
+
+__device__ inline uint64_t ord_of(double x) {
+  const uint64_t b = __double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ inline double double_of(uint64_t o) {
+  const uint64_t b = (o >> 63) ? (o & 0x7fffffffffffffffull) : ~o;
+  return __longlong_as_double(b);
+}
+
+// min/max of three with the reference's comparisons (util.go:47-66); the
+// result is one of the inputs, so the value is order independent.
+__device__ inline double min3(double a, double b, double c) {
+  if (a <= b && a <= c) return a;
+  if (b <= a && b <= c) return b;
+  return c;
+}
+__device__ inline double max3(double a, double b, double c) {
+  if (a >= b && a >= c) return a;
+  if (b >= a && b >= c) return b;
+  return c;
+}
+
+__global__ void init_enclosing_kernel(unsigned long long* enc) {
+  if (threadIdx.x < 3) enc[threadIdx.x] = ~0ull;  // min slots
+  else if (threadIdx.x < 6) enc[threadIdx.x] = 0ull;  // max slots
+}
+
+__global__ __launch_bounds__(256) void enclosing_kernel(const double* __restrict__ pos, int64_t n,
+                                                        unsigned long long* enc) {
+  uint64_t lo[3] = {~0ull, ~0ull, ~0ull};
+  uint64_t hi[3] = {0ull, 0ull, 0ull};
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double* p = pos + 9 * i;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const uint64_t mn = ord_of(min3(p[a], p[3 + a], p[6 + a]));
+      const uint64_t mx = ord_of(max3(p[a], p[3 + a], p[6 + a]));
+      lo[a] = mn < lo[a] ? mn : lo[a];
+      hi[a] = mx > hi[a] ? mx : hi[a];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t l = __shfl_xor(lo[a], o, 64);
+      const uint64_t h = __shfl_xor(hi[a], o, 64);
+      lo[a] = l < lo[a] ? l : lo[a];
+      hi[a] = h > hi[a] ? h : hi[a];
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&enc[a], (unsigned long long)lo[a]);
+      atomicMax(&enc[3 + a], (unsigned long long)hi[a]);
+    }
+  }
+}
+
+// Writes the scene box into the header and, unless the caller supplied one,
+// the quantisation box used by the Morton kernel.
+__global__ void header_kernel(const unsigned long long* enc, BvhHeader* hdr, double* qbox,
+                              int caller_box, int32_t n) {
+  if (threadIdx.x != 0) return;
+  hdr->n_prims = n;
+  for (int a = 0; a < 3; ++a) {
+    const double lo = double_of(enc[a]);
+    const double hi = double_of(enc[3 + a]);
+    hdr->enc_lo[a] = lo;
+    hdr->enc_hi[a] = hi;
+    hdr->root_lo[a] = f32_down(lo);
+    hdr->root_hi[a] = f32_up(hi);
+    if (!caller_box) {
+      qbox[a] = lo;
+      qbox[3 + a] = hi;
+    }
+  }
+}
+
+__device__ inline uint64_t split_by3(uint64_t a) {
+  // Spread the low 21 bits of a so that bit k lands on bit 3k.
+  uint64_t x = a & 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+// q = uint64(|enc.min - c| / (extent / (2^21 - 1))), evaluated in FP64 with
+// the reference's operation order (lbvh.go:64-73).  No contraction: every
+// operation is rounded on its own, as Go does on amd64.  A zero extent gives
+// 0/0; the reference's result is platform dependent and here q = 0.
+__device__ inline uint64_t quantise(double enc_lo, double extent, double c) {
+#pragma clang fp contract(off)
+  const double delta = fabs(enc_lo - c);
+  const double step = extent / 2097151.0;
+  const double f = delta / step;
+  if (!(f >= 0.0)) return 0;      // NaN
+  if (f >= 18446744073709551615.0) return ~0ull;
+  return (uint64_t)f;
+}
+
+__global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ pos, int64_t n,
+                                                     const double* __restrict__ qbox, int key_mode,
+                                                     uint64_t* __restrict__ keys) {
+#pragma clang fp contract(off)
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const double* p = pos + 9 * i;
+  uint64_t q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double lo = min3(p[a], p[3 + a], p[6 + a]);
+    const double hi = max3(p[a], p[3 + a], p[6 + a]);
+    const double c = (lo + hi) * 0.5;  // aabb.update, bounding.go:36-43
+    q[a] = quantise(qbox[a], qbox[3 + a] - qbox[a], c);
+  }
+  uint64_t code = split_by3(q[0]) | (split_by3(q[1]) << 1) | (split_by3(q[2]) << 2);
+  if (key_mode == TRACE_KEY_M30) code >>= 33;
+  keys[i] = code;
+}
+
+// ---- unique keys --------------------------------------------------------
+constexpr int kUThreads = 256;
+constexpr int kUPerThread = 16;
+constexpr int kUTile = kUThreads * kUPerThread;
+
+__device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t* s_wave, uint32_t* total) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[w] = x;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i < w) off += s_wave[i];
+    tot += s_wave[i];
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return off + x - v;
+}
+
+__global__ __launch_bounds__(kUThreads) void count_heads_kernel(const uint64_t* __restrict__ keys,
+                                                                int64_t n, uint32_t* blk_cnt) {
+  __shared__ uint32_t s_wave[4];
+  const int64_t base = blockIdx.x * int64_t(kUTile) + threadIdx.x * int64_t(kUPerThread);
+  uint32_t c = 0;
+  for (int j = 0; j < kUPerThread; ++j) {
+    const int64_t i = base + j;
+    if (i < n) c += (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+  }
+  uint32_t tot;
+  block_excl_scan256(c, s_wave, &tot);
+  if (threadIdx.x == 0) blk_cnt[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of the per-tile head counts; total -> header.
+__global__ __launch_bounds__(256) void scan_blocks_kernel(uint32_t* blk, int64_t nblk,
+                                                          BvhHeader* hdr) {
+  __shared__ uint32_t s_wave[4];
+  uint32_t carry = 0;
+  for (int64_t b0 = 0; b0 < nblk; b0 += 256) {
+    const int64_t b = b0 + threadIdx.x;
+    const uint32_t v = b < nblk ? blk[b] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan256(v, s_wave, &tot);
+    if (b < nblk) blk[b] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    hdr->n_leaves = int32_t(carry);
+    hdr->n_inner = int32_t(carry) - 1;
+    hdr->depth = 0;
+  }
+}
+
+__global__ __launch_bounds__(kUThreads) void write_unique_kernel(
+    const uint64_t* __restrict__ keys, int64_t n, const uint32_t* __restrict__ blk_off,
+    uint64_t* __restrict__ ukeys, uint32_t* __restrict__ run_start, BvhHeader* hdr) {
+  __shared__ uint32_t s_wave[4];
+  const int64_t base = blockIdx.x * int64_t(kUTile) + threadIdx.x * int64_t(kUPerThread);
+  uint32_t heads = 0;
+  for (int j = 0; j < kUPerThread; ++j) {
+    const int64_t i = base + j;
+    if (i < n && (i == 0 || keys[i] != keys[i - 1])) heads |= 1u << j;
+  }
+  uint32_t r = blk_off[blockIdx.x] + block_excl_scan256(__popc(heads), s_wave, nullptr);
+  for (int j = 0; j < kUPerThread; ++j) {
+    if (heads & (1u << j)) {
+      ukeys[r] = keys[base + j];
+      run_start[r] = uint32_t(base + j);
+      ++r;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) run_start[hdr->n_leaves] = uint32_t(n);
+}
+
+// ---- hierarchy ------------------------------------------------------------
+__device__ inline int delta_of(const uint64_t* k, int32_t u, int32_t i, int32_t j) {
+  if (j < 0 || j >= u) return -1;
+  return __clzll(k[i] ^ k[j]);
+}
+
+// Inner node i of the radix tree over u distinct keys (Karras 2012, Alg. 1).
+// The lower key range becomes children[0], as lbvh.go:163-176 orders them.
+__global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict__ k,
+                                                     const uint32_t* __restrict__ run_start,
+                                                     const BvhHeader* __restrict__ hdr,
+                                                     BvhNode* __restrict__ nodes,
+                                                     int32_t* __restrict__ parent_inner,
+                                                     int32_t* __restrict__ parent_leaf) {
+  const int32_t u = hdr->n_leaves;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= u - 1) return;
+  const int d = (delta_of(k, u, i, i + 1) - delta_of(k, u, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta_of(k, u, i, i - d);
+  int32_t lmax = 2;
+  while (delta_of(k, u, i, i + lmax * d) > dmin) lmax <<= 1;
+  int32_t l = 0;
+  for (int32_t t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta_of(k, u, i, i + (l + t) * d) > dmin) l += t;
+  const int32_t j = i + l * d;
+  const int dnode = delta_of(k, u, i, j);
+  int32_t s = 0;
+  int32_t t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (delta_of(k, u, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int32_t gamma = i + s * d + (d < 0 ? -1 : 0);
+  const int32_t first = i < j ? i : j;
+  const int32_t last = i < j ? j : i;
+
+  BvhNode& nd = nodes[i];
+  // child 0 covers [first, gamma], child 1 covers [gamma + 1, last]
+  if (first == gamma) {
+    nd.ref[0] = int32_t(run_start[gamma]);
+    nd.cnt[0] = int32_t(run_start[gamma + 1] - run_start[gamma]);
+    parent_leaf[gamma] = (i << 1) | 0;
+  } else {
+    nd.ref[0] = gamma;
+    nd.cnt[0] = 0;
+    parent_inner[gamma] = (i << 1) | 0;
+  }
+  if (last == gamma + 1) {
+    nd.ref[1] = int32_t(run_start[gamma + 1]);
+    nd.cnt[1] = int32_t(run_start[gamma + 2] - run_start[gamma + 1]);
+    parent_leaf[gamma + 1] = (i << 1) | 1;
+  } else {
+    nd.ref[1] = gamma + 1;
+    nd.cnt[1] = 0;
+    parent_inner[gamma + 1] = (i << 1) | 1;
+  }
+  if (i == 0) parent_inner[0] = -1;
+}
+
+// FP32 triangles (v0, v1 - v0, v2 - v0; differences taken in FP64 as
+// primitive.go:111-112 does) and outward-rounded FP32 boxes, in sorted order.
+__global__ __launch_bounds__(256) void reorder_kernel(const double* __restrict__ pos, int64_t n,
+                                                      const uint32_t* __restrict__ perm,
+                                                      float* __restrict__ tris,
+                                                      float* __restrict__ boxes) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t src = perm[i] < uint64_t(n) ? perm[i] : 0u;  // perm is a bijection on [0, n)
+  const double* p = pos + 9 * int64_t(src);
+  double v[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v[k] = p[k];
+  float* t = tris + 9 * i;
+  float* b = boxes + 6 * i;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    t[a] = float(v[a]);
+    t[3 + a] = float(v[3 + a] - v[a]);
+    t[6 + a] = float(v[6 + a] - v[a]);
+    b[a] = f32_down(min3(v[a], v[3 + a], v[6 + a]));
+    b[3 + a] = f32_up(max3(v[a], v[3 + a], v[6 + a]));
+  }
+}
+
+__device__ inline void st_f32(float* p, float v) {
+  __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float ld_f32(const float* p) {
+  return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Bottom-up refit: the leaf of distinct key u unions its triangles' boxes and
+// writes the result into its parent's child slot; the second child to arrive
+// at a node (per-node arrival counter, as bvh.go:206/219) unions both slots
+// and continues upwards.  Box words cross workgroups through agent-scope
+// atomics; workgroup fences order them against the counter.
+__global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ boxes,
+                                                    const uint32_t* __restrict__ run_start,
+                                                    const int32_t* __restrict__ parent_inner,
+                                                    const int32_t* __restrict__ parent_leaf,
+                                                    BvhNode* nodes, uint32_t* counters,
+                                                    int32_t* heights, BvhHeader* hdr,
+                                                    uint32_t* d_err) {
+  const int32_t u = hdr->n_leaves;
+  const int32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  if (leaf >= u) return;
+  if (u == 1) {
+    hdr->root_ref = 0;
+    hdr->root_cnt = int32_t(run_start[1]);
+    hdr->depth = 0;
+    return;
+  }
+  if (leaf == 0) {
+    hdr->root_ref = 0;
+    hdr->root_cnt = 0;
+  }
+  float lo[3], hi[3];
+  const uint32_t s = run_start[leaf], e = run_start[leaf + 1];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = boxes[6 * size_t(s) + a];
+    hi[a] = boxes[6 * size_t(s) + 3 + a];
+  }
+  for (uint32_t k = s + 1; k < e; ++k) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], boxes[6 * size_t(k) + a]);
+      hi[a] = fmaxf(hi[a], boxes[6 * size_t(k) + 3 + a]);
+    }
+  }
+  int32_t h = 0;
+  int32_t link = parent_leaf[leaf];
+  // A radix tree over 64-bit keys is at most 64 levels deep; the bound and
+  // the index check only guard against a malformed hierarchy.
+  for (int step = 0; link >= 0; ++step) {
+    const int32_t node = link >> 1;
+    const int slot = link & 1;
+    if (step > 128 || node >= u - 1) {
+      atomicOr(d_err, TRACE_ERR_TREE);
+      return;
+    }
+    float* nb = reinterpret_cast<float*>(&nodes[node]);
+    float* my = nb + 6 * slot;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      st_f32(my + a, lo[a]);
+      st_f32(my + 3 + a, hi[a]);
+    }
+    __hip_atomic_store(heights + 2 * node + slot, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    const uint32_t prev = atomicAdd(counters + node, 1u);
+    if (prev == 0) return;  // sibling not done yet; it will continue
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const float* other = nb + 6 * (slot ^ 1);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], ld_f32(other + a));
+      hi[a] = fmaxf(hi[a], ld_f32(other + 3 + a));
+    }
+    const int32_t ho = __hip_atomic_load(heights + 2 * node + (slot ^ 1), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    h = 1 + (h > ho ? h : ho);
+    link = parent_inner[node];
+  }
+  // reached the root (inner node 0)
+  hdr->depth = h;
+}
+
+static int grid_for(int64_t n, int threads) { return int(std::max<int64_t>(1, ceil_div(n, threads))); }
+
+int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int key_mode,
+          trace_bvh* bvh) {
+  const int64_t n = mesh->ntri;
+  hipStream_t st = ctx->stream;
+  const int passes = key_mode == TRACE_KEY_M30 ? 4 : 8;
+
+  bvh->n_prims = n;
+  bvh->key_mode = key_mode;
+  bvh->arity = 2;
+  HIP_TRY(ctx, bvh->keys.alloc(n));
+  HIP_TRY(ctx, bvh->perm.alloc(n));
+  HIP_TRY(ctx, bvh->tris.alloc(9 * n));
+  HIP_TRY(ctx, bvh->prim_box.alloc(6 * n));
+  HIP_TRY(ctx, bvh->nodes.alloc(std::max<int64_t>(1, n - 1)));
+  HIP_TRY(ctx, bvh->parent.alloc(2 * n));
+  HIP_TRY(ctx, bvh->run_start.alloc(n + 1));
+  HIP_TRY(ctx, bvh->header.alloc(1));
+
+  // scratch: enclosing words, qbox, sort ping-pong buffers, sort status,
+  // unique keys, per-tile head counts, refit counters and heights
+  const int64_t utiles = ceil_div(n, kUTile);
+  ScratchPlan plan;
+  const size_t o_enc = plan.take(6 * sizeof(unsigned long long));
+  const size_t o_qbox = plan.take(6 * sizeof(double));
+  const size_t o_kb = plan.take(n * sizeof(uint64_t));
+  const size_t o_va = plan.take(n * sizeof(uint32_t));
+  const size_t o_vb = plan.take(n * sizeof(uint32_t));
+  const size_t o_st = plan.take(onesweep::status_words(n, passes) * sizeof(uint32_t));
+  const size_t o_uk = plan.take(n * sizeof(uint64_t));
+  const size_t o_blk = plan.take(utiles * sizeof(uint32_t));
+  const size_t o_cnt = plan.take(std::max<int64_t>(1, n) * sizeof(uint32_t));
+  const size_t o_hgt = plan.take(2 * std::max<int64_t>(1, n) * sizeof(int32_t));
+  int rc = ctx_reserve_scratch(ctx, plan.off);
+  if (rc) return rc;
+  char* base = static_cast<char*>(ctx->scratch);
+  auto* enc = reinterpret_cast<unsigned long long*>(base + o_enc);
+  auto* qbox = reinterpret_cast<double*>(base + o_qbox);
+  auto* keys_b = reinterpret_cast<uint64_t*>(base + o_kb);
+  auto* vals_a = reinterpret_cast<uint32_t*>(base + o_va);
+  auto* vals_b = reinterpret_cast<uint32_t*>(base + o_vb);
+  auto* status = reinterpret_cast<uint32_t*>(base + o_st);
+  auto* ukeys = reinterpret_cast<uint64_t*>(base + o_uk);
+  auto* blk = reinterpret_cast<uint32_t*>(base + o_blk);
+  auto* counters = reinterpret_cast<uint32_t*>(base + o_cnt);
+  auto* heights = reinterpret_cast<int32_t*>(base + o_hgt);
+
+  if (h_enclosing) HIP_TRY(ctx, hipMemcpyAsync(qbox, h_enclosing, 6 * sizeof(double),
+                                               hipMemcpyHostToDevice, st));
+  init_enclosing_kernel<<<1, 64, 0, st>>>(enc);
+  enclosing_kernel<<<int(std::min<int64_t>(grid_for(n, 256), 2048)), 256, 0, st>>>(mesh->pos.p, n,
+                                                                                   enc);
+  header_kernel<<<1, 64, 0, st>>>(enc, bvh->header.p, qbox, h_enclosing ? 1 : 0, int32_t(n));
+  morton_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, qbox, key_mode, bvh->keys.p);
+
+  const int where = onesweep::sort_pairs(st, bvh->keys.p, vals_a, keys_b, vals_b, n, passes,
+                                         status, ctx->d_err, true);
+  uint64_t* sorted_keys = where ? keys_b : bvh->keys.p;
+  uint32_t* sorted_vals = where ? vals_b : vals_a;
+  if (where) {
+    HIP_TRY(ctx, hipMemcpyAsync(bvh->keys.p, keys_b, n * sizeof(uint64_t),
+                                hipMemcpyDeviceToDevice, st));
+    sorted_keys = bvh->keys.p;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(bvh->perm.p, sorted_vals, n * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, st));
+
+  count_heads_kernel<<<int(utiles), kUThreads, 0, st>>>(sorted_keys, n, blk);
+  scan_blocks_kernel<<<1, 256, 0, st>>>(blk, utiles, bvh->header.p);
+  write_unique_kernel<<<int(utiles), kUThreads, 0, st>>>(sorted_keys, n, blk, ukeys,
+                                                         bvh->run_start.p, bvh->header.p);
+  int32_t* parent_inner = bvh->parent.p;
+  int32_t* parent_leaf = bvh->parent.p + n;
+  karras_kernel<<<grid_for(n, 256), 256, 0, st>>>(ukeys, bvh->run_start.p, bvh->header.p,
+                                                  bvh->nodes.p, parent_inner, parent_leaf);
+  reorder_kernel<<<grid_for(n, 256), 256, 0, st>>>(
+      mesh->pos.p, n, reinterpret_cast<const uint32_t*>(bvh->perm.p), bvh->tris.p,
+      bvh->prim_box.p);
+  fill_async<uint32_t>(st, counters, size_t(n), 0u);
+  refit_kernel<<<grid_for(n, 256), 256, 0, st>>>(bvh->prim_box.p, bvh->run_start.p, parent_inner,
+                                                 parent_leaf, bvh->nodes.p, counters, heights,
+                                                 bvh->header.p, ctx->d_err);
+  HIP_TRY(ctx, hipGetLastError());
+  return TRACE_OK;
+}
+
+}  // namespace lbvh

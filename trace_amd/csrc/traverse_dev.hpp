@@ -1,0 +1,170 @@
+// Device-side closest-hit traversal shared by the bulk-ray kernel and the
+// path tracer.
+//
+// Semantics follow the reference (bvh.go:179-199, bounding.go:53-91,
+// primitive.go:172-208):
+//   - slab test against [tmin, best t]; a box is rejected only if it starts
+//     beyond the current best hit or ends before tmin;
+//   - Moeller-Trumbore with the reference's absolute |det| <= 1e-7 cut and
+//     inclusive u, v, t bounds;
+//   - on an exact tie in t the reference keeps the triangle its depth-first,
+//     children[0]-first walk visits last, i.e. the one with the largest sorted
+//     position; the test below accepts an equal t only for a larger position,
+//     which reproduces that winner for any visiting order.
+// The walk itself is near-child first with a short per-thread stack of inner
+// node indices; leaves are tested as soon as their box is hit.
+#pragma once
+
+#include "common.hpp"
+#include "traverse.hpp"
+
+namespace traverse {
+
+constexpr float kDetEps = 1e-7f;  // util.go:8 APPROX_THRESH
+// Slab exits are stretched by 4 ulp so FP32 rounding never culls a box that
+// the exact test would enter (conservative traversal, Ize 2013).
+constexpr float kSlabSlack = 1.0f + 4.0f * 1.1920929e-7f;
+
+struct RayPre {
+  float ox, oy, oz;
+  float ix, iy, iz;  // 1 / direction, IEEE (+-inf on zero components)
+  float dx, dy, dz;
+  float tmin;
+};
+
+__device__ inline RayPre ray_pre(const Ray& r) {
+  RayPre p;
+  p.ox = r.ox; p.oy = r.oy; p.oz = r.oz;
+  p.dx = r.dx; p.dy = r.dy; p.dz = r.dz;
+  p.ix = 1.0f / r.dx; p.iy = 1.0f / r.dy; p.iz = 1.0f / r.dz;
+  p.tmin = r.tmin;
+  return p;
+}
+
+// Entry distance in tn; true if [tn, tf] overlaps [tmin, tmax].
+__device__ inline bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
+                            float hz, float tmax, float& tn) {
+  const float x0 = (lx - r.ox) * r.ix, x1 = (hx - r.ox) * r.ix;
+  const float y0 = (ly - r.oy) * r.iy, y1 = (hy - r.oy) * r.iy;
+  const float z0 = (lz - r.oz) * r.iz, z1 = (hz - r.oz) * r.iz;
+  const float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+  const float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+  tn = n;
+  return n <= f * kSlabSlack;
+}
+
+// Moeller-Trumbore against sorted triangle k; updates (best_t, best) and the
+// barycentrics on acceptance.
+__device__ inline void tri_test(const RayPre& r, const float* __restrict__ tris, int32_t k,
+                                Hit& h) {
+  const float* t = tris + 9 * size_t(k);
+  const float v0x = t[0], v0y = t[1], v0z = t[2];
+  const float e1x = t[3], e1y = t[4], e1z = t[5];
+  const float e2x = t[6], e2y = t[7], e2z = t[8];
+  // p = d x e2, det = e1 . p
+  const float px = r.dy * e2z - r.dz * e2y;
+  const float py = r.dz * e2x - r.dx * e2z;
+  const float pz = r.dx * e2y - r.dy * e2x;
+  const float det = e1x * px + e1y * py + e1z * pz;
+  if (fabsf(det) <= kDetEps) return;
+  const float inv = 1.0f / det;
+  const float tx = r.ox - v0x, ty = r.oy - v0y, tz = r.oz - v0z;
+  const float u = (tx * px + ty * py + tz * pz) * inv;
+  if (u < 0.0f || u > 1.0f) return;
+  // q = tvec x e1
+  const float qx = ty * e1z - tz * e1y;
+  const float qy = tz * e1x - tx * e1z;
+  const float qz = tx * e1y - ty * e1x;
+  const float v = (r.dx * qx + r.dy * qy + r.dz * qz) * inv;
+  if (v < 0.0f || u + v > 1.0f) return;
+  const float tt = (e2x * qx + e2y * qy + e2z * qz) * inv;
+  if (tt < r.tmin || tt > h.t) return;
+  if (tt == h.t && k < h.pos) return;
+  h.t = tt;
+  h.pos = k;
+  h.u = u;
+  h.v = v;
+  h.front = det > 0.0f;
+}
+
+// Leaf [first, first + cnt) must lie inside the triangle array; a malformed
+// reference is skipped and reported instead of read.
+template <bool COUNT>
+__device__ inline void leaf_test(const RayPre& r, const float* __restrict__ tris, int32_t first,
+                                 int32_t cnt, int32_t n_prims, Hit& h, int32_t& n_tris,
+                                 bool& bad) {
+  if (first < 0 || cnt > n_prims - first) {
+    bad = true;
+    return;
+  }
+  for (int32_t k = first; k < first + cnt; ++k) {
+    if (COUNT) ++n_tris;
+    tri_test(r, tris, k, h);
+  }
+}
+
+// `overflow` reports a full stack, `bad` a reference outside the arrays or a
+// walk longer than any valid tree allows (every node is visited at most once).
+template <int STACK, bool COUNT>
+__device__ inline Hit closest_hit_one(const BvhNode* __restrict__ nodes,
+                                      const float* __restrict__ tris, const BvhHeader& hdr,
+                                      const Ray& ray, int32_t& n_nodes, int32_t& n_tris,
+                                      bool& overflow, bool& bad) {
+  const RayPre r = ray_pre(ray);
+  Hit h;
+  h.t = ray.tmax;
+  h.pos = -1;
+  h.u = h.v = 0.0f;
+  h.front = true;
+  float tn;
+  if (!slab(r, hdr.root_lo[0], hdr.root_lo[1], hdr.root_lo[2], hdr.root_hi[0], hdr.root_hi[1],
+            hdr.root_hi[2], h.t, tn))
+    return h;
+  if (hdr.root_cnt > 0) {
+    leaf_test<COUNT>(r, tris, hdr.root_ref, hdr.root_cnt, hdr.n_prims, h, n_tris, bad);
+    return h;
+  }
+  int32_t stack[STACK];
+  int sp = 0;
+  int32_t node = hdr.root_ref;
+  const uint32_t n_inner = uint32_t(hdr.n_inner);
+  for (uint32_t visits = 0;; ++visits) {
+    if (uint32_t(node) >= n_inner || visits > n_inner) {
+      bad = true;
+      break;
+    }
+    const float4* np = reinterpret_cast<const float4*>(nodes + node);
+    const float4 q0 = np[0], q1 = np[1], q2 = np[2];
+    const int4 q3 = *reinterpret_cast<const int4*>(np + 3);
+    if (COUNT) ++n_nodes;
+    float t0, t1;
+    bool h0 = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, h.t, t0);
+    bool h1 = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, h.t, t1);
+    if (h0 && q3.z > 0) {
+      leaf_test<COUNT>(r, tris, q3.x, q3.z, hdr.n_prims, h, n_tris, bad);
+      h0 = false;
+    }
+    if (h1 && q3.w > 0) {
+      leaf_test<COUNT>(r, tris, q3.y, q3.w, hdr.n_prims, h, n_tris, bad);
+      h1 = false;
+    }
+    if (h0 && h1) {
+      const bool first0 = t0 <= t1;
+      const int32_t nearer = first0 ? q3.x : q3.y;
+      const int32_t farther = first0 ? q3.y : q3.x;
+      if (sp < STACK) stack[sp++] = farther;
+      else overflow = true;
+      node = nearer;
+    } else if (h0) {
+      node = q3.x;
+    } else if (h1) {
+      node = q3.y;
+    } else {
+      if (sp == 0) break;
+      node = stack[--sp];
+    }
+  }
+  return h;
+}
+
+}  // namespace traverse

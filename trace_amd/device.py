@@ -1,0 +1,176 @@
+"""Thin object layer over the C-ABI: contexts, device meshes and BVH handles.
+
+All arrays cross as numpy buffers (host) or raw device addresses (ints).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import BvhInfo, check, load
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """A HIP device context (one stream).  ``trace_ctx_create``."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.trace_ctx_create(device, C.byref(h))
+        if rc != _lib.TRACE_OK:
+            raise _lib.TraceError(rc, f"trace_ctx_create(device={device}) failed")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.trace_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int) -> None:
+        check(rc, self.h)
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        self.check(self.lib.trace_ctx_set_stream(self.h, C.c_void_p(stream_ptr or 0)))
+
+    def synchronize(self) -> None:
+        self.check(self.lib.trace_ctx_synchronize(self.h))
+
+
+class Mesh:
+    """Device-resident triangles (9 doubles each), ``trace_mesh_create``."""
+
+    def __init__(self, ctx: Context, pos: np.ndarray | None = None, *, device_ptr: int | None = None,
+                 ntri: int | None = None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        if device_ptr is not None:
+            self.ntri = int(ntri)
+            ctx.check(ctx.lib.trace_mesh_create_device(ctx.h, C.c_void_p(device_ptr), self.ntri, C.byref(h)))
+        else:
+            p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 9)
+            self.ntri = p.shape[0]
+            ctx.check(ctx.lib.trace_mesh_create(ctx.h, _ptr(p), self.ntri, C.byref(h)))
+        self.h = h
+
+    def update(self, pos: np.ndarray) -> None:
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 9)
+        if p.shape[0] != self.ntri:
+            raise ValueError("update must keep the triangle count")
+        self.ctx.check(self.ctx.lib.trace_mesh_update(self.ctx.h, self.h, _ptr(p)))
+
+    def close(self) -> None:
+        if self.h:
+            self.ctx.lib.trace_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Bvh:
+    """A device BVH handle."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, n_prims: int):
+        self.ctx = ctx
+        self.h = handle
+        self.n_prims = n_prims
+
+    @classmethod
+    def lbvh(cls, ctx: Context, mesh: Mesh, enclosing=None, key_mode: int = _lib.KEY_REF63) -> "Bvh":
+        h = C.c_void_p()
+        enc = None if enclosing is None else np.ascontiguousarray(enclosing, dtype=np.float64)
+        ctx.check(ctx.lib.trace_lbvh_build(ctx.h, mesh.h, None if enc is None else _ptr(enc), key_mode,
+                                           C.byref(h)))
+        b = cls(ctx, h, mesh.ntri)
+        b._enc = enc
+        return b
+
+    def rebuild(self, mesh: Mesh, enclosing=None, key_mode: int = _lib.KEY_REF63) -> None:
+        """Asynchronous rebuild into this handle (no device allocation once sized)."""
+        enc = None if enclosing is None else np.ascontiguousarray(enclosing, dtype=np.float64)
+        self._enc = enc
+        self.ctx.check(self.ctx.lib.trace_lbvh_rebuild(self.ctx.h, mesh.h, None if enc is None else _ptr(enc),
+                                                       key_mode, self.h))
+
+    def close(self) -> None:
+        if self.h:
+            self.ctx.lib.trace_bvh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> BvhInfo:
+        out = BvhInfo()
+        self.ctx.check(self.ctx.lib.trace_bvh_get_info(self.ctx.h, self.h, C.byref(out)))
+        return out
+
+    def sorted(self):
+        keys = np.empty(self.n_prims, np.uint64)
+        perm = np.empty(self.n_prims, np.int32)
+        self.ctx.check(self.ctx.lib.trace_bvh_export_sorted(self.ctx.h, self.h, _ptr(keys), _ptr(perm)))
+        return keys, perm
+
+    def canonical(self):
+        n = C.c_int64()
+        self.ctx.check(self.ctx.lib.trace_bvh_export_canonical(self.ctx.h, self.h, 0, None, None, C.byref(n)))
+        rec = np.empty((n.value, 4), np.int32)
+        box = np.empty((n.value, 6), np.float32)
+        self.ctx.check(self.ctx.lib.trace_bvh_export_canonical(self.ctx.h, self.h, n.value, _ptr(rec), _ptr(box),
+                                                               C.byref(n)))
+        return rec, box
+
+    def cost(self) -> float:
+        c = C.c_double()
+        self.ctx.check(self.ctx.lib.trace_bvh_cost(self.ctx.h, self.h, C.byref(c)))
+        return c.value
+
+    def size(self) -> int:
+        s = C.c_int64()
+        self.ctx.check(self.ctx.lib.trace_bvh_size(self.ctx.h, self.h, C.byref(s)))
+        return s.value
+
+    def closest_hit(self, rays: np.ndarray):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        hit = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        self.ctx.check(self.ctx.lib.trace_closest_hit(self.ctx.h, self.h, _ptr(r), n, _ptr(hit), _ptr(t)))
+        return hit, t
+
+    def closest_hit_device(self, d_rays: int, n: int, d_hit: int, d_t: int) -> None:
+        self.ctx.check(self.ctx.lib.trace_closest_hit_device(self.ctx.h, self.h, C.c_void_p(d_rays), n,
+                                                             C.c_void_p(d_hit), C.c_void_p(d_t)))
+
+    def counts(self, rays: np.ndarray):
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        nodes = np.empty(n, np.int32)
+        tris = np.empty(n, np.int32)
+        self.ctx.check(self.ctx.lib.trace_closest_hit_counts(self.ctx.h, self.h, _ptr(r), n, _ptr(nodes),
+                                                             _ptr(tris)))
+        return nodes, tris
