@@ -107,7 +107,11 @@ def main():
     pos = scenes.random_triangles(args.tris, seed=1)
     rays = scenes.incoherent_rays(args.rays, seed=2)
     ctx = Context(0)
-    stream = torch.cuda.current_stream()
+    # A dedicated (default-priority) stream: torch's default stream has handle
+    # 0, which the C-ABI reads as "use the context's own stream".  Events are
+    # recorded on this same stream, so they bracket the kernels exactly.
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     ctx.set_stream(stream.cuda_stream)
     mesh = Mesh(ctx, pos)
     bvh = Bvh.lbvh(ctx, mesh, key_mode=KEY_REF63)

@@ -99,7 +99,7 @@ def test_config2_sampled(ctx, oracle, key_mode):
     inc = scenes.incoherent_rays(200_000, seed=2)
     hit, t = bvh.closest_hit(inc)
     ohit, ot = tree.closest_hit(inc)
-    _check(hit, t, ohit, ot, min_hits=0.9)
+    _check(hit, t, ohit, ot, min_hits=0.8)
     coh = scenes.coherent_rays_cfg2(4096, 2048)[::41]
     hit, t = bvh.closest_hit(coh)
     ohit, ot = tree.closest_hit(coh)
@@ -115,7 +115,7 @@ def test_config2_full_batch_properties(ctx):
     rays = scenes.incoherent_rays(8_000_000, seed=2)
     hit, t = bvh.closest_hit(rays)
     h = hit >= 0
-    assert h.mean() > 0.9
+    assert h.mean() > 0.8
     assert np.all(t[h] >= rays[h, 3])
     sel = np.flatnonzero(h)[::97]
     tri = pos[hit[sel]].reshape(-1, 3, 3)

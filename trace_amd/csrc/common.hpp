@@ -128,7 +128,7 @@ struct trace_bvh {
   int32_t arity = 2;
   DevArray<uint64_t> keys;     // sorted Morton keys (LBVH only)
   DevArray<int32_t> perm;      // sorted position -> caller triangle index
-  DevArray<float> tris;        // sorted order: v0, v1-v0, v2-v0 (9 floats)
+  DevArray<float> tris;        // sorted order: v0, v1, v2 (9 floats)
   DevArray<float> prim_box;    // sorted order: lo.xyz hi.xyz (6 floats, outward-rounded)
   DevArray<BvhNode> nodes;     // binary traversal tree
   DevArray<int32_t> parent;    // inner parents then leaf parents ((p << 1) | slot, -1 = root)

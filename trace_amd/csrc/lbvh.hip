@@ -289,8 +289,9 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
   if (i == 0) parent_inner[0] = -1;
 }
 
-// FP32 triangles (v0, v1 - v0, v2 - v0; differences taken in FP64 as
-// primitive.go:111-112 does) and outward-rounded FP32 boxes, in sorted order.
+// FP32 triangles (v0, v1, v2) and outward-rounded FP32 boxes, in sorted
+// order.  Vertices rather than edges are kept so that the winning hit of a
+// ray can be re-evaluated in FP64 from the same vertices the reference uses.
 __global__ __launch_bounds__(256) void reorder_kernel(const double* __restrict__ pos, int64_t n,
                                                       const uint32_t* __restrict__ perm,
                                                       float* __restrict__ tris,
@@ -307,8 +308,8 @@ __global__ __launch_bounds__(256) void reorder_kernel(const double* __restrict__
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     t[a] = float(v[a]);
-    t[3 + a] = float(v[3 + a] - v[a]);
-    t[6 + a] = float(v[6 + a] - v[a]);
+    t[3 + a] = float(v[3 + a]);
+    t[6 + a] = float(v[6 + a]);
     b[a] = f32_down(min3(v[a], v[3 + a], v[6 + a]));
     b[3 + a] = f32_up(max3(v[a], v[3 + a], v[6 + a]));
   }

@@ -21,7 +21,8 @@ __global__ __launch_bounds__(kThreads) void closest_hit_kernel(
   Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   int32_t nn = 0, nt = 0;
   bool overflow = false, bad = false;
-  const Hit h = closest_hit_one<kStack, COUNT>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
+  Hit h = closest_hit_one<kStack, COUNT>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
+  refine_hit_fp64(tris, ray, h);
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);
   hit_out[i] = h.pos >= 0 ? perm[h.pos] : -1;

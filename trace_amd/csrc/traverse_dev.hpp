@@ -59,8 +59,8 @@ __device__ inline void tri_test(const RayPre& r, const float* __restrict__ tris,
                                 Hit& h) {
   const float* t = tris + 9 * size_t(k);
   const float v0x = t[0], v0y = t[1], v0z = t[2];
-  const float e1x = t[3], e1y = t[4], e1z = t[5];
-  const float e2x = t[6], e2y = t[7], e2z = t[8];
+  const float e1x = t[3] - v0x, e1y = t[4] - v0y, e1z = t[5] - v0z;
+  const float e2x = t[6] - v0x, e2y = t[7] - v0y, e2z = t[8] - v0z;
   // p = d x e2, det = e1 . p
   const float px = r.dy * e2z - r.dz * e2y;
   const float py = r.dz * e2x - r.dx * e2z;
@@ -85,6 +85,33 @@ __device__ inline void tri_test(const RayPre& r, const float* __restrict__ tris,
   h.u = u;
   h.v = v;
   h.front = det > 0.0f;
+}
+
+// Re-evaluates the accepted triangle in FP64 with the reference's operation
+// order (primitive.go:172-208: p = d x e2, det = e1.p, invDet = 1/det,
+// t = e2.q * invDet), so the reported t and barycentrics carry FP64 accuracy
+// even where the FP32 search was ill-conditioned (grazing rays).
+__device__ inline void refine_hit_fp64(const float* __restrict__ tris, const Ray& ray, Hit& h) {
+#pragma clang fp contract(off)
+  if (h.pos < 0) return;
+  const float* t = tris + 9 * size_t(h.pos);
+  const double v0x = t[0], v0y = t[1], v0z = t[2];
+  const double e1x = double(t[3]) - v0x, e1y = double(t[4]) - v0y, e1z = double(t[5]) - v0z;
+  const double e2x = double(t[6]) - v0x, e2y = double(t[7]) - v0y, e2z = double(t[8]) - v0z;
+  const double dx = ray.dx, dy = ray.dy, dz = ray.dz;
+  const double px = dy * e2z - dz * e2y;
+  const double py = dz * e2x - dx * e2z;
+  const double pz = dx * e2y - dy * e2x;
+  const double det = e1x * px + e1y * py + e1z * pz;
+  if (det == 0.0) return;
+  const double inv = 1.0 / det;
+  const double tx = double(ray.ox) - v0x, ty = double(ray.oy) - v0y, tz = double(ray.oz) - v0z;
+  const double qx = ty * e1z - tz * e1y;
+  const double qy = tz * e1x - tx * e1z;
+  const double qz = tx * e1y - ty * e1x;
+  h.u = float((tx * px + ty * py + tz * pz) * inv);
+  h.v = float((dx * qx + dy * qy + dz * qz) * inv);
+  h.t = float((e2x * qx + e2y * qy + e2z * qz) * inv);
 }
 
 // Leaf [first, first + cnt) must lie inside the triangle array; a malformed
