@@ -130,6 +130,49 @@ int trace_closest_hit_device(trace_ctx* ctx, const trace_bvh* bvh, const float* 
 int trace_closest_hit_counts(trace_ctx* ctx, const trace_bvh* bvh, const float* h_rays,
                              int64_t nrays, int32_t* h_nodes, int32_t* h_tris);
 
+/* ---- path tracing -------------------------------------------------------- */
+/* Per-triangle shading: vertex normals (9 doubles per triangle as
+ * NewTriangle takes them, primitive.go:102-114; NULL = geometric normals as
+ * NewTriangleWithoutNormals computes them, primitive.go:116-138), a material
+ * index per triangle, and a material table of 4 floats each: kind
+ * (TRACE_MAT_DIFFUSE / TRACE_MAT_LIGHT), r, g, b (Diffuse.Albedo or
+ * Light.Color, material.go:13-42).  The reference assigns one Material per
+ * Mesh (scene.go:118-139); a per-triangle index expresses any set of meshes. */
+int trace_mesh_set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
+                           const int32_t* h_mat_index, const float* h_materials, int32_t nmat);
+
+/* Camera as camera.go:34-66 stores it (origin, lowerLeftCorner, horizontal,
+ * vertical); the direction of sample (s, t) is LL + s*H + t*V - origin. */
+typedef struct {
+  double origin[3];
+  double lower_left[3];
+  double horizontal[3];
+  double vertical[3];
+} trace_camera;
+
+typedef struct {
+  int32_t width;
+  int32_t height;
+  int32_t spp;           /* ImageRenderer.Spp */
+  int32_t max_depth;     /* ImageRenderer.MaxDepth */
+  int32_t miss;          /* TRACE_MISS_* (ImageRenderer.Miss) */
+  int32_t pad;
+  uint64_t seed;         /* Philox key of the sample stream */
+  int64_t sample_offset; /* index of the first sample (progressive rendering) */
+} trace_render_params;
+
+/* ImageRenderer.RenderToBuffer (render.go:91-129) with DefaultClosestHitShader
+ * and RandomSampling.  Outputs 3 floats per pixel, row-major with row 0 at the
+ * bottom (y as the reference indexes its buffers): the sum and the sum of
+ * squares of the spp samples, and the last sample (NULL to skip; FrameBuffer
+ * keeps only the last pass, buffer.go:90-92).  The bvh must be built from mesh. */
+int trace_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                 const trace_camera* cam, const trace_render_params* prm, float* h_sum,
+                 float* h_sumsq, float* h_last);
+int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                        const trace_camera* cam, const trace_render_params* prm, float* d_sum,
+                        float* d_sumsq, float* d_last);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,8 @@
 #include <thread>
 #include <vector>
 
+#include "../trace_amd/csrc/philox.hpp"
+
 namespace {
 
 struct V3 {
@@ -427,6 +429,151 @@ void oracle_closest_hit_brute(const double* pos, int64_t n, const float* rays, i
       for (int64_t k = 0; k < n; ++k) tri_hit(pos + 9 * k, r, rr[3], h.t, h, int32_t(k));
       hit[i] = h.prim;
       tout[i] = h.t;
+    }
+  });
+}
+
+void oracle_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                   uint32_t* out4) {
+  const Philox4 r = philox4x32_10(c0, c1, c2, c3, k0, k1);
+  std::memcpy(out4, r.v, sizeof(r.v));
+}
+
+}  // extern "C"
+
+// ---- path tracing (render.go:91-129, 201-246; material.go:13-42) -----------
+namespace {
+
+struct Shading {
+  const double* nrm;  // 9 per triangle (caller order)
+  const int32_t* mat_idx;
+  const float* mats;  // kind, r, g, b
+  int miss;
+  int max_depth;
+  uint64_t seed;
+};
+
+V3 miss_color(int miss, V3 d) {
+  switch (miss) {
+    case 1:
+      return {1, 1, 1};
+    case 2: {  // SkyMissShader
+      const double len = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+      const double t = 0.5 * (d.y / len + 1);
+      return {0.8 * (1.0 - t) + 0.25 * t, 0.8 * (1.0 - t) + 0.35 * t, 0.8 * (1.0 - t) + 0.5 * t};
+    }
+    case 3:
+      return {1, .96, .78};
+    default:
+      return {0, 0, 0};
+  }
+}
+
+// BVH.intersected with the hit record the shader needs.
+bool trace_ray(const Tree& t, V3 o, V3 d, double tmin, HitRec& h) {
+  Ray r;  // camera.go:138-158 ray.reuse
+  r.o = o;
+  r.d = d;
+  r.inv = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+  r.sign[0] = r.inv.x < 0;
+  r.sign[1] = r.inv.y < 0;
+  r.sign[2] = r.inv.z < 0;
+  h.t = INFINITY;
+  h.prim = -1;
+  return node_hit(t, t.root, r, tmin, INFINITY, h);
+}
+
+// DefaultClosestHitShader (render.go:209-224): vertex at depth k.
+V3 shade(const Tree& t, const Shading& s, uint64_t path, int k, V3 o, V3 d, const HitRec& h) {
+  if (k > s.max_depth) return {0, 0, 0};
+  const float* m = s.mats + 4 * s.mat_idx[h.prim];
+  if (m[0] == 1.0f) return {m[1], m[2], m[3]};  // Light: emission, no scatter
+  // Diffuse.scatter (material.go:29-38)
+  const double* n = s.nrm + 9 * size_t(h.prim);
+  const double u = h.u, v = h.v, w0 = 1 - u - v;
+  V3 nn{(n[3] * u + n[6] * v) + n[0] * w0, (n[4] * u + n[7] * v) + n[1] * w0,
+        (n[5] * u + n[8] * v) + n[2] * w0};
+  if (!h.front) nn = {-nn.x, -nn.y, -nn.z};
+  const Philox4 r = path_draw(s.seed, path, uint32_t(k + 1));
+  const double z = 1.0 - 2.0 * double(u01(r.v[0]));
+  const double rad = std::sqrt(std::max(0.0, 1.0 - z * z));
+  const double phi = 6.283185307179586 * double(u01(r.v[1]));
+  V3 dir{nn.x + rad * std::cos(phi), nn.y + rad * std::sin(phi), nn.z + z};
+  if (std::fabs(dir.x) <= 1e-7 && std::fabs(dir.y) <= 1e-7 && std::fabs(dir.z) <= 1e-7) dir = nn;
+  const V3 p{o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t};
+  HitRec h2{};
+  V3 next;
+  if (trace_ray(t, p, dir, 0.0001, h2)) next = shade(t, s, path, k + 1, p, dir, h2);
+  else next = miss_color(s.miss, dir);
+  return {next.x * m[1], next.y * m[2], next.z * m[3]};  // Le = 0 for Diffuse
+}
+
+}  // namespace
+
+extern "C" {
+
+// Per-pixel sum, sum of squares and last sample of spp paths, FP64.  cam =
+// origin, lower_left, horizontal, vertical (12 doubles).  nrm = NULL uses
+// geometric normals (primitive.go:116-138).  path_out (optional) receives
+// every path's value, indexed [sample][pixel][rgb].
+void oracle_render(const oracle_tree* t, const double* nrm, const int32_t* mat_idx,
+                   const float* mats, const double* cam, int32_t W, int32_t H, int32_t spp,
+                   int32_t max_depth, int32_t miss, uint64_t seed, int64_t sample_offset,
+                   int threads, double* sum, double* sumsq, double* last, double* path_out) {
+  std::vector<double> geo;
+  if (!nrm) {
+    geo.resize(9 * size_t(t->n));
+    for (int64_t i = 0; i < t->n; ++i) {
+      const double* p = &t->pos[9 * size_t(i)];
+      for (int c = 0; c < 3; ++c) {
+        const V3 a{p[3 * c], p[3 * c + 1], p[3 * c + 2]};
+        const int ri = (c + 1) % 3, li = (c + 2) % 3;
+        const V3 rgt{p[3 * ri], p[3 * ri + 1], p[3 * ri + 2]};
+        const V3 lft{p[3 * li], p[3 * li + 1], p[3 * li + 2]};
+        const V3 cr = cross(sub(rgt, a), sub(lft, a));
+        const double len = std::sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+        geo[9 * i + 3 * c + 0] = cr.x / len;
+        geo[9 * i + 3 * c + 1] = cr.y / len;
+        geo[9 * i + 3 * c + 2] = cr.z / len;
+      }
+    }
+    nrm = geo.data();
+  }
+  const Shading s{nrm, mat_idx, mats, miss, max_depth, seed};
+  const V3 org{cam[0], cam[1], cam[2]}, ll{cam[3], cam[4], cam[5]}, hz{cam[6], cam[7], cam[8]},
+      vt{cam[9], cam[10], cam[11]};
+  const int64_t wh = int64_t(W) * H;
+  parallel_for(wh, threads, [&](int64_t b, int64_t e) {
+    for (int64_t pix = b; pix < e; ++pix) {
+      const int32_t x = int32_t(pix % W), y = int32_t(pix / W);
+      double S[3] = {0, 0, 0}, Q[3] = {0, 0, 0};
+      V3 L{0, 0, 0};
+      for (int32_t smp = 0; smp < spp; ++smp) {
+        const uint64_t path = uint64_t((sample_offset + smp) * wh + pix);
+        const Philox4 r = path_draw(seed, path, 0u);
+        const double sx = (double(x) + double(u01(r.v[0]))) / double(W - 1);
+        const double sy = (double(y) + double(u01(r.v[1]))) / double(H - 1);
+        const V3 d{ll.x + hz.x * sx + vt.x * sy - org.x, ll.y + hz.y * sx + vt.y * sy - org.y,
+                   ll.z + hz.z * sx + vt.z * sy - org.z};
+        HitRec h{};
+        if (trace_ray(*t, org, d, 0.001, h)) L = shade(*t, s, path, 0, org, d, h);
+        else L = miss_color(miss, d);
+        S[0] += L.x; S[1] += L.y; S[2] += L.z;
+        Q[0] += L.x * L.x; Q[1] += L.y * L.y; Q[2] += L.z * L.z;
+        if (path_out) {
+          double* po = path_out + 3 * (size_t(smp) * wh + pix);
+          po[0] = L.x; po[1] = L.y; po[2] = L.z;
+        }
+      }
+      for (int c = 0; c < 3; ++c) {
+        sum[3 * pix + c] = S[c];
+        sumsq[3 * pix + c] = Q[c];
+      }
+      if (last) {
+        last[3 * pix + 0] = L.x;
+        last[3 * pix + 1] = L.y;
+        last[3 * pix + 2] = L.z;
+      }
     }
   });
 }

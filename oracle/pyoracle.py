@@ -44,6 +44,9 @@ def load() -> C.CDLL:
         "oracle_tree_cost": ([_P], C.c_double),
         "oracle_closest_hit": ([_P, _P, C.c_int64, _P, _P, C.c_int], None),
         "oracle_closest_hit_brute": ([_P, C.c_int64, _P, C.c_int64, _P, _P, C.c_int], None),
+        "oracle_philox": ([C.c_uint32] * 6 + [_P], None),
+        "oracle_render": ([_P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                           C.c_uint64, C.c_int64, C.c_int, _P, _P, _P, _P], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -132,6 +135,33 @@ class Tree:
         t = np.empty(n, np.float64)
         self.lib.oracle_closest_hit(self.h, _p(r), n, _p(hit), _p(t), threads or _threads())
         return hit, t
+
+
+def philox(counter, key) -> list[int]:
+    out = np.empty(4, np.uint32)
+    load().oracle_philox(*[int(c) for c in counter], *[int(k) for k in key], _p(out))
+    return [int(v) for v in out]
+
+
+def render(tree: Tree, mat_idx, materials, cam, width: int, height: int, spp: int, max_depth: int,
+           miss: int, seed: int, sample_offset: int = 0, normals=None, path_values: bool = False,
+           threads: int | None = None):
+    """FP64 restatement of RenderToBuffer.  cam has origin, lower_left,
+    horizontal, vertical (scenes.Camera).  Returns (sum, sumsq, last[, paths]),
+    each (height, width, 3) float64 (paths: (spp, height, width, 3))."""
+    mi = np.ascontiguousarray(mat_idx, np.int32)
+    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 4)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.float64)
+    c = np.ascontiguousarray(np.concatenate([cam.origin, cam.lower_left, cam.horizontal, cam.vertical]), np.float64)
+    shape = (height, width, 3)
+    s = np.empty(shape, np.float64)
+    q = np.empty(shape, np.float64)
+    last = np.empty(shape, np.float64)
+    paths = np.empty((spp, *shape), np.float64) if path_values else None
+    load().oracle_render(tree.h, None if nrm is None else _p(nrm), _p(mi), _p(mats), _p(c), width, height, spp,
+                         max_depth, miss, seed, sample_offset, threads or _threads(), _p(s), _p(q), _p(last),
+                         None if paths is None else _p(paths))
+    return (s, q, last, paths) if path_values else (s, q, last)
 
 
 def closest_hit_brute(pos: np.ndarray, rays: np.ndarray, threads: int | None = None):

@@ -73,7 +73,34 @@ SIGNATURES = {
     "trace_closest_hit": [_P, _P, _P, _I64, _P, _P],
     "trace_closest_hit_device": [_P, _P, _P, _I64, _P, _P],
     "trace_closest_hit_counts": [_P, _P, _P, _I64, _P, _P],
+    "trace_mesh_set_shading": [_P, _P, _P, _P, _P, _I32],
+    "trace_render": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "trace_render_device": [_P, _P, _P, _P, _P, _P, _P, _P],
 }
+
+
+class Camera(C.Structure):
+    """trace_camera: origin, lower_left, horizontal, vertical."""
+
+    _fields_ = [
+        ("origin", C.c_double * 3),
+        ("lower_left", C.c_double * 3),
+        ("horizontal", C.c_double * 3),
+        ("vertical", C.c_double * 3),
+    ]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("spp", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("miss", C.c_int32),
+        ("pad", C.c_int32),
+        ("seed", C.c_uint64),
+        ("sample_offset", C.c_int64),
+    ]
 _RESTYPES = {"trace_last_error": C.c_char_p}
 
 _lib = None

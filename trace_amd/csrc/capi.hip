@@ -6,6 +6,7 @@
 
 #include "common.hpp"
 #include "lbvh.hpp"
+#include "render.hpp"
 #include "traverse.hpp"
 
 namespace {
@@ -429,6 +430,81 @@ int trace_closest_hit_counts(trace_ctx* ctx, const trace_bvh* bvh, const float* 
   std::vector<float> t(nrays > 0 ? nrays : 0);
   return closest_hit_host(ctx, bvh, h_rays, nrays, hit.data(), t.data(), h_nodes, h_tris,
                           "trace_closest_hit_counts");
+}
+
+}  // extern "C"
+
+// ---- path tracing --------------------------------------------------------------
+extern "C" {
+
+int trace_mesh_set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
+                           const int32_t* h_mat_index, const float* h_materials, int32_t nmat) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !h_mat_index || !h_materials || nmat <= 0)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_set_shading: need material indices and nmat > 0");
+  for (int64_t i = 0; i < mesh->ntri; ++i)
+    if (h_mat_index[i] < 0 || h_mat_index[i] >= nmat)
+      TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_set_shading: triangle %lld has material %d of %d",
+                 (long long)i, h_mat_index[i], nmat);
+  for (int32_t m = 0; m < nmat; ++m) {
+    const float kind = h_materials[4 * m];
+    if (kind != float(TRACE_MAT_DIFFUSE) && kind != float(TRACE_MAT_LIGHT))
+      TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_mesh_set_shading: material %d kind %g (only Diffuse and Light)",
+                 m, double(kind));
+  }
+  return render::set_shading(ctx, mesh, h_normals, h_mat_index, h_materials, nmat);
+}
+
+static int check_render_args(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                             const trace_camera* cam, const trace_render_params* prm,
+                             const char* what) {
+  if (!mesh || !bvh || !cam || !prm) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: null argument", what);
+  if (!mesh->has_shading) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: mesh has no shading data (trace_mesh_set_shading)", what);
+  if (bvh->n_prims != mesh->ntri) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: bvh has %lld triangles, mesh %lld", what, (long long)bvh->n_prims, (long long)mesh->ntri);
+  if (prm->width < 2 || prm->height < 2 || prm->spp < 0 || prm->max_depth < 0 || prm->sample_offset < 0)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: need width, height >= 2 (samples divide by W-1, H-1), spp, max_depth >= 0", what);
+  if (prm->miss < TRACE_MISS_DEFAULT || prm->miss > TRACE_MISS_SUN) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: unknown miss shader %d", what, prm->miss);
+  if (int64_t(prm->width) * prm->height > (int64_t(1) << 28)) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "%s: image too large", what);
+  return TRACE_OK;
+}
+
+int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                        const trace_camera* cam, const trace_render_params* prm, float* d_sum,
+                        float* d_sumsq, float* d_last) {
+  if (!ctx) return TRACE_E_INVALID;
+  int rc = check_render_args(ctx, mesh, bvh, cam, prm, "trace_render_device");
+  if (rc) return rc;
+  if (!d_sum || !d_sumsq) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_device: null output");
+  if (prm->spp == 0) return TRACE_OK;
+  return render::render(ctx, mesh, bvh, cam, prm, d_sum, d_sumsq, d_last);
+}
+
+int trace_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                 const trace_camera* cam, const trace_render_params* prm, float* h_sum,
+                 float* h_sumsq, float* h_last) {
+  if (!ctx) return TRACE_E_INVALID;
+  int rc = check_render_args(ctx, mesh, bvh, cam, prm, "trace_render");
+  if (rc) return rc;
+  if (!h_sum || !h_sumsq) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render: null output");
+  const size_t bytes = size_t(prm->width) * prm->height * 3 * sizeof(float);
+  if (prm->spp == 0) {
+    std::memset(h_sum, 0, bytes);
+    std::memset(h_sumsq, 0, bytes);
+    if (h_last) std::memset(h_last, 0, bytes);
+    return TRACE_OK;
+  }
+  DevArray<float> out;
+  HIP_TRY(ctx, out.alloc(3 * bytes / sizeof(float)));
+  float* d_sum = out.p;
+  float* d_sq = out.p + bytes / sizeof(float);
+  float* d_last = out.p + 2 * bytes / sizeof(float);
+  rc = render::render(ctx, mesh, bvh, cam, prm, d_sum, d_sq, h_last ? d_last : nullptr);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(h_sum, d_sum, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(h_sumsq, d_sq, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  if (h_last) HIP_TRY(ctx, hipMemcpyAsync(h_last, d_last, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return ctx_check_device_error(ctx, "trace_render");
 }
 
 }  // extern "C"

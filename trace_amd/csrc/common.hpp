@@ -87,6 +87,12 @@ int ctx_check_device_error(trace_ctx* ctx, const char* what);
 struct trace_mesh {
   int64_t ntri = 0;
   DevArray<double> pos;  // 9 doubles per triangle, caller order
+  // shading data, caller order (set by trace_mesh_set_shading)
+  DevArray<float> nrm;        // 9 floats per triangle: vertex normals n0, n1, n2
+  DevArray<int32_t> mat_idx;  // material index per triangle
+  DevArray<float4> mats;      // (kind, r, g, b) per material
+  int32_t nmat = 0;
+  bool has_shading = false;
 };
 
 // Inner node of the binary traversal tree: both child boxes in one 64-byte

@@ -1,0 +1,303 @@
+// Wavefront path tracer (ImageRenderer.RenderToBuffer, render.go:91-129, with
+// DefaultClosestHitShader, render.go:209-224).
+//
+// Paths are processed in batches of whole samples.  Per batch:
+//   camera_kernel   one camera ray per (sample, pixel), jittered as
+//                   RandomSampling does (render.go:201-205)
+//   bounce_kernel   for segment k = 0 .. max_depth+1: closest hit of every
+//                   live path, then shade -- miss adds throughput x miss
+//                   colour; a hit at vertex k > max_depth ends the path with
+//                   nothing added; a light adds throughput x emission and ends
+//                   the path; a diffuse hit scatters (material.go:29-38) and
+//                   the path is appended to the next ray queue (wave-aggregated
+//                   atomic compaction)
+//   accumulate      per pixel: sum, sum of squares and last sample, in sample
+//                   order (deterministic)
+// Forward accumulation of throughput x emission gives the same value as the
+// reference's recursion L = Le + albedo (x) L_next.
+#include "philox.hpp"
+#include "render.hpp"
+#include "traverse_dev.hpp"
+
+namespace render {
+
+constexpr int kThreads = 128;
+constexpr int kStack = 64;
+constexpr int64_t kMaxBatchPaths = int64_t(1) << 22;  // ~4M paths per batch
+
+struct Cam {
+  float o[3], ll[3], h[3], v[3];
+};
+
+__global__ void geo_normals_kernel(const double* __restrict__ pos, int64_t n, float* __restrict__ nrm) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const double* p = pos + 9 * i;
+  // calcNormal(point, right, left) = unit((right - point) x (left - point)),
+  // primitive.go:116-138, for each vertex in turn.
+  for (int c = 0; c < 3; ++c) {
+    const double* a = p + 3 * c;
+    const double* r = p + 3 * ((c + 1) % 3);
+    const double* l = p + 3 * ((c + 2) % 3);
+    const double bx = r[0] - a[0], by = r[1] - a[1], bz = r[2] - a[2];
+    const double ax = l[0] - a[0], ay = l[1] - a[1], az = l[2] - a[2];
+    const double cx = by * az - bz * ay, cy = bz * ax - bx * az, cz = bx * ay - by * ax;
+    const double len = sqrt(cx * cx + cy * cy + cz * cz);
+    nrm[9 * i + 3 * c + 0] = float(cx / len);
+    nrm[9 * i + 3 * c + 1] = float(cy / len);
+    nrm[9 * i + 3 * c + 2] = float(cz / len);
+  }
+}
+
+__global__ void d2f_kernel(const double* __restrict__ in, size_t n, float* __restrict__ out) {
+  const size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (i < n) out[i] = float(in[i]);
+}
+
+__global__ __launch_bounds__(256) void camera_kernel(Cam cam, int32_t W, int32_t H,
+                                                     int64_t n_paths, int64_t path_base,
+                                                     uint64_t seed, float4* __restrict__ ro,
+                                                     float4* __restrict__ rd,
+                                                     float4* __restrict__ thr,
+                                                     float4* __restrict__ rad,
+                                                     int32_t* __restrict__ queue,
+                                                     uint32_t* __restrict__ counts) {
+  const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p == 0) {
+    counts[0] = uint32_t(n_paths);
+    counts[1] = 0;
+  }
+  if (p >= n_paths) return;
+  const int64_t wh = int64_t(W) * H;
+  const int64_t pix = p % wh;
+  const int32_t x = int32_t(pix % W), y = int32_t(pix / W);
+  const Philox4 r = path_draw(seed, uint64_t(path_base + p), 0u);
+  const float s = (float(x) + u01(r.v[0])) / float(W - 1);
+  const float t = (float(y) + u01(r.v[1])) / float(H - 1);
+  const float dx = cam.ll[0] + s * cam.h[0] + t * cam.v[0] - cam.o[0];
+  const float dy = cam.ll[1] + s * cam.h[1] + t * cam.v[1] - cam.o[1];
+  const float dz = cam.ll[2] + s * cam.h[2] + t * cam.v[2] - cam.o[2];
+  ro[p] = make_float4(cam.o[0], cam.o[1], cam.o[2], 0.0f);
+  rd[p] = make_float4(dx, dy, dz, 0.0f);
+  thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  queue[p] = int32_t(p);
+}
+
+// render.go:228-246
+__device__ inline float3 miss_color(int miss, float dx, float dy, float dz) {
+  switch (miss) {
+    case TRACE_MISS_WHITE:
+      return make_float3(1.0f, 1.0f, 1.0f);
+    case TRACE_MISS_SKY: {
+      const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float t = 0.5f * (dy / len + 1.0f);
+      return make_float3(0.8f * (1.0f - t) + 0.25f * t, 0.8f * (1.0f - t) + 0.35f * t,
+                         0.8f * (1.0f - t) + 0.5f * t);
+    }
+    case TRACE_MISS_SUN:
+      return make_float3(1.0f, 0.96f, 0.78f);
+    default:
+      return make_float3(0.0f, 0.0f, 0.0f);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void bounce_kernel(
+    const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
+    const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
+    const float* __restrict__ nrm, const int32_t* __restrict__ mat_idx,
+    const float4* __restrict__ mats, int32_t k, int32_t max_depth, int32_t miss,
+    uint64_t seed, int64_t path_base, float4* __restrict__ ro, float4* __restrict__ rd,
+    float4* __restrict__ thr, float4* __restrict__ rad, const int32_t* __restrict__ q_in,
+    int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
+    uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err) {
+  const uint32_t count = *count_in;
+  const uint32_t i = blockIdx.x * uint32_t(kThreads) + threadIdx.x;
+  bool push = false;
+  int32_t p = 0;
+  if (i < count) {
+    const BvhHeader hdr = *hdr_p;
+    p = q_in[i];
+    const float4 o4 = ro[p];
+    const float4 d4 = rd[p];
+    traverse::Ray ray{o4.x, o4.y, o4.z, k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
+    int32_t nn = 0, nt = 0;
+    bool overflow = false, bad = false;
+    traverse::Hit h =
+        traverse::closest_hit_one<kStack, false>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
+    if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
+    if (bad) atomicOr(d_err, TRACE_ERR_TREE);
+    traverse::refine_hit_fp64(tris, ray, h);
+    const float4 T = thr[p];
+    float4 L = rad[p];
+    if (h.pos < 0) {
+      const float3 m = miss_color(miss, d4.x, d4.y, d4.z);
+      L.x += T.x * m.x;
+      L.y += T.y * m.y;
+      L.z += T.z * m.z;
+      rad[p] = L;
+    } else if (k <= max_depth) {
+      const int32_t prim = perm[h.pos];
+      const float4 m = mats[mat_idx[prim]];
+      if (m.x == float(TRACE_MAT_LIGHT)) {
+        L.x += T.x * m.y;
+        L.y += T.y * m.z;
+        L.z += T.z * m.w;
+        rad[p] = L;
+      } else {
+        // interpolated vertex normal, flipped to face the ray (primitive.go:145-150, 200-205)
+        const float* n = nrm + 9 * size_t(prim);
+        const float w0 = 1.0f - h.u - h.v;
+        float nx = (n[3] * h.u + n[6] * h.v) + n[0] * w0;
+        float ny = (n[4] * h.u + n[7] * h.v) + n[1] * w0;
+        float nz = (n[5] * h.u + n[8] * h.v) + n[2] * w0;
+        if (!h.front) {
+          nx = -nx;
+          ny = -ny;
+          nz = -nz;
+        }
+        // n + uniform unit vector (util.go:15-26 draws a uniform direction)
+        const Philox4 r = path_draw(seed, uint64_t(path_base + p), uint32_t(k + 1));
+        const float z = 1.0f - 2.0f * u01(r.v[0]);
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sphi, cphi;
+        sincosf(6.2831853071795864f * u01(r.v[1]), &sphi, &cphi);
+        float sx = nx + rr * cphi, sy = ny + rr * sphi, sz = nz + z;
+        if (fabsf(sx) <= 1e-7f && fabsf(sy) <= 1e-7f && fabsf(sz) <= 1e-7f) {
+          sx = nx;
+          sy = ny;
+          sz = nz;
+        }
+        // new origin = ray.position(t) (camera.go:108-111), no offset
+        ro[p] = make_float4(o4.x + d4.x * h.t, o4.y + d4.y * h.t, o4.z + d4.z * h.t, 0.0f);
+        rd[p] = make_float4(sx, sy, sz, 0.0f);
+        thr[p] = make_float4(T.x * m.y, T.y * m.z, T.z * m.w, 0.0f);
+        push = true;
+      }
+    }
+  }
+  // wave-aggregated append to the next queue
+  const uint64_t b = ballot64(push);
+  if (b) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)b) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count_out, uint32_t(__popcll(b)));
+    base = __shfl(base, leader, 64);
+    if (push) q_out[base + __popcll(b & lanemask_lt())] = p;
+  }
+}
+
+__global__ void accumulate_kernel(const float4* __restrict__ rad, int64_t wh, int32_t nsamp,
+                                  int32_t first_batch, float* __restrict__ sum,
+                                  float* __restrict__ sumsq, float* __restrict__ last) {
+  const int64_t pix = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (pix >= wh) return;
+  float s0 = 0, s1 = 0, s2 = 0, q0 = 0, q1 = 0, q2 = 0;
+  float4 v = make_float4(0, 0, 0, 0);
+  for (int32_t s = 0; s < nsamp; ++s) {
+    v = rad[int64_t(s) * wh + pix];
+    s0 += v.x;
+    s1 += v.y;
+    s2 += v.z;
+    q0 += v.x * v.x;
+    q1 += v.y * v.y;
+    q2 += v.z * v.z;
+  }
+  float* S = sum + 3 * pix;
+  float* Q = sumsq + 3 * pix;
+  if (first_batch) {
+    S[0] = s0; S[1] = s1; S[2] = s2;
+    Q[0] = q0; Q[1] = q1; Q[2] = q2;
+  } else {
+    S[0] += s0; S[1] += s1; S[2] += s2;
+    Q[0] += q0; Q[1] += q1; Q[2] += q2;
+  }
+  if (last) {
+    last[3 * pix + 0] = v.x;
+    last[3 * pix + 1] = v.y;
+    last[3 * pix + 2] = v.z;
+  }
+}
+
+int set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
+                const int32_t* h_mat_index, const float* h_materials, int32_t nmat) {
+  const int64_t n = mesh->ntri;
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, mesh->nrm.alloc(9 * n));
+  HIP_TRY(ctx, mesh->mat_idx.alloc(n));
+  HIP_TRY(ctx, mesh->mats.alloc(nmat));
+  if (h_normals) {
+    DevArray<double> tmp;
+    HIP_TRY(ctx, tmp.alloc(9 * n));
+    HIP_TRY(ctx, hipMemcpyAsync(tmp.p, h_normals, 9 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    d2f_kernel<<<int(ceil_div(9 * n, 256)), 256, 0, st>>>(tmp.p, size_t(9 * n), mesh->nrm.p);
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+  } else {
+    geo_normals_kernel<<<int(ceil_div(n, 256)), 256, 0, st>>>(mesh->pos.p, n, mesh->nrm.p);
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(mesh->mat_idx.p, h_mat_index, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(mesh->mats.p, h_materials, nmat * sizeof(float4), hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  mesh->nmat = nmat;
+  mesh->has_shading = true;
+  return TRACE_OK;
+}
+
+int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
+           const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last) {
+  const int64_t wh = int64_t(prm->width) * prm->height;
+  const int64_t spb = std::max<int64_t>(1, std::min<int64_t>(prm->spp, kMaxBatchPaths / wh));
+  const int64_t max_paths = spb * wh;
+  hipStream_t st = ctx->stream;
+
+  ScratchPlan plan;
+  const size_t o_ro = plan.take(max_paths * sizeof(float4));
+  const size_t o_rd = plan.take(max_paths * sizeof(float4));
+  const size_t o_thr = plan.take(max_paths * sizeof(float4));
+  const size_t o_rad = plan.take(max_paths * sizeof(float4));
+  const size_t o_qa = plan.take(max_paths * sizeof(int32_t));
+  const size_t o_qb = plan.take(max_paths * sizeof(int32_t));
+  const size_t o_cnt = plan.take(64 * sizeof(uint32_t));
+  int rc = ctx_reserve_scratch(ctx, plan.off);
+  if (rc) return rc;
+  char* base = static_cast<char*>(ctx->scratch);
+  auto* ro = reinterpret_cast<float4*>(base + o_ro);
+  auto* rd = reinterpret_cast<float4*>(base + o_rd);
+  auto* thr = reinterpret_cast<float4*>(base + o_thr);
+  auto* rad = reinterpret_cast<float4*>(base + o_rad);
+  int32_t* q[2] = {reinterpret_cast<int32_t*>(base + o_qa), reinterpret_cast<int32_t*>(base + o_qb)};
+  auto* counts = reinterpret_cast<uint32_t*>(base + o_cnt);
+
+  Cam c;
+  for (int a = 0; a < 3; ++a) {
+    c.o[a] = float(cam->origin[a]);
+    c.ll[a] = float(cam->lower_left[a]);
+    c.h[a] = float(cam->horizontal[a]);
+    c.v[a] = float(cam->vertical[a]);
+  }
+  for (int64_t s0 = 0; s0 < prm->spp; s0 += spb) {
+    const int64_t nsamp = std::min<int64_t>(spb, prm->spp - s0);
+    const int64_t npaths = nsamp * wh;
+    const int64_t path_base = (prm->sample_offset + s0) * wh;
+    camera_kernel<<<int(ceil_div(npaths, 256)), 256, 0, st>>>(c, prm->width, prm->height, npaths,
+                                                            path_base, prm->seed, ro, rd, thr, rad,
+                                                            q[0], counts);
+    for (int32_t k = 0; k <= prm->max_depth + 1; ++k) {
+      const int in = k & 1, out = in ^ 1;
+      fill_async<uint32_t>(st, counts + out, 1, 0u);
+      bounce_kernel<<<int(ceil_div(npaths, kThreads)), kThreads, 0, st>>>(
+          bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, mesh->nrm.p, mesh->mat_idx.p,
+          mesh->mats.p, k, prm->max_depth, prm->miss, prm->seed, path_base, ro, rd, thr, rad,
+          q[in], q[out], counts + in, counts + out, ctx->d_err);
+    }
+    const bool last_batch = s0 + nsamp >= prm->spp;
+    accumulate_kernel<<<int(ceil_div(wh, 256)), 256, 0, st>>>(rad, wh, int32_t(nsamp), s0 == 0 ? 1 : 0,
+                                                             d_sum, d_sumsq,
+                                                             last_batch ? d_last : nullptr);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  return TRACE_OK;
+}
+
+}  // namespace render

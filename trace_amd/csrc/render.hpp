@@ -1,0 +1,13 @@
+// Wavefront path tracer entry points (see render.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace render {
+int set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
+                const int32_t* h_mat_index, const float* h_materials, int32_t nmat);
+// Enqueues a render of prm->spp samples per pixel; writes per-pixel sums,
+// sums of squares and the last sample (d_last may be null), 3 floats each.
+int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
+           const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last);
+}  // namespace render

@@ -70,6 +70,17 @@ class Mesh:
             ctx.check(ctx.lib.trace_mesh_create(ctx.h, _ptr(p), self.ntri, C.byref(h)))
         self.h = h
 
+    def set_shading(self, mat_idx, materials, normals=None) -> None:
+        """Per-triangle material index, material table (kind, r, g, b) and
+        optional vertex normals (n, 9); None = geometric normals."""
+        mi = np.ascontiguousarray(mat_idx, np.int32)
+        mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 4)
+        if mi.shape[0] != self.ntri:
+            raise ValueError("one material index per triangle")
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float64).reshape(-1, 9)
+        self.ctx.check(self.ctx.lib.trace_mesh_set_shading(self.ctx.h, self.h, None if nrm is None else _ptr(nrm),
+                                                           _ptr(mi), _ptr(mats), mats.shape[0]))
+
     def update(self, pos: np.ndarray) -> None:
         p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 9)
         if p.shape[0] != self.ntri:
@@ -174,3 +185,41 @@ class Bvh:
         self.ctx.check(self.ctx.lib.trace_closest_hit_counts(self.ctx.h, self.h, _ptr(r), n, _ptr(nodes),
                                                              _ptr(tris)))
         return nodes, tris
+
+
+def camera_struct(cam) -> "_lib.Camera":
+    c = _lib.Camera()
+    for name in ("origin", "lower_left", "horizontal", "vertical"):
+        getattr(c, name)[:] = [float(v) for v in getattr(cam, name)]
+    return c
+
+
+def render_params(width: int, height: int, spp: int, max_depth: int, miss: int, seed: int,
+                  sample_offset: int = 0) -> "_lib.RenderParams":
+    p = _lib.RenderParams()
+    p.width, p.height, p.spp, p.max_depth, p.miss = width, height, spp, max_depth, miss
+    p.seed, p.sample_offset = seed, sample_offset
+    return p
+
+
+def render(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: int, spp: int, max_depth: int,
+           miss: int, seed: int, sample_offset: int = 0):
+    """trace_render: returns (sum, sumsq, last), each (height, width, 3) float32."""
+    shape = (height, width, 3)
+    s = np.empty(shape, np.float32)
+    q = np.empty(shape, np.float32)
+    last = np.empty(shape, np.float32)
+    c = camera_struct(cam)
+    p = render_params(width, height, spp, max_depth, miss, seed, sample_offset)
+    ctx.check(ctx.lib.trace_render(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), _ptr(s), _ptr(q), _ptr(last)))
+    return s, q, last
+
+
+def render_device(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: int, spp: int, max_depth: int,
+                  miss: int, seed: int, d_sum: int, d_sumsq: int, d_last: int | None = None,
+                  sample_offset: int = 0) -> None:
+    """trace_render_device (asynchronous on the context stream)."""
+    c = camera_struct(cam)
+    p = render_params(width, height, spp, max_depth, miss, seed, sample_offset)
+    ctx.check(ctx.lib.trace_render_device(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), C.c_void_p(d_sum),
+                                          C.c_void_p(d_sumsq), C.c_void_p(d_last or 0)))

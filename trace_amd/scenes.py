@@ -91,6 +91,81 @@ def coherent_rays_cfg2(width: int = 4096, height: int = 2048) -> np.ndarray:
     return camera_rays(cam, width, height)
 
 
+MAT_DIFFUSE, MAT_LIGHT = 0, 1
+
+
+def _quad(p0, p1, p2, p3) -> np.ndarray:
+    return np.array([[*p0, *p1, *p2], [*p0, *p2, *p3]], np.float64)
+
+
+def uv_sphere(center, radius: float, slices: int, rings: int, jitter: float = 0.0,
+              rng: np.random.Generator | None = None) -> np.ndarray:
+    """UV sphere with `slices` segments around and `rings` - 1 latitude bands
+    (2*slices*(rings - 2) triangles); every vertex (poles included, shared
+    vertices once) is moved radially by U(-jitter, jitter).  (n, 9) float64."""
+    c = np.asarray(center, np.float64)
+    bands = rings - 1
+    theta = np.linspace(0.0, math.pi, bands + 1)
+    phi = np.linspace(0.0, 2 * math.pi, slices + 1)[:-1]
+    grid = np.empty((bands + 1, slices, 3))
+    grid[..., 0] = np.sin(theta)[:, None] * np.cos(phi)[None, :]
+    grid[..., 1] = np.cos(theta)[:, None] * np.ones_like(phi)[None, :]
+    grid[..., 2] = np.sin(theta)[:, None] * np.sin(phi)[None, :]
+    r = np.full((bands + 1, slices), radius)
+    if jitter and rng is not None:
+        r = r + rng.uniform(-jitter, jitter, r.shape)
+        r[0, :] = r[0, 0]  # one pole vertex
+        r[-1, :] = r[-1, 0]
+    verts = c + grid * r[..., None]
+    tris = []
+    for b in range(bands):
+        for s in range(slices):
+            s1 = (s + 1) % slices
+            a, bb, cc, d = verts[b, s], verts[b, s1], verts[b + 1, s1], verts[b + 1, s]
+            if b == 0:
+                tris.append([*a, *cc, *d])
+            elif b == bands - 1:
+                tris.append([*a, *bb, *cc])
+            else:
+                tris.append([*a, *bb, *cc])
+                tris.append([*a, *cc, *d])
+    return np.array(tris, np.float64)
+
+
+def box_scene(seed: int = 1, slices: int = 100, rings: int = 102, width: int = 640, height: int = 360):
+    """Five-wall box [-1,1]^3 open towards +z (10 triangles: white floor,
+    ceiling and back 0.75, red left (0.75, 0.25, 0.25), blue right
+    (0.25, 0.25, 0.75)), a 2-triangle area light under the ceiling with
+    emission 15, and a UV sphere of 2*slices*(rings-2) triangles with radial
+    jitter U(-0.01, 0.01) (seeded).  Vertices are FP32-exact.  Returns a dict
+    with pos, mat_idx, materials, camera, width, height."""
+    rng = np.random.default_rng(seed)
+    L, H_ = -1.0, 1.0
+    walls = [
+        (_quad((L, L, H_), (H_, L, H_), (H_, L, L), (L, L, L)), 0),   # floor
+        (_quad((L, H_, L), (H_, H_, L), (H_, H_, H_), (L, H_, H_)), 0),  # ceiling
+        (_quad((L, L, L), (H_, L, L), (H_, H_, L), (L, H_, L)), 0),   # back
+        (_quad((L, L, H_), (L, L, L), (L, H_, L), (L, H_, H_)), 1),   # left (red)
+        (_quad((H_, L, L), (H_, L, H_), (H_, H_, H_), (H_, H_, L)), 2),  # right (blue)
+    ]
+    y = 0.98
+    light = _quad((-0.3, y, -0.3), (0.3, y, -0.3), (0.3, y, 0.3), (-0.3, y, 0.3))
+    sphere = uv_sphere((0.0, -0.45, 0.0), 0.55, slices, rings, jitter=0.01, rng=rng)
+    pos = np.concatenate([w for w, _ in walls] + [light, sphere])
+    mat_idx = np.concatenate([np.full(2, m, np.int32) for _, m in walls]
+                             + [np.full(2, 3, np.int32), np.full(len(sphere), 0, np.int32)])
+    materials = np.array([
+        [MAT_DIFFUSE, 0.75, 0.75, 0.75],
+        [MAT_DIFFUSE, 0.75, 0.25, 0.25],
+        [MAT_DIFFUSE, 0.25, 0.25, 0.75],
+        [MAT_LIGHT, 15.0, 15.0, 15.0],
+    ], np.float32)
+    cam = Camera(width / height, 45.0, (0.0, 0.0, 3.4), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    pos = pos.astype(np.float32).astype(np.float64)
+    return {"pos": pos, "mat_idx": mat_idx, "materials": materials, "camera": cam, "width": width,
+            "height": height}
+
+
 def parse_obj(path: str) -> tuple[np.ndarray, np.ndarray | None]:
     """OBJ triangles as obj.go:19-106 reads them: 'v', 'vn', 'f' (fan
     triangulation, negative indices relative to the current count, normals only
