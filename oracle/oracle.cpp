@@ -441,6 +441,388 @@ void oracle_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t 
 
 }  // extern "C"
 
+// ---- PHR (phr.go) -----------------------------------------------------------
+// Progressive Hierarchical Refinement over the LBVH.  Deterministic: the
+// initial cut is taken in FIFO order from the root (what the reference's
+// channel queue does with one worker while it holds < 1024 nodes,
+// phr.go:257-311), and subtrees are built depth first.  Every other step is
+// order independent in the reference.
+namespace {
+
+struct PNode {
+  int kind = 0;               // 0 inner (PHR branch), 1 PHR-made leaf, 2 reused LBVH node
+  int32_t aux = -1;           // kind 2: LBVH node index
+  std::vector<int32_t> kids;  // kind 0
+  std::vector<int32_t> prims; // kind 1: sorted positions
+  Box box;
+};
+
+struct Ref {  // a member of a cut: an LBVH node or a PHR-made leaf
+  int32_t id;
+  bool made;
+};
+
+struct Cut {
+  std::vector<Ref> nodes;
+  Box bounding;
+  int depth = 0;
+};
+
+struct Phr {
+  const Tree* aux;
+  double alpha, delta, S;
+  int bf;
+  std::vector<PNode> out;
+  std::vector<PNode> made;      // PHR-made leaves referenced from cuts
+  std::vector<int64_t> size;    // LBVH subtree sizes (bvh.go:159-167)
+  int64_t split_work = 0;       // BuildWithCost's counter (phr.go:221)
+  int64_t max_split = 0;        // largest cut handed to SweepSAH
+
+  const Box& box_of(Ref r) const { return r.made ? made[r.id].box : aux->nodes[r.id].box; }
+  bool is_leaf(Ref r) const { return r.made || aux->nodes[r.id].leaf; }
+  int64_t subtree(Ref r) const { return r.made ? 1 : size[r.id]; }
+
+  // phr.go:356-358
+  double threshold(int depth) const { return S / std::pow(2.0, alpha * double(depth) + delta); }
+
+  int64_t compute_size(int32_t id) {
+    const Node& nd = aux->nodes[id];
+    int64_t s = 1;
+    if (!nd.leaf) s += compute_size(nd.child[0]) + compute_size(nd.child[1]);
+    size[id] = s;
+    return s;
+  }
+
+  // phr.go:340-352: leaves of each node in order, their prims concatenated
+  Ref make_leaf(const Box& bounding, const std::vector<Ref>& nodes) {
+    PNode leaf;
+    leaf.kind = 1;
+    leaf.box = bounding;
+    for (const Ref& r : nodes) {
+      if (r.made) {
+        const auto& p = made[r.id].prims;
+        leaf.prims.insert(leaf.prims.end(), p.begin(), p.end());
+        continue;
+      }
+      std::vector<int32_t> st{r.id};
+      while (!st.empty()) {
+        const int32_t id = st.back();
+        st.pop_back();
+        const Node& nd = aux->nodes[id];
+        if (nd.leaf) {
+          for (int32_t i = nd.first; i <= nd.last; ++i) leaf.prims.push_back(i);
+        } else {
+          st.push_back(nd.child[1]);
+          st.push_back(nd.child[0]);
+        }
+      }
+    }
+    made.push_back(leaf);
+    return Ref{int32_t(made.size() - 1), true};
+  }
+
+  // phr.go:313-338
+  Cut refined(const Cut& cut) {
+    Cut r;
+    r.bounding = cut.bounding;
+    r.depth = cut.depth + 1;
+    const double th = threshold(cut.depth);
+    for (const Ref& n : cut.nodes) {
+      if (is_leaf(n) || surface(box_of(n)) < th) {
+        r.nodes.push_back(n);
+      } else {
+        const Node& nd = aux->nodes[n.id];
+        r.nodes.push_back(Ref{nd.child[0], false});
+        r.nodes.push_back(Ref{nd.child[1], false});
+      }
+    }
+    if (r.nodes.size() == 1) r.nodes[0] = make_leaf(cut.bounding, r.nodes);
+    return r;
+  }
+
+  struct Sah {
+    double cost;
+    int64_t split;  // 0 = no split (left = all)
+    Box lbox, rbox;
+  };
+
+  // phr.go:414-459
+  Sah min_cost(const std::vector<Ref>& s, const Box& bounding) const {
+    const int64_t m = int64_t(s.size());
+    std::vector<double> right_cost(m, 0.0);
+    std::vector<Box> right_box(m);
+    Box sr = box_of(s[m - 1]);
+    int64_t count = 0;
+    for (int64_t i = m - 1; i > 0; --i) {
+      sr = box_union(sr, box_of(s[i]));
+      count += subtree(s[i]);
+      right_cost[i] = surface(sr) * double(count);
+      right_box[i] = sr;
+    }
+    count += subtree(s[0]);
+    Sah best{surface(bounding) * double(count), 0, bounding, Box{}};
+    count = subtree(s[0]);
+    Box sl = box_of(s[0]);
+    for (int64_t i = 1; i < m; ++i) {
+      const double c = right_cost[i] + surface(sl) * double(count);
+      if (c < best.cost) best = Sah{c, i, sl, right_box[i]};
+      sl = box_union(sl, box_of(s[i]));
+      count += subtree(s[i]);
+    }
+    return best;
+  }
+
+  std::vector<Ref> sorted_by(std::vector<Ref> v, int axis) const {
+    std::stable_sort(v.begin(), v.end(), [&](const Ref& a, const Ref& b) {
+      const Box& ba = box_of(a);
+      const Box& bb = box_of(b);
+      return (ba.lo[axis] + ba.hi[axis]) * (1.0 / 2.0) < (bb.lo[axis] + bb.hi[axis]) * (1.0 / 2.0);
+    });
+    return v;
+  }
+
+  // phr.go:368-405: returns false when no split beats keeping the cut whole
+  bool sweep_sah(const Cut& cut, Cut& left, Cut& right) {
+    const std::vector<Ref> xs = sorted_by(cut.nodes, 0);
+    const std::vector<Ref> ys = sorted_by(xs, 1);
+    const Sah x = min_cost(xs, cut.bounding);
+    const Sah y = min_cost(ys, cut.bounding);
+    const std::vector<Ref>* chosen;
+    Sah best;
+    if (x.cost < y.cost) {
+      const std::vector<Ref> zs = sorted_by(ys, 2);
+      const Sah z = min_cost(zs, cut.bounding);
+      if (x.cost < z.cost) { best = x; chosen = &xs; finish(cut, *chosen, best, left, right); }
+      else { best = z; finish(cut, zs, best, left, right); }
+    } else {
+      const std::vector<Ref> zs = sorted_by(xs, 2);
+      const Sah z = min_cost(zs, cut.bounding);
+      if (y.cost < z.cost) { best = y; chosen = &ys; finish(cut, *chosen, best, left, right); }
+      else { best = z; finish(cut, zs, best, left, right); }
+    }
+    return best.split != 0;
+  }
+
+  void finish(const Cut& cut, const std::vector<Ref>& s, const Sah& b, Cut& left, Cut& right) {
+    left.depth = right.depth = cut.depth;
+    if (b.split == 0) {
+      left.nodes = s;
+      left.bounding = cut.bounding;
+      return;
+    }
+    left.nodes.assign(s.begin(), s.begin() + b.split);
+    left.bounding = b.lbox;
+    right.nodes.assign(s.begin() + b.split, s.end());
+    right.bounding = b.rbox;
+  }
+
+  int32_t attach(Ref r) {
+    PNode p;
+    if (r.made) {
+      p = made[r.id];
+    } else {
+      p.kind = 2;
+      p.aux = r.id;
+      p.box = aux->nodes[r.id].box;
+    }
+    out.push_back(p);
+    return int32_t(out.size() - 1);
+  }
+
+  // phr.go:92-152 (and the split counter of buildSubTreeCost, phr.go:221)
+  int32_t build(const Cut& cut) {
+    if (cut.nodes.size() <= 1) return attach(cut.nodes[0]);
+    std::vector<Cut> cuts{cut};
+    while (int(cuts.size()) < bf) {
+      size_t mx = 0, mi = 0;
+      for (size_t i = 0; i < cuts.size(); ++i)
+        if (cuts[i].nodes.size() > mx) { mx = cuts[i].nodes.size(); mi = i; }
+      if (mx <= 1) break;
+      split_work += int64_t(cuts[mi].nodes.size());
+      max_split = std::max<int64_t>(max_split, int64_t(cuts[mi].nodes.size()));
+      Cut l, r;
+      if (sweep_sah(cuts[mi], l, r)) {
+        Cut rl = refined(l), rr = refined(r);
+        cuts[mi] = rl;
+        cuts.push_back(rr);
+      } else {
+        Cut c;
+        c.nodes.push_back(make_leaf(l.bounding, l.nodes));
+        cuts[mi] = c;
+      }
+    }
+    PNode br;
+    br.kind = 0;
+    br.box = cut.bounding;
+    out.push_back(br);
+    const int32_t id = int32_t(out.size() - 1);
+    for (const Cut& c : cuts) {
+      const int32_t k = build(c);
+      out[id].kids.push_back(k);
+    }
+    return id;
+  }
+
+  // phr.go:257-311 in FIFO order
+  Cut initial_cut(int64_t max_cut) {
+    Cut cut;
+    cut.bounding = aux->nodes[aux->root].box;
+    cut.depth = 1;
+    const double th = threshold(0);
+    int64_t counter = 0;
+    std::vector<int32_t> fifo{aux->root};
+    for (size_t h = 0; h < fifo.size(); ++h) {
+      const int32_t id = fifo[h];
+      const Node& nd = aux->nodes[id];
+      if (nd.leaf) {
+        cut.nodes.push_back(Ref{id, false});
+      } else if (surface(nd.box) > th) {
+        counter += 1;  // len(children) - 1
+        if (counter >= max_cut) {
+          cut.nodes.push_back(Ref{id, false});
+        } else {
+          fifo.push_back(nd.child[0]);
+          fifo.push_back(nd.child[1]);
+        }
+      } else {
+        cut.nodes.push_back(Ref{id, false});
+      }
+    }
+    return cut;
+  }
+};
+
+struct PhrTree {
+  const Tree* aux;
+  std::vector<PNode> nodes;
+  int32_t root;
+  int64_t split_work;
+  int64_t initial_cut;
+  int64_t max_split;
+};
+
+double phr_cost(const PhrTree& t, const Tree& aux, int32_t id);
+
+double aux_cost(const Tree& aux, int32_t id) { return cost_of(aux, id); }
+
+double phr_cost(const PhrTree& t, const Tree& aux, int32_t id) {
+  const PNode& p = t.nodes[id];
+  if (p.kind == 2) return aux_cost(aux, p.aux);
+  if (p.kind == 1) return 1.0 * double(p.prims.size());
+  double acc = 0.0;
+  const double sa = surface(p.box);
+  for (int32_t k : p.kids) {
+    const PNode& c = t.nodes[k];
+    const Box& cb = c.kind == 2 ? aux.nodes[c.aux].box : c.box;
+    if (sa != 0) acc += surface(cb) / sa * phr_cost(t, aux, k);
+  }
+  return 2.0 + acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct PhrTree oracle_phr;
+
+// PhrBuilder.BuildFromAuxilary (phr.go:47-84) on the oracle's LBVH.
+oracle_phr* oracle_phr_build(const oracle_tree* aux, double alpha, double delta, int branching,
+                             int64_t max_cut) {
+  Phr b;
+  b.aux = aux;
+  b.alpha = alpha;
+  b.delta = delta;
+  b.bf = branching;
+  b.S = surface(aux->nodes[aux->root].box);
+  b.size.assign(aux->nodes.size(), 0);
+  b.compute_size(aux->root);
+  Cut cut = b.initial_cut(max_cut);
+  const int64_t n_cut = int64_t(cut.nodes.size());
+  b.split_work = n_cut;  // BuildWithCost counts the initial cut (phr.go:160)
+  // The temporary parent of phr.go:64-65 takes job.cut.bounding = root box.
+  const int32_t root = b.build(cut);
+  PhrTree* t = new PhrTree{aux, std::move(b.out), root, b.split_work, n_cut, b.max_split};
+  return t;
+}
+
+void oracle_phr_free(oracle_phr* t) { delete t; }
+
+double oracle_phr_cost(const oracle_phr* t) { return phr_cost(*t, *t->aux, t->root); }
+
+int64_t oracle_phr_split_work(const oracle_phr* t) { return t->split_work; }
+int64_t oracle_phr_initial_cut(const oracle_phr* t) { return t->initial_cut; }
+int64_t oracle_phr_max_split(const oracle_phr* t) { return t->max_split; }
+
+// Canonical preorder of the PHR tree with reused LBVH subtrees expanded:
+// rec = kind (0 inner, 1 leaf), prim count (leaf) or -1, n_children, 0;
+// box FP64; leaf prims (sorted positions) concatenated in preorder.
+// Returns the node count; call with null outputs to size the buffers
+// (*n_prims receives the prim-list length).
+int64_t oracle_phr_canonical(const oracle_phr* t, int32_t* rec, double* box, int32_t* prims,
+                             int64_t* n_prims) {
+  struct It {
+    int32_t id;
+    bool aux;
+  };
+  std::vector<It> st{{t->root, false}};
+  int64_t k = 0, np = 0;
+  const Tree& aux = *t->aux;
+  while (!st.empty()) {
+    It it = st.back();
+    st.pop_back();
+    if (!it.aux && t->nodes[it.id].kind == 2) it = It{t->nodes[it.id].aux, true};
+    const Box* b;
+    int kind, cnt = -1, nch = 0;
+    if (it.aux) {
+      const Node& nd = aux.nodes[it.id];
+      b = &nd.box;
+      if (nd.leaf) {
+        kind = 1;
+        cnt = nd.last - nd.first + 1;
+        for (int32_t i = nd.first; i <= nd.last; ++i) {
+          if (prims) prims[np] = i;
+          ++np;
+        }
+      } else {
+        kind = 0;
+        nch = 2;
+        st.push_back({nd.child[1], true});
+        st.push_back({nd.child[0], true});
+      }
+    } else {
+      const PNode& p = t->nodes[it.id];
+      b = &p.box;
+      if (p.kind == 1) {
+        kind = 1;
+        cnt = int(p.prims.size());
+        for (int32_t v : p.prims) {
+          if (prims) prims[np] = v;
+          ++np;
+        }
+      } else {
+        kind = 0;
+        nch = int(p.kids.size());
+        for (size_t c = p.kids.size(); c-- > 0;) st.push_back({p.kids[c], false});
+      }
+    }
+    if (rec) {
+      rec[4 * k + 0] = kind;
+      rec[4 * k + 1] = cnt;
+      rec[4 * k + 2] = nch;
+      rec[4 * k + 3] = 0;
+      for (int a = 0; a < 3; ++a) {
+        box[6 * k + a] = b->lo[a];
+        box[6 * k + 3 + a] = b->hi[a];
+      }
+    }
+    ++k;
+  }
+  if (n_prims) *n_prims = np;
+  return k;
+}
+
+}  // extern "C"
+
 // ---- path tracing (render.go:91-129, 201-246; material.go:13-42) -----------
 namespace {
 

@@ -45,6 +45,12 @@ def load() -> C.CDLL:
         "oracle_closest_hit": ([_P, _P, C.c_int64, _P, _P, C.c_int], None),
         "oracle_closest_hit_brute": ([_P, C.c_int64, _P, C.c_int64, _P, _P, C.c_int], None),
         "oracle_philox": ([C.c_uint32] * 6 + [_P], None),
+        "oracle_phr_build": ([_P, C.c_double, C.c_double, C.c_int, C.c_int64], _P),
+        "oracle_phr_free": ([_P], None),
+        "oracle_phr_cost": ([_P], C.c_double),
+        "oracle_phr_split_work": ([_P], C.c_int64),
+        "oracle_phr_initial_cut": ([_P], C.c_int64),
+        "oracle_phr_canonical": ([_P, _P, _P, _P, _P], C.c_int64),
         "oracle_render": ([_P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                            C.c_uint64, C.c_int64, C.c_int, _P, _P, _P, _P], None),
     }
@@ -135,6 +141,40 @@ class Tree:
         t = np.empty(n, np.float64)
         self.lib.oracle_closest_hit(self.h, _p(r), n, _p(hit), _p(t), threads or _threads())
         return hit, t
+
+
+class PhrTree:
+    """PhrBuilder.BuildFromAuxilary on an oracle LBVH (deterministic FIFO initial cut)."""
+
+    def __init__(self, aux: Tree, alpha: float = 0.5, delta: float = 6.0, branching: int = 2, max_cut: int = 2500):
+        self.lib = load()
+        self.aux = aux  # keep the LBVH alive
+        self.h = self.lib.oracle_phr_build(aux.h, alpha, delta, branching, max_cut)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_phr_free(self.h)
+            self.h = None
+
+    def cost(self) -> float:
+        return float(self.lib.oracle_phr_cost(self.h))
+
+    def split_work(self) -> int:
+        return int(self.lib.oracle_phr_split_work(self.h))
+
+    def initial_cut(self) -> int:
+        return int(self.lib.oracle_phr_initial_cut(self.h))
+
+    def canonical(self):
+        """(rec (n,4): kind, prim count or -1, n_children, 0; box (n,6) FP64;
+        prims: sorted positions of the leaves in preorder)."""
+        npr = C.c_int64()
+        n = int(self.lib.oracle_phr_canonical(self.h, None, None, None, C.byref(npr)))
+        rec = np.empty((n, 4), np.int32)
+        box = np.empty((n, 6), np.float64)
+        prims = np.empty(npr.value, np.int32)
+        self.lib.oracle_phr_canonical(self.h, _p(rec), _p(box), _p(prims), C.byref(npr))
+        return rec, box, prims
 
 
 def philox(counter, key) -> list[int]:

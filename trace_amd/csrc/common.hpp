@@ -139,7 +139,14 @@ struct trace_bvh {
   DevArray<BvhNode> nodes;     // binary traversal tree
   DevArray<int32_t> parent;    // inner parents then leaf parents ((p << 1) | slot, -1 = root)
   DevArray<uint32_t> run_start;// leaf i covers [run_start[i], run_start[i+1])
+  DevArray<int32_t> inner_range; // LBVH: leaf range [lo, hi] of each inner node
   DevArray<BvhHeader> header;
+  // PHR trees: output nodes (n-ary structure for Cost/export) and the LBVH
+  // sorted position of each PHR-ordered triangle
+  DevArray<uint8_t> phr_out;
+  int64_t phr_out_count = 0;
+  DevArray<int32_t> phr_src;
+  bool phr_clone = false;  // the PHR tree is the LBVH itself (initial cut of one node)
   // host mirror of the n-ary structure for PHR trees (empty for LBVH)
   std::vector<int32_t> nary_first;   // per node: leaf -> first prim pos, inner -> -1
   std::vector<int32_t> nary_count;   // per node: leaf -> prim count, inner -> #children

@@ -45,13 +45,14 @@ __device__ inline double max3(double a, double b, double c) {
   return c;
 }
 
-__global__ void init_enclosing_kernel(unsigned long long* enc) {
-  if (threadIdx.x < 3) enc[threadIdx.x] = ~0ull;  // min slots
-  else if (threadIdx.x < 6) enc[threadIdx.x] = 0ull;  // max slots
-}
+constexpr int kEncBlocks = 512;
 
+// Stage 1 of the scene box: each workgroup reduces a grid-stride share of the
+// triangles (min/max are exact, so any order gives the reference's box) and
+// writes one partial box as order-preserving u64 words; no atomics.
 __global__ __launch_bounds__(256) void enclosing_kernel(const double* __restrict__ pos, int64_t n,
-                                                        unsigned long long* enc) {
+                                                        unsigned long long* __restrict__ partial) {
+  __shared__ uint64_t s_part[6][4];
   uint64_t lo[3] = {~0ull, ~0ull, ~0ull};
   uint64_t hi[3] = {0ull, 0ull, 0ull};
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
@@ -74,24 +75,60 @@ __global__ __launch_bounds__(256) void enclosing_kernel(const double* __restrict
       hi[a] = h > hi[a] ? h : hi[a];
     }
   }
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&enc[a], (unsigned long long)lo[a]);
-      atomicMax(&enc[3 + a], (unsigned long long)hi[a]);
+      s_part[a][w] = lo[a];
+      s_part[3 + a][w] = hi[a];
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    uint64_t v = s_part[a][0];
+    for (int k = 1; k < 4; ++k) v = a < 3 ? (s_part[a][k] < v ? s_part[a][k] : v)
+                                          : (s_part[a][k] > v ? s_part[a][k] : v);
+    partial[6 * blockIdx.x + a] = v;
   }
 }
 
-// Writes the scene box into the header and, unless the caller supplied one,
-// the quantisation box used by the Morton kernel.
-__global__ void header_kernel(const unsigned long long* enc, BvhHeader* hdr, double* qbox,
-                              int caller_box, int32_t n) {
+// Stage 2: reduces the partial boxes, writes the scene box into the header
+// and, unless the caller supplied one, the quantisation box used by the
+// Morton kernel.
+__global__ void header_kernel(const unsigned long long* __restrict__ partial, int nparts,
+                              BvhHeader* hdr, double* qbox, int caller_box, int32_t n) {
+  __shared__ uint64_t s_box[6][4];
+  uint64_t v[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) v[a] = a < 3 ? ~0ull : 0ull;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      const uint64_t p = partial[6 * k + a];
+      v[a] = a < 3 ? (p < v[a] ? p : v[a]) : (p > v[a] ? p : v[a]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t p = __shfl_xor(v[a], o, 64);
+      v[a] = a < 3 ? (p < v[a] ? p : v[a]) : (p > v[a] ? p : v[a]);
+    }
+    if ((threadIdx.x & 63) == 0) s_box[a][threadIdx.x >> 6] = v[a];
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  for (int a = 0; a < 6; ++a) {
+    uint64_t r = s_box[a][0];
+    for (int k = 1; k < int(blockDim.x >> 6); ++k)
+      r = a < 3 ? (s_box[a][k] < r ? s_box[a][k] : r) : (s_box[a][k] > r ? s_box[a][k] : r);
+    s_box[a][0] = r;
+  }
   hdr->n_prims = n;
   for (int a = 0; a < 3; ++a) {
-    const double lo = double_of(enc[a]);
-    const double hi = double_of(enc[3 + a]);
+    const double lo = double_of(s_box[a][0]);
+    const double hi = double_of(s_box[3 + a][0]);
     hdr->enc_lo[a] = lo;
     hdr->enc_hi[a] = hi;
     hdr->root_lo[a] = f32_down(lo);
@@ -243,7 +280,8 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
                                                      const BvhHeader* __restrict__ hdr,
                                                      BvhNode* __restrict__ nodes,
                                                      int32_t* __restrict__ parent_inner,
-                                                     int32_t* __restrict__ parent_leaf) {
+                                                     int32_t* __restrict__ parent_leaf,
+                                                     int32_t* __restrict__ inner_range) {
   const int32_t u = hdr->n_leaves;
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= u - 1) return;
@@ -265,6 +303,8 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
   const int32_t gamma = i + s * d + (d < 0 ? -1 : 0);
   const int32_t first = i < j ? i : j;
   const int32_t last = i < j ? j : i;
+  inner_range[2 * i] = first;
+  inner_range[2 * i + 1] = last;
 
   BvhNode& nd = nodes[i];
   // child 0 covers [first, gamma], child 1 covers [gamma + 1, last]
@@ -419,13 +459,14 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   HIP_TRY(ctx, bvh->nodes.alloc(std::max<int64_t>(1, n - 1)));
   HIP_TRY(ctx, bvh->parent.alloc(2 * n));
   HIP_TRY(ctx, bvh->run_start.alloc(n + 1));
+  HIP_TRY(ctx, bvh->inner_range.alloc(2 * std::max<int64_t>(1, n - 1)));
   HIP_TRY(ctx, bvh->header.alloc(1));
 
   // scratch: enclosing words, qbox, sort ping-pong buffers, sort status,
   // unique keys, per-tile head counts, refit counters and heights
   const int64_t utiles = ceil_div(n, kUTile);
   ScratchPlan plan;
-  const size_t o_enc = plan.take(6 * sizeof(unsigned long long));
+  const size_t o_enc = plan.take(6 * kEncBlocks * sizeof(unsigned long long));
   const size_t o_qbox = plan.take(6 * sizeof(double));
   const size_t o_kb = plan.take(n * sizeof(uint64_t));
   const size_t o_va = plan.take(n * sizeof(uint32_t));
@@ -451,10 +492,10 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
 
   if (h_enclosing) HIP_TRY(ctx, hipMemcpyAsync(qbox, h_enclosing, 6 * sizeof(double),
                                                hipMemcpyHostToDevice, st));
-  init_enclosing_kernel<<<1, 64, 0, st>>>(enc);
-  enclosing_kernel<<<int(std::min<int64_t>(grid_for(n, 256), 2048)), 256, 0, st>>>(mesh->pos.p, n,
-                                                                                   enc);
-  header_kernel<<<1, 64, 0, st>>>(enc, bvh->header.p, qbox, h_enclosing ? 1 : 0, int32_t(n));
+  const int enc_blocks = int(std::min<int64_t>(grid_for(n, 256), kEncBlocks));
+  enclosing_kernel<<<enc_blocks, 256, 0, st>>>(mesh->pos.p, n, enc);
+  header_kernel<<<1, 256, 0, st>>>(enc, enc_blocks, bvh->header.p, qbox, h_enclosing ? 1 : 0,
+                                  int32_t(n));
   morton_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, qbox, key_mode, bvh->keys.p);
 
   const int where = onesweep::sort_pairs(st, bvh->keys.p, vals_a, keys_b, vals_b, n, passes,
@@ -476,7 +517,8 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   int32_t* parent_inner = bvh->parent.p;
   int32_t* parent_leaf = bvh->parent.p + n;
   karras_kernel<<<grid_for(n, 256), 256, 0, st>>>(ukeys, bvh->run_start.p, bvh->header.p,
-                                                  bvh->nodes.p, parent_inner, parent_leaf);
+                                                  bvh->nodes.p, parent_inner, parent_leaf,
+                                                  bvh->inner_range.p);
   reorder_kernel<<<grid_for(n, 256), 256, 0, st>>>(
       mesh->pos.p, n, reinterpret_cast<const uint32_t*>(bvh->perm.p), bvh->tris.p,
       bvh->prim_box.p);

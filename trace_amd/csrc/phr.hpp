@@ -1,0 +1,32 @@
+// Progressive Hierarchical Refinement on the GPU (see phr.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace phr {
+
+// One node of the PHR tree as the reference builds it (n-ary, here at most 2
+// children): a branch whose box is its cut's bounding, or a leaf made from a
+// list of LBVH nodes (phr.go:340-352).  Triangles of the subtree occupy
+// [prim_off, prim_off + prim_cnt) of the PHR triangle order.
+struct OutNode {
+  int32_t kind;      // 0 branch, 1 leaf
+  int32_t nchild;    // branch: 1 or 2
+  int32_t child[2];  // branch: output node indices
+  int32_t list_off;  // leaf: LBVH node ids in list[list_off, list_off + list_cnt)
+  int32_t list_cnt;
+  int32_t prim_off;
+  int32_t prim_cnt;
+  int32_t level;     // depth below the root
+  int32_t pad[3];
+  float box[6];
+  int32_t pad2[2];
+};
+static_assert(sizeof(OutNode) == 64, "OutNode is 64 bytes");
+
+// BuildFromAuxilary(aux) with PhrBuilder{alpha, delta, branching} and the
+// initial-cut cap max_cut (phr.go:11, 47-84).  Synchronous.
+int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int branching,
+          int32_t max_cut, trace_bvh* out);
+
+}  // namespace phr

@@ -84,6 +84,22 @@ __global__ __launch_bounds__(256) void camera_kernel(Cam cam, int32_t W, int32_t
   queue[p] = int32_t(p);
 }
 
+// Moves a surface point a few float ULPs along n (Waechter & Binder, "A Fast
+// and Robust Method for Avoiding Self-Intersection", Ray Tracing Gems, 2019):
+// an integer offset of the float bits away from zero, and a small absolute
+// offset near the origin where ULPs vanish.  The reference works in FP64,
+// where a new ray starting on a triangle never re-enters it; in FP32 the
+// rounded hit point can sit on the wrong side of the plane, and a grazing
+// scattered ray would then hit the surface it left (on a closed mesh, from
+// inside).  Offsetting to the side the new ray leaves toward restores the
+// FP64 behaviour.
+__device__ inline float offset_coord(float p, float n) {
+  constexpr float kOrigin = 1.0f / 32.0f, kFloatScale = 1.0f / 65536.0f, kIntScale = 32.0f;
+  const int oi = int(kIntScale * n);
+  const float pi = __int_as_float(__float_as_int(p) + (p < 0.0f ? -oi : oi));
+  return fabsf(p) < kOrigin ? p + kFloatScale * n : pi;
+}
+
 // render.go:228-246
 __device__ inline float3 miss_color(int miss, float dx, float dy, float dz) {
   switch (miss) {
@@ -168,8 +184,21 @@ __global__ __launch_bounds__(kThreads) void bounce_kernel(
           sy = ny;
           sz = nz;
         }
-        // new origin = ray.position(t) (camera.go:108-111), no offset
-        ro[p] = make_float4(o4.x + d4.x * h.t, o4.y + d4.y * h.t, o4.z + d4.z * h.t, 0.0f);
+        // new origin = ray.position(t) (camera.go:108-111), evaluated in FP64,
+        // then kept on the side of the triangle's plane the new ray leaves to
+        const float* tv = tris + 9 * size_t(h.pos);
+        const float e1x = tv[3] - tv[0], e1y = tv[4] - tv[1], e1z = tv[5] - tv[2];
+        const float e2x = tv[6] - tv[0], e2y = tv[7] - tv[1], e2z = tv[8] - tv[2];
+        float gx = e1y * e2z - e1z * e2y, gy = e1z * e2x - e1x * e2z, gz = e1x * e2y - e1y * e2x;
+        const float gl = sqrtf(gx * gx + gy * gy + gz * gz);
+        const float gs = (gx * sx + gy * sy + gz * sz) >= 0.0f ? 1.0f / gl : -1.0f / gl;
+        gx *= gs;
+        gy *= gs;
+        gz *= gs;
+        const float px = float(double(o4.x) + double(d4.x) * h.t64);
+        const float py = float(double(o4.y) + double(d4.y) * h.t64);
+        const float pz = float(double(o4.z) + double(d4.z) * h.t64);
+        ro[p] = make_float4(offset_coord(px, gx), offset_coord(py, gy), offset_coord(pz, gz), 0.0f);
         rd[p] = make_float4(sx, sy, sz, 0.0f);
         thr[p] = make_float4(T.x * m.y, T.y * m.z, T.z * m.w, 0.0f);
         push = true;

@@ -28,6 +28,7 @@ struct Hit {
   int32_t pos;  // sorted triangle position, -1 = miss
   float u, v;   // barycentrics of the accepted hit
   bool front;   // det > 0
+  double t64;   // FP64 distance after refine_hit_fp64 (else t)
 };
 
 }  // namespace traverse

@@ -81,6 +81,7 @@ __device__ inline void tri_test(const RayPre& r, const float* __restrict__ tris,
   if (tt < r.tmin || tt > h.t) return;
   if (tt == h.t && k < h.pos) return;
   h.t = tt;
+  h.t64 = tt;
   h.pos = k;
   h.u = u;
   h.v = v;
@@ -111,7 +112,8 @@ __device__ inline void refine_hit_fp64(const float* __restrict__ tris, const Ray
   const double qz = tx * e1y - ty * e1x;
   h.u = float((tx * px + ty * py + tz * pz) * inv);
   h.v = float((dx * qx + dy * qy + dz * qz) * inv);
-  h.t = float((e2x * qx + e2y * qy + e2z * qz) * inv);
+  h.t64 = (e2x * qx + e2y * qy + e2z * qz) * inv;
+  h.t = float(h.t64);
 }
 
 // Leaf [first, first + cnt) must lie inside the triangle array; a malformed
@@ -140,6 +142,7 @@ __device__ inline Hit closest_hit_one(const BvhNode* __restrict__ nodes,
   const RayPre r = ray_pre(ray);
   Hit h;
   h.t = ray.tmax;
+  h.t64 = ray.tmax;
   h.pos = -1;
   h.u = h.v = 0.0f;
   h.front = true;
