@@ -99,6 +99,9 @@ typedef struct {
   int32_t arity;     /* maximum children per inner node */
   int32_t pad;
   double root_box[6];
+  int64_t phr_initial_cut; /* PHR: size of the initial cut (0 for an LBVH) */
+  int32_t phr_levels;      /* PHR: job levels processed */
+  int32_t pad2;
 } trace_bvh_info;
 int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out);
 
@@ -116,6 +119,19 @@ int trace_bvh_export_canonical(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap
 /* BVH.Cost() (bvh.go:46-48, 226-239) and BVH.Size() (bvh.go:50-52, 159-167). */
 int trace_bvh_cost(trace_ctx* ctx, const trace_bvh* bvh, double* cost);
 int trace_bvh_size(trace_ctx* ctx, const trace_bvh* bvh, int64_t* size);
+
+/* ---- PHR ------------------------------------------------------------------ */
+/* PhrBuilder{Alpha, Delta, BranchingFactor}.BuildFromAuxilary(aux) (phr.go:30-84, 92-459)
+ * with MAX_CUT_SIZE = max_cut (phr.go:11, 2500 in the reference); aux is an LBVH built
+ * by trace_lbvh_build.  NewDefaultBuilder is alpha 0.5, delta 6, branching 2
+ * (phr.go:26-28).  The initial cut is taken in FIFO order from the root.  This build
+ * supports branching 2 (TRACE_E_UNSUPPORTED otherwise).  The result is a new BVH handle
+ * (its triangles in PHR leaf order); Cost/Size/export/closest-hit/render accept it. */
+int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,
+                    int branching, int32_t max_cut, trace_bvh** out);
+/* For a PHR tree: per PHR-ordered triangle, its sorted position in the LBVH it was
+ * built from (relates PHR leaves to the LBVH order). */
+int trace_phr_export_source(trace_ctx* ctx, const trace_bvh* phr, int32_t* h_src);
 
 /* ---- closest hit -------------------------------------------------------- */
 /* Bulk BVH.intersected (bvh.go:74-77, 179-199 + primitive.go:172-208): per ray the

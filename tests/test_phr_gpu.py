@@ -1,0 +1,108 @@
+"""GPU parity of PHR refinement against the FP64 oracle.
+
+Both sides take the initial cut in FIFO order and follow phr.go step by step,
+so the tree is compared exactly: node kinds and arities in preorder, leaf
+triangle lists (as LBVH sorted positions), boxes, and Cost()."""
+import numpy as np
+import pytest
+
+from trace_amd import _lib, scenes
+from trace_amd.device import Bvh, Mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(ctx, oracle, pos, alpha=0.5, delta=6.0, max_cut=2500):
+    mesh = Mesh(ctx, pos)
+    aux = Bvh.lbvh(ctx, mesh)
+    phr = Bvh.phr(ctx, aux, alpha, delta, 2, max_cut)
+    otree = oracle.Tree(pos)
+    ophr = oracle.PhrTree(otree, alpha, delta, 2, max_cut)
+    rec, box = phr.canonical()
+    orec, obox, oprims = ophr.canonical()
+    assert rec.shape[0] == orec.shape[0], (rec.shape, orec.shape)
+    assert np.array_equal(rec[:, 0], orec[:, 0]), "node kinds differ"
+    assert np.array_equal(rec[:, 3], orec[:, 2]), "arities differ"
+    leaf = rec[:, 0] == 1
+    assert np.array_equal(rec[leaf, 2] - rec[leaf, 1] + 1, orec[leaf, 1]), "leaf sizes differ"
+    assert np.array_equal(box.astype(np.float64), obox), "boxes differ"
+    # leaf triangles, in leaf order, as LBVH sorted positions
+    src = phr.phr_source()
+    gprims = np.concatenate([src[a:b + 1] for a, b in rec[leaf, 1:3]])
+    assert np.array_equal(gprims, oprims), "leaf triangle lists differ"
+    assert phr.cost() == pytest.approx(ophr.cost(), rel=1e-12)
+    assert phr.size() == rec.shape[0]
+    # every triangle is in exactly one leaf and perm is a bijection
+    _, perm = phr.sorted()
+    assert np.array_equal(np.sort(perm), np.arange(len(pos)))
+    return mesh, aux, phr, otree, ophr
+
+
+@pytest.mark.parametrize("n,seed", [(2, 1), (7, 2), (100, 3), (5000, 4), (50_000, 5)])
+def test_phr_matches_oracle(ctx, oracle, n, seed):
+    print(f"phr parity n={n}", flush=True)
+    _compare(ctx, oracle, scenes.random_triangles(n, seed=seed))
+
+
+def test_phr_hq_settings_and_cap(ctx, oracle):
+    """PHR_HQ (alpha 0.55, delta 9, benchmark_test.go:16-17): the 2500 cap binds
+    and cuts above 4096 members run through the global-memory sort path."""
+    print("phr hq", flush=True)
+    _, _, phr, _, ophr = _compare(ctx, oracle, scenes.random_triangles(200_000, seed=6), 0.55, 9.0)
+    assert phr.info().phr_initial_cut == ophr.initial_cut() == 2500
+
+
+def test_phr_small_cap(ctx, oracle):
+    print("phr small cap", flush=True)
+    _compare(ctx, oracle, scenes.random_triangles(20_000, seed=7), max_cut=37)
+
+
+def test_phr_box_scene(ctx, oracle):
+    print("phr box scene", flush=True)
+    sc = scenes.box_scene(seed=1)
+    _compare(ctx, oracle, sc["pos"])
+
+
+def test_phr_config2_full(ctx, oracle):
+    """1M triangles, default builder (alpha 0.5, delta 6, branching 2)."""
+    print("phr 1M", flush=True)
+    _, _, phr, otree, ophr = _compare(ctx, oracle, scenes.random_triangles(1_000_000, seed=1))
+    assert phr.cost() < otree.cost()  # PHR improves the LBVH's SAH cost
+
+
+def test_phr_traversal_and_render(ctx, oracle):
+    """Closest hit on the PHR tree equals the oracle's LBVH answer (the hit does
+    not depend on the tree), and a render over it matches the oracle."""
+    print("phr traversal", flush=True)
+    pos = scenes.random_triangles(100_000, seed=8, mean_edge=0.01)
+    mesh = Mesh(ctx, pos)
+    aux = Bvh.lbvh(ctx, mesh)
+    phr = Bvh.phr(ctx, aux)
+    rays = scenes.incoherent_rays(100_000, seed=9)
+    hit, t = phr.closest_hit(rays)
+    ohit, ot = oracle.Tree(pos).closest_hit(rays)
+    assert np.mean(hit == ohit) >= 0.9999
+    both = (hit == ohit) & (hit >= 0)
+    assert np.max(np.abs(t[both] - ot[both]) / ot[both]) < 1e-5
+    sc = scenes.box_scene(seed=1, slices=40, rings=22, width=96, height=54)
+    m2 = Mesh(ctx, sc["pos"])
+    m2.set_shading(sc["mat_idx"], sc["materials"])
+    p2 = Bvh.phr(ctx, Bvh.lbvh(ctx, m2))
+    from trace_amd.device import render
+
+    s, q, _ = render(ctx, m2, p2, sc["camera"], 96, 54, 8, 5, _lib.MISS_SKY, seed=4)
+    os_, oq, _ = oracle.render(oracle.Tree(sc["pos"]), sc["mat_idx"], sc["materials"], sc["camera"], 96, 54, 8, 5,
+                               _lib.MISS_SKY, seed=4)
+    from test_render_gpu import within_3_sigma
+
+    frac, _ = within_3_sigma(s, q, os_, oq, 8)
+    assert frac < 0.01
+
+
+def test_phr_rejects_unsupported(ctx):
+    pos = scenes.random_triangles(100, seed=1)
+    aux = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    with pytest.raises(_lib.TraceError):
+        Bvh.phr(ctx, aux, branching=4)
+    with pytest.raises(_lib.TraceError):
+        Bvh.phr(ctx, aux, max_cut=0)

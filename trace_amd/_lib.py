@@ -47,6 +47,9 @@ class BvhInfo(C.Structure):
         ("arity", C.c_int32),
         ("pad", C.c_int32),
         ("root_box", C.c_double * 6),
+        ("phr_initial_cut", C.c_int64),
+        ("phr_levels", C.c_int32),
+        ("pad2", C.c_int32),
     ]
 
 
@@ -70,6 +73,8 @@ SIGNATURES = {
     "trace_bvh_export_canonical": [_P, _P, _I64, _P, _P, C.POINTER(_I64)],
     "trace_bvh_cost": [_P, _P, _DP],
     "trace_bvh_size": [_P, _P, C.POINTER(_I64)],
+    "trace_phr_build": [_P, _P, C.c_double, C.c_double, C.c_int, _I32, C.POINTER(_P)],
+    "trace_phr_export_source": [_P, _P, _P],
     "trace_closest_hit": [_P, _P, _P, _I64, _P, _P],
     "trace_closest_hit_device": [_P, _P, _P, _I64, _P, _P],
     "trace_closest_hit_counts": [_P, _P, _P, _I64, _P, _P],

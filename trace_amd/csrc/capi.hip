@@ -6,6 +6,7 @@
 
 #include "common.hpp"
 #include "lbvh.hpp"
+#include "phr.hpp"
 #include "render.hpp"
 #include "traverse.hpp"
 
@@ -70,11 +71,11 @@ int trace_ctx_create(int device, trace_ctx** out) {
 
 int trace_ctx_destroy(trace_ctx* ctx) {
   if (!ctx) return TRACE_E_INVALID;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  if (ctx->scratch) hipFree(ctx->scratch);
-  if (ctx->d_err) hipFree(ctx->d_err);
-  if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->d_err) (void)hipFree(ctx->d_err);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return TRACE_OK;
 }
@@ -205,6 +206,83 @@ int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out
     out->root_box[a] = h.root_lo[a];
     out->root_box[3 + a] = h.root_hi[a];
   }
+  if (bvh->key_mode < 0) {
+    out->depth = bvh->host_depth;
+    out->phr_initial_cut = bvh->phr_initial_cut;
+    out->phr_levels = bvh->phr_levels;
+  }
+  return TRACE_OK;
+}
+
+// Canonical preorder of a PHR tree from its n-ary output nodes: a branch's
+// range is its subtree's triangle range, a leaf's its own; one-child branches
+// (a cut that no split improved, phr.go:121-126) are kept, as Cost() sees them.
+static int export_phr(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap, int32_t* h_rec,
+                      float* h_box, int64_t* n_out) {
+  std::vector<phr::OutNode> out(bvh->phr_out_count);
+  HIP_TRY(ctx, hipMemcpyAsync(out.data(), bvh->phr_out.p, out.size() * sizeof(phr::OutNode),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int32_t> st{0};
+  int64_t k = 0;
+  std::vector<int32_t> order;
+  order.reserve(out.size());
+  while (!st.empty()) {
+    const int32_t o = st.back();
+    st.pop_back();
+    if (o < 0 || o >= int32_t(out.size()))
+      TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_bvh_export_canonical: PHR node %d out of range", o);
+    order.push_back(o);
+    const phr::OutNode& nd = out[o];
+    if (nd.kind == 0)
+      for (int c = nd.nchild - 1; c >= 0; --c) st.push_back(nd.child[c]);
+  }
+  *n_out = int64_t(order.size());
+  if (!h_rec || !h_box) return TRACE_OK;
+  if (cap < int64_t(order.size()))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_canonical: capacity %lld < %lld nodes",
+               (long long)cap, (long long)order.size());
+  for (int32_t o : order) {
+    const phr::OutNode& nd = out[o];
+    int32_t* r = h_rec + 4 * k;
+    r[0] = nd.kind == 0 ? 0 : 1;
+    r[1] = nd.prim_off;
+    r[2] = nd.prim_off + nd.prim_cnt - 1;
+    r[3] = nd.kind == 0 ? nd.nchild : 0;
+    for (int a = 0; a < 6; ++a) h_box[6 * k + a] = nd.box[a];
+    ++k;
+  }
+  return TRACE_OK;
+}
+
+int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,
+                    int branching, int32_t max_cut, trace_bvh** out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!aux || !out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: null argument");
+  *out = nullptr;
+  if (aux->key_mode < 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: the auxiliary tree must be an LBVH");
+  if (branching != 2) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: branching factor %d (this build supports 2)", branching);
+  if (!(alpha >= 0) || !std::isfinite(delta) || max_cut < 1)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: need alpha >= 0, finite delta, max_cut >= 1");
+  trace_bvh* b = new (std::nothrow) trace_bvh();
+  if (!b) TRACE_FAIL(ctx, TRACE_E_OOM, "trace_phr_build: host allocation failed");
+  int rc = phr::build(ctx, aux, alpha, delta, branching, max_cut, b);
+  if (rc == TRACE_OK) rc = ctx_check_device_error(ctx, "trace_phr_build");
+  if (rc != TRACE_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return TRACE_OK;
+}
+
+int trace_phr_export_source(trace_ctx* ctx, const trace_bvh* phr, int32_t* h_src) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!phr || !h_src) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_export_source: null argument");
+  if (phr->key_mode >= 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_export_source: not a PHR tree");
+  HIP_TRY(ctx, hipMemcpyAsync(h_src, phr->phr_src.p, phr->n_prims * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return TRACE_OK;
 }
 
@@ -250,6 +328,7 @@ int trace_bvh_export_canonical(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap
                                float* h_box, int64_t* n_out) {
   if (!ctx) return TRACE_E_INVALID;
   if (!bvh || !n_out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_canonical: null argument");
+  if (bvh->key_mode < 0 && !bvh->phr_clone) return export_phr(ctx, bvh, cap, h_rec, h_box, n_out);
   HostTree t;
   int rc = fetch_tree(ctx, bvh, &t);
   if (rc) return rc;

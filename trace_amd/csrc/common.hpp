@@ -54,7 +54,7 @@ struct DevArray {
   DevArray& operator=(const DevArray&) = delete;
   ~DevArray() { release(); }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
   }
@@ -147,6 +147,12 @@ struct trace_bvh {
   int64_t phr_out_count = 0;
   DevArray<int32_t> phr_src;
   bool phr_clone = false;  // the PHR tree is the LBVH itself (initial cut of one node)
+  double phr_alpha = 0, phr_delta = 0;
+  int64_t phr_initial_cut = 0;
+  int32_t phr_levels = 0;
+  // upper bound on inner nodes along any root-to-leaf path (sizes the
+  // traversal stack); an LBVH over 64-bit keys never exceeds 63
+  int32_t host_depth = 63;
   // host mirror of the n-ary structure for PHR trees (empty for LBVH)
   std::vector<int32_t> nary_first;   // per node: leaf -> first prim pos, inner -> -1
   std::vector<int32_t> nary_count;   // per node: leaf -> prim count, inner -> #children

@@ -1,0 +1,847 @@
+// Progressive Hierarchical Refinement (PhrBuilder.BuildFromAuxilary,
+// phr.go:47-459) on the GPU, branching factor 2.
+//
+// The reference recursion is a tree of independent jobs: a job holds a cut
+// (LBVH nodes, a bounding box, a depth), splits it with SweepSAH and refines
+// both halves into child jobs.  Here the jobs of one tree level run as one
+// launch, one wave per job:
+//   sort      three stable sorts of the cut by box centre x, y, z (rank by
+//             counting: rank = #smaller + #equal-and-earlier), with the z sort
+//             fed from the y or x order exactly as phr.go:384-404 does
+//   minCost   suffix and prefix box unions and subtree-size sums per order
+//             (wave scans), FP64 costs with the reference's operation order,
+//             first minimum wins, the no-split cost wins ties
+//   refine    keep a node if it is a leaf or SA < S / 2^(alpha*depth+delta),
+//             else replace it by its children; one survivor becomes a leaf of
+//             all its triangles (phr.go:313-352)
+// Child jobs, output nodes and leaf lists are appended with atomics; the tree
+// *structure* and every triangle range are fixed by the parent, so the result
+// is deterministic.  A final pass lays the tree out as the binary traversal
+// format, with triangles in PHR leaf order.
+//
+// The initial cut (phr.go:257-311) is taken in FIFO order from the root with
+// the MAX_CUT_SIZE cap -- the order the reference's channel queue follows
+// with one worker.  Thresholds per depth are computed on the host with
+// std::pow so they equal the oracle's bit for bit.
+#include <cmath>
+
+#include "phr.hpp"
+#include "scan.hpp"
+
+namespace phr {
+
+constexpr int kLdsKeys = 2048;
+constexpr int kMaxDepth = 1024;
+
+struct Flat {  // LBVH as flat ids: 0..U-2 inner nodes, U-1..2U-2 leaves
+  float* box;      // 6 per id
+  int32_t* child;  // 2 per id, -1 for leaves
+  int32_t* size;   // subtree node count (bvh.go:159-167)
+  int32_t* prim;   // 2 per id: [first, end) in LBVH sorted order
+};
+
+struct Task {
+  int32_t off, cnt;  // cut members in the pool
+  float box[6];      // cut bounding
+  int32_t depth;     // cut depth (initial cut: 1)
+  int32_t out;       // output node this job fills
+  int32_t prim_off;  // first PHR-ordered triangle of this subtree
+  int32_t level;     // depth of the output node
+};
+
+struct Ctl {
+  uint32_t n_tasks;  // jobs appended for the next level
+  uint32_t pool;     // pool entries appended for the next level
+  uint32_t out;      // output nodes allocated
+  uint32_t list;     // leaf-list entries allocated
+  uint32_t max_level;
+  uint32_t n_leaves;
+  uint32_t err;      // 1 capacity, 2 depth table
+  uint32_t cut;      // initial cut size
+};
+
+// bounding.go:45-47, FP64 from FP32 box corners (exact differences)
+__device__ inline double sa_box(const float* b) {
+#pragma clang fp contract(off)
+  const double w = double(b[3]) - double(b[0]);
+  const double h = double(b[4]) - double(b[1]);
+  const double d = double(b[5]) - double(b[2]);
+  return 2 * w * h + 2 * w * d + 2 * h * d;
+}
+
+__global__ void flat_kernel(const BvhNode* __restrict__ nodes, const int32_t* __restrict__ parent_inner,
+                            const int32_t* __restrict__ parent_leaf,
+                            const int32_t* __restrict__ inner_range,
+                            const uint32_t* __restrict__ run_start, const BvhHeader* __restrict__ hdr,
+                            Flat f) {
+  const int32_t U = hdr->n_leaves;
+  const int32_t ni = U - 1;
+  const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 2 * U - 1) return;
+  int32_t link;
+  if (id < ni) {
+    const int32_t lo = inner_range[2 * id], hi = inner_range[2 * id + 1];
+    const BvhNode& nd = nodes[id];
+    // a leaf child 0 is the leaf at the range's low end, a leaf child 1 the high end
+    f.child[2 * id] = nd.cnt[0] > 0 ? ni + lo : nd.ref[0];
+    f.child[2 * id + 1] = nd.cnt[1] > 0 ? ni + hi : nd.ref[1];
+    f.size[id] = 2 * (hi - lo + 1) - 1;
+    f.prim[2 * id] = int32_t(run_start[lo]);
+    f.prim[2 * id + 1] = int32_t(run_start[hi + 1]);
+    link = parent_inner[id];
+  } else {
+    const int32_t u = id - ni;
+    f.child[2 * id] = -1;
+    f.child[2 * id + 1] = -1;
+    f.size[id] = 1;
+    f.prim[2 * id] = int32_t(run_start[u]);
+    f.prim[2 * id + 1] = int32_t(run_start[u + 1]);
+    link = parent_leaf[u];
+  }
+  float* b = f.box + 6 * size_t(id);
+  if (link < 0) {
+    for (int a = 0; a < 3; ++a) {
+      b[a] = hdr->root_lo[a];
+      b[3 + a] = hdr->root_hi[a];
+    }
+  } else {
+    const float* src = reinterpret_cast<const float*>(nodes + (link >> 1)) + 6 * (link & 1);
+    for (int a = 0; a < 6; ++a) b[a] = src[a];
+  }
+}
+
+__device__ inline uint32_t block_scan_1024(uint32_t v, uint32_t* s_wave, uint32_t* total) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[w] = x;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int i = 0; i < 16; ++i) {
+    if (i < w) off += s_wave[i];
+    tot += s_wave[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + x - v;
+}
+
+// phr.go:257-311 in FIFO order (one workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void initial_cut_kernel(Flat f, double th0, int32_t max_cut,
+                                                           int32_t* fa, int32_t* fb, int32_t fcap,
+                                                           int32_t* pool, int32_t pool_cap,
+                                                           Task* tasks, Ctl* ctl,
+                                                           const BvhHeader* __restrict__ hdr) {
+  __shared__ uint32_t s_wave[16];
+  __shared__ int32_t s_m, s_cut, s_next;
+  __shared__ long long s_counter;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    fa[0] = 0;  // the LBVH root
+    s_m = 1;
+    s_cut = 0;
+    s_counter = 0;
+  }
+  __syncthreads();
+  int32_t* cur = fa;
+  int32_t* nxt = fb;
+  while (true) {
+    const int32_t m = s_m;
+    if (m == 0) break;
+    __syncthreads();
+    if (tid == 0) s_next = 0;
+    __syncthreads();
+    for (int32_t c0 = 0; c0 < m; c0 += 1024) {
+      const int32_t j = c0 + tid;
+      const bool valid = j < m;
+      const int32_t id = valid ? cur[j] : 0;
+      const bool exp = valid && f.child[2 * id] >= 0 && sa_box(f.box + 6 * size_t(id)) > th0;
+      uint32_t etot, ctot, xtot;
+      const uint32_t er = block_scan_1024(exp ? 1u : 0u, s_wave, &etot);
+      const long long before = s_counter;
+      // the counter grows by len(children) - 1 = 1 for every node above the
+      // threshold, expanded or not; the node expands while it stays < cap
+      const bool expands = exp && (before + er + 1 < max_cut);
+      const bool tocut = valid && !expands;
+      const uint32_t cr = block_scan_1024(tocut ? 1u : 0u, s_wave, &ctot);
+      const uint32_t xr = block_scan_1024(expands ? 1u : 0u, s_wave, &xtot);
+      if (tocut) {
+        const int32_t p = s_cut + int32_t(cr);
+        if (p < pool_cap) pool[p] = id;
+        else atomicOr(&ctl->err, 1u);
+      }
+      if (expands) {
+        const int32_t q = s_next + 2 * int32_t(xr);
+        if (q + 1 < fcap) {
+          nxt[q] = f.child[2 * id];
+          nxt[q + 1] = f.child[2 * id + 1];
+        } else {
+          atomicOr(&ctl->err, 1u);
+        }
+      }
+      __syncthreads();
+      if (tid == 0) {
+        s_counter += etot;
+        s_cut += int32_t(ctot);
+        s_next += 2 * int32_t(xtot);
+      }
+      __syncthreads();
+    }
+    int32_t* t = cur;
+    cur = nxt;
+    nxt = t;
+    if (tid == 0) s_m = s_next < fcap ? s_next : 0;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    Task t;
+    t.off = 0;
+    t.cnt = s_cut < pool_cap ? s_cut : pool_cap;
+    for (int a = 0; a < 3; ++a) {
+      t.box[a] = hdr->root_lo[a];
+      t.box[3 + a] = hdr->root_hi[a];
+    }
+    t.depth = 1;
+    t.out = 0;
+    t.prim_off = 0;
+    t.level = 0;
+    tasks[0] = t;
+    ctl->cut = uint32_t(s_cut);
+    ctl->n_tasks = 1;
+    ctl->out = 1;  // output node 0 is the root
+  }
+}
+
+// ---- wave helpers -------------------------------------------------------------
+struct WBox {
+  float lo[3], hi[3];
+  uint32_t cnt;
+};
+
+__device__ inline void wb_empty(WBox& b) {
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = INFINITY;
+    b.hi[a] = -INFINITY;
+  }
+  b.cnt = 0;
+}
+__device__ inline void wb_merge(WBox& a, const WBox& b) {
+  for (int k = 0; k < 3; ++k) {
+    a.lo[k] = fminf(a.lo[k], b.lo[k]);
+    a.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+  }
+  a.cnt += b.cnt;
+}
+__device__ inline WBox wb_shfl_up(const WBox& b, int o) {
+  WBox r;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = __shfl_up(b.lo[k], o, 64);
+    r.hi[k] = __shfl_up(b.hi[k], o, 64);
+  }
+  r.cnt = __shfl_up(b.cnt, o, 64);
+  return r;
+}
+__device__ inline WBox wb_shfl(const WBox& b, int src) {
+  WBox r;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = __shfl(b.lo[k], src, 64);
+    r.hi[k] = __shfl(b.hi[k], src, 64);
+  }
+  r.cnt = __shfl(b.cnt, src, 64);
+  return r;
+}
+__device__ inline WBox wb_shfl_xor(const WBox& b, int o) {
+  WBox r;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = __shfl_xor(b.lo[k], o, 64);
+    r.hi[k] = __shfl_xor(b.hi[k], o, 64);
+  }
+  r.cnt = __shfl_xor(b.cnt, o, 64);
+  return r;
+}
+// inclusive scan across lanes; unions and counts are exact in any order
+__device__ inline WBox wb_scan(WBox v) {
+  const int lane = threadIdx.x;
+  for (int o = 1; o < 64; o <<= 1) {
+    const WBox u = wb_shfl_up(v, o);
+    if (lane >= o) wb_merge(v, u);
+  }
+  return v;
+}
+__device__ inline WBox wb_reduce(WBox v) {
+  for (int o = 32; o > 0; o >>= 1) wb_merge(v, wb_shfl_xor(v, o));
+  return v;
+}
+__device__ inline double sa_wb(const WBox& b) {
+  const float bb[6] = {b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]};
+  return sa_box(bb);
+}
+__device__ inline uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline uint32_t wave_excl(uint32_t v, uint32_t* total) {
+  const int lane = threadIdx.x;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  *total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+__device__ inline void load_member(const Flat& f, int32_t node, WBox& b, bool with_size) {
+  const float* p = f.box + 6 * size_t(node);
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = p[a];
+    b.hi[a] = p[3 + a];
+  }
+  b.cnt = with_size ? uint32_t(f.size[node]) : 1u;
+}
+
+// sort.SliceStable of the cut by box centre on `axis`, starting from order
+// `in` (identity when null).  Keys are exact FP64 centres.
+__device__ void sort_axis(const Flat& f, const int32_t* ids, const int32_t* in, int32_t* out,
+                          int axis, int m, double* keys) {
+  const int lane = threadIdx.x;
+  for (int r = lane; r < m; r += 64) {
+    const int32_t j = in ? in[r] : r;
+    const float* b = f.box + 6 * size_t(ids[j]);
+    keys[r] = (double(b[axis]) + double(b[3 + axis])) * (1.0 / 2.0);
+  }
+  __syncthreads();
+  for (int r = lane; r < m; r += 64) {
+    const double kr = keys[r];
+    int rank = 0;
+    for (int q = 0; q < m; ++q) {
+      const double kq = keys[q];
+      rank += (kq < kr || (kq == kr && q < r)) ? 1 : 0;
+    }
+    out[rank] = in ? in[r] : r;
+  }
+  __syncthreads();
+}
+
+struct Split {
+  double cost;
+  int32_t at;  // 0 = no split
+};
+
+// minCost (phr.go:414-459) for the cut in order `ord`.
+__device__ Split min_cost(const Flat& f, const int32_t* ids, const int32_t* ord, int m,
+                          const float* cut_box, double* rcost) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x;
+  WBox carry;
+  wb_empty(carry);
+  for (int end = m; end > 0; end -= 64) {
+    const int r = end - 1 - lane;
+    WBox v;
+    wb_empty(v);
+    if (r >= 0) load_member(f, ids[ord[r]], v, true);
+    v = wb_scan(v);  // lane l: positions [end - 1 - l, end)
+    WBox R = v;
+    wb_merge(R, carry);
+    if (r >= 1) rcost[r] = sa_wb(R) * double(R.cnt);
+    wb_merge(carry, wb_shfl(v, 63));
+  }
+  __syncthreads();
+  const uint32_t total = carry.cnt;
+  wb_empty(carry);
+  double best = INFINITY;
+  int32_t best_at = m;
+  for (int start = 0; start < m; start += 64) {
+    const int r = start + lane;
+    WBox v;
+    wb_empty(v);
+    if (r < m) load_member(f, ids[ord[r]], v, true);
+    const WBox incl = wb_scan(v);
+    WBox P = wb_shfl_up(incl, 1);
+    if (lane == 0) wb_empty(P);
+    wb_merge(P, carry);
+    if (r >= 1 && r < m) {
+      const double c = rcost[r] + sa_wb(P) * double(P.cnt);
+      if (c < best) {
+        best = c;
+        best_at = r;
+      }
+    }
+    wb_merge(carry, wb_shfl(incl, 63));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const int32_t oa = __shfl_xor(best_at, o, 64);
+    if (ob < best || (ob == best && oa < best_at)) {
+      best = ob;
+      best_at = oa;
+    }
+  }
+  __syncthreads();
+  const double nosplit = sa_box(cut_box) * double(total);
+  if (best < nosplit) return Split{best, best_at};
+  return Split{nosplit, 0};
+}
+
+__device__ inline void write_out(OutNode* out, int32_t o, int kind, int nchild, int32_t c0,
+                                 int32_t c1, int32_t list_off, int32_t list_cnt, int32_t prim_off,
+                                 int32_t prim_cnt, int32_t level, const float* box) {
+  OutNode n;
+  n.kind = kind;
+  n.nchild = nchild;
+  n.child[0] = c0;
+  n.child[1] = c1;
+  n.list_off = list_off;
+  n.list_cnt = list_cnt;
+  n.prim_off = prim_off;
+  n.prim_cnt = prim_cnt;
+  n.level = level;
+  for (int a = 0; a < 6; ++a) n.box[a] = box[a];
+  n.pad = 0;
+  out[o] = n;
+}
+
+struct Caps {
+  int64_t pool;
+  int32_t out, list, tasks;
+};
+
+// Emits one refined half [a, b) of order C: a leaf when a single node
+// survives, otherwise a job of the next level.
+__device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, int a, int b,
+                          double th, const WBox& hb, int32_t prim_off, int32_t prim_cnt,
+                          const Task& t, int32_t o, int32_t* pool_next, Task* tasks_next,
+                          OutNode* out, int32_t* list, Ctl* ctl, const Caps& caps) {
+  const int lane = threadIdx.x;
+  uint32_t k_total = 0;
+  for (int r = a + lane; r < b; r += 64) {
+    const int32_t node = ids[C[r]];
+    const bool keep = f.child[2 * node] < 0 || sa_box(f.box + 6 * size_t(node)) < th;
+    k_total += keep ? 1u : 2u;
+  }
+  k_total = wave_sum(k_total);
+  const float box[6] = {hb.lo[0], hb.lo[1], hb.lo[2], hb.hi[0], hb.hi[1], hb.hi[2]};
+  if (k_total == 1) {  // one kept node: a leaf of all its triangles
+    int32_t ls = 0;
+    if (lane == 0) {
+      ls = int32_t(atomicAdd(&ctl->list, 1u));
+      if (ls < caps.list) {
+        list[ls] = ids[C[a]];
+        write_out(out, o, 1, 0, -1, -1, ls, 1, prim_off, prim_cnt, t.level + 1, box);
+        atomicMax(&ctl->max_level, uint32_t(t.level + 1));
+        atomicAdd(&ctl->n_leaves, 1u);
+      } else {
+        atomicOr(&ctl->err, 1u);
+      }
+    }
+    return;
+  }
+  int32_t slot = 0, poff = 0;
+  if (lane == 0) {
+    slot = int32_t(atomicAdd(&ctl->n_tasks, 1u));
+    poff = int32_t(atomicAdd(&ctl->pool, k_total));
+  }
+  slot = __shfl(slot, 0, 64);
+  poff = __shfl(poff, 0, 64);
+  if (slot >= caps.tasks || int64_t(poff) + k_total > caps.pool) {
+    if (lane == 0) atomicOr(&ctl->err, 1u);
+    return;
+  }
+  uint32_t carry = 0;
+  for (int r0 = a; r0 < b; r0 += 64) {
+    const int r = r0 + lane;
+    int32_t node = 0;
+    uint32_t k = 0;
+    if (r < b) {
+      node = ids[C[r]];
+      const bool keep = f.child[2 * node] < 0 || sa_box(f.box + 6 * size_t(node)) < th;
+      k = keep ? 1u : 2u;
+    }
+    uint32_t tot;
+    const uint32_t ex = wave_excl(k, &tot);
+    const int32_t p = poff + int32_t(carry + ex);
+    if (k == 1) {
+      pool_next[p] = node;
+    } else if (k == 2) {
+      pool_next[p] = f.child[2 * node];
+      pool_next[p + 1] = f.child[2 * node + 1];
+    }
+    carry += tot;
+  }
+  if (lane == 0) {
+    Task n;
+    n.off = poff;
+    n.cnt = int32_t(k_total);
+    for (int q = 0; q < 6; ++q) n.box[q] = box[q];
+    n.depth = t.depth + 1;
+    n.out = o;
+    n.prim_off = prim_off;
+    n.level = t.level + 1;
+    tasks_next[slot] = n;
+  }
+}
+
+// One wave per job of the current level (phr.go:92-152 with BranchingFactor 2).
+__global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restrict__ th,
+                                                  const Task* __restrict__ tasks,
+                                                  const int32_t* __restrict__ pool,
+                                                  int32_t* __restrict__ pool_next,
+                                                  Task* __restrict__ tasks_next,
+                                                  int32_t* __restrict__ ordbuf, double* gkey,
+                                                  double* rcost_buf, OutNode* __restrict__ out,
+                                                  int32_t* __restrict__ list, Ctl* ctl, Caps caps) {
+  __shared__ double s_key[kLdsKeys];
+  const int lane = threadIdx.x;
+  const Task t = tasks[blockIdx.x];
+  const int m = t.cnt;
+  const int32_t* ids = pool + t.off;
+  int32_t* oX = ordbuf + t.off;
+  int32_t* oY = ordbuf + caps.pool + t.off;
+  int32_t* oZ = ordbuf + 2 * caps.pool + t.off;
+  double* keys = m <= kLdsKeys ? s_key : gkey + t.off;
+  double* rc = rcost_buf + t.off;
+  if (t.depth >= kMaxDepth) {
+    if (lane == 0) atomicOr(&ctl->err, 2u);
+    return;
+  }
+
+  // SweepSAH (phr.go:368-405)
+  sort_axis(f, ids, nullptr, oX, 0, m, keys);
+  sort_axis(f, ids, oX, oY, 1, m, keys);
+  const Split x = min_cost(f, ids, oX, m, t.box, rc);
+  const Split y = min_cost(f, ids, oY, m, t.box, rc);
+  sort_axis(f, ids, x.cost < y.cost ? oY : oX, oZ, 2, m, keys);
+  const Split z = min_cost(f, ids, oZ, m, t.box, rc);
+  const int32_t* C;
+  Split s;
+  if (x.cost < y.cost) {
+    if (x.cost < z.cost) { C = oX; s = x; } else { C = oZ; s = z; }
+  } else {
+    if (y.cost < z.cost) { C = oY; s = y; } else { C = oZ; s = z; }
+  }
+
+  if (s.at > 0) {
+    WBox L, R;
+    wb_empty(L);
+    wb_empty(R);
+    uint32_t pl = 0, pr = 0;
+    for (int r = lane; r < m; r += 64) {
+      const int32_t node = ids[C[r]];
+      WBox v;
+      load_member(f, node, v, false);
+      const uint32_t pc = uint32_t(f.prim[2 * node + 1] - f.prim[2 * node]);
+      if (r < s.at) { wb_merge(L, v); pl += pc; } else { wb_merge(R, v); pr += pc; }
+    }
+    L = wb_reduce(L);
+    R = wb_reduce(R);
+    pl = wave_sum(pl);
+    pr = wave_sum(pr);
+    int32_t o = 0;
+    if (lane == 0) o = int32_t(atomicAdd(&ctl->out, 2u));
+    o = __shfl(o, 0, 64);
+    if (o + 1 >= caps.out) {
+      if (lane == 0) atomicOr(&ctl->err, 1u);
+      return;
+    }
+    if (lane == 0)
+      write_out(out, t.out, 0, 2, o, o + 1, -1, 0, t.prim_off, int32_t(pl + pr), t.level, t.box);
+    const double thd = th[t.depth];  // refined() uses the split cut's depth
+    emit_half(f, ids, C, 0, s.at, thd, L, t.prim_off, int32_t(pl), t, o, pool_next, tasks_next,
+              out, list, ctl, caps);
+    emit_half(f, ids, C, s.at, m, thd, R, t.prim_off + int32_t(pl), int32_t(pr), t, o + 1,
+              pool_next, tasks_next, out, list, ctl, caps);
+  } else {
+    // no split beats the whole cut: one leaf of every triangle, in order C
+    uint32_t pc = 0;
+    for (int r = lane; r < m; r += 64) {
+      const int32_t node = ids[C[r]];
+      pc += uint32_t(f.prim[2 * node + 1] - f.prim[2 * node]);
+    }
+    pc = wave_sum(pc);
+    int32_t o = 0, ls = 0;
+    if (lane == 0) {
+      o = int32_t(atomicAdd(&ctl->out, 1u));
+      ls = int32_t(atomicAdd(&ctl->list, uint32_t(m)));
+    }
+    o = __shfl(o, 0, 64);
+    ls = __shfl(ls, 0, 64);
+    if (o >= caps.out || ls + m > caps.list) {
+      if (lane == 0) atomicOr(&ctl->err, 1u);
+      return;
+    }
+    for (int r = lane; r < m; r += 64) list[ls + r] = ids[C[r]];
+    if (lane == 0) {
+      write_out(out, o, 1, 0, -1, -1, ls, m, t.prim_off, int32_t(pc), t.level + 1, t.box);
+      write_out(out, t.out, 0, 1, o, -1, -1, 0, t.prim_off, int32_t(pc), t.level, t.box);
+      atomicMax(&ctl->max_level, uint32_t(t.level + 1));
+      atomicAdd(&ctl->n_leaves, 1u);
+    }
+  }
+}
+
+__global__ void reset_level_kernel(Ctl* ctl) {
+  ctl->n_tasks = 0;
+  ctl->pool = 0;
+}
+
+// ---- layout of the traversal tree ------------------------------------------------
+__global__ void branch_flag_kernel(const OutNode* __restrict__ out, int32_t n, uint32_t* flag) {
+  const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < n) flag[o] = (out[o].kind == 0 && out[o].nchild == 2) ? 1u : 0u;
+}
+
+__global__ void trav_nodes_kernel(const OutNode* __restrict__ out, int32_t n,
+                                  const uint32_t* __restrict__ flag,
+                                  const uint32_t* __restrict__ tindex, BvhNode* __restrict__ nodes) {
+  const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n || !flag[o]) return;
+  BvhNode nd;
+  float* boxes = reinterpret_cast<float*>(&nd);
+  for (int c = 0; c < 2; ++c) {
+    const int32_t k = out[o].child[c];
+    int32_t kk = k;
+    if (out[kk].kind == 0 && out[kk].nchild == 1) kk = out[kk].child[0];  // branch over one leaf
+    if (out[kk].kind == 0) {
+      nd.ref[c] = int32_t(tindex[kk]);
+      nd.cnt[c] = 0;
+    } else {
+      nd.ref[c] = out[kk].prim_off;
+      nd.cnt[c] = out[kk].prim_cnt;
+    }
+    for (int a = 0; a < 6; ++a) boxes[6 * c + a] = out[k].box[a];
+  }
+  nodes[tindex[o]] = nd;
+}
+
+// One wave per output leaf: copy its LBVH triangle ranges into PHR order.
+__global__ __launch_bounds__(64) void leaf_copy_kernel(
+    const OutNode* __restrict__ out, const int32_t* __restrict__ list, Flat f,
+    const int32_t* __restrict__ perm_in, const float* __restrict__ tris_in,
+    const float* __restrict__ box_in, int32_t* __restrict__ perm_out, int32_t* __restrict__ src_out,
+    float* __restrict__ tris_out, float* __restrict__ box_out, int32_t n, Ctl* ctl) {
+  const OutNode nd = out[blockIdx.x];
+  if (nd.kind != 1) return;
+  if (nd.prim_off < 0 || nd.prim_cnt < 0 || nd.prim_cnt > n - nd.prim_off) {
+    if (threadIdx.x == 0) atomicOr(&ctl->err, 4u);
+    return;
+  }
+  int32_t dst = nd.prim_off;
+  const int32_t end = nd.prim_off + nd.prim_cnt;
+  for (int32_t e = 0; e < nd.list_cnt; ++e) {
+    const int32_t node = list[nd.list_off + e];
+    const int32_t a = f.prim[2 * node], b = f.prim[2 * node + 1];
+    if (b - a > end - dst) {  // the list must fill exactly [prim_off, end)
+      if (threadIdx.x == 0) atomicOr(&ctl->err, 4u);
+      return;
+    }
+    for (int32_t k = a + int32_t(threadIdx.x); k < b; k += 64) {
+      const int32_t d = dst + (k - a);
+      perm_out[d] = perm_in[k];
+      src_out[d] = k;
+      for (int q = 0; q < 9; ++q) tris_out[9 * size_t(d) + q] = tris_in[9 * size_t(k) + q];
+      for (int q = 0; q < 6; ++q) box_out[6 * size_t(d) + q] = box_in[6 * size_t(k) + q];
+    }
+    dst += b - a;
+  }
+}
+
+__global__ void phr_header_kernel(const OutNode* __restrict__ out, const BvhHeader* __restrict__ aux,
+                                  const Ctl* __restrict__ ctl, const uint32_t* __restrict__ n_trav,
+                                  int32_t n, BvhHeader* __restrict__ hdr) {
+  BvhHeader h = *aux;
+  const OutNode& root = out[0];
+  if (root.kind == 0 && root.nchild == 2) {
+    h.root_ref = 0;  // the root is the first two-child branch in index order
+    h.root_cnt = 0;
+  } else {
+    h.root_ref = 0;
+    h.root_cnt = n;
+  }
+  h.n_inner = int32_t(*n_trav);
+  h.n_leaves = int32_t(ctl->n_leaves);
+  h.depth = int32_t(ctl->max_level);
+  h.n_prims = n;
+  *hdr = h;
+}
+
+// ---- host driver ---------------------------------------------------------------
+static int read_ctl(trace_ctx* ctx, const Ctl* d, Ctl* h) {
+  HIP_TRY(ctx, hipMemcpyAsync(h, d, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return TRACE_OK;
+}
+
+static int clone_lbvh(trace_ctx* ctx, const trace_bvh* aux, trace_bvh* out) {
+  const int64_t n = aux->n_prims;
+  hipStream_t st = ctx->stream;
+  BvhHeader h;
+  HIP_TRY(ctx, hipMemcpyAsync(&h, aux->header.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  const int64_t ninner = std::max<int32_t>(1, h.root_cnt > 0 ? 1 : h.n_inner);
+  HIP_TRY(ctx, out->perm.alloc(n));
+  HIP_TRY(ctx, out->tris.alloc(9 * n));
+  HIP_TRY(ctx, out->prim_box.alloc(6 * n));
+  HIP_TRY(ctx, out->nodes.alloc(ninner));
+  HIP_TRY(ctx, out->header.alloc(1));
+  HIP_TRY(ctx, out->phr_src.alloc(n));
+  HIP_TRY(ctx, out->inner_range.alloc(2 * ninner));
+  HIP_TRY(ctx, out->run_start.alloc(n + 1));
+  HIP_TRY(ctx, hipMemcpyAsync(out->perm.p, aux->perm.p, n * 4, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(out->tris.p, aux->tris.p, 9 * n * 4, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(out->prim_box.p, aux->prim_box.p, 6 * n * 4, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(out->nodes.p, aux->nodes.p, ninner * sizeof(BvhNode), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(out->header.p, aux->header.p, sizeof(BvhHeader), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(out->inner_range.p, aux->inner_range.p, 2 * ninner * 4, hipMemcpyDeviceToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(out->run_start.p, aux->run_start.p, (n + 1) * 4, hipMemcpyDeviceToDevice, st));
+  std::vector<int32_t> iota(n);
+  for (int64_t i = 0; i < n; ++i) iota[i] = int32_t(i);
+  HIP_TRY(ctx, hipMemcpyAsync(out->phr_src.p, iota.data(), n * 4, hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  out->phr_clone = true;
+  out->host_depth = h.depth;
+  return TRACE_OK;
+}
+
+int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int branching,
+          int32_t max_cut, trace_bvh* out) {
+#pragma clang fp contract(off)
+  const int64_t n = aux->n_prims;
+  hipStream_t st = ctx->stream;
+  out->n_prims = n;
+  out->key_mode = -1;
+  out->arity = branching;
+  out->phr_alpha = alpha;
+  out->phr_delta = delta;
+
+  BvhHeader ah;
+  HIP_TRY(ctx, hipMemcpyAsync(&ah, aux->header.p, sizeof(ah), hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  if (ah.root_cnt > 0) return clone_lbvh(ctx, aux, out);  // a one-leaf LBVH
+
+  const int32_t U = ah.n_leaves;
+  const int64_t F = 2 * int64_t(U) - 1;
+  // thresholds S / 2^(alpha*depth + delta) (phr.go:356-358), FP64 on the host
+  const float rb[6] = {ah.root_lo[0], ah.root_lo[1], ah.root_lo[2], ah.root_hi[0], ah.root_hi[1], ah.root_hi[2]};
+  const double w = double(rb[3]) - double(rb[0]), hgt = double(rb[4]) - double(rb[1]),
+               dep = double(rb[5]) - double(rb[2]);
+  const double S = 2 * w * hgt + 2 * w * dep + 2 * hgt * dep;
+  std::vector<double> th(kMaxDepth);
+  for (int d = 0; d < kMaxDepth; ++d) th[d] = S / std::pow(2.0, alpha * double(d) + delta);
+
+  const int64_t fcap = std::min<int64_t>(2 * int64_t(std::max<int32_t>(max_cut, 1)) + 64, 2 * F + 64);
+  const int64_t pool_cap = F + 64;
+  const int32_t out_cap = int32_t(std::min<int64_t>(3 * n + 64, (int64_t(1) << 31) - 1));
+  const int32_t list_cap = int32_t(F + 64);
+  const int32_t task_cap = int32_t(n + 64);
+
+  ScratchPlan plan;
+  const size_t o_fbox = plan.take(F * 6 * sizeof(float));
+  const size_t o_fch = plan.take(F * 2 * sizeof(int32_t));
+  const size_t o_fsz = plan.take(F * sizeof(int32_t));
+  const size_t o_fpr = plan.take(F * 2 * sizeof(int32_t));
+  const size_t o_th = plan.take(kMaxDepth * sizeof(double));
+  const size_t o_fa = plan.take(fcap * sizeof(int32_t));
+  const size_t o_fb = plan.take(fcap * sizeof(int32_t));
+  const size_t o_pa = plan.take(pool_cap * sizeof(int32_t));
+  const size_t o_pb = plan.take(pool_cap * sizeof(int32_t));
+  const size_t o_ta = plan.take(task_cap * sizeof(Task));
+  const size_t o_tb = plan.take(task_cap * sizeof(Task));
+  const size_t o_ord = plan.take(3 * pool_cap * sizeof(int32_t));
+  const size_t o_key = plan.take(pool_cap * sizeof(double));
+  const size_t o_rc = plan.take(pool_cap * sizeof(double));
+  const size_t o_list = plan.take(list_cap * sizeof(int32_t));
+  const size_t o_ctl = plan.take(sizeof(Ctl));
+  const size_t o_flag = plan.take(out_cap * sizeof(uint32_t));
+  const size_t o_tidx = plan.take(out_cap * sizeof(uint32_t));
+  const size_t o_scan = plan.take(scan::scratch_words(out_cap) * sizeof(uint32_t));
+  const size_t o_ntr = plan.take(64);
+  int rc = ctx_reserve_scratch(ctx, plan.off);
+  if (rc) return rc;
+  char* base = static_cast<char*>(ctx->scratch);
+  Flat f;
+  f.box = reinterpret_cast<float*>(base + o_fbox);
+  f.child = reinterpret_cast<int32_t*>(base + o_fch);
+  f.size = reinterpret_cast<int32_t*>(base + o_fsz);
+  f.prim = reinterpret_cast<int32_t*>(base + o_fpr);
+  auto* d_th = reinterpret_cast<double*>(base + o_th);
+  auto* fa = reinterpret_cast<int32_t*>(base + o_fa);
+  auto* fb = reinterpret_cast<int32_t*>(base + o_fb);
+  int32_t* pools[2] = {reinterpret_cast<int32_t*>(base + o_pa), reinterpret_cast<int32_t*>(base + o_pb)};
+  Task* tasks[2] = {reinterpret_cast<Task*>(base + o_ta), reinterpret_cast<Task*>(base + o_tb)};
+  auto* ord = reinterpret_cast<int32_t*>(base + o_ord);
+  auto* gkey = reinterpret_cast<double*>(base + o_key);
+  auto* rcost = reinterpret_cast<double*>(base + o_rc);
+  auto* list = reinterpret_cast<int32_t*>(base + o_list);
+  auto* ctl = reinterpret_cast<Ctl*>(base + o_ctl);
+  auto* flag = reinterpret_cast<uint32_t*>(base + o_flag);
+  auto* tidx = reinterpret_cast<uint32_t*>(base + o_tidx);
+  auto* sscr = reinterpret_cast<uint32_t*>(base + o_scan);
+  auto* n_trav = reinterpret_cast<uint32_t*>(base + o_ntr);
+
+  HIP_TRY(ctx, out->phr_out.alloc(size_t(out_cap) * sizeof(OutNode)));
+  OutNode* outn = reinterpret_cast<OutNode*>(out->phr_out.p);
+
+  HIP_TRY(ctx, hipMemcpyAsync(d_th, th.data(), kMaxDepth * sizeof(double), hipMemcpyHostToDevice, st));
+  fill_async<uint32_t>(st, reinterpret_cast<uint32_t*>(ctl), sizeof(Ctl) / 4, 0u);
+  flat_kernel<<<int(ceil_div(F, 256)), 256, 0, st>>>(aux->nodes.p, aux->parent.p, aux->parent.p + n,
+                                                    aux->inner_range.p, aux->run_start.p,
+                                                    aux->header.p, f);
+  initial_cut_kernel<<<1, 1024, 0, st>>>(f, th[0], max_cut, fa, fb, int32_t(fcap), pools[0],
+                                         int32_t(pool_cap), tasks[0], ctl, aux->header.p);
+  Ctl hc;
+  rc = read_ctl(ctx, ctl, &hc);
+  if (rc) return rc;
+  if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: initial cut exceeded its buffers");
+  out->phr_initial_cut = hc.cut;
+  if (hc.cut <= 1) return clone_lbvh(ctx, aux, out);  // the whole LBVH is attached as is
+
+  const Caps caps{pool_cap, out_cap, list_cap, task_cap};
+  uint32_t n_tasks = 1;
+  int cur = 0;
+  int levels = 0;
+  while (n_tasks > 0) {
+    reset_level_kernel<<<1, 1, 0, st>>>(ctl);
+    task_kernel<<<int(n_tasks), 64, 0, st>>>(f, d_th, tasks[cur], pools[cur], pools[cur ^ 1],
+                                            tasks[cur ^ 1], ord, gkey, rcost, outn, list, ctl,
+                                            caps);
+    rc = read_ctl(ctx, ctl, &hc);
+    if (rc) return rc;
+    if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x at level %d", hc.err, levels);
+    n_tasks = hc.n_tasks;
+    cur ^= 1;
+    ++levels;
+  }
+  const int32_t n_out = int32_t(hc.out);
+  out->phr_out_count = n_out;
+  out->phr_levels = levels;
+
+  branch_flag_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag);
+  scan::exclusive_u32(st, flag, tidx, n_out, n_trav, sscr);
+  uint32_t h_ntrav = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&h_ntrav, n_trav, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  HIP_TRY(ctx, out->nodes.alloc(std::max<uint32_t>(1, h_ntrav)));
+  HIP_TRY(ctx, out->perm.alloc(n));
+  HIP_TRY(ctx, out->phr_src.alloc(n));
+  HIP_TRY(ctx, out->tris.alloc(9 * n));
+  HIP_TRY(ctx, out->prim_box.alloc(6 * n));
+  HIP_TRY(ctx, out->header.alloc(1));
+  trav_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, out->nodes.p);
+  leaf_copy_kernel<<<n_out, 64, 0, st>>>(outn, list, f, aux->perm.p, aux->tris.p, aux->prim_box.p,
+                                        out->perm.p, out->phr_src.p, out->tris.p, out->prim_box.p,
+                                        int32_t(n), ctl);
+  phr_header_kernel<<<1, 1, 0, st>>>(outn, aux->header.p, ctl, n_trav, int32_t(n), out->header.p);
+  HIP_TRY(ctx, hipGetLastError());
+  rc = read_ctl(ctx, ctl, &hc);
+  if (rc) return rc;
+  if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: layout error 0x%x", hc.err);
+  out->host_depth = int32_t(hc.max_level);
+  return TRACE_OK;
+}
+
+}  // namespace phr

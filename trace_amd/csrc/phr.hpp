@@ -18,9 +18,8 @@ struct OutNode {
   int32_t prim_off;
   int32_t prim_cnt;
   int32_t level;     // depth below the root
-  int32_t pad[3];
   float box[6];
-  int32_t pad2[2];
+  int32_t pad;
 };
 static_assert(sizeof(OutNode) == 64, "OutNode is 64 bytes");
 

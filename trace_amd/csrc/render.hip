@@ -22,7 +22,6 @@
 namespace render {
 
 constexpr int kThreads = 128;
-constexpr int kStack = 64;
 constexpr int64_t kMaxBatchPaths = int64_t(1) << 22;  // ~4M paths per batch
 
 struct Cam {
@@ -118,6 +117,7 @@ __device__ inline float3 miss_color(int miss, float dx, float dy, float dz) {
   }
 }
 
+template <int STACK>
 __global__ __launch_bounds__(kThreads) void bounce_kernel(
     const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kThreads) void bounce_kernel(
     int32_t nn = 0, nt = 0;
     bool overflow = false, bad = false;
     traverse::Hit h =
-        traverse::closest_hit_one<kStack, false>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
+        traverse::closest_hit_one<STACK, false>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
     if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
     if (bad) atomicOr(d_err, TRACE_ERR_TREE);
     traverse::refine_hit_fp64(tris, ray, h);
@@ -275,6 +275,8 @@ int set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
 
 int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
            const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last) {
+  if (bvh->host_depth > 255)
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "render: tree depth %d exceeds 255", bvh->host_depth);
   const int64_t wh = int64_t(prm->width) * prm->height;
   const int64_t spb = std::max<int64_t>(1, std::min<int64_t>(prm->spp, kMaxBatchPaths / wh));
   const int64_t max_paths = spb * wh;
@@ -315,10 +317,17 @@ int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const t
     for (int32_t k = 0; k <= prm->max_depth + 1; ++k) {
       const int in = k & 1, out = in ^ 1;
       fill_async<uint32_t>(st, counts + out, 1, 0u);
-      bounce_kernel<<<int(ceil_div(npaths, kThreads)), kThreads, 0, st>>>(
-          bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, mesh->nrm.p, mesh->mat_idx.p,
-          mesh->mats.p, k, prm->max_depth, prm->miss, prm->seed, path_base, ro, rd, thr, rad,
-          q[in], q[out], counts + in, counts + out, ctx->d_err);
+      const int grid = int(ceil_div(npaths, kThreads));
+      if (bvh->host_depth > 63)
+        bounce_kernel<256><<<grid, kThreads, 0, st>>>(
+            bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, mesh->nrm.p, mesh->mat_idx.p,
+            mesh->mats.p, k, prm->max_depth, prm->miss, prm->seed, path_base, ro, rd, thr, rad,
+            q[in], q[out], counts + in, counts + out, ctx->d_err);
+      else
+        bounce_kernel<64><<<grid, kThreads, 0, st>>>(
+            bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, mesh->nrm.p, mesh->mat_idx.p,
+            mesh->mats.p, k, prm->max_depth, prm->miss, prm->seed, path_base, ro, rd, thr, rad,
+            q[in], q[out], counts + in, counts + out, ctx->d_err);
     }
     const bool last_batch = s0 + nsamp >= prm->spp;
     accumulate_kernel<<<int(ceil_div(wh, 256)), 256, 0, st>>>(rad, wh, int32_t(nsamp), s0 == 0 ? 1 : 0,

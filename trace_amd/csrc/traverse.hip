@@ -4,9 +4,9 @@
 namespace traverse {
 
 constexpr int kThreads = 128;
-constexpr int kStack = 64;  // LBVH depth <= 63 (one level per key bit)
 
-template <bool COUNT>
+// STACK 64 covers every LBVH (one level per key bit); deeper PHR trees use 256.
+template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kThreads) void closest_hit_kernel(
     const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
@@ -21,7 +21,7 @@ __global__ __launch_bounds__(kThreads) void closest_hit_kernel(
   Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   int32_t nn = 0, nt = 0;
   bool overflow = false, bad = false;
-  Hit h = closest_hit_one<kStack, COUNT>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
+  Hit h = closest_hit_one<STACK, COUNT>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
   refine_hit_fp64(tris, ray, h);
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);
@@ -36,15 +36,30 @@ __global__ __launch_bounds__(kThreads) void closest_hit_kernel(
 int closest_hit(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays, int64_t n,
                 int32_t* d_hit, float* d_t, int32_t* d_nodes, int32_t* d_tris) {
   if (n == 0) return TRACE_OK;
+  if (bvh->host_depth > 255)
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "closest hit: tree depth %d exceeds 255", bvh->host_depth);
   const int64_t blocks = ceil_div(n, kThreads);
+  const auto* rays = reinterpret_cast<const float4*>(d_rays);
+  hipStream_t st = ctx->stream;
+  const bool deep = bvh->host_depth > 63;
   if (d_nodes && d_tris) {
-    closest_hit_kernel<true><<<int(blocks), kThreads, 0, ctx->stream>>>(
-        bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p,
-        reinterpret_cast<const float4*>(d_rays), n, d_hit, d_t, d_nodes, d_tris, ctx->d_err);
+    if (deep)
+      closest_hit_kernel<256, true><<<int(blocks), kThreads, 0, st>>>(
+          bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, rays, n, d_hit, d_t, d_nodes,
+          d_tris, ctx->d_err);
+    else
+      closest_hit_kernel<64, true><<<int(blocks), kThreads, 0, st>>>(
+          bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, rays, n, d_hit, d_t, d_nodes,
+          d_tris, ctx->d_err);
   } else {
-    closest_hit_kernel<false><<<int(blocks), kThreads, 0, ctx->stream>>>(
-        bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p,
-        reinterpret_cast<const float4*>(d_rays), n, d_hit, d_t, nullptr, nullptr, ctx->d_err);
+    if (deep)
+      closest_hit_kernel<256, false><<<int(blocks), kThreads, 0, st>>>(
+          bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, rays, n, d_hit, d_t, nullptr,
+          nullptr, ctx->d_err);
+    else
+      closest_hit_kernel<64, false><<<int(blocks), kThreads, 0, st>>>(
+          bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, rays, n, d_hit, d_t, nullptr,
+          nullptr, ctx->d_err);
   }
   HIP_TRY(ctx, hipGetLastError());
   return TRACE_OK;

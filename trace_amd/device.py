@@ -117,6 +117,21 @@ class Bvh:
         b._enc = enc
         return b
 
+    @classmethod
+    def phr(cls, ctx: Context, aux: "Bvh", alpha: float = 0.5, delta: float = 6.0, branching: int = 2,
+            max_cut: int = 2500) -> "Bvh":
+        """PhrBuilder.BuildFromAuxilary on an LBVH (trace_phr_build)."""
+        h = C.c_void_p()
+        ctx.check(ctx.lib.trace_phr_build(ctx.h, aux.h, alpha, delta, branching, max_cut, C.byref(h)))
+        b = cls(ctx, h, aux.n_prims)
+        b._aux = aux
+        return b
+
+    def phr_source(self) -> np.ndarray:
+        src = np.empty(self.n_prims, np.int32)
+        self.ctx.check(self.ctx.lib.trace_phr_export_source(self.ctx.h, self.h, _ptr(src)))
+        return src
+
     def rebuild(self, mesh: Mesh, enclosing=None, key_mode: int = _lib.KEY_REF63) -> None:
         """Asynchronous rebuild into this handle (no device allocation once sized)."""
         enc = None if enclosing is None else np.ascontiguousarray(enclosing, dtype=np.float64)
