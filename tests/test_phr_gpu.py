@@ -33,8 +33,11 @@ def _compare(ctx, oracle, pos, alpha=0.5, delta=6.0, max_cut=2500):
     assert phr.cost() == pytest.approx(ophr.cost(), rel=1e-12)
     assert phr.size() == rec.shape[0]
     # every triangle is in exactly one leaf and perm is a bijection
-    _, perm = phr.sorted()
+    perm = phr.perm()
     assert np.array_equal(np.sort(perm), np.arange(len(pos)))
+    # the PHR order maps back to the LBVH order through the source positions
+    _, lperm = aux.sorted()
+    assert np.array_equal(perm, lperm[src])
     return mesh, aux, phr, otree, ophr
 
 

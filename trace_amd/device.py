@@ -161,6 +161,12 @@ class Bvh:
         self.ctx.check(self.ctx.lib.trace_bvh_export_sorted(self.ctx.h, self.h, _ptr(keys), _ptr(perm)))
         return keys, perm
 
+    def perm(self) -> np.ndarray:
+        """Triangle index of each position in the tree's triangle order."""
+        perm = np.empty(self.n_prims, np.int32)
+        self.ctx.check(self.ctx.lib.trace_bvh_export_sorted(self.ctx.h, self.h, None, _ptr(perm)))
+        return perm
+
     def canonical(self):
         n = C.c_int64()
         self.ctx.check(self.ctx.lib.trace_bvh_export_canonical(self.ctx.h, self.h, 0, None, None, C.byref(n)))
