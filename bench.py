@@ -3,12 +3,15 @@
 Workload (the build + traverse microbenchmark): 1,000,000 triangles with
 centroids ~U[0,1]^3 and vertex offsets of length ~Exp(mean 0.005) (seed 1);
 8,000,000 incoherent rays (origins ~U[0,1]^3, directions uniform on the
-sphere, seed 2) and 4096x2048 coherent pinhole rays.
+sphere, seed 2) and 4096x2048 coherent pinhole rays.  Pipeline: LBVH
+(63-bit keys) -> PHR refinement (alpha 0.5, delta 6, branching 2) -> closest
+hit, as the reference's Scene.Compile does.
 
-A step is one closest-hit launch over the 8M incoherent rays, with rays, BVH
-and outputs resident in HBM.  `value` = incoherent Mrays/s.  The line also
-carries the LBVH build time (63-bit and 30-bit keys), coherent Mrays/s, the
-roofline of the traversal kernel and the FP64 CPU oracle timed on a sample.
+A step is one closest-hit launch over the 8M incoherent rays on the PHR tree,
+with rays, BVH and outputs resident in HBM.  `value` = incoherent Mrays/s.
+The line also carries closest hit on the LBVH, coherent rays, the LBVH build
+(63-bit and 30-bit keys) and PHR build times, the roofline of the traversal
+kernel and the FP64 CPU oracle timed on a sample.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -35,7 +38,7 @@ def _events(torch, n):
 
 def time_launches(torch, fn, steps, warmup):
     """Per-launch device times (ms) from events on the current stream, and the
-    wall time of the whole timed region."""
+    wall time (ms) of the whole timed region."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -52,12 +55,13 @@ def time_launches(torch, fn, steps, warmup):
 
 
 def cpu_baseline(pos, rays, budget_s=12.0):
-    """FP64 oracle (recursive reference-order traversal) on a bounded sample
-    of the same rays, on the host's cores."""
+    """FP64 oracle: the reference's recursive traversal in stored child order on
+    the same PHR tree (Scene.Compile's default builder), on a bounded sample of
+    the same rays, using the host's cores."""
     from oracle import pyoracle
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    tree = pyoracle.Tree(pos)
+    tree = pyoracle.PhrTree(pyoracle.Tree(pos))
     probe = 20_000
     t0 = time.perf_counter()
     tree.closest_hit(rays[:probe], threads)
@@ -71,8 +75,9 @@ def cpu_baseline(pos, rays, budget_s=12.0):
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {n} of the 8M incoherent rays on the oracle's LBVH of the same 1M triangles "
-                  f"(FP64 recursive traversal in reference order, {threads} threads, {dt:.1f} s)",
+        "sample": f"first {n} of the 8M incoherent rays, FP64 restatement of BVH.intersected (recursive, "
+                  f"stored child order) on the oracle's PHR tree of the same 1M triangles, "
+                  f"{threads} threads, {dt:.1f} s",
     }
 
 
@@ -92,7 +97,7 @@ def main():
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--rays", type=int, default=8_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--quick", action="store_true", help="skip coherent rays and build timing")
+    ap.add_argument("--quick", action="store_true", help="skip the extra measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -114,24 +119,27 @@ def main():
     torch.cuda.set_stream(stream)
     ctx.set_stream(stream.cuda_stream)
     mesh = Mesh(ctx, pos)
-    bvh = Bvh.lbvh(ctx, mesh, key_mode=KEY_REF63)
+    lbvh = Bvh.lbvh(ctx, mesh, key_mode=KEY_REF63)
+    t0 = time.perf_counter()
+    phr = Bvh.phr(ctx, lbvh)
+    phr_ms = (time.perf_counter() - t0) * 1e3
 
     d_rays = torch.from_numpy(rays).cuda()
     n = rays.shape[0]
     d_hit = torch.empty(n, dtype=torch.int32, device="cuda")
     d_t = torch.empty(n, dtype=torch.float32, device="cuda")
 
-    def trace_inc():
-        bvh.closest_hit_device(d_rays.data_ptr(), n, d_hit.data_ptr(), d_t.data_ptr())
+    def tracer(bvh, dr, cnt):
+        return lambda: bvh.closest_hit_device(dr.data_ptr(), cnt, d_hit.data_ptr(), d_t.data_ptr())
 
-    per, wall = time_launches(torch, trace_inc, args.steps, args.warmup)
+    per, wall = time_launches(torch, tracer(phr, d_rays, n), args.steps, args.warmup)
     ctx.synchronize()
     ms_per_step = wall / args.steps
     value = n / (ms_per_step * 1e-3) / 1e6
     launch_ms = float(per.mean())
 
     # algorithmic bytes per launch from a counting-mode run of the same traversal
-    nodes, tris = bvh.counts(rays)
+    nodes, tris = phr.counts(rays)
     hit = d_hit.cpu().numpy()
     bytes_launch = (n * (32 + 8) + 4 * int((hit >= 0).sum()) + 64 * int(nodes.astype(np.int64).sum())
                     + 36 * int(tris.astype(np.int64).sum()))
@@ -151,7 +159,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "build+traverse microbench: 1M random triangles (seed 1), 8M incoherent rays "
-                        "(seed 2), closest hit on the 63-bit-key LBVH",
+                        "(seed 2), LBVH (63-bit keys) -> PHR (0.5, 6, 2) -> closest hit",
             "tris": args.tris,
             "rays": n,
             "tmin": 0.001,
@@ -170,20 +178,25 @@ def main():
             "tris_per_ray": round(float(tris.mean()), 2),
         },
         "hit_fraction": round(float((hit >= 0).mean()), 4),
+        "phr_build_ms": round(phr_ms, 2),
     }
 
     if not args.quick:
+        per_l, _ = time_launches(torch, tracer(lbvh, d_rays, n), max(5, args.steps // 2), 2)
+        out["lbvh_incoherent_mrays_s"] = round(n / (float(per_l.mean()) * 1e-3) / 1e6, 2)
         coh = scenes.coherent_rays_cfg2(4096, 2048)
         d_coh = torch.from_numpy(coh).cuda()
         m = coh.shape[0]
         d_hit2 = torch.empty(m, dtype=torch.int32, device="cuda")
         d_t2 = torch.empty(m, dtype=torch.float32, device="cuda")
 
-        def trace_coh():
-            bvh.closest_hit_device(d_coh.data_ptr(), m, d_hit2.data_ptr(), d_t2.data_ptr())
+        def trace_coh(bvh):
+            return lambda: bvh.closest_hit_device(d_coh.data_ptr(), m, d_hit2.data_ptr(), d_t2.data_ptr())
 
-        per_c, _ = time_launches(torch, trace_coh, max(5, args.steps // 2), 2)
+        per_c, _ = time_launches(torch, trace_coh(phr), max(5, args.steps // 2), 2)
         out["coherent_mrays_s"] = round(m / (float(per_c.mean()) * 1e-3) / 1e6, 2)
+        per_c, _ = time_launches(torch, trace_coh(lbvh), max(5, args.steps // 2), 2)
+        out["lbvh_coherent_mrays_s"] = round(m / (float(per_c.mean()) * 1e-3) / 1e6, 2)
         builds = {}
         for name, mode in (("ref63", KEY_REF63), ("m30", KEY_M30)):
             b = Bvh.lbvh(ctx, mesh, key_mode=mode)

@@ -749,6 +749,48 @@ void oracle_phr_free(oracle_phr* t) { delete t; }
 
 double oracle_phr_cost(const oracle_phr* t) { return phr_cost(*t, *t->aux, t->root); }
 
+}  // extern "C"
+
+namespace {
+// bvh.go:179-199 over the PHR tree: node box first, then children in stored
+// order (reused LBVH subtrees walk the LBVH), leaf triangles in list order.
+bool phr_node_hit(const PhrTree& t, int32_t id, const Ray& r, double tmin, double tmax, HitRec& h) {
+  const PNode& p = t.nodes[id];
+  if (p.kind == 2) return node_hit(*t.aux, p.aux, r, tmin, tmax, h);
+  if (!box_hit(p.box, r, tmin, tmax)) return false;
+  bool did = false;
+  if (p.kind == 1) {
+    for (int32_t pos : p.prims) {
+      const int32_t prim = t.aux->perm[pos];
+      if (tri_hit(&t.aux->pos[9 * size_t(prim)], r, tmin, h.t, h, prim)) did = true;
+    }
+  } else {
+    for (int32_t k : p.kids)
+      if (phr_node_hit(t, k, r, tmin, h.t, h)) did = true;
+  }
+  return did;
+}
+}  // namespace
+
+extern "C" {
+
+// BVH.intersected on the PHR tree for a batch of rays.
+void oracle_phr_closest_hit(const oracle_phr* t, const float* rays, int64_t nrays, int32_t* hit,
+                            double* tout, int threads) {
+  parallel_for(nrays, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const float* rr = rays + 8 * i;
+      const Ray r = make_ray(rr);
+      HitRec h{};
+      h.t = rr[7];
+      h.prim = -1;
+      phr_node_hit(*t, t->root, r, rr[3], rr[7], h);
+      hit[i] = h.prim;
+      tout[i] = h.t;
+    }
+  });
+}
+
 int64_t oracle_phr_split_work(const oracle_phr* t) { return t->split_work; }
 int64_t oracle_phr_initial_cut(const oracle_phr* t) { return t->initial_cut; }
 int64_t oracle_phr_max_split(const oracle_phr* t) { return t->max_split; }

@@ -51,6 +51,7 @@ def load() -> C.CDLL:
         "oracle_phr_split_work": ([_P], C.c_int64),
         "oracle_phr_initial_cut": ([_P], C.c_int64),
         "oracle_phr_canonical": ([_P, _P, _P, _P, _P], C.c_int64),
+        "oracle_phr_closest_hit": ([_P, _P, C.c_int64, _P, _P, C.c_int], None),
         "oracle_render": ([_P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                            C.c_uint64, C.c_int64, C.c_int, _P, _P, _P, _P], None),
     }
@@ -164,6 +165,14 @@ class PhrTree:
 
     def initial_cut(self) -> int:
         return int(self.lib.oracle_phr_initial_cut(self.h))
+
+    def closest_hit(self, rays: np.ndarray, threads: int | None = None):
+        r = np.ascontiguousarray(rays, np.float32)
+        n = r.shape[0]
+        hit = np.empty(n, np.int32)
+        t = np.empty(n, np.float64)
+        self.lib.oracle_phr_closest_hit(self.h, _p(r), n, _p(hit), _p(t), threads or _threads())
+        return hit, t
 
     def canonical(self):
         """(rec (n,4): kind, prim count or -1, n_children, 0; box (n,6) FP64;

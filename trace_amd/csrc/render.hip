@@ -127,6 +127,7 @@ __global__ __launch_bounds__(kThreads) void bounce_kernel(
     float4* __restrict__ thr, float4* __restrict__ rad, const int32_t* __restrict__ q_in,
     int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
     uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err) {
+  __shared__ int32_t s_stack[traverse::kLdsStack * kThreads];
   const uint32_t count = *count_in;
   const uint32_t i = blockIdx.x * uint32_t(kThreads) + threadIdx.x;
   bool push = false;
@@ -139,8 +140,8 @@ __global__ __launch_bounds__(kThreads) void bounce_kernel(
     traverse::Ray ray{o4.x, o4.y, o4.z, k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
     int32_t nn = 0, nt = 0;
     bool overflow = false, bad = false;
-    traverse::Hit h =
-        traverse::closest_hit_one<STACK, false>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
+    traverse::Hit h = traverse::closest_hit_one<STACK, false>(
+        nodes, tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, kThreads);
     if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
     if (bad) atomicOr(d_err, TRACE_ERR_TREE);
     traverse::refine_hit_fp64(tris, ray, h);

@@ -13,6 +13,7 @@ __global__ __launch_bounds__(kThreads) void closest_hit_kernel(
     const float4* __restrict__ rays, int64_t n, int32_t* __restrict__ hit_out,
     float* __restrict__ t_out, int32_t* __restrict__ cnt_nodes, int32_t* __restrict__ cnt_tris,
     uint32_t* d_err) {
+  __shared__ int32_t s_stack[kLdsStack * kThreads];
   const int64_t i = blockIdx.x * int64_t(kThreads) + threadIdx.x;
   if (i >= n) return;
   const BvhHeader hdr = *hdr_p;
@@ -21,7 +22,8 @@ __global__ __launch_bounds__(kThreads) void closest_hit_kernel(
   Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   int32_t nn = 0, nt = 0;
   bool overflow = false, bad = false;
-  Hit h = closest_hit_one<STACK, COUNT>(nodes, tris, hdr, ray, nn, nt, overflow, bad);
+  Hit h = closest_hit_one<STACK, COUNT>(nodes, tris, hdr, ray, nn, nt, overflow, bad,
+                                        s_stack + threadIdx.x, kThreads);
   refine_hit_fp64(tris, ray, h);
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);

@@ -132,13 +132,34 @@ __device__ inline void leaf_test(const RayPre& r, const float* __restrict__ tris
   }
 }
 
+// Traversal stack: the top kLdsStack entries of each lane live in LDS, laid
+// out [entry][lane] so a wave's pushes and pops hit distinct banks; deeper
+// entries (rare: only paths with more than kLdsStack deferred siblings) spill
+// to a per-lane private array.
+constexpr int kLdsStack = 16;
+
+template <int STACK>
+struct TravStack {
+  int32_t* lds;  // this lane's column: entry e at lds[e * stride]
+  int stride;
+  int32_t spill[STACK > kLdsStack ? STACK - kLdsStack : 1];
+  __device__ inline void put(int sp, int32_t v) {
+    if (sp < kLdsStack) lds[sp * stride] = v;
+    else spill[sp - kLdsStack] = v;
+  }
+  __device__ inline int32_t get(int sp) const {
+    return sp < kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+  }
+};
+
 // `overflow` reports a full stack, `bad` a reference outside the arrays or a
 // walk longer than any valid tree allows (every node is visited at most once).
+// `lds_col` points at this lane's column of a kLdsStack x stride LDS array.
 template <int STACK, bool COUNT>
 __device__ inline Hit closest_hit_one(const BvhNode* __restrict__ nodes,
                                       const float* __restrict__ tris, const BvhHeader& hdr,
                                       const Ray& ray, int32_t& n_nodes, int32_t& n_tris,
-                                      bool& overflow, bool& bad) {
+                                      bool& overflow, bool& bad, int32_t* lds_col, int stride) {
   const RayPre r = ray_pre(ray);
   Hit h;
   h.t = ray.tmax;
@@ -154,7 +175,9 @@ __device__ inline Hit closest_hit_one(const BvhNode* __restrict__ nodes,
     leaf_test<COUNT>(r, tris, hdr.root_ref, hdr.root_cnt, hdr.n_prims, h, n_tris, bad);
     return h;
   }
-  int32_t stack[STACK];
+  TravStack<STACK> stack;
+  stack.lds = lds_col;
+  stack.stride = stride;
   int sp = 0;
   int32_t node = hdr.root_ref;
   const uint32_t n_inner = uint32_t(hdr.n_inner);
@@ -173,25 +196,25 @@ __device__ inline Hit closest_hit_one(const BvhNode* __restrict__ nodes,
     if (h0 && q3.z > 0) {
       leaf_test<COUNT>(r, tris, q3.x, q3.z, hdr.n_prims, h, n_tris, bad);
       h0 = false;
+      h1 = h1 && t1 <= h.t * kSlabSlack;
     }
     if (h1 && q3.w > 0) {
       leaf_test<COUNT>(r, tris, q3.y, q3.w, hdr.n_prims, h, n_tris, bad);
       h1 = false;
+      h0 = h0 && t0 <= h.t * kSlabSlack;
     }
     if (h0 && h1) {
       const bool first0 = t0 <= t1;
-      const int32_t nearer = first0 ? q3.x : q3.y;
-      const int32_t farther = first0 ? q3.y : q3.x;
-      if (sp < STACK) stack[sp++] = farther;
+      if (sp < STACK) stack.put(sp++, first0 ? q3.y : q3.x);
       else overflow = true;
-      node = nearer;
+      node = first0 ? q3.x : q3.y;
     } else if (h0) {
       node = q3.x;
     } else if (h1) {
       node = q3.y;
     } else {
       if (sp == 0) break;
-      node = stack[--sp];
+      node = stack.get(--sp);
     }
   }
   return h;
