@@ -72,8 +72,20 @@ int trace_ctx_synchronize(trace_ctx* ctx);
 int trace_mesh_create(trace_ctx* ctx, const double* h_pos, int64_t ntri, trace_mesh** out);
 /* Same from device memory (9 doubles per triangle), asynchronous. */
 int trace_mesh_create_device(trace_ctx* ctx, const double* d_pos, int64_t ntri, trace_mesh** out);
+/* Many meshes in one buffer: segment s holds triangles [offsets[s], offsets[s+1]),
+ * offsets[0] = 0, offsets[nseg] = ntri, no empty segment.  trace_lbvh_build on such a
+ * mesh builds one LBVH per segment in a single pass (a forest): each tree equals the
+ * LBVH (lbvh.go:13-25) of its mesh alone.  No reference counterpart: the reference
+ * builds one scene at a time. */
+int trace_mesh_create_segmented(trace_ctx* ctx, const double* h_pos, int64_t ntri,
+                                const int64_t* h_seg_offsets, int32_t nseg, trace_mesh** out);
+/* A handle on tree `seg` of a forest.  It aliases the forest's memory (destroy it with
+ * trace_bvh_destroy before the forest); triangle indices it reports are local to the
+ * segment.  Closest hit, export, Cost and render take views; a forest itself is queried
+ * through them. */
+int trace_bvh_segment(trace_ctx* ctx, const trace_bvh* forest, int32_t seg, trace_bvh** out);
 /* Replace the positions of an existing mesh (same ntri); used for per-frame rebuilds
- * (cmd/dynamic/main.go:113-117). */
+ * (cmd/dynamic/main.go:113-117).  Geometric shading normals follow the new positions. */
 int trace_mesh_update(trace_ctx* ctx, trace_mesh* mesh, const double* h_pos);
 int trace_mesh_destroy(trace_mesh* mesh);
 
@@ -101,7 +113,7 @@ typedef struct {
   double root_box[6];
   int64_t phr_initial_cut; /* PHR: size of the initial cut (0 for an LBVH) */
   int32_t phr_levels;      /* PHR: job levels processed */
-  int32_t pad2;
+  int32_t n_segments;      /* trees in a forest (1 for a single tree or a segment view) */
 } trace_bvh_info;
 int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out);
 
@@ -188,6 +200,60 @@ int trace_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
 int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
                         const trace_camera* cam, const trace_render_params* prm, float* d_sum,
                         float* d_sumsq, float* d_last);
+
+/* ---- batching, graphs, tuning (no reference counterpart) ------------------- */
+/* nscene images in one wavefront: scene c traces tree h_seg_ids[c] of a forest (built
+ * from mesh, a segmented mesh) with camera h_cams[c] and the Philox stream of
+ * prm->seed + c, so it equals a trace_render of that segment's view with seed + c.
+ * Device outputs hold nscene * W * H * 3 floats each, scene-major.  Asynchronous. */
+int trace_render_batched(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* forest,
+                         int32_t nscene, const int32_t* h_seg_ids, const trace_camera* h_cams,
+                         const trace_render_params* prm, float* d_sum, float* d_sumsq,
+                         float* d_last);
+
+/* Per-frame vertex animation on the device: mesh positions = rest positions moved by
+ * amplitude * sin(2*pi*frame/period + k.p) along a unit direction u (k, u drawn from
+ * seed; a vertex's displacement depends only on its rest position, so shared vertices
+ * stay shared).  frame is read from *d_frame and then incremented, so a captured graph
+ * advances one frame per launch.  Geometric shading normals (trace_mesh_set_shading
+ * without normals) follow the new positions; caller normals are kept.  The per-frame
+ * rebuild of cmd/dynamic/main.go:113-117. */
+int trace_mesh_animate(trace_ctx* ctx, trace_mesh* mesh, const trace_mesh* rest,
+                       double amplitude, double period, uint64_t seed, uint32_t* d_frame);
+/* Copy a mesh's current positions (9 doubles per triangle) to the host. */
+int trace_mesh_export(trace_ctx* ctx, const trace_mesh* mesh, double* h_pos);
+
+/* HIP graph capture of the context stream: asynchronous calls issued between begin
+ * and end (trace_mesh_animate, trace_lbvh_rebuild, trace_closest_hit_device,
+ * trace_render_device) are recorded once and replayed by trace_graph_launch.  Device
+ * buffers must already be sized (run the work once eagerly first); no call that
+ * synchronises may be issued while capturing.  While a graph is alive the context's
+ * scratch arena cannot grow (a call that needs more fails with TRACE_E_INVALID);
+ * destroy graphs before their context. */
+typedef struct trace_graph trace_graph;
+int trace_graph_begin(trace_ctx* ctx);
+int trace_graph_end(trace_ctx* ctx, trace_graph** out);
+int trace_graph_launch(trace_ctx* ctx, const trace_graph* graph);
+int trace_graph_destroy(trace_graph* graph);
+
+/* Launch parameters chosen by measurement on the device at hand. */
+typedef struct {
+  int32_t trav_threads;   /* closest-hit workgroup size: 64, 128 or 256 */
+  int32_t render_threads; /* path-segment workgroup size: 64, 128 or 256 */
+  int64_t render_batch;   /* paths per wavefront batch */
+  int32_t tuned;          /* 1 once an autotune call has run */
+  int32_t pad;
+} trace_tune;
+int trace_get_tune(trace_ctx* ctx, trace_tune* out);
+int trace_set_tune(trace_ctx* ctx, const trace_tune* tune);
+/* Times closest hit on d_rays (device, n rays) for each workgroup size, keeps the
+ * fastest in the context, and reports the per-candidate times in ms (3 entries). */
+int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays,
+                             int64_t n, float* h_ms);
+/* Times a render for each workgroup size and wavefront batch size (candidate batches
+ * 2^20, 2^21, 2^22, 2^23 paths) and keeps the fastest; h_ms receives 3 + 4 times. */
+int trace_autotune_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                          const trace_camera* cam, const trace_render_params* prm, float* h_ms);
 
 #ifdef __cplusplus
 }

@@ -49,7 +49,7 @@ class BvhInfo(C.Structure):
         ("root_box", C.c_double * 6),
         ("phr_initial_cut", C.c_int64),
         ("phr_levels", C.c_int32),
-        ("pad2", C.c_int32),
+        ("n_segments", C.c_int32),
     ]
 
 
@@ -81,7 +81,30 @@ SIGNATURES = {
     "trace_mesh_set_shading": [_P, _P, _P, _P, _P, _I32],
     "trace_render": [_P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_device": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "trace_mesh_create_segmented": [_P, _P, _I64, _P, _I32, C.POINTER(_P)],
+    "trace_bvh_segment": [_P, _P, _I32, C.POINTER(_P)],
+    "trace_render_batched": [_P, _P, _P, _I32, _P, _P, _P, _P, _P, _P],
+    "trace_mesh_animate": [_P, _P, _P, C.c_double, C.c_double, C.c_uint64, _P],
+    "trace_mesh_export": [_P, _P, _P],
+    "trace_graph_begin": [_P],
+    "trace_graph_end": [_P, C.POINTER(_P)],
+    "trace_graph_launch": [_P, _P],
+    "trace_graph_destroy": [_P],
+    "trace_get_tune": [_P, _P],
+    "trace_set_tune": [_P, _P],
+    "trace_autotune_traversal": [_P, _P, _P, _I64, _P],
+    "trace_autotune_render": [_P, _P, _P, _P, _P, _P],
 }
+
+
+class Tune(C.Structure):
+    _fields_ = [
+        ("trav_threads", C.c_int32),
+        ("render_threads", C.c_int32),
+        ("render_batch", C.c_int64),
+        ("tuned", C.c_int32),
+        ("pad", C.c_int32),
+    ]
 
 
 class Camera(C.Structure):

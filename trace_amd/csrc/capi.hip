@@ -18,6 +18,10 @@ bool valid_key_mode(int m) { return m == TRACE_KEY_REF63 || m == TRACE_KEY_M30; 
 
 int ctx_reserve_scratch(trace_ctx* ctx, size_t bytes) {
   if (bytes <= ctx->scratch_bytes) return TRACE_OK;
+  if (ctx->capturing || ctx->live_graphs > 0)
+    TRACE_FAIL(ctx, TRACE_E_INVALID,
+               "scratch arena must grow to %zu bytes while %s; run the work once before capturing",
+               bytes, ctx->capturing ? "a graph capture is open" : "graphs recorded from this context are alive");
   if (ctx->scratch) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx, hipFree(ctx->scratch));
@@ -126,11 +130,63 @@ int trace_mesh_create_device(trace_ctx* ctx, const double* d_pos, int64_t ntri, 
   return mesh_create_impl(ctx, d_pos, ntri, out, hipMemcpyDeviceToDevice);
 }
 
+int trace_mesh_create_segmented(trace_ctx* ctx, const double* h_pos, int64_t ntri,
+                                const int64_t* h_seg_offsets, int32_t nseg, trace_mesh** out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!h_seg_offsets || nseg < 1) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_create_segmented: need nseg >= 1 offsets");
+  if (h_seg_offsets[0] != 0 || h_seg_offsets[nseg] != ntri)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_create_segmented: offsets must run from 0 to ntri");
+  for (int32_t s = 0; s < nseg; ++s)
+    if (h_seg_offsets[s + 1] <= h_seg_offsets[s])
+      TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_create_segmented: segment %d is empty or offsets decrease", s);
+  int rc = mesh_create_impl(ctx, h_pos, ntri, out, hipMemcpyHostToDevice);
+  if (rc) return rc;
+  trace_mesh* m = *out;
+  m->nseg = nseg;
+  m->seg_off_h.assign(h_seg_offsets, h_seg_offsets + nseg + 1);
+  std::vector<int32_t> tseg(ntri);
+  for (int32_t s = 0; s < nseg; ++s)
+    for (int64_t i = h_seg_offsets[s]; i < h_seg_offsets[s + 1]; ++i) tseg[i] = s;
+  hipError_t e = m->tri_seg.alloc(ntri);
+  if (e == hipSuccess) e = m->seg_off.alloc(nseg + 1);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->tri_seg.p, tseg.data(), ntri * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(m->seg_off.p, h_seg_offsets, (nseg + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    delete m;
+    *out = nullptr;
+    TRACE_FAIL(ctx, TRACE_E_HIP, "trace_mesh_create_segmented: %s", hipGetErrorString(e));
+  }
+  return TRACE_OK;
+}
+
+int trace_bvh_segment(trace_ctx* ctx, const trace_bvh* forest, int32_t seg, trace_bvh** out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!forest || !out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_segment: null argument");
+  if (forest->base) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_segment: already a segment view");
+  if (forest->nseg < 2) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_segment: not a forest");
+  if (seg < 0 || seg >= forest->nseg) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_segment: segment %d of %d", seg, forest->nseg);
+  trace_bvh* v = new (std::nothrow) trace_bvh();
+  if (!v) TRACE_FAIL(ctx, TRACE_E_OOM, "trace_bvh_segment: host allocation failed");
+  v->base = forest;
+  v->seg = seg;
+  v->n_prims = forest->seg_off_h[seg + 1] - forest->seg_off_h[seg];
+  v->key_mode = forest->key_mode;
+  v->arity = 2;
+  v->host_depth = forest->host_depth;
+  *out = v;
+  return TRACE_OK;
+}
+
 int trace_mesh_update(trace_ctx* ctx, trace_mesh* mesh, const double* h_pos) {
   if (!ctx) return TRACE_E_INVALID;
   if (!mesh || !h_pos) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_update: null argument");
   HIP_TRY(ctx, hipMemcpyAsync(mesh->pos.p, h_pos, 9 * mesh->ntri * sizeof(double),
                               hipMemcpyHostToDevice, ctx->stream));
+  if (mesh->has_shading && mesh->geo_normals) {
+    const int rc = render::refresh_geo_normals(ctx, mesh);
+    if (rc) return rc;
+  }
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return TRACE_OK;
 }
@@ -173,7 +229,9 @@ int trace_lbvh_rebuild(trace_ctx* ctx, const trace_mesh* mesh, const double* h_e
   if (!ctx) return TRACE_E_INVALID;
   if (!mesh || !bvh) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_rebuild: null argument");
   if (!valid_key_mode(key_mode)) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_rebuild: unknown key mode %d", key_mode);
-  if (bvh->key_mode < 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_rebuild: not an LBVH handle");
+  if (bvh->key_mode < 0 || bvh->base) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_rebuild: not an LBVH handle");
+  if (mesh->ntri != bvh->n_prims || mesh->nseg != bvh->nseg)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_lbvh_rebuild: mesh shape differs from the one the tree was built for");
   return lbvh::build(ctx, mesh, h_enclosing, key_mode, bvh);
 }
 
@@ -184,7 +242,7 @@ int trace_bvh_destroy(trace_bvh* bvh) {
 }
 
 static int read_header(trace_ctx* ctx, const trace_bvh* bvh, BvhHeader* h) {
-  HIP_TRY(ctx, hipMemcpyAsync(h, bvh->header.p, sizeof(BvhHeader), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(h, bvh->header_p(), sizeof(BvhHeader), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return TRACE_OK;
 }
@@ -198,7 +256,10 @@ int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out
   std::memset(out, 0, sizeof(*out));
   out->n_prims = bvh->n_prims;
   out->n_leaves = h.n_leaves;
-  out->n_inner = h.root_cnt > 0 ? 0 : h.n_inner;
+  // a view's header holds forest totals as bounds; its own tree has one inner
+  // node fewer than leaves
+  out->n_inner = h.root_cnt > 0 ? 0 : (bvh->base ? h.n_leaves - 1 : h.n_inner);
+  out->n_segments = bvh->base ? 1 : bvh->nseg;
   out->depth = h.depth;
   out->key_mode = bvh->key_mode;
   out->arity = bvh->arity;
@@ -261,6 +322,7 @@ int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double d
   if (!aux || !out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: null argument");
   *out = nullptr;
   if (aux->key_mode < 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: the auxiliary tree must be an LBVH");
+  if (aux->base || aux->nseg > 1) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: forests and segment views are not refined by this build");
   if (branching != 2) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: branching factor %d (this build supports 2)", branching);
   if (!(alpha >= 0) || !std::isfinite(delta) || max_cut < 1)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: need alpha >= 0, finite delta, max_cut >= 1");
@@ -289,13 +351,18 @@ int trace_phr_export_source(trace_ctx* ctx, const trace_bvh* phr, int32_t* h_src
 int trace_bvh_export_sorted(trace_ctx* ctx, const trace_bvh* bvh, uint64_t* h_keys, int32_t* h_perm) {
   if (!ctx) return TRACE_E_INVALID;
   if (!bvh) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_sorted: null bvh");
+  // a segment view's triangles occupy its segment's range of the forest order
+  const int64_t off = bvh->base ? bvh->base->seg_off_h[bvh->seg] : 0;
+  const uint64_t* keys = bvh->base ? bvh->base->keys.p : bvh->keys.p;
   if (h_keys) {
-    if (!bvh->keys.p) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_sorted: tree has no Morton keys");
-    HIP_TRY(ctx, hipMemcpyAsync(h_keys, bvh->keys.p, bvh->n_prims * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (!keys) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_sorted: tree has no Morton keys");
+    HIP_TRY(ctx, hipMemcpyAsync(h_keys, keys + off, bvh->n_prims * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
   }
   if (h_perm)
-    HIP_TRY(ctx, hipMemcpyAsync(h_perm, bvh->perm.p, bvh->n_prims * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h_perm, bvh->perm_p() + off, bvh->n_prims * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (h_perm && off)  // triangle indices local to the segment
+    for (int64_t i = 0; i < bvh->n_prims; ++i) h_perm[i] -= int32_t(off);
   return TRACE_OK;
 }
 
@@ -311,7 +378,7 @@ static int fetch_tree(trace_ctx* ctx, const trace_bvh* bvh, HostTree* t) {
   const int64_t ninner = t->h.root_cnt > 0 ? 0 : t->h.n_inner;
   t->nodes.resize(ninner);
   if (ninner > 0) {
-    HIP_TRY(ctx, hipMemcpyAsync(t->nodes.data(), bvh->nodes.p, ninner * sizeof(BvhNode), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(t->nodes.data(), bvh->nodes_p(), ninner * sizeof(BvhNode), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   }
   return TRACE_OK;
@@ -329,16 +396,17 @@ int trace_bvh_export_canonical(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap
   if (!ctx) return TRACE_E_INVALID;
   if (!bvh || !n_out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_canonical: null argument");
   if (bvh->key_mode < 0 && !bvh->phr_clone) return export_phr(ctx, bvh, cap, h_rec, h_box, n_out);
+  if (!bvh->base && bvh->nseg > 1)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_canonical: a forest is exported through its segment views");
   HostTree t;
   int rc = fetch_tree(ctx, bvh, &t);
   if (rc) return rc;
-  const int64_t total = t.h.root_cnt > 0 ? 1 : 2 * int64_t(t.h.n_inner) + 1;
-  *n_out = total;
-  if (!h_rec || !h_box) return TRACE_OK;  // size query
-  if (cap < total) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_canonical: capacity %lld < %lld nodes", (long long)cap, (long long)total);
-  // Preorder with children[0] first.  Each emitted record's [first, last]
-  // is filled once its subtree is complete (post-visit), so keep a second
-  // stack of open inner records.
+  // leaf ranges of a segment view are reported relative to its segment
+  const int32_t off = bvh->base ? int32_t(bvh->base->seg_off_h[bvh->seg]) : 0;
+  // Preorder with children[0] first, collected first (the node count of a
+  // view is only known by walking it), inner ranges filled afterwards.
+  std::vector<int32_t> rec;
+  std::vector<float> box;
   std::vector<Item> stack;
   Item root;
   root.ref = t.h.root_ref;
@@ -349,20 +417,21 @@ int trace_bvh_export_canonical(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap
   }
   stack.push_back(root);
   int64_t k = 0;
-  // first/last of inner nodes = first of leftmost leaf, last of rightmost leaf
-  std::vector<int64_t> open;  // record index of inner nodes awaiting their last leaf
-  std::vector<int32_t> pending_children;
   while (!stack.empty()) {
     Item it = stack.back();
     stack.pop_back();
-    int32_t* r = h_rec + 4 * k;
-    std::memcpy(h_box + 6 * k, it.box, sizeof(it.box));
+    rec.resize(4 * (k + 1));
+    box.resize(6 * (k + 1));
+    int32_t* r = rec.data() + 4 * k;
+    std::memcpy(box.data() + 6 * k, it.box, sizeof(it.box));
     if (it.cnt > 0) {
       r[0] = 1;
-      r[1] = it.ref;
-      r[2] = it.ref + it.cnt - 1;
+      r[1] = it.ref - off;
+      r[2] = it.ref + it.cnt - 1 - off;
       r[3] = 0;
     } else {
+      if (it.ref < 0 || it.ref >= int32_t(t.nodes.size()))
+        TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_bvh_export_canonical: node %d out of range", it.ref);
       const BvhNode& nd = t.nodes[it.ref];
       r[0] = 0;
       r[1] = -1;
@@ -384,6 +453,12 @@ int trace_bvh_export_canonical(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap
     }
     ++k;
   }
+  const int64_t total = k;
+  *n_out = total;
+  if (!h_rec || !h_box) return TRACE_OK;  // size query
+  if (cap < total) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_export_canonical: capacity %lld < %lld nodes", (long long)cap, (long long)total);
+  std::memcpy(h_rec, rec.data(), rec.size() * sizeof(int32_t));
+  std::memcpy(h_box, box.data(), box.size() * sizeof(float));
   // Fill inner ranges bottom-up: in preorder, an inner node's range is from the
   // first leaf after it to the last leaf of its subtree.  Walk backwards with a
   // stack of subtree spans.
@@ -539,7 +614,9 @@ static int check_render_args(trace_ctx* ctx, const trace_mesh* mesh, const trace
                              const char* what) {
   if (!mesh || !bvh || !cam || !prm) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: null argument", what);
   if (!mesh->has_shading) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: mesh has no shading data (trace_mesh_set_shading)", what);
-  if (bvh->n_prims != mesh->ntri) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: bvh has %lld triangles, mesh %lld", what, (long long)bvh->n_prims, (long long)mesh->ntri);
+  if (!bvh->base && bvh->nseg > 1) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: render a forest through its segment views", what);
+  const int64_t tree_tris = bvh->base ? bvh->base->n_prims : bvh->n_prims;
+  if (tree_tris != mesh->ntri) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: bvh has %lld triangles, mesh %lld", what, (long long)tree_tris, (long long)mesh->ntri);
   if (prm->width < 2 || prm->height < 2 || prm->spp < 0 || prm->max_depth < 0 || prm->sample_offset < 0)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: need width, height >= 2 (samples divide by W-1, H-1), spp, max_depth >= 0", what);
   if (prm->miss < TRACE_MISS_DEFAULT || prm->miss > TRACE_MISS_SUN) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: unknown miss shader %d", what, prm->miss);

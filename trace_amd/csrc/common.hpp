@@ -15,7 +15,16 @@
 // ---------------------------------------------------------------------------
 // host-side error plumbing
 // ---------------------------------------------------------------------------
+// Launch parameters the autotuner may change (defaults are the untuned choice).
+struct TuneState {
+  int trav_threads = 128;               // closest-hit workgroup size: 64, 128 or 256
+  int render_threads = 128;             // path-segment workgroup size: 64, 128 or 256
+  int64_t render_batch = int64_t(1) << 22;  // paths per wavefront batch
+  bool tuned = false;
+};
+
 struct trace_ctx {
+  TuneState tune;
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
@@ -24,6 +33,10 @@ struct trace_ctx {
   // grow-only scratch arena, reused across builds (never shrinks)
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // HIP graphs recorded from this context hold scratch addresses: while any is
+  // alive (or a capture is open) the arena must not move
+  int live_graphs = 0;
+  bool capturing = false;
 };
 
 #define TRACE_FAIL(ctx, code, ...)                                \
@@ -93,6 +106,12 @@ struct trace_mesh {
   DevArray<float4> mats;      // (kind, r, g, b) per material
   int32_t nmat = 0;
   bool has_shading = false;
+  bool geo_normals = false;  // nrm derived from pos (recomputed when pos animates)
+  // segmented meshes (many meshes in one buffer, built as a forest in one pass)
+  int32_t nseg = 1;
+  std::vector<int64_t> seg_off_h;   // nseg + 1 triangle offsets
+  DevArray<int64_t> seg_off;        // same, on the device
+  DevArray<int32_t> tri_seg;        // segment of each triangle
 };
 
 // Inner node of the binary traversal tree: both child boxes in one 64-byte
@@ -153,14 +172,19 @@ struct trace_bvh {
   // upper bound on inner nodes along any root-to-leaf path (sizes the
   // traversal stack); an LBVH over 64-bit keys never exceeds 63
   int32_t host_depth = 63;
-  // host mirror of the n-ary structure for PHR trees (empty for LBVH)
-  std::vector<int32_t> nary_first;   // per node: leaf -> first prim pos, inner -> -1
-  std::vector<int32_t> nary_count;   // per node: leaf -> prim count, inner -> #children
-  std::vector<int32_t> nary_child;   // per inner node: offset into nary_children
-  std::vector<int32_t> nary_children;
-  std::vector<double> nary_box;      // 6 per node, FP64
-  int32_t nary_root = -1;
-  std::vector<int32_t> leaf_prims;   // PHR leaves list prims explicitly (sorted positions)
+  // forests (built from a segmented mesh): one header per segment
+  int32_t nseg = 1;
+  DevArray<BvhHeader> seg_headers;
+  std::vector<int64_t> seg_off_h;
+  // a segment view aliases a forest's arrays and reads its segment's header
+  const trace_bvh* base = nullptr;
+  int32_t seg = -1;
+
+  const BvhNode* nodes_p() const { return base ? base->nodes.p : nodes.p; }
+  const float* tris_p() const { return base ? base->tris.p : tris.p; }
+  const int32_t* perm_p() const { return base ? base->perm.p : perm.p; }
+  const BvhHeader* header_p() const { return base ? base->seg_headers.p + seg : header.p; }
+  int32_t depth_bound() const { return base ? base->host_depth : host_depth; }
 };
 
 // ---------------------------------------------------------------------------

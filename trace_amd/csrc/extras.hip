@@ -1,0 +1,285 @@
+// Batched rendering, on-device animation, HIP-graph capture and the
+// autotuner (C-ABI entry points of include/trace_hip.h that have no reference
+// counterpart).
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "philox.hpp"
+#include "render.hpp"
+#include "traverse.hpp"
+
+struct trace_graph {
+  trace_ctx* ctx = nullptr;  // its scratch addresses are baked into the graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// p' = p + amplitude * sin(2*pi*frame/period + k.p) * u for every vertex; the
+// frame number is read from device memory so a captured graph advances it.
+__global__ void animate_kernel(const double* __restrict__ rest, double* __restrict__ pos,
+                               int64_t nvert, double amplitude, double period, double kx,
+                               double ky, double kz, double ux, double uy, double uz,
+                               const uint32_t* __restrict__ d_frame) {
+  const int64_t v = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (v >= nvert) return;
+  const double f = double(*d_frame);
+  const double* p = rest + 3 * v;
+  const double w = amplitude * sin(6.283185307179586 * f / period + (kx * p[0] + ky * p[1] + kz * p[2]));
+  pos[3 * v + 0] = p[0] + w * ux;
+  pos[3 * v + 1] = p[1] + w * uy;
+  pos[3 * v + 2] = p[2] + w * uz;
+}
+
+__global__ void frame_step_kernel(uint32_t* d_frame) { *d_frame += 1u; }
+
+void unit_from(double a, double b, double out[3]) {
+  const double z = 1.0 - 2.0 * a;
+  const double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+  const double phi = 6.283185307179586 * b;
+  out[0] = r * std::cos(phi);
+  out[1] = r * std::sin(phi);
+  out[2] = z;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trace_render_batched(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* forest,
+                         int32_t nscene, const int32_t* h_seg_ids, const trace_camera* h_cams,
+                         const trace_render_params* prm, float* d_sum, float* d_sumsq,
+                         float* d_last) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !forest || !h_seg_ids || !h_cams || !prm || !d_sum || !d_sumsq || nscene < 1)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_batched: null argument or no scene");
+  if (forest->base || forest->nseg < 2)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_batched: needs a forest (a segmented mesh's LBVH)");
+  if (!mesh->has_shading) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_batched: mesh has no shading data");
+  if (mesh->ntri != forest->n_prims) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_batched: mesh does not match the forest");
+  if (prm->width < 2 || prm->height < 2 || prm->spp < 0 || prm->max_depth < 0 || prm->sample_offset < 0 ||
+      prm->miss < TRACE_MISS_DEFAULT || prm->miss > TRACE_MISS_SUN)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_batched: bad render parameters");
+  std::vector<trace_bvh> views(nscene);
+  std::vector<const trace_bvh*> trees(nscene);
+  for (int32_t c = 0; c < nscene; ++c) {
+    const int32_t s = h_seg_ids[c];
+    if (s < 0 || s >= forest->nseg) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_batched: scene %d names segment %d", c, s);
+    views[c].base = forest;
+    views[c].seg = s;
+    views[c].host_depth = forest->host_depth;
+    trees[c] = &views[c];
+  }
+  if (prm->spp == 0) return TRACE_OK;
+  return render::render_scenes(ctx, mesh, trees.data(), nscene, h_cams, prm, d_sum, d_sumsq, d_last);
+}
+
+int trace_mesh_animate(trace_ctx* ctx, trace_mesh* mesh, const trace_mesh* rest, double amplitude,
+                       double period, uint64_t seed, uint32_t* d_frame) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !rest || !d_frame) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_animate: null argument");
+  if (mesh->ntri != rest->ntri) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_animate: meshes differ in size");
+  if (!(period > 0)) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_animate: period must be positive");
+  const Philox4 r = philox4x32_10(0u, 0u, 0x616e696du, 0u, uint32_t(seed), uint32_t(seed >> 32));
+  double u[3], kd[3];
+  unit_from(double(u01(r.v[0])), double(u01(r.v[1])), u);
+  unit_from(double(u01(r.v[2])), double(u01(r.v[3])), kd);
+  const double kmag = 6.283185307179586 * 3.0;  // three waves across a unit length
+  const int64_t nvert = 3 * mesh->ntri;
+  animate_kernel<<<int(ceil_div(nvert, 256)), 256, 0, ctx->stream>>>(
+      rest->pos.p, mesh->pos.p, nvert, amplitude, period, kmag * kd[0], kmag * kd[1], kmag * kd[2],
+      u[0], u[1], u[2], d_frame);
+  frame_step_kernel<<<1, 1, 0, ctx->stream>>>(d_frame);
+  HIP_TRY(ctx, hipGetLastError());
+  if (mesh->has_shading && mesh->geo_normals) return render::refresh_geo_normals(ctx, mesh);
+  return TRACE_OK;
+}
+
+int trace_mesh_export(trace_ctx* ctx, const trace_mesh* mesh, double* h_pos) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !h_pos) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_export: null argument");
+  HIP_TRY(ctx, hipMemcpyAsync(h_pos, mesh->pos.p, 9 * mesh->ntri * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return TRACE_OK;
+}
+
+int trace_graph_begin(trace_ctx* ctx) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (ctx->stream == nullptr) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_graph_begin: the legacy stream cannot be captured");
+  if (ctx->capturing) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_graph_begin: a capture is already open");
+  HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+  ctx->capturing = true;
+  return TRACE_OK;
+}
+
+int trace_graph_end(trace_ctx* ctx, trace_graph** out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_graph_end: null output");
+  *out = nullptr;
+  if (!ctx->capturing) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_graph_end: no capture is open");
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+  ctx->capturing = false;
+  trace_graph* g = new (std::nothrow) trace_graph();
+  if (!g) {
+    if (graph) (void)hipGraphDestroy(graph);
+    TRACE_FAIL(ctx, TRACE_E_OOM, "trace_graph_end: host allocation failed");
+  }
+  g->graph = graph;
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    TRACE_FAIL(ctx, TRACE_E_HIP, "trace_graph_end: %s", hipGetErrorString(e));
+  }
+  g->ctx = ctx;
+  ctx->live_graphs += 1;
+  *out = g;
+  return TRACE_OK;
+}
+
+int trace_graph_launch(trace_ctx* ctx, const trace_graph* graph) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!graph) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_graph_launch: null graph");
+  HIP_TRY(ctx, hipGraphLaunch(graph->exec, ctx->stream));
+  return TRACE_OK;
+}
+
+int trace_graph_destroy(trace_graph* graph) {
+  if (!graph) return TRACE_E_INVALID;
+  if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+  if (graph->graph) (void)hipGraphDestroy(graph->graph);
+  if (graph->ctx) graph->ctx->live_graphs -= 1;
+  delete graph;
+  return TRACE_OK;
+}
+
+int trace_get_tune(trace_ctx* ctx, trace_tune* out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_get_tune: null output");
+  std::memset(out, 0, sizeof(*out));
+  out->trav_threads = ctx->tune.trav_threads;
+  out->render_threads = ctx->tune.render_threads;
+  out->render_batch = ctx->tune.render_batch;
+  out->tuned = ctx->tune.tuned ? 1 : 0;
+  return TRACE_OK;
+}
+
+int trace_set_tune(trace_ctx* ctx, const trace_tune* t) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!t) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_set_tune: null argument");
+  auto ok_threads = [](int v) { return v == 64 || v == 128 || v == 256; };
+  if (!ok_threads(t->trav_threads) || !ok_threads(t->render_threads) || t->render_batch < 1)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_set_tune: workgroup sizes must be 64, 128 or 256");
+  ctx->tune.trav_threads = t->trav_threads;
+  ctx->tune.render_threads = t->render_threads;
+  ctx->tune.render_batch = t->render_batch;
+  return TRACE_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Mean device time (ms) of `reps` runs of fn on the context stream, after one
+// untimed run.
+template <typename F>
+int time_ms(trace_ctx* ctx, F fn, int reps, float* ms) {
+  hipEvent_t a, b;
+  HIP_TRY(ctx, hipEventCreate(&a));
+  HIP_TRY(ctx, hipEventCreate(&b));
+  int rc = fn();
+  if (rc == TRACE_OK) {
+    HIP_TRY(ctx, hipEventRecord(a, ctx->stream));
+    for (int i = 0; i < reps && rc == TRACE_OK; ++i) rc = fn();
+    HIP_TRY(ctx, hipEventRecord(b, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(b));
+    float t = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&t, a, b));
+    *ms = t / reps;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays, int64_t n,
+                             float* h_ms) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!bvh || !d_rays || n <= 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_autotune_traversal: bad arguments");
+  DevArray<char> out;
+  HIP_TRY(ctx, out.alloc(size_t(n) * 8));
+  int32_t* d_hit = reinterpret_cast<int32_t*>(out.p);
+  float* d_t = reinterpret_cast<float*>(out.p + size_t(n) * 4);
+  const int cand[3] = {64, 128, 256};
+  int best = ctx->tune.trav_threads;
+  float best_ms = INFINITY;
+  for (int c = 0; c < 3; ++c) {
+    ctx->tune.trav_threads = cand[c];
+    float ms = 0;
+    int rc = time_ms(ctx, [&] { return traverse::closest_hit(ctx, bvh, d_rays, n, d_hit, d_t, nullptr, nullptr); }, 3, &ms);
+    if (rc) return rc;
+    if (h_ms) h_ms[c] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cand[c];
+    }
+  }
+  ctx->tune.trav_threads = best;
+  ctx->tune.tuned = true;
+  return ctx_check_device_error(ctx, "trace_autotune_traversal");
+}
+
+int trace_autotune_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                          const trace_camera* cam, const trace_render_params* prm, float* h_ms) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !bvh || !cam || !prm || prm->spp < 1) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_autotune_render: bad arguments");
+  const size_t px = size_t(prm->width) * prm->height * 3;
+  DevArray<float> out;
+  HIP_TRY(ctx, out.alloc(3 * px));
+  auto run = [&] {
+    return trace_render_device(ctx, mesh, bvh, cam, prm, out.p, out.p + px, out.p + 2 * px);
+  };
+  const int cand[3] = {64, 128, 256};
+  int best = ctx->tune.render_threads;
+  float best_ms = INFINITY;
+  for (int c = 0; c < 3; ++c) {
+    ctx->tune.render_threads = cand[c];
+    float ms = 0;
+    int rc = time_ms(ctx, run, 2, &ms);
+    if (rc) return rc;
+    if (h_ms) h_ms[c] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cand[c];
+    }
+  }
+  ctx->tune.render_threads = best;
+  const int64_t batches[4] = {int64_t(1) << 20, int64_t(1) << 21, int64_t(1) << 22, int64_t(1) << 23};
+  int64_t best_b = ctx->tune.render_batch;
+  best_ms = INFINITY;
+  for (int c = 0; c < 4; ++c) {
+    ctx->tune.render_batch = batches[c];
+    float ms = 0;
+    int rc = time_ms(ctx, run, 2, &ms);
+    if (rc) return rc;
+    if (h_ms) h_ms[3 + c] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_b = batches[c];
+    }
+  }
+  ctx->tune.render_batch = best_b;
+  ctx->tune.tuned = true;
+  return ctx_check_device_error(ctx, "trace_autotune_render");
+}
+
+}  // extern "C"

@@ -140,6 +140,91 @@ __global__ void header_kernel(const unsigned long long* __restrict__ partial, in
   }
 }
 
+// ---- forests: one scene box per segment ---------------------------------------
+__global__ void seg_enc_init_kernel(unsigned long long* enc, int32_t nseg) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * nseg) enc[i] = (i % 6) < 3 ? ~0ull : 0ull;
+}
+
+constexpr int kSegTile = 4096;
+
+// Each workgroup covers kSegTile consecutive triangles.  Segments are
+// contiguous, so a tile lies in one segment unless it straddles a boundary:
+// such tiles (at most nseg - 1 of them) fall back to per-triangle atomics.
+__global__ __launch_bounds__(256) void seg_enclosing_kernel(const double* __restrict__ pos,
+                                                            int64_t n,
+                                                            const int32_t* __restrict__ tseg,
+                                                            unsigned long long* enc) {
+  __shared__ uint64_t s_part[6][4];
+  const int64_t t0 = blockIdx.x * int64_t(kSegTile);
+  const int64_t t1 = t0 + kSegTile < n ? t0 + kSegTile : n;
+  const int32_t s0 = tseg[t0];
+  const bool uniform = s0 == tseg[t1 - 1];
+  uint64_t lo[3] = {~0ull, ~0ull, ~0ull};
+  uint64_t hi[3] = {0ull, 0ull, 0ull};
+  for (int64_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) {
+    const double* p = pos + 9 * i;
+    for (int a = 0; a < 3; ++a) {
+      const uint64_t mn = ord_of(min3(p[a], p[3 + a], p[6 + a]));
+      const uint64_t mx = ord_of(max3(p[a], p[3 + a], p[6 + a]));
+      if (uniform) {
+        lo[a] = mn < lo[a] ? mn : lo[a];
+        hi[a] = mx > hi[a] ? mx : hi[a];
+      } else {
+        atomicMin(&enc[6 * size_t(tseg[i]) + a], (unsigned long long)mn);
+        atomicMax(&enc[6 * size_t(tseg[i]) + 3 + a], (unsigned long long)mx);
+      }
+    }
+  }
+  if (!uniform) return;  // uniform is the same for the whole workgroup
+  for (int a = 0; a < 3; ++a) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t l = __shfl_xor(lo[a], o, 64);
+      const uint64_t h = __shfl_xor(hi[a], o, 64);
+      lo[a] = l < lo[a] ? l : lo[a];
+      hi[a] = h > hi[a] ? h : hi[a];
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    for (int a = 0; a < 3; ++a) {
+      s_part[a][w] = lo[a];
+      s_part[3 + a][w] = hi[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    uint64_t v = s_part[a][0];
+    for (int k = 1; k < 4; ++k)
+      v = a < 3 ? (s_part[a][k] < v ? s_part[a][k] : v) : (s_part[a][k] > v ? s_part[a][k] : v);
+    if (a < 3) atomicMin(&enc[6 * size_t(s0) + a], (unsigned long long)v);
+    else atomicMax(&enc[6 * size_t(s0) + a], (unsigned long long)v);
+  }
+}
+
+// Per segment: FP64 scene box, outward FP32 root box, quantisation box.
+__global__ void seg_header_kernel(const unsigned long long* __restrict__ enc, int32_t nseg,
+                                  BvhHeader* __restrict__ seg_hdr, double* __restrict__ qbox,
+                                  BvhHeader* __restrict__ hdr, int32_t n) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) hdr->n_prims = n;
+  if (s >= nseg) return;
+  BvhHeader& h = seg_hdr[s];
+  for (int a = 0; a < 3; ++a) {
+    const double lo = double_of(enc[6 * size_t(s) + a]);
+    const double hi = double_of(enc[6 * size_t(s) + 3 + a]);
+    h.enc_lo[a] = lo;
+    h.enc_hi[a] = hi;
+    h.root_lo[a] = f32_down(lo);
+    h.root_hi[a] = f32_up(hi);
+    qbox[6 * size_t(s) + a] = lo;
+    qbox[6 * size_t(s) + 3 + a] = hi;
+  }
+  h.n_prims = n;
+  h.depth = 0;
+}
+
 __device__ inline uint64_t split_by3(uint64_t a) {
   // Spread the low 21 bits of a so that bit k lands on bit 3k.
   uint64_t x = a & 0x1fffffull;
@@ -165,12 +250,15 @@ __device__ inline uint64_t quantise(double enc_lo, double extent, double c) {
   return (uint64_t)f;
 }
 
+// tseg (segmented meshes): each triangle quantises in its own segment's box.
 __global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ pos, int64_t n,
-                                                     const double* __restrict__ qbox, int key_mode,
+                                                     const double* __restrict__ qbox_all,
+                                                     const int32_t* __restrict__ tseg, int key_mode,
                                                      uint64_t* __restrict__ keys) {
 #pragma clang fp contract(off)
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
+  const double* qbox = tseg ? qbox_all + 6 * size_t(tseg[i]) : qbox_all;
   const double* p = pos + 9 * i;
   uint64_t q[3];
 #pragma unroll
@@ -212,14 +300,25 @@ __device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t* s_wave, uint
   return off + x - v;
 }
 
+// Sorted position i starts a new leaf when its key differs from its
+// predecessor's or (segmented meshes) its triangle belongs to another segment.
+__device__ inline bool is_head(const uint64_t* __restrict__ keys, int64_t i,
+                               const int32_t* __restrict__ tseg, const uint32_t* __restrict__ perm) {
+  if (i == 0 || keys[i] != keys[i - 1]) return true;
+  return tseg && tseg[perm[i]] != tseg[perm[i - 1]];
+}
+
 __global__ __launch_bounds__(kUThreads) void count_heads_kernel(const uint64_t* __restrict__ keys,
-                                                                int64_t n, uint32_t* blk_cnt) {
+                                                                int64_t n,
+                                                                const int32_t* __restrict__ tseg,
+                                                                const uint32_t* __restrict__ perm,
+                                                                uint32_t* blk_cnt) {
   __shared__ uint32_t s_wave[4];
   const int64_t base = blockIdx.x * int64_t(kUTile) + threadIdx.x * int64_t(kUPerThread);
   uint32_t c = 0;
   for (int j = 0; j < kUPerThread; ++j) {
     const int64_t i = base + j;
-    if (i < n) c += (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+    if (i < n) c += is_head(keys, i, tseg, perm) ? 1u : 0u;
   }
   uint32_t tot;
   block_excl_scan256(c, s_wave, &tot);
@@ -248,19 +347,22 @@ __global__ __launch_bounds__(256) void scan_blocks_kernel(uint32_t* blk, int64_t
 
 __global__ __launch_bounds__(kUThreads) void write_unique_kernel(
     const uint64_t* __restrict__ keys, int64_t n, const uint32_t* __restrict__ blk_off,
-    uint64_t* __restrict__ ukeys, uint32_t* __restrict__ run_start, BvhHeader* hdr) {
+    const int32_t* __restrict__ tseg, const uint32_t* __restrict__ perm,
+    uint64_t* __restrict__ ukeys, uint32_t* __restrict__ run_start, int32_t* __restrict__ useg,
+    BvhHeader* hdr) {
   __shared__ uint32_t s_wave[4];
   const int64_t base = blockIdx.x * int64_t(kUTile) + threadIdx.x * int64_t(kUPerThread);
   uint32_t heads = 0;
   for (int j = 0; j < kUPerThread; ++j) {
     const int64_t i = base + j;
-    if (i < n && (i == 0 || keys[i] != keys[i - 1])) heads |= 1u << j;
+    if (i < n && is_head(keys, i, tseg, perm)) heads |= 1u << j;
   }
   uint32_t r = blk_off[blockIdx.x] + block_excl_scan256(__popc(heads), s_wave, nullptr);
   for (int j = 0; j < kUPerThread; ++j) {
     if (heads & (1u << j)) {
       ukeys[r] = keys[base + j];
       run_start[r] = uint32_t(base + j);
+      if (useg) useg[r] = tseg[perm[base + j]];
       ++r;
     }
   }
@@ -275,16 +377,36 @@ __device__ inline int delta_of(const uint64_t* k, int32_t u, int32_t i, int32_t 
 
 // Inner node i of the radix tree over u distinct keys (Karras 2012, Alg. 1).
 // The lower key range becomes children[0], as lbvh.go:163-176 orders them.
-__global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict__ k,
-                                                     const uint32_t* __restrict__ run_start,
+// Forests (useg != null): unique key r belongs to segment s = useg[r], whose
+// keys are [b, b + u) with b = u_begin[s]; the segment's tree is built over
+// its own keys only (delta is -1 outside them) and its inner nodes take the
+// indices b - s + [0, u - 1).
+__global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict__ k_all,
+                                                     const uint32_t* __restrict__ run_start_all,
                                                      const BvhHeader* __restrict__ hdr,
-                                                     BvhNode* __restrict__ nodes,
-                                                     int32_t* __restrict__ parent_inner,
-                                                     int32_t* __restrict__ parent_leaf,
-                                                     int32_t* __restrict__ inner_range) {
-  const int32_t u = hdr->n_leaves;
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                     const int32_t* __restrict__ useg,
+                                                     const int32_t* __restrict__ u_begin,
+                                                     BvhNode* __restrict__ nodes_all,
+                                                     int32_t* __restrict__ parent_inner_all,
+                                                     int32_t* __restrict__ parent_leaf_all,
+                                                     int32_t* __restrict__ inner_range_all) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= hdr->n_leaves) return;
+  int32_t b = 0, s_id = 0, u = hdr->n_leaves;
+  if (useg) {
+    s_id = useg[r];
+    b = u_begin[s_id];
+    u = u_begin[s_id + 1] - b;
+  }
+  const int32_t i = r - b;
   if (i >= u - 1) return;
+  const int32_t ib = b - s_id;  // first inner-node index of this tree
+  const uint64_t* k = k_all + b;
+  const uint32_t* run_start = run_start_all + b;
+  BvhNode* nodes = nodes_all + ib;
+  int32_t* parent_inner = parent_inner_all + ib;
+  int32_t* parent_leaf = parent_leaf_all + b;
+  int32_t* inner_range = inner_range_all + 2 * ib;
   const int d = (delta_of(k, u, i, i + 1) - delta_of(k, u, i, i - 1)) >= 0 ? 1 : -1;
   const int dmin = delta_of(k, u, i, i - d);
   int32_t lmax = 2;
@@ -303,30 +425,41 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
   const int32_t gamma = i + s * d + (d < 0 ? -1 : 0);
   const int32_t first = i < j ? i : j;
   const int32_t last = i < j ? j : i;
-  inner_range[2 * i] = first;
-  inner_range[2 * i + 1] = last;
+  inner_range[2 * i] = b + first;  // unique-key indices of the whole array
+  inner_range[2 * i + 1] = b + last;
 
+  const int32_t me = ib + i;  // global inner index
   BvhNode& nd = nodes[i];
   // child 0 covers [first, gamma], child 1 covers [gamma + 1, last]
   if (first == gamma) {
     nd.ref[0] = int32_t(run_start[gamma]);
     nd.cnt[0] = int32_t(run_start[gamma + 1] - run_start[gamma]);
-    parent_leaf[gamma] = (i << 1) | 0;
+    parent_leaf[gamma] = (me << 1) | 0;
   } else {
-    nd.ref[0] = gamma;
+    nd.ref[0] = ib + gamma;
     nd.cnt[0] = 0;
-    parent_inner[gamma] = (i << 1) | 0;
+    parent_inner[gamma] = (me << 1) | 0;
   }
   if (last == gamma + 1) {
     nd.ref[1] = int32_t(run_start[gamma + 1]);
     nd.cnt[1] = int32_t(run_start[gamma + 2] - run_start[gamma + 1]);
-    parent_leaf[gamma + 1] = (i << 1) | 1;
+    parent_leaf[gamma + 1] = (me << 1) | 1;
   } else {
-    nd.ref[1] = gamma + 1;
+    nd.ref[1] = ib + gamma + 1;
     nd.cnt[1] = 0;
-    parent_inner[gamma + 1] = (i << 1) | 1;
+    parent_inner[gamma + 1] = (me << 1) | 1;
   }
-  if (i == 0) parent_inner[0] = -1;
+  if (i == 0) parent_inner[0] = -1;  // the tree's root
+}
+
+// Forests: first unique key of each segment (u_begin[nseg] = total).
+__global__ void seg_begin_kernel(const int32_t* __restrict__ useg, const BvhHeader* __restrict__ hdr,
+                                 int32_t nseg, int32_t* __restrict__ u_begin) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t U = hdr->n_leaves;
+  if (r == 0) u_begin[nseg] = U;
+  if (r >= U) return;
+  if (r == 0 || useg[r] != useg[r - 1]) u_begin[useg[r]] = r;
 }
 
 // FP32 triangles (v0, v1, v2) and outward-rounded FP32 boxes, in sorted
@@ -375,20 +508,24 @@ __global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ bo
                                                     const int32_t* __restrict__ parent_leaf,
                                                     BvhNode* nodes, uint32_t* counters,
                                                     int32_t* heights, BvhHeader* hdr,
-                                                    uint32_t* d_err) {
+                                                    uint32_t* d_err, const int32_t* __restrict__ useg,
+                                                    BvhHeader* seg_hdr, int32_t nseg) {
   const int32_t u = hdr->n_leaves;
   const int32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
   if (leaf >= u) return;
-  if (u == 1) {
-    hdr->root_ref = 0;
-    hdr->root_cnt = int32_t(run_start[1]);
-    hdr->depth = 0;
-    return;
+  if (!useg) {
+    if (u == 1) {
+      hdr->root_ref = 0;
+      hdr->root_cnt = int32_t(run_start[1]);
+      hdr->depth = 0;
+      return;
+    }
+    if (leaf == 0) {
+      hdr->root_ref = 0;
+      hdr->root_cnt = 0;
+    }
   }
-  if (leaf == 0) {
-    hdr->root_ref = 0;
-    hdr->root_cnt = 0;
-  }
+  const int32_t n_inner = u - nseg;  // every tree of the forest has one leaf more than inner nodes
   float lo[3], hi[3];
   const uint32_t s = run_start[leaf], e = run_start[leaf + 1];
 #pragma unroll
@@ -410,7 +547,7 @@ __global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ bo
   for (int step = 0; link >= 0; ++step) {
     const int32_t node = link >> 1;
     const int slot = link & 1;
-    if (step > 128 || node >= u - 1) {
+    if (step > 128 || node >= n_inner) {
       atomicOr(d_err, TRACE_ERR_TREE);
       return;
     }
@@ -437,8 +574,32 @@ __global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ bo
     h = 1 + (h > ho ? h : ho);
     link = parent_inner[node];
   }
-  // reached the root (inner node 0)
-  hdr->depth = h;
+  // reached a root (inner node 0, or a forest tree's first inner node)
+  if (useg) seg_hdr[useg[leaf]].depth = h;
+  else hdr->depth = h;
+}
+
+// Forests: per-segment headers.  A segment with one distinct key is a root
+// leaf.  n_inner and n_prims hold the forest totals: kernels use them only as
+// bounds for node and triangle references.
+__global__ void seg_finalize_kernel(const int32_t* __restrict__ u_begin,
+                                    const uint32_t* __restrict__ run_start, int32_t nseg,
+                                    const BvhHeader* __restrict__ hdr, BvhHeader* __restrict__ seg_hdr) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const int32_t b = u_begin[s], e = u_begin[s + 1];
+  BvhHeader& h = seg_hdr[s];
+  h.n_leaves = e - b;
+  h.n_inner = hdr->n_leaves - nseg;
+  h.n_prims = hdr->n_prims;
+  if (e - b == 1) {
+    h.root_ref = int32_t(run_start[b]);
+    h.root_cnt = int32_t(run_start[e] - run_start[b]);
+    h.depth = 0;
+  } else {
+    h.root_ref = b - s;
+    h.root_cnt = 0;
+  }
 }
 
 static int grid_for(int64_t n, int threads) { return int(std::max<int64_t>(1, ceil_div(n, threads))); }
@@ -448,10 +609,20 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   const int64_t n = mesh->ntri;
   hipStream_t st = ctx->stream;
   const int passes = key_mode == TRACE_KEY_M30 ? 4 : 8;
+  const int32_t nseg = mesh->nseg;
+  const bool forest = nseg > 1;
+  const int seg_passes = !forest ? 0 : nseg <= 256 ? 1 : nseg <= 65536 ? 2 : 3;
+  if (forest && h_enclosing)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "LBVH: a segmented mesh takes no caller enclosing box");
+  if (nseg > (1 << 24))
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "LBVH: %d segments exceed the 2^24 the sort's segment digits cover", nseg);
 
   bvh->n_prims = n;
   bvh->key_mode = key_mode;
   bvh->arity = 2;
+  bvh->nseg = nseg;
+  bvh->seg_off_h = mesh->seg_off_h;
+  if (forest) HIP_TRY(ctx, bvh->seg_headers.alloc(nseg));
   HIP_TRY(ctx, bvh->keys.alloc(n));
   HIP_TRY(ctx, bvh->perm.alloc(n));
   HIP_TRY(ctx, bvh->tris.alloc(9 * n));
@@ -466,21 +637,26 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   // unique keys, per-tile head counts, refit counters and heights
   const int64_t utiles = ceil_div(n, kUTile);
   ScratchPlan plan;
-  const size_t o_enc = plan.take(6 * kEncBlocks * sizeof(unsigned long long));
-  const size_t o_qbox = plan.take(6 * sizeof(double));
+  const size_t o_enc = plan.take(6 * std::max<int64_t>(kEncBlocks, nseg) * sizeof(unsigned long long));
+  const size_t o_qbox = plan.take(6 * size_t(nseg) * sizeof(double));
   const size_t o_kb = plan.take(n * sizeof(uint64_t));
   const size_t o_va = plan.take(n * sizeof(uint32_t));
   const size_t o_vb = plan.take(n * sizeof(uint32_t));
-  const size_t o_st = plan.take(onesweep::status_words(n, passes) * sizeof(uint32_t));
+  const size_t o_st = plan.take(onesweep::status_words(n, passes + seg_passes) * sizeof(uint32_t));
   const size_t o_uk = plan.take(n * sizeof(uint64_t));
   const size_t o_blk = plan.take(utiles * sizeof(uint32_t));
   const size_t o_cnt = plan.take(std::max<int64_t>(1, n) * sizeof(uint32_t));
   const size_t o_hgt = plan.take(2 * std::max<int64_t>(1, n) * sizeof(int32_t));
+  const size_t o_useg = plan.take(forest ? n * sizeof(int32_t) : 0);
+  const size_t o_ub = plan.take(forest ? (size_t(nseg) + 1) * sizeof(int32_t) : 0);
   int rc = ctx_reserve_scratch(ctx, plan.off);
   if (rc) return rc;
   char* base = static_cast<char*>(ctx->scratch);
   auto* enc = reinterpret_cast<unsigned long long*>(base + o_enc);
   auto* qbox = reinterpret_cast<double*>(base + o_qbox);
+  int32_t* useg = forest ? reinterpret_cast<int32_t*>(base + o_useg) : nullptr;
+  int32_t* u_begin = forest ? reinterpret_cast<int32_t*>(base + o_ub) : nullptr;
+  const int32_t* tseg = forest ? mesh->tri_seg.p : nullptr;
   auto* keys_b = reinterpret_cast<uint64_t*>(base + o_kb);
   auto* vals_a = reinterpret_cast<uint32_t*>(base + o_va);
   auto* vals_b = reinterpret_cast<uint32_t*>(base + o_vb);
@@ -492,14 +668,22 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
 
   if (h_enclosing) HIP_TRY(ctx, hipMemcpyAsync(qbox, h_enclosing, 6 * sizeof(double),
                                                hipMemcpyHostToDevice, st));
-  const int enc_blocks = int(std::min<int64_t>(grid_for(n, 256), kEncBlocks));
-  enclosing_kernel<<<enc_blocks, 256, 0, st>>>(mesh->pos.p, n, enc);
-  header_kernel<<<1, 256, 0, st>>>(enc, enc_blocks, bvh->header.p, qbox, h_enclosing ? 1 : 0,
-                                  int32_t(n));
-  morton_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, qbox, key_mode, bvh->keys.p);
+  if (!forest) {
+    const int enc_blocks = int(std::min<int64_t>(grid_for(n, 256), kEncBlocks));
+    enclosing_kernel<<<enc_blocks, 256, 0, st>>>(mesh->pos.p, n, enc);
+    header_kernel<<<1, 256, 0, st>>>(enc, enc_blocks, bvh->header.p, qbox, h_enclosing ? 1 : 0,
+                                    int32_t(n));
+  } else {
+    seg_enc_init_kernel<<<grid_for(6 * int64_t(nseg), 256), 256, 0, st>>>(enc, nseg);
+    seg_enclosing_kernel<<<int(ceil_div(n, kSegTile)), 256, 0, st>>>(mesh->pos.p, n, tseg, enc);
+    seg_header_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(enc, nseg, bvh->seg_headers.p, qbox,
+                                                          bvh->header.p, int32_t(n));
+  }
+  morton_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, qbox, tseg, key_mode,
+                                                  bvh->keys.p);
 
   const int where = onesweep::sort_pairs(st, bvh->keys.p, vals_a, keys_b, vals_b, n, passes,
-                                         status, ctx->d_err, true);
+                                         status, ctx->d_err, true, tseg, seg_passes);
   uint64_t* sorted_keys = where ? keys_b : bvh->keys.p;
   uint32_t* sorted_vals = where ? vals_b : vals_a;
   if (where) {
@@ -510,22 +694,28 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   HIP_TRY(ctx, hipMemcpyAsync(bvh->perm.p, sorted_vals, n * sizeof(uint32_t),
                               hipMemcpyDeviceToDevice, st));
 
-  count_heads_kernel<<<int(utiles), kUThreads, 0, st>>>(sorted_keys, n, blk);
+  const uint32_t* sperm = reinterpret_cast<const uint32_t*>(bvh->perm.p);
+  count_heads_kernel<<<int(utiles), kUThreads, 0, st>>>(sorted_keys, n, tseg, sperm, blk);
   scan_blocks_kernel<<<1, 256, 0, st>>>(blk, utiles, bvh->header.p);
-  write_unique_kernel<<<int(utiles), kUThreads, 0, st>>>(sorted_keys, n, blk, ukeys,
-                                                         bvh->run_start.p, bvh->header.p);
+  write_unique_kernel<<<int(utiles), kUThreads, 0, st>>>(sorted_keys, n, blk, tseg, sperm, ukeys,
+                                                         bvh->run_start.p, useg, bvh->header.p);
+  if (forest) seg_begin_kernel<<<grid_for(n, 256), 256, 0, st>>>(useg, bvh->header.p, nseg, u_begin);
   int32_t* parent_inner = bvh->parent.p;
   int32_t* parent_leaf = bvh->parent.p + n;
-  karras_kernel<<<grid_for(n, 256), 256, 0, st>>>(ukeys, bvh->run_start.p, bvh->header.p,
-                                                  bvh->nodes.p, parent_inner, parent_leaf,
-                                                  bvh->inner_range.p);
-  reorder_kernel<<<grid_for(n, 256), 256, 0, st>>>(
-      mesh->pos.p, n, reinterpret_cast<const uint32_t*>(bvh->perm.p), bvh->tris.p,
-      bvh->prim_box.p);
+  fill_async<int32_t>(st, bvh->parent.p, size_t(2 * n), -1);
+  karras_kernel<<<grid_for(n, 256), 256, 0, st>>>(ukeys, bvh->run_start.p, bvh->header.p, useg,
+                                                  u_begin, bvh->nodes.p, parent_inner,
+                                                  parent_leaf, bvh->inner_range.p);
+  reorder_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, sperm, bvh->tris.p,
+                                                   bvh->prim_box.p);
   fill_async<uint32_t>(st, counters, size_t(n), 0u);
   refit_kernel<<<grid_for(n, 256), 256, 0, st>>>(bvh->prim_box.p, bvh->run_start.p, parent_inner,
                                                  parent_leaf, bvh->nodes.p, counters, heights,
-                                                 bvh->header.p, ctx->d_err);
+                                                 bvh->header.p, ctx->d_err, useg,
+                                                 forest ? bvh->seg_headers.p : nullptr, nseg);
+  if (forest)
+    seg_finalize_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(u_begin, bvh->run_start.p, nseg,
+                                                            bvh->header.p, bvh->seg_headers.p);
   HIP_TRY(ctx, hipGetLastError());
   return TRACE_OK;
 }

@@ -32,27 +32,42 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave) {
   return off + x - v;
 }
 
-// Digit histograms of all passes in one read of the keys.
+// Digit histograms of all passes in one read of the keys (and segment ids).
 __global__ __launch_bounds__(256) void histogram_kernel(const uint64_t* __restrict__ keys,
                                                         int64_t n, int passes,
+                                                        const int32_t* __restrict__ seg,
+                                                        int seg_passes,
                                                         uint32_t* __restrict__ hist) {
-  __shared__ uint32_t sh[8 * kRadix];
-  for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) sh[i] = 0;
+  __shared__ uint32_t sh[11 * kRadix];  // 8 key digits + up to 3 segment digits
+  const int all = passes + seg_passes;
+  for (int i = threadIdx.x; i < all * kRadix; i += blockDim.x) sh[i] = 0;
   __syncthreads();
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     const uint64_t k = keys[i];
     for (int p = 0; p < passes; ++p) atomicAdd(&sh[p * kRadix + ((k >> (8 * p)) & 255)], 1u);
+    if (seg_passes) {
+      const uint32_t s = uint32_t(seg[i]);
+      for (int p = 0; p < seg_passes; ++p)
+        atomicAdd(&sh[(passes + p) * kRadix + ((s >> (8 * p)) & 255)], 1u);
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x)
+  for (int i = threadIdx.x; i < all * kRadix; i += blockDim.x)
     if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
+// Digit of an element: from its key, or (segment passes) from seg[value].
+__device__ inline uint32_t digit_of(uint64_t key, uint32_t val, int shift,
+                                    const int32_t* __restrict__ seg) {
+  return seg ? (uint32_t(seg[val]) >> shift) & 255u : uint32_t(key >> shift) & 255u;
 }
 
 __global__ __launch_bounds__(kThreads) void onesweep_pass_kernel(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n, int shift,
-    const uint32_t* __restrict__ hist, uint32_t* status, uint32_t* ticket, uint32_t* d_err) {
+    const uint32_t* __restrict__ hist, uint32_t* status, uint32_t* ticket, uint32_t* d_err,
+    const int32_t* __restrict__ seg) {
   __shared__ uint64_t s_keys[kTile];
   __shared__ uint32_t s_vals[kTile];
   __shared__ uint32_t s_wh[kWaves * kRadix];  // per-wave digit counters, then offsets
@@ -93,7 +108,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_pass_kernel(
   for (int j = 0; j < kKeysPerThread; ++j) {
     const int64_t idx = tile_base + w * (kKeysPerThread * 64) + j * 64 + lane;
     const bool valid = idx < n;
-    const uint32_t digit = uint32_t(key[j] >> shift) & 255u;
+    const uint32_t digit = valid ? digit_of(key[j], val[j], shift, seg) : 255u;
     uint64_t m = ballot64(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -163,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_pass_kernel(
   for (int j = 0; j < kKeysPerThread; ++j) {
     const int64_t idx = tile_base + w * (kKeysPerThread * 64) + j * 64 + lane;
     if (idx < n) {
-      const uint32_t digit = uint32_t(key[j] >> shift) & 255u;
+      const uint32_t digit = digit_of(key[j], val[j], shift, seg);
       const uint32_t pos = s_local[digit] + s_wh[w * kRadix + digit] + rank[j];
       s_keys[pos] = key[j];
       s_vals[pos] = val[j];
@@ -175,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_pass_kernel(
   const int count = remaining < kTile ? int(remaining) : kTile;
   for (int i = tid; i < count; i += kThreads) {
     const uint64_t k = s_keys[i];
-    const uint32_t digit = uint32_t(k >> shift) & 255u;
+    const uint32_t digit = digit_of(k, s_vals[i], shift, seg);
     const uint32_t g = s_goff[digit] + uint32_t(i);
     if (g < uint64_t(n)) {  // always true unless a look-back wait expired
       keys_out[g] = k;
@@ -186,27 +201,30 @@ __global__ __launch_bounds__(kThreads) void onesweep_pass_kernel(
 
 int sort_pairs(hipStream_t stream, uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b,
                uint32_t* vals_b, int64_t n, int passes, uint32_t* status, uint32_t* d_err,
-               bool vals_are_iota) {
+               bool vals_are_iota, const int32_t* seg, int seg_passes) {
   if (n <= 0) return 0;
+  if (!seg) seg_passes = 0;
+  const int all = passes + seg_passes;
   const int64_t tiles = ceil_div(n, kTile);
   const size_t per_pass = size_t(tiles) * kRadix;
-  uint32_t* hist = status + size_t(passes) * per_pass;
-  uint32_t* tickets = hist + size_t(passes) * kRadix;
-  const size_t words = status_words(n, passes);
+  uint32_t* hist = status + size_t(all) * per_pass;
+  uint32_t* tickets = hist + size_t(all) * kRadix;
+  const size_t words = status_words(n, all);
   fill_async<uint32_t>(stream, status, words, 0u);
   {
     const int blocks = int(std::min<int64_t>(ceil_div(n, 256 * 8), 1024));
-    histogram_kernel<<<blocks, 256, 0, stream>>>(keys_a, n, passes, hist);
+    histogram_kernel<<<blocks, 256, 0, stream>>>(keys_a, n, passes, seg, seg_passes, hist);
   }
   uint64_t* kin = keys_a;
   uint32_t* vin = vals_are_iota ? nullptr : vals_a;
   uint64_t* kout = keys_b;
   uint32_t* vout = vals_b;
   int where = 0;
-  for (int p = 0; p < passes; ++p) {
+  for (int p = 0; p < all; ++p) {
+    const bool seg_pass = p >= passes;
     onesweep_pass_kernel<<<int(tiles), kThreads, 0, stream>>>(
-        kin, vin, kout, vout, n, 8 * p, hist + size_t(p) * kRadix, status + size_t(p) * per_pass,
-        tickets + p, d_err);
+        kin, vin, kout, vout, n, 8 * (seg_pass ? p - passes : p), hist + size_t(p) * kRadix,
+        status + size_t(p) * per_pass, tickets + p, d_err, seg_pass ? seg : nullptr);
     where ^= 1;
     if (where) {
       kin = keys_b; vin = vals_b; kout = keys_a; vout = vals_a;

@@ -36,11 +36,14 @@ inline size_t status_words(int64_t n, int passes) {
 }
 
 // Sort keys[0..n) (values: vals or iota if vals_in == nullptr) on bits
-// [0, 8*passes).  Ping-pongs between (keys_a, vals_a) and (keys_b, vals_b);
-// returns 0 if the result ends in the a-buffers, 1 if in the b-buffers.
-// `status` must hold status_words(n, passes) uint32 and is cleared here.
+// [0, 8*passes).  When `seg` is given, seg_passes more passes follow whose
+// digits are taken from seg[value] (a segment id per original element), so the
+// result is ordered by (segment, key) with ties in input order.  Ping-pongs
+// between (keys_a, vals_a) and (keys_b, vals_b); returns 0 if the result ends
+// in the a-buffers, 1 if in the b-buffers.  `status` must hold
+// status_words(n, passes + seg_passes) uint32 and is cleared here.
 int sort_pairs(hipStream_t stream, uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b,
                uint32_t* vals_b, int64_t n, int passes, uint32_t* status, uint32_t* d_err,
-               bool vals_are_iota);
+               bool vals_are_iota, const int32_t* seg = nullptr, int seg_passes = 0);
 
 }  // namespace onesweep

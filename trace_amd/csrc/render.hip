@@ -21,8 +21,6 @@
 
 namespace render {
 
-constexpr int kThreads = 128;
-constexpr int64_t kMaxBatchPaths = int64_t(1) << 22;  // ~4M paths per batch
 
 struct Cam {
   float o[3], ll[3], h[3], v[3];
@@ -53,9 +51,20 @@ __global__ void d2f_kernel(const double* __restrict__ in, size_t n, float* __res
   if (i < n) out[i] = float(in[i]);
 }
 
-__global__ __launch_bounds__(256) void camera_kernel(Cam cam, int32_t W, int32_t H,
-                                                     int64_t n_paths, int64_t path_base,
-                                                     uint64_t seed, float4* __restrict__ ro,
+// One image of a (possibly batched) render: its camera, its Philox key and
+// the header of the tree it is traced against.  Paths of a batch are ordered
+// scene-major: path p belongs to scene p / pps (pps = paths per scene).
+struct Scene {
+  Cam cam;
+  uint64_t seed;
+  const BvhHeader* hdr;
+};
+
+__global__ void scene_fill_kernel(Scene s, Scene* out) { *out = s; }
+
+__global__ __launch_bounds__(256) void camera_kernel(const Scene* __restrict__ scenes, int32_t W,
+                                                     int32_t H, int64_t n_paths, int64_t pps,
+                                                     int64_t path_base, float4* __restrict__ ro,
                                                      float4* __restrict__ rd,
                                                      float4* __restrict__ thr,
                                                      float4* __restrict__ rad,
@@ -67,10 +76,13 @@ __global__ __launch_bounds__(256) void camera_kernel(Cam cam, int32_t W, int32_t
     counts[1] = 0;
   }
   if (p >= n_paths) return;
+  const Scene& sc = scenes[p / pps];
+  const Cam& cam = sc.cam;
+  const int64_t q = p % pps;
   const int64_t wh = int64_t(W) * H;
-  const int64_t pix = p % wh;
+  const int64_t pix = q % wh;
   const int32_t x = int32_t(pix % W), y = int32_t(pix / W);
-  const Philox4 r = path_draw(seed, uint64_t(path_base + p), 0u);
+  const Philox4 r = path_draw(sc.seed, uint64_t(path_base + q), 0u);
   const float s = (float(x) + u01(r.v[0])) / float(W - 1);
   const float t = (float(y) + u01(r.v[1])) / float(H - 1);
   const float dx = cam.ll[0] + s * cam.h[0] + t * cam.v[0] - cam.o[0];
@@ -117,24 +129,28 @@ __device__ inline float3 miss_color(int miss, float dx, float dy, float dz) {
   }
 }
 
-template <int STACK>
-__global__ __launch_bounds__(kThreads) void bounce_kernel(
+template <int THREADS, int STACK>
+__global__ __launch_bounds__(THREADS) void bounce_kernel(
     const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
-    const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
+    const int32_t* __restrict__ perm, const Scene* __restrict__ scenes, int64_t pps,
     const float* __restrict__ nrm, const int32_t* __restrict__ mat_idx,
     const float4* __restrict__ mats, int32_t k, int32_t max_depth, int32_t miss,
-    uint64_t seed, int64_t path_base, float4* __restrict__ ro, float4* __restrict__ rd,
+    int64_t path_base, float4* __restrict__ ro, float4* __restrict__ rd,
     float4* __restrict__ thr, float4* __restrict__ rad, const int32_t* __restrict__ q_in,
     int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
     uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err) {
+  constexpr int kThreads = THREADS;
   __shared__ int32_t s_stack[traverse::kLdsStack * kThreads];
   const uint32_t count = *count_in;
   const uint32_t i = blockIdx.x * uint32_t(kThreads) + threadIdx.x;
   bool push = false;
   int32_t p = 0;
   if (i < count) {
-    const BvhHeader hdr = *hdr_p;
     p = q_in[i];
+    const Scene& sc = scenes[p / pps];
+    const BvhHeader hdr = *sc.hdr;
+    const uint64_t seed = sc.seed;
+    const int64_t path_id = path_base + p % pps;
     const float4 o4 = ro[p];
     const float4 d4 = rd[p];
     traverse::Ray ray{o4.x, o4.y, o4.z, k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
@@ -174,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void bounce_kernel(
           nz = -nz;
         }
         // n + uniform unit vector (util.go:15-26 draws a uniform direction)
-        const Philox4 r = path_draw(seed, uint64_t(path_base + p), uint32_t(k + 1));
+        const Philox4 r = path_draw(seed, uint64_t(path_id), uint32_t(k + 1));
         const float z = 1.0f - 2.0f * u01(r.v[0]);
         const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
         float sphi, cphi;
@@ -218,15 +234,20 @@ __global__ __launch_bounds__(kThreads) void bounce_kernel(
   }
 }
 
-__global__ void accumulate_kernel(const float4* __restrict__ rad, int64_t wh, int32_t nsamp,
-                                  int32_t first_batch, float* __restrict__ sum,
+// One thread per (scene, pixel): the batch's samples in sample order.  Output
+// is scene-major: scene c's image starts at 3 * c * wh.
+__global__ void accumulate_kernel(const float4* __restrict__ rad, int64_t wh, int32_t nscene,
+                                  int32_t nsamp, int32_t first_batch, float* __restrict__ sum,
                                   float* __restrict__ sumsq, float* __restrict__ last) {
-  const int64_t pix = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (pix >= wh) return;
+  const int64_t qx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (qx >= wh * nscene) return;
+  const int64_t scene = qx / wh, pix = qx % wh;
+  const float4* base = rad + scene * nsamp * wh + pix;
+  const int64_t px = qx;  // output index (scene-major)
   float s0 = 0, s1 = 0, s2 = 0, q0 = 0, q1 = 0, q2 = 0;
   float4 v = make_float4(0, 0, 0, 0);
   for (int32_t s = 0; s < nsamp; ++s) {
-    v = rad[int64_t(s) * wh + pix];
+    v = base[int64_t(s) * wh];
     s0 += v.x;
     s1 += v.y;
     s2 += v.z;
@@ -234,8 +255,8 @@ __global__ void accumulate_kernel(const float4* __restrict__ rad, int64_t wh, in
     q1 += v.y * v.y;
     q2 += v.z * v.z;
   }
-  float* S = sum + 3 * pix;
-  float* Q = sumsq + 3 * pix;
+  float* S = sum + 3 * px;
+  float* Q = sumsq + 3 * px;
   if (first_batch) {
     S[0] = s0; S[1] = s1; S[2] = s2;
     Q[0] = q0; Q[1] = q1; Q[2] = q2;
@@ -244,9 +265,9 @@ __global__ void accumulate_kernel(const float4* __restrict__ rad, int64_t wh, in
     Q[0] += q0; Q[1] += q1; Q[2] += q2;
   }
   if (last) {
-    last[3 * pix + 0] = v.x;
-    last[3 * pix + 1] = v.y;
-    last[3 * pix + 2] = v.z;
+    last[3 * px + 0] = v.x;
+    last[3 * px + 1] = v.y;
+    last[3 * px + 2] = v.z;
   }
 }
 
@@ -271,19 +292,56 @@ int set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
   HIP_TRY(ctx, hipStreamSynchronize(st));
   mesh->nmat = nmat;
   mesh->has_shading = true;
+  mesh->geo_normals = h_normals == nullptr;
   return TRACE_OK;
+}
+
+int refresh_geo_normals(trace_ctx* ctx, trace_mesh* mesh) {
+  const int64_t n = mesh->ntri;
+  geo_normals_kernel<<<int(ceil_div(n, 256)), 256, 0, ctx->stream>>>(mesh->pos.p, n, mesh->nrm.p);
+  HIP_TRY(ctx, hipGetLastError());
+  return TRACE_OK;
+}
+
+template <int THREADS>
+static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, int grid,
+                          const Scene* scenes, int64_t pps, int32_t k, const trace_render_params* prm,
+                          int64_t path_base, float4* ro, float4* rd, float4* thr, float4* rad,
+                          int32_t* qin, int32_t* qout, uint32_t* cin, uint32_t* cout) {
+  hipStream_t st = ctx->stream;
+  if (bvh->depth_bound() > 63)
+    bounce_kernel<THREADS, 256><<<grid, THREADS, 0, st>>>(
+        bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
+        mesh->mats.p, k, prm->max_depth, prm->miss, path_base, ro, rd, thr, rad, qin, qout, cin,
+        cout, ctx->d_err);
+  else
+    bounce_kernel<THREADS, 64><<<grid, THREADS, 0, st>>>(
+        bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
+        mesh->mats.p, k, prm->max_depth, prm->miss, path_base, ro, rd, thr, rad, qin, qout, cin,
+        cout, ctx->d_err);
 }
 
 int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
            const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last) {
-  if (bvh->host_depth > 255)
-    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "render: tree depth %d exceeds 255", bvh->host_depth);
+  const trace_bvh* one[1] = {bvh};
+  return render_scenes(ctx, mesh, one, 1, cam, prm, d_sum, d_sumsq, d_last);
+}
+
+int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
+                  int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
+                  float* d_sum, float* d_sumsq, float* d_last) {
+  const trace_bvh* bvh = trees[0];  // every tree shares bvh's node and triangle arrays
+  if (bvh->depth_bound() > 255)
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "render: tree depth %d exceeds 255", bvh->depth_bound());
   const int64_t wh = int64_t(prm->width) * prm->height;
-  const int64_t spb = std::max<int64_t>(1, std::min<int64_t>(prm->spp, kMaxBatchPaths / wh));
-  const int64_t max_paths = spb * wh;
+  const int64_t per_sample = wh * nscene;
+  const int64_t spb =
+      std::max<int64_t>(1, std::min<int64_t>(prm->spp, ctx->tune.render_batch / per_sample));
+  const int64_t max_paths = spb * per_sample;
   hipStream_t st = ctx->stream;
 
   ScratchPlan plan;
+  const size_t o_scn = plan.take(size_t(nscene) * sizeof(Scene));
   const size_t o_ro = plan.take(max_paths * sizeof(float4));
   const size_t o_rd = plan.take(max_paths * sizeof(float4));
   const size_t o_thr = plan.take(max_paths * sizeof(float4));
@@ -300,40 +358,50 @@ int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const t
   auto* rad = reinterpret_cast<float4*>(base + o_rad);
   int32_t* q[2] = {reinterpret_cast<int32_t*>(base + o_qa), reinterpret_cast<int32_t*>(base + o_qb)};
   auto* counts = reinterpret_cast<uint32_t*>(base + o_cnt);
+  auto* scenes = reinterpret_cast<Scene*>(base + o_scn);
 
-  Cam c;
-  for (int a = 0; a < 3; ++a) {
-    c.o[a] = float(cam->origin[a]);
-    c.ll[a] = float(cam->lower_left[a]);
-    c.h[a] = float(cam->horizontal[a]);
-    c.v[a] = float(cam->vertical[a]);
+  // Scene table: written with kernels (arguments by value), so a single-scene
+  // render stays capturable in a HIP graph with no host-memory copy node.
+  for (int32_t c = 0; c < nscene; ++c) {
+    Scene s;
+    const trace_camera* cam = cams + c;
+    for (int a = 0; a < 3; ++a) {
+      s.cam.o[a] = float(cam->origin[a]);
+      s.cam.ll[a] = float(cam->lower_left[a]);
+      s.cam.h[a] = float(cam->horizontal[a]);
+      s.cam.v[a] = float(cam->vertical[a]);
+    }
+    s.seed = prm->seed + uint64_t(c);  // scene c of a batch draws the stream of seed + c
+    s.hdr = trees[c]->header_p();
+    scene_fill_kernel<<<1, 1, 0, st>>>(s, scenes + c);
   }
   for (int64_t s0 = 0; s0 < prm->spp; s0 += spb) {
     const int64_t nsamp = std::min<int64_t>(spb, prm->spp - s0);
-    const int64_t npaths = nsamp * wh;
+    const int64_t pps = nsamp * wh;
+    const int64_t npaths = pps * nscene;
     const int64_t path_base = (prm->sample_offset + s0) * wh;
-    camera_kernel<<<int(ceil_div(npaths, 256)), 256, 0, st>>>(c, prm->width, prm->height, npaths,
-                                                            path_base, prm->seed, ro, rd, thr, rad,
-                                                            q[0], counts);
+    camera_kernel<<<int(ceil_div(npaths, 256)), 256, 0, st>>>(scenes, prm->width, prm->height,
+                                                            npaths, pps, path_base, ro, rd, thr,
+                                                            rad, q[0], counts);
     for (int32_t k = 0; k <= prm->max_depth + 1; ++k) {
       const int in = k & 1, out = in ^ 1;
       fill_async<uint32_t>(st, counts + out, 1, 0u);
-      const int grid = int(ceil_div(npaths, kThreads));
-      if (bvh->host_depth > 63)
-        bounce_kernel<256><<<grid, kThreads, 0, st>>>(
-            bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, mesh->nrm.p, mesh->mat_idx.p,
-            mesh->mats.p, k, prm->max_depth, prm->miss, prm->seed, path_base, ro, rd, thr, rad,
-            q[in], q[out], counts + in, counts + out, ctx->d_err);
+      const int threads = ctx->tune.render_threads;
+      const int grid = int(ceil_div(npaths, threads));
+      if (threads == 64)
+        launch_bounce<64>(ctx, mesh, bvh, grid, scenes, pps, k, prm, path_base, ro, rd, thr, rad,
+                          q[in], q[out], counts + in, counts + out);
+      else if (threads == 256)
+        launch_bounce<256>(ctx, mesh, bvh, grid, scenes, pps, k, prm, path_base, ro, rd, thr, rad,
+                           q[in], q[out], counts + in, counts + out);
       else
-        bounce_kernel<64><<<grid, kThreads, 0, st>>>(
-            bvh->nodes.p, bvh->tris.p, bvh->perm.p, bvh->header.p, mesh->nrm.p, mesh->mat_idx.p,
-            mesh->mats.p, k, prm->max_depth, prm->miss, prm->seed, path_base, ro, rd, thr, rad,
-            q[in], q[out], counts + in, counts + out, ctx->d_err);
+        launch_bounce<128>(ctx, mesh, bvh, grid, scenes, pps, k, prm, path_base, ro, rd, thr, rad,
+                           q[in], q[out], counts + in, counts + out);
     }
     const bool last_batch = s0 + nsamp >= prm->spp;
-    accumulate_kernel<<<int(ceil_div(wh, 256)), 256, 0, st>>>(rad, wh, int32_t(nsamp), s0 == 0 ? 1 : 0,
-                                                             d_sum, d_sumsq,
-                                                             last_batch ? d_last : nullptr);
+    accumulate_kernel<<<int(ceil_div(wh * nscene, 256)), 256, 0, st>>>(
+        rad, wh, nscene, int32_t(nsamp), s0 == 0 ? 1 : 0, d_sum, d_sumsq,
+        last_batch ? d_last : nullptr);
   }
   HIP_TRY(ctx, hipGetLastError());
   return TRACE_OK;

@@ -6,8 +6,16 @@
 namespace render {
 int set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
                 const int32_t* h_mat_index, const float* h_materials, int32_t nmat);
+// Recompute geometric vertex normals from the current positions (asynchronous).
+int refresh_geo_normals(trace_ctx* ctx, trace_mesh* mesh);
 // Enqueues a render of prm->spp samples per pixel; writes per-pixel sums,
 // sums of squares and the last sample (d_last may be null), 3 floats each.
 int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
            const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last);
+// nscene images in one wavefront; trees[c] (all views of one forest, or one
+// tree) and cams[c] belong to scene c, which draws the Philox stream of
+// prm->seed + c.  Outputs are scene-major.
+int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
+                  int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
+                  float* d_sum, float* d_sumsq, float* d_last);
 }  // namespace render

@@ -70,6 +70,34 @@ class Mesh:
             ctx.check(ctx.lib.trace_mesh_create(ctx.h, _ptr(p), self.ntri, C.byref(h)))
         self.h = h
 
+    @classmethod
+    def segmented(cls, ctx: Context, meshes) -> "Mesh":
+        """Many meshes in one buffer (trace_mesh_create_segmented); meshes is a
+        list of (n_i, 9) arrays.  Its LBVH is a forest with one tree per mesh."""
+        parts = [np.ascontiguousarray(m, np.float64).reshape(-1, 9) for m in meshes]
+        pos = np.ascontiguousarray(np.concatenate(parts))
+        off = np.zeros(len(parts) + 1, np.int64)
+        off[1:] = np.cumsum([len(p) for p in parts])
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.ntri = pos.shape[0]
+        self.offsets = off
+        h = C.c_void_p()
+        ctx.check(ctx.lib.trace_mesh_create_segmented(ctx.h, _ptr(pos), self.ntri, _ptr(off), len(parts),
+                                                      C.byref(h)))
+        self.h = h
+        return self
+
+    def export(self) -> np.ndarray:
+        out = np.empty((self.ntri, 9), np.float64)
+        self.ctx.check(self.ctx.lib.trace_mesh_export(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def animate(self, rest: "Mesh", amplitude: float, period: float, seed: int, d_frame: int) -> None:
+        """trace_mesh_animate (asynchronous; reads and increments *d_frame)."""
+        self.ctx.check(self.ctx.lib.trace_mesh_animate(self.ctx.h, self.h, rest.h, amplitude, period, seed,
+                                                       C.c_void_p(d_frame)))
+
     def set_shading(self, mat_idx, materials, normals=None) -> None:
         """Per-triangle material index, material table (kind, r, g, b) and
         optional vertex normals (n, 9); None = geometric normals."""
@@ -126,6 +154,15 @@ class Bvh:
         b = cls(ctx, h, aux.n_prims)
         b._aux = aux
         return b
+
+    def segment(self, seg: int) -> "Bvh":
+        """View of tree `seg` of a forest (trace_bvh_segment)."""
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.trace_bvh_segment(self.ctx.h, self.h, seg, C.byref(h)))
+        v = Bvh(self.ctx, h, 0)
+        v._forest = self  # keep the forest alive while the view exists
+        v.n_prims = int(v.info().n_prims)
+        return v
 
     def phr_source(self) -> np.ndarray:
         src = np.empty(self.n_prims, np.int32)
@@ -244,3 +281,76 @@ def render_device(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: i
     p = render_params(width, height, spp, max_depth, miss, seed, sample_offset)
     ctx.check(ctx.lib.trace_render_device(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), C.c_void_p(d_sum),
                                           C.c_void_p(d_sumsq), C.c_void_p(d_last or 0)))
+
+
+def render_batched(ctx: Context, mesh: Mesh, forest: Bvh, seg_ids, cams, width: int, height: int, spp: int,
+                   max_depth: int, miss: int, seed: int, d_sum: int, d_sumsq: int, d_last: int | None = None) -> None:
+    """trace_render_batched: scene c traces segment seg_ids[c] with cams[c] and seed + c."""
+    segs = np.ascontiguousarray(seg_ids, np.int32)
+    carr = (_lib.Camera * len(cams))(*[camera_struct(c) for c in cams])
+    p = render_params(width, height, spp, max_depth, miss, seed)
+    ctx.check(ctx.lib.trace_render_batched(ctx.h, mesh.h, forest.h, len(cams), _ptr(segs), carr, C.byref(p),
+                                           C.c_void_p(d_sum), C.c_void_p(d_sumsq), C.c_void_p(d_last or 0)))
+
+
+class Graph:
+    """A HIP graph captured from the context stream (trace_graph_*)."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self.h = h
+
+    @classmethod
+    def capture(cls, ctx: Context, fn) -> "Graph":
+        ctx.check(ctx.lib.trace_graph_begin(ctx.h))
+        h = C.c_void_p()
+        try:
+            fn()
+        except BaseException:
+            # close the capture; a failed recording is dropped
+            if ctx.lib.trace_graph_end(ctx.h, C.byref(h)) == _lib.TRACE_OK and h:
+                ctx.lib.trace_graph_destroy(h)
+            raise
+        ctx.check(ctx.lib.trace_graph_end(ctx.h, C.byref(h)))
+        return cls(ctx, h)
+
+    def launch(self) -> None:
+        self.ctx.check(self.ctx.lib.trace_graph_launch(self.ctx.h, self.h))
+
+    def close(self) -> None:
+        if self.h:
+            self.ctx.lib.trace_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def get_tune(ctx: Context) -> "_lib.Tune":
+    t = _lib.Tune()
+    ctx.check(ctx.lib.trace_get_tune(ctx.h, C.byref(t)))
+    return t
+
+
+def set_tune(ctx: Context, trav_threads: int = 128, render_threads: int = 128, render_batch: int = 1 << 22) -> None:
+    t = _lib.Tune()
+    t.trav_threads, t.render_threads, t.render_batch = trav_threads, render_threads, render_batch
+    ctx.check(ctx.lib.trace_set_tune(ctx.h, C.byref(t)))
+
+
+def autotune_traversal(ctx: Context, bvh: Bvh, d_rays: int, n: int) -> np.ndarray:
+    ms = np.zeros(3, np.float32)
+    ctx.check(ctx.lib.trace_autotune_traversal(ctx.h, bvh.h, C.c_void_p(d_rays), n, _ptr(ms)))
+    return ms
+
+
+def autotune_render(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: int, spp: int, max_depth: int,
+                    miss: int, seed: int) -> np.ndarray:
+    ms = np.zeros(7, np.float32)
+    c = camera_struct(cam)
+    p = render_params(width, height, spp, max_depth, miss, seed)
+    ctx.check(ctx.lib.trace_autotune_render(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), _ptr(ms)))
+    return ms

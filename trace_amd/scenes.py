@@ -117,19 +117,39 @@ def uv_sphere(center, radius: float, slices: int, rings: int, jitter: float = 0.
         r[0, :] = r[0, 0]  # one pole vertex
         r[-1, :] = r[-1, 0]
     verts = c + grid * r[..., None]
-    tris = []
-    for b in range(bands):
-        for s in range(slices):
-            s1 = (s + 1) % slices
-            a, bb, cc, d = verts[b, s], verts[b, s1], verts[b + 1, s1], verts[b + 1, s]
-            if b == 0:
-                tris.append([*a, *cc, *d])
-            elif b == bands - 1:
-                tris.append([*a, *bb, *cc])
-            else:
-                tris.append([*a, *bb, *cc])
-                tris.append([*a, *cc, *d])
-    return np.array(tris, np.float64)
+    s = np.arange(slices)
+    s1 = (s + 1) % slices
+    parts = [np.concatenate([verts[0, s], verts[1, s1], verts[1, s]], axis=1)]  # cap at the top pole
+    if bands > 2:
+        a, bb = verts[1:bands - 1, s], verts[1:bands - 1, s1]
+        cc, d = verts[2:bands, s1], verts[2:bands, s]
+        quads = np.stack([np.concatenate([a, bb, cc], axis=-1), np.concatenate([a, cc, d], axis=-1)], axis=2)
+        parts.append(quads.reshape(-1, 9))
+    if bands > 1:
+        b = bands - 1
+        parts.append(np.concatenate([verts[b, s], verts[b, s1], verts[b + 1, s1]], axis=1))  # bottom cap
+    return np.ascontiguousarray(np.concatenate(parts), np.float64)
+
+
+def displaced_sphere(ntri: int, seed: int, center=(0.0, 0.0, 0.0), radius: float = 1.0,
+                     jitter: float = 0.01) -> np.ndarray:
+    """A UV sphere of about `ntri` triangles (slices ~ sqrt(ntri)) with seeded
+    radial vertex jitter U(-jitter, jitter) * radius; FP32-exact vertices."""
+    rng = np.random.default_rng(seed)
+    slices = max(3, int(round(math.sqrt(ntri))))
+    rings = max(3, int(round(ntri / (2 * slices))) + 2)
+    pos = uv_sphere(center, radius, slices, rings, jitter=jitter * radius, rng=rng)
+    return pos.astype(np.float32).astype(np.float64)
+
+
+def batch_meshes(count: int = 64, lo: int = 1000, hi: int = 500_000, seed: int = 11):
+    """`count` displaced spheres with triangle counts log-uniform in [lo, hi],
+    each at its own seeded centre in [0, 10]^3 and radius in [0.5, 2]."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), count)).astype(np.int64)
+    centres = rng.uniform(0.0, 10.0, (count, 3))
+    radii = rng.uniform(0.5, 2.0, count)
+    return [displaced_sphere(int(n), seed + 1 + i, centres[i], float(radii[i])) for i, n in enumerate(sizes)]
 
 
 def box_scene(seed: int = 1, slices: int = 100, rings: int = 102, width: int = 640, height: int = 360):
