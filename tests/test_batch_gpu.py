@@ -156,6 +156,14 @@ def test_batched_render_argument_checks(ctx):
         render(ctx, mesh, forest, cams[0], 8, 8, 1, 1, 0, 1)
 
 
+def _outward_f32(box64):
+    lo, hi = box64[:, :3], box64[:, 3:]
+    flo, fhi = lo.astype(np.float32), hi.astype(np.float32)
+    flo = np.where(flo.astype(np.float64) > lo, np.nextafter(flo, np.float32(-np.inf)), flo)
+    fhi = np.where(fhi.astype(np.float64) < hi, np.nextafter(fhi, np.float32(np.inf)), fhi)
+    return np.concatenate([flo, fhi], axis=1)
+
+
 def _anim_scene(ctx, ntri):
     pos = scenes.displaced_sphere(ntri, seed=71, radius=0.5)
     mats = np.array([[scenes.MAT_DIFFUSE, 0.6, 0.6, 0.6]], np.float32)
@@ -226,7 +234,10 @@ def test_animation_graph_matches_eager(ctx, oracle):
         assert np.array_equal(keys, okeys) and np.array_equal(perm, operm)
         rec, box = g["bvh"].canonical()
         orec, obox = tree.canonical()
-        assert np.array_equal(rec, orec) and np.array_equal(box.astype(np.float64), obox)
+        assert np.array_equal(rec, orec)
+        # animated vertices are not FP32-exact: the FP32 boxes are the FP64 boxes
+        # rounded outwards
+        assert np.array_equal(box, _outward_f32(obox))
         # the scratch arena cannot grow while the graph is alive
         big = torch.empty((256, 256, 3), dtype=torch.float32, device="cuda")
         with pytest.raises(_lib.TraceError):
