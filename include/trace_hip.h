@@ -201,6 +201,24 @@ int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh*
                         const trace_camera* cam, const trace_render_params* prm, float* d_sum,
                         float* d_sumsq, float* d_last);
 
+/* ---- differentiation (no reference counterpart) ----------------------------- */
+/* Bind per-triangle parameters: d_albedo replaces the colour of Diffuse triangles
+ * (Diffuse.Albedo, material.go:25-38) and d_emission that of Light triangles
+ * (Light.Color, material.go:13-23); 3 floats per triangle, caller order, device memory.
+ * The mesh reads them at every render and backward pass and does not copy them: keep
+ * them alive while work that reads them is queued.  NULL unbinds (material colours). */
+int trace_mesh_bind_params(trace_ctx* ctx, trace_mesh* mesh, const float* d_albedo,
+                           const float* d_emission);
+/* Gradient of a scalar loss on the mean image (sum / spp) of trace_render_device with
+ * the same arguments, with respect to the per-triangle albedo and emission.
+ * d_grad_image = dLoss/d(mean image), W*H*3 floats; d_grad_albedo and d_grad_emission
+ * receive 3 floats per triangle (caller order; overwritten).  Path replay: each path is
+ * traced once to find where it ends and once more to hand every vertex its share, with
+ * the same random numbers as the render, and no path is stored.  Asynchronous. */
+int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                          const trace_camera* cam, const trace_render_params* prm,
+                          const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission);
+
 /* ---- batching, graphs, tuning (no reference counterpart) ------------------- */
 /* nscene images in one wavefront: scene c traces tree h_seg_ids[c] of a forest (built
  * from mesh, a segmented mesh) with camera h_cams[c] and the Philox stream of

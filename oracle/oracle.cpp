@@ -875,6 +875,18 @@ struct Shading {
   int miss;
   int max_depth;
   uint64_t seed;
+  // optional per-triangle RGB albedo (Diffuse) and emission (Light), caller
+  // order; they replace the material table's colour
+  const float* tri_alb = nullptr;
+  const float* tri_emi = nullptr;
+
+  V3 colour(int32_t prim) const {
+    const float* m = mats + 4 * mat_idx[prim];
+    const float* o = m[0] == 1.0f ? tri_emi : tri_alb;
+    if (o) return {o[3 * size_t(prim)], o[3 * size_t(prim) + 1], o[3 * size_t(prim) + 2]};
+    return {m[1], m[2], m[3]};
+  }
+  bool light(int32_t prim) const { return mats[4 * mat_idx[prim]] == 1.0f; }
 };
 
 V3 miss_color(int miss, V3 d) {
@@ -907,12 +919,8 @@ bool trace_ray(const Tree& t, V3 o, V3 d, double tmin, HitRec& h) {
   return node_hit(t, t.root, r, tmin, INFINITY, h);
 }
 
-// DefaultClosestHitShader (render.go:209-224): vertex at depth k.
-V3 shade(const Tree& t, const Shading& s, uint64_t path, int k, V3 o, V3 d, const HitRec& h) {
-  if (k > s.max_depth) return {0, 0, 0};
-  const float* m = s.mats + 4 * s.mat_idx[h.prim];
-  if (m[0] == 1.0f) return {m[1], m[2], m[3]};  // Light: emission, no scatter
-  // Diffuse.scatter (material.go:29-38)
+// Diffuse.scatter (material.go:29-38): the scattered direction at vertex k.
+V3 scatter_dir(const Shading& s, uint64_t path, int k, const HitRec& h) {
   const double* n = s.nrm + 9 * size_t(h.prim);
   const double u = h.u, v = h.v, w0 = 1 - u - v;
   V3 nn{(n[3] * u + n[6] * v) + n[0] * w0, (n[4] * u + n[7] * v) + n[1] * w0,
@@ -924,61 +932,117 @@ V3 shade(const Tree& t, const Shading& s, uint64_t path, int k, V3 o, V3 d, cons
   const double phi = 6.283185307179586 * double(u01(r.v[1]));
   V3 dir{nn.x + rad * std::cos(phi), nn.y + rad * std::sin(phi), nn.z + z};
   if (std::fabs(dir.x) <= 1e-7 && std::fabs(dir.y) <= 1e-7 && std::fabs(dir.z) <= 1e-7) dir = nn;
+  return dir;
+}
+
+// DefaultClosestHitShader (render.go:209-224): vertex at depth k.
+V3 shade(const Tree& t, const Shading& s, uint64_t path, int k, V3 o, V3 d, const HitRec& h) {
+  if (k > s.max_depth) return {0, 0, 0};
+  const V3 c = s.colour(h.prim);
+  if (s.light(h.prim)) return c;  // Light: emission, no scatter
+  const V3 dir = scatter_dir(s, path, k, h);
   const V3 p{o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t};
   HitRec h2{};
   V3 next;
   if (trace_ray(t, p, dir, 0.0001, h2)) next = shade(t, s, path, k + 1, p, dir, h2);
   else next = miss_color(s.miss, dir);
-  return {next.x * m[1], next.y * m[2], next.z * m[3]};  // Le = 0 for Diffuse
+  return {next.x * c.x, next.y * c.y, next.z * c.z};  // Le = 0 for Diffuse
+}
+
+// The same path as shade() traced iteratively, keeping its diffuse vertices
+// (triangle, albedo) and how it ends: *light = the light triangle it reached
+// (-1 for a miss or the depth limit), *E = the radiance there.
+void path_tape(const Tree& t, const Shading& s, uint64_t path, V3 o, V3 d,
+               std::vector<std::pair<int32_t, V3>>& tape, int32_t* light, V3* E) {
+  tape.clear();
+  *light = -1;
+  double tmin = 0.001;
+  for (int k = 0;; ++k) {
+    HitRec h{};
+    if (!trace_ray(t, o, d, tmin, h)) {
+      *E = miss_color(s.miss, d);
+      return;
+    }
+    if (k > s.max_depth) {
+      *E = {0, 0, 0};
+      return;
+    }
+    const V3 c = s.colour(h.prim);
+    if (s.light(h.prim)) {
+      *light = h.prim;
+      *E = c;
+      return;
+    }
+    tape.push_back({h.prim, c});
+    const V3 dir = scatter_dir(s, path, k, h);
+    o = {o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t};
+    d = dir;
+    tmin = 0.0001;
+  }
 }
 
 }  // namespace
 
 extern "C" {
 
+// geometric vertex normals (primitive.go:116-138) of every triangle
+static std::vector<double> geo_normals(const oracle_tree* t) {
+  std::vector<double> geo(9 * size_t(t->n));
+  for (int64_t i = 0; i < t->n; ++i) {
+    const double* p = &t->pos[9 * size_t(i)];
+    for (int c = 0; c < 3; ++c) {
+      const V3 a{p[3 * c], p[3 * c + 1], p[3 * c + 2]};
+      const int ri = (c + 1) % 3, li = (c + 2) % 3;
+      const V3 rgt{p[3 * ri], p[3 * ri + 1], p[3 * ri + 2]};
+      const V3 lft{p[3 * li], p[3 * li + 1], p[3 * li + 2]};
+      const V3 cr = cross(sub(rgt, a), sub(lft, a));
+      const double len = std::sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+      geo[9 * i + 3 * c + 0] = cr.x / len;
+      geo[9 * i + 3 * c + 1] = cr.y / len;
+      geo[9 * i + 3 * c + 2] = cr.z / len;
+    }
+  }
+  return geo;
+}
+
+// camera ray of (pixel, path) (camera.go:95-97, render.go:201-205)
+static V3 camera_dir(const double* cam, int32_t W, int32_t H, int64_t pix, uint64_t seed, uint64_t path) {
+  const int32_t x = int32_t(pix % W), y = int32_t(pix / W);
+  const Philox4 r = path_draw(seed, path, 0u);
+  const double sx = (double(x) + double(u01(r.v[0]))) / double(W - 1);
+  const double sy = (double(y) + double(u01(r.v[1]))) / double(H - 1);
+  return {cam[3] + cam[6] * sx + cam[9] * sy - cam[0], cam[4] + cam[7] * sx + cam[10] * sy - cam[1],
+          cam[5] + cam[8] * sx + cam[11] * sy - cam[2]};
+}
+
 // Per-pixel sum, sum of squares and last sample of spp paths, FP64.  cam =
 // origin, lower_left, horizontal, vertical (12 doubles).  nrm = NULL uses
-// geometric normals (primitive.go:116-138).  path_out (optional) receives
-// every path's value, indexed [sample][pixel][rgb].
+// geometric normals (primitive.go:116-138).  tri_alb / tri_emi (optional, 3
+// floats per triangle) replace the material colour of Diffuse / Light
+// triangles.  path_out (optional) receives every path's value, indexed
+// [sample][pixel][rgb].
 void oracle_render(const oracle_tree* t, const double* nrm, const int32_t* mat_idx,
-                   const float* mats, const double* cam, int32_t W, int32_t H, int32_t spp,
-                   int32_t max_depth, int32_t miss, uint64_t seed, int64_t sample_offset,
-                   int threads, double* sum, double* sumsq, double* last, double* path_out) {
+                   const float* mats, const float* tri_alb, const float* tri_emi, const double* cam,
+                   int32_t W, int32_t H, int32_t spp, int32_t max_depth, int32_t miss, uint64_t seed,
+                   int64_t sample_offset, int threads, double* sum, double* sumsq, double* last,
+                   double* path_out) {
   std::vector<double> geo;
   if (!nrm) {
-    geo.resize(9 * size_t(t->n));
-    for (int64_t i = 0; i < t->n; ++i) {
-      const double* p = &t->pos[9 * size_t(i)];
-      for (int c = 0; c < 3; ++c) {
-        const V3 a{p[3 * c], p[3 * c + 1], p[3 * c + 2]};
-        const int ri = (c + 1) % 3, li = (c + 2) % 3;
-        const V3 rgt{p[3 * ri], p[3 * ri + 1], p[3 * ri + 2]};
-        const V3 lft{p[3 * li], p[3 * li + 1], p[3 * li + 2]};
-        const V3 cr = cross(sub(rgt, a), sub(lft, a));
-        const double len = std::sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
-        geo[9 * i + 3 * c + 0] = cr.x / len;
-        geo[9 * i + 3 * c + 1] = cr.y / len;
-        geo[9 * i + 3 * c + 2] = cr.z / len;
-      }
-    }
+    geo = geo_normals(t);
     nrm = geo.data();
   }
-  const Shading s{nrm, mat_idx, mats, miss, max_depth, seed};
-  const V3 org{cam[0], cam[1], cam[2]}, ll{cam[3], cam[4], cam[5]}, hz{cam[6], cam[7], cam[8]},
-      vt{cam[9], cam[10], cam[11]};
+  Shading s{nrm, mat_idx, mats, miss, max_depth, seed};
+  s.tri_alb = tri_alb;
+  s.tri_emi = tri_emi;
+  const V3 org{cam[0], cam[1], cam[2]};
   const int64_t wh = int64_t(W) * H;
   parallel_for(wh, threads, [&](int64_t b, int64_t e) {
     for (int64_t pix = b; pix < e; ++pix) {
-      const int32_t x = int32_t(pix % W), y = int32_t(pix / W);
       double S[3] = {0, 0, 0}, Q[3] = {0, 0, 0};
       V3 L{0, 0, 0};
       for (int32_t smp = 0; smp < spp; ++smp) {
         const uint64_t path = uint64_t((sample_offset + smp) * wh + pix);
-        const Philox4 r = path_draw(seed, path, 0u);
-        const double sx = (double(x) + double(u01(r.v[0]))) / double(W - 1);
-        const double sy = (double(y) + double(u01(r.v[1]))) / double(H - 1);
-        const V3 d{ll.x + hz.x * sx + vt.x * sy - org.x, ll.y + hz.y * sx + vt.y * sy - org.y,
-                   ll.z + hz.z * sx + vt.z * sy - org.z};
+        const V3 d = camera_dir(cam, W, H, pix, seed, path);
         HitRec h{};
         if (trace_ray(*t, org, d, 0.001, h)) L = shade(*t, s, path, 0, org, d, h);
         else L = miss_color(miss, d);
@@ -1000,6 +1064,93 @@ void oracle_render(const oracle_tree* t, const double* nrm, const int32_t* mat_i
       }
     }
   });
+}
+
+// Gradient of a loss on the mean image (sum / spp) of the render above with
+// respect to the per-triangle albedo and emission, given g_img = dLoss/d(mean
+// image) (W*H*3).  Each path's value is L = a_0 (x) a_1 (x) ... (x) a_{m-1} (x) E
+// (diffuse albedos along the path, E the light's emission or the miss colour),
+// and the albedo does not steer the path, so
+//   dL/da_j = (prod_{i<j} a_i) (x) (prod_{i>j} a_i) (x) E   (summed over repeats)
+//   dL/dEmission(light) = prod_i a_i
+// evaluated with explicit prefix and suffix products (no division).  Also
+// writes the mean image.  Outputs are FP64; g_alb and g_emi hold 3 per
+// triangle, caller order, and are overwritten.
+void oracle_render_grad(const oracle_tree* t, const double* nrm, const int32_t* mat_idx,
+                        const float* mats, const float* tri_alb, const float* tri_emi,
+                        const double* cam, int32_t W, int32_t H, int32_t spp, int32_t max_depth,
+                        int32_t miss, uint64_t seed, int64_t sample_offset, int threads,
+                        const double* g_img, double* mean, double* g_alb, double* g_emi) {
+  std::vector<double> geo;
+  if (!nrm) {
+    geo = geo_normals(t);
+    nrm = geo.data();
+  }
+  Shading s{nrm, mat_idx, mats, miss, max_depth, seed};
+  s.tri_alb = tri_alb;
+  s.tri_emi = tri_emi;
+  const V3 org{cam[0], cam[1], cam[2]};
+  const int64_t wh = int64_t(W) * H;
+  const int64_t n3 = 3 * t->n;
+  // one accumulator per parallel_for chunk, summed in chunk order afterwards
+  const bool serial = threads <= 1 || wh < 1024;
+  const int nchunk = serial ? 1 : threads;
+  const int64_t chunk = serial ? wh : (wh + threads - 1) / threads;
+  std::vector<std::vector<double>> acc(nchunk);
+  parallel_for(wh, threads, [&](int64_t b, int64_t e) {
+    std::vector<double>& A = acc[size_t(b / chunk)];
+    A.assign(2 * size_t(n3), 0.0);
+    double* ga = A.data();
+    double* ge = A.data() + n3;
+    std::vector<std::pair<int32_t, V3>> tape;
+    std::vector<V3> suffix;
+    for (int64_t pix = b; pix < e; ++pix) {
+      const double w[3] = {g_img[3 * pix] / spp, g_img[3 * pix + 1] / spp, g_img[3 * pix + 2] / spp};
+      double S[3] = {0, 0, 0};
+      for (int32_t smp = 0; smp < spp; ++smp) {
+        const uint64_t path = uint64_t((sample_offset + smp) * wh + pix);
+        const V3 d = camera_dir(cam, W, H, pix, seed, path);
+        int32_t light;
+        V3 E;
+        path_tape(*t, s, path, org, d, tape, &light, &E);
+        const size_t m = tape.size();
+        suffix.resize(m + 1);
+        suffix[m] = E;  // radiance arriving from beyond vertex j: prod_{i>j} a_i (x) E
+        for (size_t j = m; j-- > 0;) {
+          const V3 a = tape[j].second;
+          suffix[j] = {suffix[j + 1].x * a.x, suffix[j + 1].y * a.y, suffix[j + 1].z * a.z};
+        }
+        S[0] += suffix[0].x;
+        S[1] += suffix[0].y;
+        S[2] += suffix[0].z;
+        V3 pre{1, 1, 1};
+        for (size_t j = 0; j < m; ++j) {
+          const int32_t tri = tape[j].first;
+          ga[3 * tri + 0] += w[0] * (pre.x * suffix[j + 1].x);
+          ga[3 * tri + 1] += w[1] * (pre.y * suffix[j + 1].y);
+          ga[3 * tri + 2] += w[2] * (pre.z * suffix[j + 1].z);
+          const V3 a = tape[j].second;
+          pre = {pre.x * a.x, pre.y * a.y, pre.z * a.z};
+        }
+        if (light >= 0) {
+          ge[3 * light + 0] += w[0] * pre.x;
+          ge[3 * light + 1] += w[1] * pre.y;
+          ge[3 * light + 2] += w[2] * pre.z;
+        }
+      }
+      for (int c = 0; c < 3; ++c) mean[3 * pix + c] = S[c] / spp;
+    }
+  });
+  for (int64_t i = 0; i < n3; ++i) {
+    double a = 0, em = 0;
+    for (const auto& A : acc) {
+      if (A.empty()) continue;
+      a += A[size_t(i)];
+      em += A[size_t(n3 + i)];
+    }
+    g_alb[i] = a;
+    g_emi[i] = em;
+  }
 }
 
 }  // extern "C"

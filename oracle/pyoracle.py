@@ -52,8 +52,10 @@ def load() -> C.CDLL:
         "oracle_phr_initial_cut": ([_P], C.c_int64),
         "oracle_phr_canonical": ([_P, _P, _P, _P, _P], C.c_int64),
         "oracle_phr_closest_hit": ([_P, _P, C.c_int64, _P, _P, C.c_int], None),
-        "oracle_render": ([_P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+        "oracle_render": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                            C.c_uint64, C.c_int64, C.c_int, _P, _P, _P, _P], None),
+        "oracle_render_grad": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                C.c_int32, C.c_uint64, C.c_int64, C.c_int, _P, _P, _P, _P], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -192,25 +194,61 @@ def philox(counter, key) -> list[int]:
     return [int(v) for v in out]
 
 
+def _opt_f32(a, n: int):
+    if a is None:
+        return None, None
+    arr = np.ascontiguousarray(a, np.float32).reshape(-1)
+    if arr.shape[0] != 3 * n:
+        raise ValueError("per-triangle parameters need 3 floats per triangle")
+    return arr, _p(arr)
+
+
 def render(tree: Tree, mat_idx, materials, cam, width: int, height: int, spp: int, max_depth: int,
            miss: int, seed: int, sample_offset: int = 0, normals=None, path_values: bool = False,
-           threads: int | None = None):
+           threads: int | None = None, albedo=None, emission=None):
     """FP64 restatement of RenderToBuffer.  cam has origin, lower_left,
-    horizontal, vertical (scenes.Camera).  Returns (sum, sumsq, last[, paths]),
-    each (height, width, 3) float64 (paths: (spp, height, width, 3))."""
+    horizontal, vertical (scenes.Camera).  albedo / emission (optional, (n, 3))
+    replace the material colour of Diffuse / Light triangles.  Returns
+    (sum, sumsq, last[, paths]), each (height, width, 3) float64 (paths:
+    (spp, height, width, 3))."""
     mi = np.ascontiguousarray(mat_idx, np.int32)
     mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 4)
     nrm = None if normals is None else np.ascontiguousarray(normals, np.float64)
+    alb, palb = _opt_f32(albedo, len(mi))
+    emi, pemi = _opt_f32(emission, len(mi))
     c = np.ascontiguousarray(np.concatenate([cam.origin, cam.lower_left, cam.horizontal, cam.vertical]), np.float64)
     shape = (height, width, 3)
     s = np.empty(shape, np.float64)
     q = np.empty(shape, np.float64)
     last = np.empty(shape, np.float64)
     paths = np.empty((spp, *shape), np.float64) if path_values else None
-    load().oracle_render(tree.h, None if nrm is None else _p(nrm), _p(mi), _p(mats), _p(c), width, height, spp,
-                         max_depth, miss, seed, sample_offset, threads or _threads(), _p(s), _p(q), _p(last),
-                         None if paths is None else _p(paths))
+    load().oracle_render(tree.h, None if nrm is None else _p(nrm), _p(mi), _p(mats), palb, pemi, _p(c), width,
+                         height, spp, max_depth, miss, seed, sample_offset, threads or _threads(), _p(s), _p(q),
+                         _p(last), None if paths is None else _p(paths))
     return (s, q, last, paths) if path_values else (s, q, last)
+
+
+def render_grad(tree: Tree, mat_idx, materials, cam, width: int, height: int, spp: int, max_depth: int,
+                miss: int, seed: int, g_img, sample_offset: int = 0, normals=None, threads: int | None = None,
+                albedo=None, emission=None):
+    """FP64 gradient of a loss on the mean image with respect to per-triangle
+    albedo and emission, given g_img = dLoss/d(mean image) (height, width, 3).
+    Returns (mean image, g_albedo (n, 3), g_emission (n, 3))."""
+    mi = np.ascontiguousarray(mat_idx, np.int32)
+    n = len(mi)
+    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 4)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.float64)
+    alb, palb = _opt_f32(albedo, n)
+    emi, pemi = _opt_f32(emission, n)
+    c = np.ascontiguousarray(np.concatenate([cam.origin, cam.lower_left, cam.horizontal, cam.vertical]), np.float64)
+    g = np.ascontiguousarray(g_img, np.float64).reshape(height, width, 3)
+    mean = np.empty((height, width, 3), np.float64)
+    ga = np.empty((n, 3), np.float64)
+    ge = np.empty((n, 3), np.float64)
+    load().oracle_render_grad(tree.h, None if nrm is None else _p(nrm), _p(mi), _p(mats), palb, pemi, _p(c), width,
+                              height, spp, max_depth, miss, seed, sample_offset, threads or _threads(), _p(g),
+                              _p(mean), _p(ga), _p(ge))
+    return mean, ga, ge
 
 
 def closest_hit_brute(pos: np.ndarray, rays: np.ndarray, threads: int | None = None):

@@ -94,6 +94,8 @@ SIGNATURES = {
     "trace_set_tune": [_P, _P],
     "trace_autotune_traversal": [_P, _P, _P, _I64, _P],
     "trace_autotune_render": [_P, _P, _P, _P, _P, _P],
+    "trace_mesh_bind_params": [_P, _P, _P, _P],
+    "trace_render_backward": [_P, _P, _P, _P, _P, _P, _P, _P],
 }
 
 

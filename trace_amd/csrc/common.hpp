@@ -107,6 +107,10 @@ struct trace_mesh {
   int32_t nmat = 0;
   bool has_shading = false;
   bool geo_normals = false;  // nrm derived from pos (recomputed when pos animates)
+  // bound per-triangle parameters (caller-owned device memory, 3 floats per
+  // triangle): Diffuse albedo and Light emission replacing the material colour
+  const float* tri_alb = nullptr;
+  const float* tri_emi = nullptr;
   // segmented meshes (many meshes in one buffer, built as a forest in one pass)
   int32_t nseg = 1;
   std::vector<int64_t> seg_off_h;   // nseg + 1 triangle offsets

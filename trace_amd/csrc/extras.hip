@@ -99,6 +99,35 @@ int trace_mesh_animate(trace_ctx* ctx, trace_mesh* mesh, const trace_mesh* rest,
   return TRACE_OK;
 }
 
+int trace_mesh_bind_params(trace_ctx* ctx, trace_mesh* mesh, const float* d_albedo,
+                           const float* d_emission) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_bind_params: null mesh");
+  mesh->tri_alb = d_albedo;
+  mesh->tri_emi = d_emission;
+  return TRACE_OK;
+}
+
+int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                          const trace_camera* cam, const trace_render_params* prm,
+                          const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !bvh || !cam || !prm || !d_grad_image || !d_grad_albedo || !d_grad_emission)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: null argument");
+  if (!mesh->has_shading) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: mesh has no shading data");
+  if (!bvh->base && bvh->nseg > 1)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: differentiate a forest through its segment views");
+  const int64_t tree_tris = bvh->base ? bvh->base->n_prims : bvh->n_prims;
+  if (tree_tris != mesh->ntri) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: bvh does not match the mesh");
+  if (prm->width < 2 || prm->height < 2 || prm->spp < 1 || prm->max_depth < 0 || prm->sample_offset < 0 ||
+      prm->miss < TRACE_MISS_DEFAULT || prm->miss > TRACE_MISS_SUN)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: bad render parameters (spp >= 1 needed)");
+  if (int64_t(prm->width) * prm->height > (int64_t(1) << 28))
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_backward: image too large");
+  const trace_bvh* one[1] = {bvh};
+  return render::backward_scenes(ctx, mesh, one, 1, cam, prm, d_grad_image, d_grad_albedo, d_grad_emission);
+}
+
 int trace_mesh_export(trace_ctx* ctx, const trace_mesh* mesh, double* h_pos) {
   if (!ctx) return TRACE_E_INVALID;
   if (!mesh || !h_pos) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_export: null argument");

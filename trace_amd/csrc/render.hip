@@ -62,6 +62,84 @@ struct Scene {
 
 __global__ void scene_fill_kernel(Scene s, Scene* out) { *out = s; }
 
+// Per-triangle parameters and the adjoint's inputs and outputs.  mode 0 is the
+// plain render; mode 1 (summary) traces the same paths and leaves, per path,
+// S = E (x) prod(non-zero albedos) and flag bits in rad; mode 2 (replay)
+// traces them once more and adds each vertex's share of the gradient.
+struct Adj {
+  const float* alb;    // Diffuse albedo per triangle (3 floats), or null: material colour
+  const float* emi;    // Light emission per triangle (3 floats), or null
+  const float* g_img;  // replay: dLoss/d(mean image), scene-major, 3 per pixel
+  float* g_alb;        // replay: gradient outputs, 3 per triangle
+  float* g_emi;
+  float inv_spp;
+  int32_t mode;
+  int64_t wh;
+};
+constexpr int32_t kModePrimal = 0, kModeSummary = 1, kModeReplay = 2;
+// summary flag bits (rad.w / thr.w hold them as raw bits): per channel, the
+// number of zero albedos on the path so far (2 bits, saturating at 2)
+constexpr uint32_t kFlagLight = 64u;    // the path ended on a light
+constexpr uint32_t kFlagRadiant = 128u; // the radiance E where it ended is non-zero
+
+__device__ inline float3 tri_colour(const Adj& a, float4 m, int32_t prim) {
+  const float* o = m.x == float(TRACE_MAT_LIGHT) ? a.emi : a.alb;
+  if (o) return make_float3(o[3 * size_t(prim)], o[3 * size_t(prim) + 1], o[3 * size_t(prim) + 2]);
+  return make_float3(m.y, m.z, m.w);
+}
+
+// out[3 idx + c] += v_c over the active lanes of a wave.  Lanes that share an
+// index (a wall, a light seen by a whole tile) are summed in registers first:
+// up to four distinct indices are reduced that way, the rest go straight to
+// memory.
+__device__ inline void wave_add3(float* out, int32_t idx, float x, float y, float z, bool active) {
+  const int lane = __lane_id();
+  uint64_t todo = ballot64(active);
+  for (int it = 0; it < 4 && todo; ++it) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const int32_t key = __shfl(idx, leader, 64);
+    const bool in = ((todo >> lane) & 1ull) && idx == key;
+    const uint64_t grp = ballot64(in);
+    float sx = in ? x : 0.0f, sy = in ? y : 0.0f, sz = in ? z : 0.0f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      sx += __shfl_xor(sx, o, 64);
+      sy += __shfl_xor(sy, o, 64);
+      sz += __shfl_xor(sz, o, 64);
+    }
+    if (lane == leader) {
+      float* d = out + 3 * size_t(key);
+      if (sx != 0.0f) unsafeAtomicAdd(d, sx);
+      if (sy != 0.0f) unsafeAtomicAdd(d + 1, sy);
+      if (sz != 0.0f) unsafeAtomicAdd(d + 2, sz);
+    }
+    todo &= ~grp;
+  }
+  if ((todo >> lane) & 1ull) {
+    float* d = out + 3 * size_t(idx);
+    if (x != 0.0f) unsafeAtomicAdd(d, x);
+    if (y != 0.0f) unsafeAtomicAdd(d + 1, y);
+    if (z != 0.0f) unsafeAtomicAdd(d + 2, z);
+  }
+}
+
+// Camera ray of path q of a scene (camera.go:95-97, RandomSampling render.go:201-205).
+__device__ inline void camera_ray(const Scene& sc, int32_t W, int32_t H, int64_t q, int64_t path_base,
+                                  float4& o, float4& d) {
+  const Cam& cam = sc.cam;
+  const int64_t wh = int64_t(W) * H;
+  const int64_t pix = q % wh;
+  const int32_t x = int32_t(pix % W), y = int32_t(pix / W);
+  const Philox4 r = path_draw(sc.seed, uint64_t(path_base + q), 0u);
+  const float s = (float(x) + u01(r.v[0])) / float(W - 1);
+  const float t = (float(y) + u01(r.v[1])) / float(H - 1);
+  const float dx = cam.ll[0] + s * cam.h[0] + t * cam.v[0] - cam.o[0];
+  const float dy = cam.ll[1] + s * cam.h[1] + t * cam.v[1] - cam.o[1];
+  const float dz = cam.ll[2] + s * cam.h[2] + t * cam.v[2] - cam.o[2];
+  o = make_float4(cam.o[0], cam.o[1], cam.o[2], 0.0f);
+  d = make_float4(dx, dy, dz, 0.0f);
+}
+
 __global__ __launch_bounds__(256) void camera_kernel(const Scene* __restrict__ scenes, int32_t W,
                                                      int32_t H, int64_t n_paths, int64_t pps,
                                                      int64_t path_base, float4* __restrict__ ro,
@@ -76,23 +154,48 @@ __global__ __launch_bounds__(256) void camera_kernel(const Scene* __restrict__ s
     counts[1] = 0;
   }
   if (p >= n_paths) return;
-  const Scene& sc = scenes[p / pps];
-  const Cam& cam = sc.cam;
-  const int64_t q = p % pps;
-  const int64_t wh = int64_t(W) * H;
-  const int64_t pix = q % wh;
-  const int32_t x = int32_t(pix % W), y = int32_t(pix / W);
-  const Philox4 r = path_draw(sc.seed, uint64_t(path_base + q), 0u);
-  const float s = (float(x) + u01(r.v[0])) / float(W - 1);
-  const float t = (float(y) + u01(r.v[1])) / float(H - 1);
-  const float dx = cam.ll[0] + s * cam.h[0] + t * cam.v[0] - cam.o[0];
-  const float dy = cam.ll[1] + s * cam.h[1] + t * cam.v[1] - cam.o[1];
-  const float dz = cam.ll[2] + s * cam.h[2] + t * cam.v[2] - cam.o[2];
-  ro[p] = make_float4(cam.o[0], cam.o[1], cam.o[2], 0.0f);
-  rd[p] = make_float4(dx, dy, dz, 0.0f);
+  float4 o, d;
+  camera_ray(scenes[p / pps], W, H, p % pps, path_base, o, d);
+  ro[p] = o;
+  rd[p] = d;
   thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   queue[p] = int32_t(p);
+}
+
+// Replay start: re-creates the camera ray of every path whose summary can
+// carry a gradient (it ended on a light, or with non-zero radiance, and its
+// pixel's image gradient is non-zero) and queues it; rad keeps the summary.
+__global__ __launch_bounds__(256) void replay_camera_kernel(
+    const Scene* __restrict__ scenes, int32_t W, int32_t H, int64_t n_paths, int64_t pps,
+    int64_t path_base, const float* __restrict__ g_img, float4* __restrict__ ro,
+    float4* __restrict__ rd, float4* __restrict__ thr, const float4* __restrict__ rad,
+    int32_t* __restrict__ queue, uint32_t* __restrict__ count) {
+  const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  bool push = false;
+  if (p < n_paths) {
+    const int64_t wh = int64_t(W) * H;
+    const int64_t scene = p / pps, q = p % pps;
+    const float* g = g_img + 3 * (scene * wh + q % wh);
+    const uint32_t flags = __float_as_uint(rad[p].w);
+    push = (flags & (kFlagLight | kFlagRadiant)) && (g[0] != 0.0f || g[1] != 0.0f || g[2] != 0.0f);
+    if (push) {
+      float4 o, d;
+      camera_ray(scenes[scene], W, H, q, path_base, o, d);
+      ro[p] = o;
+      rd[p] = d;
+      thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    }
+  }
+  const uint64_t b = ballot64(push);
+  if (b) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)b) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, uint32_t(__popcll(b)));
+    base = __shfl(base, leader, 64);
+    if (push) queue[base + __popcll(b & lanemask_lt())] = int32_t(p);
+  }
 }
 
 // Moves a surface point a few float ULPs along n (Waechter & Binder, "A Fast
@@ -138,13 +241,15 @@ __global__ __launch_bounds__(THREADS) void bounce_kernel(
     int64_t path_base, float4* __restrict__ ro, float4* __restrict__ rd,
     float4* __restrict__ thr, float4* __restrict__ rad, const int32_t* __restrict__ q_in,
     int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
-    uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err) {
+    uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err, const Adj adj) {
   constexpr int kThreads = THREADS;
   __shared__ int32_t s_stack[traverse::kLdsStack * kThreads];
   const uint32_t count = *count_in;
   const uint32_t i = blockIdx.x * uint32_t(kThreads) + threadIdx.x;
   bool push = false;
   int32_t p = 0;
+  int32_t al_tri = -1, em_tri = -1;  // replay: triangles that receive a gradient
+  float4 al_w = make_float4(0.0f, 0.0f, 0.0f, 0.0f), em_w = al_w;
   if (i < count) {
     p = q_in[i];
     const Scene& sc = scenes[p / pps];
@@ -162,21 +267,38 @@ __global__ __launch_bounds__(THREADS) void bounce_kernel(
     if (bad) atomicOr(d_err, TRACE_ERR_TREE);
     traverse::refine_hit_fp64(tris, ray, h);
     const float4 T = thr[p];
-    float4 L = rad[p];
     if (h.pos < 0) {
       const float3 m = miss_color(miss, d4.x, d4.y, d4.z);
-      L.x += T.x * m.x;
-      L.y += T.y * m.y;
-      L.z += T.z * m.z;
-      rad[p] = L;
+      if (adj.mode == kModePrimal) {
+        float4 L = rad[p];
+        L.x += T.x * m.x;
+        L.y += T.y * m.y;
+        L.z += T.z * m.z;
+        rad[p] = L;
+      } else if (adj.mode == kModeSummary) {
+        const uint32_t f = __float_as_uint(T.w) | ((m.x != 0.0f || m.y != 0.0f || m.z != 0.0f) ? kFlagRadiant : 0u);
+        rad[p] = make_float4(m.x * T.x, m.y * T.y, m.z * T.z, __uint_as_float(f));
+      }
     } else if (k <= max_depth) {
       const int32_t prim = perm[h.pos];
       const float4 m = mats[mat_idx[prim]];
+      const float3 c = tri_colour(adj, m, prim);
       if (m.x == float(TRACE_MAT_LIGHT)) {
-        L.x += T.x * m.y;
-        L.y += T.y * m.z;
-        L.z += T.z * m.w;
-        rad[p] = L;
+        if (adj.mode == kModePrimal) {
+          float4 L = rad[p];
+          L.x += T.x * c.x;
+          L.y += T.y * c.y;
+          L.z += T.z * c.z;
+          rad[p] = L;
+        } else if (adj.mode == kModeSummary) {
+          const uint32_t f = __float_as_uint(T.w) | kFlagLight |
+                             ((c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) ? kFlagRadiant : 0u);
+          rad[p] = make_float4(c.x * T.x, c.y * T.y, c.z * T.z, __uint_as_float(f));
+        } else {
+          // dL/dEmission = product of the albedos before the light
+          em_tri = prim;
+          em_w = T;
+        }
       } else {
         // interpolated vertex normal, flipped to face the ray (primitive.go:145-150, 200-205)
         const float* n = nrm + 9 * size_t(prim);
@@ -217,10 +339,56 @@ __global__ __launch_bounds__(THREADS) void bounce_kernel(
         const float pz = float(double(o4.z) + double(d4.z) * h.t64);
         ro[p] = make_float4(offset_coord(px, gx), offset_coord(py, gy), offset_coord(pz, gz), 0.0f);
         rd[p] = make_float4(sx, sy, sz, 0.0f);
-        thr[p] = make_float4(T.x * m.y, T.y * m.z, T.z * m.w, 0.0f);
+        if (adj.mode == kModeSummary) {
+          // product of the non-zero albedos, and a count of the zero ones
+          uint32_t f = __float_as_uint(T.w);
+          float tc[3] = {T.x, T.y, T.z};
+          const float cc[3] = {c.x, c.y, c.z};
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            if (cc[ch] != 0.0f) {
+              tc[ch] *= cc[ch];
+            } else {
+              const uint32_t z = (f >> (2 * ch)) & 3u;
+              if (z < 2u) f += 1u << (2 * ch);
+            }
+          }
+          thr[p] = make_float4(tc[0], tc[1], tc[2], __uint_as_float(f));
+        } else {
+          thr[p] = make_float4(T.x * c.x, T.y * c.y, T.z * c.z, 0.0f);
+        }
+        if (adj.mode == kModeReplay) {
+          // dL/da_j = S / a_j when the path has no zero albedo; with exactly
+          // one zero albedo only that vertex has a gradient, S itself
+          const float4 S = rad[p];
+          const uint32_t f = __float_as_uint(S.w);
+          const float sc3[3] = {S.x, S.y, S.z};
+          const float cc[3] = {c.x, c.y, c.z};
+          float gv[3];
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) {
+            const uint32_t z = (f >> (2 * ch)) & 3u;
+            gv[ch] = cc[ch] != 0.0f ? (z == 0u ? sc3[ch] / cc[ch] : 0.0f) : (z == 1u ? sc3[ch] : 0.0f);
+          }
+          al_tri = prim;
+          al_w = make_float4(gv[0], gv[1], gv[2], 0.0f);
+        }
         push = true;
       }
     }
+  }
+  if (adj.mode == kModeReplay) {
+    // weight of the path: dLoss/d(mean image) at its pixel / spp
+    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    if (al_tri >= 0 || em_tri >= 0) {
+      const int64_t q = p % pps;
+      const float* g = adj.g_img + 3 * ((p / pps) * adj.wh + q % adj.wh);
+      wx = g[0] * adj.inv_spp;
+      wy = g[1] * adj.inv_spp;
+      wz = g[2] * adj.inv_spp;
+    }
+    wave_add3(adj.g_alb, al_tri, wx * al_w.x, wy * al_w.y, wz * al_w.z, al_tri >= 0);
+    wave_add3(adj.g_emi, em_tri, wx * em_w.x, wy * em_w.y, wz * em_w.z, em_tri >= 0);
   }
   // wave-aggregated append to the next queue
   const uint64_t b = ballot64(push);
@@ -307,37 +475,42 @@ template <int THREADS>
 static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, int grid,
                           const Scene* scenes, int64_t pps, int32_t k, const trace_render_params* prm,
                           int64_t path_base, float4* ro, float4* rd, float4* thr, float4* rad,
-                          int32_t* qin, int32_t* qout, uint32_t* cin, uint32_t* cout) {
+                          int32_t* qin, int32_t* qout, uint32_t* cin, uint32_t* cout, const Adj& adj) {
   hipStream_t st = ctx->stream;
   if (bvh->depth_bound() > 63)
     bounce_kernel<THREADS, 256><<<grid, THREADS, 0, st>>>(
         bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
         mesh->mats.p, k, prm->max_depth, prm->miss, path_base, ro, rd, thr, rad, qin, qout, cin,
-        cout, ctx->d_err);
+        cout, ctx->d_err, adj);
   else
     bounce_kernel<THREADS, 64><<<grid, THREADS, 0, st>>>(
         bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
         mesh->mats.p, k, prm->max_depth, prm->miss, path_base, ro, rd, thr, rad, qin, qout, cin,
-        cout, ctx->d_err);
+        cout, ctx->d_err, adj);
 }
 
-int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
-           const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last) {
-  const trace_bvh* one[1] = {bvh};
-  return render_scenes(ctx, mesh, one, 1, cam, prm, d_sum, d_sumsq, d_last);
-}
+namespace {
 
-int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
-                  int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
-                  float* d_sum, float* d_sumsq, float* d_last) {
+// Device state of one wavefront: per-path rays, throughput, radiance (or
+// adjoint summary), two ray queues and their counters, in the context's
+// scratch arena.
+struct Wave {
+  Scene* scenes;
+  float4 *ro, *rd, *thr, *rad;
+  int32_t* q[2];
+  uint32_t* counts;
+  int64_t spb;  // samples per batch
+};
+
+int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const trace_camera* cams,
+            const trace_render_params* prm, Wave& w) {
   const trace_bvh* bvh = trees[0];  // every tree shares bvh's node and triangle arrays
   if (bvh->depth_bound() > 255)
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "render: tree depth %d exceeds 255", bvh->depth_bound());
   const int64_t wh = int64_t(prm->width) * prm->height;
   const int64_t per_sample = wh * nscene;
-  const int64_t spb =
-      std::max<int64_t>(1, std::min<int64_t>(prm->spp, ctx->tune.render_batch / per_sample));
-  const int64_t max_paths = spb * per_sample;
+  w.spb = std::max<int64_t>(1, std::min<int64_t>(prm->spp, ctx->tune.render_batch / per_sample));
+  const int64_t max_paths = w.spb * per_sample;
   hipStream_t st = ctx->stream;
 
   ScratchPlan plan;
@@ -352,56 +525,131 @@ int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const
   int rc = ctx_reserve_scratch(ctx, plan.off);
   if (rc) return rc;
   char* base = static_cast<char*>(ctx->scratch);
-  auto* ro = reinterpret_cast<float4*>(base + o_ro);
-  auto* rd = reinterpret_cast<float4*>(base + o_rd);
-  auto* thr = reinterpret_cast<float4*>(base + o_thr);
-  auto* rad = reinterpret_cast<float4*>(base + o_rad);
-  int32_t* q[2] = {reinterpret_cast<int32_t*>(base + o_qa), reinterpret_cast<int32_t*>(base + o_qb)};
-  auto* counts = reinterpret_cast<uint32_t*>(base + o_cnt);
-  auto* scenes = reinterpret_cast<Scene*>(base + o_scn);
+  w.ro = reinterpret_cast<float4*>(base + o_ro);
+  w.rd = reinterpret_cast<float4*>(base + o_rd);
+  w.thr = reinterpret_cast<float4*>(base + o_thr);
+  w.rad = reinterpret_cast<float4*>(base + o_rad);
+  w.q[0] = reinterpret_cast<int32_t*>(base + o_qa);
+  w.q[1] = reinterpret_cast<int32_t*>(base + o_qb);
+  w.counts = reinterpret_cast<uint32_t*>(base + o_cnt);
+  w.scenes = reinterpret_cast<Scene*>(base + o_scn);
 
-  // Scene table: written with kernels (arguments by value), so a single-scene
-  // render stays capturable in a HIP graph with no host-memory copy node.
+  // Scene table: written with kernels (arguments by value), so a render stays
+  // capturable in a HIP graph with no host-memory copy node.
   for (int32_t c = 0; c < nscene; ++c) {
-    Scene s;
+    Scene sc;
     const trace_camera* cam = cams + c;
     for (int a = 0; a < 3; ++a) {
-      s.cam.o[a] = float(cam->origin[a]);
-      s.cam.ll[a] = float(cam->lower_left[a]);
-      s.cam.h[a] = float(cam->horizontal[a]);
-      s.cam.v[a] = float(cam->vertical[a]);
+      sc.cam.o[a] = float(cam->origin[a]);
+      sc.cam.ll[a] = float(cam->lower_left[a]);
+      sc.cam.h[a] = float(cam->horizontal[a]);
+      sc.cam.v[a] = float(cam->vertical[a]);
     }
-    s.seed = prm->seed + uint64_t(c);  // scene c of a batch draws the stream of seed + c
-    s.hdr = trees[c]->header_p();
-    scene_fill_kernel<<<1, 1, 0, st>>>(s, scenes + c);
+    sc.seed = prm->seed + uint64_t(c);  // scene c of a batch draws the stream of seed + c
+    sc.hdr = trees[c]->header_p();
+    scene_fill_kernel<<<1, 1, 0, st>>>(sc, w.scenes + c);
   }
-  for (int64_t s0 = 0; s0 < prm->spp; s0 += spb) {
-    const int64_t nsamp = std::min<int64_t>(spb, prm->spp - s0);
+  return TRACE_OK;
+}
+
+// Segments k = 0 .. max_depth + 1 of the paths queued in q[0] (count in counts[0]).
+void bounces(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const Wave& w,
+             int64_t npaths, int64_t pps, int64_t path_base, const trace_render_params* prm,
+             const Adj& adj) {
+  hipStream_t st = ctx->stream;
+  for (int32_t k = 0; k <= prm->max_depth + 1; ++k) {
+    const int in = k & 1, out = in ^ 1;
+    fill_async<uint32_t>(st, w.counts + out, 1, 0u);
+    const int threads = ctx->tune.render_threads;
+    const int grid = int(ceil_div(npaths, threads));
+    if (threads == 64)
+      launch_bounce<64>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
+                        w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, adj);
+    else if (threads == 256)
+      launch_bounce<256>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
+                         w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, adj);
+    else
+      launch_bounce<128>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
+                         w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, adj);
+  }
+}
+
+Adj mesh_adj(const trace_mesh* mesh, int32_t mode) {
+  Adj a{};
+  a.alb = mesh->tri_alb;
+  a.emi = mesh->tri_emi;
+  a.mode = mode;
+  return a;
+}
+
+}  // namespace
+
+int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
+           const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last) {
+  const trace_bvh* one[1] = {bvh};
+  return render_scenes(ctx, mesh, one, 1, cam, prm, d_sum, d_sumsq, d_last);
+}
+
+int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
+                  int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
+                  float* d_sum, float* d_sumsq, float* d_last) {
+  Wave w;
+  int rc = prepare(ctx, trees, nscene, cams, prm, w);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  const int64_t wh = int64_t(prm->width) * prm->height;
+  const Adj adj = mesh_adj(mesh, kModePrimal);
+  for (int64_t s0 = 0; s0 < prm->spp; s0 += w.spb) {
+    const int64_t nsamp = std::min<int64_t>(w.spb, prm->spp - s0);
     const int64_t pps = nsamp * wh;
     const int64_t npaths = pps * nscene;
     const int64_t path_base = (prm->sample_offset + s0) * wh;
-    camera_kernel<<<int(ceil_div(npaths, 256)), 256, 0, st>>>(scenes, prm->width, prm->height,
-                                                            npaths, pps, path_base, ro, rd, thr,
-                                                            rad, q[0], counts);
-    for (int32_t k = 0; k <= prm->max_depth + 1; ++k) {
-      const int in = k & 1, out = in ^ 1;
-      fill_async<uint32_t>(st, counts + out, 1, 0u);
-      const int threads = ctx->tune.render_threads;
-      const int grid = int(ceil_div(npaths, threads));
-      if (threads == 64)
-        launch_bounce<64>(ctx, mesh, bvh, grid, scenes, pps, k, prm, path_base, ro, rd, thr, rad,
-                          q[in], q[out], counts + in, counts + out);
-      else if (threads == 256)
-        launch_bounce<256>(ctx, mesh, bvh, grid, scenes, pps, k, prm, path_base, ro, rd, thr, rad,
-                           q[in], q[out], counts + in, counts + out);
-      else
-        launch_bounce<128>(ctx, mesh, bvh, grid, scenes, pps, k, prm, path_base, ro, rd, thr, rad,
-                           q[in], q[out], counts + in, counts + out);
-    }
+    camera_kernel<<<int(ceil_div(npaths, 256)), 256, 0, st>>>(w.scenes, prm->width, prm->height,
+                                                            npaths, pps, path_base, w.ro, w.rd,
+                                                            w.thr, w.rad, w.q[0], w.counts);
+    bounces(ctx, mesh, trees[0], w, npaths, pps, path_base, prm, adj);
     const bool last_batch = s0 + nsamp >= prm->spp;
     accumulate_kernel<<<int(ceil_div(wh * nscene, 256)), 256, 0, st>>>(
-        rad, wh, nscene, int32_t(nsamp), s0 == 0 ? 1 : 0, d_sum, d_sumsq,
+        w.rad, wh, nscene, int32_t(nsamp), s0 == 0 ? 1 : 0, d_sum, d_sumsq,
         last_batch ? d_last : nullptr);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  return TRACE_OK;
+}
+
+int backward_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
+                    int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
+                    const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission) {
+  Wave w;
+  int rc = prepare(ctx, trees, nscene, cams, prm, w);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  fill_async<float>(st, d_grad_albedo, size_t(3 * mesh->ntri), 0.0f);
+  fill_async<float>(st, d_grad_emission, size_t(3 * mesh->ntri), 0.0f);
+  const int64_t wh = int64_t(prm->width) * prm->height;
+  Adj summary = mesh_adj(mesh, kModeSummary);
+  Adj replay = mesh_adj(mesh, kModeReplay);
+  replay.g_img = d_grad_image;
+  replay.g_alb = d_grad_albedo;
+  replay.g_emi = d_grad_emission;
+  replay.inv_spp = 1.0f / float(prm->spp);
+  replay.wh = wh;
+  for (int64_t s0 = 0; s0 < prm->spp; s0 += w.spb) {
+    const int64_t nsamp = std::min<int64_t>(w.spb, prm->spp - s0);
+    const int64_t pps = nsamp * wh;
+    const int64_t npaths = pps * nscene;
+    const int64_t path_base = (prm->sample_offset + s0) * wh;
+    const int grid = int(ceil_div(npaths, 256));
+    // sweep 1: trace every path, keep its summary
+    camera_kernel<<<grid, 256, 0, st>>>(w.scenes, prm->width, prm->height, npaths, pps, path_base,
+                                        w.ro, w.rd, w.thr, w.rad, w.q[0], w.counts);
+    bounces(ctx, mesh, trees[0], w, npaths, pps, path_base, prm, summary);
+    // sweep 2: replay the paths that carry a gradient and scatter it
+    fill_async<uint32_t>(st, w.counts, 2, 0u);
+    replay_camera_kernel<<<grid, 256, 0, st>>>(w.scenes, prm->width, prm->height, npaths, pps,
+                                               path_base, d_grad_image, w.ro, w.rd, w.thr, w.rad,
+                                               w.q[0], w.counts);
+    bounces(ctx, mesh, trees[0], w, npaths, pps, path_base, prm, replay);
   }
   HIP_TRY(ctx, hipGetLastError());
   return TRACE_OK;

@@ -18,4 +18,12 @@ int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const t
 int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
                   int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
                   float* d_sum, float* d_sumsq, float* d_last);
+// Gradient of a loss on the mean image (sum / spp) of render_scenes with the
+// same arguments, with respect to the per-triangle albedo and emission
+// (3 floats per triangle each, overwritten); d_grad_image = dLoss/d(mean
+// image), scene-major.  Path replay: every path is traced once to find its
+// end, then once more to hand each vertex its share; no path is stored.
+int backward_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
+                    int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
+                    const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission);
 }  // namespace render

@@ -98,6 +98,13 @@ class Mesh:
         self.ctx.check(self.ctx.lib.trace_mesh_animate(self.ctx.h, self.h, rest.h, amplitude, period, seed,
                                                        C.c_void_p(d_frame)))
 
+    def bind_params(self, d_albedo: int | None, d_emission: int | None) -> None:
+        """trace_mesh_bind_params: per-triangle albedo / emission (device
+        pointers to 3 floats per triangle, kept alive by the caller); None
+        unbinds."""
+        self.ctx.check(self.ctx.lib.trace_mesh_bind_params(self.ctx.h, self.h, C.c_void_p(d_albedo or 0),
+                                                           C.c_void_p(d_emission or 0)))
+
     def set_shading(self, mat_idx, materials, normals=None) -> None:
         """Per-triangle material index, material table (kind, r, g, b) and
         optional vertex normals (n, 9); None = geometric normals."""
@@ -281,6 +288,17 @@ def render_device(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: i
     p = render_params(width, height, spp, max_depth, miss, seed, sample_offset)
     ctx.check(ctx.lib.trace_render_device(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), C.c_void_p(d_sum),
                                           C.c_void_p(d_sumsq), C.c_void_p(d_last or 0)))
+
+
+def render_backward(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: int, spp: int, max_depth: int,
+                    miss: int, seed: int, d_grad_image: int, d_grad_albedo: int, d_grad_emission: int,
+                    sample_offset: int = 0) -> None:
+    """trace_render_backward (asynchronous): dLoss/d(per-triangle albedo,
+    emission) from dLoss/d(mean image)."""
+    c = camera_struct(cam)
+    p = render_params(width, height, spp, max_depth, miss, seed, sample_offset)
+    ctx.check(ctx.lib.trace_render_backward(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), C.c_void_p(d_grad_image),
+                                            C.c_void_p(d_grad_albedo), C.c_void_p(d_grad_emission)))
 
 
 def render_batched(ctx: Context, mesh: Mesh, forest: Bvh, seg_ids, cams, width: int, height: int, spp: int,

@@ -186,6 +186,31 @@ def box_scene(seed: int = 1, slices: int = 100, rings: int = 102, width: int = 6
             "height": height}
 
 
+def adjoint_scene(ntri: int = 200_000, seed: int = 7, width: int = 512, height: int = 512):
+    """The five-wall box and area light of box_scene (12 triangles) with four
+    displaced spheres sharing the remaining ~ntri triangles, a ground-truth
+    per-triangle RGB albedo ~U(0.1, 0.9) (seed) and per-triangle emission
+    (15 on the light, 0 elsewhere).  Vertices are FP32-exact.  Returns a dict
+    with pos, mat_idx, materials, albedo (n, 3) float32, emission (n, 3)
+    float32, camera, width, height."""
+    base = box_scene(seed=seed, slices=3, rings=3, width=width, height=height)
+    walls = base["pos"][:12]
+    wall_mi = base["mat_idx"][:12]
+    per = max(8, (ntri - 12) // 4)
+    centres = [(-0.45, -0.6, -0.35), (0.45, -0.6, -0.3), (-0.4, -0.55, 0.45), (0.42, -0.62, 0.45)]
+    radii = [0.38, 0.36, 0.33, 0.34]
+    spheres = [displaced_sphere(per, seed + 10 + i, centres[i], radii[i]) for i in range(4)]
+    pos = np.concatenate([walls] + spheres).astype(np.float32).astype(np.float64)
+    n = len(pos)
+    mat_idx = np.concatenate([wall_mi, np.zeros(n - 12, np.int32)]).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    albedo = rng.uniform(0.1, 0.9, (n, 3)).astype(np.float32)
+    emission = np.zeros((n, 3), np.float32)
+    emission[mat_idx == 3] = 15.0
+    return {"pos": pos, "mat_idx": mat_idx, "materials": base["materials"], "albedo": albedo,
+            "emission": emission, "camera": base["camera"], "width": width, "height": height}
+
+
 def parse_obj(path: str) -> tuple[np.ndarray, np.ndarray | None]:
     """OBJ triangles as obj.go:19-106 reads them: 'v', 'vn', 'f' (fan
     triangulation, negative indices relative to the current count, normals only
