@@ -11,7 +11,10 @@ A step is one closest-hit launch over the 8M incoherent rays on the PHR tree,
 with rays, BVH and outputs resident in HBM.  `value` = incoherent Mrays/s.
 The line also carries closest hit on the LBVH, coherent rays, the LBVH build
 (63-bit and 30-bit keys) and PHR build times, the roofline of the traversal
-kernel and the FP64 CPU oracle timed on a sample.
+kernel, the FP64 CPU oracle timed on a sample, and the other workloads: the
+small render (640x360, 16 spp), batched builds and renders (64 meshes, 32
+scenes), a 120-frame animation under one HIP graph, and the adjoint/primal
+cost of the differentiation workload (200k triangles, 512x512, 32 spp).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,6 +33,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mrays/s incoherent closest-hit @1M tris; LBVH build ms; adjoint/primal cost"
+
+
+def note(msg):
+    """Progress line on stderr (stdout carries only the JSON result)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def _events(torch, n):
@@ -81,6 +89,190 @@ def cpu_baseline(pos, rays, budget_s=12.0):
     }
 
 
+def bench_render_small(torch, ctx, steps):
+    """The small plain-method scene: 640x360, 16 spp, depth 5, box + light +
+    20k-triangle jittered sphere, 30-bit-key LBVH, default (black) miss."""
+    from trace_amd import KEY_M30, _lib, scenes
+    from trace_amd.device import Bvh, Mesh, render_device
+
+    sc = scenes.box_scene(seed=1)
+    W, H, spp, depth = sc["width"], sc["height"], 16, 5
+    mesh = Mesh(ctx, sc["pos"])
+    mesh.set_shading(sc["mat_idx"], sc["materials"])
+    t0 = time.perf_counter()
+    bvh = Bvh.lbvh(ctx, mesh, key_mode=KEY_M30)
+    ctx.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    s = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    q = torch.empty_like(s)
+    per, _ = time_launches(torch, lambda: render_device(ctx, mesh, bvh, sc["camera"], W, H, spp, depth,
+                                                        _lib.MISS_DEFAULT, 1, s.data_ptr(), q.data_ptr()),
+                           steps, 1)
+    ctx.synchronize()
+    ms = float(np.median(per))
+    return {"tris": len(sc["pos"]), "image": [W, H], "spp": spp, "max_depth": depth, "render_ms": round(ms, 3),
+            "samples_per_s": round(W * H * spp / (ms * 1e-3), 1), "build_wall_ms": round(build_ms, 2)}
+
+
+def bench_batched(torch, ctx, steps):
+    """64 jittered spheres with log-uniform [1k, 500k] triangles: one forest
+    build against a loop of per-mesh builds; then 32 of them rendered at
+    512x512, 8 spp, depth 4 in one wavefront against a loop of renders."""
+    from trace_amd import _lib, scenes
+    from trace_amd.device import Bvh, Mesh, render_batched, render_device
+
+    meshes = scenes.batch_meshes(64, 1000, 500_000, seed=11)
+    total = sum(len(m) for m in meshes)
+    seg = Mesh.segmented(ctx, meshes)
+    forest = Bvh.lbvh(ctx, seg)
+    per_f, _ = time_launches(torch, lambda: forest.rebuild(seg), steps, 1)
+    singles = [Mesh(ctx, m) for m in meshes]
+    bvhs = [Bvh.lbvh(ctx, m) for m in singles]
+
+    def loop():
+        for b, m in zip(bvhs, singles):
+            b.rebuild(m)
+
+    per_l, _ = time_launches(torch, loop, max(3, steps // 2), 1)
+    ctx.synchronize()
+    for b in bvhs:
+        b.close()
+    for m in singles:
+        m.close()
+    out = {"meshes": 64, "tris": total, "forest_build_ms": round(float(np.median(per_f)), 3),
+           "per_mesh_loop_ms": round(float(np.median(per_l)), 3)}
+    out["speedup"] = round(out["per_mesh_loop_ms"] / out["forest_build_ms"], 2)
+    out["forest_mtris_per_s"] = round(total / (out["forest_build_ms"] * 1e-3) / 1e6, 1)
+
+    ns, W, H, spp, depth = 32, 512, 512, 8, 4
+    mi = np.zeros(total, np.int32)
+    seg.set_shading(mi, np.array([[scenes.MAT_DIFFUSE, 0.7, 0.6, 0.5]], np.float32))
+    cams = []
+    for m in meshes[:ns]:
+        v = m.reshape(-1, 3)
+        c = (v.min(0) + v.max(0)) / 2
+        r = float((v.max(0) - v.min(0)).max()) / 2
+        cams.append(scenes.Camera(1.0, 40.0, tuple(c + np.array([0.3, 0.5, 3.2]) * r), tuple(c), (0, 1, 0)))
+    img = torch.empty((ns, H, W, 3), dtype=torch.float32, device="cuda")
+    sq = torch.empty_like(img)
+    segs = list(range(ns))
+    per_b, _ = time_launches(torch, lambda: render_batched(ctx, seg, forest, segs, cams, W, H, spp, depth,
+                                                           _lib.MISS_SKY, 3, img.data_ptr(), sq.data_ptr()),
+                             max(3, steps // 4), 1)
+    views = [forest.segment(i) for i in range(ns)]
+
+    def loop_render():
+        for i in range(ns):
+            render_device(ctx, seg, views[i], cams[i], W, H, spp, depth, _lib.MISS_SKY, 3 + i, img[i].data_ptr(),
+                          sq[i].data_ptr())
+
+    per_r, _ = time_launches(torch, loop_render, max(3, steps // 4), 1)
+    ctx.synchronize()
+    for v in views:
+        v.close()
+    b_ms, r_ms = float(np.median(per_b)), float(np.median(per_r))
+    samples = ns * W * H * spp
+    out["render"] = {"scenes": ns, "image": [W, H], "spp": spp, "max_depth": depth, "batched_ms": round(b_ms, 2),
+                     "loop_ms": round(r_ms, 2), "samples_per_s": round(samples / (b_ms * 1e-3), 1),
+                     "speedup": round(r_ms / b_ms, 2)}
+    forest.close()
+    seg.close()
+    return out
+
+
+def bench_animation(torch, ctx, frames=120):
+    """120 frames of a 200k-triangle jittered sphere under seeded sinusoidal
+    vertex displacement (amplitude 0.02), rebuilt every frame: all frames
+    recorded in one HIP graph, against the same calls issued eagerly."""
+    from trace_amd import scenes
+    from trace_amd.device import Bvh, Graph, Mesh
+
+    pos = scenes.displaced_sphere(200_000, seed=21, radius=1.0)
+    rest = Mesh(ctx, pos)
+    mesh = Mesh(ctx, pos)
+    bvh = Bvh.lbvh(ctx, mesh)
+    frame = torch.zeros(1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+
+    def one():
+        mesh.animate(rest, 0.02, 60.0, 5, frame.data_ptr())
+        bvh.rebuild(mesh)
+
+    def eager():
+        for _ in range(frames):
+            one()
+
+    per_e, _ = time_launches(torch, eager, 3, 1)
+    ctx.synchronize()
+    graph = Graph.capture(ctx, eager)
+    per_g, _ = time_launches(torch, graph.launch, 3, 1)
+    ctx.synchronize()
+    graph.close()
+    e_ms, g_ms = float(np.median(per_e)), float(np.median(per_g))
+    out = {"tris": len(pos), "frames": frames, "graph_ms_per_frame": round(g_ms / frames, 4),
+           "eager_ms_per_frame": round(e_ms / frames, 4), "graph_speedup": round(e_ms / g_ms, 2),
+           "frames_done": int(frame.cpu()[0])}
+    bvh.close()
+    mesh.close()
+    rest.close()
+    return out
+
+
+def bench_adjoint(torch, ctx, steps):
+    """The differentiation workload: a 200k-triangle scene (box, four jittered
+    spheres, area light) at 512x512, 32 spp, depth 5; target rendered from the
+    ground-truth albedo (seed 7), current albedo 0.5, L2 image loss.  Times one
+    primal render and one backward pass (path replay) of the same paths."""
+    from trace_amd import _lib, scenes
+    from trace_amd.device import Bvh, Mesh, render_backward, render_device
+
+    d = scenes.adjoint_scene(200_000, seed=7, width=512, height=512)
+    W, H, spp, depth = 512, 512, 32, 5
+    mesh = Mesh(ctx, d["pos"])
+    mesh.set_shading(d["mat_idx"], d["materials"])
+    bvh = Bvh.lbvh(ctx, mesh)
+    n = len(d["pos"])
+    true_alb = torch.from_numpy(d["albedo"]).cuda()
+    emi = torch.from_numpy(d["emission"]).cuda()
+    alb = torch.full((n, 3), 0.5, dtype=torch.float32, device="cuda")
+    s = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    q = torch.empty_like(s)
+    torch.cuda.synchronize()
+    mesh.bind_params(true_alb.data_ptr(), emi.data_ptr())
+    render_device(ctx, mesh, bvh, d["camera"], W, H, spp, depth, _lib.MISS_DEFAULT, 7, s.data_ptr(), q.data_ptr())
+    ctx.synchronize()
+    target = s / spp
+    mesh.bind_params(alb.data_ptr(), emi.data_ptr())
+
+    def primal():
+        render_device(ctx, mesh, bvh, d["camera"], W, H, spp, depth, _lib.MISS_DEFAULT, 7, s.data_ptr(),
+                      q.data_ptr())
+
+    per_p, _ = time_launches(torch, primal, steps, 1)
+    ctx.synchronize()
+    g_img = (2.0 / target.numel()) * (s / spp - target)
+    loss = float(((s / spp - target) ** 2).mean())
+    ga = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    ge = torch.empty_like(ga)
+    torch.cuda.synchronize()
+
+    def backward():
+        render_backward(ctx, mesh, bvh, d["camera"], W, H, spp, depth, _lib.MISS_DEFAULT, 7, g_img.data_ptr(),
+                        ga.data_ptr(), ge.data_ptr())
+
+    per_b, _ = time_launches(torch, backward, steps, 1)
+    ctx.synchronize()
+    p_ms, b_ms = float(np.median(per_p)), float(np.median(per_b))
+    out = {"tris": n, "image": [W, H], "spp": spp, "max_depth": depth, "primal_ms": round(p_ms, 3),
+           "backward_ms": round(b_ms, 3), "adjoint_over_primal": round(b_ms / p_ms, 3), "loss": loss,
+           "grad_albedo_norm": float(torch.linalg.vector_norm(ga).cpu()),
+           "grad_emission_norm": float(torch.linalg.vector_norm(ge).cpu())}
+    mesh.bind_params(None, None)
+    bvh.close()
+    mesh.close()
+    return out
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "closest_hit_incoherent_1M_traffic.json")
     if os.path.exists(path):
@@ -107,7 +299,7 @@ def main():
     import torch
 
     from trace_amd import KEY_M30, KEY_REF63, scenes
-    from trace_amd.device import Bvh, Context, Mesh
+    from trace_amd.device import Bvh, Context, Mesh, autotune_traversal, get_tune
 
     pos = scenes.random_triangles(args.tris, seed=1)
     rays = scenes.incoherent_rays(args.rays, seed=2)
@@ -132,6 +324,11 @@ def main():
     def tracer(bvh, dr, cnt):
         return lambda: bvh.closest_hit_device(dr.data_ptr(), cnt, d_hit.data_ptr(), d_t.data_ptr())
 
+    # the autotuner times each workgroup size on these rays and keeps the fastest
+    tune_ms = autotune_traversal(ctx, phr, d_rays.data_ptr(), n)
+    trav_threads = int(get_tune(ctx).trav_threads)
+
+    note(f"trees built, autotuned ({trav_threads} threads); timing closest hit")
     per, wall = time_launches(torch, tracer(phr, d_rays, n), args.steps, args.warmup)
     ctx.synchronize()
     ms_per_step = wall / args.steps
@@ -178,6 +375,8 @@ def main():
             "tris_per_ray": round(float(tris.mean()), 2),
         },
         "hit_fraction": round(float((hit >= 0).mean()), 4),
+        "autotune": {"trav_threads": trav_threads,
+                     "ms_by_threads": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms)}},
         "phr_build_ms": round(phr_ms, 2),
     }
 
@@ -206,8 +405,18 @@ def main():
                             "mtris_per_s": round(args.tris / (float(per_b.mean()) * 1e-3) / 1e6, 1)}
             b.close()
         out["lbvh_build"] = builds
+        note("traversal and builds timed; small render")
+        out["render_small"] = bench_render_small(torch, ctx, max(3, args.steps // 4))
+        note("batched builds and renders")
+        out["batched"] = bench_batched(torch, ctx, max(3, args.steps // 4))
+        note("animation")
+        out["animation"] = bench_animation(torch, ctx)
+        note("adjoint")
+        out["adjoint"] = bench_adjoint(torch, ctx, max(3, args.steps // 4))
+        out["adjoint_over_primal"] = out["adjoint"]["adjoint_over_primal"]
 
     if not args.no_cpu:
+        note("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(pos, rays)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
