@@ -273,12 +273,77 @@ def bench_adjoint(torch, ctx, steps):
     return out
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "closest_hit_incoherent_1M_traffic.json")
+COPY_BYTES = 1 << 30  # reference copy kernel: reads and writes 1 GiB
+
+
 def load_traffic():
-    path = os.path.join(ROOT, "profiles", "closest_hit_incoherent_1M_traffic.json")
-    if os.path.exists(path):
-        with open(path) as f:
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as f:
             return json.load(f).get("hbm_bytes_per_launch")
     return None
+
+
+def pmc_probe(torch, ctx, phr, d_rays, n, d_hit, d_t):
+    """Workload for the two counter runs (FETCH_SIZE, WRITE_SIZE): a plain
+    copy of COPY_BYTES (torch.mul by 1 into a second buffer, read once and
+    written once), then three closest-hit launches of the timed workload."""
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        phr.closest_hit_device(d_rays.data_ptr(), n, d_hit.data_ptr(), d_t.data_ptr())
+    ctx.synchronize()
+
+
+def _counter_rows(path):
+    import csv
+
+    rows = []
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+            val = r.get("Counter_Value") or r.get("Counter-Value")
+            cname = r.get("Counter_Name") or r.get("Counter-Name") or ""
+            if val is None:
+                continue
+            did = r.get("Dispatch_Id") or r.get("Dispatch-Id") or len(rows)
+            rows.append((int(did), name, cname, float(val)))
+    rows.sort()
+    return [(k, c, v) for _, k, c, v in rows]
+
+
+def traffic_from(fetch_csv, write_csv, hitmiss_csv=None):
+    """HBM bytes per closest-hit launch from the two counter runs, each scaled
+    so that the copy kernel reads / writes exactly COPY_BYTES; with a third
+    run of TCC_HIT_sum and TCC_MISS_sum, the kernel's L2 hit rate."""
+    out = {}
+    for label, path in (("fetch", fetch_csv), ("write", write_csv)):
+        rows = _counter_rows(path)
+        copy = [v for k, c, v in rows if "elementwise" in k.lower() or "mul" in k.lower()]
+        trav = [v for k, c, v in rows if "closest_hit_kernel" in k][-3:]  # the probe's launches
+        if not copy or not trav:
+            raise SystemExit(f"{path}: copy or closest_hit_kernel dispatches not found")
+        copy_kib = max(copy)
+        scale = (COPY_BYTES / 1024.0) / copy_kib
+        out[label] = {"copy_kib": copy_kib, "scale": scale, "kernel_kib_raw": float(np.mean(trav)),
+                      "kernel_bytes": float(np.mean(trav)) * 1024.0 * scale}
+    total = out["fetch"]["kernel_bytes"] + out["write"]["kernel_bytes"]
+    res = {"hbm_bytes_per_launch": round(total), "fetch_bytes": round(out["fetch"]["kernel_bytes"]),
+           "write_bytes": round(out["write"]["kernel_bytes"]), "fetch_scale": round(out["fetch"]["scale"], 4),
+           "write_scale": round(out["write"]["scale"], 4), "copy_bytes": COPY_BYTES,
+           "raw": out}
+    if hitmiss_csv:
+        rows = [(k, c, v) for k, c, v in _counter_rows(hitmiss_csv) if "closest_hit_kernel" in k]
+        hits = [v for k, c, v in rows if c.startswith("TCC_HIT")][-3:]
+        miss = [v for k, c, v in rows if c.startswith("TCC_MISS")][-3:]
+        if hits and miss:
+            res["l2_hit_rate"] = round(float(np.sum(hits) / (np.sum(hits) + np.sum(miss))), 4)
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
+    return 0
 
 
 def main():
@@ -290,7 +355,14 @@ def main():
     ap.add_argument("--rays", type=int, default=8_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick", action="store_true", help="skip the extra measurements")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run only the counter workload (copy kernel + closest hit) and exit")
+    ap.add_argument("--traffic-from", nargs="+", metavar="CSV",
+                    help="FETCH_CSV WRITE_CSV [HITMISS_CSV]: turn rocprofv3 counter CSVs into the traffic "
+                         "file and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        return traffic_from(*args.traffic_from)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the path would shard rays/pixels across GPUs with a replicated BVH)", file=sys.stderr)
@@ -327,6 +399,10 @@ def main():
     # the autotuner times each workgroup size on these rays and keeps the fastest
     tune_ms = autotune_traversal(ctx, phr, d_rays.data_ptr(), n)
     trav_threads = int(get_tune(ctx).trav_threads)
+    if args.pmc_probe:
+        pmc_probe(torch, ctx, phr, d_rays, n, d_hit, d_t)
+        ctx.close()
+        return 0
 
     note(f"trees built, autotuned ({trav_threads} threads); timing closest hit")
     per, wall = time_launches(torch, tracer(phr, d_rays, n), args.steps, args.warmup)
