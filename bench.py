@@ -457,6 +457,12 @@ def main():
     }
 
     if not args.quick:
+        # the host-buffer entry point (rays in and hits out over PCIe, device
+        # staging allocated per call), for comparison with the resident rate
+        phr.closest_hit(rays[:1000])
+        t0 = time.perf_counter()
+        phr.closest_hit(rays)
+        out["host_buffers_mrays_s"] = round(n / (time.perf_counter() - t0) / 1e6, 2)
         per_l, _ = time_launches(torch, tracer(lbvh, d_rays, n), max(5, args.steps // 2), 2)
         out["lbvh_incoherent_mrays_s"] = round(n / (float(per_l.mean()) * 1e-3) / 1e6, 2)
         coh = scenes.coherent_rays_cfg2(4096, 2048)
