@@ -398,13 +398,15 @@ def main():
 
     # the autotuner times each workgroup size on these rays and keeps the fastest
     tune_ms = autotune_traversal(ctx, phr, d_rays.data_ptr(), n)
-    trav_threads = int(get_tune(ctx).trav_threads)
+    tune = get_tune(ctx)
+    trav_threads = int(tune.trav_threads)
+    trav_persistent = bool(tune.trav_persistent)
     if args.pmc_probe:
         pmc_probe(torch, ctx, phr, d_rays, n, d_hit, d_t)
         ctx.close()
         return 0
 
-    note(f"trees built, autotuned ({trav_threads} threads); timing closest hit")
+    note(f"trees built, autotuned ({trav_threads} threads, persistent={trav_persistent}); timing closest hit")
     per, wall = time_launches(torch, tracer(phr, d_rays, n), args.steps, args.warmup)
     ctx.synchronize()
     ms_per_step = wall / args.steps
@@ -451,8 +453,9 @@ def main():
             "tris_per_ray": round(float(tris.mean()), 2),
         },
         "hit_fraction": round(float((hit >= 0).mean()), 4),
-        "autotune": {"trav_threads": trav_threads,
-                     "ms_by_threads": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms)}},
+        "autotune": {"trav_threads": trav_threads, "trav_persistent": trav_persistent,
+                     "ms_one_ray_per_lane": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms[:3])},
+                     "ms_persistent": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms[3:])}},
         "phr_build_ms": round(phr_ms, 2),
     }
 

@@ -260,12 +260,14 @@ typedef struct {
   int32_t render_threads; /* path-segment workgroup size: 64, 128 or 256 */
   int64_t render_batch;   /* paths per wavefront batch */
   int32_t tuned;          /* 1 once an autotune call has run */
-  int32_t pad;
+  int32_t trav_persistent; /* closest hit: 1 = persistent lanes refilled from a work counter,
+                              0 = one ray per lane */
 } trace_tune;
 int trace_get_tune(trace_ctx* ctx, trace_tune* out);
 int trace_set_tune(trace_ctx* ctx, const trace_tune* tune);
-/* Times closest hit on d_rays (device, n rays) for each workgroup size, keeps the
- * fastest in the context, and reports the per-candidate times in ms (3 entries). */
+/* Times closest hit on d_rays (device, n rays) for each workgroup size (64, 128, 256)
+ * with one ray per lane and then with persistent lanes, keeps the fastest in the
+ * context, and reports the per-candidate times in ms (6 entries, in that order). */
 int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays,
                              int64_t n, float* h_ms);
 /* Times a render for each workgroup size and wavefront batch size (candidate batches

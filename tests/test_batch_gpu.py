@@ -283,16 +283,17 @@ def test_autotune_keeps_results(ctx):
     rays = torch.from_numpy(rays_h).cuda()
     torch.cuda.synchronize()
     ms = autotune_traversal(ctx, bvh, rays.data_ptr(), len(rays))
-    assert np.all(ms > 0)
+    assert ms.shape == (6,) and np.all(ms > 0)
     t = get_tune(ctx)
-    assert t.tuned == 1 and t.trav_threads in (64, 128, 256)
+    assert t.tuned == 1 and t.trav_threads in (64, 128, 256) and t.trav_persistent in (0, 1)
     ref = None
-    for threads in (64, 128, 256):
-        set_tune(ctx, trav_threads=threads)
-        hit, tt = bvh.closest_hit(rays_h)
-        if ref is None:
-            ref = (hit, tt)
-        assert np.array_equal(hit, ref[0]) and np.array_equal(tt, ref[1])
+    for persistent in (False, True):
+        for threads in (64, 128, 256):
+            set_tune(ctx, trav_threads=threads, trav_persistent=persistent)
+            hit, tt = bvh.closest_hit(rays_h)
+            if ref is None:
+                ref = (hit, tt)
+            assert np.array_equal(hit, ref[0]) and np.array_equal(tt, ref[1])
 
     sc = scenes.box_scene(seed=3, slices=30, rings=20, width=64, height=36)
     smesh = Mesh(ctx, sc["pos"])

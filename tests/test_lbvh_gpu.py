@@ -97,3 +97,18 @@ def test_rebuild_reuses_handle(ctx, oracle):
     bvh.rebuild(mesh)
     ctx.synchronize()
     _assert_same_tree(bvh, oracle.Tree(b), b)
+
+
+def test_refit_boxes_stable_across_rebuilds(ctx, oracle):
+    """Regression test for the refit's cross-workgroup publication: the second
+    child to arrive at a node must see its sibling's box.  Node boxes of eight
+    rebuilds of the 1M-triangle LBVH all equal the oracle's."""
+    print("lbvh refit stability 1M x 8", flush=True)
+    pos = scenes.random_triangles(1_000_000, seed=1)
+    mesh = Mesh(ctx, pos)
+    bvh = Bvh.lbvh(ctx, mesh)
+    _, obox = oracle.Tree(pos).canonical()
+    for k in range(8):
+        bvh.rebuild(mesh)
+        _, box = bvh.canonical()
+        assert np.array_equal(box.astype(np.float64), obox), f"rebuild {k}: node boxes differ"

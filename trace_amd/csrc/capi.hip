@@ -64,10 +64,16 @@ int trace_ctx_create(int device, trace_ctx** out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&ctx->d_err), sizeof(uint32_t)) != hipSuccess ||
-      hipMemset(ctx->d_err, 0, sizeof(uint32_t)) != hipSuccess) {
+      hipMemset(ctx->d_err, 0, sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&ctx->d_work), 64 * sizeof(uint32_t)) != hipSuccess ||
+      hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    if (ctx->d_work) (void)hipFree(ctx->d_work);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return TRACE_E_HIP;
   }
+  if (ctx->n_cu < 1) ctx->n_cu = 1;
   ctx->stream = ctx->own_stream;
   *out = ctx;
   return TRACE_OK;
@@ -79,6 +85,7 @@ int trace_ctx_destroy(trace_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->d_err) (void)hipFree(ctx->d_err);
+  if (ctx->d_work) (void)hipFree(ctx->d_work);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return TRACE_OK;

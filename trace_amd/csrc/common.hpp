@@ -17,9 +17,10 @@
 // ---------------------------------------------------------------------------
 // Launch parameters the autotuner may change (defaults are the untuned choice).
 struct TuneState {
-  int trav_threads = 128;               // closest-hit workgroup size: 64, 128 or 256
+  int trav_threads = 64;                // closest-hit workgroup size: 64, 128 or 256
   int render_threads = 128;             // path-segment workgroup size: 64, 128 or 256
   int64_t render_batch = int64_t(1) << 22;  // paths per wavefront batch
+  bool trav_persistent = true;          // closest hit: persistent lanes refilled from a counter
   bool tuned = false;
 };
 
@@ -33,6 +34,8 @@ struct trace_ctx {
   // grow-only scratch arena, reused across builds (never shrinks)
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  uint32_t* d_work = nullptr;  // work counters of persistent kernels (64 words)
+  int n_cu = 256;              // compute units of the device
   // HIP graphs recorded from this context hold scratch addresses: while any is
   // alive (or a capture is open) the arena must not move
   int live_graphs = 0;
@@ -145,7 +148,27 @@ struct BvhHeader {
   int32_t n_prims;
   double enc_lo[3];   // FP64 scene box (union of triangle boxes)
   double enc_hi[3];
+  int32_t wroot;      // root of the 4-wide traversal tree (-1: the root is a leaf)
+  int32_t wpad[3];
 };
+
+// Node of the 4-wide traversal tree (64 bytes, one load per visit).  Child
+// boxes are stored as 8-bit offsets on a per-node grid: axis a of child c
+// spans [o_a + qlo * s_a, o_a + qhi * s_a] with s_a = 2^(ebits_a - 127),
+// evaluated as fmaf(q, s_a, o_a); the grid is chosen so that this box
+// contains the child's FP32 box.  cnt == 0: inner child, ref = wide node
+// index; 0 < cnt < 0xFFFF: leaf of triangles [ref, ref + cnt); cnt == 0xFFFF:
+// a larger leaf, ref = (binary node << 1) | slot holds its range.
+struct __align__(64) WideNode {
+  float o[3];
+  uint32_t meta;    // ebits x | y << 8 | z << 16 | n_children << 24
+  uint32_t qlo[3];  // per axis: child c in bits 8c .. 8c + 7
+  uint32_t qhi[3];
+  int32_t ref[4];
+  uint32_t cnt[2];  // child c: bits 16 (c & 1) .. of cnt[c >> 1]
+};
+static_assert(sizeof(WideNode) == 64, "wide node record must be 64 bytes");
+constexpr uint32_t kWideBigLeaf = 0xFFFFu;
 
 // A tree as the reference stores it (for Cost/Size/export and as PHR input):
 // node i has n_children children listed at child_list[child_off[i] ...].
@@ -159,7 +182,8 @@ struct trace_bvh {
   DevArray<int32_t> perm;      // sorted position -> caller triangle index
   DevArray<float> tris;        // sorted order: v0, v1, v2 (9 floats)
   DevArray<float> prim_box;    // sorted order: lo.xyz hi.xyz (6 floats, outward-rounded)
-  DevArray<BvhNode> nodes;     // binary traversal tree
+  DevArray<BvhNode> nodes;     // binary tree (build, refit, export, PHR input)
+  DevArray<WideNode> wnodes;   // 4-wide traversal tree derived from nodes
   DevArray<int32_t> parent;    // inner parents then leaf parents ((p << 1) | slot, -1 = root)
   DevArray<uint32_t> run_start;// leaf i covers [run_start[i], run_start[i+1])
   DevArray<int32_t> inner_range; // LBVH: leaf range [lo, hi] of each inner node
@@ -185,6 +209,7 @@ struct trace_bvh {
   int32_t seg = -1;
 
   const BvhNode* nodes_p() const { return base ? base->nodes.p : nodes.p; }
+  const WideNode* wnodes_p() const { return base ? base->wnodes.p : wnodes.p; }
   const float* tris_p() const { return base ? base->tris.p : tris.p; }
   const int32_t* perm_p() const { return base ? base->perm.p : perm.p; }
   const BvhHeader* header_p() const { return base ? base->seg_headers.p + seg : header.p; }

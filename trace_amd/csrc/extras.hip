@@ -195,6 +195,7 @@ int trace_get_tune(trace_ctx* ctx, trace_tune* out) {
   out->render_threads = ctx->tune.render_threads;
   out->render_batch = ctx->tune.render_batch;
   out->tuned = ctx->tune.tuned ? 1 : 0;
+  out->trav_persistent = ctx->tune.trav_persistent ? 1 : 0;
   return TRACE_OK;
 }
 
@@ -207,6 +208,7 @@ int trace_set_tune(trace_ctx* ctx, const trace_tune* t) {
   ctx->tune.trav_threads = t->trav_threads;
   ctx->tune.render_threads = t->render_threads;
   ctx->tune.render_batch = t->render_batch;
+  ctx->tune.trav_persistent = t->trav_persistent != 0;
   return TRACE_OK;
 }
 
@@ -250,19 +252,25 @@ int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* 
   float* d_t = reinterpret_cast<float*>(out.p + size_t(n) * 4);
   const int cand[3] = {64, 128, 256};
   int best = ctx->tune.trav_threads;
+  bool best_p = ctx->tune.trav_persistent;
   float best_ms = INFINITY;
-  for (int c = 0; c < 3; ++c) {
-    ctx->tune.trav_threads = cand[c];
-    float ms = 0;
-    int rc = time_ms(ctx, [&] { return traverse::closest_hit(ctx, bvh, d_rays, n, d_hit, d_t, nullptr, nullptr); }, 3, &ms);
-    if (rc) return rc;
-    if (h_ms) h_ms[c] = ms;
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = cand[c];
+  for (int m = 0; m < 2; ++m) {
+    for (int c = 0; c < 3; ++c) {
+      ctx->tune.trav_threads = cand[c];
+      ctx->tune.trav_persistent = m == 1;
+      float ms = 0;
+      int rc = time_ms(ctx, [&] { return traverse::closest_hit(ctx, bvh, d_rays, n, d_hit, d_t, nullptr, nullptr); }, 3, &ms);
+      if (rc) return rc;
+      if (h_ms) h_ms[3 * m + c] = ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = cand[c];
+        best_p = m == 1;
+      }
     }
   }
   ctx->tune.trav_threads = best;
+  ctx->tune.trav_persistent = best_p;
   ctx->tune.tuned = true;
   return ctx_check_device_error(ctx, "trace_autotune_traversal");
 }

@@ -20,6 +20,7 @@
 
 #include "lbvh.hpp"
 #include "onesweep.hpp"
+#include "wide.hpp"
 
 namespace lbvh {
 
@@ -501,7 +502,7 @@ __device__ inline float ld_f32(const float* p) {
 // writes the result into its parent's child slot; the second child to arrive
 // at a node (per-node arrival counter, as bvh.go:206/219) unions both slots
 // and continues upwards.  Box words cross workgroups through agent-scope
-// atomics; workgroup fences order them against the counter.
+// atomics whose completion is awaited before the counter is bumped.
 __global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ boxes,
                                                     const uint32_t* __restrict__ run_start,
                                                     const int32_t* __restrict__ parent_inner,
@@ -559,7 +560,13 @@ __global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ bo
       st_f32(my + 3 + a, hi[a]);
     }
     __hip_atomic_store(heights + 2 * node + slot, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // The box and height words are agent-scope atomic stores (written through
+    // to the device coherence point); waiting for their completion before the
+    // arrival counter is bumped makes them visible to whichever workgroup, on
+    // whichever XCD, arrives second.  A workgroup-scope fence alone emits no
+    // wait for global memory, and the counter could overtake the boxes.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
     const uint32_t prev = atomicAdd(counters + node, 1u);
     if (prev == 0) return;  // sibling not done yet; it will continue
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -584,8 +591,9 @@ __global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ bo
 // bounds for node and triangle references.
 __global__ void seg_finalize_kernel(const int32_t* __restrict__ u_begin,
                                     const uint32_t* __restrict__ run_start, int32_t nseg,
-                                    const BvhHeader* __restrict__ hdr, BvhHeader* __restrict__ seg_hdr) {
+                                    BvhHeader* __restrict__ hdr, BvhHeader* __restrict__ seg_hdr) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) hdr->n_inner = hdr->n_leaves - nseg;  // the forest's inner nodes, all trees
   if (s >= nseg) return;
   const int32_t b = u_begin[s], e = u_begin[s + 1];
   BvhHeader& h = seg_hdr[s];
@@ -717,7 +725,7 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
     seg_finalize_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(u_begin, bvh->run_start.p, nseg,
                                                             bvh->header.p, bvh->seg_headers.p);
   HIP_TRY(ctx, hipGetLastError());
-  return TRACE_OK;
+  return wide::widen(ctx, bvh);
 }
 
 }  // namespace lbvh

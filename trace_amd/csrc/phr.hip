@@ -27,6 +27,7 @@
 
 #include "phr.hpp"
 #include "scan.hpp"
+#include "wide.hpp"
 
 namespace phr {
 
@@ -704,7 +705,7 @@ static int clone_lbvh(trace_ctx* ctx, const trace_bvh* aux, trace_bvh* out) {
   HIP_TRY(ctx, hipStreamSynchronize(st));
   out->phr_clone = true;
   out->host_depth = h.depth;
-  return TRACE_OK;
+  return wide::widen(ctx, out);
 }
 
 int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int branching,
@@ -841,7 +842,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   if (rc) return rc;
   if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: layout error 0x%x", hc.err);
   out->host_depth = int32_t(hc.max_level);
-  return TRACE_OK;
+  return wide::widen(ctx, out);
 }
 
 }  // namespace phr

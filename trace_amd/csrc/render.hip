@@ -234,7 +234,7 @@ __device__ inline float3 miss_color(int miss, float dx, float dy, float dz) {
 
 template <int THREADS, int STACK>
 __global__ __launch_bounds__(THREADS) void bounce_kernel(
-    const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
+    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const Scene* __restrict__ scenes, int64_t pps,
     const float* __restrict__ nrm, const int32_t* __restrict__ mat_idx,
     const float4* __restrict__ mats, int32_t k, int32_t max_depth, int32_t miss,
@@ -261,8 +261,8 @@ __global__ __launch_bounds__(THREADS) void bounce_kernel(
     traverse::Ray ray{o4.x, o4.y, o4.z, k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
     int32_t nn = 0, nt = 0;
     bool overflow = false, bad = false;
-    traverse::Hit h = traverse::closest_hit_one<STACK, false>(
-        nodes, tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, kThreads);
+    traverse::Hit h = traverse::closest_hit_wide<STACK, false>(
+        wnodes, nodes, tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, kThreads);
     if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
     if (bad) atomicOr(d_err, TRACE_ERR_TREE);
     traverse::refine_hit_fp64(tris, ray, h);
@@ -478,13 +478,13 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
                           int32_t* qin, int32_t* qout, uint32_t* cin, uint32_t* cout, const Adj& adj) {
   hipStream_t st = ctx->stream;
   if (bvh->depth_bound() > 63)
-    bounce_kernel<THREADS, 256><<<grid, THREADS, 0, st>>>(
-        bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
+    bounce_kernel<THREADS, 512><<<grid, THREADS, 0, st>>>(
+        bvh->wnodes_p(), bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
         mesh->mats.p, k, prm->max_depth, prm->miss, path_base, ro, rd, thr, rad, qin, qout, cin,
         cout, ctx->d_err, adj);
   else
-    bounce_kernel<THREADS, 64><<<grid, THREADS, 0, st>>>(
-        bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
+    bounce_kernel<THREADS, 128><<<grid, THREADS, 0, st>>>(
+        bvh->wnodes_p(), bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
         mesh->mats.p, k, prm->max_depth, prm->miss, path_base, ro, rd, thr, rad, qin, qout, cin,
         cout, ctx->d_err, adj);
 }

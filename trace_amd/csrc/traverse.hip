@@ -1,13 +1,16 @@
 // Bulk closest-hit kernel: one ray per lane, rays streamed from HBM.
+#include <algorithm>
+
 #include "traverse_dev.hpp"
 
 namespace traverse {
 
-// THREADS: workgroup size (chosen by the autotuner, default 128).  STACK 64
-// covers every LBVH (one level per key bit); deeper PHR trees use 256.
+// THREADS: workgroup size (chosen by the autotuner, default 128).  STACK 128
+// covers every LBVH's wide tree (at most 32 wide levels, 3 deferred children
+// each); deeper PHR trees use 512.
 template <int THREADS, int STACK, bool COUNT>
 __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
-    const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
+    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
     const float4* __restrict__ rays, int64_t n, int32_t id_offset, int32_t* __restrict__ hit_out,
     float* __restrict__ t_out, int32_t* __restrict__ cnt_nodes, int32_t* __restrict__ cnt_tris,
@@ -21,8 +24,8 @@ __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
   Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   int32_t nn = 0, nt = 0;
   bool overflow = false, bad = false;
-  Hit h = closest_hit_one<STACK, COUNT>(nodes, tris, hdr, ray, nn, nt, overflow, bad,
-                                        s_stack + threadIdx.x, THREADS);
+  Hit h = closest_hit_wide<STACK, COUNT>(wnodes, nodes, tris, hdr, ray, nn, nt, overflow, bad,
+                                         s_stack + threadIdx.x, THREADS);
   refine_hit_fp64(tris, ray, h);
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);
@@ -34,11 +37,102 @@ __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
   }
 }
 
+// Persistent variant: a grid sized to the resident waves.  Each wave claims
+// chunks of kChunk consecutive ray indices from a global counter (one atomic
+// per chunk) and hands them to its lanes as their rays finish, so a wave keeps
+// its lanes busy until the rays run out instead of waiting for its slowest
+// ray.  A wave leaves the loop once the counter has passed n, its chunk is
+// used up and its lanes are idle.
+constexpr uint32_t kChunk = 256;
+
+template <int THREADS, int STACK, bool COUNT>
+__global__ __launch_bounds__(THREADS) void closest_hit_persistent(
+    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
+    const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
+    const float4* __restrict__ rays, int64_t n, int32_t id_offset, int32_t* __restrict__ hit_out,
+    float* __restrict__ t_out, int32_t* __restrict__ cnt_nodes, int32_t* __restrict__ cnt_tris,
+    uint32_t* d_err, uint32_t* work) {
+  __shared__ int32_t s_stack[kLdsStack * THREADS];
+  const BvhHeader hdr = *hdr_p;
+  const int lane = __lane_id();
+  const uint64_t lt = lanemask_lt();
+  TravStack<STACK> stack;
+  stack.lds = s_stack + threadIdx.x;
+  stack.stride = THREADS;
+  WideState s;
+  Ray ray{};
+  int64_t rid = -1;
+  bool active = false, overflow = false, bad = false;
+  bool fetching = true;                  // the same in every lane of the wave
+  int64_t pool_next = 0, pool_end = 0;   // this wave's unclaimed ray indices
+  int32_t nn = 0, nt = 0;
+  auto finish = [&]() {
+    refine_hit_fp64(tris, ray, s.h);
+    hit_out[rid] = s.h.pos >= 0 ? perm[s.h.pos] - id_offset : -1;
+    t_out[rid] = s.h.t;
+    if (COUNT) {
+      cnt_nodes[rid] = nn;
+      cnt_tris[rid] = nt;
+    }
+  };
+  while (true) {
+    const uint64_t need = ballot64(!active);
+    if (need) {
+      if (pool_next >= pool_end && fetching) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(work, kChunk);
+        base = __shfl(base, 0, 64);
+        if (int64_t(base) >= n) {
+          fetching = false;
+        } else {
+          pool_next = base;
+          pool_end = std::min<int64_t>(int64_t(base) + kChunk, n);
+        }
+      }
+      const int64_t avail = pool_end - pool_next;
+      if (avail > 0) {
+        const int64_t rank = __popcll(need & lt);
+        if (!active && rank < avail) {
+          rid = pool_next + rank;
+          const float4 ra = rays[2 * rid];
+          const float4 rb = rays[2 * rid + 1];
+          ray = Ray{ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+          nn = nt = 0;
+          active = wide_begin<COUNT>(tris, hdr, ray, s, nt, bad);
+          if (!active) finish();
+        }
+        pool_next += std::min<int64_t>(__popcll(need), avail);
+      }
+    }
+    if (!ballot64(active)) {
+      if (!fetching && pool_next >= pool_end) break;
+      continue;
+    }
+    if (active) {
+      active = wide_step<STACK, COUNT>(wnodes, nodes, tris, hdr, s, stack, nn, nt, overflow, bad);
+      if (!active) finish();
+    }
+  }
+  if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
+  if (bad) atomicOr(d_err, TRACE_ERR_TREE);
+}
+
 template <int THREADS, int STACK, bool COUNT>
 static void launch(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, int64_t n,
                    int32_t id_offset, int32_t* d_hit, float* d_t, int32_t* d_nodes, int32_t* d_tris) {
+  if (ctx->tune.trav_persistent) {
+    // up to 32 resident waves per CU; workgroups beyond what fits start as
+    // others leave and find the counter exhausted
+    const int64_t blocks = std::min<int64_t>(ceil_div(n, THREADS), int64_t(ctx->n_cu) * 32 * 64 / THREADS);
+    uint32_t* work = ctx->d_work;
+    fill_async<uint32_t>(ctx->stream, work, 1, 0u);
+    closest_hit_persistent<THREADS, STACK, COUNT><<<int(blocks), THREADS, 0, ctx->stream>>>(
+        bvh->wnodes_p(), bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n,
+        id_offset, d_hit, d_t, d_nodes, d_tris, ctx->d_err, work);
+    return;
+  }
   closest_hit_kernel<THREADS, STACK, COUNT><<<int(ceil_div(n, THREADS)), THREADS, 0, ctx->stream>>>(
-      bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n, id_offset, d_hit, d_t,
+      bvh->wnodes_p(), bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n, id_offset, d_hit, d_t,
       d_nodes, d_tris, ctx->d_err);
 }
 
@@ -49,11 +143,11 @@ static void dispatch(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, i
   const bool deep = bvh->depth_bound() > 63;
   const bool count = d_nodes && d_tris;
   if (count) {
-    if (deep) launch<THREADS, 256, true>(ctx, bvh, rays, n, id_offset, d_hit, d_t, d_nodes, d_tris);
-    else launch<THREADS, 64, true>(ctx, bvh, rays, n, id_offset, d_hit, d_t, d_nodes, d_tris);
+    if (deep) launch<THREADS, 512, true>(ctx, bvh, rays, n, id_offset, d_hit, d_t, d_nodes, d_tris);
+    else launch<THREADS, 128, true>(ctx, bvh, rays, n, id_offset, d_hit, d_t, d_nodes, d_tris);
   } else {
-    if (deep) launch<THREADS, 256, false>(ctx, bvh, rays, n, id_offset, d_hit, d_t, nullptr, nullptr);
-    else launch<THREADS, 64, false>(ctx, bvh, rays, n, id_offset, d_hit, d_t, nullptr, nullptr);
+    if (deep) launch<THREADS, 512, false>(ctx, bvh, rays, n, id_offset, d_hit, d_t, nullptr, nullptr);
+    else launch<THREADS, 128, false>(ctx, bvh, rays, n, id_offset, d_hit, d_t, nullptr, nullptr);
   }
 }
 

@@ -67,7 +67,9 @@ __device__ inline void tri_test(const RayPre& r, const float* __restrict__ tris,
   const float pz = r.dx * e2y - r.dy * e2x;
   const float det = e1x * px + e1y * py + e1z * pz;
   if (fabsf(det) <= kDetEps) return;
-  const float inv = 1.0f / det;
+  // hardware reciprocal (1 ulp): this FP32 test only picks the winner, whose
+  // t, u, v are re-evaluated in FP64 (refine_hit_fp64)
+  const float inv = __builtin_amdgcn_rcpf(det);
   const float tx = r.ox - v0x, ty = r.oy - v0y, tz = r.oz - v0z;
   const float u = (tx * px + ty * py + tz * pz) * inv;
   if (u < 0.0f || u > 1.0f) return;
@@ -218,6 +220,181 @@ __device__ inline Hit closest_hit_one(const BvhNode* __restrict__ nodes,
     }
   }
   return h;
+}
+
+// ---- wide tree (WideNode: 4 children, quantised boxes) -----------------------
+//
+// Traversal state of one lane, so that a ray can be advanced one node visit at
+// a time (the persistent kernel refills lanes whose ray has finished).
+struct WideState {
+  RayPre r;
+  Hit h;
+  int32_t node;
+  int sp;
+  uint32_t visits;
+};
+
+// Starts a ray: root box test, root leaf.  Returns true if node visits follow.
+template <bool COUNT>
+__device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeader& hdr,
+                                  const Ray& ray, WideState& s, int32_t& n_tris, bool& bad) {
+  s.r = ray_pre(ray);
+  s.h.t = ray.tmax;
+  s.h.t64 = ray.tmax;
+  s.h.pos = -1;
+  s.h.u = s.h.v = 0.0f;
+  s.h.front = true;
+  s.sp = 0;
+  s.visits = 0;
+  s.node = hdr.wroot;
+  float tn;
+  if (!slab(s.r, hdr.root_lo[0], hdr.root_lo[1], hdr.root_lo[2], hdr.root_hi[0], hdr.root_hi[1],
+            hdr.root_hi[2], s.h.t, tn))
+    return false;
+  if (hdr.root_cnt > 0) {
+    leaf_test<COUNT>(s.r, tris, hdr.root_ref, hdr.root_cnt, hdr.n_prims, s.h, n_tris, bad);
+    return false;
+  }
+  return true;
+}
+
+// One wide-node visit.  All children's boxes are decoded relative to the ray
+// origin (plane - origin = fmaf(q, cell, o - origin), exact for q = 0 and off
+// by far less than the one-cell padding otherwise) and slab-tested with the
+// near / far planes picked by the direction's signs; hit leaves are tested at
+// once, then the surviving inner children are ordered by entry distance: the
+// nearest is visited next and the others are pushed farthest first.  Returns
+// false when the ray is done.
+template <int STACK, bool COUNT>
+__device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
+                                 const BvhNode* __restrict__ bnodes,
+                                 const float* __restrict__ tris, const BvhHeader& hdr,
+                                 WideState& s, TravStack<STACK>& stack, int32_t& n_nodes,
+                                 int32_t& n_tris, bool& overflow, bool& bad) {
+  // wide nodes are at most as many as binary inner nodes
+  const uint32_t n_bound = uint32_t(hdr.n_inner);
+  if (uint32_t(s.node) >= n_bound || ++s.visits > n_bound) {
+    bad = true;
+    return false;
+  }
+  const uint4* np = reinterpret_cast<const uint4*>(wnodes + s.node);
+  const uint4 a = np[0], b = np[1], c = np[2], d = np[3];
+  if (COUNT) ++n_nodes;
+  const RayPre& r = s.r;
+  const float rx = __uint_as_float(a.x) - r.ox;
+  const float ry = __uint_as_float(a.y) - r.oy;
+  const float rz = __uint_as_float(a.z) - r.oz;
+  const float sx = __uint_as_float((a.w & 0xffu) << 23);
+  const float sy = __uint_as_float(((a.w >> 8) & 0xffu) << 23);
+  const float sz = __uint_as_float(((a.w >> 16) & 0xffu) << 23);
+  const int nch = int(a.w >> 24);
+  // near planes are the low ones on axes the ray runs along positively
+  const bool px = !(r.ix < 0.0f), py = !(r.iy < 0.0f), pz = !(r.iz < 0.0f);
+  const uint32_t nqx = px ? b.x : b.w, fqx = px ? b.w : b.x;
+  const uint32_t nqy = py ? b.y : c.x, fqy = py ? c.x : b.y;
+  const uint32_t nqz = pz ? b.z : c.y, fqz = pz ? c.y : b.z;
+  const int32_t ref[4] = {int32_t(c.z), int32_t(c.w), int32_t(d.x), int32_t(d.y)};
+  const uint32_t cnt[4] = {d.z & 0xffffu, d.z >> 16, d.w & 0xffffu, d.w >> 16};
+  const float tlim = s.h.t;
+  float tk[4];
+  bool hk[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int sh = 8 * k;
+    const float nx = fmaf(float((nqx >> sh) & 0xffu), sx, rx) * r.ix;
+    const float ny = fmaf(float((nqy >> sh) & 0xffu), sy, ry) * r.iy;
+    const float nz = fmaf(float((nqz >> sh) & 0xffu), sz, rz) * r.iz;
+    const float fx = fmaf(float((fqx >> sh) & 0xffu), sx, rx) * r.ix;
+    const float fy = fmaf(float((fqy >> sh) & 0xffu), sy, ry) * r.iy;
+    const float fz = fmaf(float((fqz >> sh) & 0xffu), sz, rz) * r.iz;
+    const float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
+    const float tf = fminf(fminf(fx, fy), fminf(fz, tlim));
+    tk[k] = tn;
+    hk[k] = k < nch && tn <= tf * kSlabSlack;
+  }
+  // leaves first (they can only shorten the ray), in one loop over the hit ones
+  uint32_t leaves = 0u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (hk[k] && cnt[k] != 0u) {
+      leaves |= 1u << k;
+      hk[k] = false;
+    }
+  }
+  while (leaves) {
+    const int k = __builtin_ctz(leaves);
+    leaves &= leaves - 1u;
+    int32_t first = k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3];
+    const uint32_t ck = k == 0 ? cnt[0] : k == 1 ? cnt[1] : k == 2 ? cnt[2] : cnt[3];
+    int32_t count = int32_t(ck);
+    if (ck == kWideBigLeaf) {
+      const int32_t code = first;
+      const int32_t bi = code >> 1;
+      if (code < 0 || uint32_t(bi) >= n_bound) {
+        bad = true;
+        continue;
+      }
+      const int4 rc = *reinterpret_cast<const int4*>(&bnodes[bi].ref[0]);
+      first = (code & 1) ? rc.y : rc.x;
+      count = (code & 1) ? rc.w : rc.z;
+    }
+    leaf_test<COUNT>(r, tris, first, count, hdr.n_prims, s.h, n_tris, bad);
+  }
+  // inner children still in reach, sorted by entry distance
+  float key[4];
+  int32_t id[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const bool live = hk[k] && tk[k] <= s.h.t * kSlabSlack;
+    key[k] = live ? tk[k] : INFINITY;
+    id[k] = live ? ref[k] : -1;
+  }
+#define TRACE_CSWAP(i, j)                      \
+  if (key[j] < key[i]) {                       \
+    const float tf = key[i];                   \
+    key[i] = key[j];                           \
+    key[j] = tf;                               \
+    const int32_t ti = id[i];                  \
+    id[i] = id[j];                             \
+    id[j] = ti;                                \
+  }
+  TRACE_CSWAP(0, 1)
+  TRACE_CSWAP(2, 3)
+  TRACE_CSWAP(0, 2)
+  TRACE_CSWAP(1, 3)
+  TRACE_CSWAP(1, 2)
+#undef TRACE_CSWAP
+  if (id[0] < 0) {
+    if (s.sp == 0) return false;
+    s.node = stack.get(--s.sp);
+    return true;
+  }
+#pragma unroll
+  for (int k = 3; k >= 1; --k) {
+    if (id[k] >= 0) {
+      if (s.sp < STACK) stack.put(s.sp++, id[k]);
+      else overflow = true;
+    }
+  }
+  s.node = id[0];
+  return true;
+}
+
+// Whole-ray wide traversal (used where one lane keeps one ray).
+template <int STACK, bool COUNT>
+__device__ inline Hit closest_hit_wide(const WideNode* __restrict__ wnodes,
+                                       const BvhNode* __restrict__ bnodes,
+                                       const float* __restrict__ tris, const BvhHeader& hdr,
+                                       const Ray& ray, int32_t& n_nodes, int32_t& n_tris,
+                                       bool& overflow, bool& bad, int32_t* lds_col, int stride) {
+  WideState s;
+  TravStack<STACK> stack;
+  stack.lds = lds_col;
+  stack.stride = stride;
+  if (wide_begin<COUNT>(tris, hdr, ray, s, n_tris, bad))
+    while (wide_step<STACK, COUNT>(wnodes, bnodes, tris, hdr, s, stack, n_nodes, n_tris, overflow, bad)) {
+    }
+  return s.h;
 }
 
 }  // namespace traverse

@@ -1,0 +1,11 @@
+// 4-wide traversal tree derived from a binary tree (see wide.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace wide {
+// Enqueue the widening of bvh's binary tree (LBVH, forest or PHR layout) into
+// bvh->wnodes and set wroot in its header(s).  The binary inner-node count is
+// read on the device from the header; bvh->nodes' allocation bounds it.
+int widen(trace_ctx* ctx, trace_bvh* bvh);
+}  // namespace wide

@@ -353,14 +353,17 @@ def get_tune(ctx: Context) -> "_lib.Tune":
     return t
 
 
-def set_tune(ctx: Context, trav_threads: int = 128, render_threads: int = 128, render_batch: int = 1 << 22) -> None:
+def set_tune(ctx: Context, trav_threads: int = 64, render_threads: int = 128, render_batch: int = 1 << 22,
+             trav_persistent: bool = True) -> None:
     t = _lib.Tune()
     t.trav_threads, t.render_threads, t.render_batch = trav_threads, render_threads, render_batch
+    t.trav_persistent = 1 if trav_persistent else 0
     ctx.check(ctx.lib.trace_set_tune(ctx.h, C.byref(t)))
 
 
 def autotune_traversal(ctx: Context, bvh: Bvh, d_rays: int, n: int) -> np.ndarray:
-    ms = np.zeros(3, np.float32)
+    """Times (64, 128, 256 threads) x (one ray per lane, persistent); keeps the best."""
+    ms = np.zeros(6, np.float32)
     ctx.check(ctx.lib.trace_autotune_traversal(ctx.h, bvh.h, C.c_void_p(d_rays), n, _ptr(ms)))
     return ms
 
