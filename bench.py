@@ -322,7 +322,7 @@ def traffic_from(fetch_csv, write_csv, hitmiss_csv=None):
     for label, path in (("fetch", fetch_csv), ("write", write_csv)):
         rows = _counter_rows(path)
         copy = [v for k, c, v in rows if "elementwise" in k.lower() or "mul" in k.lower()]
-        trav = [v for k, c, v in rows if "closest_hit_kernel" in k][-3:]  # the probe's launches
+        trav = [v for k, c, v in rows if "traverse::closest_hit" in k][-3:]  # the probe's launches
         if not copy or not trav:
             raise SystemExit(f"{path}: copy or closest_hit_kernel dispatches not found")
         copy_kib = max(copy)
@@ -335,7 +335,7 @@ def traffic_from(fetch_csv, write_csv, hitmiss_csv=None):
            "write_scale": round(out["write"]["scale"], 4), "copy_bytes": COPY_BYTES,
            "raw": out}
     if hitmiss_csv:
-        rows = [(k, c, v) for k, c, v in _counter_rows(hitmiss_csv) if "closest_hit_kernel" in k]
+        rows = [(k, c, v) for k, c, v in _counter_rows(hitmiss_csv) if "traverse::closest_hit" in k]
         hits = [v for k, c, v in rows if c.startswith("TCC_HIT")][-3:]
         miss = [v for k, c, v in rows if c.startswith("TCC_MISS")][-3:]
         if hits and miss:
@@ -446,7 +446,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic(),
-            "kernel": "closest_hit_kernel",
+            "kernel": "closest_hit_persistent" if trav_persistent else "closest_hit_kernel",
             "kernel_ms": round(launch_ms, 4),
             "bytes_per_ray": round(bytes_launch / n, 1),
             "inner_nodes_per_ray": round(float(nodes.mean()), 2),

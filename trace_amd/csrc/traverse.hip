@@ -44,6 +44,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
 // ray.  A wave leaves the loop once the counter has passed n, its chunk is
 // used up and its lanes are idle.
 constexpr uint32_t kChunk = 256;
+constexpr int kRefillAt = 16;
 
 template <int THREADS, int STACK, bool COUNT>
 __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
@@ -75,9 +76,17 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
       cnt_tris[rid] = nt;
     }
   };
+  bool pending = false;  // a finished ray whose result is not written yet
   while (true) {
     const uint64_t need = ballot64(!active);
-    if (need) {
+    // Lanes whose ray is done wait until a quarter of the wave is idle: the
+    // refinement, the result writes and the next ray's set-up then run for
+    // many lanes at once instead of costing the whole wave for one lane.
+    if (__popcll(need) >= kRefillAt || need == ~0ull) {
+      if (pending) {
+        finish();
+        pending = false;
+      }
       if (pool_next >= pool_end && fetching) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(work, kChunk);
@@ -110,7 +119,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
     }
     if (active) {
       active = wide_step<STACK, COUNT>(wnodes, nodes, tris, hdr, s, stack, nn, nt, overflow, bad);
-      if (!active) finish();
+      pending = !active;
     }
   }
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
