@@ -129,3 +129,28 @@ def test_config2_full_batch_properties(ctx):
     q = np.cross(tv, e1)
     tt = np.einsum("ij,ij->i", e2, q) / det
     assert np.max(np.abs(tt - t[sel]) / np.abs(tt)) < T_REL
+
+
+def test_axis_parallel_rays_stay_cheap(ctx, oracle):
+    """Rays with zero direction components (1/d = +-inf) must keep full slab
+    culling: they visit about as many nodes as slightly tilted copies, and
+    their hits match the oracle."""
+    print("traverse axis-parallel rays", flush=True)
+    pos = scenes.random_triangles(50_000, seed=25, mean_edge=0.01)
+    rng = np.random.default_rng(26)
+    n = 20_000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = rng.uniform(0, 1, (n, 3))
+    rays[:, 3] = 0.001
+    axis = rng.integers(0, 3, n)
+    rays[np.arange(n), 4 + axis] = rng.choice([-1.0, 1.0], n)
+    rays[:, 7] = np.inf
+    tilted = rays.copy()
+    tilted[:, 4:7] += np.float32(1e-3) * rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    nodes, _ = bvh.counts(rays)
+    tnodes, _ = bvh.counts(tilted)
+    assert nodes.mean() <= 2.0 * tnodes.mean(), (nodes.mean(), tnodes.mean())
+    hit, t = bvh.closest_hit(rays)
+    ohit, ot = oracle.Tree(pos).closest_hit(rays)
+    _check(hit, t, ohit, ot, min_hits=0.0)

@@ -161,7 +161,7 @@ struct BvhHeader {
 // a larger leaf, ref = (binary node << 1) | slot holds its range.
 struct __align__(64) WideNode {
   float o[3];
-  uint32_t meta;    // ebits x | y << 8 | z << 16 | n_children << 24
+  uint32_t meta;    // ebits x | y << 8 | z << 16 | (mask of used child slots) << 24
   uint32_t qlo[3];  // per axis: child c in bits 8c .. 8c + 7
   uint32_t qhi[3];
   int32_t ref[4];

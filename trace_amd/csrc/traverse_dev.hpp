@@ -262,9 +262,9 @@ __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeade
   return true;
 }
 
-// One wide-node visit.  All children's boxes are slab-tested in the node's
-// grid coordinates (see below) with the near / far planes picked by the
-// direction's signs; hit leaves are tested at
+// One wide-node visit.  All children's boxes are decoded relative to the ray
+// origin and slab-tested with the near / far planes picked by the direction's
+// signs; hit leaves are tested at
 // once, then the surviving inner children are ordered by entry distance: the
 // nearest is visited next and the others are pushed farthest first.  Returns
 // false when the ray is done.
@@ -284,16 +284,16 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
   const uint4 a = np[0], b = np[1], c = np[2], d = np[3];
   if (COUNT) ++n_nodes;
   const RayPre& r = s.r;
-  // t(plane q) = q * (cell / d) + (o - origin) / d, one FMA per plane; for
-  // q = 0 it is exactly (o - origin) / d, and for q > 0 its rounding stays far
-  // below the one-cell padding of the stored boxes
-  const float bx = (__uint_as_float(a.x) - r.ox) * r.ix;
-  const float by = (__uint_as_float(a.y) - r.oy) * r.iy;
-  const float bz = (__uint_as_float(a.z) - r.oz) * r.iz;
-  const float ax = __uint_as_float((a.w & 0xffu) << 23) * r.ix;
-  const float ay = __uint_as_float(((a.w >> 8) & 0xffu) << 23) * r.iy;
-  const float az = __uint_as_float(((a.w >> 16) & 0xffu) << 23) * r.iz;
-  const int nch = int(a.w >> 24);
+  // plane - origin = fmaf(q, cell, o - origin): exactly (o - origin) for
+  // q = 0, and for q > 0 off by far less than the one-cell padding of the
+  // stored boxes; times 1/d it keeps the IEEE infinities of axis-parallel rays
+  const float rx = __uint_as_float(a.x) - r.ox;
+  const float ry = __uint_as_float(a.y) - r.oy;
+  const float rz = __uint_as_float(a.z) - r.oz;
+  const float sx = __uint_as_float((a.w & 0xffu) << 23);
+  const float sy = __uint_as_float(((a.w >> 8) & 0xffu) << 23);
+  const float sz = __uint_as_float(((a.w >> 16) & 0xffu) << 23);
+  const uint32_t used = a.w >> 24;  // child slots in use
   // near planes are the low ones on axes the ray runs along positively
   const bool px = !(r.ix < 0.0f), py = !(r.iy < 0.0f), pz = !(r.iz < 0.0f);
   const uint32_t nqx = px ? b.x : b.w, fqx = px ? b.w : b.x;
@@ -307,16 +307,16 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int sh = 8 * k;
-    const float nx = fmaf(float((nqx >> sh) & 0xffu), ax, bx);
-    const float ny = fmaf(float((nqy >> sh) & 0xffu), ay, by);
-    const float nz = fmaf(float((nqz >> sh) & 0xffu), az, bz);
-    const float fx = fmaf(float((fqx >> sh) & 0xffu), ax, bx);
-    const float fy = fmaf(float((fqy >> sh) & 0xffu), ay, by);
-    const float fz = fmaf(float((fqz >> sh) & 0xffu), az, bz);
+    const float nx = fmaf(float((nqx >> sh) & 0xffu), sx, rx) * r.ix;
+    const float ny = fmaf(float((nqy >> sh) & 0xffu), sy, ry) * r.iy;
+    const float nz = fmaf(float((nqz >> sh) & 0xffu), sz, rz) * r.iz;
+    const float fx = fmaf(float((fqx >> sh) & 0xffu), sx, rx) * r.ix;
+    const float fy = fmaf(float((fqy >> sh) & 0xffu), sy, ry) * r.iy;
+    const float fz = fmaf(float((fqz >> sh) & 0xffu), sz, rz) * r.iz;
     const float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
     const float tf = fminf(fminf(fx, fy), fminf(fz, tlim));
     tk[k] = tn;
-    hk[k] = k < nch && tn <= tf * kSlabSlack;
+    hk[k] = ((used >> k) & 1u) && tn <= tf * kSlabSlack;
   }
   // leaves first (they can only shorten the ray), in one loop over the hit ones
   uint32_t leaves = 0u;

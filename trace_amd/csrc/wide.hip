@@ -59,108 +59,121 @@ struct Slot {
   float lo[3], hi[3];
   int32_t ref;
   uint32_t cnt;  // 0: inner (ref = wide index)
+  bool used;
 };
 
-__device__ inline void leaf_slot(Slot& s, const float* box, int32_t ref, int32_t cnt, int32_t code) {
+__device__ inline Slot empty_slot() {
+  Slot s{};
+  s.used = false;
+  s.ref = -1;
+  s.cnt = 0u;
+  return s;
+}
+
+// Child d of binary node j (the box is read from memory): a leaf slot, or an
+// inner slot pointing at the wide node of that child.
+__device__ inline Slot child_slot(const BvhNode* __restrict__ nodes, const uint32_t* __restrict__ widx,
+                                  int32_t n_inner, int32_t j, int d, bool& ok) {
+  Slot s;
+  const float* box = reinterpret_cast<const float*>(nodes + j) + 6 * d;
   for (int a = 0; a < 3; ++a) {
     s.lo[a] = box[a];
     s.hi[a] = box[3 + a];
   }
-  if (uint32_t(cnt) < kWideBigLeaf) {
-    s.ref = ref;
-    s.cnt = uint32_t(cnt);
+  s.used = true;
+  const int32_t ref = nodes[j].ref[d], cnt = nodes[j].cnt[d];
+  if (cnt > 0) {
+    if (uint32_t(cnt) < kWideBigLeaf) {
+      s.ref = ref;
+      s.cnt = uint32_t(cnt);
+    } else {
+      s.ref = (j << 1) | d;
+      s.cnt = kWideBigLeaf;
+    }
   } else {
-    s.ref = code;
-    s.cnt = kWideBigLeaf;
+    ok = ok && ref >= 0 && ref < n_inner;
+    s.ref = ok ? int32_t(widx[ref]) : -1;
+    s.cnt = 0u;
   }
+  return s;
 }
 
-// Grid of one axis: cell size 2^k (returned as k) such that the extent spans
-// at most 253 cells and a cell is at least four ulps of the coordinates.
+// e with x < 2^e (x > 0 and normal): the binary exponent of x, plus one
+__device__ inline int exp_above(double x) {
+  return int((uint64_t(__double_as_longlong(x)) >> 52) & 0x7ff) - 1022;
+}
+
 __device__ inline int grid_exponent(float lo, float hi) {
   const double ext = double(hi) - double(lo);
   int k = -126;
-  if (ext > 0.0) {
-    int e;
-    frexp(ext / 253.0, &e);  // ext / 253 <= 2^e
-    k = max(k, e);
-  }
+  if (ext > 0.0) k = max(k, exp_above(ext / 253.0));  // ext / 253 < 2^k
   const double mag = fmax(fabs(double(lo)), fabs(double(hi)));
-  if (mag > 0.0) {
-    int e;
-    frexp(mag, &e);  // mag < 2^e, so ulp(mag) <= 2^(e - 24)
-    k = max(k, e - 22);
-  }
+  if (mag > 0.0) k = max(k, exp_above(mag) - 22);  // ulp(mag) <= 2^(k - 2)
   return min(k, 127);
 }
 
 __device__ inline float decode(uint32_t q, float s, float o) { return fmaf(float(q), s, o); }
 
+// Binary child c of a wide node fills slots 2c and 2c + 1: a leaf child takes
+// slot 2c, an inner child (odd depth) is replaced by its two children.
 __global__ void widen_kernel(const BvhNode* __restrict__ nodes, const BvhHeader* __restrict__ hdr,
                              const uint32_t* __restrict__ flag, const uint32_t* __restrict__ widx,
                              WideNode* __restrict__ out, uint32_t* d_err) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const int32_t n_inner = hdr->n_inner;
   if (i >= n_inner || !flag[i]) return;
-  const BvhNode b = nodes[i];
+  bool ok = true;
   Slot s[4];
-  int n = 0;
+#pragma unroll
   for (int c = 0; c < 2; ++c) {
-    const float* box = reinterpret_cast<const float*>(&b) + 6 * c;  // child c: lo.xyz, hi.xyz
-    if (b.cnt[c] > 0) {
-      leaf_slot(s[n++], box, b.ref[c], b.cnt[c], (i << 1) | c);
-      continue;
-    }
-    const int32_t j = b.ref[c];
-    if (j < 0 || j >= n_inner) {
-      atomicOr(d_err, TRACE_ERR_TREE);
-      return;
-    }
-    const BvhNode cj = nodes[j];
-    for (int d = 0; d < 2; ++d) {
-      const float* cb = reinterpret_cast<const float*>(&cj) + 6 * d;
-      if (cj.cnt[d] > 0) {
-        leaf_slot(s[n++], cb, cj.ref[d], cj.cnt[d], (j << 1) | d);
-      } else {
-        const int32_t g = cj.ref[d];
-        if (g < 0 || g >= n_inner) {
-          atomicOr(d_err, TRACE_ERR_TREE);
-          return;
-        }
-        Slot& t = s[n++];
-        for (int a = 0; a < 3; ++a) {
-          t.lo[a] = cb[a];
-          t.hi[a] = cb[3 + a];
-        }
-        t.ref = int32_t(widx[g]);
-        t.cnt = 0u;
-      }
+    const int32_t cref = nodes[i].ref[c], ccnt = nodes[i].cnt[c];
+    if (ccnt > 0) {
+      s[2 * c] = child_slot(nodes, widx, n_inner, i, c, ok);
+      s[2 * c + 1] = empty_slot();
+    } else if (cref >= 0 && cref < n_inner) {
+      s[2 * c] = child_slot(nodes, widx, n_inner, cref, 0, ok);
+      s[2 * c + 1] = child_slot(nodes, widx, n_inner, cref, 1, ok);
+    } else {
+      ok = false;
+      s[2 * c] = empty_slot();
+      s[2 * c + 1] = empty_slot();
     }
   }
+  if (!ok) {
+    atomicOr(d_err, TRACE_ERR_TREE);
+    return;
+  }
   WideNode w;
-  uint32_t meta = uint32_t(n) << 24;
+  uint32_t valid = 0u;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) valid |= s[c].used ? 1u << c : 0u;
+  uint32_t meta = valid << 24;
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
-    float lo = s[0].lo[a], hi = s[0].hi[a];
-    for (int c = 1; c < n; ++c) {
-      lo = fminf(lo, s[c].lo[a]);
-      hi = fmaxf(hi, s[c].hi[a]);
+    float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      lo = s[c].used ? fminf(lo, s[c].lo[a]) : lo;
+      hi = s[c].used ? fmaxf(hi, s[c].hi[a]) : hi;
     }
     const int k = grid_exponent(lo, hi);
-    const float sc = ldexpf(1.0f, k);
-    const double inv = ldexp(1.0, -k);
+    const float sc = __uint_as_float(uint32_t(k + 127) << 23);          // 2^k
+    const double inv = __longlong_as_double(int64_t(1023 - k) << 52);   // 2^-k
     w.o[a] = lo;
     meta |= uint32_t(k + 127) << (8 * a);
     uint32_t ql = 0, qh = 0;
+#pragma unroll
     for (int c = 0; c < 4; ++c) {
       uint32_t l = 255u, h = 0u;  // empty slot
-      if (c < n) {
-        const double fl = floor((double(s[c].lo[a]) - double(lo)) * inv) - 1.0;
-        const double fh = ceil((double(s[c].hi[a]) - double(lo)) * inv) + 1.0;
+      if (s[c].used) {
+        const float clo = s[c].lo[a], chi = s[c].hi[a];
+        const double fl = floor((double(clo) - double(lo)) * inv) - 1.0;
+        const double fh = ceil((double(chi) - double(lo)) * inv) + 1.0;
         l = uint32_t(fmin(fmax(fl, 0.0), 255.0));
         h = uint32_t(fmin(fmax(fh, 0.0), 255.0));
-        while (l > 0u && decode(l, sc, lo) > s[c].lo[a]) --l;
-        while (h < 255u && decode(h, sc, lo) < s[c].hi[a]) ++h;
-        if (decode(h, sc, lo) < s[c].hi[a]) atomicOr(d_err, TRACE_ERR_TREE);  // grid too small
+        while (l > 0u && decode(l, sc, lo) > clo) --l;
+        while (h < 255u && decode(h, sc, lo) < chi) ++h;
+        if (decode(h, sc, lo) < chi) atomicOr(d_err, TRACE_ERR_TREE);  // grid too small
       }
       ql |= l << (8 * c);
       qh |= h << (8 * c);
@@ -169,11 +182,10 @@ __global__ void widen_kernel(const BvhNode* __restrict__ nodes, const BvhHeader*
     w.qhi[a] = qh;
   }
   w.meta = meta;
-  w.cnt[0] = w.cnt[1] = 0u;
-  for (int c = 0; c < 4; ++c) {
-    w.ref[c] = c < n ? s[c].ref : -1;
-    if (c < n) w.cnt[c >> 1] |= s[c].cnt << (16 * (c & 1));
-  }
+  w.cnt[0] = s[0].cnt | (s[1].cnt << 16);
+  w.cnt[1] = s[2].cnt | (s[3].cnt << 16);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) w.ref[c] = s[c].ref;
   out[widx[i]] = w;
 }
 
