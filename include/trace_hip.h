@@ -262,6 +262,8 @@ typedef struct {
   int32_t tuned;          /* 1 once an autotune call has run */
   int32_t trav_persistent; /* closest hit: 1 = persistent lanes refilled from a work counter,
                               0 = one ray per lane */
+  int32_t render_persistent; /* path segments: the same choice for the render's ray queues */
+  int32_t pad;
 } trace_tune;
 int trace_get_tune(trace_ctx* ctx, trace_tune* out);
 int trace_set_tune(trace_ctx* ctx, const trace_tune* tune);
@@ -270,8 +272,9 @@ int trace_set_tune(trace_ctx* ctx, const trace_tune* tune);
  * context, and reports the per-candidate times in ms (6 entries, in that order). */
 int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays,
                              int64_t n, float* h_ms);
-/* Times a render for each workgroup size and wavefront batch size (candidate batches
- * 2^20, 2^21, 2^22, 2^23 paths) and keeps the fastest; h_ms receives 3 + 4 times. */
+/* Times a render for each workgroup size, wavefront batch size (candidate batches
+ * 2^20, 2^21, 2^22, 2^23 paths) and queue mode (one segment per lane, persistent) in
+ * turn and keeps the fastest; h_ms receives 3 + 4 + 2 times. */
 int trace_autotune_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
                           const trace_camera* cam, const trace_render_params* prm, float* h_ms);
 

@@ -301,13 +301,14 @@ def test_autotune_keeps_results(ctx):
     sbvh = Bvh.lbvh(ctx, smesh)
     set_tune(ctx)
     rms = autotune_render(ctx, smesh, sbvh, sc["camera"], 64, 36, 2, 3, _lib.MISS_DEFAULT, 3)
-    assert rms.shape == (7,) and np.all(rms > 0)
+    assert rms.shape == (9,) and np.all(rms > 0)
     t = get_tune(ctx)
     assert t.render_threads in (64, 128, 256) and t.render_batch in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
     imgs = []
-    for threads in (64, 128, 256):
-        set_tune(ctx, render_threads=threads)
-        imgs.append(render(ctx, smesh, sbvh, sc["camera"], 64, 36, 2, 3, _lib.MISS_DEFAULT, 3))
+    for persistent in (False, True):
+        for threads in (64, 128, 256):
+            set_tune(ctx, render_threads=threads, render_persistent=persistent)
+            imgs.append(render(ctx, smesh, sbvh, sc["camera"], 64, 36, 2, 3, _lib.MISS_DEFAULT, 3))
     for other in imgs[1:]:
         for a, b in zip(imgs[0], other):
             assert np.array_equal(a, b)

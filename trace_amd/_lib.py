@@ -106,6 +106,8 @@ class Tune(C.Structure):
         ("render_batch", C.c_int64),
         ("tuned", C.c_int32),
         ("trav_persistent", C.c_int32),
+        ("render_persistent", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 

@@ -21,6 +21,7 @@ struct TuneState {
   int render_threads = 128;             // path-segment workgroup size: 64, 128 or 256
   int64_t render_batch = int64_t(1) << 22;  // paths per wavefront batch
   bool trav_persistent = true;          // closest hit: persistent lanes refilled from a counter
+  bool render_persistent = false;       // path segments: the same for the render's queues
   bool tuned = false;
 };
 

@@ -196,6 +196,7 @@ int trace_get_tune(trace_ctx* ctx, trace_tune* out) {
   out->render_batch = ctx->tune.render_batch;
   out->tuned = ctx->tune.tuned ? 1 : 0;
   out->trav_persistent = ctx->tune.trav_persistent ? 1 : 0;
+  out->render_persistent = ctx->tune.render_persistent ? 1 : 0;
   return TRACE_OK;
 }
 
@@ -209,6 +210,7 @@ int trace_set_tune(trace_ctx* ctx, const trace_tune* t) {
   ctx->tune.render_threads = t->render_threads;
   ctx->tune.render_batch = t->render_batch;
   ctx->tune.trav_persistent = t->trav_persistent != 0;
+  ctx->tune.render_persistent = t->render_persistent != 0;
   return TRACE_OK;
 }
 
@@ -315,6 +317,20 @@ int trace_autotune_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
     }
   }
   ctx->tune.render_batch = best_b;
+  bool best_p = ctx->tune.render_persistent;
+  best_ms = INFINITY;
+  for (int c = 0; c < 2; ++c) {
+    ctx->tune.render_persistent = c == 1;
+    float ms = 0;
+    int rc = time_ms(ctx, run, 2, &ms);
+    if (rc) return rc;
+    if (h_ms) h_ms[7 + c] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_p = c == 1;
+    }
+  }
+  ctx->tune.render_persistent = best_p;
   ctx->tune.tuned = true;
   return ctx_check_device_error(ctx, "trace_autotune_render");
 }

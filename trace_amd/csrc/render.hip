@@ -15,6 +15,8 @@
 //                   order (deterministic)
 // Forward accumulation of throughput x emission gives the same value as the
 // reference's recursion L = Le + albedo (x) L_next.
+#include <algorithm>
+
 #include "philox.hpp"
 #include "render.hpp"
 #include "traverse_dev.hpp"
@@ -232,151 +234,209 @@ __device__ inline float3 miss_color(int miss, float dx, float dy, float dz) {
   }
 }
 
-template <int THREADS, int STACK>
-__global__ __launch_bounds__(THREADS) void bounce_kernel(
-    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
-    const int32_t* __restrict__ perm, const Scene* __restrict__ scenes, int64_t pps,
-    const float* __restrict__ nrm, const int32_t* __restrict__ mat_idx,
-    const float4* __restrict__ mats, int32_t k, int32_t max_depth, int32_t miss,
-    int64_t path_base, float4* __restrict__ ro, float4* __restrict__ rd,
-    float4* __restrict__ thr, float4* __restrict__ rad, const int32_t* __restrict__ q_in,
-    int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
-    uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err, const Adj adj) {
-  constexpr int kThreads = THREADS;
-  __shared__ int32_t s_stack[traverse::kLdsStack * kThreads];
-  const uint32_t count = *count_in;
-  const uint32_t i = blockIdx.x * uint32_t(kThreads) + threadIdx.x;
-  bool push = false;
-  int32_t p = 0;
-  int32_t al_tri = -1, em_tri = -1;  // replay: triangles that receive a gradient
-  float4 al_w = make_float4(0.0f, 0.0f, 0.0f, 0.0f), em_w = al_w;
-  if (i < count) {
-    p = q_in[i];
-    const Scene& sc = scenes[p / pps];
-    const BvhHeader hdr = *sc.hdr;
-    const uint64_t seed = sc.seed;
-    const int64_t path_id = path_base + p % pps;
-    const float4 o4 = ro[p];
-    const float4 d4 = rd[p];
-    traverse::Ray ray{o4.x, o4.y, o4.z, k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
-    int32_t nn = 0, nt = 0;
-    bool overflow = false, bad = false;
-    traverse::Hit h = traverse::closest_hit_wide<STACK, false>(
-        wnodes, nodes, tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, kThreads);
-    if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
-    if (bad) atomicOr(d_err, TRACE_ERR_TREE);
-    traverse::refine_hit_fp64(tris, ray, h);
-    const float4 T = thr[p];
-    if (h.pos < 0) {
-      const float3 m = miss_color(miss, d4.x, d4.y, d4.z);
+// Everything the kernels below share about one path segment.
+struct SegArgs {
+  const float* tris;
+  const int32_t* perm;
+  const Scene* scenes;
+  int64_t pps;
+  const float* nrm;
+  const int32_t* mat_idx;
+  const float4* mats;
+  int32_t k, max_depth, miss;
+  int64_t path_base;
+  float4 *ro, *rd, *thr, *rad;
+};
+
+// Shades segment k of path p after its closest hit h (primitive.go:145-150,
+// render.go:209-246, material.go:13-38): miss, light or diffuse scatter, in
+// the mode adj names.  Sets push for a path that continues, and in replay
+// mode the triangles that receive a gradient (al_tri, em_tri) with weights.
+__device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p, float4 o4, float4 d4,
+                                     const traverse::Hit& h, bool& push, int32_t& al_tri, float4& al_w,
+                                     int32_t& em_tri, float4& em_w) {
+  const float* __restrict__ tris = g.tris;
+  const int32_t* __restrict__ perm = g.perm;
+  const float* __restrict__ nrm = g.nrm;
+  const int32_t* __restrict__ mat_idx = g.mat_idx;
+  const float4* __restrict__ mats = g.mats;
+  float4* __restrict__ ro = g.ro;
+  float4* __restrict__ rd = g.rd;
+  float4* __restrict__ thr = g.thr;
+  float4* __restrict__ rad = g.rad;
+  const int32_t k = g.k, max_depth = g.max_depth, miss = g.miss;
+  const Scene& sc = g.scenes[p / g.pps];
+  const uint64_t seed = sc.seed;
+  const int64_t path_id = g.path_base + p % g.pps;
+  const float4 T = thr[p];
+  if (h.pos < 0) {
+    const float3 m = miss_color(miss, d4.x, d4.y, d4.z);
+    if (adj.mode == kModePrimal) {
+      float4 L = rad[p];
+      L.x += T.x * m.x;
+      L.y += T.y * m.y;
+      L.z += T.z * m.z;
+      rad[p] = L;
+    } else if (adj.mode == kModeSummary) {
+      const uint32_t f = __float_as_uint(T.w) | ((m.x != 0.0f || m.y != 0.0f || m.z != 0.0f) ? kFlagRadiant : 0u);
+      rad[p] = make_float4(m.x * T.x, m.y * T.y, m.z * T.z, __uint_as_float(f));
+    }
+  } else if (k <= max_depth) {
+    const int32_t prim = perm[h.pos];
+    const float4 m = mats[mat_idx[prim]];
+    const float3 c = tri_colour(adj, m, prim);
+    if (m.x == float(TRACE_MAT_LIGHT)) {
       if (adj.mode == kModePrimal) {
         float4 L = rad[p];
-        L.x += T.x * m.x;
-        L.y += T.y * m.y;
-        L.z += T.z * m.z;
+        L.x += T.x * c.x;
+        L.y += T.y * c.y;
+        L.z += T.z * c.z;
         rad[p] = L;
       } else if (adj.mode == kModeSummary) {
-        const uint32_t f = __float_as_uint(T.w) | ((m.x != 0.0f || m.y != 0.0f || m.z != 0.0f) ? kFlagRadiant : 0u);
-        rad[p] = make_float4(m.x * T.x, m.y * T.y, m.z * T.z, __uint_as_float(f));
-      }
-    } else if (k <= max_depth) {
-      const int32_t prim = perm[h.pos];
-      const float4 m = mats[mat_idx[prim]];
-      const float3 c = tri_colour(adj, m, prim);
-      if (m.x == float(TRACE_MAT_LIGHT)) {
-        if (adj.mode == kModePrimal) {
-          float4 L = rad[p];
-          L.x += T.x * c.x;
-          L.y += T.y * c.y;
-          L.z += T.z * c.z;
-          rad[p] = L;
-        } else if (adj.mode == kModeSummary) {
-          const uint32_t f = __float_as_uint(T.w) | kFlagLight |
-                             ((c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) ? kFlagRadiant : 0u);
-          rad[p] = make_float4(c.x * T.x, c.y * T.y, c.z * T.z, __uint_as_float(f));
-        } else {
-          // dL/dEmission = product of the albedos before the light
-          em_tri = prim;
-          em_w = T;
-        }
+        const uint32_t f = __float_as_uint(T.w) | kFlagLight |
+                           ((c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) ? kFlagRadiant : 0u);
+        rad[p] = make_float4(c.x * T.x, c.y * T.y, c.z * T.z, __uint_as_float(f));
       } else {
-        // interpolated vertex normal, flipped to face the ray (primitive.go:145-150, 200-205)
-        const float* n = nrm + 9 * size_t(prim);
-        const float w0 = 1.0f - h.u - h.v;
-        float nx = (n[3] * h.u + n[6] * h.v) + n[0] * w0;
-        float ny = (n[4] * h.u + n[7] * h.v) + n[1] * w0;
-        float nz = (n[5] * h.u + n[8] * h.v) + n[2] * w0;
-        if (!h.front) {
-          nx = -nx;
-          ny = -ny;
-          nz = -nz;
-        }
-        // n + uniform unit vector (util.go:15-26 draws a uniform direction)
-        const Philox4 r = path_draw(seed, uint64_t(path_id), uint32_t(k + 1));
-        const float z = 1.0f - 2.0f * u01(r.v[0]);
-        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-        float sphi, cphi;
-        sincosf(6.2831853071795864f * u01(r.v[1]), &sphi, &cphi);
-        float sx = nx + rr * cphi, sy = ny + rr * sphi, sz = nz + z;
-        if (fabsf(sx) <= 1e-7f && fabsf(sy) <= 1e-7f && fabsf(sz) <= 1e-7f) {
-          sx = nx;
-          sy = ny;
-          sz = nz;
-        }
-        // new origin = ray.position(t) (camera.go:108-111), evaluated in FP64,
-        // then kept on the side of the triangle's plane the new ray leaves to
-        const float* tv = tris + 9 * size_t(h.pos);
-        const float e1x = tv[3] - tv[0], e1y = tv[4] - tv[1], e1z = tv[5] - tv[2];
-        const float e2x = tv[6] - tv[0], e2y = tv[7] - tv[1], e2z = tv[8] - tv[2];
-        float gx = e1y * e2z - e1z * e2y, gy = e1z * e2x - e1x * e2z, gz = e1x * e2y - e1y * e2x;
-        const float gl = sqrtf(gx * gx + gy * gy + gz * gz);
-        const float gs = (gx * sx + gy * sy + gz * sz) >= 0.0f ? 1.0f / gl : -1.0f / gl;
-        gx *= gs;
-        gy *= gs;
-        gz *= gs;
-        const float px = float(double(o4.x) + double(d4.x) * h.t64);
-        const float py = float(double(o4.y) + double(d4.y) * h.t64);
-        const float pz = float(double(o4.z) + double(d4.z) * h.t64);
-        ro[p] = make_float4(offset_coord(px, gx), offset_coord(py, gy), offset_coord(pz, gz), 0.0f);
-        rd[p] = make_float4(sx, sy, sz, 0.0f);
-        if (adj.mode == kModeSummary) {
-          // product of the non-zero albedos, and a count of the zero ones
-          uint32_t f = __float_as_uint(T.w);
-          float tc[3] = {T.x, T.y, T.z};
-          const float cc[3] = {c.x, c.y, c.z};
-#pragma unroll
-          for (int ch = 0; ch < 3; ++ch) {
-            if (cc[ch] != 0.0f) {
-              tc[ch] *= cc[ch];
-            } else {
-              const uint32_t z = (f >> (2 * ch)) & 3u;
-              if (z < 2u) f += 1u << (2 * ch);
-            }
-          }
-          thr[p] = make_float4(tc[0], tc[1], tc[2], __uint_as_float(f));
-        } else {
-          thr[p] = make_float4(T.x * c.x, T.y * c.y, T.z * c.z, 0.0f);
-        }
-        if (adj.mode == kModeReplay) {
-          // dL/da_j = S / a_j when the path has no zero albedo; with exactly
-          // one zero albedo only that vertex has a gradient, S itself
-          const float4 S = rad[p];
-          const uint32_t f = __float_as_uint(S.w);
-          const float sc3[3] = {S.x, S.y, S.z};
-          const float cc[3] = {c.x, c.y, c.z};
-          float gv[3];
-#pragma unroll
-          for (int ch = 0; ch < 3; ++ch) {
-            const uint32_t z = (f >> (2 * ch)) & 3u;
-            gv[ch] = cc[ch] != 0.0f ? (z == 0u ? sc3[ch] / cc[ch] : 0.0f) : (z == 1u ? sc3[ch] : 0.0f);
-          }
-          al_tri = prim;
-          al_w = make_float4(gv[0], gv[1], gv[2], 0.0f);
-        }
-        push = true;
+        // dL/dEmission = product of the albedos before the light
+        em_tri = prim;
+        em_w = T;
       }
+    } else {
+      // interpolated vertex normal, flipped to face the ray (primitive.go:145-150, 200-205)
+      const float* n = nrm + 9 * size_t(prim);
+      const float w0 = 1.0f - h.u - h.v;
+      float nx = (n[3] * h.u + n[6] * h.v) + n[0] * w0;
+      float ny = (n[4] * h.u + n[7] * h.v) + n[1] * w0;
+      float nz = (n[5] * h.u + n[8] * h.v) + n[2] * w0;
+      if (!h.front) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+      }
+      // n + uniform unit vector (util.go:15-26 draws a uniform direction)
+      const Philox4 r = path_draw(seed, uint64_t(path_id), uint32_t(k + 1));
+      const float z = 1.0f - 2.0f * u01(r.v[0]);
+      const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+      float sphi, cphi;
+      sincosf(6.2831853071795864f * u01(r.v[1]), &sphi, &cphi);
+      float sx = nx + rr * cphi, sy = ny + rr * sphi, sz = nz + z;
+      if (fabsf(sx) <= 1e-7f && fabsf(sy) <= 1e-7f && fabsf(sz) <= 1e-7f) {
+        sx = nx;
+        sy = ny;
+        sz = nz;
+      }
+      // new origin = ray.position(t) (camera.go:108-111), evaluated in FP64,
+      // then kept on the side of the triangle's plane the new ray leaves to
+      const float* tv = tris + 9 * size_t(h.pos);
+      const float e1x = tv[3] - tv[0], e1y = tv[4] - tv[1], e1z = tv[5] - tv[2];
+      const float e2x = tv[6] - tv[0], e2y = tv[7] - tv[1], e2z = tv[8] - tv[2];
+      float gx = e1y * e2z - e1z * e2y, gy = e1z * e2x - e1x * e2z, gz = e1x * e2y - e1y * e2x;
+      const float gl = sqrtf(gx * gx + gy * gy + gz * gz);
+      const float gs = (gx * sx + gy * sy + gz * sz) >= 0.0f ? 1.0f / gl : -1.0f / gl;
+      gx *= gs;
+      gy *= gs;
+      gz *= gs;
+      const float px = float(double(o4.x) + double(d4.x) * h.t64);
+      const float py = float(double(o4.y) + double(d4.y) * h.t64);
+      const float pz = float(double(o4.z) + double(d4.z) * h.t64);
+      ro[p] = make_float4(offset_coord(px, gx), offset_coord(py, gy), offset_coord(pz, gz), 0.0f);
+      rd[p] = make_float4(sx, sy, sz, 0.0f);
+      if (adj.mode == kModeSummary) {
+        // product of the non-zero albedos, and a count of the zero ones
+        uint32_t f = __float_as_uint(T.w);
+        float tc[3] = {T.x, T.y, T.z};
+        const float cc[3] = {c.x, c.y, c.z};
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          if (cc[ch] != 0.0f) {
+            tc[ch] *= cc[ch];
+          } else {
+            const uint32_t z = (f >> (2 * ch)) & 3u;
+            if (z < 2u) f += 1u << (2 * ch);
+          }
+        }
+        thr[p] = make_float4(tc[0], tc[1], tc[2], __uint_as_float(f));
+      } else {
+        thr[p] = make_float4(T.x * c.x, T.y * c.y, T.z * c.z, 0.0f);
+      }
+      if (adj.mode == kModeReplay) {
+        // dL/da_j = S / a_j when the path has no zero albedo; with exactly
+        // one zero albedo only that vertex has a gradient, S itself
+        const float4 S = rad[p];
+        const uint32_t f = __float_as_uint(S.w);
+        const float sc3[3] = {S.x, S.y, S.z};
+        const float cc[3] = {c.x, c.y, c.z};
+        float gv[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          const uint32_t z = (f >> (2 * ch)) & 3u;
+          gv[ch] = cc[ch] != 0.0f ? (z == 0u ? sc3[ch] / cc[ch] : 0.0f) : (z == 1u ? sc3[ch] : 0.0f);
+        }
+        al_tri = prim;
+        al_w = make_float4(gv[0], gv[1], gv[2], 0.0f);
+      }
+      push = true;
     }
   }
+}
+
+// Wave-staged append for the persistent kernel: continuing paths collect in
+// this wave's LDS slice and go to the next queue 64 at a time (one atomic per
+// 64 paths).  Every lane of the wave must take part.
+struct WaveQueue {
+  int32_t* buf;    // this wave's 128 LDS slots
+  uint32_t count;  // staged entries (the same in every lane)
+
+  __device__ inline void flush(uint32_t n, int32_t* __restrict__ q_out, uint32_t* __restrict__ count_out) {
+    const int lane = __lane_id();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count_out, n);
+    base = __shfl(base, 0, 64);
+    const int32_t v = buf[lane];
+    const int32_t rest = buf[64 + lane];
+    if (uint32_t(lane) < n) q_out[base + lane] = v;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (count > 64u && uint32_t(lane) < count - 64u) buf[lane] = rest;
+    count = count > n ? count - n : 0u;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  __device__ inline void push(bool want, int32_t p, int32_t* __restrict__ q_out,
+                              uint32_t* __restrict__ count_out) {
+    const uint64_t b = ballot64(want);
+    if (!b) return;
+    if (want) buf[count + __popcll(b & lanemask_lt())] = p;
+    count += uint32_t(__popcll(b));
+    if (count >= 64u) flush(64u, q_out, count_out);
+  }
+};
+
+// Replay-mode gradient scatter; every lane of the wave must take part.
+__device__ inline void segment_gradients(const Adj& adj, int64_t pps, int32_t p, int32_t al_tri, float4 al_w,
+                                         int32_t em_tri, float4 em_w) {
+  if (adj.mode == kModeReplay) {
+    // weight of the path: dLoss/d(mean image) at its pixel / spp
+    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    if (al_tri >= 0 || em_tri >= 0) {
+      const int64_t q = p % pps;
+      const float* g = adj.g_img + 3 * ((p / pps) * adj.wh + q % adj.wh);
+      wx = g[0] * adj.inv_spp;
+      wy = g[1] * adj.inv_spp;
+      wz = g[2] * adj.inv_spp;
+    }
+    wave_add3(adj.g_alb, al_tri, wx * al_w.x, wy * al_w.y, wz * al_w.z, al_tri >= 0);
+    wave_add3(adj.g_emi, em_tri, wx * em_w.x, wy * em_w.y, wz * em_w.z, em_tri >= 0);
+  }
+}
+
+// Replay-mode gradient scatter and the wave-aggregated append of continuing
+// paths to the next queue; every lane of the wave must take part.
+__device__ inline void segment_epilogue(const Adj& adj, int64_t pps, int32_t p, bool push, int32_t al_tri,
+                                        float4 al_w, int32_t em_tri, float4 em_w, int32_t* __restrict__ q_out,
+                                        uint32_t* __restrict__ count_out) {
   if (adj.mode == kModeReplay) {
     // weight of the path: dLoss/d(mean image) at its pixel / spp
     float wx = 0.0f, wy = 0.0f, wz = 0.0f;
@@ -471,22 +531,166 @@ int refresh_geo_normals(trace_ctx* ctx, trace_mesh* mesh) {
   return TRACE_OK;
 }
 
+template <int THREADS, int STACK>
+__global__ __launch_bounds__(THREADS) void bounce_kernel(
+    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const SegArgs g,
+    const int32_t* __restrict__ q_in, int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
+    uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err, const Adj adj) {
+  __shared__ int32_t s_stack[traverse::kLdsStack * THREADS];
+  const uint32_t count = *count_in;
+  const uint32_t i = blockIdx.x * uint32_t(THREADS) + threadIdx.x;
+  bool push = false;
+  int32_t p = 0;
+  int32_t al_tri = -1, em_tri = -1;  // replay: triangles that receive a gradient
+  float4 al_w = make_float4(0.0f, 0.0f, 0.0f, 0.0f), em_w = al_w;
+  if (i < count) {
+    p = q_in[i];
+    const BvhHeader hdr = *g.scenes[p / g.pps].hdr;
+    const float4 o4 = g.ro[p];
+    const float4 d4 = g.rd[p];
+    traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
+    int32_t nn = 0, nt = 0;
+    bool overflow = false, bad = false;
+    traverse::Hit h = traverse::closest_hit_wide<STACK, false>(
+        wnodes, nodes, g.tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, THREADS);
+    if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
+    if (bad) atomicOr(d_err, TRACE_ERR_TREE);
+    traverse::refine_hit_fp64(g.tris, ray, h);
+    shade_segment(g, adj, p, o4, d4, h, push, al_tri, al_w, em_tri, em_w);
+  }
+  segment_epilogue(adj, g.pps, p, push, al_tri, al_w, em_tri, em_w, q_out, count_out);
+}
+
+// Persistent variant (see closest_hit_persistent): a resident grid whose
+// waves claim chunks of the input queue from a counter and refill lanes as
+// their segments finish; shading, the gradient scatter and the append run in
+// batches when a quarter of the wave is idle.
+constexpr uint32_t kSegChunk = 256;
+constexpr int kSegRefillAt = 16;
+
+template <int THREADS, int STACK>
+__global__ __launch_bounds__(THREADS) void bounce_persistent(
+    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const SegArgs g,
+    const int32_t* __restrict__ q_in, int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
+    uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err, const Adj adj,
+    uint32_t* __restrict__ claim) {
+  __shared__ int32_t s_stack[traverse::kLdsStack * THREADS];
+  __shared__ int32_t s_queue[THREADS / 64][128];
+  WaveQueue wq{s_queue[threadIdx.x >> 6], 0u};
+  const int64_t count = *count_in;
+  const int lane = __lane_id();
+  const uint64_t lt = lanemask_lt();
+  traverse::TravStack<STACK> stack;
+  stack.lds = s_stack + threadIdx.x;
+  stack.stride = THREADS;
+  traverse::WideState s;
+  float4 o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = o4;
+  int32_t p = 0;
+  uint32_t n_bound = 0;
+  int32_t n_prims = 0;
+  bool active = false, pending = false, overflow = false, bad = false;
+  bool fetching = true;                  // the same in every lane of the wave
+  int64_t pool_next = 0, pool_end = 0;   // this wave's unclaimed queue slots
+  int32_t nn = 0, nt = 0;
+  while (true) {
+    const uint64_t need = ballot64(!active);
+    if (__popcll(need) >= kSegRefillAt || need == ~0ull) {
+      bool push = false;
+      int32_t al_tri = -1, em_tri = -1;
+      float4 al_w = make_float4(0.0f, 0.0f, 0.0f, 0.0f), em_w = al_w;
+      if (pending) {
+        traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
+        traverse::refine_hit_fp64(g.tris, ray, s.h);
+        shade_segment(g, adj, p, o4, d4, s.h, push, al_tri, al_w, em_tri, em_w);
+        pending = false;
+      }
+      segment_gradients(adj, g.pps, p, al_tri, al_w, em_tri, em_w);
+      wq.push(push, p, q_out, count_out);
+      if (pool_next >= pool_end && fetching) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(claim, kSegChunk);
+        base = __shfl(base, 0, 64);
+        if (int64_t(base) >= count) {
+          fetching = false;
+        } else {
+          pool_next = base;
+          pool_end = base + kSegChunk < uint64_t(count) ? int64_t(base) + kSegChunk : count;
+        }
+      }
+      const int64_t avail = pool_end - pool_next;
+      if (avail > 0) {
+        const int64_t rank = __popcll(need & lt);
+        if (!active && rank < avail) {
+          p = q_in[pool_next + rank];
+          const BvhHeader hdr = *g.scenes[p / g.pps].hdr;
+          n_bound = uint32_t(hdr.n_inner);
+          n_prims = hdr.n_prims;
+          o4 = g.ro[p];
+          d4 = g.rd[p];
+          traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
+          active = traverse::wide_begin<false>(g.tris, hdr, ray, s, nt, bad);
+          pending = !active;
+        }
+        pool_next += __popcll(need) < avail ? int64_t(__popcll(need)) : avail;
+      }
+    }
+    if (!ballot64(active)) {
+      if (!fetching && pool_next >= pool_end && !ballot64(pending)) break;
+      continue;
+    }
+    if (active) {
+      active = traverse::wide_step<STACK, false>(wnodes, nodes, g.tris, n_bound, n_prims, s, stack, nn, nt,
+                                                 overflow, bad);
+      pending = !active;
+    }
+  }
+  if (wq.count) wq.flush(wq.count, q_out, count_out);
+  if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
+  if (bad) atomicOr(d_err, TRACE_ERR_TREE);
+}
+
 template <int THREADS>
 static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, int grid,
                           const Scene* scenes, int64_t pps, int32_t k, const trace_render_params* prm,
                           int64_t path_base, float4* ro, float4* rd, float4* thr, float4* rad,
                           int32_t* qin, int32_t* qout, uint32_t* cin, uint32_t* cout, const Adj& adj) {
   hipStream_t st = ctx->stream;
-  if (bvh->depth_bound() > 63)
-    bounce_kernel<THREADS, 512><<<grid, THREADS, 0, st>>>(
-        bvh->wnodes_p(), bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
-        mesh->mats.p, k, prm->max_depth, prm->miss, path_base, ro, rd, thr, rad, qin, qout, cin,
-        cout, ctx->d_err, adj);
+  SegArgs g;
+  g.tris = bvh->tris_p();
+  g.perm = bvh->perm_p();
+  g.scenes = scenes;
+  g.pps = pps;
+  g.nrm = mesh->nrm.p;
+  g.mat_idx = mesh->mat_idx.p;
+  g.mats = mesh->mats.p;
+  g.k = k;
+  g.max_depth = prm->max_depth;
+  g.miss = prm->miss;
+  g.path_base = path_base;
+  g.ro = ro;
+  g.rd = rd;
+  g.thr = thr;
+  g.rad = rad;
+  const bool deep = bvh->depth_bound() > 63;
+  if (ctx->tune.render_persistent) {
+    // resident grid (up to 32 waves per CU); queue slots are claimed from a counter
+    const int pgrid = int(std::min<int64_t>(grid, int64_t(ctx->n_cu) * 32 * 64 / THREADS));
+    uint32_t* claim = ctx->d_work + 1;
+    fill_async<uint32_t>(st, claim, 1, 0u);
+    if (deep)
+      bounce_persistent<THREADS, 512><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), bvh->nodes_p(), g, qin, qout,
+                                                                 cin, cout, ctx->d_err, adj, claim);
+    else
+      bounce_persistent<THREADS, 128><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), bvh->nodes_p(), g, qin, qout,
+                                                                 cin, cout, ctx->d_err, adj, claim);
+    return;
+  }
+  if (deep)
+    bounce_kernel<THREADS, 512><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), bvh->nodes_p(), g, qin, qout, cin,
+                                                          cout, ctx->d_err, adj);
   else
-    bounce_kernel<THREADS, 128><<<grid, THREADS, 0, st>>>(
-        bvh->wnodes_p(), bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), scenes, pps, mesh->nrm.p, mesh->mat_idx.p,
-        mesh->mats.p, k, prm->max_depth, prm->miss, path_base, ro, rd, thr, rad, qin, qout, cin,
-        cout, ctx->d_err, adj);
+    bounce_kernel<THREADS, 128><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), bvh->nodes_p(), g, qin, qout, cin,
+                                                          cout, ctx->d_err, adj);
 }
 
 namespace {

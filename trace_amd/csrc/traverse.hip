@@ -55,6 +55,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
     uint32_t* d_err, uint32_t* work) {
   __shared__ int32_t s_stack[kLdsStack * THREADS];
   const BvhHeader hdr = *hdr_p;
+  const uint32_t n_bound = uint32_t(hdr.n_inner);
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
   TravStack<STACK> stack;
@@ -118,7 +119,8 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
       continue;
     }
     if (active) {
-      active = wide_step<STACK, COUNT>(wnodes, nodes, tris, hdr, s, stack, nn, nt, overflow, bad);
+      active = wide_step<STACK, COUNT>(wnodes, nodes, tris, n_bound, hdr.n_prims, s, stack, nn, nt,
+                                       overflow, bad);
       pending = !active;
     }
   }

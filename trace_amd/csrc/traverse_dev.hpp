@@ -11,8 +11,8 @@
 //     children[0]-first walk visits last, i.e. the one with the largest sorted
 //     position; the test below accepts an equal t only for a larger position,
 //     which reproduces that winner for any visiting order.
-// The walk itself is near-child first with a short per-thread stack of inner
-// node indices; leaves are tested as soon as their box is hit.
+// The walk runs over a 4-wide tree (WideNode), nearest child first, with a
+// per-lane stack of node indices; leaves are tested as soon as their box is hit.
 #pragma once
 
 #include "common.hpp"
@@ -158,74 +158,6 @@ struct TravStack {
   }
 };
 
-// `overflow` reports a full stack, `bad` a reference outside the arrays or a
-// walk longer than any valid tree allows (every node is visited at most once).
-// `lds_col` points at this lane's column of a kLdsStack x stride LDS array.
-template <int STACK, bool COUNT>
-__device__ inline Hit closest_hit_one(const BvhNode* __restrict__ nodes,
-                                      const float* __restrict__ tris, const BvhHeader& hdr,
-                                      const Ray& ray, int32_t& n_nodes, int32_t& n_tris,
-                                      bool& overflow, bool& bad, int32_t* lds_col, int stride) {
-  const RayPre r = ray_pre(ray);
-  Hit h;
-  h.t = ray.tmax;
-  h.t64 = ray.tmax;
-  h.pos = -1;
-  h.u = h.v = 0.0f;
-  h.front = true;
-  float tn;
-  if (!slab(r, hdr.root_lo[0], hdr.root_lo[1], hdr.root_lo[2], hdr.root_hi[0], hdr.root_hi[1],
-            hdr.root_hi[2], h.t, tn))
-    return h;
-  if (hdr.root_cnt > 0) {
-    leaf_test<COUNT>(r, tris, hdr.root_ref, hdr.root_cnt, hdr.n_prims, h, n_tris, bad);
-    return h;
-  }
-  TravStack<STACK> stack;
-  stack.lds = lds_col;
-  stack.stride = stride;
-  int sp = 0;
-  int32_t node = hdr.root_ref;
-  const uint32_t n_inner = uint32_t(hdr.n_inner);
-  for (uint32_t visits = 0;; ++visits) {
-    if (uint32_t(node) >= n_inner || visits > n_inner) {
-      bad = true;
-      break;
-    }
-    const float4* np = reinterpret_cast<const float4*>(nodes + node);
-    const float4 q0 = np[0], q1 = np[1], q2 = np[2];
-    const int4 q3 = *reinterpret_cast<const int4*>(np + 3);
-    if (COUNT) ++n_nodes;
-    float t0, t1;
-    bool h0 = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, h.t, t0);
-    bool h1 = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, h.t, t1);
-    if (h0 && q3.z > 0) {
-      leaf_test<COUNT>(r, tris, q3.x, q3.z, hdr.n_prims, h, n_tris, bad);
-      h0 = false;
-      h1 = h1 && t1 <= h.t * kSlabSlack;
-    }
-    if (h1 && q3.w > 0) {
-      leaf_test<COUNT>(r, tris, q3.y, q3.w, hdr.n_prims, h, n_tris, bad);
-      h1 = false;
-      h0 = h0 && t0 <= h.t * kSlabSlack;
-    }
-    if (h0 && h1) {
-      const bool first0 = t0 <= t1;
-      if (sp < STACK) stack.put(sp++, first0 ? q3.y : q3.x);
-      else overflow = true;
-      node = first0 ? q3.x : q3.y;
-    } else if (h0) {
-      node = q3.x;
-    } else if (h1) {
-      node = q3.y;
-    } else {
-      if (sp == 0) break;
-      node = stack.get(--sp);
-    }
-  }
-  return h;
-}
-
 // ---- wide tree (WideNode: 4 children, quantised boxes) -----------------------
 //
 // Traversal state of one lane, so that a ray can be advanced one node visit at
@@ -268,14 +200,14 @@ __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeade
 // once, then the surviving inner children are ordered by entry distance: the
 // nearest is visited next and the others are pushed farthest first.  Returns
 // false when the ray is done.
+// n_bound = the tree's binary inner-node count (wide nodes are at most as
+// many), n_prims = its triangle count: bounds for every reference followed.
 template <int STACK, bool COUNT>
 __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
                                  const BvhNode* __restrict__ bnodes,
-                                 const float* __restrict__ tris, const BvhHeader& hdr,
+                                 const float* __restrict__ tris, uint32_t n_bound, int32_t n_prims,
                                  WideState& s, TravStack<STACK>& stack, int32_t& n_nodes,
                                  int32_t& n_tris, bool& overflow, bool& bad) {
-  // wide nodes are at most as many as binary inner nodes
-  const uint32_t n_bound = uint32_t(hdr.n_inner);
   if (uint32_t(s.node) >= n_bound || ++s.visits > n_bound) {
     bad = true;
     return false;
@@ -344,7 +276,7 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
       first = (code & 1) ? rc.y : rc.x;
       count = (code & 1) ? rc.w : rc.z;
     }
-    leaf_test<COUNT>(r, tris, first, count, hdr.n_prims, s.h, n_tris, bad);
+    leaf_test<COUNT>(r, tris, first, count, n_prims, s.h, n_tris, bad);
   }
   // inner children still in reach, sorted by entry distance
   float key[4];
@@ -398,7 +330,8 @@ __device__ inline Hit closest_hit_wide(const WideNode* __restrict__ wnodes,
   stack.lds = lds_col;
   stack.stride = stride;
   if (wide_begin<COUNT>(tris, hdr, ray, s, n_tris, bad))
-    while (wide_step<STACK, COUNT>(wnodes, bnodes, tris, hdr, s, stack, n_nodes, n_tris, overflow, bad)) {
+    while (wide_step<STACK, COUNT>(wnodes, bnodes, tris, uint32_t(hdr.n_inner), hdr.n_prims, s, stack,
+                                   n_nodes, n_tris, overflow, bad)) {
     }
   return s.h;
 }

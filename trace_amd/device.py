@@ -354,10 +354,11 @@ def get_tune(ctx: Context) -> "_lib.Tune":
 
 
 def set_tune(ctx: Context, trav_threads: int = 64, render_threads: int = 128, render_batch: int = 1 << 22,
-             trav_persistent: bool = True) -> None:
+             trav_persistent: bool = True, render_persistent: bool = False) -> None:
     t = _lib.Tune()
     t.trav_threads, t.render_threads, t.render_batch = trav_threads, render_threads, render_batch
     t.trav_persistent = 1 if trav_persistent else 0
+    t.render_persistent = 1 if render_persistent else 0
     ctx.check(ctx.lib.trace_set_tune(ctx.h, C.byref(t)))
 
 
@@ -370,7 +371,7 @@ def autotune_traversal(ctx: Context, bvh: Bvh, d_rays: int, n: int) -> np.ndarra
 
 def autotune_render(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: int, spp: int, max_depth: int,
                     miss: int, seed: int) -> np.ndarray:
-    ms = np.zeros(7, np.float32)
+    ms = np.zeros(9, np.float32)
     c = camera_struct(cam)
     p = render_params(width, height, spp, max_depth, miss, seed)
     ctx.check(ctx.lib.trace_autotune_render(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), _ptr(ms)))
