@@ -501,18 +501,31 @@ __device__ inline float ld_f32(const float* p) {
 // Bottom-up refit: the leaf of distinct key u unions its triangles' boxes and
 // writes the result into its parent's child slot; the second child to arrive
 // at a node (per-node arrival counter, as bvh.go:206/219) unions both slots
-// and continues upwards.  Box words cross workgroups through agent-scope
-// atomics whose completion is awaited before the counter is bumped.
-__global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ boxes,
-                                                    const uint32_t* __restrict__ run_start,
-                                                    const int32_t* __restrict__ parent_inner,
-                                                    const int32_t* __restrict__ parent_leaf,
-                                                    BvhNode* nodes, uint32_t* counters,
-                                                    int32_t* heights, BvhHeader* hdr,
-                                                    uint32_t* d_err, const int32_t* __restrict__ useg,
-                                                    BvhHeader* seg_hdr, int32_t nseg) {
+// and continues upwards.
+//
+// Workgroup g handles leaves [256g, 256g + 256).  A node whose whole leaf range
+// lies inside that span (most nodes: all but the top few levels) can only be
+// reached by this workgroup, so its counter, child boxes and heights live in
+// LDS, indexed by the node's own key index (a Karras node's index is one end
+// of its range).  Nodes whose range crosses the span are met by two
+// workgroups: their box words cross through agent-scope atomics whose
+// completion is awaited before the global counter is bumped.
+constexpr int kRefitThreads = 256;
+
+__global__ __launch_bounds__(kRefitThreads) void refit_kernel(
+    const float* __restrict__ boxes, const uint32_t* __restrict__ run_start,
+    const int32_t* __restrict__ parent_inner, const int32_t* __restrict__ parent_leaf,
+    const int32_t* __restrict__ inner_range, BvhNode* nodes, uint32_t* counters, int32_t* heights,
+    BvhHeader* hdr, uint32_t* d_err, const int32_t* __restrict__ useg, BvhHeader* seg_hdr,
+    int32_t nseg) {
+  __shared__ uint32_t l_cnt[kRefitThreads];
+  __shared__ float l_box[kRefitThreads][2][6];
+  __shared__ int32_t l_h[kRefitThreads][2];
+  l_cnt[threadIdx.x] = 0u;
+  __syncthreads();
   const int32_t u = hdr->n_leaves;
-  const int32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t span0 = blockIdx.x * kRefitThreads;  // first leaf of this workgroup
+  const int32_t leaf = span0 + threadIdx.x;
   if (leaf >= u) return;
   if (!useg) {
     if (u == 1) {
@@ -554,6 +567,37 @@ __global__ __launch_bounds__(256) void refit_kernel(const float* __restrict__ bo
     }
     float* nb = reinterpret_cast<float*>(&nodes[node]);
     float* my = nb + 6 * slot;
+    const int32_t f = inner_range[2 * node], l = inner_range[2 * node + 1];
+    if (f >= span0 && l < span0 + kRefitThreads) {
+      // local node: key index = node index (+ segment number in a forest)
+      const int32_t key = node + (useg ? useg[f] : 0);
+      if (key < f || key > l) {  // not a Karras node: the tree is malformed
+        atomicOr(d_err, TRACE_ERR_TREE);
+        return;
+      }
+      const int li = key - span0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        my[a] = lo[a];
+        my[3 + a] = hi[a];
+        l_box[li][slot][a] = lo[a];
+        l_box[li][slot][3 + a] = hi[a];
+      }
+      l_h[li][slot] = h;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      const uint32_t prev = atomicAdd(&l_cnt[li], 1u);
+      if (prev == 0) return;  // sibling not done yet; it will continue
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fminf(lo[a], l_box[li][slot ^ 1][a]);
+        hi[a] = fmaxf(hi[a], l_box[li][slot ^ 1][3 + a]);
+      }
+      const int32_t ho = l_h[li][slot ^ 1];
+      h = 1 + (h > ho ? h : ho);
+      link = parent_inner[node];
+      continue;
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       st_f32(my + a, lo[a]);
@@ -717,10 +761,9 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   reorder_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, sperm, bvh->tris.p,
                                                    bvh->prim_box.p);
   fill_async<uint32_t>(st, counters, size_t(n), 0u);
-  refit_kernel<<<grid_for(n, 256), 256, 0, st>>>(bvh->prim_box.p, bvh->run_start.p, parent_inner,
-                                                 parent_leaf, bvh->nodes.p, counters, heights,
-                                                 bvh->header.p, ctx->d_err, useg,
-                                                 forest ? bvh->seg_headers.p : nullptr, nseg);
+  refit_kernel<<<grid_for(n, kRefitThreads), kRefitThreads, 0, st>>>(
+      bvh->prim_box.p, bvh->run_start.p, parent_inner, parent_leaf, bvh->inner_range.p, bvh->nodes.p,
+      counters, heights, bvh->header.p, ctx->d_err, useg, forest ? bvh->seg_headers.p : nullptr, nseg);
   if (forest)
     seg_finalize_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(u_begin, bvh->run_start.p, nseg,
                                                             bvh->header.p, bvh->seg_headers.p);
