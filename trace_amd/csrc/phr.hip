@@ -31,7 +31,7 @@
 
 namespace phr {
 
-constexpr int kLdsKeys = 2048;
+constexpr int kSortLds = 4096;  // cuts of 65 .. this size sort in LDS (bitonic); larger ones by ranking
 constexpr int kMaxDepth = 1024;
 
 struct Flat {  // LBVH as flat ids: 0..U-2 inner nodes, U-1..2U-2 leaves
@@ -50,16 +50,24 @@ struct Task {
   int32_t level;     // depth of the output node
 };
 
+// Level counters.  Every job of a level bumps several of them with device-scope
+// atomics; each sits on its own 128-byte line so that they do not serialise
+// behind one another.
+#define TRACE_CTL_FIELD(name) \
+  uint32_t name;              \
+  uint32_t name##_pad[31]
 struct Ctl {
-  uint32_t n_tasks;  // jobs appended for the next level
-  uint32_t pool;     // pool entries appended for the next level
-  uint32_t out;      // output nodes allocated
-  uint32_t list;     // leaf-list entries allocated
-  uint32_t max_level;
-  uint32_t n_leaves;
-  uint32_t err;      // 1 capacity, 2 depth table
-  uint32_t cut;      // initial cut size
+  TRACE_CTL_FIELD(n_tasks);    // jobs appended for the next level
+  TRACE_CTL_FIELD(pool);       // pool entries appended for the next level
+  TRACE_CTL_FIELD(out);        // output nodes allocated
+  TRACE_CTL_FIELD(list);       // leaf-list entries allocated
+  TRACE_CTL_FIELD(max_level);
+  TRACE_CTL_FIELD(n_leaves);
+  TRACE_CTL_FIELD(err);        // 1 capacity, 2 depth table
+  TRACE_CTL_FIELD(cut);        // initial cut size
+  TRACE_CTL_FIELD(max_m);      // largest cut among the jobs appended for the next level
 };
+#undef TRACE_CTL_FIELD
 
 // bounding.go:45-47, FP64 from FP32 box corners (exact differences)
 __device__ inline double sa_box(const float* b) {
@@ -218,6 +226,12 @@ __global__ __launch_bounds__(1024) void initial_cut_kernel(Flat f, double th0, i
   }
 }
 
+// atomicMax that skips the atomic when the counter already holds v or more
+// (most callers of a level find the maximum already there).
+__device__ inline void raise_max(uint32_t* c, uint32_t v) {
+  if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) atomicMax(c, v);
+}
+
 // ---- wave helpers -------------------------------------------------------------
 struct WBox {
   float lo[3], hi[3];
@@ -307,10 +321,73 @@ __device__ inline void load_member(const Flat& f, int32_t node, WBox& b, bool wi
 }
 
 // sort.SliceStable of the cut by box centre on `axis`, starting from order
-// `in` (identity when null).  Keys are exact FP64 centres.
+// `in` (identity when null).  Keys are exact FP64 centres.  Sorting the pairs
+// (centre, input position) makes every key distinct, so any correct sort gives
+// the stable order: a bitonic network in LDS (P = m rounded up to a power of
+// two, padded with +inf) for cuts that fit, rank by counting otherwise.
 __device__ void sort_axis(const Flat& f, const int32_t* ids, const int32_t* in, int32_t* out,
-                          int axis, int m, double* keys) {
+                          int axis, int m, double* keys, double* lkey, int32_t* lpos, int P) {
   const int lane = threadIdx.x;
+  if (m <= 64) {
+    // one pair per lane, bitonic network over lane shuffles
+    double key = INFINITY;
+    int32_t pos = lane;
+    if (lane < m) {
+      const int32_t j = in ? in[lane] : lane;
+      const float* b = f.box + 6 * size_t(ids[j]);
+      key = (double(b[axis]) + double(b[3 + axis])) * (1.0 / 2.0);
+    }
+    for (int k = 2; k <= 64; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const double ok = __shfl_xor(key, j, 64);
+        const int32_t op = __shfl_xor(pos, j, 64);
+        const bool other_less = ok < key || (ok == key && op < pos);
+        const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+        if (keep_min == other_less) {
+          key = ok;
+          pos = op;
+        }
+      }
+    }
+    if (lane < m) out[lane] = in ? in[pos] : pos;
+    __syncthreads();
+    return;
+  }
+  if (P > 0) {
+    for (int r = lane; r < P; r += 64) {
+      if (r < m) {
+        const int32_t j = in ? in[r] : r;
+        const float* b = f.box + 6 * size_t(ids[j]);
+        lkey[r] = (double(b[axis]) + double(b[3 + axis])) * (1.0 / 2.0);
+        lpos[r] = r;
+      } else {
+        lkey[r] = INFINITY;
+        lpos[r] = 0x7fffffff;
+      }
+    }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int t = lane; t < (P >> 1); t += 64) {
+          const int i = 2 * t - (t & (j - 1));  // lower index of pair (i, i + j)
+          const int l = i + j;
+          const double ka = lkey[i], kb = lkey[l];
+          const int32_t pa = lpos[i], pb = lpos[l];
+          const bool gt = ka > kb || (ka == kb && pa > pb);
+          if (gt == ((i & k) == 0)) {
+            lkey[i] = kb;
+            lkey[l] = ka;
+            lpos[i] = pb;
+            lpos[l] = pa;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int r = lane; r < m; r += 64) out[r] = in ? in[lpos[r]] : lpos[r];
+    __syncthreads();
+    return;
+  }
   for (int r = lane; r < m; r += 64) {
     const int32_t j = in ? in[r] : r;
     const float* b = f.box + 6 * size_t(ids[j]);
@@ -434,8 +511,7 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
       if (ls < caps.list) {
         list[ls] = ids[C[a]];
         write_out(out, o, 1, 0, -1, -1, ls, 1, prim_off, prim_cnt, t.level + 1, box);
-        atomicMax(&ctl->max_level, uint32_t(t.level + 1));
-        atomicAdd(&ctl->n_leaves, 1u);
+        raise_max(&ctl->max_level, uint32_t(t.level + 1));
       } else {
         atomicOr(&ctl->err, 1u);
       }
@@ -475,6 +551,7 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
     carry += tot;
   }
   if (lane == 0) {
+    raise_max(&ctl->max_m, k_total);
     Task n;
     n.off = poff;
     n.cnt = int32_t(k_total);
@@ -495,16 +572,20 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
                                                   Task* __restrict__ tasks_next,
                                                   int32_t* __restrict__ ordbuf, double* gkey,
                                                   double* rcost_buf, OutNode* __restrict__ out,
-                                                  int32_t* __restrict__ list, Ctl* ctl, Caps caps) {
-  __shared__ double s_key[kLdsKeys];
+                                                  int32_t* __restrict__ list, Ctl* ctl, Caps caps,
+                                                  int lds_p) {
+  extern __shared__ double s_dyn[];  // P keys then P positions (P = lds_p, 0 = no LDS sort)
   const int lane = threadIdx.x;
   const Task t = tasks[blockIdx.x];
   const int m = t.cnt;
+  const int P = m <= lds_p ? lds_p : 0;
+  double* lkey = s_dyn;
+  int32_t* lpos = reinterpret_cast<int32_t*>(s_dyn + lds_p);
   const int32_t* ids = pool + t.off;
   int32_t* oX = ordbuf + t.off;
   int32_t* oY = ordbuf + caps.pool + t.off;
   int32_t* oZ = ordbuf + 2 * caps.pool + t.off;
-  double* keys = m <= kLdsKeys ? s_key : gkey + t.off;
+  double* keys = gkey + t.off;
   double* rc = rcost_buf + t.off;
   if (t.depth >= kMaxDepth) {
     if (lane == 0) atomicOr(&ctl->err, 2u);
@@ -512,11 +593,11 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
   }
 
   // SweepSAH (phr.go:368-405)
-  sort_axis(f, ids, nullptr, oX, 0, m, keys);
-  sort_axis(f, ids, oX, oY, 1, m, keys);
+  sort_axis(f, ids, nullptr, oX, 0, m, keys, lkey, lpos, P);
+  sort_axis(f, ids, oX, oY, 1, m, keys, lkey, lpos, P);
   const Split x = min_cost(f, ids, oX, m, t.box, rc);
   const Split y = min_cost(f, ids, oY, m, t.box, rc);
-  sort_axis(f, ids, x.cost < y.cost ? oY : oX, oZ, 2, m, keys);
+  sort_axis(f, ids, x.cost < y.cost ? oY : oX, oZ, 2, m, keys, lkey, lpos, P);
   const Split z = min_cost(f, ids, oZ, m, t.box, rc);
   const int32_t* C;
   Split s;
@@ -579,15 +660,27 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
     if (lane == 0) {
       write_out(out, o, 1, 0, -1, -1, ls, m, t.prim_off, int32_t(pc), t.level + 1, t.box);
       write_out(out, t.out, 0, 1, o, -1, -1, 0, t.prim_off, int32_t(pc), t.level, t.box);
-      atomicMax(&ctl->max_level, uint32_t(t.level + 1));
-      atomicAdd(&ctl->n_leaves, 1u);
+      raise_max(&ctl->max_level, uint32_t(t.level + 1));
     }
   }
+}
+
+// Leaves of the output tree: one atomic per workgroup.
+__global__ __launch_bounds__(256) void count_leaves_kernel(const OutNode* __restrict__ out, int32_t n,
+                                                           Ctl* ctl) {
+  __shared__ uint32_t s_n;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < n && out[o].kind == 1) atomicAdd(&s_n, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_n) atomicAdd(&ctl->n_leaves, s_n);
 }
 
 __global__ void reset_level_kernel(Ctl* ctl) {
   ctl->n_tasks = 0;
   ctl->pool = 0;
+  ctl->max_m = 0;
 }
 
 // ---- layout of the traversal tree ------------------------------------------------
@@ -803,17 +896,28 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
 
   const Caps caps{pool_cap, out_cap, list_cap, task_cap};
   uint32_t n_tasks = 1;
+  uint32_t max_m = hc.cut;  // the one job of the first level holds the initial cut
   int cur = 0;
   int levels = 0;
   while (n_tasks > 0) {
+    // LDS for the bitonic sort, sized to the level's largest cut
+    // (cuts of up to 64 sort in registers; larger ones up to kSortLds in LDS,
+    // which every workgroup of the level then reserves)
+    int lds_p = 0;
+    if (max_m > 64u && max_m <= uint32_t(kSortLds)) {
+      lds_p = 128;
+      while (uint32_t(lds_p) < max_m) lds_p <<= 1;
+    }
+    const size_t lds_bytes = size_t(lds_p) * (sizeof(double) + sizeof(int32_t));
     reset_level_kernel<<<1, 1, 0, st>>>(ctl);
-    task_kernel<<<int(n_tasks), 64, 0, st>>>(f, d_th, tasks[cur], pools[cur], pools[cur ^ 1],
-                                            tasks[cur ^ 1], ord, gkey, rcost, outn, list, ctl,
-                                            caps);
+    task_kernel<<<int(n_tasks), 64, lds_bytes, st>>>(f, d_th, tasks[cur], pools[cur], pools[cur ^ 1],
+                                                    tasks[cur ^ 1], ord, gkey, rcost, outn, list, ctl,
+                                                    caps, lds_p);
     rc = read_ctl(ctx, ctl, &hc);
     if (rc) return rc;
     if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x at level %d", hc.err, levels);
     n_tasks = hc.n_tasks;
+    max_m = hc.max_m;
     cur ^= 1;
     ++levels;
   }
@@ -822,6 +926,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   out->phr_levels = levels;
 
   branch_flag_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag);
+  count_leaves_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, ctl);
   scan::exclusive_u32(st, flag, tidx, n_out, n_trav, sscr);
   uint32_t h_ntrav = 0;
   HIP_TRY(ctx, hipMemcpyAsync(&h_ntrav, n_trav, 4, hipMemcpyDeviceToHost, st));
