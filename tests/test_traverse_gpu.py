@@ -154,3 +154,28 @@ def test_axis_parallel_rays_stay_cheap(ctx, oracle):
     hit, t = bvh.closest_hit(rays)
     ohit, ot = oracle.Tree(pos).closest_hit(rays)
     _check(hit, t, ohit, ot, min_hits=0.0)
+
+
+def test_leaf_larger_than_wide_count_field(ctx, oracle):
+    """70,000 copies of one triangle share one Morton key, so the LBVH has a
+    leaf of 70,000 triangles -- beyond the 16-bit count of a wide-tree child,
+    which then refers back to the binary node.  Hits (the tie goes to the last
+    copy) and t match the oracle."""
+    print("traverse huge leaf", flush=True)
+    base = scenes.random_triangles(2000, seed=27, mean_edge=0.02)
+    dup = np.repeat(base[:1], 70_000, axis=0)
+    pos = np.concatenate([base[:1000], dup, base[1000:]])
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    c = dup[0].reshape(3, 3).mean(0)
+    rng = np.random.default_rng(28)
+    n = 4000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = c + rng.normal(0, 0.3, (n, 3))
+    rays[:, 4:7] = (c + rng.normal(0, 0.002, (n, 3))) - rays[:, :3]
+    rays[:, 3] = 0.001
+    rays[:, 7] = np.inf
+    rays = np.concatenate([rays, scenes.incoherent_rays(4000, seed=29)])
+    hit, t = bvh.closest_hit(rays)
+    ohit, ot = oracle.Tree(pos).closest_hit(rays)
+    _check(hit, t, ohit, ot, min_hits=0.3)
+    assert np.any((hit >= 1000) & (hit < 71_000))  # the big leaf is hit
