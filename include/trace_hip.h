@@ -219,6 +219,15 @@ int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
                           const trace_camera* cam, const trace_render_params* prm,
                           const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission);
 
+/* One Adam step (Kingma & Ba 2015) on n device parameters, in place: m, v are the
+ * caller's moment buffers (zero before step 1), step counts from 1; the result is
+ * clamped to [lo, hi] (albedos stay in [0, 1]).  Asynchronous.  The reference's
+ * optimiser (optimizer.go:44-155) searches PHR parameters instead; this drives the
+ * inverse-rendering loop over per-triangle albedo. */
+int trace_adam_step(trace_ctx* ctx, float* d_param, const float* d_grad, float* d_m, float* d_v,
+                    int64_t n, float lr, float beta1, float beta2, float eps, int32_t step, float lo,
+                    float hi);
+
 /* ---- batching, graphs, tuning (no reference counterpart) ------------------- */
 /* nscene images in one wavefront: scene c traces tree h_seg_ids[c] of a forest (built
  * from mesh, a segmented mesh) with camera h_cams[c] and the Philox stream of

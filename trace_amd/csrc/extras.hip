@@ -37,6 +37,22 @@ __global__ void animate_kernel(const double* __restrict__ rest, double* __restri
 
 __global__ void frame_step_kernel(uint32_t* d_frame) { *d_frame += 1u; }
 
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, int64_t n,
+                                                   float lr, float b1, float b2, float eps, float c1,
+                                                   float c2, float lo, float hi) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / c1;  // bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t
+  const float vh = vi / c2;
+  p[i] = fminf(fmaxf(p[i] - lr * mh / (sqrtf(vh) + eps), lo), hi);
+}
+
 void unit_from(double a, double b, double out[3]) {
   const double z = 1.0 - 2.0 * a;
   const double r = std::sqrt(std::max(0.0, 1.0 - z * z));
@@ -126,6 +142,22 @@ int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_backward: image too large");
   const trace_bvh* one[1] = {bvh};
   return render::backward_scenes(ctx, mesh, one, 1, cam, prm, d_grad_image, d_grad_albedo, d_grad_emission);
+}
+
+int trace_adam_step(trace_ctx* ctx, float* d_param, const float* d_grad, float* d_m, float* d_v,
+                    int64_t n, float lr, float beta1, float beta2, float eps, int32_t step, float lo,
+                    float hi) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (n < 0 || (n > 0 && (!d_param || !d_grad || !d_m || !d_v)) || step < 1 || !(lr > 0) ||
+      !(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1) || !(eps > 0) || !(lo <= hi))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_adam_step: bad arguments");
+  if (n == 0) return TRACE_OK;
+  const float c1 = float(1.0 - std::pow(double(beta1), double(step)));
+  const float c2 = float(1.0 - std::pow(double(beta2), double(step)));
+  adam_kernel<<<int(ceil_div(n, 256)), 256, 0, ctx->stream>>>(d_param, d_grad, d_m, d_v, n, lr, beta1, beta2,
+                                                             eps, c1, c2, lo, hi);
+  HIP_TRY(ctx, hipGetLastError());
+  return TRACE_OK;
 }
 
 int trace_mesh_export(trace_ctx* ctx, const trace_mesh* mesh, double* h_pos) {

@@ -301,6 +301,14 @@ def render_backward(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height:
                                             C.c_void_p(d_grad_albedo), C.c_void_p(d_grad_emission)))
 
 
+def adam_step(ctx: Context, d_param: int, d_grad: int, d_m: int, d_v: int, n: int, lr: float, step: int,
+              beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, lo: float = 0.0,
+              hi: float = 1.0) -> None:
+    """trace_adam_step (asynchronous, in place on device buffers of n floats)."""
+    ctx.check(ctx.lib.trace_adam_step(ctx.h, C.c_void_p(d_param), C.c_void_p(d_grad), C.c_void_p(d_m),
+                                      C.c_void_p(d_v), n, lr, beta1, beta2, eps, step, lo, hi))
+
+
 def render_batched(ctx: Context, mesh: Mesh, forest: Bvh, seg_ids, cams, width: int, height: int, spp: int,
                    max_depth: int, miss: int, seed: int, d_sum: int, d_sumsq: int, d_last: int | None = None) -> None:
     """trace_render_batched: scene c traces segment seg_ids[c] with cams[c] and seed + c."""

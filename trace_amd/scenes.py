@@ -211,6 +211,69 @@ def adjoint_scene(ntri: int = 200_000, seed: int = 7, width: int = 512, height: 
             "emission": emission, "camera": base["camera"], "width": width, "height": height}
 
 
+def fbm_heightfield(n: int = 2048, octaves: int = 8, seed: int = 5, base_freq: int = 4,
+                    gain: float = 0.5) -> np.ndarray:
+    """n x n fractional Brownian motion: `octaves` layers of smoothstep-interpolated
+    value noise on lattices of base_freq * 2^o cells, amplitude gain^o (seeded).
+    float32, scaled to [0, 0.35]."""
+    rng = np.random.default_rng(seed)
+    h = np.zeros((n, n), np.float64)
+    amp, freq = 1.0, base_freq
+    for _ in range(octaves):
+        lat = rng.standard_normal((freq + 1, freq + 1))
+        x = np.linspace(0.0, freq, n)
+        i = np.minimum(x.astype(np.int64), freq - 1)
+        f = x - i
+        f = f * f * (3.0 - 2.0 * f)
+        a = lat[i][:, i] * (1 - f)[None, :] + lat[i][:, i + 1] * f[None, :]
+        b = lat[i + 1][:, i] * (1 - f)[None, :] + lat[i + 1][:, i + 1] * f[None, :]
+        h += amp * (a * (1 - f)[:, None] + b * f[:, None])
+        amp *= gain
+        freq *= 2
+    h = (h - h.min()) / (h.max() - h.min()) * 0.35
+    return h.astype(np.float32)
+
+
+def terrain_scene(n: int = 2048, octaves: int = 8, seed: int = 5, n_spheres: int = 5000,
+                  width: int = 1920, height: int = 1080):
+    """The large configuration: an n x n fBm heightfield over [-1, 1]^2 (2 (n-1)^2
+    triangles, first in the buffer) plus n_spheres copies of one 400-triangle
+    sphere (radius 0.006 .. 0.02, seeded positions resting on the terrain).
+    Terrain triangles carry a ground-truth albedo ~U(0.1, 0.9) (seed); spheres
+    are diffuse 0.8.  Vertices are FP32-exact.  Returns pos, mat_idx,
+    materials, albedo, emission, camera, n_terrain, width, height."""
+    hf = fbm_heightfield(n, octaves, seed).astype(np.float64)
+    g = np.linspace(-1.0, 1.0, n)
+    X, Z = np.meshgrid(g, g)
+    V = np.stack([X, hf, Z], axis=-1)  # [row (z), col (x)]
+    a, b = V[:-1, :-1], V[:-1, 1:]
+    c, d = V[1:, 1:], V[1:, :-1]
+    t1 = np.concatenate([a, d, c], axis=-1).reshape(-1, 9)
+    t2 = np.concatenate([a, c, b], axis=-1).reshape(-1, 9)
+    terrain = np.stack([t1, t2], axis=1).reshape(-1, 9)
+    rng = np.random.default_rng(seed + 1)
+    unit = uv_sphere((0.0, 0.0, 0.0), 1.0, 20, 12)  # 400 triangles
+    r = rng.uniform(0.006, 0.02, n_spheres)
+    px = rng.uniform(-0.95, 0.95, n_spheres)
+    pz = rng.uniform(-0.95, 0.95, n_spheres)
+    col = np.clip(np.round((px + 1.0) / 2.0 * (n - 1)).astype(np.int64), 0, n - 1)
+    row = np.clip(np.round((pz + 1.0) / 2.0 * (n - 1)).astype(np.int64), 0, n - 1)
+    py = hf[row, col] + r
+    centre = np.stack([px, py, pz], axis=1)
+    spheres = (unit.reshape(1, -1, 3, 3) * r[:, None, None, None] + centre[:, None, None, :]).reshape(-1, 9)
+    pos = np.concatenate([terrain, spheres]).astype(np.float32).astype(np.float64)
+    n_terrain = len(terrain)
+    mat_idx = np.concatenate([np.zeros(n_terrain, np.int32), np.ones(len(spheres), np.int32)])
+    materials = np.array([[MAT_DIFFUSE, 0.5, 0.5, 0.5], [MAT_DIFFUSE, 0.8, 0.8, 0.8]], np.float32)
+    albedo = np.empty((len(pos), 3), np.float32)
+    albedo[:n_terrain] = np.random.default_rng(seed + 2).uniform(0.1, 0.9, (n_terrain, 3))
+    albedo[n_terrain:] = 0.8
+    emission = np.zeros((len(pos), 3), np.float32)
+    cam = Camera(width / height, 40.0, (0.0, 1.1, 2.3), (0.0, 0.1, 0.0), (0.0, 1.0, 0.0))
+    return {"pos": pos, "mat_idx": mat_idx, "materials": materials, "albedo": albedo, "emission": emission,
+            "camera": cam, "n_terrain": n_terrain, "width": width, "height": height}
+
+
 def parse_obj(path: str) -> tuple[np.ndarray, np.ndarray | None]:
     """OBJ triangles as obj.go:19-106 reads them: 'v', 'vn', 'f' (fan
     triangulation, negative indices relative to the current count, normals only
