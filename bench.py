@@ -14,7 +14,9 @@ The line also carries closest hit on the LBVH, coherent rays, the LBVH build
 kernel, the FP64 CPU oracle timed on a sample, and the other workloads: the
 small render (640x360, 16 spp), batched builds and renders (64 meshes, 32
 scenes), a 120-frame animation under one HIP graph, and the adjoint/primal
-cost of the differentiation workload (200k triangles, 512x512, 32 spp).
+cost of the differentiation workload (200k triangles, 512x512, 32 spp), and
+the large configuration (10.4M triangles, PHR tree, 1920x1080, 64 spp, depth
+8, Adam steps of inverse rendering).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -277,6 +279,84 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "closest_hit_incoherent_1M_traffic
 COPY_BYTES = 1 << 30  # reference copy kernel: reads and writes 1 GiB
 
 
+def bench_large(torch, ctx, steps=2):
+    """The large configuration: 10.38M triangles (2048x2048 fBm terrain + 5000
+    400-triangle spheres), PHR tree, 1920x1080, 64 spp, depth 8, sky miss;
+    inverse rendering of the per-triangle terrain albedo with Adam (lr 1e-2)
+    from 0.5 towards a target rendered from the ground truth, L2 image loss.
+    Times the builds, one primal render and `steps` optimisation steps
+    (primal + backward + Adam)."""
+    from trace_amd import _lib, scenes
+    from trace_amd.device import Bvh, Mesh, adam_step, render_backward, render_device, render_segments
+
+    d = scenes.terrain_scene()
+    n, nt = len(d["pos"]), d["n_terrain"]
+    W, H, spp, depth, seed = d["width"], d["height"], 64, 8, 9
+    mesh = Mesh(ctx, d["pos"])
+    mesh.set_shading(d["mat_idx"], d["materials"])
+    ev = _events(torch, 4)
+    ev[0].record()
+    lbvh = Bvh.lbvh(ctx, mesh)
+    ev[1].record()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    phr = Bvh.phr(ctx, lbvh)
+    ctx.synchronize()
+    phr_ms = (time.perf_counter() - t0) * 1e3
+    true_alb = torch.from_numpy(d["albedo"]).cuda()
+    emi = torch.from_numpy(d["emission"]).cuda()
+    alb = true_alb.clone()
+    alb[:nt] = 0.5
+    m = torch.zeros_like(alb)
+    v = torch.zeros_like(alb)
+    img = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    sq = torch.empty_like(img)
+    ga = torch.empty_like(alb)
+    ge = torch.empty_like(alb)
+    torch.cuda.synchronize()
+    mesh.bind_params(true_alb.data_ptr(), emi.data_ptr())
+    ev[2].record()
+    render_device(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed, img.data_ptr(), sq.data_ptr())
+    ev[3].record()
+    torch.cuda.synchronize()
+    render_rays = render_segments(ctx)
+    target = img / spp
+    mesh.bind_params(alb.data_ptr(), emi.data_ptr())
+    losses, step_ms, step_rays = [], [], []
+    for k in range(1, steps + 1):
+        note(f"large: Adam step {k}")
+        e0, e1, e2 = _events(torch, 3)
+        e0.record()
+        render_device(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed, img.data_ptr(),
+                      sq.data_ptr())
+        e1.record()
+        diff = img / spp - target
+        g_img = (2.0 / diff.numel()) * diff
+        render_backward(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed, g_img.data_ptr(),
+                        ga.data_ptr(), ge.data_ptr())
+        adam_step(ctx, alb.data_ptr(), ga.data_ptr(), m.data_ptr(), v.data_ptr(), 3 * nt, 1e-2, k)
+        e2.record()
+        torch.cuda.synchronize()
+        # rays of the primal render (the same paths as the target's) and of both backward sweeps
+        step_rays.append(render_rays + render_segments(ctx))
+        losses.append(float((diff * diff).mean()))
+        step_ms.append(e0.elapsed_time(e2))
+    ctx.synchronize()
+    render_ms = ev[2].elapsed_time(ev[3])
+    out = {"tris": n, "terrain_tris": nt, "image": [W, H], "spp": spp, "max_depth": depth,
+           "lbvh_build_ms": round(ev[0].elapsed_time(ev[1]), 3), "phr_build_ms": round(phr_ms, 2),
+           "render_ms": round(render_ms, 2), "render_rays": render_rays,
+           "render_mrays_s": round(render_rays / (render_ms * 1e3), 1),
+           "samples_per_s": round(W * H * spp / (render_ms * 1e-3), 1),
+           "adam_steps": steps, "step_ms": [round(x, 2) for x in step_ms], "step_rays": step_rays,
+           "step_mrays_s": [round(r / (t * 1e3), 1) for r, t in zip(step_rays, step_ms)], "loss": losses}
+    mesh.bind_params(None, None)
+    phr.close()
+    lbvh.close()
+    mesh.close()
+    return out
+
+
 def load_traffic():
     if os.path.exists(TRAFFIC_JSON):
         with open(TRAFFIC_JSON) as f:
@@ -355,6 +435,7 @@ def main():
     ap.add_argument("--rays", type=int, default=8_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick", action="store_true", help="skip the extra measurements")
+    ap.add_argument("--no-large", action="store_true", help="skip the 10.4M-triangle inverse-rendering leg")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run only the counter workload (copy kernel + closest hit) and exit")
     ap.add_argument("--traffic-from", nargs="+", metavar="CSV",
@@ -499,6 +580,9 @@ def main():
         note("adjoint")
         out["adjoint"] = bench_adjoint(torch, ctx, max(3, args.steps // 4))
         out["adjoint_over_primal"] = out["adjoint"]["adjoint_over_primal"]
+        if not args.no_large:
+            note("large scene (10.4M triangles) and inverse-rendering steps")
+            out["large"] = bench_large(torch, ctx)
 
     if not args.no_cpu:
         note("CPU baseline")

@@ -201,6 +201,12 @@ int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh*
                         const trace_camera* cam, const trace_render_params* prm, float* d_sum,
                         float* d_sumsq, float* d_last);
 
+/* Path segments (rays traced, every bounce) of the context's last trace_render*,
+ * trace_render_batched or trace_render_backward call (the backward counts both of its
+ * sweeps).  Waits for the context stream.  No reference counterpart: the reference's
+ * renderer keeps no ray count; this gives the rays-per-second figure of a render. */
+int trace_render_segments(trace_ctx* ctx, int64_t* out);
+
 /* ---- differentiation (no reference counterpart) ----------------------------- */
 /* Bind per-triangle parameters: d_albedo replaces the colour of Diffuse triangles
  * (Diffuse.Albedo, material.go:25-38) and d_emission that of Light triangles

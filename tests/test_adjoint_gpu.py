@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from trace_amd import _lib, scenes
-from trace_amd.device import Bvh, Mesh, render, render_backward, render_device
+from trace_amd.device import Bvh, Mesh, render, render_backward, render_device, render_segments
 
 pytestmark = pytest.mark.gpu
 
@@ -165,10 +165,15 @@ def test_furnace_gradient_sum(ctx):
     sc.bind(np.full((n, 3), a, np.float32), np.zeros((n, 3), np.float32))
     w, h, spp = 96, 64, 4
     mean = sc.mean(w, h, spp, 4, _lib.MISS_WHITE, 21)
+    primal_segments = render_segments(ctx)
     ga, ge = sc.grad(w, h, spp, 4, _lib.MISS_WHITE, 21, np.ones((h, w, 3)))
     for c in range(3):
         assert ga[:, c].sum() == pytest.approx((w * h - mean[..., c].sum()) / (1 - a), rel=1e-4)
     assert np.all(ge == 0) and np.all(ga >= 0)
+    # every path ends in the white sky and carries a gradient: the replay
+    # sweep retraces each one, so the backward traces twice the primal's rays
+    assert primal_segments > w * h * spp
+    assert render_segments(ctx) == 2 * primal_segments
 
 
 def test_direct_light_emission_gradient(ctx):

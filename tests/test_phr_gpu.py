@@ -102,6 +102,34 @@ def test_phr_traversal_and_render(ctx, oracle):
     assert frac < 0.01
 
 
+def test_phr_terrain_big_leaves_split(ctx, oracle):
+    """On a heightfield, PHR's makeLeaf flattens whole LBVH subtrees into
+    leaves of hundreds of triangles (phr.go:313-352); the tree still equals
+    the oracle's.  The 4-wide traversal tree splits those leaves: closest
+    hits equal the oracle's, and triangle tests per ray stay close to the
+    LBVH's instead of growing with the leaf size."""
+    print("phr terrain big leaves", flush=True)
+    d = scenes.terrain_scene(n=384, n_spheres=200, width=192, height=108)
+    pos = d["pos"]
+    _, aux, phr, otree, _ = _compare(ctx, oracle, pos)
+    rec, _ = phr.canonical()
+    leaf = rec[:, 0] == 1
+    assert (rec[leaf, 2] - rec[leaf, 1] + 1).max() > 64  # the premise: big PHR leaves
+    lo, hi = pos.reshape(-1, 3).min(0), pos.reshape(-1, 3).max(0)
+    inc = scenes.incoherent_rays(20_000, seed=31).astype(np.float32)
+    inc[:, :3] = lo + inc[:, :3] * (hi - lo)
+    rays = np.concatenate([scenes.camera_rays(d["camera"], 192, 108).astype(np.float32), inc])
+    hit, t = phr.closest_hit(rays)
+    ohit, ot = otree.closest_hit(rays)
+    assert np.mean(hit == ohit) >= 0.9999
+    both = (hit == ohit) & (hit >= 0)
+    assert both.sum() > 10_000
+    assert np.max(np.abs(t[both] - ot[both]) / ot[both]) < 1e-5
+    _, tp = phr.counts(rays)
+    _, tl = aux.counts(rays)
+    assert tp.mean() <= 2.0 * tl.mean() + 4.0, (tp.mean(), tl.mean())
+
+
 def test_phr_rejects_unsupported(ctx):
     pos = scenes.random_triangles(100, seed=1)
     aux = Bvh.lbvh(ctx, Mesh(ctx, pos))

@@ -46,6 +46,29 @@ def test_furnace_sphere(ctx, oracle):
     assert np.mean(np.abs(s - os_) < 1e-5) > 0.995
 
 
+def test_segment_count_of_furnace_sphere(ctx):
+    """trace_render_segments: at 1 spp a camera ray that misses the convex
+    sphere is one segment (value 1), one that hits it is two (the bounce
+    leaves the sphere: value a); the count equals that sum exactly."""
+    from trace_amd.device import render_segments
+
+    print("render segment count", flush=True)
+    pos = scenes.uv_sphere((0, 0, 0), 1.0, 24, 14).astype(np.float32).astype(np.float64)
+    mi = np.zeros(len(pos), np.int32)
+    a = np.float32(0.6)
+    mats = np.array([[scenes.MAT_DIFFUSE, a, a, a]], np.float32)
+    cam = scenes.Camera(2.0, 60.0, (0, 0, 3.0), (0, 0, 0), (0, 1, 0))
+    mesh, bvh = _setup(ctx, pos, mi, mats)
+    _, _, last = render(ctx, mesh, bvh, cam, 128, 64, 1, 5, _lib.MISS_WHITE, seed=7)
+    n_miss = int(np.count_nonzero(np.all(last == 1.0, axis=-1)))
+    n_hit = int(np.count_nonzero(np.all(last == a, axis=-1)))
+    assert n_miss + n_hit == 128 * 64 and n_hit > 1000
+    assert render_segments(ctx) == n_miss + 2 * n_hit
+    # a second call restarts the count
+    render(ctx, mesh, bvh, cam, 128, 64, 1, 5, _lib.MISS_WHITE, seed=7)
+    assert render_segments(ctx) == n_miss + 2 * n_hit
+
+
 def test_depth_rule_value_set(ctx, oracle):
     print("render depth rule", flush=True)
     sc = scenes.box_scene(seed=3, slices=16, rings=10, width=96, height=54)

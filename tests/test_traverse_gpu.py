@@ -158,9 +158,9 @@ def test_axis_parallel_rays_stay_cheap(ctx, oracle):
 
 def test_leaf_larger_than_wide_count_field(ctx, oracle):
     """70,000 copies of one triangle share one Morton key, so the LBVH has a
-    leaf of 70,000 triangles -- beyond the 16-bit count of a wide-tree child,
-    which then refers back to the binary node.  Hits (the tie goes to the last
-    copy) and t match the oracle."""
+    leaf of 70,000 triangles -- beyond the 16-bit count of a wide-tree child;
+    the widening pass splits it into a 4-ary sub-tree of small leaves.  Hits
+    (the tie goes to the last copy) and t match the oracle."""
     print("traverse huge leaf", flush=True)
     base = scenes.random_triangles(2000, seed=27, mean_edge=0.02)
     dup = np.repeat(base[:1], 70_000, axis=0)

@@ -96,6 +96,7 @@ SIGNATURES = {
     "trace_autotune_render": [_P, _P, _P, _P, _P, _P],
     "trace_mesh_bind_params": [_P, _P, _P, _P],
     "trace_render_backward": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "trace_render_segments": [_P, _P],
     "trace_adam_step": [_P, _P, _P, _P, _P, _I64, C.c_float, C.c_float, C.c_float, C.c_float, _I32, C.c_float,
                         C.c_float],
 }

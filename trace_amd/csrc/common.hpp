@@ -149,8 +149,9 @@ struct BvhHeader {
   int32_t n_prims;
   double enc_lo[3];   // FP64 scene box (union of triangle boxes)
   double enc_hi[3];
-  int32_t wroot;      // root of the 4-wide traversal tree (-1: the root is a leaf)
-  int32_t wpad[3];
+  int32_t wroot;      // root of the 4-wide traversal tree (-1: the root is a leaf kept whole)
+  int32_t n_wide;     // wide nodes of the traversal tree (a bound for every wide index)
+  int32_t wpad[2];
 };
 
 // Node of the 4-wide traversal tree (64 bytes, one load per visit).  Child
@@ -158,8 +159,8 @@ struct BvhHeader {
 // spans [o_a + qlo * s_a, o_a + qhi * s_a] with s_a = 2^(ebits_a - 127),
 // evaluated as fmaf(q, s_a, o_a); the grid is chosen so that this box
 // contains the child's FP32 box.  cnt == 0: inner child, ref = wide node
-// index; 0 < cnt < 0xFFFF: leaf of triangles [ref, ref + cnt); cnt == 0xFFFF:
-// a larger leaf, ref = (binary node << 1) | slot holds its range.
+// index; cnt > 0: leaf of triangles [ref, ref + cnt) (the widening pass
+// splits larger binary leaves, so every count fits 16 bits).
 struct __align__(64) WideNode {
   float o[3];
   uint32_t meta;    // ebits x | y << 8 | z << 16 | (mask of used child slots) << 24
@@ -169,7 +170,6 @@ struct __align__(64) WideNode {
   uint32_t cnt[2];  // child c: bits 16 (c & 1) .. of cnt[c >> 1]
 };
 static_assert(sizeof(WideNode) == 64, "wide node record must be 64 bytes");
-constexpr uint32_t kWideBigLeaf = 0xFFFFu;
 
 // A tree as the reference stores it (for Cost/Size/export and as PHR input):
 // node i has n_children children listed at child_list[child_off[i] ...].
@@ -260,4 +260,10 @@ inline void fill_async(hipStream_t st, T* p, size_t n, T v) {
   if (n == 0) return;
   const int64_t blocks = std::min<int64_t>(ceil_div(int64_t(n), 256), 2048);
   fill_kernel<T><<<int(blocks), 256, 0, st>>>(p, n, v);
+}
+
+// Path segments traced by the context's last render or backward call: a
+// 64-bit counter in its own 128-byte line of the work-counter block.
+inline unsigned long long* segment_counter(trace_ctx* ctx) {
+  return reinterpret_cast<unsigned long long*>(ctx->d_work + 32);
 }

@@ -160,6 +160,16 @@ int trace_adam_step(trace_ctx* ctx, float* d_param, const float* d_grad, float* 
   return TRACE_OK;
 }
 
+int trace_render_segments(trace_ctx* ctx, int64_t* out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_segments: null output");
+  unsigned long long v = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&v, segment_counter(ctx), sizeof(v), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *out = int64_t(v);
+  return TRACE_OK;
+}
+
 int trace_mesh_export(trace_ctx* ctx, const trace_mesh* mesh, double* h_pos) {
   if (!ctx) return TRACE_E_INVALID;
   if (!mesh || !h_pos) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_export: null argument");

@@ -246,6 +246,7 @@ struct SegArgs {
   int32_t k, max_depth, miss;
   int64_t path_base;
   float4 *ro, *rd, *thr, *rad;
+  unsigned long long* seg_total;  // segments traced since the call began
 };
 
 // Shades segment k of path p after its closest hit h (primitive.go:145-150,
@@ -533,12 +534,13 @@ int refresh_geo_normals(trace_ctx* ctx, trace_mesh* mesh) {
 
 template <int THREADS, int STACK>
 __global__ __launch_bounds__(THREADS) void bounce_kernel(
-    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const SegArgs g,
+    const WideNode* __restrict__ wnodes, const SegArgs g,
     const int32_t* __restrict__ q_in, int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
     uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err, const Adj adj) {
   __shared__ int32_t s_stack[traverse::kLdsStack * THREADS];
   const uint32_t count = *count_in;
   const uint32_t i = blockIdx.x * uint32_t(THREADS) + threadIdx.x;
+  if (i == 0) atomicAdd(g.seg_total, static_cast<unsigned long long>(count));
   bool push = false;
   int32_t p = 0;
   int32_t al_tri = -1, em_tri = -1;  // replay: triangles that receive a gradient
@@ -552,7 +554,7 @@ __global__ __launch_bounds__(THREADS) void bounce_kernel(
     int32_t nn = 0, nt = 0;
     bool overflow = false, bad = false;
     traverse::Hit h = traverse::closest_hit_wide<STACK, false>(
-        wnodes, nodes, g.tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, THREADS);
+        wnodes, g.tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, THREADS);
     if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
     if (bad) atomicOr(d_err, TRACE_ERR_TREE);
     traverse::refine_hit_fp64(g.tris, ray, h);
@@ -570,7 +572,7 @@ constexpr int kSegRefillAt = 16;
 
 template <int THREADS, int STACK>
 __global__ __launch_bounds__(THREADS) void bounce_persistent(
-    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const SegArgs g,
+    const WideNode* __restrict__ wnodes, const SegArgs g,
     const int32_t* __restrict__ q_in, int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
     uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err, const Adj adj,
     uint32_t* __restrict__ claim) {
@@ -578,6 +580,7 @@ __global__ __launch_bounds__(THREADS) void bounce_persistent(
   __shared__ int32_t s_queue[THREADS / 64][128];
   WaveQueue wq{s_queue[threadIdx.x >> 6], 0u};
   const int64_t count = *count_in;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(g.seg_total, static_cast<unsigned long long>(count));
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
   traverse::TravStack<STACK> stack;
@@ -623,7 +626,7 @@ __global__ __launch_bounds__(THREADS) void bounce_persistent(
         if (!active && rank < avail) {
           p = q_in[pool_next + rank];
           const BvhHeader hdr = *g.scenes[p / g.pps].hdr;
-          n_bound = uint32_t(hdr.n_inner);
+          n_bound = uint32_t(hdr.n_wide);
           n_prims = hdr.n_prims;
           o4 = g.ro[p];
           d4 = g.rd[p];
@@ -639,7 +642,7 @@ __global__ __launch_bounds__(THREADS) void bounce_persistent(
       continue;
     }
     if (active) {
-      active = traverse::wide_step<STACK, false>(wnodes, nodes, g.tris, n_bound, n_prims, s, stack, nn, nt,
+      active = traverse::wide_step<STACK, false>(wnodes, g.tris, n_bound, n_prims, s, stack, nn, nt,
                                                  overflow, bad);
       pending = !active;
     }
@@ -671,6 +674,7 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
   g.rd = rd;
   g.thr = thr;
   g.rad = rad;
+  g.seg_total = segment_counter(ctx);
   const bool deep = bvh->depth_bound() > 63;
   if (ctx->tune.render_persistent) {
     // resident grid (up to 32 waves per CU); queue slots are claimed from a counter
@@ -678,18 +682,18 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
     uint32_t* claim = ctx->d_work + 1;
     fill_async<uint32_t>(st, claim, 1, 0u);
     if (deep)
-      bounce_persistent<THREADS, 512><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), bvh->nodes_p(), g, qin, qout,
+      bounce_persistent<THREADS, 512><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout,
                                                                  cin, cout, ctx->d_err, adj, claim);
     else
-      bounce_persistent<THREADS, 128><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), bvh->nodes_p(), g, qin, qout,
+      bounce_persistent<THREADS, 128><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout,
                                                                  cin, cout, ctx->d_err, adj, claim);
     return;
   }
   if (deep)
-    bounce_kernel<THREADS, 512><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), bvh->nodes_p(), g, qin, qout, cin,
+    bounce_kernel<THREADS, 512><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin,
                                                           cout, ctx->d_err, adj);
   else
-    bounce_kernel<THREADS, 128><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), bvh->nodes_p(), g, qin, qout, cin,
+    bounce_kernel<THREADS, 128><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin,
                                                           cout, ctx->d_err, adj);
 }
 
@@ -801,6 +805,7 @@ int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const
   int rc = prepare(ctx, trees, nscene, cams, prm, w);
   if (rc) return rc;
   hipStream_t st = ctx->stream;
+  fill_async<unsigned long long>(st, segment_counter(ctx), 1, 0ull);
   const int64_t wh = int64_t(prm->width) * prm->height;
   const Adj adj = mesh_adj(mesh, kModePrimal);
   for (int64_t s0 = 0; s0 < prm->spp; s0 += w.spb) {
@@ -828,6 +833,7 @@ int backward_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* con
   int rc = prepare(ctx, trees, nscene, cams, prm, w);
   if (rc) return rc;
   hipStream_t st = ctx->stream;
+  fill_async<unsigned long long>(st, segment_counter(ctx), 1, 0ull);
   fill_async<float>(st, d_grad_albedo, size_t(3 * mesh->ntri), 0.0f);
   fill_async<float>(st, d_grad_emission, size_t(3 * mesh->ntri), 0.0f);
   const int64_t wh = int64_t(prm->width) * prm->height;

@@ -6,11 +6,12 @@
 namespace traverse {
 
 // THREADS: workgroup size (chosen by the autotuner, default 128).  STACK 128
-// covers every LBVH's wide tree (at most 32 wide levels, 3 deferred children
-// each); deeper PHR trees use 512.
+// covers the wide tree of any binary tree of depth <= 63 (at most 32 wide
+// levels, 3 deferred children each, plus the split-leaf levels the widening
+// pass sizes to fit); deeper PHR trees use 512.
 template <int THREADS, int STACK, bool COUNT>
 __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
-    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
+    const WideNode* __restrict__ wnodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
     const float4* __restrict__ rays, int64_t n, int32_t id_offset, int32_t* __restrict__ hit_out,
     float* __restrict__ t_out, int32_t* __restrict__ cnt_nodes, int32_t* __restrict__ cnt_tris,
@@ -24,7 +25,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
   Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   int32_t nn = 0, nt = 0;
   bool overflow = false, bad = false;
-  Hit h = closest_hit_wide<STACK, COUNT>(wnodes, nodes, tris, hdr, ray, nn, nt, overflow, bad,
+  Hit h = closest_hit_wide<STACK, COUNT>(wnodes, tris, hdr, ray, nn, nt, overflow, bad,
                                          s_stack + threadIdx.x, THREADS);
   refine_hit_fp64(tris, ray, h);
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
@@ -48,14 +49,14 @@ constexpr int kRefillAt = 16;
 
 template <int THREADS, int STACK, bool COUNT>
 __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
-    const WideNode* __restrict__ wnodes, const BvhNode* __restrict__ nodes, const float* __restrict__ tris,
+    const WideNode* __restrict__ wnodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
     const float4* __restrict__ rays, int64_t n, int32_t id_offset, int32_t* __restrict__ hit_out,
     float* __restrict__ t_out, int32_t* __restrict__ cnt_nodes, int32_t* __restrict__ cnt_tris,
     uint32_t* d_err, uint32_t* work) {
   __shared__ int32_t s_stack[kLdsStack * THREADS];
   const BvhHeader hdr = *hdr_p;
-  const uint32_t n_bound = uint32_t(hdr.n_inner);
+  const uint32_t n_bound = uint32_t(hdr.n_wide);
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
   TravStack<STACK> stack;
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
       continue;
     }
     if (active) {
-      active = wide_step<STACK, COUNT>(wnodes, nodes, tris, n_bound, hdr.n_prims, s, stack, nn, nt,
+      active = wide_step<STACK, COUNT>(wnodes, tris, n_bound, hdr.n_prims, s, stack, nn, nt,
                                        overflow, bad);
       pending = !active;
     }
@@ -138,12 +139,12 @@ static void launch(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, int
     uint32_t* work = ctx->d_work;
     fill_async<uint32_t>(ctx->stream, work, 1, 0u);
     closest_hit_persistent<THREADS, STACK, COUNT><<<int(blocks), THREADS, 0, ctx->stream>>>(
-        bvh->wnodes_p(), bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n,
+        bvh->wnodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n,
         id_offset, d_hit, d_t, d_nodes, d_tris, ctx->d_err, work);
     return;
   }
   closest_hit_kernel<THREADS, STACK, COUNT><<<int(ceil_div(n, THREADS)), THREADS, 0, ctx->stream>>>(
-      bvh->wnodes_p(), bvh->nodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n, id_offset, d_hit, d_t,
+      bvh->wnodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n, id_offset, d_hit, d_t,
       d_nodes, d_tris, ctx->d_err);
 }
 

@@ -187,7 +187,7 @@ __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeade
   if (!slab(s.r, hdr.root_lo[0], hdr.root_lo[1], hdr.root_lo[2], hdr.root_hi[0], hdr.root_hi[1],
             hdr.root_hi[2], s.h.t, tn))
     return false;
-  if (hdr.root_cnt > 0) {
+  if (hdr.root_cnt > 0 && hdr.wroot < 0) {  // a root leaf kept whole
     leaf_test<COUNT>(s.r, tris, hdr.root_ref, hdr.root_cnt, hdr.n_prims, s.h, n_tris, bad);
     return false;
   }
@@ -200,11 +200,10 @@ __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeade
 // once, then the surviving inner children are ordered by entry distance: the
 // nearest is visited next and the others are pushed farthest first.  Returns
 // false when the ray is done.
-// n_bound = the tree's binary inner-node count (wide nodes are at most as
-// many), n_prims = its triangle count: bounds for every reference followed.
+// n_bound = the tree's wide-node count, n_prims = its triangle count:
+// bounds for every reference followed.
 template <int STACK, bool COUNT>
 __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
-                                 const BvhNode* __restrict__ bnodes,
                                  const float* __restrict__ tris, uint32_t n_bound, int32_t n_prims,
                                  WideState& s, TravStack<STACK>& stack, int32_t& n_nodes,
                                  int32_t& n_tris, bool& overflow, bool& bad) {
@@ -262,20 +261,8 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
   while (leaves) {
     const int k = __builtin_ctz(leaves);
     leaves &= leaves - 1u;
-    int32_t first = k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3];
-    const uint32_t ck = k == 0 ? cnt[0] : k == 1 ? cnt[1] : k == 2 ? cnt[2] : cnt[3];
-    int32_t count = int32_t(ck);
-    if (ck == kWideBigLeaf) {
-      const int32_t code = first;
-      const int32_t bi = code >> 1;
-      if (code < 0 || uint32_t(bi) >= n_bound) {
-        bad = true;
-        continue;
-      }
-      const int4 rc = *reinterpret_cast<const int4*>(&bnodes[bi].ref[0]);
-      first = (code & 1) ? rc.y : rc.x;
-      count = (code & 1) ? rc.w : rc.z;
-    }
+    const int32_t first = k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3];
+    const int32_t count = int32_t(k == 0 ? cnt[0] : k == 1 ? cnt[1] : k == 2 ? cnt[2] : cnt[3]);
     leaf_test<COUNT>(r, tris, first, count, n_prims, s.h, n_tris, bad);
   }
   // inner children still in reach, sorted by entry distance
@@ -321,7 +308,6 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
 // Whole-ray wide traversal (used where one lane keeps one ray).
 template <int STACK, bool COUNT>
 __device__ inline Hit closest_hit_wide(const WideNode* __restrict__ wnodes,
-                                       const BvhNode* __restrict__ bnodes,
                                        const float* __restrict__ tris, const BvhHeader& hdr,
                                        const Ray& ray, int32_t& n_nodes, int32_t& n_tris,
                                        bool& overflow, bool& bad, int32_t* lds_col, int stride) {
@@ -330,7 +316,7 @@ __device__ inline Hit closest_hit_wide(const WideNode* __restrict__ wnodes,
   stack.lds = lds_col;
   stack.stride = stride;
   if (wide_begin<COUNT>(tris, hdr, ray, s, n_tris, bad))
-    while (wide_step<STACK, COUNT>(wnodes, bnodes, tris, uint32_t(hdr.n_inner), hdr.n_prims, s, stack,
+    while (wide_step<STACK, COUNT>(wnodes, tris, uint32_t(hdr.n_wide), hdr.n_prims, s, stack,
                                    n_nodes, n_tris, overflow, bad)) {
     }
   return s.h;

@@ -17,6 +17,20 @@
 // cell where the grid allows.  The quantised box is checked with the same
 // fmaf decode the traversal uses, so it always contains the child's FP32 box
 // and the traversal stays conservative.
+//
+// Binary leaves of more than kSplitAbove triangles (PHR's makeLeaf flattens
+// whole LBVH subtrees into one leaf, thousands of triangles on regular
+// geometry such as a heightfield) get a 4-ary sub-tree over their index
+// range: level l holds 4^l nodes, node j covering [n j / 4^l, n (j+1) / 4^l)
+// of the leaf's n triangles, down to ranges of at most kSplitLeaf.  The
+// triangles of a leaf are in Morton order, so index quarters are spatially
+// compact.  Only the traversal tree changes: the binary tree, its cost and
+// every closest hit stay the same (hits are ordered by t, then by triangle
+// position, whatever order nodes are visited in).  Sub-tree nodes follow the
+// binary-derived ones; an exclusive scan over per-entry node counts places
+// them.  The sub-tree depth is capped so that the traversal stack (three
+// entries per level) still holds; every leaf then keeps at most 2048
+// triangles, within the 16-bit count of a wide child.
 #include <algorithm>
 #include <cmath>
 
@@ -55,6 +69,34 @@ __global__ void flag_kernel(const int32_t* __restrict__ parent, const BvhHeader*
   flag[i] = (d & 1) == 0 ? 1u : 0u;
 }
 
+constexpr int64_t kSplitAbove = 4;  // leaves of more triangles get a sub-tree
+constexpr int64_t kSplitLeaf = 4;   // largest leaf of a sub-tree (depth cap aside)
+
+// Levels of the sub-tree of an n-triangle leaf: the fewest h with
+// ceil(n / 4^h) <= kSplitLeaf, at most max_levels; 0 for a leaf kept whole.
+__device__ inline int split_levels(int64_t n, int max_levels) {
+  if (n <= kSplitAbove) return 0;
+  int h = 0;
+  for (int64_t cap = kSplitLeaf; cap < n && h < max_levels; cap *= 4) ++h;
+  return h;
+}
+
+// Nodes on the levels above l of a complete 4-ary tree: (4^l - 1) / 3.
+__device__ inline uint32_t level_base(int l) { return uint32_t(((uint64_t(1) << (2 * l)) - 1u) / 3u); }
+
+__device__ inline uint32_t split_nodes(int64_t n, int max_levels) {
+  return level_base(split_levels(n, max_levels));
+}
+
+// Where the sub-trees go: wide index = *base + soff[entry] + local index.
+// Entry j < cap: the leaf children of binary node j (slot 0's sub-tree
+// first); entry cap + t: the root leaf of tree t.
+struct SplitMap {
+  const uint32_t* soff;
+  const uint32_t* base;
+  int max_levels;
+};
+
 struct Slot {
   float lo[3], hi[3];
   int32_t ref;
@@ -73,7 +115,7 @@ __device__ inline Slot empty_slot() {
 // Child d of binary node j (the box is read from memory): a leaf slot, or an
 // inner slot pointing at the wide node of that child.
 __device__ inline Slot child_slot(const BvhNode* __restrict__ nodes, const uint32_t* __restrict__ widx,
-                                  int32_t n_inner, int32_t j, int d, bool& ok) {
+                                  int32_t n_inner, int32_t j, int d, const SplitMap& sm, bool& ok) {
   Slot s;
   const float* box = reinterpret_cast<const float*>(nodes + j) + 6 * d;
   for (int a = 0; a < 3; ++a) {
@@ -83,12 +125,14 @@ __device__ inline Slot child_slot(const BvhNode* __restrict__ nodes, const uint3
   s.used = true;
   const int32_t ref = nodes[j].ref[d], cnt = nodes[j].cnt[d];
   if (cnt > 0) {
-    if (uint32_t(cnt) < kWideBigLeaf) {
+    if (cnt <= kSplitAbove) {
       s.ref = ref;
       s.cnt = uint32_t(cnt);
     } else {
-      s.ref = (j << 1) | d;
-      s.cnt = kWideBigLeaf;
+      const int32_t c0 = nodes[j].cnt[0];
+      const uint32_t skip = d == 1 && c0 > 0 ? split_nodes(c0, sm.max_levels) : 0u;
+      s.ref = int32_t(*sm.base + sm.soff[j] + skip);
+      s.cnt = 0u;
     }
   } else {
     ok = ok && ref >= 0 && ref < n_inner;
@@ -114,35 +158,12 @@ __device__ inline int grid_exponent(float lo, float hi) {
 
 __device__ inline float decode(uint32_t q, float s, float o) { return fmaf(float(q), s, o); }
 
+
 // Binary child c of a wide node fills slots 2c and 2c + 1: a leaf child takes
 // slot 2c, an inner child (odd depth) is replaced by its two children.
-__global__ void widen_kernel(const BvhNode* __restrict__ nodes, const BvhHeader* __restrict__ hdr,
-                             const uint32_t* __restrict__ flag, const uint32_t* __restrict__ widx,
-                             WideNode* __restrict__ out, uint32_t* d_err) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int32_t n_inner = hdr->n_inner;
-  if (i >= n_inner || !flag[i]) return;
-  bool ok = true;
-  Slot s[4];
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int32_t cref = nodes[i].ref[c], ccnt = nodes[i].cnt[c];
-    if (ccnt > 0) {
-      s[2 * c] = child_slot(nodes, widx, n_inner, i, c, ok);
-      s[2 * c + 1] = empty_slot();
-    } else if (cref >= 0 && cref < n_inner) {
-      s[2 * c] = child_slot(nodes, widx, n_inner, cref, 0, ok);
-      s[2 * c + 1] = child_slot(nodes, widx, n_inner, cref, 1, ok);
-    } else {
-      ok = false;
-      s[2 * c] = empty_slot();
-      s[2 * c + 1] = empty_slot();
-    }
-  }
-  if (!ok) {
-    atomicOr(d_err, TRACE_ERR_TREE);
-    return;
-  }
+// Quantises four child slots into wide node *dst.
+__device__ __attribute__((always_inline)) inline void write_node(const Slot (&s)[4], WideNode* __restrict__ dst,
+                                                                  uint32_t* d_err) {
   WideNode w;
   uint32_t valid = 0u;
 #pragma unroll
@@ -186,50 +207,226 @@ __global__ void widen_kernel(const BvhNode* __restrict__ nodes, const BvhHeader*
   w.cnt[1] = s[2].cnt | (s[3].cnt << 16);
 #pragma unroll
   for (int c = 0; c < 4; ++c) w.ref[c] = s[c].ref;
-  out[widx[i]] = w;
+  *dst = w;
+}
+
+__global__ void widen_kernel(const BvhNode* __restrict__ nodes, const BvhHeader* __restrict__ hdr,
+                             const uint32_t* __restrict__ flag, const uint32_t* __restrict__ widx,
+                             const SplitMap sm, WideNode* __restrict__ out, uint32_t* d_err) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t n_inner = hdr->n_inner;
+  if (i >= n_inner || !flag[i]) return;
+  bool ok = true;
+  Slot s[4];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int32_t cref = nodes[i].ref[c], ccnt = nodes[i].cnt[c];
+    if (ccnt > 0) {
+      s[2 * c] = child_slot(nodes, widx, n_inner, i, c, sm, ok);
+      s[2 * c + 1] = empty_slot();
+    } else if (cref >= 0 && cref < n_inner) {
+      s[2 * c] = child_slot(nodes, widx, n_inner, cref, 0, sm, ok);
+      s[2 * c + 1] = child_slot(nodes, widx, n_inner, cref, 1, sm, ok);
+    } else {
+      ok = false;
+      s[2 * c] = empty_slot();
+      s[2 * c + 1] = empty_slot();
+    }
+  }
+  if (!ok) {
+    atomicOr(d_err, TRACE_ERR_TREE);
+    return;
+  }
+  write_node(s, out + widx[i], d_err);
+}
+
+// need[e]: sub-tree nodes of entry e (see SplitMap).
+__global__ void split_count_kernel(const BvhNode* __restrict__ nodes, const BvhHeader* __restrict__ hdr,
+                                   int64_t cap, const BvhHeader* __restrict__ roots, int32_t n_roots,
+                                   int max_levels, uint32_t* __restrict__ need) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= cap + n_roots) return;
+  uint32_t w = 0u;
+  if (e < cap) {
+    if (e < hdr->n_inner) {
+      const int32_t c0 = nodes[e].cnt[0], c1 = nodes[e].cnt[1];
+      if (c0 > 0) w += split_nodes(c0, max_levels);
+      if (c1 > 0) w += split_nodes(c1, max_levels);
+    }
+  } else {
+    const int32_t rc = roots[e - cap].root_cnt;
+    if (rc > 0) w = split_nodes(rc, max_levels);
+  }
+  need[e] = w;
+}
+
+// One sub-tree node per thread (grid-stride over the device-side total):
+// finds its leaf by binary search over the offsets, then its level and
+// index, and writes its four children (sub-tree nodes, or leaves of the
+// last level), their boxes the union of the triangle boxes in range.
+__global__ void split_kernel(const float* __restrict__ prim_box, int64_t n_prims,
+                             const BvhNode* __restrict__ nodes, int64_t cap,
+                             const BvhHeader* __restrict__ roots, int32_t n_roots, const SplitMap sm,
+                             const uint32_t* __restrict__ n_split, WideNode* __restrict__ out,
+                             int64_t out_cap, uint32_t* d_err) {
+  const uint32_t total = *n_split, base = *sm.base;
+  const int64_t n_entries = cap + n_roots;
+  if (int64_t(base) + total > out_cap) {  // leaves overlap: not a tree
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(d_err, TRACE_ERR_TREE);
+    return;
+  }
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = n_entries;  // last entry e with soff[e] <= u
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (sm.soff[mid] <= u) lo = mid;
+      else hi = mid;
+    }
+    const int64_t e = lo;
+    uint32_t local = u - sm.soff[e];
+    uint32_t root = base + sm.soff[e];
+    int64_t first, n;
+    if (e < cap) {
+      const int32_t c0 = nodes[e].cnt[0];
+      const uint32_t w0 = c0 > 0 ? split_nodes(c0, sm.max_levels) : 0u;
+      const int d = local < w0 ? 0 : 1;
+      if (d == 1) {
+        local -= w0;
+        root += w0;
+      }
+      first = nodes[e].ref[d];
+      n = nodes[e].cnt[d];
+    } else {
+      first = roots[e - cap].root_ref;
+      n = roots[e - cap].root_cnt;
+    }
+    if (first < 0 || first + n > n_prims) {
+      atomicOr(d_err, TRACE_ERR_TREE);
+      continue;
+    }
+    const int levels = split_levels(n, sm.max_levels);
+    int l = 0;
+    while (l + 1 < levels && level_base(l + 1) <= local) ++l;
+    const uint64_t j = local - level_base(l);
+    if (levels == 0 || j >= (uint64_t(1) << (2 * l))) {  // offsets disagree with the counts
+      atomicOr(d_err, TRACE_ERR_TREE);
+      continue;
+    }
+    const int sh = 2 * (l + 1);
+    Slot s[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t a = int64_t((uint64_t(n) * (4 * j + k)) >> sh);
+      const int64_t b = int64_t((uint64_t(n) * (4 * j + k + 1)) >> sh);
+      if (b <= a) {
+        s[k] = empty_slot();
+        continue;
+      }
+      Slot c;
+      c.used = true;
+      for (int q = 0; q < 3; ++q) {
+        c.lo[q] = INFINITY;
+        c.hi[q] = -INFINITY;
+      }
+      for (int64_t t = first + a; t < first + b; ++t) {
+        const float* pb = prim_box + 6 * t;
+        for (int q = 0; q < 3; ++q) {
+          c.lo[q] = fminf(c.lo[q], pb[q]);
+          c.hi[q] = fmaxf(c.hi[q], pb[3 + q]);
+        }
+      }
+      if (l + 1 < levels) {
+        c.ref = int32_t(root + level_base(l + 1) + uint32_t(4 * j + k));
+        c.cnt = 0u;
+      } else {
+        c.ref = int32_t(first + a);
+        c.cnt = uint32_t(b - a);
+      }
+      s[k] = c;
+    }
+    write_node(s, out + base + u, d_err);
+  }
 }
 
 // Wide root of each tree header (the forest's own header is not a tree).
+// A root leaf kept whole has wroot -1; n_wide bounds every wide index.
 __global__ void wroot_kernel(BvhHeader* __restrict__ hdrs, int32_t count,
-                             const uint32_t* __restrict__ widx, BvhHeader* __restrict__ forest_hdr) {
+                             const uint32_t* __restrict__ widx, const SplitMap sm, int64_t cap,
+                             const uint32_t* __restrict__ n_split, BvhHeader* __restrict__ forest_hdr) {
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == 0 && forest_hdr) forest_hdr->wroot = -1;
+  const int32_t n_wide = int32_t(*sm.base + *n_split);
+  if (k == 0 && forest_hdr) {
+    forest_hdr->wroot = -1;
+    forest_hdr->n_wide = n_wide;
+  }
   if (k >= count) return;
   BvhHeader& h = hdrs[k];
-  h.wroot = h.root_cnt > 0 ? -1 : int32_t(widx[h.root_ref]);
+  if (h.root_cnt > 0)
+    h.wroot = h.root_cnt > kSplitAbove ? int32_t(*sm.base + sm.soff[cap + k]) : -1;
+  else
+    h.wroot = int32_t(widx[h.root_ref]);
+  h.n_wide = n_wide;
 }
 
 }  // namespace
 
 int widen(trace_ctx* ctx, trace_bvh* bvh) {
   const int64_t cap = std::max<int64_t>(1, int64_t(bvh->nodes.n));
+  const int32_t n_roots = bvh->nseg > 1 ? bvh->nseg : 1;
+  BvhHeader* roots = bvh->nseg > 1 ? bvh->seg_headers.p : bvh->header.p;
+  const int64_t n_entries = cap + n_roots;
+  // A sub-tree of h levels has (4^h - 1) / 3 < 4 n / (3 kSplitLeaf) nodes
+  // (h is the fewest levels with ceil(n / 4^h) <= kSplitLeaf, so
+  // n > kSplitLeaf 4^(h-1)), and split leaves hold at most n_prims triangles.
+  const int64_t split_cap = 4 * bvh->n_prims / (3 * kSplitLeaf) + 1;
+  if (cap + split_cap > int64_t(INT32_MAX))
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "widen: %lld wide nodes exceed the 32-bit index",
+               static_cast<long long>(cap + split_cap));
+  // Stack bound: a lane holds at most 3 entries per wide level above it.
+  // Binary depth D puts binary-derived wide nodes at levels <= (D - 1) / 2,
+  // and the kernels keep 128 entries for D <= 63, 512 up to 255.
+  const int depth = bvh->depth_bound();
+  const int stack = depth > 63 ? 512 : 128;
+  const int max_levels = std::min(15, std::max(10, stack / 3 - 1 - std::max(0, (depth - 1) / 2)));
   hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, bvh->wnodes.alloc(cap));
+  HIP_TRY(ctx, bvh->wnodes.alloc(size_t(cap + split_cap)));
   ScratchPlan plan;
   const size_t o_par = plan.take(cap * sizeof(int32_t));
   const size_t o_flag = plan.take(cap * sizeof(uint32_t));
   const size_t o_idx = plan.take(cap * sizeof(uint32_t));
-  const size_t o_tot = plan.take(sizeof(uint32_t));
-  const size_t o_scan = plan.take(scan::scratch_words(cap) * sizeof(uint32_t));
+  const size_t o_tot = plan.take(2 * sizeof(uint32_t));
+  const size_t o_need = plan.take(n_entries * sizeof(uint32_t));
+  const size_t o_soff = plan.take(n_entries * sizeof(uint32_t));
+  const size_t o_scan = plan.take(scan::scratch_words(n_entries) * sizeof(uint32_t));
   int rc = ctx_reserve_scratch(ctx, plan.off);
   if (rc) return rc;
   char* base = static_cast<char*>(ctx->scratch);
   auto* parent = reinterpret_cast<int32_t*>(base + o_par);
   auto* flag = reinterpret_cast<uint32_t*>(base + o_flag);
   auto* widx = reinterpret_cast<uint32_t*>(base + o_idx);
-  auto* total = reinterpret_cast<uint32_t*>(base + o_tot);
+  auto* total = reinterpret_cast<uint32_t*>(base + o_tot);  // [0] binary-derived, [1] sub-tree nodes
+  auto* need = reinterpret_cast<uint32_t*>(base + o_need);
+  auto* soff = reinterpret_cast<uint32_t*>(base + o_soff);
   auto* sscr = reinterpret_cast<uint32_t*>(base + o_scan);
   const int grid = int(ceil_div(cap, 256));
   fill_async<int32_t>(st, parent, size_t(cap), -1);
   parent_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, parent);
   flag_kernel<<<grid, 256, 0, st>>>(parent, bvh->header.p, cap, flag, ctx->d_err);
   scan::exclusive_u32(st, flag, widx, cap, total, sscr);
-  widen_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, flag, widx, bvh->wnodes.p, ctx->d_err);
+  split_count_kernel<<<int(ceil_div(n_entries, 256)), 256, 0, st>>>(bvh->nodes.p, bvh->header.p, cap, roots,
+                                                                    n_roots, max_levels, need);
+  scan::exclusive_u32(st, need, soff, n_entries, total + 1, sscr);
+  const SplitMap sm{soff, total, max_levels};
+  widen_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, flag, widx, sm, bvh->wnodes.p, ctx->d_err);
+  const int sgrid = int(std::min<int64_t>(ceil_div(split_cap, 256), int64_t(ctx->n_cu) * 16));
+  split_kernel<<<std::max(1, sgrid), 256, 0, st>>>(bvh->prim_box.p, bvh->n_prims, bvh->nodes.p, cap, roots,
+                                                   n_roots, sm, total + 1, bvh->wnodes.p, cap + split_cap,
+                                                   ctx->d_err);
   if (bvh->nseg > 1)
-    wroot_kernel<<<int(ceil_div(bvh->nseg, 256)), 256, 0, st>>>(bvh->seg_headers.p, bvh->nseg, widx,
-                                                               bvh->header.p);
+    wroot_kernel<<<int(ceil_div(bvh->nseg, 256)), 256, 0, st>>>(bvh->seg_headers.p, bvh->nseg, widx, sm, cap,
+                                                               total + 1, bvh->header.p);
   else
-    wroot_kernel<<<1, 64, 0, st>>>(bvh->header.p, 1, widx, nullptr);
+    wroot_kernel<<<1, 64, 0, st>>>(bvh->header.p, 1, widx, sm, cap, total + 1, nullptr);
   HIP_TRY(ctx, hipGetLastError());
   return TRACE_OK;
 }

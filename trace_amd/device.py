@@ -301,6 +301,14 @@ def render_backward(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height:
                                             C.c_void_p(d_grad_albedo), C.c_void_p(d_grad_emission)))
 
 
+def render_segments(ctx: Context) -> int:
+    """trace_render_segments: rays traced by the context's last render or
+    backward call (waits for the context stream)."""
+    out = C.c_int64()
+    ctx.check(ctx.lib.trace_render_segments(ctx.h, C.byref(out)))
+    return out.value
+
+
 def adam_step(ctx: Context, d_param: int, d_grad: int, d_m: int, d_v: int, n: int, lr: float, step: int,
               beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, lo: float = 0.0,
               hi: float = 1.0) -> None:
