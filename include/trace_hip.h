@@ -273,11 +273,12 @@ int trace_graph_destroy(trace_graph* graph);
 typedef struct {
   int32_t trav_threads;   /* closest-hit workgroup size: 64, 128 or 256 */
   int32_t render_threads; /* path-segment workgroup size: 64, 128 or 256 */
-  int64_t render_batch;   /* paths per wavefront batch */
+  int64_t render_batch;   /* paths per wavefront batch (default 2^27: <= 9.7 GB of path state) */
   int32_t tuned;          /* 1 once an autotune call has run */
   int32_t trav_persistent; /* closest hit: 1 = persistent lanes refilled from a work counter,
                               0 = one ray per lane */
-  int32_t render_persistent; /* path segments: the same choice for the render's ray queues */
+  int32_t render_persistent; /* path segments: 1 = split (persistent trace pass over the
+                                queue, then a shade pass), 0 = fused (one segment per lane) */
   int32_t pad;
 } trace_tune;
 int trace_get_tune(trace_ctx* ctx, trace_tune* out);
@@ -288,8 +289,9 @@ int trace_set_tune(trace_ctx* ctx, const trace_tune* tune);
 int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays,
                              int64_t n, float* h_ms);
 /* Times a render for each workgroup size, wavefront batch size (candidate batches
- * 2^20, 2^21, 2^22, 2^23 paths) and queue mode (one segment per lane, persistent) in
- * turn and keeps the fastest; h_ms receives 3 + 4 + 2 times. */
+ * 2^22, 2^24, 2^26, 2^27 paths) and segment mode (fused: one segment per lane traced
+ * and shaded; split: a persistent trace pass, then a shade pass) in turn and keeps the
+ * fastest; h_ms receives 3 + 4 + 2 times. */
 int trace_autotune_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
                           const trace_camera* cam, const trace_render_params* prm, float* h_ms);
 

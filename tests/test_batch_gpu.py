@@ -303,7 +303,7 @@ def test_autotune_keeps_results(ctx):
     rms = autotune_render(ctx, smesh, sbvh, sc["camera"], 64, 36, 2, 3, _lib.MISS_DEFAULT, 3)
     assert rms.shape == (9,) and np.all(rms > 0)
     t = get_tune(ctx)
-    assert t.render_threads in (64, 128, 256) and t.render_batch in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+    assert t.render_threads in (64, 128, 256) and t.render_batch in (1 << 22, 1 << 24, 1 << 26, 1 << 27)
     imgs = []
     for persistent in (False, True):
         for threads in (64, 128, 256):

@@ -18,8 +18,11 @@
 // Launch parameters the autotuner may change (defaults are the untuned choice).
 struct TuneState {
   int trav_threads = 64;                // closest-hit workgroup size: 64, 128 or 256
-  int render_threads = 128;             // path-segment workgroup size: 64, 128 or 256
-  int64_t render_batch = int64_t(1) << 22;  // paths per wavefront batch
+  int render_threads = 64;              // path-segment workgroup size: 64, 128 or 256
+  // Paths per wavefront batch.  Path state is 72 B/path, so 2^27 paths hold
+  // at most 9.7 GB of scratch (a 1920x1080x64-spp frame in one batch); large
+  // batches keep the late bounces, where few paths remain, filling the GPU.
+  int64_t render_batch = int64_t(1) << 27;
   bool trav_persistent = true;          // closest hit: persistent lanes refilled from a counter
   bool render_persistent = false;       // path segments: the same for the render's queues
   bool tuned = false;

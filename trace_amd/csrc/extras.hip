@@ -344,7 +344,7 @@ int trace_autotune_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
     }
   }
   ctx->tune.render_threads = best;
-  const int64_t batches[4] = {int64_t(1) << 20, int64_t(1) << 21, int64_t(1) << 22, int64_t(1) << 23};
+  const int64_t batches[4] = {int64_t(1) << 22, int64_t(1) << 24, int64_t(1) << 26, int64_t(1) << 27};
   int64_t best_b = ctx->tune.render_batch;
   best_ms = INFINITY;
   for (int c = 0; c < 4; ++c) {

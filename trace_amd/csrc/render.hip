@@ -381,58 +381,6 @@ __device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p
   }
 }
 
-// Wave-staged append for the persistent kernel: continuing paths collect in
-// this wave's LDS slice and go to the next queue 64 at a time (one atomic per
-// 64 paths).  Every lane of the wave must take part.
-struct WaveQueue {
-  int32_t* buf;    // this wave's 128 LDS slots
-  uint32_t count;  // staged entries (the same in every lane)
-
-  __device__ inline void flush(uint32_t n, int32_t* __restrict__ q_out, uint32_t* __restrict__ count_out) {
-    const int lane = __lane_id();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(count_out, n);
-    base = __shfl(base, 0, 64);
-    const int32_t v = buf[lane];
-    const int32_t rest = buf[64 + lane];
-    if (uint32_t(lane) < n) q_out[base + lane] = v;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (count > 64u && uint32_t(lane) < count - 64u) buf[lane] = rest;
-    count = count > n ? count - n : 0u;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-  }
-  __device__ inline void push(bool want, int32_t p, int32_t* __restrict__ q_out,
-                              uint32_t* __restrict__ count_out) {
-    const uint64_t b = ballot64(want);
-    if (!b) return;
-    if (want) buf[count + __popcll(b & lanemask_lt())] = p;
-    count += uint32_t(__popcll(b));
-    if (count >= 64u) flush(64u, q_out, count_out);
-  }
-};
-
-// Replay-mode gradient scatter; every lane of the wave must take part.
-__device__ inline void segment_gradients(const Adj& adj, int64_t pps, int32_t p, int32_t al_tri, float4 al_w,
-                                         int32_t em_tri, float4 em_w) {
-  if (adj.mode == kModeReplay) {
-    // weight of the path: dLoss/d(mean image) at its pixel / spp
-    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
-    if (al_tri >= 0 || em_tri >= 0) {
-      const int64_t q = p % pps;
-      const float* g = adj.g_img + 3 * ((p / pps) * adj.wh + q % adj.wh);
-      wx = g[0] * adj.inv_spp;
-      wy = g[1] * adj.inv_spp;
-      wz = g[2] * adj.inv_spp;
-    }
-    wave_add3(adj.g_alb, al_tri, wx * al_w.x, wy * al_w.y, wz * al_w.z, al_tri >= 0);
-    wave_add3(adj.g_emi, em_tri, wx * em_w.x, wy * em_w.y, wz * em_w.z, em_tri >= 0);
-  }
-}
-
 // Replay-mode gradient scatter and the wave-aggregated append of continuing
 // paths to the next queue; every lane of the wave must take part.
 __device__ inline void segment_epilogue(const Adj& adj, int64_t pps, int32_t p, bool push, int32_t al_tri,
@@ -563,32 +511,37 @@ __global__ __launch_bounds__(THREADS) void bounce_kernel(
   segment_epilogue(adj, g.pps, p, push, al_tri, al_w, em_tri, em_w, q_out, count_out);
 }
 
-// Persistent variant (see closest_hit_persistent): a resident grid whose
-// waves claim chunks of the input queue from a counter and refill lanes as
-// their segments finish; shading, the gradient scatter and the append run in
-// batches when a quarter of the wave is idle.
+// Split form of a segment (the autotuner's alternative to bounce_kernel): a
+// persistent closest-hit pass over the queue (the scheme of
+// closest_hit_persistent: waves claim chunks of queue slots from a counter and
+// refill lanes as rays finish) writes one 16-byte hit record per slot, then
+// shade_kernel shades slot i in thread i.  The traversal lanes never wait on
+// shading; the records cost 32 bytes of traffic per segment.
 constexpr uint32_t kSegChunk = 256;
 constexpr int kSegRefillAt = 16;
 
+// Hit record: triangle position (-1: miss), t with the sign of the facing
+// (t > 0 for every hit, + for front), FP32 barycentrics.
+__device__ inline int4 pack_hit(const traverse::Hit& h) {
+  return make_int4(h.pos, __float_as_int(h.front ? h.t : -h.t), __float_as_int(h.u), __float_as_int(h.v));
+}
+
 template <int THREADS, int STACK>
-__global__ __launch_bounds__(THREADS) void bounce_persistent(
-    const WideNode* __restrict__ wnodes, const SegArgs g,
-    const int32_t* __restrict__ q_in, int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
-    uint32_t* __restrict__ count_out, uint32_t* __restrict__ d_err, const Adj adj,
+__global__ __launch_bounds__(THREADS) void trace_persistent(
+    const WideNode* __restrict__ wnodes, const SegArgs g, const int32_t* __restrict__ q_in,
+    const uint32_t* __restrict__ count_in, int4* __restrict__ hits, uint32_t* __restrict__ d_err,
     uint32_t* __restrict__ claim) {
   __shared__ int32_t s_stack[traverse::kLdsStack * THREADS];
-  __shared__ int32_t s_queue[THREADS / 64][128];
-  WaveQueue wq{s_queue[threadIdx.x >> 6], 0u};
   const int64_t count = *count_in;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(g.seg_total, static_cast<unsigned long long>(count));
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
+  const float tmin = g.k == 0 ? 0.001f : 0.0001f;
   traverse::TravStack<STACK> stack;
   stack.lds = s_stack + threadIdx.x;
   stack.stride = THREADS;
   traverse::WideState s;
-  float4 o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = o4;
-  int32_t p = 0;
+  int64_t slot = 0;
   uint32_t n_bound = 0;
   int32_t n_prims = 0;
   bool active = false, pending = false, overflow = false, bad = false;
@@ -598,17 +551,10 @@ __global__ __launch_bounds__(THREADS) void bounce_persistent(
   while (true) {
     const uint64_t need = ballot64(!active);
     if (__popcll(need) >= kSegRefillAt || need == ~0ull) {
-      bool push = false;
-      int32_t al_tri = -1, em_tri = -1;
-      float4 al_w = make_float4(0.0f, 0.0f, 0.0f, 0.0f), em_w = al_w;
       if (pending) {
-        traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
-        traverse::refine_hit_fp64(g.tris, ray, s.h);
-        shade_segment(g, adj, p, o4, d4, s.h, push, al_tri, al_w, em_tri, em_w);
+        hits[slot] = pack_hit(s.h);
         pending = false;
       }
-      segment_gradients(adj, g.pps, p, al_tri, al_w, em_tri, em_w);
-      wq.push(push, p, q_out, count_out);
       if (pool_next >= pool_end && fetching) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(claim, kSegChunk);
@@ -624,13 +570,14 @@ __global__ __launch_bounds__(THREADS) void bounce_persistent(
       if (avail > 0) {
         const int64_t rank = __popcll(need & lt);
         if (!active && rank < avail) {
-          p = q_in[pool_next + rank];
+          slot = pool_next + rank;
+          const int32_t p = q_in[slot];
           const BvhHeader hdr = *g.scenes[p / g.pps].hdr;
           n_bound = uint32_t(hdr.n_wide);
           n_prims = hdr.n_prims;
-          o4 = g.ro[p];
-          d4 = g.rd[p];
-          traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
+          const float4 o4 = g.ro[p];
+          const float4 d4 = g.rd[p];
+          const traverse::Ray ray{o4.x, o4.y, o4.z, tmin, d4.x, d4.y, d4.z, INFINITY};
           active = traverse::wide_begin<false>(g.tris, hdr, ray, s, nt, bad);
           pending = !active;
         }
@@ -647,16 +594,48 @@ __global__ __launch_bounds__(THREADS) void bounce_persistent(
       pending = !active;
     }
   }
-  if (wq.count) wq.flush(wq.count, q_out, count_out);
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);
+}
+
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void shade_kernel(const SegArgs g, const int32_t* __restrict__ q_in,
+                                                        int32_t* __restrict__ q_out,
+                                                        const uint32_t* __restrict__ count_in,
+                                                        uint32_t* __restrict__ count_out,
+                                                        const int4* __restrict__ hits, const Adj adj) {
+  const uint32_t count = *count_in;
+  const uint32_t i = blockIdx.x * uint32_t(THREADS) + threadIdx.x;
+  bool push = false;
+  int32_t p = 0;
+  int32_t al_tri = -1, em_tri = -1;
+  float4 al_w = make_float4(0.0f, 0.0f, 0.0f, 0.0f), em_w = al_w;
+  if (i < count) {
+    p = q_in[i];
+    const float4 o4 = g.ro[p];
+    const float4 d4 = g.rd[p];
+    const int4 r = hits[i];
+    const float ts = __int_as_float(r.y);
+    traverse::Hit h;
+    h.pos = r.x;
+    h.t = fabsf(ts);
+    h.t64 = h.t;
+    h.front = ts > 0.0f;
+    h.u = __int_as_float(r.z);
+    h.v = __int_as_float(r.w);
+    const traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
+    traverse::refine_hit_fp64(g.tris, ray, h);
+    shade_segment(g, adj, p, o4, d4, h, push, al_tri, al_w, em_tri, em_w);
+  }
+  segment_epilogue(adj, g.pps, p, push, al_tri, al_w, em_tri, em_w, q_out, count_out);
 }
 
 template <int THREADS>
 static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, int grid,
                           const Scene* scenes, int64_t pps, int32_t k, const trace_render_params* prm,
                           int64_t path_base, float4* ro, float4* rd, float4* thr, float4* rad,
-                          int32_t* qin, int32_t* qout, uint32_t* cin, uint32_t* cout, const Adj& adj) {
+                          int32_t* qin, int32_t* qout, uint32_t* cin, uint32_t* cout, int4* hits,
+                          const Adj& adj) {
   hipStream_t st = ctx->stream;
   SegArgs g;
   g.tris = bvh->tris_p();
@@ -682,11 +661,12 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
     uint32_t* claim = ctx->d_work + 1;
     fill_async<uint32_t>(st, claim, 1, 0u);
     if (deep)
-      bounce_persistent<THREADS, 512><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout,
-                                                                 cin, cout, ctx->d_err, adj, claim);
+      trace_persistent<THREADS, 512><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, cin, hits, ctx->d_err,
+                                                                claim);
     else
-      bounce_persistent<THREADS, 128><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout,
-                                                                 cin, cout, ctx->d_err, adj, claim);
+      trace_persistent<THREADS, 128><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, cin, hits, ctx->d_err,
+                                                                claim);
+    shade_kernel<THREADS><<<grid, THREADS, 0, st>>>(g, qin, qout, cin, cout, hits, adj);
     return;
   }
   if (deep)
@@ -707,6 +687,7 @@ struct Wave {
   float4 *ro, *rd, *thr, *rad;
   int32_t* q[2];
   uint32_t* counts;
+  int4* hits;   // hit record per queue slot (split segments only)
   int64_t spb;  // samples per batch
 };
 
@@ -730,6 +711,7 @@ int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const
   const size_t o_qa = plan.take(max_paths * sizeof(int32_t));
   const size_t o_qb = plan.take(max_paths * sizeof(int32_t));
   const size_t o_cnt = plan.take(64 * sizeof(uint32_t));
+  const size_t o_hit = ctx->tune.render_persistent ? plan.take(max_paths * sizeof(int4)) : 0;
   int rc = ctx_reserve_scratch(ctx, plan.off);
   if (rc) return rc;
   char* base = static_cast<char*>(ctx->scratch);
@@ -740,6 +722,7 @@ int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const
   w.q[0] = reinterpret_cast<int32_t*>(base + o_qa);
   w.q[1] = reinterpret_cast<int32_t*>(base + o_qb);
   w.counts = reinterpret_cast<uint32_t*>(base + o_cnt);
+  w.hits = ctx->tune.render_persistent ? reinterpret_cast<int4*>(base + o_hit) : nullptr;
   w.scenes = reinterpret_cast<Scene*>(base + o_scn);
 
   // Scene table: written with kernels (arguments by value), so a render stays
@@ -772,13 +755,13 @@ void bounces(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const
     const int grid = int(ceil_div(npaths, threads));
     if (threads == 64)
       launch_bounce<64>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
-                        w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, adj);
+                        w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, w.hits, adj);
     else if (threads == 256)
       launch_bounce<256>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
-                         w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, adj);
+                         w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, w.hits, adj);
     else
       launch_bounce<128>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
-                         w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, adj);
+                         w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, w.hits, adj);
   }
 }
 

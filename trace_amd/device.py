@@ -369,7 +369,7 @@ def get_tune(ctx: Context) -> "_lib.Tune":
     return t
 
 
-def set_tune(ctx: Context, trav_threads: int = 64, render_threads: int = 128, render_batch: int = 1 << 22,
+def set_tune(ctx: Context, trav_threads: int = 64, render_threads: int = 64, render_batch: int = 1 << 27,
              trav_persistent: bool = True, render_persistent: bool = False) -> None:
     t = _lib.Tune()
     t.trav_threads, t.render_threads, t.render_batch = trav_threads, render_threads, render_batch
