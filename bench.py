@@ -279,6 +279,33 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "closest_hit_incoherent_1M_traffic
 COPY_BYTES = 1 << 30  # reference copy kernel: reads and writes 1 GiB
 
 
+def bench_heatmap(torch, ctx, bvh, steps):
+    """HeatMapRenderer.RenderToBuffer on the 1M-triangle PHR tree: 1920x1080
+    camera rays from outside the unit cube (the coherent-ray camera),
+    threshold 50; FP64 reference-order walk, one ray per lane."""
+    import ctypes
+
+    from trace_amd import scenes
+    from trace_amd.device import camera_struct
+
+    W, H, th = 1920, 1080, 50
+    cam = scenes.camera_cfg2(W, H)
+    rgb = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    cnt = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    c = camera_struct(cam)
+    torch.cuda.synchronize()
+
+    def run():
+        ctx.check(ctx.lib.trace_render_heatmap_device(ctx.h, bvh.h, ctypes.byref(c), W, H, th, rgb.data_ptr(),
+                                                      cnt.data_ptr()))
+
+    per, _ = time_launches(torch, run, steps, 1)
+    ms = float(np.median(per))
+    counts = cnt.cpu().numpy()
+    return {"image": [W, H], "threshold": th, "ms": round(ms, 3), "mrays_s": round(W * H / (ms * 1e3), 1),
+            "mean_steps": round(float(counts.mean()), 2), "max_steps": int(counts.max())}
+
+
 def bench_large(torch, ctx, steps=2):
     """The large configuration: 10.38M triangles (2048x2048 fBm terrain + 5000
     400-triangle spheres), PHR tree, 1920x1080, 64 spp, depth 8, sky miss;
@@ -571,6 +598,8 @@ def main():
                             "mtris_per_s": round(args.tris / (float(per_b.mean()) * 1e-3) / 1e6, 1)}
             b.close()
         out["lbvh_build"] = builds
+        note("heat map (traversal steps in the reference's walk order, FP64)")
+        out["heatmap"] = bench_heatmap(torch, ctx, phr, max(3, args.steps // 4))
         note("traversal and builds timed; small render")
         out["render_small"] = bench_render_small(torch, ctx, max(3, args.steps // 4))
         note("batched builds and renders")

@@ -201,6 +201,25 @@ int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh*
                         const trace_camera* cam, const trace_render_params* prm, float* d_sum,
                         float* d_sumsq, float* d_last);
 
+/* ---- traversal cost view (bvh.go:21-44, render.go:137-197, 250-258) ---------- */
+/* BVH.traversalSteps (bvh.go:21-44) per ray: the reference's own stack walk (children
+ * pushed in stored order, FP64 slab and Moeller-Trumbore under the current best t), so
+ * the count -- inner nodes whose box the ray enters -- equals the reference's.  Rays are
+ * 8 doubles each (origin, tMin, direction, tMax).  Host buffers, synchronous. */
+int trace_traversal_steps(trace_ctx* ctx, const trace_bvh* bvh, const double* h_rays, int64_t n,
+                          int32_t* h_counts);
+int trace_traversal_steps_device(trace_ctx* ctx, const trace_bvh* bvh, const double* d_rays, int64_t n,
+                                 int32_t* d_counts);
+/* HeatMapRenderer.RenderToBuffer (render.go:167-197): one camera ray per pixel through
+ * (x / (W-1), y / (H-1)), tMin 0.001, tMax MaxFloat64, coloured by
+ * DefaultTraversalCountShader (render.go:250-258) with the given threshold.  h_rgb gets
+ * W*H*3 floats (pixel y*W + x, the PixelBuffer mean of its one sample); h_counts
+ * (optional, W*H) the step counts.  The _device variant is asynchronous. */
+int trace_render_heatmap(trace_ctx* ctx, const trace_bvh* bvh, const trace_camera* cam, int32_t width,
+                         int32_t height, int32_t threshold, float* h_rgb, int32_t* h_counts);
+int trace_render_heatmap_device(trace_ctx* ctx, const trace_bvh* bvh, const trace_camera* cam, int32_t width,
+                                int32_t height, int32_t threshold, float* d_rgb, int32_t* d_counts);
+
 /* Path segments (rays traced, every bounce) of the context's last trace_render*,
  * trace_render_batched or trace_render_backward call (the backward counts both of its
  * sweeps).  Waits for the context stream.  No reference counterpart: the reference's

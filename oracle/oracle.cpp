@@ -18,6 +18,7 @@
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off), output oracle/_build/.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -789,6 +790,142 @@ void oracle_phr_closest_hit(const oracle_phr* t, const float* rays, int64_t nray
       tout[i] = h.t;
     }
   });
+}
+
+}  // extern "C"
+
+namespace {
+// camera.go:113-135 newRay from FP64 components.
+Ray make_ray64(const double* r) {
+  Ray ray;
+  ray.o = {r[0], r[1], r[2]};
+  ray.d = {r[4], r[5], r[6]};
+  ray.inv = {1.0 / ray.d.x, 1.0 / ray.d.y, 1.0 / ray.d.z};
+  ray.sign[0] = ray.inv.x < 0;
+  ray.sign[1] = ray.inv.y < 0;
+  ray.sign[2] = ray.inv.z < 0;
+  return ray;
+}
+
+// bvh.go:21-44 traversalSteps: an explicit stack that pushes a node's children
+// in stored order (so the last child is popped first); an inner node whose box
+// the ray enters under the current best t counts one step and pushes its
+// children, a leaf tests its triangles, which tighten t.  A stack entry is an
+// LBVH node (aux = true) or a PHR node; PHR nodes of kind 2 are LBVH nodes.
+struct StepRef {
+  int32_t id;
+  bool aux;
+};
+
+int32_t steps_walk(const Tree& lb, const PhrTree* ph, StepRef root, const Ray& r, double tmin, double tmax) {
+  std::vector<StepRef> st{root};
+  int32_t count = 0;
+  HitRec h{};
+  h.t = tmax;
+  h.prim = -1;
+  while (!st.empty()) {
+    StepRef e = st.back();
+    st.pop_back();
+    if (!e.aux && ph->nodes[e.id].kind == 2) e = StepRef{ph->nodes[e.id].aux, true};
+    if (e.aux) {
+      const Node& nd = lb.nodes[e.id];
+      if (!box_hit(nd.box, r, tmin, h.t)) continue;
+      if (nd.leaf) {
+        for (int32_t i = nd.first; i <= nd.last; ++i) {
+          const int32_t prim = lb.perm[i];
+          tri_hit(&lb.pos[9 * size_t(prim)], r, tmin, h.t, h, prim);
+        }
+      } else {
+        st.push_back(StepRef{nd.child[0], true});
+        st.push_back(StepRef{nd.child[1], true});
+        ++count;
+      }
+    } else {
+      const PNode& p = ph->nodes[e.id];
+      if (!box_hit(p.box, r, tmin, h.t)) continue;
+      if (p.kind == 1) {
+        for (int32_t pos : p.prims) {
+          const int32_t prim = lb.perm[pos];
+          tri_hit(&lb.pos[9 * size_t(prim)], r, tmin, h.t, h, prim);
+        }
+      } else {
+        for (int32_t k : p.kids) st.push_back(StepRef{k, false});
+        ++count;
+      }
+    }
+  }
+  return count;
+}
+
+// render.go:250-258 DefaultTraversalCountShader
+void count_colour(int32_t count, int32_t threshold, float* rgb) {
+  if (count > threshold) {
+    rgb[0] = float(double(count) / double(threshold) * 2);
+    rgb[1] = 0.0f;
+    rgb[2] = 0.0f;
+    return;
+  }
+  const double f = double(count) / (double(threshold) * 1.25);
+  rgb[0] = 0.0f;
+  rgb[1] = float(f);
+  rgb[2] = float(1 - f);
+}
+
+// HeatMapRenderer.RenderToBuffer (render.go:167-197): pixel (x, y) casts the
+// camera ray through (x / (W-1), y / (H-1)) (camera.go:95-97) with tMin 0.001
+// and tMax = math.MaxFloat64; pixel index y * W + x.
+void heatmap(const Tree& lb, const PhrTree* ph, StepRef root, const double* cam, int32_t W, int32_t H,
+             int32_t threshold, float* rgb, int32_t* counts, int threads) {
+  parallel_for(int64_t(W) * H, threads, [&](int64_t b, int64_t e) {
+    for (int64_t pix = b; pix < e; ++pix) {
+      const int32_t x = int32_t(pix % W), y = int32_t(pix / W);
+      const double u = double(x) / double(W - 1);
+      const double v = double(y) / double(H - 1);
+      const double r8[8] = {cam[0], cam[1], cam[2], 0.0,
+                            cam[3] + cam[6] * u + cam[9] * v - cam[0],
+                            cam[4] + cam[7] * u + cam[10] * v - cam[1],
+                            cam[5] + cam[8] * u + cam[11] * v - cam[2], 0.0};
+      const int32_t c = steps_walk(lb, ph, root, make_ray64(r8), 0.001, DBL_MAX);
+      if (counts) counts[pix] = c;
+      count_colour(c, threshold, rgb + 3 * pix);
+    }
+  });
+}
+}  // namespace
+
+extern "C" {
+
+// BVH.traversalSteps (bvh.go:21-44) for a batch of FP64 rays (o, tMin, d, tMax).
+void oracle_traversal_steps(const oracle_tree* t, const double* rays, int64_t nrays, int32_t* counts,
+                            int threads) {
+  parallel_for(nrays, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const double* rr = rays + 8 * i;
+      counts[i] = steps_walk(*t, nullptr, StepRef{t->root, true}, make_ray64(rr), rr[3], rr[7]);
+    }
+  });
+}
+
+void oracle_phr_traversal_steps(const oracle_phr* t, const double* rays, int64_t nrays, int32_t* counts,
+                                int threads) {
+  parallel_for(nrays, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const double* rr = rays + 8 * i;
+      counts[i] = steps_walk(*t->aux, t, StepRef{t->root, false}, make_ray64(rr), rr[3], rr[7]);
+    }
+  });
+}
+
+// HeatMapRenderer.RenderToBuffer into RGB floats (one sample per pixel, so the
+// PixelBuffer mean is the colour) and the per-pixel counts (optional).
+void oracle_heatmap(const oracle_tree* t, const double* cam, int32_t W, int32_t H, int32_t threshold,
+                    float* rgb, int32_t* counts, int threads) {
+  heatmap(*t, nullptr, StepRef{t->root, true}, cam, W, H, threshold, rgb, counts, threads);
+}
+
+void oracle_phr_heatmap(const oracle_phr* t, const double* cam, int32_t W, int32_t H, int32_t threshold,
+                        float* rgb, int32_t* counts, int threads) {
+  heatmap(*t->aux, t, StepRef{t->root, false}, cam, W, H, threshold, rgb, counts, threads);
 }
 
 int64_t oracle_phr_split_work(const oracle_phr* t) { return t->split_work; }

@@ -52,6 +52,10 @@ def load() -> C.CDLL:
         "oracle_phr_initial_cut": ([_P], C.c_int64),
         "oracle_phr_canonical": ([_P, _P, _P, _P, _P], C.c_int64),
         "oracle_phr_closest_hit": ([_P, _P, C.c_int64, _P, _P, C.c_int], None),
+        "oracle_traversal_steps": ([_P, _P, C.c_int64, _P, C.c_int], None),
+        "oracle_phr_traversal_steps": ([_P, _P, C.c_int64, _P, C.c_int], None),
+        "oracle_heatmap": ([_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int], None),
+        "oracle_phr_heatmap": ([_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int], None),
         "oracle_render": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                            C.c_uint64, C.c_int64, C.c_int, _P, _P, _P, _P], None),
         "oracle_render_grad": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -100,6 +104,21 @@ def stable_sort(keys: np.ndarray):
     return s, perm
 
 
+def _steps(fn, h, rays: np.ndarray, threads: int | None) -> np.ndarray:
+    r = np.ascontiguousarray(rays, np.float64).reshape(-1, 8)
+    counts = np.empty(r.shape[0], np.int32)
+    fn(h, _p(r), r.shape[0], _p(counts), threads or _threads())
+    return counts
+
+
+def _heatmap(fn, h, cam, width: int, height: int, threshold: int, threads: int | None):
+    c = np.array([*cam.origin, *cam.lower_left, *cam.horizontal, *cam.vertical], np.float64)
+    rgb = np.empty((height, width, 3), np.float32)
+    counts = np.empty((height, width), np.int32)
+    fn(h, _p(c), width, height, threshold, _p(rgb), _p(counts), threads or _threads())
+    return rgb, counts
+
+
 class Tree:
     """Reference LBVH (top-down split, FP64 boxes) built by the oracle."""
 
@@ -145,6 +164,14 @@ class Tree:
         self.lib.oracle_closest_hit(self.h, _p(r), n, _p(hit), _p(t), threads or _threads())
         return hit, t
 
+    def traversal_steps(self, rays: np.ndarray, threads: int | None = None) -> np.ndarray:
+        """BVH.traversalSteps per FP64 ray (o, tMin, d, tMax)."""
+        return _steps(self.lib.oracle_traversal_steps, self.h, rays, threads)
+
+    def heatmap(self, cam, width: int, height: int, threshold: int, threads: int | None = None):
+        """HeatMapRenderer.RenderToBuffer: (rgb (H, W, 3) float32, counts (H, W))."""
+        return _heatmap(self.lib.oracle_heatmap, self.h, cam, width, height, threshold, threads)
+
 
 class PhrTree:
     """PhrBuilder.BuildFromAuxilary on an oracle LBVH (deterministic FIFO initial cut)."""
@@ -175,6 +202,12 @@ class PhrTree:
         t = np.empty(n, np.float64)
         self.lib.oracle_phr_closest_hit(self.h, _p(r), n, _p(hit), _p(t), threads or _threads())
         return hit, t
+
+    def traversal_steps(self, rays: np.ndarray, threads: int | None = None) -> np.ndarray:
+        return _steps(self.lib.oracle_phr_traversal_steps, self.h, rays, threads)
+
+    def heatmap(self, cam, width: int, height: int, threshold: int, threads: int | None = None):
+        return _heatmap(self.lib.oracle_phr_heatmap, self.h, cam, width, height, threshold, threads)
 
     def canonical(self):
         """(rec (n,4): kind, prim count or -1, n_children, 0; box (n,6) FP64;

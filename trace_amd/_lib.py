@@ -97,6 +97,10 @@ SIGNATURES = {
     "trace_mesh_bind_params": [_P, _P, _P, _P],
     "trace_render_backward": [_P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_segments": [_P, _P],
+    "trace_traversal_steps": [_P, _P, _P, _I64, _P],
+    "trace_traversal_steps_device": [_P, _P, _P, _I64, _P],
+    "trace_render_heatmap": [_P, _P, _P, _I32, _I32, _I32, _P, _P],
+    "trace_render_heatmap_device": [_P, _P, _P, _I32, _I32, _I32, _P, _P],
     "trace_adam_step": [_P, _P, _P, _P, _P, _I64, C.c_float, C.c_float, C.c_float, C.c_float, _I32, C.c_float,
                         C.c_float],
 }

@@ -242,6 +242,22 @@ class Bvh:
         self.ctx.check(self.ctx.lib.trace_closest_hit_device(self.ctx.h, self.h, C.c_void_p(d_rays), n,
                                                              C.c_void_p(d_hit), C.c_void_p(d_t)))
 
+    def traversal_steps(self, rays: np.ndarray) -> np.ndarray:
+        """BVH.traversalSteps per FP64 ray (origin, tMin, direction, tMax)."""
+        r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 8)
+        out = np.empty(r.shape[0], np.int32)
+        self.ctx.check(self.ctx.lib.trace_traversal_steps(self.ctx.h, self.h, _ptr(r), r.shape[0], _ptr(out)))
+        return out
+
+    def heatmap(self, cam, width: int, height: int, threshold: int):
+        """HeatMapRenderer.RenderToBuffer: (rgb (H, W, 3) float32, counts (H, W) int32)."""
+        rgb = np.empty((height, width, 3), np.float32)
+        counts = np.empty((height, width), np.int32)
+        c = camera_struct(cam)
+        self.ctx.check(self.ctx.lib.trace_render_heatmap(self.ctx.h, self.h, C.byref(c), width, height, threshold,
+                                                         _ptr(rgb), _ptr(counts)))
+        return rgb, counts
+
     def counts(self, rays: np.ndarray):
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         n = r.shape[0]

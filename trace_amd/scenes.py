@@ -84,11 +84,15 @@ def camera_rays(cam: Camera, width: int, height: int, tmin: float = T_MIN_CAMERA
     return r
 
 
+def camera_cfg2(width: int = 4096, height: int = 2048) -> Camera:
+    """Pinhole camera outside the unit cube looking at its centre; the cube
+    fills most of the frame."""
+    return Camera(width / height, 60.0, (0.5, 0.5, -0.6), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0))
+
+
 def coherent_rays_cfg2(width: int = 4096, height: int = 2048) -> np.ndarray:
-    """Primary rays of a pinhole camera outside the unit cube looking at its
-    centre; the cube fills most of the frame."""
-    cam = Camera(width / height, 60.0, (0.5, 0.5, -0.6), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0))
-    return camera_rays(cam, width, height)
+    """Primary rays of camera_cfg2."""
+    return camera_rays(camera_cfg2(width, height), width, height)
 
 
 MAT_DIFFUSE, MAT_LIGHT = 0, 1
