@@ -598,6 +598,13 @@ def main():
                             "mtris_per_s": round(args.tris / (float(per_b.mean()) * 1e-3) / 1e6, 1)}
             b.close()
         out["lbvh_build"] = builds
+        note("PHR grid optimiser (16 BuildWithCost + Cost)")
+        from trace_amd.device import phr_optimize_grid
+        t0 = time.perf_counter()
+        ga, gd, gsah, gbuild = phr_optimize_grid(ctx, lbvh, width=1920, height=1080)
+        out["phr_grid"] = {"ms": round((time.perf_counter() - t0) * 1e3, 1), "alpha": ga, "delta": gd,
+                           "builds": int(len(gsah)), "sah_min": round(float(gsah.min()), 4),
+                           "build_cost_min": int(gbuild.min())}
         note("heat map (traversal steps in the reference's walk order, FP64)")
         out["heatmap"] = bench_heatmap(torch, ctx, phr, max(3, args.steps // 4))
         note("traversal and builds timed; small render")

@@ -114,6 +114,8 @@ typedef struct {
   int64_t phr_initial_cut; /* PHR: size of the initial cut (0 for an LBVH) */
   int32_t phr_levels;      /* PHR: job levels processed */
   int32_t n_segments;      /* trees in a forest (1 for a single tree or a segment view) */
+  int64_t phr_build_cost;  /* PHR: BuildWithCost's cost (phr.go:154-255): the initial cut's
+                              size plus the size of every cut handed to SweepSAH */
 } trace_bvh_info;
 int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out);
 
@@ -141,6 +143,20 @@ int trace_bvh_size(trace_ctx* ctx, const trace_bvh* bvh, int64_t* size);
  * (its triangles in PHR leaf order); Cost/Size/export/closest-hit/render accept it. */
 int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,
                     int branching, int32_t max_cut, trace_bvh** out);
+/* PhrBuilder.BuildWithCost (phr.go:154-255): trace_phr_build plus its build cost -- the
+ * initial cut's size plus the size of every cut handed to SweepSAH. */
+int trace_phr_build_with_cost(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,
+                              int branching, int32_t max_cut, trace_bvh** out, int64_t* build_cost);
+/* gridOptimizer.OptimizedPHRparams (optimizer.go:44-94; NewDefaultGridOptimizer is alphas
+ * {0.4, 0.45, 0.5, 0.55} x deltas {6, 7, 8, 9}): BuildWithCost and Cost() for every
+ * (alpha, delta), alphas outer, then the (alpha, delta) of the first minimum of
+ * build / max build + omega^2 * Cost / max Cost with omega = (prims / 2e6)^2 *
+ * (frame_width * frame_height / 2e5).  h_sah and h_build (optional, n_alpha * n_delta)
+ * receive every evaluation.  Synchronous. */
+int trace_phr_optimize_grid(trace_ctx* ctx, const trace_bvh* aux, const double* h_alphas, int32_t n_alpha,
+                            const double* h_deltas, int32_t n_delta, int branching, int32_t max_cut,
+                            int32_t frame_width, int32_t frame_height, double* h_sah, int64_t* h_build,
+                            double* alpha_out, double* delta_out);
 /* For a PHR tree: per PHR-ordered triangle, its sorted position in the LBVH it was
  * built from (relates PHR leaves to the LBVH order). */
 int trace_phr_export_source(trace_ctx* ctx, const trace_bvh* phr, int32_t* h_src);

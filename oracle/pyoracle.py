@@ -190,7 +190,11 @@ class PhrTree:
         return float(self.lib.oracle_phr_cost(self.h))
 
     def split_work(self) -> int:
+        """BuildWithCost's cost (phr.go:154-255): initial cut + cuts split."""
         return int(self.lib.oracle_phr_split_work(self.h))
+
+    def build_cost(self) -> int:
+        return self.split_work()
 
     def initial_cut(self) -> int:
         return int(self.lib.oracle_phr_initial_cut(self.h))
@@ -219,6 +223,30 @@ class PhrTree:
         prims = np.empty(npr.value, np.int32)
         self.lib.oracle_phr_canonical(self.h, _p(rec), _p(box), _p(prims), C.byref(npr))
         return rec, box, prims
+
+
+def grid_optimize(tree: Tree, alphas, deltas, width: int, height: int, branching: int = 2,
+                  max_cut: int = 2500):
+    """gridOptimizer.OptimizedPHRparams (optimizer.go:44-94): BuildWithCost
+    (phr.go:154-255) for every (alpha, delta), alphas outer; each evaluation
+    scores buildCost / max buildCost + omega^2 * SAH / max SAH with omega =
+    (prims / PRIM_THRESH)^2 * pixels / RES_THRESH (optimizer.go:10-13, 87-94);
+    the first minimum wins.  Returns (alpha, delta, [(sah, build_cost)...])."""
+    evals = []
+    for a in alphas:
+        for d in deltas:
+            p = PhrTree(tree, a, d, branching, max_cut)
+            evals.append((p.cost(), p.build_cost(), float(a), float(d)))
+    max_build = max(e[1] for e in evals)
+    max_sah = max(e[0] for e in evals)
+    r = float(tree.n) / 2000000
+    omega = (r * r) * (float(width * height) / 200000)
+    best, best_cost = None, np.finfo(np.float64).max
+    for sah, build, a, d in evals:
+        c = float(build) / float(max_build) + (omega * omega) * (sah / max_sah)
+        if c < best_cost:
+            best_cost, best = c, (a, d)
+    return best[0], best[1], [(e[0], e[1]) for e in evals]
 
 
 def philox(counter, key) -> list[int]:

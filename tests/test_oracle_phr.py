@@ -121,6 +121,7 @@ class Phr:
 
     def build(self, aux_root):
         self.surface = aux_root.surface()
+        self.split_work = 0  # buildSubTreeCost's counter (phr.go:195-255)
         return self.sub_tree(self.initial_cut(aux_root))
 
     def sub_tree(self, cut):  # buildSubTree, phr.go:92-152
@@ -134,6 +135,7 @@ class Phr:
                     mx, mi = len(c["nodes"]), i
             if mx <= 1:
                 break
+            self.split_work += mx
             left, right = sweep_sah(cuts[mi])
             if right is not None:
                 cuts[mi] = self.refined(left)
@@ -246,7 +248,10 @@ def test_phr_oracle_equals_python_restatement(oracle, n, seed, alpha, delta, max
     assert np.array_equal(box, obox)
     assert np.array_equal(prims, oprims)
     assert want.cost() == cost(root)
-    assert want.initial_cut() == len(py.initial_cut(lbvh_from_oracle(aux))["nodes"])
+    n_cut = len(py.initial_cut(lbvh_from_oracle(aux))["nodes"])
+    assert want.initial_cut() == n_cut
+    # BuildWithCost's cost: the initial cut plus every cut handed to Split
+    assert want.build_cost() == n_cut + py.split_work
 
 
 def test_phr_duplicates_and_flat(oracle):
@@ -293,3 +298,25 @@ def test_phr_invariants_larger_tree(oracle):
         assert np.all(box[i, :3] <= t.min(axis=(0, 1))) and np.all(t.max(axis=(0, 1)) <= box[i, 3:])
         off += cnt
     assert phr.cost() < aux.cost()  # refinement lowers the SAH cost of the LBVH
+
+
+def test_grid_optimizer_selection(oracle):
+    """optimizer.go:44-94: with omega ~ 0 the score is the build cost alone,
+    with a huge omega the SAH term decides; the first minimum wins."""
+    from oracle.pyoracle import grid_optimize
+
+    pos = scenes.random_triangles(3000, seed=21, mean_edge=0.02)
+    tree = oracle.Tree(pos)
+    alphas, deltas = (0.4, 0.5, 0.55), (5.0, 6.0, 9.0)
+    a, d, ev = grid_optimize(tree, alphas, deltas, 2, 2)  # omega ~ 5e-10
+    sah = np.array([e[0] for e in ev])
+    build = np.array([e[1] for e in ev])
+    assert len(set(build.tolist())) > 1 and len(set(sah.tolist())) > 1
+    k = int(np.argmin(build))
+    assert (a, d) == (alphas[k // 3], deltas[k % 3])
+    a, d, _ = grid_optimize(tree, alphas, deltas, 3_000_000, 3_000_000)  # omega ~ 100
+    k = int(np.argmin(sah))
+    assert (a, d) == (alphas[k // 3], deltas[k % 3])
+    # every evaluation is BuildWithCost + Cost of that (alpha, delta)
+    p = oracle.PhrTree(tree, alphas[1], deltas[2])
+    assert ev[5] == (p.cost(), p.build_cost())

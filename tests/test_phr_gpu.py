@@ -130,6 +130,47 @@ def test_phr_terrain_big_leaves_split(ctx, oracle):
     assert tp.mean() <= 2.0 * tl.mean() + 4.0, (tp.mean(), tl.mean())
 
 
+@pytest.mark.parametrize("alpha,delta", [(0.5, 6.0), (0.4, 9.0), (0.55, 7.0)])
+def test_build_with_cost_matches_oracle(ctx, oracle, alpha, delta):
+    """PhrBuilder.BuildWithCost (phr.go:154-255): the build cost (initial cut +
+    every cut handed to SweepSAH) equals the oracle's, and so does Cost()."""
+    print(f"phr build cost alpha={alpha} delta={delta}", flush=True)
+    pos = scenes.random_triangles(50_000, seed=23)
+    aux = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    phr, cost = Bvh.phr_with_cost(ctx, aux, alpha, delta)
+    ref = oracle.PhrTree(oracle.Tree(pos), alpha, delta)
+    assert cost == ref.build_cost()
+    assert phr.info().phr_build_cost == cost
+    assert phr.cost() == pytest.approx(ref.cost(), rel=1e-12)
+
+
+def test_build_with_cost_small_trees(ctx, oracle):
+    for n in (1, 2, 3):
+        pos = scenes.random_triangles(n, seed=24)
+        _, cost = Bvh.phr_with_cost(ctx, Bvh.lbvh(ctx, Mesh(ctx, pos)))
+        assert cost == oracle.PhrTree(oracle.Tree(pos)).build_cost(), n
+
+
+def test_grid_optimizer_matches_oracle(ctx, oracle):
+    """gridOptimizer.OptimizedPHRparams over the default 4 x 4 grid at 1920x1080:
+    every evaluation and the chosen (alpha, delta) equal the oracle's."""
+    from oracle.pyoracle import grid_optimize
+    from trace_amd.device import phr_optimize_grid
+
+    print("phr grid optimizer", flush=True)
+    pos = scenes.random_triangles(20_000, seed=25)
+    aux = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    a, d, sah, build = phr_optimize_grid(ctx, aux)
+    oa, od, ev = grid_optimize(oracle.Tree(pos), (0.4, 0.45, 0.5, 0.55), (6.0, 7.0, 8.0, 9.0), 1920, 1080)
+    assert np.array_equal(build, np.array([e[1] for e in ev]))
+    assert np.allclose(sah, np.array([e[0] for e in ev]), rtol=1e-12, atol=0)
+    assert (a, d) == (oa, od)
+    # a frame large enough that the SAH term decides
+    a2, d2, _, _ = phr_optimize_grid(ctx, aux, width=100_000, height=100_000)
+    oa2, od2, _ = grid_optimize(oracle.Tree(pos), (0.4, 0.45, 0.5, 0.55), (6.0, 7.0, 8.0, 9.0), 100_000, 100_000)
+    assert (a2, d2) == (oa2, od2)
+
+
 def test_phr_rejects_unsupported(ctx):
     pos = scenes.random_triangles(100, seed=1)
     aux = Bvh.lbvh(ctx, Mesh(ctx, pos))

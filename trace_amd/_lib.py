@@ -50,6 +50,7 @@ class BvhInfo(C.Structure):
         ("phr_initial_cut", C.c_int64),
         ("phr_levels", C.c_int32),
         ("n_segments", C.c_int32),
+        ("phr_build_cost", C.c_int64),
     ]
 
 
@@ -97,6 +98,8 @@ SIGNATURES = {
     "trace_mesh_bind_params": [_P, _P, _P, _P],
     "trace_render_backward": [_P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_segments": [_P, _P],
+    "trace_phr_build_with_cost": [_P, _P, C.c_double, C.c_double, C.c_int, _I32, _P, _P],
+    "trace_phr_optimize_grid": [_P, _P, _P, _I32, _P, _I32, C.c_int, _I32, _I32, _I32, _P, _P, _P, _P],
     "trace_traversal_steps": [_P, _P, _P, _I64, _P],
     "trace_traversal_steps_device": [_P, _P, _P, _I64, _P],
     "trace_render_heatmap": [_P, _P, _P, _I32, _I32, _I32, _P, _P],

@@ -1,4 +1,6 @@
 // C-ABI entry points (include/trace_hip.h).
+#include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -278,6 +280,7 @@ int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out
     out->depth = bvh->host_depth;
     out->phr_initial_cut = bvh->phr_initial_cut;
     out->phr_levels = bvh->phr_levels;
+    out->phr_build_cost = bvh->phr_initial_cut + bvh->phr_split_work;
   }
   return TRACE_OK;
 }
@@ -342,6 +345,65 @@ int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double d
     return rc;
   }
   *out = b;
+  return TRACE_OK;
+}
+
+int trace_phr_build_with_cost(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,
+                              int branching, int32_t max_cut, trace_bvh** out, int64_t* build_cost) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!build_cost) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build_with_cost: null cost output");
+  int rc = trace_phr_build(ctx, aux, alpha, delta, branching, max_cut, out);
+  if (rc) return rc;
+  *build_cost = (*out)->phr_initial_cut + (*out)->phr_split_work;
+  return TRACE_OK;
+}
+
+int trace_phr_optimize_grid(trace_ctx* ctx, const trace_bvh* aux, const double* h_alphas, int32_t n_alpha,
+                            const double* h_deltas, int32_t n_delta, int branching, int32_t max_cut,
+                            int32_t frame_width, int32_t frame_height, double* h_sah, int64_t* h_build,
+                            double* alpha_out, double* delta_out) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!aux || !h_alphas || !h_deltas || n_alpha < 1 || n_delta < 1 || frame_width < 1 || frame_height < 1 ||
+      !alpha_out || !delta_out)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_optimize_grid: bad arguments");
+  const int64_t n_eval = int64_t(n_alpha) * n_delta;
+  std::vector<double> sah(n_eval);
+  std::vector<int64_t> build(n_eval);
+  // every (alpha, delta), alphas outer (optimizer.go:47-61)
+  for (int32_t i = 0; i < n_alpha; ++i) {
+    for (int32_t j = 0; j < n_delta; ++j) {
+      const int64_t k = int64_t(i) * n_delta + j;
+      trace_bvh* t = nullptr;
+      int rc = trace_phr_build_with_cost(ctx, aux, h_alphas[i], h_deltas[j], branching, max_cut, &t, &build[k]);
+      if (rc) return rc;
+      rc = trace_bvh_cost(ctx, t, &sah[k]);
+      trace_bvh_destroy(t);
+      if (rc) return rc;
+    }
+  }
+  // scores (optimizer.go:63-94): build / max build + omega^2 * SAH / max SAH,
+  // omega = (prims / PRIM_THRESH)^2 * pixels / RES_THRESH; the first minimum wins
+  int64_t max_build = 0;
+  double max_sah = 0.0;
+  for (int64_t k = 0; k < n_eval; ++k) {
+    max_build = std::max(max_build, build[k]);
+    max_sah = std::max(max_sah, sah[k]);
+  }
+  const double r = double(aux->n_prims) / 2000000.0;
+  const double omega = (r * r) * (double(int64_t(frame_width) * frame_height) / 200000.0);
+  double best = DBL_MAX;
+  int64_t best_k = 0;
+  for (int64_t k = 0; k < n_eval; ++k) {
+    const double c = double(build[k]) / double(max_build) + (omega * omega) * (sah[k] / max_sah);
+    if (c < best) {
+      best = c;
+      best_k = k;
+    }
+  }
+  *alpha_out = h_alphas[best_k / n_delta];
+  *delta_out = h_deltas[best_k % n_delta];
+  if (h_sah) std::copy(sah.begin(), sah.end(), h_sah);
+  if (h_build) std::copy(build.begin(), build.end(), h_build);
   return TRACE_OK;
 }
 
@@ -487,16 +549,96 @@ int trace_bvh_export_canonical(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap
   return TRACE_OK;
 }
 
-static double surface(const float* b) {
-  const double w = double(b[3]) - double(b[0]);
-  const double h = double(b[4]) - double(b[1]);
-  const double d = double(b[5]) - double(b[2]);
-  return 2 * w * h + 2 * w * d + 2 * h * d;  // bounding.go:45-47
+// bounding.go:45-47, FP64 from FP32 corners
+__host__ __device__ inline double surface(double lx, double ly, double lz, double hx, double hy, double hz) {
+  const double w = hx - lx, h = hy - ly, d = hz - lz;
+  return 2 * w * h + 2 * w * d + 2 * h * d;
+}
+__host__ __device__ inline double surface_f(const float* b) { return surface(b[0], b[1], b[2], b[3], b[4], b[5]); }
+static double surface(const float* b) { return surface_f(b); }
+
+// Cost() telescoped: with p(c) = SA(c) / SA(parent), a node's weight in the
+// root's cost is the product of p along its path, SA(node) / SA(root).  So
+// cost(root) = 2 + (sum over non-root nodes of w(node) SA(node)) / SA(root),
+// w = TRAVERSAL_COST for inner nodes, INTERSECTION_COST * prims for leaves.
+// (A zero-area node's children have zero area too, so the reference's
+// skipping of them changes nothing.)  Each thread adds the child terms of one
+// binary node; blocks reduce in LDS, then one block sums the partials in
+// order: the same result on every run.
+__global__ __launch_bounds__(256) void cost_terms_kernel(const BvhNode* __restrict__ nodes,
+                                                         const BvhHeader* __restrict__ hdr,
+                                                         double* __restrict__ partial) {
+  __shared__ double s[256];
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  double v = 0.0;
+  if (i < hdr->n_inner) {
+    const float* b = reinterpret_cast<const float*>(nodes + i);
+    for (int c = 0; c < 2; ++c) {
+      const int32_t cnt = nodes[i].cnt[c];
+      v += (cnt > 0 ? double(cnt) : 2.0) * surface_f(b + 6 * c);
+    }
+  }
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void cost_sum_kernel(const double* __restrict__ partial, int64_t n,
+                                                       double* __restrict__ out) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += 256) v += partial[k];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+static int device_cost(trace_ctx* ctx, const trace_bvh* bvh, double* cost) {
+  BvhHeader h;
+  int rc = read_header(ctx, bvh, &h);
+  if (rc) return rc;
+  if (h.root_cnt > 0) {  // a leaf root
+    *cost = double(h.root_cnt);
+    return TRACE_OK;
+  }
+  const double sa_root = surface(h.root_lo[0], h.root_lo[1], h.root_lo[2], h.root_hi[0], h.root_hi[1],
+                                 h.root_hi[2]);
+  if (sa_root == 0) {
+    *cost = 2.0;
+    return TRACE_OK;
+  }
+  const int64_t blocks = std::max<int64_t>(1, ceil_div(int64_t(h.n_inner), 256));
+  ScratchPlan plan;
+  const size_t o_part = plan.take(blocks * sizeof(double));
+  const size_t o_out = plan.take(sizeof(double));
+  rc = ctx_reserve_scratch(ctx, plan.off);
+  if (rc) return rc;
+  char* base = static_cast<char*>(ctx->scratch);
+  auto* partial = reinterpret_cast<double*>(base + o_part);
+  auto* d_out = reinterpret_cast<double*>(base + o_out);
+  cost_terms_kernel<<<int(blocks), 256, 0, ctx->stream>>>(bvh->nodes.p, bvh->header.p, partial);
+  cost_sum_kernel<<<1, 256, 0, ctx->stream>>>(partial, blocks, d_out);
+  double sum = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&sum, d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *cost = 2.0 + sum / sa_root;
+  return ctx_check_device_error(ctx, "trace_bvh_cost");
 }
 
 int trace_bvh_cost(trace_ctx* ctx, const trace_bvh* bvh, double* cost) {
   if (!ctx) return TRACE_E_INVALID;
   if (!bvh || !cost) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_bvh_cost: null argument");
+  // a whole binary tree reduces on the device; a forest's tree view walks its
+  // own nodes on the host
+  if (!bvh->base && bvh->nseg == 1 && bvh->arity == 2) return device_cost(ctx, bvh, cost);
   int64_t n = 0;
   int rc = trace_bvh_export_canonical(ctx, bvh, 0, nullptr, nullptr, &n);
   if (rc) return rc;

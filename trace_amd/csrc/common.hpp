@@ -200,6 +200,7 @@ struct trace_bvh {
   bool phr_clone = false;  // the PHR tree is the LBVH itself (initial cut of one node)
   double phr_alpha = 0, phr_delta = 0;
   int64_t phr_initial_cut = 0;
+  int64_t phr_split_work = 0;  // sizes of the cuts handed to SweepSAH (BuildWithCost's counter)
   int32_t phr_levels = 0;
   // upper bound on inner nodes along any root-to-leaf path (sizes the
   // traversal stack); an LBVH over 64-bit keys never exceeds 63

@@ -815,7 +815,11 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   BvhHeader ah;
   HIP_TRY(ctx, hipMemcpyAsync(&ah, aux->header.p, sizeof(ah), hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
-  if (ah.root_cnt > 0) return clone_lbvh(ctx, aux, out);  // a one-leaf LBVH
+  out->phr_split_work = 0;
+  if (ah.root_cnt > 0) {  // a one-leaf LBVH: the initial cut is that leaf
+    out->phr_initial_cut = 1;
+    return clone_lbvh(ctx, aux, out);
+  }
 
   const int32_t U = ah.n_leaves;
   const int64_t F = 2 * int64_t(U) - 1;
@@ -897,6 +901,10 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const Caps caps{pool_cap, out_cap, list_cap, task_cap};
   uint32_t n_tasks = 1;
   uint32_t max_m = hc.cut;  // the one job of the first level holds the initial cut
+  // Every job holds a cut of at least two members and makes one split attempt
+  // (branching 2), so BuildWithCost's counter (phr.go:221) gains the level's
+  // pool size: the initial cut, then what each level appends for the next.
+  uint64_t level_pool = hc.cut;
   int cur = 0;
   int levels = 0;
   while (n_tasks > 0) {
@@ -909,6 +917,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
       while (uint32_t(lds_p) < max_m) lds_p <<= 1;
     }
     const size_t lds_bytes = size_t(lds_p) * (sizeof(double) + sizeof(int32_t));
+    out->phr_split_work += int64_t(level_pool);
     reset_level_kernel<<<1, 1, 0, st>>>(ctl);
     task_kernel<<<int(n_tasks), 64, lds_bytes, st>>>(f, d_th, tasks[cur], pools[cur], pools[cur ^ 1],
                                                     tasks[cur ^ 1], ord, gkey, rcost, outn, list, ctl,
@@ -918,6 +927,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x at level %d", hc.err, levels);
     n_tasks = hc.n_tasks;
     max_m = hc.max_m;
+    level_pool = hc.pool;
     cur ^= 1;
     ++levels;
   }

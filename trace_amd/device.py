@@ -162,6 +162,18 @@ class Bvh:
         b._aux = aux
         return b
 
+    @classmethod
+    def phr_with_cost(cls, ctx: Context, aux: "Bvh", alpha: float = 0.5, delta: float = 6.0, branching: int = 2,
+                      max_cut: int = 2500):
+        """PhrBuilder.BuildWithCost: (tree, build cost) (trace_phr_build_with_cost)."""
+        h = C.c_void_p()
+        cost = C.c_int64()
+        ctx.check(ctx.lib.trace_phr_build_with_cost(ctx.h, aux.h, alpha, delta, branching, max_cut, C.byref(h),
+                                                    C.byref(cost)))
+        b = cls(ctx, h, aux.n_prims)
+        b._aux = aux
+        return b, int(cost.value)
+
     def segment(self, seg: int) -> "Bvh":
         """View of tree `seg` of a forest (trace_bvh_segment)."""
         h = C.c_void_p()
@@ -323,6 +335,20 @@ def render_segments(ctx: Context) -> int:
     out = C.c_int64()
     ctx.check(ctx.lib.trace_render_segments(ctx.h, C.byref(out)))
     return out.value
+
+
+def phr_optimize_grid(ctx: Context, aux: Bvh, alphas=(0.4, 0.45, 0.5, 0.55), deltas=(6.0, 7.0, 8.0, 9.0),
+                      width: int = 1920, height: int = 1080, branching: int = 2, max_cut: int = 2500):
+    """gridOptimizer.OptimizedPHRparams (trace_phr_optimize_grid): (alpha, delta,
+    SAH costs, build costs), evaluations alphas-major."""
+    a = np.ascontiguousarray(alphas, np.float64)
+    d = np.ascontiguousarray(deltas, np.float64)
+    sah = np.empty(len(a) * len(d), np.float64)
+    build = np.empty(len(a) * len(d), np.int64)
+    ao, do = C.c_double(), C.c_double()
+    ctx.check(ctx.lib.trace_phr_optimize_grid(ctx.h, aux.h, _ptr(a), len(a), _ptr(d), len(d), branching, max_cut,
+                                              width, height, _ptr(sah), _ptr(build), C.byref(ao), C.byref(do)))
+    return ao.value, do.value, sah, build
 
 
 def adam_step(ctx: Context, d_param: int, d_grad: int, d_m: int, d_v: int, n: int, lr: float, step: int,
