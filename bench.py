@@ -306,6 +306,36 @@ def bench_heatmap(torch, ctx, bvh, steps):
             "mean_steps": round(float(counts.mean()), 2), "max_steps": int(counts.max())}
 
 
+def bench_scene_compile(torch, ctx, steps):
+    """Scene.collectTracables on the device for the large configuration's scene
+    graph: the 2048x2048 terrain mesh plus 5,000 placements (scale and
+    translate) of one 400-triangle sphere mesh -> 10,380,418 triangles with
+    shading; then the LBVH of the result."""
+    from trace_amd import scenes
+    from trace_amd.device import Bvh, Scene
+
+    meshes, mm, parent, node_mesh, mats = scenes.terrain_scene_graph()
+    materials = np.array([[scenes.MAT_DIFFUSE, 0.5, 0.5, 0.5], [scenes.MAT_DIFFUSE, 0.8, 0.8, 0.8]], np.float32)
+    t0 = time.perf_counter()
+    sc = Scene(ctx, meshes, mm, materials)
+    upload_ms = (time.perf_counter() - t0) * 1e3
+    mesh = sc.compile(parent, node_mesh, mats)
+    torch.cuda.synchronize()
+    per, _ = time_launches(torch, lambda: sc.compile(parent, node_mesh, mats, mesh=mesh), steps, 1)
+    bvh = Bvh.lbvh(ctx, mesh)
+    ctx.synchronize()
+    # the same scene as flattened on the host by the generator (FP32-rounded there)
+    ref = scenes.terrain_scene()["pos"]
+    err = float(np.abs(mesh.export() - ref).max())
+    out = {"tris": int(mesh.ntri), "nodes": int(len(parent)), "source_tris": int(sum(len(m) for m in meshes)),
+           "upload_ms": round(upload_ms, 1), "compile_ms": round(float(np.median(per)), 3),
+           "max_abs_diff_vs_host_flatten": err, "lbvh_n_leaves": int(bvh.info().n_leaves)}
+    bvh.close()
+    mesh.close()
+    sc.close()
+    return out
+
+
 def bench_large(torch, ctx, steps=2):
     """The large configuration: 10.38M triangles (2048x2048 fBm terrain + 5000
     400-triangle spheres), PHR tree, 1920x1080, 64 spp, depth 8, sky miss;
@@ -619,6 +649,8 @@ def main():
         if not args.no_large:
             note("large scene (10.4M triangles) and inverse-rendering steps")
             out["large"] = bench_large(torch, ctx)
+            note("large scene compiled on the device from its scene graph")
+            out["scene_compile"] = bench_scene_compile(torch, ctx, max(3, args.steps // 4))
 
     if not args.no_cpu:
         note("CPU baseline")

@@ -31,6 +31,7 @@ extern "C" {
 typedef struct trace_ctx trace_ctx;
 typedef struct trace_mesh trace_mesh;
 typedef struct trace_bvh trace_bvh;
+typedef struct trace_scene trace_scene;
 
 enum {
   TRACE_OK = 0,
@@ -216,6 +217,33 @@ int trace_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
 int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
                         const trace_camera* cam, const trace_render_params* prm, float* d_sum,
                         float* d_sumsq, float* d_last);
+
+/* ---- scene compile (scene.go:7-151, primitive.go:152-170) ----------------------- */
+/* The meshes of a scene graph (Mesh{geometry, material}, scene.go:116-139), uploaded
+ * once: mesh k is triangles [h_mesh_off[k], h_mesh_off[k+1]) of h_pos (9 doubles per
+ * triangle) with vertex normals h_nrm (9 doubles per triangle; NULL: geometric normals
+ * of the flattened triangles) and material h_mesh_mat[k] of the table h_materials
+ * (nmat x (kind, r, g, b), Diffuse or Light). */
+int trace_scene_create(trace_ctx* ctx, const double* h_pos, const double* h_nrm, const int64_t* h_mesh_off,
+                       int32_t n_mesh, const int32_t* h_mesh_mat, const float* h_materials, int32_t nmat,
+                       trace_scene** out);
+/* Scene.Compile's collectTracables (scene.go:21-36, 95-114) on the device.  The
+ * SceneNode tree below Scene.root (identity, no mesh): node k has parent h_parent[k]
+ * (-1: a child of the root; otherwise an earlier node), mesh h_node_mesh[k] (-1: none)
+ * and transformation h_matrix[16k .. 16k+15] (Matrix4, row-major, math.go:144-162);
+ * siblings keep index order.  Flattened depth first (a node's mesh, then its children)
+ * with t = node.transformation x parent's t, positions transformed as points, normals
+ * by (t^T)^-1 as points, not renormalised (primitive.go:152-170), and meshes whose own
+ * node matrix is the identity copied untransformed, as the reference does.  Writes a
+ * mesh with shading data ready for trace_lbvh_build: *io_mesh NULL creates it; an
+ * existing mesh of the same triangle count is refilled in place. */
+int trace_scene_compile(trace_ctx* ctx, const trace_scene* scene, const int32_t* h_parent,
+                        const int32_t* h_node_mesh, const double* h_matrix, int32_t n_nodes,
+                        trace_mesh** io_mesh);
+int trace_scene_destroy(trace_scene* scene);
+/* A mesh's shading data as the renderer holds it: FP32 vertex normals (9 per triangle)
+ * and material indices; either output may be NULL. */
+int trace_mesh_export_shading(trace_ctx* ctx, const trace_mesh* mesh, float* h_nrm, int32_t* h_mat_idx);
 
 /* ---- traversal cost view (bvh.go:21-44, render.go:137-197, 250-258) ---------- */
 /* BVH.traversalSteps (bvh.go:21-44) per ray: the reference's own stack walk (children

@@ -928,6 +928,158 @@ void oracle_phr_heatmap(const oracle_phr* t, const double* cam, int32_t W, int32
   heatmap(*t->aux, t, StepRef{t->root, false}, cam, W, H, threshold, rgb, counts, threads);
 }
 
+}  // extern "C"
+
+namespace {
+// ---- scene compile (scene.go:95-151, primitive.go:152-170, math.go:144-320) ----
+struct M4 {
+  double m[16];
+};
+
+M4 m4_identity() {
+  M4 r{};
+  r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1;
+  return r;
+}
+
+// MultiplyMatrix (math.go:299-320)
+M4 m4_mul(const M4& A, const M4& B) {
+  const double* a = A.m;
+  const double* b = B.m;
+  M4 r;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      r.m[4 * i + j] = a[4 * i + 0] * b[0 + j] + a[4 * i + 1] * b[4 + j] + a[4 * i + 2] * b[8 + j] +
+                       a[4 * i + 3] * b[12 + j];
+  return r;
+}
+
+M4 m4_transpose(const M4& A) {
+  M4 r;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) r.m[4 * j + i] = A.m[4 * i + j];
+  return r;
+}
+
+// Inverse (math.go:164-297): adjugate by cofactors, each a signed sum of six
+// triple products evaluated left to right, then scaled by 1 / det.
+M4 m4_inverse(const M4& A) {
+  const double* m = A.m;
+  double v[16];
+  v[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+         m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  v[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+         m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  v[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+         m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  v[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+          m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  v[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+         m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  v[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+         m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  v[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+         m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  v[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+          m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  v[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+         m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  v[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+         m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  v[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+          m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  v[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+          m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  v[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+         m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  v[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+         m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  v[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+          m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  v[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+          m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  double det = m[0] * v[0] + m[1] * v[4] + m[2] * v[8] + m[3] * v[12];
+  det = 1.0 / det;
+  M4 r;
+  for (int k = 0; k < 16; ++k) r.m[k] = v[k] * det;
+  return r;
+}
+
+// Vector4.Transformed of the point (x, y, z, 1), then ToV3 (math.go:108-162)
+void m4_point(const M4& A, const double* p, double* out) {
+  const double* m = A.m;
+  for (int r = 0; r < 3; ++r) out[r] = m[4 * r] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3] * 1.0;
+}
+
+struct SceneIn {
+  const double* pos;
+  const double* nrm;
+  const int64_t* off;
+  const int32_t* mesh_mat;
+  const int32_t* node_mesh;
+  const double* matrix;
+  std::vector<std::vector<int32_t>> kids;  // [n_nodes] = the root
+};
+
+// collectTracables (scene.go:95-114): this node's mesh (raw when its own
+// transformation is the identity, else transformed by t), then its children.
+void collect(const SceneIn& s, int32_t node, const M4& parent_t, bool is_root, std::vector<double>& pos,
+             std::vector<double>& nrm, std::vector<int32_t>& mat) {
+  M4 own = m4_identity();
+  if (!is_root) std::memcpy(own.m, s.matrix + 16 * size_t(node), sizeof(own.m));
+  const M4 t = m4_mul(own, parent_t);
+  const int32_t mesh = is_root ? -1 : s.node_mesh[node];
+  if (mesh >= 0) {
+    bool ident = true;
+    const M4 I = m4_identity();
+    for (int k = 0; k < 16; ++k) ident = ident && own.m[k] == I.m[k];
+    const M4 tinv = m4_inverse(m4_transpose(t));
+    for (int64_t i = s.off[mesh]; i < s.off[mesh + 1]; ++i) {
+      for (int v = 0; v < 3; ++v) {
+        double p[3], n[3];
+        const double* sp = s.pos + 9 * i + 3 * v;
+        if (ident) {
+          std::memcpy(p, sp, sizeof(p));
+        } else {
+          m4_point(t, sp, p);
+        }
+        pos.insert(pos.end(), p, p + 3);
+        if (s.nrm) {
+          const double* sn = s.nrm + 9 * i + 3 * v;
+          if (ident) std::memcpy(n, sn, sizeof(n));
+          else m4_point(tinv, sn, n);
+          nrm.insert(nrm.end(), n, n + 3);
+        }
+      }
+      mat.push_back(s.mesh_mat[mesh]);
+    }
+  }
+  for (int32_t c : s.kids[is_root ? s.kids.size() - 1 : size_t(node)]) collect(s, c, t, false, pos, nrm, mat);
+}
+}  // namespace
+
+extern "C" {
+
+// Scene.collectTracables from Scene.root (identity, no mesh) over a node tree:
+// node k has parent[k] (-1: the root's child), node_mesh[k] (-1: none) and
+// matrix[16k ..] (row-major Matrix4).  Writes positions (9 doubles), normals
+// (9 doubles, if nrm is given) and material per flattened triangle when the
+// outputs are non-NULL; returns the triangle count.
+int64_t oracle_scene_flatten(const double* pos, const double* nrm, const int64_t* off, const int32_t* mesh_mat,
+                             const int32_t* parent, const int32_t* node_mesh, const double* matrix,
+                             int32_t n_nodes, double* out_pos, double* out_nrm, int32_t* out_mat) {
+  SceneIn s{pos, nrm, off, mesh_mat, node_mesh, matrix, {}};
+  s.kids.resize(size_t(n_nodes) + 1);
+  for (int32_t k = 0; k < n_nodes; ++k) s.kids[parent[k] < 0 ? size_t(n_nodes) : size_t(parent[k])].push_back(k);
+  std::vector<double> p, n;
+  std::vector<int32_t> m;
+  collect(s, n_nodes, m4_identity(), true, p, n, m);
+  if (out_pos) std::memcpy(out_pos, p.data(), p.size() * sizeof(double));
+  if (out_nrm && nrm) std::memcpy(out_nrm, n.data(), n.size() * sizeof(double));
+  if (out_mat) std::memcpy(out_mat, m.data(), m.size() * sizeof(int32_t));
+  return int64_t(m.size());
+}
+
 int64_t oracle_phr_split_work(const oracle_phr* t) { return t->split_work; }
 int64_t oracle_phr_initial_cut(const oracle_phr* t) { return t->initial_cut; }
 int64_t oracle_phr_max_split(const oracle_phr* t) { return t->max_split; }

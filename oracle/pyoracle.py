@@ -53,6 +53,7 @@ def load() -> C.CDLL:
         "oracle_phr_canonical": ([_P, _P, _P, _P, _P], C.c_int64),
         "oracle_phr_closest_hit": ([_P, _P, C.c_int64, _P, _P, C.c_int], None),
         "oracle_traversal_steps": ([_P, _P, C.c_int64, _P, C.c_int], None),
+        "oracle_scene_flatten": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, _P, _P, _P], C.c_int64),
         "oracle_phr_traversal_steps": ([_P, _P, C.c_int64, _P, C.c_int], None),
         "oracle_heatmap": ([_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int], None),
         "oracle_phr_heatmap": ([_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int], None),
@@ -247,6 +248,30 @@ def grid_optimize(tree: Tree, alphas, deltas, width: int, height: int, branching
         if c < best_cost:
             best_cost, best = c, (a, d)
     return best[0], best[1], [(e[0], e[1]) for e in evals]
+
+
+def scene_flatten(meshes, mesh_materials, parent, node_mesh, matrices, normals=None):
+    """Scene.collectTracables (scene.go:95-151, primitive.go:152-170) in FP64:
+    (positions (n, 9), normals (n, 9) or None, material per triangle)."""
+    lib = load()
+    parts = [np.ascontiguousarray(m, np.float64).reshape(-1, 9) for m in meshes]
+    pos = np.ascontiguousarray(np.concatenate(parts))
+    off = np.zeros(len(parts) + 1, np.int64)
+    off[1:] = np.cumsum([len(p) for p in parts])
+    nrm = None
+    if normals is not None:
+        nrm = np.ascontiguousarray(np.concatenate([np.asarray(n, np.float64).reshape(-1, 9) for n in normals]))
+    mm = np.ascontiguousarray(mesh_materials, np.int32)
+    par = np.ascontiguousarray(parent, np.int32)
+    nm = np.ascontiguousarray(node_mesh, np.int32)
+    mx = np.ascontiguousarray(matrices, np.float64).reshape(-1, 16)
+    args = [_p(pos), None if nrm is None else _p(nrm), _p(off), _p(mm), _p(par), _p(nm), _p(mx), len(par)]
+    n = int(lib.oracle_scene_flatten(*args, None, None, None))
+    out_pos = np.empty((n, 9), np.float64)
+    out_nrm = np.empty((n, 9), np.float64) if nrm is not None else None
+    out_mat = np.empty(n, np.int32)
+    lib.oracle_scene_flatten(*args, _p(out_pos), None if out_nrm is None else _p(out_nrm), _p(out_mat))
+    return out_pos, out_nrm, out_mat
 
 
 def philox(counter, key) -> list[int]:

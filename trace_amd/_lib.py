@@ -98,6 +98,10 @@ SIGNATURES = {
     "trace_mesh_bind_params": [_P, _P, _P, _P],
     "trace_render_backward": [_P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_segments": [_P, _P],
+    "trace_scene_create": [_P, _P, _P, _P, _I32, _P, _P, _I32, _P],
+    "trace_scene_compile": [_P, _P, _P, _P, _P, _I32, _P],
+    "trace_scene_destroy": [_P],
+    "trace_mesh_export_shading": [_P, _P, _P, _P],
     "trace_phr_build_with_cost": [_P, _P, C.c_double, C.c_double, C.c_int, _I32, _P, _P],
     "trace_phr_optimize_grid": [_P, _P, _P, _I32, _P, _I32, C.c_int, _I32, _I32, _I32, _P, _P, _P, _P],
     "trace_traversal_steps": [_P, _P, _P, _I64, _P],

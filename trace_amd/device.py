@@ -116,6 +116,13 @@ class Mesh:
         self.ctx.check(self.ctx.lib.trace_mesh_set_shading(self.ctx.h, self.h, None if nrm is None else _ptr(nrm),
                                                            _ptr(mi), _ptr(mats), mats.shape[0]))
 
+    def export_shading(self):
+        """(normals (n, 9) float32, material index (n,)) as the renderer holds them."""
+        nrm = np.empty((self.ntri, 9), np.float32)
+        mi = np.empty(self.ntri, np.int32)
+        self.ctx.check(self.ctx.lib.trace_mesh_export_shading(self.ctx.h, self.h, _ptr(nrm), _ptr(mi)))
+        return nrm, mi
+
     def update(self, pos: np.ndarray) -> None:
         p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 9)
         if p.shape[0] != self.ntri:
@@ -125,6 +132,56 @@ class Mesh:
     def close(self) -> None:
         if self.h:
             self.ctx.lib.trace_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Meshes of a scene graph on the device (trace_scene_create); compile()
+    flattens a node tree into a Mesh with shading (trace_scene_compile)."""
+
+    def __init__(self, ctx: Context, meshes, mesh_materials, materials, normals=None):
+        self.ctx = ctx
+        parts = [np.ascontiguousarray(m, np.float64).reshape(-1, 9) for m in meshes]
+        pos = np.ascontiguousarray(np.concatenate(parts))
+        off = np.zeros(len(parts) + 1, np.int64)
+        off[1:] = np.cumsum([len(p) for p in parts])
+        nrm = None
+        if normals is not None:
+            nrm = np.ascontiguousarray(np.concatenate([np.asarray(n, np.float64).reshape(-1, 9) for n in normals]))
+        mm = np.ascontiguousarray(mesh_materials, np.int32)
+        mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 4)
+        self.sizes = [len(p) for p in parts]
+        h = C.c_void_p()
+        ctx.check(ctx.lib.trace_scene_create(ctx.h, _ptr(pos), None if nrm is None else _ptr(nrm), _ptr(off),
+                                             len(parts), _ptr(mm), _ptr(mats), mats.shape[0], C.byref(h)))
+        self.h = h
+
+    def compile(self, parent, node_mesh, matrices, mesh: Mesh | None = None) -> Mesh:
+        """Scene.collectTracables on the device; matrices (n, 16) row-major.
+        Refills `mesh` in place when given (same triangle count)."""
+        par = np.ascontiguousarray(parent, np.int32)
+        nm = np.ascontiguousarray(node_mesh, np.int32)
+        mx = np.ascontiguousarray(matrices, np.float64).reshape(-1, 16)
+        h = C.c_void_p(mesh.h.value if mesh is not None else None)
+        self.ctx.check(self.ctx.lib.trace_scene_compile(self.ctx.h, self.h, _ptr(par), _ptr(nm), _ptr(mx), len(par),
+                                                        C.byref(h)))
+        if mesh is not None:
+            return mesh
+        m = Mesh.__new__(Mesh)
+        m.ctx = self.ctx
+        m.h = h
+        m.ntri = int(sum(self.sizes[k] for k in nm if k >= 0))  # every mesh occurrence is emitted
+        return m
+
+    def close(self) -> None:
+        if self.h:
+            self.ctx.lib.trace_scene_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -278,6 +278,39 @@ def terrain_scene(n: int = 2048, octaves: int = 8, seed: int = 5, n_spheres: int
             "camera": cam, "n_terrain": n_terrain, "width": width, "height": height}
 
 
+def terrain_scene_graph(n: int = 2048, octaves: int = 8, seed: int = 5, n_spheres: int = 5000):
+    """The large configuration as a scene graph (SceneNode tree under
+    Scene.root): node 0 holds the terrain mesh (identity), nodes 1..n_spheres
+    each place the one 400-triangle unit sphere mesh with a scale-and-translate
+    matrix.  Same sphere radii and positions as terrain_scene (before its FP32
+    rounding).  Returns meshes, mesh materials, parent, node_mesh, matrices
+    (n_nodes, 16)."""
+    hf = fbm_heightfield(n, octaves, seed).astype(np.float64)
+    g = np.linspace(-1.0, 1.0, n)
+    X, Z = np.meshgrid(g, g)
+    V = np.stack([X, hf, Z], axis=-1)
+    a, b = V[:-1, :-1], V[:-1, 1:]
+    c, d = V[1:, 1:], V[1:, :-1]
+    t1 = np.concatenate([a, d, c], axis=-1).reshape(-1, 9)
+    t2 = np.concatenate([a, c, b], axis=-1).reshape(-1, 9)
+    terrain = np.stack([t1, t2], axis=1).reshape(-1, 9)
+    rng = np.random.default_rng(seed + 1)
+    unit = uv_sphere((0.0, 0.0, 0.0), 1.0, 20, 12)
+    r = rng.uniform(0.006, 0.02, n_spheres)
+    px = rng.uniform(-0.95, 0.95, n_spheres)
+    pz = rng.uniform(-0.95, 0.95, n_spheres)
+    col = np.clip(np.round((px + 1.0) / 2.0 * (n - 1)).astype(np.int64), 0, n - 1)
+    row = np.clip(np.round((pz + 1.0) / 2.0 * (n - 1)).astype(np.int64), 0, n - 1)
+    py = hf[row, col] + r
+    mats = np.zeros((n_spheres + 1, 4, 4))
+    mats[:] = np.eye(4)
+    mats[1:, 0, 0] = mats[1:, 1, 1] = mats[1:, 2, 2] = r
+    mats[1:, 0, 3], mats[1:, 1, 3], mats[1:, 2, 3] = px, py, pz
+    parent = np.full(n_spheres + 1, -1, np.int32)
+    node_mesh = np.concatenate([[0], np.ones(n_spheres, np.int32)]).astype(np.int32)
+    return [terrain, unit], [0, 1], parent, node_mesh, mats.reshape(-1, 16)
+
+
 def parse_obj(path: str) -> tuple[np.ndarray, np.ndarray | None]:
     """OBJ triangles as obj.go:19-106 reads them: 'v', 'vn', 'f' (fan
     triangulation, negative indices relative to the current count, normals only
