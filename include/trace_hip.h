@@ -54,7 +54,7 @@ enum { TRACE_KEY_REF63 = 0, TRACE_KEY_M30 = 1 };
 enum { TRACE_MISS_DEFAULT = 0, TRACE_MISS_WHITE = 1, TRACE_MISS_SKY = 2, TRACE_MISS_SUN = 3 };
 
 /* Material kinds (material.go:13-42). */
-enum { TRACE_MAT_DIFFUSE = 0, TRACE_MAT_LIGHT = 1 };
+enum { TRACE_MAT_DIFFUSE = 0, TRACE_MAT_LIGHT = 1, TRACE_MAT_REFLECTIVE = 2, TRACE_MAT_REFRACTIVE = 3 };
 
 int trace_version(void);
 
@@ -217,6 +217,15 @@ int trace_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
 int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
                         const trace_camera* cam, const trace_render_params* prm, float* d_sum,
                         float* d_sumsq, float* d_last);
+
+/* trace_mesh_set_shading with all four material kinds of material.go:13-106: 8 floats
+ * per material, kind, r, g, b (Albedo / Color), then the parameter -- Reflective.Diffusion
+ * (>= 0) or Refractive.Ratio (> 0), else ignored -- and 3 reserved (0).  Reflective and
+ * Refractive scatter as the reference does (reflect, refract with Schlick's reflectance,
+ * material.go:44-106); the draw that chooses reflection is the vertex's own Philox
+ * stream.  Renders accept them; trace_render_backward does not. */
+int trace_mesh_set_shading_ex(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
+                              const int32_t* h_mat_index, const float* h_materials8, int32_t nmat);
 
 /* ---- scene compile (scene.go:7-151, primitive.go:152-170) ----------------------- */
 /* The meshes of a scene graph (Mesh{geometry, material}, scene.go:116-139), uploaded

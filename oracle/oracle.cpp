@@ -1168,6 +1168,8 @@ struct Shading {
   // order; they replace the material table's colour
   const float* tri_alb = nullptr;
   const float* tri_emi = nullptr;
+  // per material: Reflective.Diffusion / Refractive.Ratio (null: none in use)
+  const float* params = nullptr;
 
   V3 colour(int32_t prim) const {
     const float* m = mats + 4 * mat_idx[prim];
@@ -1208,20 +1210,69 @@ bool trace_ray(const Tree& t, V3 o, V3 d, double tmin, HitRec& h) {
   return node_hit(t, t.root, r, tmin, INFINITY, h);
 }
 
-// Diffuse.scatter (material.go:29-38): the scattered direction at vertex k.
-V3 scatter_dir(const Shading& s, uint64_t path, int k, const HitRec& h) {
+// The hit's interpolated vertex normal, flipped to face the ray
+// (primitive.go:145-150, 200-205); not renormalised.
+V3 facing_normal(const Shading& s, const HitRec& h) {
   const double* n = s.nrm + 9 * size_t(h.prim);
   const double u = h.u, v = h.v, w0 = 1 - u - v;
   V3 nn{(n[3] * u + n[6] * v) + n[0] * w0, (n[4] * u + n[7] * v) + n[1] * w0,
         (n[5] * u + n[8] * v) + n[2] * w0};
   if (!h.front) nn = {-nn.x, -nn.y, -nn.z};
-  const Philox4 r = path_draw(s.seed, path, uint32_t(k + 1));
+  return nn;
+}
+
+// The uniform unit vector of vertex k (its first two Philox words).
+V3 unit_draw(const Philox4& r) {
   const double z = 1.0 - 2.0 * double(u01(r.v[0]));
   const double rad = std::sqrt(std::max(0.0, 1.0 - z * z));
   const double phi = 6.283185307179586 * double(u01(r.v[1]));
-  V3 dir{nn.x + rad * std::cos(phi), nn.y + rad * std::sin(phi), nn.z + z};
+  return {rad * std::cos(phi), rad * std::sin(phi), z};
+}
+
+// Diffuse.scatter (material.go:29-38): the scattered direction at vertex k.
+V3 scatter_dir(const Shading& s, uint64_t path, int k, const HitRec& h) {
+  const V3 nn = facing_normal(s, h);
+  const V3 w = unit_draw(path_draw(s.seed, path, uint32_t(k + 1)));
+  V3 dir{nn.x + w.x, nn.y + w.y, nn.z + w.z};
   if (std::fabs(dir.x) <= 1e-7 && std::fabs(dir.y) <= 1e-7 && std::fabs(dir.z) <= 1e-7) dir = nn;
   return dir;
+}
+
+// reflect (material.go:90-92)
+V3 reflect_v(V3 v, V3 n) {
+  const double k = dot(v, n) * 2;
+  return {v.x - n.x * k, v.y - n.y * k, v.z - n.z * k};
+}
+
+// Reflective.scatter / Refractive.scatter (material.go:44-106) at vertex k:
+// false when the path ends here (a reflection below the surface).
+bool specular_dir(const Shading& s, int kind, uint64_t path, int k, V3 d, const HitRec& h, V3& dir) {
+  const double prm = s.params ? double(s.params[s.mat_idx[h.prim]]) : 0.0;
+  const V3 n = facing_normal(s, h);
+  const double len = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);  // math.go Unit
+  const V3 unit{d.x / len, d.y / len, d.z / len};
+  const Philox4 r = path_draw(s.seed, path, uint32_t(k + 1));
+  const V3 refl = reflect_v(unit, n);
+  if (kind == 2) {
+    const V3 w = unit_draw(r);
+    dir = {refl.x + w.x * prm, refl.y + w.y * prm, refl.z + w.z * prm};
+    return dot(refl, n) > 0;
+  }
+  const double ratio = h.front ? 1 / prm : prm;
+  const V3 neg{-unit.x, -unit.y, -unit.z};
+  const double cos_t = std::min(dot(neg, n), 1.0);
+  const double sin_t = std::sqrt(1.0 - cos_t * cos_t);
+  double r0 = (1 - ratio) / (1 + ratio);
+  r0 = r0 * r0;
+  const double reflectance = r0 + (1 - r0) * std::pow(1 - cos_t, 5);  // Schlick
+  if (ratio * sin_t > 1.0 || reflectance > double(u01(r.v[2]))) {
+    dir = refl;
+  } else {
+    const V3 perp{(unit.x + n.x * cos_t) * ratio, (unit.y + n.y * cos_t) * ratio, (unit.z + n.z * cos_t) * ratio};
+    const double par = -std::sqrt(std::fabs(1.0 - dot(perp, perp)));
+    dir = {perp.x + n.x * par, perp.y + n.y * par, perp.z + n.z * par};
+  }
+  return true;
 }
 
 // DefaultClosestHitShader (render.go:209-224): vertex at depth k.
@@ -1229,7 +1280,10 @@ V3 shade(const Tree& t, const Shading& s, uint64_t path, int k, V3 o, V3 d, cons
   if (k > s.max_depth) return {0, 0, 0};
   const V3 c = s.colour(h.prim);
   if (s.light(h.prim)) return c;  // Light: emission, no scatter
-  const V3 dir = scatter_dir(s, path, k, h);
+  const int kind = int(s.mats[4 * s.mat_idx[h.prim]]);
+  V3 dir;
+  if (kind == 0) dir = scatter_dir(s, path, k, h);
+  else if (!specular_dir(s, kind, path, k, d, h, dir)) return {0, 0, 0};
   const V3 p{o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t};
   HitRec h2{};
   V3 next;
@@ -1310,11 +1364,11 @@ static V3 camera_dir(const double* cam, int32_t W, int32_t H, int64_t pix, uint6
 // floats per triangle) replace the material colour of Diffuse / Light
 // triangles.  path_out (optional) receives every path's value, indexed
 // [sample][pixel][rgb].
-void oracle_render(const oracle_tree* t, const double* nrm, const int32_t* mat_idx,
-                   const float* mats, const float* tri_alb, const float* tri_emi, const double* cam,
-                   int32_t W, int32_t H, int32_t spp, int32_t max_depth, int32_t miss, uint64_t seed,
-                   int64_t sample_offset, int threads, double* sum, double* sumsq, double* last,
-                   double* path_out) {
+static void render_impl(const oracle_tree* t, const double* nrm, const int32_t* mat_idx, const float* mats,
+                        const float* params, const float* tri_alb, const float* tri_emi, const double* cam,
+                        int32_t W, int32_t H, int32_t spp, int32_t max_depth, int32_t miss, uint64_t seed,
+                        int64_t sample_offset, int threads, double* sum, double* sumsq, double* last,
+                        double* path_out) {
   std::vector<double> geo;
   if (!nrm) {
     geo = geo_normals(t);
@@ -1323,6 +1377,7 @@ void oracle_render(const oracle_tree* t, const double* nrm, const int32_t* mat_i
   Shading s{nrm, mat_idx, mats, miss, max_depth, seed};
   s.tri_alb = tri_alb;
   s.tri_emi = tri_emi;
+  s.params = params;
   const V3 org{cam[0], cam[1], cam[2]};
   const int64_t wh = int64_t(W) * H;
   parallel_for(wh, threads, [&](int64_t b, int64_t e) {
@@ -1353,6 +1408,29 @@ void oracle_render(const oracle_tree* t, const double* nrm, const int32_t* mat_i
       }
     }
   });
+}
+
+void oracle_render(const oracle_tree* t, const double* nrm, const int32_t* mat_idx,
+                   const float* mats, const float* tri_alb, const float* tri_emi, const double* cam,
+                   int32_t W, int32_t H, int32_t spp, int32_t max_depth, int32_t miss, uint64_t seed,
+                   int64_t sample_offset, int threads, double* sum, double* sumsq, double* last,
+                   double* path_out) {
+  render_impl(t, nrm, mat_idx, mats, nullptr, tri_alb, tri_emi, cam, W, H, spp, max_depth, miss, seed,
+              sample_offset, threads, sum, sumsq, last, path_out);
+}
+
+// The same with all four material kinds (material.go:13-106): 8 floats per
+// material, kind, r, g, b, then Reflective.Diffusion / Refractive.Ratio.
+void oracle_render_ex(const oracle_tree* t, const double* nrm, const int32_t* mat_idx, const float* mats8,
+                      int32_t nmat, const double* cam, int32_t W, int32_t H, int32_t spp, int32_t max_depth,
+                      int32_t miss, uint64_t seed, int threads, double* sum, double* sumsq, double* last) {
+  std::vector<float> mats(4 * size_t(nmat)), params(nmat);
+  for (int32_t m = 0; m < nmat; ++m) {
+    for (int c = 0; c < 4; ++c) mats[4 * size_t(m) + c] = mats8[8 * size_t(m) + c];
+    params[m] = mats8[8 * size_t(m) + 4];
+  }
+  render_impl(t, nrm, mat_idx, mats.data(), params.data(), nullptr, nullptr, cam, W, H, spp, max_depth, miss, seed, 0,
+              threads, sum, sumsq, last, nullptr);
 }
 
 // Gradient of a loss on the mean image (sum / spp) of the render above with

@@ -59,6 +59,8 @@ def load() -> C.CDLL:
         "oracle_phr_heatmap": ([_P, _P, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int], None),
         "oracle_render": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                            C.c_uint64, C.c_int64, C.c_int, _P, _P, _P, _P], None),
+        "oracle_render_ex": ([_P, _P, _P, _P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                              C.c_uint64, C.c_int, _P, _P, _P], None),
         "oracle_render_grad": ([_P, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                 C.c_int32, C.c_uint64, C.c_int64, C.c_int, _P, _P, _P, _P], None),
     }
@@ -298,8 +300,18 @@ def render(tree: Tree, mat_idx, materials, cam, width: int, height: int, spp: in
     (sum, sumsq, last[, paths]), each (height, width, 3) float64 (paths:
     (spp, height, width, 3))."""
     mi = np.ascontiguousarray(mat_idx, np.int32)
-    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 4)
     nrm = None if normals is None else np.ascontiguousarray(normals, np.float64)
+    if np.asarray(materials).reshape(len(materials), -1).shape[1] == 8:
+        # (kind, r, g, b, param, 0, 0, 0): Reflective / Refractive too
+        m8 = np.ascontiguousarray(materials, np.float32).reshape(-1, 8)
+        c = np.ascontiguousarray(np.concatenate([cam.origin, cam.lower_left, cam.horizontal, cam.vertical]),
+                                 np.float64)
+        shape = (height, width, 3)
+        s, q, last = (np.empty(shape, np.float64) for _ in range(3))
+        load().oracle_render_ex(tree.h, None if nrm is None else _p(nrm), _p(mi), _p(m8), m8.shape[0], _p(c), width,
+                                height, spp, max_depth, miss, seed, threads or _threads(), _p(s), _p(q), _p(last))
+        return s, q, last
+    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 4)
     alb, palb = _opt_f32(albedo, len(mi))
     emi, pemi = _opt_f32(emission, len(mi))
     c = np.ascontiguousarray(np.concatenate([cam.origin, cam.lower_left, cam.horizontal, cam.vertical]), np.float64)

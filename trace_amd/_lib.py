@@ -22,7 +22,7 @@ KEY_REF63 = 0
 KEY_M30 = 1
 
 MISS_DEFAULT, MISS_WHITE, MISS_SKY, MISS_SUN = 0, 1, 2, 3
-MAT_DIFFUSE, MAT_LIGHT = 0, 1
+MAT_DIFFUSE, MAT_LIGHT, MAT_REFLECTIVE, MAT_REFRACTIVE = 0, 1, 2, 3
 
 _P = C.c_void_p
 _I64 = C.c_int64
@@ -98,6 +98,7 @@ SIGNATURES = {
     "trace_mesh_bind_params": [_P, _P, _P, _P],
     "trace_render_backward": [_P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_segments": [_P, _P],
+    "trace_mesh_set_shading_ex": [_P, _P, _P, _P, _P, _I32],
     "trace_scene_create": [_P, _P, _P, _P, _I32, _P, _P, _I32, _P],
     "trace_scene_compile": [_P, _P, _P, _P, _P, _I32, _P],
     "trace_scene_destroy": [_P],

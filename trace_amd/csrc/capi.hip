@@ -755,7 +755,33 @@ int trace_mesh_set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_nor
       TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_mesh_set_shading: material %d kind %g (only Diffuse and Light)",
                  m, double(kind));
   }
-  return render::set_shading(ctx, mesh, h_normals, h_mat_index, h_materials, nmat);
+  return render::set_shading(ctx, mesh, h_normals, h_mat_index, h_materials, nmat, nullptr);
+}
+
+int trace_mesh_set_shading_ex(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
+                              const int32_t* h_mat_index, const float* h_materials8, int32_t nmat) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !h_mat_index || !h_materials8 || nmat <= 0)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_set_shading_ex: need material indices and nmat > 0");
+  for (int64_t i = 0; i < mesh->ntri; ++i)
+    if (h_mat_index[i] < 0 || h_mat_index[i] >= nmat)
+      TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_set_shading_ex: triangle %lld has material %d of %d",
+                 (long long)i, h_mat_index[i], nmat);
+  std::vector<float> mats(4 * size_t(nmat)), params(nmat);
+  for (int32_t m = 0; m < nmat; ++m) {
+    const float* e = h_materials8 + 8 * size_t(m);
+    const float kind = e[0];
+    if (kind != float(TRACE_MAT_DIFFUSE) && kind != float(TRACE_MAT_LIGHT) && kind != float(TRACE_MAT_REFLECTIVE) &&
+        kind != float(TRACE_MAT_REFRACTIVE))
+      TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_mesh_set_shading_ex: material %d kind %g", m, double(kind));
+    if (kind == float(TRACE_MAT_REFLECTIVE) && !(e[4] >= 0.0f))
+      TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_set_shading_ex: material %d: Diffusion %g < 0", m, double(e[4]));
+    if (kind == float(TRACE_MAT_REFRACTIVE) && !(e[4] > 0.0f))
+      TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_set_shading_ex: material %d: Ratio %g <= 0", m, double(e[4]));
+    for (int c = 0; c < 4; ++c) mats[4 * size_t(m) + c] = e[c];
+    params[m] = (kind == float(TRACE_MAT_REFLECTIVE) || kind == float(TRACE_MAT_REFRACTIVE)) ? e[4] : 0.0f;
+  }
+  return render::set_shading(ctx, mesh, h_normals, h_mat_index, mats.data(), nmat, params.data());
 }
 
 static int check_render_args(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,

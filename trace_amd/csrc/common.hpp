@@ -111,7 +111,9 @@ struct trace_mesh {
   DevArray<float> nrm;        // 9 floats per triangle: vertex normals n0, n1, n2
   DevArray<int32_t> mat_idx;  // material index per triangle
   DevArray<float4> mats;      // (kind, r, g, b) per material
+  DevArray<float> mat_param;  // Reflective.Diffusion / Refractive.Ratio per material (else 0)
   int32_t nmat = 0;
+  bool has_specular = false;  // a Reflective or Refractive material is in use
   bool has_shading = false;
   bool geo_normals = false;  // nrm derived from pos (recomputed when pos animates)
   // bound per-triangle parameters (caller-owned device memory, 3 floats per

@@ -131,6 +131,8 @@ int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
   if (!mesh || !bvh || !cam || !prm || !d_grad_image || !d_grad_albedo || !d_grad_emission)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: null argument");
   if (!mesh->has_shading) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: mesh has no shading data");
+  if (mesh->has_specular)
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_backward: Reflective / Refractive materials are not differentiated");
   if (!bvh->base && bvh->nseg > 1)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: differentiate a forest through its segment views");
   const int64_t tree_tris = bvh->base ? bvh->base->n_prims : bvh->n_prims;

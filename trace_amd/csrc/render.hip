@@ -16,6 +16,7 @@
 // Forward accumulation of throughput x emission gives the same value as the
 // reference's recursion L = Le + albedo (x) L_next.
 #include <algorithm>
+#include <vector>
 
 #include "philox.hpp"
 #include "render.hpp"
@@ -243,6 +244,7 @@ struct SegArgs {
   const float* nrm;
   const int32_t* mat_idx;
   const float4* mats;
+  const float* mat_param;  // per material: Reflective.Diffusion / Refractive.Ratio
   int32_t k, max_depth, miss;
   int64_t path_base;
   float4 *ro, *rd, *thr, *rad;
@@ -314,17 +316,65 @@ __device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p
         ny = -ny;
         nz = -nz;
       }
-      // n + uniform unit vector (util.go:15-26 draws a uniform direction)
+      // a uniform unit vector (util.go:15-26 draws a uniform direction)
       const Philox4 r = path_draw(seed, uint64_t(path_id), uint32_t(k + 1));
       const float z = 1.0f - 2.0f * u01(r.v[0]);
       const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
       float sphi, cphi;
       sincosf(6.2831853071795864f * u01(r.v[1]), &sphi, &cphi);
-      float sx = nx + rr * cphi, sy = ny + rr * sphi, sz = nz + z;
-      if (fabsf(sx) <= 1e-7f && fabsf(sy) <= 1e-7f && fabsf(sz) <= 1e-7f) {
-        sx = nx;
-        sy = ny;
-        sz = nz;
+      const float ux = rr * cphi, uy = rr * sphi, uz = z;
+      const int kind = int(m.x);
+      float sx, sy, sz;
+      if (kind == TRACE_MAT_DIFFUSE) {
+        // Diffuse.scatter (material.go:29-38): n + unit vector, n if that is ~0
+        sx = nx + ux;
+        sy = ny + uy;
+        sz = nz + uz;
+        if (fabsf(sx) <= 1e-7f && fabsf(sy) <= 1e-7f && fabsf(sz) <= 1e-7f) {
+          sx = nx;
+          sy = ny;
+          sz = nz;
+        }
+      } else {
+        // unit incoming direction (math.go Unit)
+        const float dl = sqrtf(d4.x * d4.x + d4.y * d4.y + d4.z * d4.z);
+        const float vx = d4.x / dl, vy = d4.y / dl, vz = d4.z / dl;
+        const float vn = vx * nx + vy * ny + vz * nz;
+        // reflect (material.go:90-92): v - n (2 v.n)
+        const float rx = vx - nx * (vn * 2.0f), ry = vy - ny * (vn * 2.0f), rz = vz - nz * (vn * 2.0f);
+        const float prm = g.mat_param[mat_idx[prim]];
+        if (kind == TRACE_MAT_REFLECTIVE) {
+          // Reflective.scatter (material.go:50-54): reflection + Diffusion x unit
+          // vector; no scatter (the path ends dark) unless reflected . n > 0
+          if (!(rx * nx + ry * ny + rz * nz > 0.0f)) return;
+          sx = rx + ux * prm;
+          sy = ry + uy * prm;
+          sz = rz + uz * prm;
+        } else {
+          // Refractive.scatter (material.go:65-84) with Schlick's reflectance
+          // (material.go:101-106); the draw deciding reflection is this
+          // vertex's third Philox word
+          const float ratio = h.front ? 1.0f / prm : prm;
+          const float cos_t = fminf(-vx * nx + -vy * ny + -vz * nz, 1.0f);
+          const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+          float r0 = (1.0f - ratio) / (1.0f + ratio);
+          r0 = r0 * r0;
+          const float om = 1.0f - cos_t;
+          const float refl = r0 + (1.0f - r0) * (om * om * om * om * om);
+          if (ratio * sin_t > 1.0f || refl > u01(r.v[2])) {
+            sx = rx;
+            sy = ry;
+            sz = rz;
+          } else {
+            // refract (material.go:94-99)
+            const float px = (vx + nx * cos_t) * ratio, py = (vy + ny * cos_t) * ratio,
+                        pz = (vz + nz * cos_t) * ratio;
+            const float par = -sqrtf(fabsf(1.0f - (px * px + py * py + pz * pz)));
+            sx = px + nx * par;
+            sy = py + ny * par;
+            sz = pz + nz * par;
+          }
+        }
       }
       // new origin = ray.position(t) (camera.go:108-111), evaluated in FP64,
       // then kept on the side of the triangle's plane the new ray leaves to
@@ -360,7 +410,7 @@ __device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p
       } else {
         thr[p] = make_float4(T.x * c.x, T.y * c.y, T.z * c.z, 0.0f);
       }
-      if (adj.mode == kModeReplay) {
+      if (adj.mode == kModeReplay && kind == TRACE_MAT_DIFFUSE) {
         // dL/da_j = S / a_j when the path has no zero albedo; with exactly
         // one zero albedo only that vertex has a gradient, S itself
         const float4 S = rad[p];
@@ -449,12 +499,20 @@ __global__ void accumulate_kernel(const float4* __restrict__ rad, int64_t wh, in
 }
 
 int set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
-                const int32_t* h_mat_index, const float* h_materials, int32_t nmat) {
+                const int32_t* h_mat_index, const float* h_materials, int32_t nmat, const float* h_params) {
   const int64_t n = mesh->ntri;
   hipStream_t st = ctx->stream;
   HIP_TRY(ctx, mesh->nrm.alloc(9 * n));
   HIP_TRY(ctx, mesh->mat_idx.alloc(n));
   HIP_TRY(ctx, mesh->mats.alloc(nmat));
+  HIP_TRY(ctx, mesh->mat_param.alloc(nmat));
+  std::vector<float> params(h_params ? h_params : nullptr, h_params ? h_params + nmat : nullptr);
+  params.resize(size_t(nmat), 0.0f);
+  mesh->has_specular = false;
+  for (int32_t m = 0; m < nmat; ++m)
+    if (h_materials[4 * m] == float(TRACE_MAT_REFLECTIVE) || h_materials[4 * m] == float(TRACE_MAT_REFRACTIVE))
+      mesh->has_specular = true;
+  HIP_TRY(ctx, hipMemcpyAsync(mesh->mat_param.p, params.data(), nmat * sizeof(float), hipMemcpyHostToDevice, st));
   if (h_normals) {
     DevArray<double> tmp;
     HIP_TRY(ctx, tmp.alloc(9 * n));
@@ -645,6 +703,7 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
   g.nrm = mesh->nrm.p;
   g.mat_idx = mesh->mat_idx.p;
   g.mats = mesh->mats.p;
+  g.mat_param = mesh->mat_param.p;
   g.k = k;
   g.max_depth = prm->max_depth;
   g.miss = prm->miss;

@@ -5,7 +5,7 @@
 
 namespace render {
 int set_shading(trace_ctx* ctx, trace_mesh* mesh, const double* h_normals,
-                const int32_t* h_mat_index, const float* h_materials, int32_t nmat);
+                const int32_t* h_mat_index, const float* h_materials, int32_t nmat, const float* h_params);
 // Recompute geometric vertex normals from the current positions (asynchronous).
 int refresh_geo_normals(trace_ctx* ctx, trace_mesh* mesh);
 // Enqueues a render of prm->spp samples per pixel; writes per-pixel sums,

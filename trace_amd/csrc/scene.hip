@@ -303,11 +303,13 @@ int trace_scene_compile(trace_ctx* ctx, const trace_scene* sc, const int32_t* h_
   if (e == hipSuccess) e = mesh->nrm.alloc(9 * dst);
   if (e == hipSuccess) e = mesh->mat_idx.alloc(dst);
   if (e == hipSuccess) e = mesh->mats.alloc(sc->nmat);
+  if (e == hipSuccess) e = mesh->mat_param.alloc(sc->nmat);
   if (e == hipSuccess) e = d_inst.alloc(inst.size());
   if (e == hipSuccess)
     e = hipMemcpyAsync(d_inst.p, inst.data(), inst.size() * sizeof(scene::Inst), hipMemcpyHostToDevice, st_);
   if (e == hipSuccess)
     e = hipMemcpyAsync(mesh->mats.p, sc->mats.data(), sc->nmat * sizeof(float4), hipMemcpyHostToDevice, st_);
+  if (e == hipSuccess) fill_async<float>(st_, mesh->mat_param.p, size_t(sc->nmat), 0.0f);
   if (e == hipSuccess) {
     scene::flatten_kernel<<<int(ceil_div(dst, 256)), 256, 0, st_>>>(d_inst.p, int32_t(inst.size()), sc->pos.p,
                                                                      sc->nrm.p, dst, mesh->pos.p, mesh->nrm.p,
@@ -316,6 +318,7 @@ int trace_scene_compile(trace_ctx* ctx, const trace_scene* sc, const int32_t* h_
   }
   mesh->nmat = sc->nmat;
   mesh->has_shading = true;
+  mesh->has_specular = false;
   mesh->geo_normals = sc->nrm.p == nullptr;
   int rc = TRACE_OK;
   if (e == hipSuccess && mesh->geo_normals) rc = render::refresh_geo_normals(ctx, mesh);

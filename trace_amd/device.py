@@ -116,6 +116,17 @@ class Mesh:
         self.ctx.check(self.ctx.lib.trace_mesh_set_shading(self.ctx.h, self.h, None if nrm is None else _ptr(nrm),
                                                            _ptr(mi), _ptr(mats), mats.shape[0]))
 
+    def set_shading_ex(self, mat_idx, materials8, normals=None) -> None:
+        """All four material kinds: materials8 rows are (kind, r, g, b, param,
+        0, 0, 0) with param = Reflective.Diffusion or Refractive.Ratio."""
+        mi = np.ascontiguousarray(mat_idx, np.int32)
+        mats = np.ascontiguousarray(materials8, np.float32).reshape(-1, 8)
+        if mi.shape[0] != self.ntri:
+            raise ValueError("one material index per triangle")
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float64).reshape(-1, 9)
+        self.ctx.check(self.ctx.lib.trace_mesh_set_shading_ex(self.ctx.h, self.h, None if nrm is None else _ptr(nrm),
+                                                              _ptr(mi), _ptr(mats), mats.shape[0]))
+
     def export_shading(self):
         """(normals (n, 9) float32, material index (n,)) as the renderer holds them."""
         nrm = np.empty((self.ntri, 9), np.float32)

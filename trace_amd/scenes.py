@@ -95,7 +95,7 @@ def coherent_rays_cfg2(width: int = 4096, height: int = 2048) -> np.ndarray:
     return camera_rays(camera_cfg2(width, height), width, height)
 
 
-MAT_DIFFUSE, MAT_LIGHT = 0, 1
+MAT_DIFFUSE, MAT_LIGHT, MAT_REFLECTIVE, MAT_REFRACTIVE = 0, 1, 2, 3
 
 
 def _quad(p0, p1, p2, p3) -> np.ndarray:
@@ -188,6 +188,40 @@ def box_scene(seed: int = 1, slices: int = 100, rings: int = 102, width: int = 6
     pos = pos.astype(np.float32).astype(np.float64)
     return {"pos": pos, "mat_idx": mat_idx, "materials": materials, "camera": cam, "width": width,
             "height": height}
+
+
+def specular_box_scene(seed: int = 1, slices: int = 48, rings: int = 26, width: int = 320, height: int = 180):
+    """The box and area light of box_scene with two smooth UV spheres instead of
+    one: a mirror (Reflective, albedo 0.9, Diffusion 0.05) on the left and a
+    glass ball (Refractive, albedo 0.95, Ratio 1.5) on the right, both with
+    analytic vertex normals (CornellBox-like, cornellbox.go:5-78).  Vertices
+    are FP32-exact.  Returns pos, normals, mat_idx, materials8 (kind, r, g, b,
+    param, 0, 0, 0), camera, width, height."""
+    base = box_scene(seed=seed, slices=3, rings=3, width=width, height=height)
+    walls = base["pos"][:12]
+    wall_mat = base["mat_idx"][:12]
+    c1, c2, r = np.array([-0.45, -0.55, -0.1]), np.array([0.45, -0.55, 0.25]), 0.42
+    s1 = uv_sphere(tuple(c1), r, slices, rings).astype(np.float32).astype(np.float64)
+    s2 = uv_sphere(tuple(c2), r, slices, rings).astype(np.float32).astype(np.float64)
+    pos = np.concatenate([walls, s1, s2])
+
+    def sphere_normals(tris, c):
+        v = tris.reshape(-1, 3, 3) - c
+        return (v / np.linalg.norm(v, axis=2, keepdims=True)).reshape(-1, 9)
+
+    wn = np.empty((12, 9))
+    for k in range(12):  # geometric normal of each wall / light triangle at its three vertices
+        t = walls[k].reshape(3, 3)
+        g = np.cross(t[1] - t[0], t[2] - t[0])
+        wn[k] = np.tile(g / np.linalg.norm(g), 3)
+    normals = np.concatenate([wn, sphere_normals(s1, c1), sphere_normals(s2, c2)])
+    mat_idx = np.concatenate([wall_mat, np.full(len(s1), 4, np.int32), np.full(len(s2), 5, np.int32)])
+    materials = np.zeros((6, 8), np.float32)
+    materials[:4, :4] = base["materials"]
+    materials[4, :5] = [MAT_REFLECTIVE, 0.9, 0.9, 0.9, 0.05]
+    materials[5, :5] = [MAT_REFRACTIVE, 0.95, 0.95, 0.95, 1.5]
+    return {"pos": pos, "normals": normals, "mat_idx": mat_idx, "materials": materials, "camera": base["camera"],
+            "width": width, "height": height}
 
 
 def adjoint_scene(ntri: int = 200_000, seed: int = 7, width: int = 512, height: int = 512):
