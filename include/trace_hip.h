@@ -70,6 +70,11 @@ int trace_ctx_synchronize(trace_ctx* ctx);
 /* ---- geometry ----------------------------------------------------------- */
 /* Device-resident triangle soup in caller order.  Replaces the marshalling of
  * Scene.collectTracables (scene.go:100-114) + NewTriangle (primitive.go:102-114). */
+/* A primitive slot whose fifth value is NaN is an analytic Sphere (NewSphere,
+ * primitive.go:25-86): centre (3 values), radius (> 0), NaN, then 4 ignored values.
+ * Its box is centre -/+ radius, its hit the reference's quadratic (bounds exclusive),
+ * its shading normal (point - centre) / radius; scene compile moves its centre only
+ * (Sphere.transformed, primitive.go:42-50), and so does trace_mesh_animate. */
 int trace_mesh_create(trace_ctx* ctx, const double* h_pos, int64_t ntri, trace_mesh** out);
 /* Same from device memory (9 doubles per triangle), asynchronous. */
 int trace_mesh_create_device(trace_ctx* ctx, const double* d_pos, int64_t ntri, trace_mesh** out);

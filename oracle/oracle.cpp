@@ -54,9 +54,21 @@ struct Box {
   double lo[3], hi[3];
 };
 
-// primitive.go:102-114 (box of a triangle)
+// A primitive slot is a triangle (9 coordinates) or a Sphere: centre, radius,
+// then NaN in the fifth value.
+inline bool is_sphere(const double* p) { return p[4] != p[4]; }
+
+// primitive.go:102-114 (box of a triangle); NewSphere's centre -/+ radius
+// (primitive.go:31-40) for a sphere
 inline Box tri_box(const double* p) {
   Box b;
+  if (is_sphere(p)) {
+    for (int a = 0; a < 3; ++a) {
+      b.lo[a] = p[a] - p[3];
+      b.hi[a] = p[a] + p[3];
+    }
+    return b;
+  }
   for (int a = 0; a < 3; ++a) {
     b.lo[a] = min3(p[a], p[3 + a], p[6 + a]);
     b.hi[a] = max3(p[a], p[3 + a], p[6 + a]);
@@ -237,13 +249,50 @@ bool box_hit(const Box& b, const Ray& r, double tmin, double tmax) {
 
 struct HitRec {
   double t;
-  int32_t prim;  // triangle index
+  int32_t prim;  // primitive index
   double u, v;
   bool front;
+  bool sphere = false;  // a Sphere hit: n holds its facing normal
+  V3 n{0, 0, 0};
 };
+
+// Sphere.intersected (primitive.go:56-86): the nearer root strictly inside
+// (tMin, tMax), else the farther; the normal (point - centre) * (1 / radius),
+// flipped to face the ray.
+bool sphere_hit(const double* p, const Ray& r, double tmin, double tmax, HitRec& h, int32_t prim) {
+  const V3 c{p[0], p[1], p[2]};
+  const double rad = p[3];
+  const V3 oc = sub(r.o, c);
+  const double dn = std::sqrt(dot(r.d, r.d));
+  const double a = dn * dn;
+  const double hb = dot(oc, r.d);
+  const double on = std::sqrt(dot(oc, oc));
+  const double cc = on * on - rad * rad;
+  const double disc = hb * hb - a * cc;
+  if (disc < 0) return false;
+  const double sq = std::sqrt(disc);
+  double t = (-hb - sq) / a;
+  if (t <= tmin || t >= tmax) {
+    t = (-hb + sq) / a;
+    if (t <= tmin || t >= tmax) return false;
+  }
+  const V3 pt{r.o.x + r.d.x * t, r.o.y + r.d.y * t, r.o.z + r.d.z * t};
+  const double inv = 1 / rad;
+  V3 n{(pt.x - c.x) * inv, (pt.y - c.y) * inv, (pt.z - c.z) * inv};
+  const bool front = dot(r.d, n) < 0;
+  if (!front) n = {-n.x, -n.y, -n.z};
+  h.t = t;
+  h.prim = prim;
+  h.u = h.v = 0;
+  h.front = front;
+  h.sphere = true;
+  h.n = n;
+  return true;
+}
 
 // primitive.go:172-208 Moeller-Trumbore, util.go:10-12 ApproxZero
 bool tri_hit(const double* p, const Ray& r, double tmin, double tmax, HitRec& h, int32_t prim) {
+  if (is_sphere(p)) return sphere_hit(p, r, tmin, tmax, h, prim);
   const V3 v0{p[0], p[1], p[2]};
   const V3 e1 = sub(V3{p[3], p[4], p[5]}, v0);
   const V3 e2 = sub(V3{p[6], p[7], p[8]}, v0);
@@ -264,6 +313,7 @@ bool tri_hit(const double* p, const Ray& r, double tmin, double tmax, HitRec& h,
   h.u = u;
   h.v = v;
   h.front = det > 0;
+  h.sphere = false;
   return true;
 }
 
@@ -1035,6 +1085,16 @@ void collect(const SceneIn& s, int32_t node, const M4& parent_t, bool is_root, s
     for (int k = 0; k < 16; ++k) ident = ident && own.m[k] == I.m[k];
     const M4 tinv = m4_inverse(m4_transpose(t));
     for (int64_t i = s.off[mesh]; i < s.off[mesh + 1]; ++i) {
+      if (is_sphere(s.pos + 9 * i)) {
+        // Sphere.transformed (primitive.go:42-50): the centre as a point, radius kept
+        double q[9];
+        std::memcpy(q, s.pos + 9 * i, sizeof(q));
+        if (!ident) m4_point(t, s.pos + 9 * i, q);
+        pos.insert(pos.end(), q, q + 9);
+        if (s.nrm) nrm.insert(nrm.end(), 9, 0.0);
+        mat.push_back(s.mesh_mat[mesh]);
+        continue;
+      }
       for (int v = 0; v < 3; ++v) {
         double p[3], n[3];
         const double* sp = s.pos + 9 * i + 3 * v;
@@ -1213,6 +1273,7 @@ bool trace_ray(const Tree& t, V3 o, V3 d, double tmin, HitRec& h) {
 // The hit's interpolated vertex normal, flipped to face the ray
 // (primitive.go:145-150, 200-205); not renormalised.
 V3 facing_normal(const Shading& s, const HitRec& h) {
+  if (h.sphere) return h.n;
   const double* n = s.nrm + 9 * size_t(h.prim);
   const double u = h.u, v = h.v, w0 = 1 - u - v;
   V3 nn{(n[3] * u + n[6] * v) + n[0] * w0, (n[4] * u + n[7] * v) + n[1] * w0,
@@ -1333,6 +1394,7 @@ static std::vector<double> geo_normals(const oracle_tree* t) {
   std::vector<double> geo(9 * size_t(t->n));
   for (int64_t i = 0; i < t->n; ++i) {
     const double* p = &t->pos[9 * size_t(i)];
+    if (is_sphere(p)) continue;  // a sphere's normal comes from its centre (zeros here)
     for (int c = 0; c < 3; ++c) {
       const V3 a{p[3 * c], p[3 * c + 1], p[3 * c + 2]};
       const int ri = (c + 1) % 3, li = (c + 2) % 3;

@@ -51,6 +51,35 @@ int ctx_check_device_error(trace_ctx* ctx, const char* what) {
   return TRACE_OK;
 }
 
+// Bit 0: some primitive is a Sphere; bit 1: a Sphere with a radius that is not
+// positive and finite, or a centre that is not finite.
+__global__ void scan_prims_kernel(const double* __restrict__ pos, int64_t n, uint32_t* __restrict__ flags) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  bool sph = false, bad = false;
+  if (i < n) {
+    const double* p = pos + 9 * i;
+    sph = prim_is_sphere(p);
+    bad = sph && !(p[3] > 0.0 && p[3] < INFINITY && fabs(p[0]) < INFINITY && fabs(p[1]) < INFINITY &&
+                   fabs(p[2]) < INFINITY);
+  }
+  const uint64_t bs = ballot64(sph), bb = ballot64(bad);
+  if (__lane_id() == 0 && (bs || bb)) atomicOr(flags, (bs ? 1u : 0u) | (bb ? 2u : 0u));
+}
+
+// Sets mesh->has_spheres from the device positions; rejects malformed spheres.
+int mesh_scan_prims(trace_ctx* ctx, trace_mesh* mesh, const char* what) {
+  DevArray<uint32_t> flag;
+  HIP_TRY(ctx, flag.alloc(1));
+  fill_async<uint32_t>(ctx->stream, flag.p, 1, 0u);
+  scan_prims_kernel<<<int(ceil_div(mesh->ntri, 256)), 256, 0, ctx->stream>>>(mesh->pos.p, mesh->ntri, flag.p);
+  uint32_t h = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&h, flag.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (h & 2u) TRACE_FAIL(ctx, TRACE_E_INVALID, "%s: a sphere needs a finite centre and a positive finite radius", what);
+  mesh->has_spheres = (h & 1u) != 0;
+  return TRACE_OK;
+}
+
 extern "C" {
 
 int trace_version(void) { return 1; }
@@ -127,6 +156,11 @@ static int mesh_create_impl(trace_ctx* ctx, const double* pos, int64_t ntri, tra
     delete m;
     TRACE_FAIL(ctx, e == hipErrorOutOfMemory ? TRACE_E_OOM : TRACE_E_HIP, "trace_mesh_create: %s", hipGetErrorString(e));
   }
+  const int rc = mesh_scan_prims(ctx, m, "trace_mesh_create");
+  if (rc) {
+    delete m;
+    return rc;
+  }
   *out = m;
   return TRACE_OK;
 }
@@ -192,6 +226,8 @@ int trace_mesh_update(trace_ctx* ctx, trace_mesh* mesh, const double* h_pos) {
   if (!mesh || !h_pos) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_update: null argument");
   HIP_TRY(ctx, hipMemcpyAsync(mesh->pos.p, h_pos, 9 * mesh->ntri * sizeof(double),
                               hipMemcpyHostToDevice, ctx->stream));
+  const int src = mesh_scan_prims(ctx, mesh, "trace_mesh_update");
+  if (src) return src;
   if (mesh->has_shading && mesh->geo_normals) {
     const int rc = render::refresh_geo_normals(ctx, mesh);
     if (rc) return rc;

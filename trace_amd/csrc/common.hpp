@@ -100,6 +100,8 @@ struct ScratchPlan {
 };
 int ctx_reserve_scratch(trace_ctx* ctx, size_t bytes);
 int ctx_check_device_error(trace_ctx* ctx, const char* what);
+// Sets mesh->has_spheres from its device positions (synchronous); fails on a malformed sphere.
+int mesh_scan_prims(trace_ctx* ctx, trace_mesh* mesh, const char* what);
 
 // ---------------------------------------------------------------------------
 // device-resident data
@@ -115,6 +117,7 @@ struct trace_mesh {
   int32_t nmat = 0;
   bool has_specular = false;  // a Reflective or Refractive material is in use
   bool has_shading = false;
+  bool has_spheres = false;  // some primitive is a Sphere (prim_is_sphere)
   bool geo_normals = false;  // nrm derived from pos (recomputed when pos animates)
   // bound per-triangle parameters (caller-owned device memory, 3 floats per
   // triangle): Diffuse albedo and Light emission replacing the material colour
@@ -182,6 +185,7 @@ static_assert(sizeof(WideNode) == 64, "wide node record must be 64 bytes");
 // keeps its binary form only.
 struct trace_bvh {
   int64_t n_prims = 0;
+  bool has_spheres = false;  // built from a mesh holding Sphere primitives
   int32_t key_mode = 0;  // -1 for PHR
   int32_t arity = 2;
   DevArray<uint64_t> keys;     // sorted Morton keys (LBVH only)
@@ -221,6 +225,7 @@ struct trace_bvh {
   const int32_t* perm_p() const { return base ? base->perm.p : perm.p; }
   const BvhHeader* header_p() const { return base ? base->seg_headers.p + seg : header.p; }
   int32_t depth_bound() const { return base ? base->host_depth : host_depth; }
+  bool spheres() const { return base ? base->has_spheres : has_spheres; }
 };
 
 // ---------------------------------------------------------------------------
@@ -266,6 +271,29 @@ inline void fill_async(hipStream_t st, T* p, size_t n, T v) {
   if (n == 0) return;
   const int64_t blocks = std::min<int64_t>(ceil_div(int64_t(n), 256), 2048);
   fill_kernel<T><<<int(blocks), 256, 0, st>>>(p, n, v);
+}
+
+// Primitives are 9 values per slot (FP64 in a mesh, FP32 in a tree's
+// triangle array): a triangle's three vertices, or a Sphere (primitive.go:
+// 25-86) as centre (3), radius, then NaN in the fifth value -- a tag no
+// triangle carries.
+template <typename T>
+__host__ __device__ inline bool prim_is_sphere(const T* p) {
+  return p[4] != p[4];
+}
+
+// Per-axis bounds of a primitive: the reference's Min3/Max3 of the vertices
+// (util.go:47-66; the result is one of the inputs) or centre -/+ radius
+// (NewSphere, primitive.go:31-40).
+__host__ __device__ inline void prim_bounds(const double* p, int a, double& lo, double& hi) {
+  if (prim_is_sphere(p)) {
+    lo = p[a] - p[3];
+    hi = p[a] + p[3];
+    return;
+  }
+  const double x = p[a], y = p[3 + a], z = p[6 + a];
+  lo = (x <= y && x <= z) ? x : ((y <= x && y <= z) ? y : z);
+  hi = (x >= y && x >= z) ? x : ((y >= x && y >= z) ? y : z);
 }
 
 // Path segments traced by the context's last render or backward call: a

@@ -29,6 +29,12 @@ __global__ void animate_kernel(const double* __restrict__ rest, double* __restri
   if (v >= nvert) return;
   const double f = double(*d_frame);
   const double* p = rest + 3 * v;
+  if (v % 3 != 0 && prim_is_sphere(rest + 9 * (v / 3))) {  // a sphere moves its centre only
+    pos[3 * v + 0] = p[0];
+    pos[3 * v + 1] = p[1];
+    pos[3 * v + 2] = p[2];
+    return;
+  }
   const double w = amplitude * sin(6.283185307179586 * f / period + (kx * p[0] + ky * p[1] + kz * p[2]));
   pos[3 * v + 0] = p[0] + w * ux;
   pos[3 * v + 1] = p[1] + w * uy;

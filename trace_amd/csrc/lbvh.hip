@@ -33,19 +33,6 @@ __device__ inline double double_of(uint64_t o) {
   return __longlong_as_double(b);
 }
 
-// min/max of three with the reference's comparisons (util.go:47-66); the
-// result is one of the inputs, so the value is order independent.
-__device__ inline double min3(double a, double b, double c) {
-  if (a <= b && a <= c) return a;
-  if (b <= a && b <= c) return b;
-  return c;
-}
-__device__ inline double max3(double a, double b, double c) {
-  if (a >= b && a >= c) return a;
-  if (b >= a && b >= c) return b;
-  return c;
-}
-
 constexpr int kEncBlocks = 512;
 
 // Stage 1 of the scene box: each workgroup reduces a grid-stride share of the
@@ -61,8 +48,10 @@ __global__ __launch_bounds__(256) void enclosing_kernel(const double* __restrict
     const double* p = pos + 9 * i;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const uint64_t mn = ord_of(min3(p[a], p[3 + a], p[6 + a]));
-      const uint64_t mx = ord_of(max3(p[a], p[3 + a], p[6 + a]));
+      double l, h;
+      prim_bounds(p, a, l, h);
+      const uint64_t mn = ord_of(l);
+      const uint64_t mx = ord_of(h);
       lo[a] = mn < lo[a] ? mn : lo[a];
       hi[a] = mx > hi[a] ? mx : hi[a];
     }
@@ -166,8 +155,10 @@ __global__ __launch_bounds__(256) void seg_enclosing_kernel(const double* __rest
   for (int64_t i = t0 + threadIdx.x; i < t1; i += blockDim.x) {
     const double* p = pos + 9 * i;
     for (int a = 0; a < 3; ++a) {
-      const uint64_t mn = ord_of(min3(p[a], p[3 + a], p[6 + a]));
-      const uint64_t mx = ord_of(max3(p[a], p[3 + a], p[6 + a]));
+      double l, h;
+      prim_bounds(p, a, l, h);
+      const uint64_t mn = ord_of(l);
+      const uint64_t mx = ord_of(h);
       if (uniform) {
         lo[a] = mn < lo[a] ? mn : lo[a];
         hi[a] = mx > hi[a] ? mx : hi[a];
@@ -264,8 +255,8 @@ __global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ 
   uint64_t q[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double lo = min3(p[a], p[3 + a], p[6 + a]);
-    const double hi = max3(p[a], p[3 + a], p[6 + a]);
+    double lo, hi;
+    prim_bounds(p, a, lo, hi);
     const double c = (lo + hi) * 0.5;  // aabb.update, bounding.go:36-43
     q[a] = quantise(qbox[a], qbox[3 + a] - qbox[a], c);
   }
@@ -484,8 +475,10 @@ __global__ __launch_bounds__(256) void reorder_kernel(const double* __restrict__
     t[a] = float(v[a]);
     t[3 + a] = float(v[3 + a]);
     t[6 + a] = float(v[6 + a]);
-    b[a] = f32_down(min3(v[a], v[3 + a], v[6 + a]));
-    b[3 + a] = f32_up(max3(v[a], v[3 + a], v[6 + a]));
+    double lo, hi;
+    prim_bounds(v, a, lo, hi);
+    b[a] = f32_down(lo);
+    b[3 + a] = f32_up(hi);
   }
 }
 
@@ -670,6 +663,7 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "LBVH: %d segments exceed the 2^24 the sort's segment digits cover", nseg);
 
   bvh->n_prims = n;
+  bvh->has_spheres = mesh->has_spheres;
   bvh->key_mode = key_mode;
   bvh->arity = 2;
   bvh->nseg = nseg;

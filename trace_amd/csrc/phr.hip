@@ -772,6 +772,7 @@ static int read_ctl(trace_ctx* ctx, const Ctl* d, Ctl* h) {
 
 static int clone_lbvh(trace_ctx* ctx, const trace_bvh* aux, trace_bvh* out) {
   const int64_t n = aux->n_prims;
+  out->has_spheres = aux->has_spheres;
   hipStream_t st = ctx->stream;
   BvhHeader h;
   HIP_TRY(ctx, hipMemcpyAsync(&h, aux->header.p, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -808,6 +809,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   hipStream_t st = ctx->stream;
   out->n_prims = n;
   out->key_mode = -1;
+  out->has_spheres = aux->has_spheres;
   out->arity = branching;
   out->phr_alpha = alpha;
   out->phr_delta = delta;

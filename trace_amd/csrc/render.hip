@@ -33,6 +33,10 @@ __global__ void geo_normals_kernel(const double* __restrict__ pos, int64_t n, fl
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   const double* p = pos + 9 * i;
+  if (prim_is_sphere(p)) {  // a sphere's shading normal comes from its centre
+    for (int c = 0; c < 9; ++c) nrm[9 * i + c] = 0.0f;
+    return;
+  }
   // calcNormal(point, right, left) = unit((right - point) x (left - point)),
   // primitive.go:116-138, for each vertex in turn.
   for (int c = 0; c < 3; ++c) {
@@ -305,13 +309,28 @@ __device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p
         em_w = T;
       }
     } else {
-      // interpolated vertex normal, flipped to face the ray (primitive.go:145-150, 200-205)
-      const float* n = nrm + 9 * size_t(prim);
-      const float w0 = 1.0f - h.u - h.v;
-      float nx = (n[3] * h.u + n[6] * h.v) + n[0] * w0;
-      float ny = (n[4] * h.u + n[7] * h.v) + n[1] * w0;
-      float nz = (n[5] * h.u + n[8] * h.v) + n[2] * w0;
-      if (!h.front) {
+      const float* tv = tris + 9 * size_t(h.pos);
+      const bool sph = prim_is_sphere(tv);
+      // hit point = ray.position(t) (camera.go:108-111), evaluated in FP64
+      const float hx = float(double(o4.x) + double(d4.x) * h.t64);
+      const float hy = float(double(o4.y) + double(d4.y) * h.t64);
+      const float hz = float(double(o4.z) + double(d4.z) * h.t64);
+      float nx, ny, nz;
+      if (sph) {
+        // Sphere: (point - centre) * (1 / radius) (primitive.go:80-84)
+        const float inv = 1.0f / tv[3];
+        nx = (hx - tv[0]) * inv;
+        ny = (hy - tv[1]) * inv;
+        nz = (hz - tv[2]) * inv;
+      } else {
+        // interpolated vertex normal (primitive.go:145-150, 200-205)
+        const float* n = nrm + 9 * size_t(prim);
+        const float w0 = 1.0f - h.u - h.v;
+        nx = (n[3] * h.u + n[6] * h.v) + n[0] * w0;
+        ny = (n[4] * h.u + n[7] * h.v) + n[1] * w0;
+        nz = (n[5] * h.u + n[8] * h.v) + n[2] * w0;
+      }
+      if (!h.front) {  // flipped to face the ray
         nx = -nx;
         ny = -ny;
         nz = -nz;
@@ -376,21 +395,26 @@ __device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p
           }
         }
       }
-      // new origin = ray.position(t) (camera.go:108-111), evaluated in FP64,
-      // then kept on the side of the triangle's plane the new ray leaves to
-      const float* tv = tris + 9 * size_t(h.pos);
-      const float e1x = tv[3] - tv[0], e1y = tv[4] - tv[1], e1z = tv[5] - tv[2];
-      const float e2x = tv[6] - tv[0], e2y = tv[7] - tv[1], e2z = tv[8] - tv[2];
-      float gx = e1y * e2z - e1z * e2y, gy = e1z * e2x - e1x * e2z, gz = e1x * e2y - e1y * e2x;
+      // the new origin is the hit point, kept on the side of the surface
+      // (triangle plane or sphere tangent plane) the new ray leaves to
+      float gx, gy, gz;
+      if (sph) {
+        gx = hx - tv[0];
+        gy = hy - tv[1];
+        gz = hz - tv[2];
+      } else {
+        const float e1x = tv[3] - tv[0], e1y = tv[4] - tv[1], e1z = tv[5] - tv[2];
+        const float e2x = tv[6] - tv[0], e2y = tv[7] - tv[1], e2z = tv[8] - tv[2];
+        gx = e1y * e2z - e1z * e2y;
+        gy = e1z * e2x - e1x * e2z;
+        gz = e1x * e2y - e1y * e2x;
+      }
       const float gl = sqrtf(gx * gx + gy * gy + gz * gz);
       const float gs = (gx * sx + gy * sy + gz * sz) >= 0.0f ? 1.0f / gl : -1.0f / gl;
       gx *= gs;
       gy *= gs;
       gz *= gs;
-      const float px = float(double(o4.x) + double(d4.x) * h.t64);
-      const float py = float(double(o4.y) + double(d4.y) * h.t64);
-      const float pz = float(double(o4.z) + double(d4.z) * h.t64);
-      ro[p] = make_float4(offset_coord(px, gx), offset_coord(py, gy), offset_coord(pz, gz), 0.0f);
+      ro[p] = make_float4(offset_coord(hx, gx), offset_coord(hy, gy), offset_coord(hz, gz), 0.0f);
       rd[p] = make_float4(sx, sy, sz, 0.0f);
       if (adj.mode == kModeSummary) {
         // product of the non-zero albedos, and a count of the zero ones
@@ -538,7 +562,7 @@ int refresh_geo_normals(trace_ctx* ctx, trace_mesh* mesh) {
   return TRACE_OK;
 }
 
-template <int THREADS, int STACK>
+template <int THREADS, int STACK, bool SPH>
 __global__ __launch_bounds__(THREADS) void bounce_kernel(
     const WideNode* __restrict__ wnodes, const SegArgs g,
     const int32_t* __restrict__ q_in, int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
@@ -559,7 +583,7 @@ __global__ __launch_bounds__(THREADS) void bounce_kernel(
     traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
     int32_t nn = 0, nt = 0;
     bool overflow = false, bad = false;
-    traverse::Hit h = traverse::closest_hit_wide<STACK, false>(
+    traverse::Hit h = traverse::closest_hit_wide<STACK, false, SPH>(
         wnodes, g.tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, THREADS);
     if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
     if (bad) atomicOr(d_err, TRACE_ERR_TREE);
@@ -584,7 +608,7 @@ __device__ inline int4 pack_hit(const traverse::Hit& h) {
   return make_int4(h.pos, __float_as_int(h.front ? h.t : -h.t), __float_as_int(h.u), __float_as_int(h.v));
 }
 
-template <int THREADS, int STACK>
+template <int THREADS, int STACK, bool SPH>
 __global__ __launch_bounds__(THREADS) void trace_persistent(
     const WideNode* __restrict__ wnodes, const SegArgs g, const int32_t* __restrict__ q_in,
     const uint32_t* __restrict__ count_in, int4* __restrict__ hits, uint32_t* __restrict__ d_err,
@@ -636,7 +660,7 @@ __global__ __launch_bounds__(THREADS) void trace_persistent(
           const float4 o4 = g.ro[p];
           const float4 d4 = g.rd[p];
           const traverse::Ray ray{o4.x, o4.y, o4.z, tmin, d4.x, d4.y, d4.z, INFINITY};
-          active = traverse::wide_begin<false>(g.tris, hdr, ray, s, nt, bad);
+          active = traverse::wide_begin<false, SPH>(g.tris, hdr, ray, s, nt, bad);
           pending = !active;
         }
         pool_next += __popcll(need) < avail ? int64_t(__popcll(need)) : avail;
@@ -647,7 +671,7 @@ __global__ __launch_bounds__(THREADS) void trace_persistent(
       continue;
     }
     if (active) {
-      active = traverse::wide_step<STACK, false>(wnodes, g.tris, n_bound, n_prims, s, stack, nn, nt,
+      active = traverse::wide_step<STACK, false, SPH>(wnodes, g.tris, n_bound, n_prims, s, stack, nn, nt,
                                                  overflow, bad);
       pending = !active;
     }
@@ -688,6 +712,33 @@ __global__ __launch_bounds__(THREADS) void shade_kernel(const SegArgs g, const i
   segment_epilogue(adj, g.pps, p, push, al_tri, al_w, em_tri, em_w, q_out, count_out);
 }
 
+template <int THREADS, bool SPH>
+static void launch_segments(trace_ctx* ctx, const trace_bvh* bvh, int grid, const SegArgs& g, int32_t* qin,
+                            int32_t* qout, uint32_t* cin, uint32_t* cout, int4* hits, const Adj& adj) {
+  hipStream_t st = ctx->stream;
+  const bool deep = bvh->depth_bound() > 63;
+  if (ctx->tune.render_persistent) {
+    // resident grid (up to 32 waves per CU); queue slots are claimed from a counter
+    const int pgrid = int(std::min<int64_t>(grid, int64_t(ctx->n_cu) * 32 * 64 / THREADS));
+    uint32_t* claim = ctx->d_work + 1;
+    fill_async<uint32_t>(st, claim, 1, 0u);
+    if (deep)
+      trace_persistent<THREADS, 512, SPH><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, cin, hits,
+                                                                     ctx->d_err, claim);
+    else
+      trace_persistent<THREADS, 128, SPH><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, cin, hits,
+                                                                     ctx->d_err, claim);
+    shade_kernel<THREADS><<<grid, THREADS, 0, st>>>(g, qin, qout, cin, cout, hits, adj);
+    return;
+  }
+  if (deep)
+    bounce_kernel<THREADS, 512, SPH><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin, cout,
+                                                               ctx->d_err, adj);
+  else
+    bounce_kernel<THREADS, 128, SPH><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin, cout,
+                                                               ctx->d_err, adj);
+}
+
 template <int THREADS>
 static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, int grid,
                           const Scene* scenes, int64_t pps, int32_t k, const trace_render_params* prm,
@@ -713,27 +764,10 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
   g.thr = thr;
   g.rad = rad;
   g.seg_total = segment_counter(ctx);
-  const bool deep = bvh->depth_bound() > 63;
-  if (ctx->tune.render_persistent) {
-    // resident grid (up to 32 waves per CU); queue slots are claimed from a counter
-    const int pgrid = int(std::min<int64_t>(grid, int64_t(ctx->n_cu) * 32 * 64 / THREADS));
-    uint32_t* claim = ctx->d_work + 1;
-    fill_async<uint32_t>(st, claim, 1, 0u);
-    if (deep)
-      trace_persistent<THREADS, 512><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, cin, hits, ctx->d_err,
-                                                                claim);
-    else
-      trace_persistent<THREADS, 128><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, cin, hits, ctx->d_err,
-                                                                claim);
-    shade_kernel<THREADS><<<grid, THREADS, 0, st>>>(g, qin, qout, cin, cout, hits, adj);
-    return;
-  }
-  if (deep)
-    bounce_kernel<THREADS, 512><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin,
-                                                          cout, ctx->d_err, adj);
+  if (bvh->spheres())
+    launch_segments<THREADS, true>(ctx, bvh, grid, g, qin, qout, cin, cout, hits, adj);
   else
-    bounce_kernel<THREADS, 128><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin,
-                                                          cout, ctx->d_err, adj);
+    launch_segments<THREADS, false>(ctx, bvh, grid, g, qin, qout, cin, cout, hits, adj);
 }
 
 namespace {

@@ -135,6 +135,16 @@ __global__ __launch_bounds__(256) void flatten_kernel(const Inst* __restrict__ i
   const int64_t s = in.src + (i - in.dst);
   const double* p = src_pos + 9 * s;
   double* q = pos + 9 * i;
+  if (prim_is_sphere(p)) {
+    // Sphere.transformed (primitive.go:42-50): the centre as a point, the
+    // radius unchanged
+    for (int a = 0; a < 9; ++a) q[a] = p[a];
+    if (!in.raw) xform(in.t, p, q);
+    if (src_nrm)
+      for (int a = 0; a < 9; ++a) nrm[9 * i + a] = 0.0f;
+    mat_idx[i] = in.mat;
+    return;
+  }
   for (int v = 0; v < 3; ++v) {
     if (in.raw) {
       for (int a = 0; a < 3; ++a) q[3 * v + a] = p[3 * v + a];

@@ -87,6 +87,28 @@ __device__ inline void tri_hit(const float* __restrict__ p, const Ray64& r, doub
   t = tt;
 }
 
+// Sphere.intersected (primitive.go:56-86) in FP64: t must lie strictly
+// inside (tmin, t).
+__device__ inline void sphere_hit(const float* __restrict__ p, const Ray64& r, double tmin, double& t) {
+#pragma clang fp contract(off)
+  const double cx = p[0], cy = p[1], cz = p[2], rad = p[3];
+  const double ox = r.o[0] - cx, oy = r.o[1] - cy, oz = r.o[2] - cz;
+  const double dn = sqrt(r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2]);
+  const double a = dn * dn;
+  const double hb = ox * r.d[0] + oy * r.d[1] + oz * r.d[2];
+  const double on = sqrt(ox * ox + oy * oy + oz * oz);
+  const double c = on * on - rad * rad;
+  const double disc = hb * hb - a * c;
+  if (disc < 0) return;
+  const double sq = sqrt(disc);
+  double tt = (-hb - sq) / a;
+  if (tt <= tmin || tt >= t) {
+    tt = (-hb + sq) / a;
+    if (tt <= tmin || tt >= t) return;
+  }
+  t = tt;
+}
+
 // Stack entries: -1 = the root (header), else (binary node << 1) | child slot.
 // The reference's stack holds at most one pending sibling per level plus the
 // two children just pushed: depth + 2 entries.
@@ -126,7 +148,11 @@ __device__ int32_t walk(const BvhNode* __restrict__ nodes, int64_t n_nodes, cons
         bad = true;
         continue;
       }
-      for (int32_t k = ref; k < ref + cnt; ++k) tri_hit(tris + 9 * size_t(k), r, tmin, t);
+      for (int32_t k = ref; k < ref + cnt; ++k) {
+        const float* p = tris + 9 * size_t(k);
+        if (prim_is_sphere(p)) sphere_hit(p, r, tmin, t);
+        else tri_hit(p, r, tmin, t);
+      }
     } else {
       if (ref < 0 || ref >= n_nodes || ref >= (1 << 30)) {
         bad = true;

@@ -9,7 +9,7 @@ namespace traverse {
 // covers the wide tree of any binary tree of depth <= 63 (at most 32 wide
 // levels, 3 deferred children each, plus the split-leaf levels the widening
 // pass sizes to fit); deeper PHR trees use 512.
-template <int THREADS, int STACK, bool COUNT>
+template <int THREADS, int STACK, bool COUNT, bool SPH>
 __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
     const WideNode* __restrict__ wnodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
@@ -25,7 +25,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
   Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   int32_t nn = 0, nt = 0;
   bool overflow = false, bad = false;
-  Hit h = closest_hit_wide<STACK, COUNT>(wnodes, tris, hdr, ray, nn, nt, overflow, bad,
+  Hit h = closest_hit_wide<STACK, COUNT, SPH>(wnodes, tris, hdr, ray, nn, nt, overflow, bad,
                                          s_stack + threadIdx.x, THREADS);
   refine_hit_fp64(tris, ray, h);
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
 constexpr uint32_t kChunk = 256;
 constexpr int kRefillAt = 16;
 
-template <int THREADS, int STACK, bool COUNT>
+template <int THREADS, int STACK, bool COUNT, bool SPH>
 __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
     const WideNode* __restrict__ wnodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
           const float4 rb = rays[2 * rid + 1];
           ray = Ray{ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
           nn = nt = 0;
-          active = wide_begin<COUNT>(tris, hdr, ray, s, nt, bad);
+          active = wide_begin<COUNT, SPH>(tris, hdr, ray, s, nt, bad);
           if (!active) finish();
         }
         pool_next += std::min<int64_t>(__popcll(need), avail);
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
       continue;
     }
     if (active) {
-      active = wide_step<STACK, COUNT>(wnodes, tris, n_bound, hdr.n_prims, s, stack, nn, nt,
+      active = wide_step<STACK, COUNT, SPH>(wnodes, tris, n_bound, hdr.n_prims, s, stack, nn, nt,
                                        overflow, bad);
       pending = !active;
     }
@@ -129,23 +129,34 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);
 }
 
-template <int THREADS, int STACK, bool COUNT>
-static void launch(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, int64_t n,
-                   int32_t id_offset, int32_t* d_hit, float* d_t, int32_t* d_nodes, int32_t* d_tris) {
+template <int THREADS, int STACK, bool COUNT, bool SPH>
+static void launch_kind(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, int64_t n,
+                        int32_t id_offset, int32_t* d_hit, float* d_t, int32_t* d_nodes, int32_t* d_tris) {
   if (ctx->tune.trav_persistent) {
     // up to 32 resident waves per CU; workgroups beyond what fits start as
     // others leave and find the counter exhausted
     const int64_t blocks = std::min<int64_t>(ceil_div(n, THREADS), int64_t(ctx->n_cu) * 32 * 64 / THREADS);
     uint32_t* work = ctx->d_work;
     fill_async<uint32_t>(ctx->stream, work, 1, 0u);
-    closest_hit_persistent<THREADS, STACK, COUNT><<<int(blocks), THREADS, 0, ctx->stream>>>(
+    closest_hit_persistent<THREADS, STACK, COUNT, SPH><<<int(blocks), THREADS, 0, ctx->stream>>>(
         bvh->wnodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n,
         id_offset, d_hit, d_t, d_nodes, d_tris, ctx->d_err, work);
     return;
   }
-  closest_hit_kernel<THREADS, STACK, COUNT><<<int(ceil_div(n, THREADS)), THREADS, 0, ctx->stream>>>(
+  closest_hit_kernel<THREADS, STACK, COUNT, SPH><<<int(ceil_div(n, THREADS)), THREADS, 0, ctx->stream>>>(
       bvh->wnodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n, id_offset, d_hit, d_t,
       d_nodes, d_tris, ctx->d_err);
+}
+
+// Trees holding Spheres take the variant whose leaf loop checks each
+// primitive's tag; triangle-only trees keep the plain loop.
+template <int THREADS, int STACK, bool COUNT>
+static void launch(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, int64_t n,
+                   int32_t id_offset, int32_t* d_hit, float* d_t, int32_t* d_nodes, int32_t* d_tris) {
+  if (bvh->spheres())
+    launch_kind<THREADS, STACK, COUNT, true>(ctx, bvh, rays, n, id_offset, d_hit, d_t, d_nodes, d_tris);
+  else
+    launch_kind<THREADS, STACK, COUNT, false>(ctx, bvh, rays, n, id_offset, d_hit, d_t, d_nodes, d_tris);
 }
 
 template <int THREADS>

@@ -94,6 +94,38 @@ __device__ inline void tri_test(const RayPre& r, const float* __restrict__ tris,
   h.front = det > 0.0f;
 }
 
+// Sphere.intersected (primitive.go:56-86) in FP32 for sorted primitive k:
+// the nearer root if it lies strictly inside (tmin, best t), else the farther
+// one (both bounds exclusive, as the reference's).  The discriminant is taken
+// as r^2 - |oc - (oc.d / d.d) d|^2 (the same quantity divided by d.d, free of
+// the cancellation of the textbook form); the winner is re-solved in FP64.
+__device__ inline void sphere_test(const RayPre& r, const float* __restrict__ t, int32_t k, Hit& h) {
+  const float ox = r.ox - t[0], oy = r.oy - t[1], oz = r.oz - t[2], rad = t[3];
+  const float a = r.dx * r.dx + r.dy * r.dy + r.dz * r.dz;
+  const float s = (ox * r.dx + oy * r.dy + oz * r.dz) / a;
+  const float lx = ox - s * r.dx, ly = oy - s * r.dy, lz = oz - s * r.dz;
+  const float dn = rad * rad - (lx * lx + ly * ly + lz * lz);
+  if (dn < 0.0f) return;
+  const float q = sqrtf(dn / a);
+  float tt = -s - q;
+  if (tt <= r.tmin || tt >= h.t) {
+    tt = -s + q;
+    if (tt <= r.tmin || tt >= h.t) return;
+  }
+  h.t = tt;
+  h.t64 = tt;
+  h.pos = k;
+  h.u = h.v = 0.0f;
+  h.front = true;  // set by refine_hit_fp64
+}
+
+// A primitive of a tree that may hold Spheres (SPH): dispatch on the tag.
+template <bool SPH>
+__device__ inline void prim_test(const RayPre& r, const float* __restrict__ tris, int32_t k, Hit& h) {
+  if (SPH && prim_is_sphere(tris + 9 * size_t(k))) sphere_test(r, tris + 9 * size_t(k), k, h);
+  else tri_test(r, tris, k, h);
+}
+
 // Re-evaluates the accepted triangle in FP64 with the reference's operation
 // order (primitive.go:172-208: p = d x e2, det = e1.p, invDet = 1/det,
 // t = e2.q * invDet), so the reported t and barycentrics carry FP64 accuracy
@@ -102,6 +134,31 @@ __device__ inline void refine_hit_fp64(const float* __restrict__ tris, const Ray
 #pragma clang fp contract(off)
   if (h.pos < 0) return;
   const float* t = tris + 9 * size_t(h.pos);
+  if (prim_is_sphere(t)) {
+    // Sphere.intersected in FP64 (primitive.go:56-86): dirNorm^2, halfB,
+    // |oc|^2 - r^2, the nearer root unless it is not beyond tMin; the normal
+    // (point - centre) * (1 / r) decides the facing
+    const double cx = t[0], cy = t[1], cz = t[2], rad = t[3];
+    const double dx = ray.dx, dy = ray.dy, dz = ray.dz;
+    const double ox = double(ray.ox) - cx, oy = double(ray.oy) - cy, oz = double(ray.oz) - cz;
+    const double dn = sqrt(dx * dx + dy * dy + dz * dz);
+    const double a = dn * dn;
+    const double hb = ox * dx + oy * dy + oz * dz;
+    const double on = sqrt(ox * ox + oy * oy + oz * oz);
+    const double c = on * on - rad * rad;
+    const double disc = hb * hb - a * c;
+    if (disc < 0) return;  // keep the FP32 answer
+    const double sq = sqrt(disc);
+    double tt = (-hb - sq) / a;
+    if (tt <= double(ray.tmin) || tt >= double(ray.tmax)) tt = (-hb + sq) / a;
+    const double px = double(ray.ox) + dx * tt, py = double(ray.oy) + dy * tt, pz = double(ray.oz) + dz * tt;
+    const double inv = 1 / rad;
+    const double nx = (px - cx) * inv, ny = (py - cy) * inv, nz = (pz - cz) * inv;
+    h.t64 = tt;
+    h.t = float(tt);
+    h.front = dx * nx + dy * ny + dz * nz < 0;
+    return;
+  }
   const double v0x = t[0], v0y = t[1], v0z = t[2];
   const double e1x = double(t[3]) - v0x, e1y = double(t[4]) - v0y, e1z = double(t[5]) - v0z;
   const double e2x = double(t[6]) - v0x, e2y = double(t[7]) - v0y, e2z = double(t[8]) - v0z;
@@ -124,7 +181,7 @@ __device__ inline void refine_hit_fp64(const float* __restrict__ tris, const Ray
 
 // Leaf [first, first + cnt) must lie inside the triangle array; a malformed
 // reference is skipped and reported instead of read.
-template <bool COUNT>
+template <bool COUNT, bool SPH>
 __device__ inline void leaf_test(const RayPre& r, const float* __restrict__ tris, int32_t first,
                                  int32_t cnt, int32_t n_prims, Hit& h, int32_t& n_tris,
                                  bool& bad) {
@@ -134,7 +191,7 @@ __device__ inline void leaf_test(const RayPre& r, const float* __restrict__ tris
   }
   for (int32_t k = first; k < first + cnt; ++k) {
     if (COUNT) ++n_tris;
-    tri_test(r, tris, k, h);
+    prim_test<SPH>(r, tris, k, h);
   }
 }
 
@@ -171,7 +228,7 @@ struct WideState {
 };
 
 // Starts a ray: root box test, root leaf.  Returns true if node visits follow.
-template <bool COUNT>
+template <bool COUNT, bool SPH>
 __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeader& hdr,
                                   const Ray& ray, WideState& s, int32_t& n_tris, bool& bad) {
   s.r = ray_pre(ray);
@@ -188,7 +245,7 @@ __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeade
             hdr.root_hi[2], s.h.t, tn))
     return false;
   if (hdr.root_cnt > 0 && hdr.wroot < 0) {  // a root leaf kept whole
-    leaf_test<COUNT>(s.r, tris, hdr.root_ref, hdr.root_cnt, hdr.n_prims, s.h, n_tris, bad);
+    leaf_test<COUNT, SPH>(s.r, tris, hdr.root_ref, hdr.root_cnt, hdr.n_prims, s.h, n_tris, bad);
     return false;
   }
   return true;
@@ -202,7 +259,7 @@ __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeade
 // false when the ray is done.
 // n_bound = the tree's wide-node count, n_prims = its triangle count:
 // bounds for every reference followed.
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool SPH>
 __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
                                  const float* __restrict__ tris, uint32_t n_bound, int32_t n_prims,
                                  WideState& s, TravStack<STACK>& stack, int32_t& n_nodes,
@@ -263,7 +320,7 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
     leaves &= leaves - 1u;
     const int32_t first = k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3];
     const int32_t count = int32_t(k == 0 ? cnt[0] : k == 1 ? cnt[1] : k == 2 ? cnt[2] : cnt[3]);
-    leaf_test<COUNT>(r, tris, first, count, n_prims, s.h, n_tris, bad);
+    leaf_test<COUNT, SPH>(r, tris, first, count, n_prims, s.h, n_tris, bad);
   }
   // inner children still in reach, sorted by entry distance
   float key[4];
@@ -306,7 +363,7 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
 }
 
 // Whole-ray wide traversal (used where one lane keeps one ray).
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool SPH>
 __device__ inline Hit closest_hit_wide(const WideNode* __restrict__ wnodes,
                                        const float* __restrict__ tris, const BvhHeader& hdr,
                                        const Ray& ray, int32_t& n_nodes, int32_t& n_tris,
@@ -315,8 +372,8 @@ __device__ inline Hit closest_hit_wide(const WideNode* __restrict__ wnodes,
   TravStack<STACK> stack;
   stack.lds = lds_col;
   stack.stride = stride;
-  if (wide_begin<COUNT>(tris, hdr, ray, s, n_tris, bad))
-    while (wide_step<STACK, COUNT>(wnodes, tris, uint32_t(hdr.n_wide), hdr.n_prims, s, stack,
+  if (wide_begin<COUNT, SPH>(tris, hdr, ray, s, n_tris, bad))
+    while (wide_step<STACK, COUNT, SPH>(wnodes, tris, uint32_t(hdr.n_wide), hdr.n_prims, s, stack,
                                    n_nodes, n_tris, overflow, bad)) {
     }
   return s.h;

@@ -84,6 +84,28 @@ def camera_rays(cam: Camera, width: int, height: int, tmin: float = T_MIN_CAMERA
     return r
 
 
+def sphere_prims(centres, radii) -> np.ndarray:
+    """Analytic spheres (NewSphere, primitive.go:31-40) as primitive slots:
+    centre, radius, then NaN as the fifth value (the tag), zeros after."""
+    c = np.asarray(centres, np.float64).reshape(-1, 3)
+    r = np.broadcast_to(np.asarray(radii, np.float64), (len(c),))
+    out = np.zeros((len(c), 9), np.float64)
+    out[:, :3] = c
+    out[:, 3] = r
+    out[:, 4] = np.nan
+    return out
+
+
+def mixed_scene(n_tris: int = 20_000, n_spheres: int = 400, seed: int = 81) -> np.ndarray:
+    """Random triangles in [0, 1]^3 with analytic spheres (radius 0.002 .. 0.03)
+    interleaved in caller order; FP32-exact."""
+    rng = np.random.default_rng(seed)
+    tris = random_triangles(n_tris, seed=seed)
+    sph = sphere_prims(rng.uniform(0, 1, (n_spheres, 3)), rng.uniform(0.002, 0.03, n_spheres))
+    pos = np.concatenate([tris, sph])[rng.permutation(n_tris + n_spheres)]
+    return pos.astype(np.float32).astype(np.float64)
+
+
 def camera_cfg2(width: int = 4096, height: int = 2048) -> Camera:
     """Pinhole camera outside the unit cube looking at its centre; the cube
     fills most of the frame."""
