@@ -259,6 +259,7 @@ struct SegArgs {
 // render.go:209-246, material.go:13-38): miss, light or diffuse scatter, in
 // the mode adj names.  Sets push for a path that continues, and in replay
 // mode the triangles that receive a gradient (al_tri, em_tri) with weights.
+template <bool SPH, bool SPEC>
 __device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p, float4 o4, float4 d4,
                                      const traverse::Hit& h, bool& push, int32_t& al_tri, float4& al_w,
                                      int32_t& em_tri, float4& em_w) {
@@ -310,7 +311,7 @@ __device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p
       }
     } else {
       const float* tv = tris + 9 * size_t(h.pos);
-      const bool sph = prim_is_sphere(tv);
+      const bool sph = SPH && prim_is_sphere(tv);
       // hit point = ray.position(t) (camera.go:108-111), evaluated in FP64
       const float hx = float(double(o4.x) + double(d4.x) * h.t64);
       const float hy = float(double(o4.y) + double(d4.y) * h.t64);
@@ -342,7 +343,7 @@ __device__ inline void shade_segment(const SegArgs& g, const Adj& adj, int32_t p
       float sphi, cphi;
       sincosf(6.2831853071795864f * u01(r.v[1]), &sphi, &cphi);
       const float ux = rr * cphi, uy = rr * sphi, uz = z;
-      const int kind = int(m.x);
+      const int kind = SPEC ? int(m.x) : TRACE_MAT_DIFFUSE;
       float sx, sy, sz;
       if (kind == TRACE_MAT_DIFFUSE) {
         // Diffuse.scatter (material.go:29-38): n + unit vector, n if that is ~0
@@ -562,7 +563,7 @@ int refresh_geo_normals(trace_ctx* ctx, trace_mesh* mesh) {
   return TRACE_OK;
 }
 
-template <int THREADS, int STACK, bool SPH>
+template <int THREADS, int STACK, bool SPH, bool SPEC>
 __global__ __launch_bounds__(THREADS) void bounce_kernel(
     const WideNode* __restrict__ wnodes, const SegArgs g,
     const int32_t* __restrict__ q_in, int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
@@ -588,7 +589,7 @@ __global__ __launch_bounds__(THREADS) void bounce_kernel(
     if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
     if (bad) atomicOr(d_err, TRACE_ERR_TREE);
     traverse::refine_hit_fp64(g.tris, ray, h);
-    shade_segment(g, adj, p, o4, d4, h, push, al_tri, al_w, em_tri, em_w);
+    shade_segment<SPH, SPEC>(g, adj, p, o4, d4, h, push, al_tri, al_w, em_tri, em_w);
   }
   segment_epilogue(adj, g.pps, p, push, al_tri, al_w, em_tri, em_w, q_out, count_out);
 }
@@ -680,7 +681,7 @@ __global__ __launch_bounds__(THREADS) void trace_persistent(
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);
 }
 
-template <int THREADS>
+template <int THREADS, bool SPH, bool SPEC>
 __global__ __launch_bounds__(THREADS) void shade_kernel(const SegArgs g, const int32_t* __restrict__ q_in,
                                                         int32_t* __restrict__ q_out,
                                                         const uint32_t* __restrict__ count_in,
@@ -707,12 +708,12 @@ __global__ __launch_bounds__(THREADS) void shade_kernel(const SegArgs g, const i
     h.v = __int_as_float(r.w);
     const traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
     traverse::refine_hit_fp64(g.tris, ray, h);
-    shade_segment(g, adj, p, o4, d4, h, push, al_tri, al_w, em_tri, em_w);
+    shade_segment<SPH, SPEC>(g, adj, p, o4, d4, h, push, al_tri, al_w, em_tri, em_w);
   }
   segment_epilogue(adj, g.pps, p, push, al_tri, al_w, em_tri, em_w, q_out, count_out);
 }
 
-template <int THREADS, bool SPH>
+template <int THREADS, bool SPH, bool SPEC>
 static void launch_segments(trace_ctx* ctx, const trace_bvh* bvh, int grid, const SegArgs& g, int32_t* qin,
                             int32_t* qout, uint32_t* cin, uint32_t* cout, int4* hits, const Adj& adj) {
   hipStream_t st = ctx->stream;
@@ -728,14 +729,14 @@ static void launch_segments(trace_ctx* ctx, const trace_bvh* bvh, int grid, cons
     else
       trace_persistent<THREADS, 128, SPH><<<pgrid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, cin, hits,
                                                                      ctx->d_err, claim);
-    shade_kernel<THREADS><<<grid, THREADS, 0, st>>>(g, qin, qout, cin, cout, hits, adj);
+    shade_kernel<THREADS, SPH, SPEC><<<grid, THREADS, 0, st>>>(g, qin, qout, cin, cout, hits, adj);
     return;
   }
   if (deep)
-    bounce_kernel<THREADS, 512, SPH><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin, cout,
+    bounce_kernel<THREADS, 512, SPH, SPEC><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin, cout,
                                                                ctx->d_err, adj);
   else
-    bounce_kernel<THREADS, 128, SPH><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin, cout,
+    bounce_kernel<THREADS, 128, SPH, SPEC><<<grid, THREADS, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin, cout,
                                                                ctx->d_err, adj);
 }
 
@@ -764,10 +765,16 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
   g.thr = thr;
   g.rad = rad;
   g.seg_total = segment_counter(ctx);
-  if (bvh->spheres())
-    launch_segments<THREADS, true>(ctx, bvh, grid, g, qin, qout, cin, cout, hits, adj);
-  else
-    launch_segments<THREADS, false>(ctx, bvh, grid, g, qin, qout, cin, cout, hits, adj);
+  // the shader's sphere and specular branches are compiled in only for
+  // meshes that need them
+  const bool spec = mesh->has_specular;
+  if (bvh->spheres()) {
+    if (spec) launch_segments<THREADS, true, true>(ctx, bvh, grid, g, qin, qout, cin, cout, hits, adj);
+    else launch_segments<THREADS, true, false>(ctx, bvh, grid, g, qin, qout, cin, cout, hits, adj);
+  } else {
+    if (spec) launch_segments<THREADS, false, true>(ctx, bvh, grid, g, qin, qout, cin, cout, hits, adj);
+    else launch_segments<THREADS, false, false>(ctx, bvh, grid, g, qin, qout, cin, cout, hits, adj);
+  }
 }
 
 namespace {
