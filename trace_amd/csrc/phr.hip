@@ -4,7 +4,7 @@
 // The reference recursion is a tree of independent jobs: a job holds a cut
 // (LBVH nodes, a bounding box, a depth), splits it with SweepSAH and refines
 // both halves into child jobs.  Here the jobs of one tree level run as one
-// launch, one wave per job:
+// launch, one wave per job of a large cut:
 //   sort      three stable sorts of the cut by box centre x, y, z (rank by
 //             counting: rank = #smaller + #equal-and-earlier), with the z sort
 //             fed from the y or x order exactly as phr.go:384-404 does
@@ -14,6 +14,8 @@
 //   refine    keep a node if it is a leaf or SA < S / 2^(alpha*depth+delta),
 //             else replace it by its children; one survivor becomes a leaf of
 //             all its triangles (phr.go:313-352)
+// Jobs whose cuts hold at most kSmallCut nodes (most of the deep levels) run
+// one per lane instead, the same steps done sequentially.
 // Child jobs, output nodes and leaf lists are appended with atomics; the tree
 // *structure* and every triangle range are fixed by the parent, so the result
 // is deterministic.  A final pass lays the tree out as the binary traversal
@@ -31,6 +33,7 @@
 
 namespace phr {
 
+constexpr int kSmallCut = 8;  // jobs with cuts this small run one per lane
 constexpr int kSortLds = 4096;  // cuts of 65 .. this size sort in LDS (bitonic); larger ones by ranking
 constexpr int kMaxDepth = 1024;
 
@@ -66,6 +69,7 @@ struct Ctl {
   TRACE_CTL_FIELD(err);        // 1 capacity, 2 depth table
   TRACE_CTL_FIELD(cut);        // initial cut size
   TRACE_CTL_FIELD(max_m);      // largest cut among the jobs appended for the next level
+  TRACE_CTL_FIELD(n_small);    // small jobs (cuts of at most kSmallCut) appended for the next level
 };
 #undef TRACE_CTL_FIELD
 
@@ -281,7 +285,7 @@ __device__ inline WBox wb_shfl_xor(const WBox& b, int o) {
 }
 // inclusive scan across lanes; unions and counts are exact in any order
 __device__ inline WBox wb_scan(WBox v) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   for (int o = 1; o < 64; o <<= 1) {
     const WBox u = wb_shfl_up(v, o);
     if (lane >= o) wb_merge(v, u);
@@ -301,7 +305,7 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
   return v;
 }
 __device__ inline uint32_t wave_excl(uint32_t v, uint32_t* total) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   uint32_t x = v;
   for (int o = 1; o < 64; o <<= 1) {
     const uint32_t y = __shfl_up(x, o, 64);
@@ -327,7 +331,7 @@ __device__ inline void load_member(const Flat& f, int32_t node, WBox& b, bool wi
 // two, padded with +inf) for cuts that fit, rank by counting otherwise.
 __device__ void sort_axis(const Flat& f, const int32_t* ids, const int32_t* in, int32_t* out,
                           int axis, int m, double* keys, double* lkey, int32_t* lpos, int P) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   if (m <= 64) {
     // one pair per lane, bitonic network over lane shuffles
     double key = INFINITY;
@@ -415,7 +419,7 @@ struct Split {
 __device__ Split min_cost(const Flat& f, const int32_t* ids, const int32_t* ord, int m,
                           const float* cut_box, double* rcost) {
 #pragma clang fp contract(off)
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   WBox carry;
   wb_empty(carry);
   for (int end = m; end > 0; end -= 64) {
@@ -494,8 +498,8 @@ struct Caps {
 __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, int a, int b,
                           double th, const WBox& hb, int32_t prim_off, int32_t prim_cnt,
                           const Task& t, int32_t o, int32_t* pool_next, Task* tasks_next,
-                          OutNode* out, int32_t* list, Ctl* ctl, const Caps& caps) {
-  const int lane = threadIdx.x;
+                          Task* small_next, OutNode* out, int32_t* list, Ctl* ctl, const Caps& caps) {
+  const int lane = threadIdx.x & 63;
   uint32_t k_total = 0;
   for (int r = a + lane; r < b; r += 64) {
     const int32_t node = ids[C[r]];
@@ -518,9 +522,10 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
     }
     return;
   }
+  const bool small = k_total <= uint32_t(kSmallCut);
   int32_t slot = 0, poff = 0;
   if (lane == 0) {
-    slot = int32_t(atomicAdd(&ctl->n_tasks, 1u));
+    slot = int32_t(atomicAdd(small ? &ctl->n_small : &ctl->n_tasks, 1u));
     poff = int32_t(atomicAdd(&ctl->pool, k_total));
   }
   slot = __shfl(slot, 0, 64);
@@ -551,7 +556,7 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
     carry += tot;
   }
   if (lane == 0) {
-    raise_max(&ctl->max_m, k_total);
+    if (!small) raise_max(&ctl->max_m, k_total);
     Task n;
     n.off = poff;
     n.cnt = int32_t(k_total);
@@ -560,7 +565,7 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
     n.out = o;
     n.prim_off = prim_off;
     n.level = t.level + 1;
-    tasks_next[slot] = n;
+    (small ? small_next : tasks_next)[slot] = n;
   }
 }
 
@@ -570,12 +575,13 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
                                                   const int32_t* __restrict__ pool,
                                                   int32_t* __restrict__ pool_next,
                                                   Task* __restrict__ tasks_next,
+                                                  Task* __restrict__ small_next,
                                                   int32_t* __restrict__ ordbuf, double* gkey,
                                                   double* rcost_buf, OutNode* __restrict__ out,
                                                   int32_t* __restrict__ list, Ctl* ctl, Caps caps,
                                                   int lds_p) {
   extern __shared__ double s_dyn[];  // P keys then P positions (P = lds_p, 0 = no LDS sort)
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const Task t = tasks[blockIdx.x];
   const int m = t.cnt;
   const int P = m <= lds_p ? lds_p : 0;
@@ -634,9 +640,9 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
       write_out(out, t.out, 0, 2, o, o + 1, -1, 0, t.prim_off, int32_t(pl + pr), t.level, t.box);
     const double thd = th[t.depth];  // refined() uses the split cut's depth
     emit_half(f, ids, C, 0, s.at, thd, L, t.prim_off, int32_t(pl), t, o, pool_next, tasks_next,
-              out, list, ctl, caps);
+              small_next, out, list, ctl, caps);
     emit_half(f, ids, C, s.at, m, thd, R, t.prim_off + int32_t(pl), int32_t(pr), t, o + 1,
-              pool_next, tasks_next, out, list, ctl, caps);
+              pool_next, tasks_next, small_next, out, list, ctl, caps);
   } else {
     // no split beats the whole cut: one leaf of every triangle, in order C
     uint32_t pc = 0;
@@ -665,6 +671,254 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
   }
 }
 
+// ---- small jobs: one lane per job ----------------------------------------------
+// The deep levels hold hundreds of thousands of jobs whose cuts have a few
+// nodes; a wave per job spends its time on memory round trips and on
+// per-job counter atomics.  Here each lane runs one such job (at most
+// kSmallCut nodes) -- the three stable sorts, minCost and the refinement --
+// sequentially, with the same FP64 operations in the same order as the wave
+// form (min/max unions are exact in any order), so the tree is the same; the
+// wave then allocates the outputs of its 64 jobs with one atomic per counter.
+
+__device__ inline double centre(const Flat& f, int32_t node, int axis) {
+  const float* b = f.box + 6 * size_t(node);
+  return (double(b[axis]) + double(b[3 + axis])) * (1.0 / 2.0);
+}
+
+// Stable sort by box centre on `axis` of the members in order `in` (indices
+// into ids): insertion sort on (centre, input position).
+__device__ inline void small_sort(const Flat& f, const int32_t* ids, const int* in, int* o, int m, int axis) {
+  double key[kSmallCut];
+  for (int i = 0; i < kSmallCut; ++i) {
+    if (i >= m) break;
+    o[i] = in ? in[i] : i;
+    key[i] = centre(f, ids[o[i]], axis);
+    for (int j = i; j > 0 && key[j - 1] > key[j]; --j) {
+      const double tk = key[j - 1];
+      key[j - 1] = key[j];
+      key[j] = tk;
+      const int to = o[j - 1];
+      o[j - 1] = o[j];
+      o[j] = to;
+    }
+  }
+}
+
+__device__ inline void box_merge(float* lo, float* hi, const Flat& f, int32_t node) {
+  const float* b = f.box + 6 * size_t(node);
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = fminf(lo[a], b[a]);
+    hi[a] = fmaxf(hi[a], b[3 + a]);
+  }
+}
+
+__device__ inline double sa_lohi(const float* lo, const float* hi) {
+  const float bb[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+  return sa_box(bb);
+}
+
+// minCost (phr.go:414-459) of order o, as min_cost computes it.
+__device__ inline Split small_cost(const Flat& f, const int32_t* ids, const int* o, int m, const float* cut_box) {
+#pragma clang fp contract(off)
+  double rc[kSmallCut];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  uint32_t cnt = 0;
+  for (int r = m - 1; r >= 1; --r) {
+    const int32_t node = ids[o[r]];
+    box_merge(lo, hi, f, node);
+    cnt += uint32_t(f.size[node]);
+    rc[r] = sa_lohi(lo, hi) * double(cnt);
+  }
+  const uint32_t total = cnt + uint32_t(f.size[ids[o[0]]]);
+  float plo[3] = {INFINITY, INFINITY, INFINITY}, phi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  uint32_t pcnt = 0;
+  double best = INFINITY;
+  int32_t best_at = m;
+  for (int r = 1; r < m; ++r) {
+    const int32_t node = ids[o[r - 1]];
+    box_merge(plo, phi, f, node);
+    pcnt += uint32_t(f.size[node]);
+    const double c = rc[r] + sa_lohi(plo, phi) * double(pcnt);
+    if (c < best) {
+      best = c;
+      best_at = r;
+    }
+  }
+  const double nosplit = sa_box(cut_box) * double(total);
+  if (best < nosplit) return Split{best, best_at};
+  return Split{nosplit, 0};
+}
+
+__device__ inline bool keep_node(const Flat& f, int32_t node, double th) {
+  return f.child[2 * node] < 0 || sa_box(f.box + 6 * size_t(node)) < th;
+}
+
+__device__ inline uint32_t wave_max(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, uint32_t(__shfl_xor(v, o, 64)));
+  return v;
+}
+
+// Wave-aggregated allocation: every lane receives base + its exclusive prefix.
+__device__ inline uint32_t wave_alloc(uint32_t* counter, uint32_t n) {
+  uint32_t total;
+  const uint32_t ex = wave_excl(n, &total);
+  uint32_t base = 0;
+  if ((threadIdx.x & 63) == 0 && total) base = atomicAdd(counter, total);
+  return __shfl(base, 0, 64) + ex;
+}
+
+__global__ __launch_bounds__(256) void small_task_kernel(Flat f, const double* __restrict__ th,
+                                                         const Task* __restrict__ tasks, int32_t n_jobs,
+                                                         const int32_t* __restrict__ pool,
+                                                         int32_t* __restrict__ pool_next,
+                                                         Task* __restrict__ tasks_next,
+                                                         Task* __restrict__ small_next,
+                                                         OutNode* __restrict__ out, int32_t* __restrict__ list,
+                                                         Ctl* ctl, Caps caps) {
+  const int32_t job = blockIdx.x * 256 + int32_t(threadIdx.x);
+  bool valid = job < n_jobs;
+  Task t{};
+  int m = 0;
+  if (valid) {
+    t = tasks[job];
+    m = t.cnt;
+    if (m < 2 || m > kSmallCut || t.depth >= kMaxDepth) {
+      atomicOr(&ctl->err, 2u);
+      valid = false;
+    }
+  }
+  int32_t ids[kSmallCut];
+  int oX[kSmallCut], oY[kSmallCut], oZ[kSmallCut];
+  const int* C = oX;
+  Split s{0.0, 0};
+  // what the job emits: output nodes, leaf-list entries, and per half a leaf
+  // (k == 1) or a job of k pool entries
+  uint32_t n_out = 0, n_list = 0, k_half[2] = {0, 0};
+  uint32_t pl = 0, pr = 0, pc = 0;
+  float L[6], R[6];
+  double thd = 0.0;
+  if (valid) {
+    for (int k = 0; k < kSmallCut; ++k)
+      if (k < m) ids[k] = pool[t.off + k];
+    // SweepSAH (phr.go:368-405)
+    small_sort(f, ids, nullptr, oX, m, 0);
+    small_sort(f, ids, oX, oY, m, 1);
+    const Split x = small_cost(f, ids, oX, m, t.box);
+    const Split y = small_cost(f, ids, oY, m, t.box);
+    small_sort(f, ids, x.cost < y.cost ? oY : oX, oZ, m, 2);
+    const Split z = small_cost(f, ids, oZ, m, t.box);
+    if (x.cost < y.cost) {
+      if (x.cost < z.cost) { C = oX; s = x; } else { C = oZ; s = z; }
+    } else {
+      if (y.cost < z.cost) { C = oY; s = y; } else { C = oZ; s = z; }
+    }
+    if (s.at > 0) {
+      for (int a = 0; a < 3; ++a) {
+        L[a] = R[a] = INFINITY;
+        L[3 + a] = R[3 + a] = -INFINITY;
+      }
+      thd = th[t.depth];  // refined() uses the split cut's depth
+      for (int r = 0; r < m; ++r) {
+        const int32_t node = ids[C[r]];
+        const uint32_t p = uint32_t(f.prim[2 * node + 1] - f.prim[2 * node]);
+        const uint32_t k = keep_node(f, node, thd) ? 1u : 2u;
+        if (r < s.at) {
+          box_merge(L, L + 3, f, node);
+          pl += p;
+          k_half[0] += k;
+        } else {
+          box_merge(R, R + 3, f, node);
+          pr += p;
+          k_half[1] += k;
+        }
+      }
+      n_out = 2;
+      for (int h = 0; h < 2; ++h) n_list += k_half[h] == 1 ? 1u : 0u;
+    } else {
+      for (int r = 0; r < m; ++r) {
+        const int32_t node = ids[C[r]];
+        pc += uint32_t(f.prim[2 * node + 1] - f.prim[2 * node]);
+      }
+      n_out = 1;
+      n_list = uint32_t(m);
+    }
+  }
+  // one allocation per counter per wave
+  uint32_t n_pool = 0, n_sm = 0, n_lg = 0, max_lg = 0, max_lvl = 0;
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t k = k_half[h];
+    if (k >= 2) {
+      n_pool += k;
+      if (k <= uint32_t(kSmallCut)) ++n_sm;
+      else {
+        ++n_lg;
+        max_lg = max(max_lg, k);
+      }
+    }
+  }
+  if (valid && (s.at == 0 || k_half[0] == 1 || k_half[1] == 1)) max_lvl = uint32_t(t.level + 1);
+  const int32_t o = int32_t(wave_alloc(&ctl->out, n_out));
+  const int32_t ls = int32_t(wave_alloc(&ctl->list, n_list));
+  const int32_t po = int32_t(wave_alloc(&ctl->pool, n_pool));
+  const int32_t sl_small = int32_t(wave_alloc(&ctl->n_small, n_sm));
+  const int32_t sl_large = int32_t(wave_alloc(&ctl->n_tasks, n_lg));
+  max_lg = wave_max(max_lg);
+  max_lvl = wave_max(max_lvl);
+  if ((threadIdx.x & 63) == 0) {
+    if (max_lg) raise_max(&ctl->max_m, max_lg);
+    if (max_lvl) raise_max(&ctl->max_level, max_lvl);
+  }
+  if (!valid) return;
+  if (int64_t(o) + n_out > caps.out || int64_t(ls) + n_list > caps.list || int64_t(po) + n_pool > caps.pool ||
+      int64_t(sl_small) + n_sm > caps.tasks || int64_t(sl_large) + n_lg > caps.tasks) {
+    atomicOr(&ctl->err, 1u);
+    return;
+  }
+  if (s.at == 0) {
+    // no split beats the whole cut: one leaf of every triangle, in order C
+    for (int r = 0; r < m; ++r) list[ls + r] = ids[C[r]];
+    write_out(out, o, 1, 0, -1, -1, ls, m, t.prim_off, int32_t(pc), t.level + 1, t.box);
+    write_out(out, t.out, 0, 1, o, -1, -1, 0, t.prim_off, int32_t(pc), t.level, t.box);
+    return;
+  }
+  write_out(out, t.out, 0, 2, o, o + 1, -1, 0, t.prim_off, int32_t(pl + pr), t.level, t.box);
+  int32_t lnext = ls, pnext = po, snext = sl_small, gnext = sl_large;
+  for (int h = 0; h < 2; ++h) {
+    const int a = h == 0 ? 0 : s.at, b = h == 0 ? s.at : m;
+    const float* hb = h == 0 ? L : R;
+    const int32_t poff_prim = h == 0 ? t.prim_off : t.prim_off + int32_t(pl);
+    const int32_t pcnt = int32_t(h == 0 ? pl : pr);
+    const uint32_t k = k_half[h];
+    if (k == 1) {  // one kept node: a leaf of all its triangles
+      list[lnext] = ids[C[a]];
+      write_out(out, o + h, 1, 0, -1, -1, lnext, 1, poff_prim, pcnt, t.level + 1, hb);
+      ++lnext;
+      continue;
+    }
+    int32_t p = pnext;
+    for (int r = a; r < b; ++r) {
+      const int32_t node = ids[C[r]];
+      if (keep_node(f, node, thd)) {
+        pool_next[p++] = node;
+      } else {
+        pool_next[p++] = f.child[2 * node];
+        pool_next[p++] = f.child[2 * node + 1];
+      }
+    }
+    Task n;
+    n.off = pnext;
+    n.cnt = int32_t(k);
+    for (int q = 0; q < 6; ++q) n.box[q] = hb[q];
+    n.depth = t.depth + 1;
+    n.out = o + h;
+    n.prim_off = poff_prim;
+    n.level = t.level + 1;
+    if (k <= uint32_t(kSmallCut)) small_next[snext++] = n;
+    else tasks_next[gnext++] = n;
+    pnext += int32_t(k);
+  }
+}
+
 // Leaves of the output tree: one atomic per workgroup.
 __global__ __launch_bounds__(256) void count_leaves_kernel(const OutNode* __restrict__ out, int32_t n,
                                                            Ctl* ctl) {
@@ -679,6 +933,7 @@ __global__ __launch_bounds__(256) void count_leaves_kernel(const OutNode* __rest
 
 __global__ void reset_level_kernel(Ctl* ctl) {
   ctl->n_tasks = 0;
+  ctl->n_small = 0;
   ctl->pool = 0;
   ctl->max_m = 0;
 }
@@ -851,6 +1106,8 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const size_t o_pb = plan.take(pool_cap * sizeof(int32_t));
   const size_t o_ta = plan.take(task_cap * sizeof(Task));
   const size_t o_tb = plan.take(task_cap * sizeof(Task));
+  const size_t o_sa = plan.take(task_cap * sizeof(Task));
+  const size_t o_sb = plan.take(task_cap * sizeof(Task));
   const size_t o_ord = plan.take(3 * pool_cap * sizeof(int32_t));
   const size_t o_key = plan.take(pool_cap * sizeof(double));
   const size_t o_rc = plan.take(pool_cap * sizeof(double));
@@ -873,6 +1130,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   auto* fb = reinterpret_cast<int32_t*>(base + o_fb);
   int32_t* pools[2] = {reinterpret_cast<int32_t*>(base + o_pa), reinterpret_cast<int32_t*>(base + o_pb)};
   Task* tasks[2] = {reinterpret_cast<Task*>(base + o_ta), reinterpret_cast<Task*>(base + o_tb)};
+  Task* small[2] = {reinterpret_cast<Task*>(base + o_sa), reinterpret_cast<Task*>(base + o_sb)};
   auto* ord = reinterpret_cast<int32_t*>(base + o_ord);
   auto* gkey = reinterpret_cast<double*>(base + o_key);
   auto* rcost = reinterpret_cast<double*>(base + o_rc);
@@ -901,7 +1159,8 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   if (hc.cut <= 1) return clone_lbvh(ctx, aux, out);  // the whole LBVH is attached as is
 
   const Caps caps{pool_cap, out_cap, list_cap, task_cap};
-  uint32_t n_tasks = 1;
+  uint32_t n_tasks = 1;  // wave-per-job list (the initial cut's job is on it)
+  uint32_t n_small = 0;  // lane-per-job list
   uint32_t max_m = hc.cut;  // the one job of the first level holds the initial cut
   // Every job holds a cut of at least two members and makes one split attempt
   // (branching 2), so BuildWithCost's counter (phr.go:221) gains the level's
@@ -909,7 +1168,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   uint64_t level_pool = hc.cut;
   int cur = 0;
   int levels = 0;
-  while (n_tasks > 0) {
+  while (n_tasks + n_small > 0) {
     // LDS for the bitonic sort, sized to the level's largest cut
     // (cuts of up to 64 sort in registers; larger ones up to kSortLds in LDS,
     // which every workgroup of the level then reserves)
@@ -921,13 +1180,19 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     const size_t lds_bytes = size_t(lds_p) * (sizeof(double) + sizeof(int32_t));
     out->phr_split_work += int64_t(level_pool);
     reset_level_kernel<<<1, 1, 0, st>>>(ctl);
-    task_kernel<<<int(n_tasks), 64, lds_bytes, st>>>(f, d_th, tasks[cur], pools[cur], pools[cur ^ 1],
-                                                    tasks[cur ^ 1], ord, gkey, rcost, outn, list, ctl,
-                                                    caps, lds_p);
+    if (n_tasks > 0)
+      task_kernel<<<int(n_tasks), 64, lds_bytes, st>>>(f, d_th, tasks[cur], pools[cur], pools[cur ^ 1],
+                                                      tasks[cur ^ 1], small[cur ^ 1], ord, gkey, rcost,
+                                                      outn, list, ctl, caps, lds_p);
+    if (n_small > 0)
+      small_task_kernel<<<int(ceil_div(n_small, 256)), 256, 0, st>>>(
+          f, d_th, small[cur], int32_t(n_small), pools[cur], pools[cur ^ 1], tasks[cur ^ 1], small[cur ^ 1],
+          outn, list, ctl, caps);
     rc = read_ctl(ctx, ctl, &hc);
     if (rc) return rc;
     if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x at level %d", hc.err, levels);
     n_tasks = hc.n_tasks;
+    n_small = hc.n_small;
     max_m = hc.max_m;
     level_pool = hc.pool;
     cur ^= 1;
