@@ -635,6 +635,13 @@ def main():
         out["phr_grid"] = {"ms": round((time.perf_counter() - t0) * 1e3, 1), "alpha": ga, "delta": gd,
                            "builds": int(len(gsah)), "sah_min": round(float(gsah.min()), 4),
                            "build_cost_min": int(gbuild.min())}
+        note("PHR Bayesian optimiser (2 normaliser builds + 20 evaluations)")
+        from trace_amd.device import phr_optimize_bayes
+        t0 = time.perf_counter()
+        ba, bd, btr = phr_optimize_bayes(ctx, lbvh, width=1920, height=1080, rounds=20, random_rounds=5, seed=1)
+        out["phr_bayes"] = {"ms": round((time.perf_counter() - t0) * 1e3, 1), "alpha": round(ba, 5),
+                            "delta": round(bd, 5), "builds": int(len(btr)),
+                            "objective_min": round(float(np.nanmin(btr[2:, 4])), 6)}
         note("heat map (traversal steps in the reference's walk order, FP64)")
         out["heatmap"] = bench_heatmap(torch, ctx, phr, max(3, args.steps // 4))
         note("traversal and builds timed; small render")

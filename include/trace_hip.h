@@ -163,6 +163,37 @@ int trace_phr_optimize_grid(trace_ctx* ctx, const trace_bvh* aux, const double* 
                             const double* h_deltas, int32_t n_delta, int branching, int32_t max_cut,
                             int32_t frame_width, int32_t frame_height, double* h_sah, int64_t* h_build,
                             double* alpha_out, double* delta_out);
+/* bayesianOptimizer.OptimizedPHRparams (optimizer.go:96-155; NewDefaultBayesianOptimizer
+ * is alpha in [0.4, 0.55], delta in [6, 9]).  The normalisers are Cost() of the tree at
+ * (alpha_range[0], delta_range[0]) and the build cost at (alpha_range[1], delta_range[1]);
+ * each of `rounds` evaluations scores build / max build + omega^2 * Cost / max Cost with
+ * omega as in trace_phr_optimize_grid (a zero alpha or delta scores the normalisers
+ * themselves, optimizer.go:142-144).  The points come from trace_bayes_minimize with
+ * `random_rounds` uniform samples first and `seed` (the reference's third-party search,
+ * changkun.de/x/bo v0.0.1, is replaced by that function).  h_trace (optional,
+ * (rounds + 2) x 5 doubles) receives per row alpha, delta, Cost, build cost, objective:
+ * the two normaliser builds (objective NaN), then the evaluations in order.  Writes the
+ * evaluated (alpha, delta) of the smallest objective (first minimum). */
+int trace_phr_optimize_bayes(trace_ctx* ctx, const trace_bvh* aux, const double* alpha_range,
+                             const double* delta_range, int branching, int32_t max_cut, int32_t frame_width,
+                             int32_t frame_height, int32_t rounds, int32_t random_rounds, uint64_t seed,
+                             double* h_trace, double* alpha_out, double* delta_out);
+/* Objective for trace_bayes_minimize: writes f(x) to *y; a non-zero return aborts the
+ * search and is passed on. */
+typedef int (*trace_objective_fn)(void* user, const double* x, double* y);
+/* bo.Optimizer.RunSerial as called at optimizer.go:141-151 (minimise, expected
+ * improvement), host only (no device, no context): minimises `objective` over the box
+ * [lo, hi] (dims 1..8) with `rounds` evaluations.  The first max(1, random_rounds) sample
+ * the box uniformly from a splitmix64 stream seeded with `seed`; each later one fits a
+ * Gaussian process (Matern 5/2 kernel in unit-cube coordinates, standardised values,
+ * length scale of largest marginal likelihood) and evaluates the point of largest
+ * expected improvement.  h_x (rounds x dims) and h_y (rounds) receive the evaluations
+ * (optional); x_best the evaluated point of smallest value (first minimum).  The same
+ * seed and objective give the same sequence.  Returns TRACE_E_INVALID for bad arguments
+ * or a NaN value. */
+int trace_bayes_minimize(const double* lo, const double* hi, int32_t dims, int32_t rounds, int32_t random_rounds,
+                         uint64_t seed, trace_objective_fn objective, void* user, double* h_x, double* h_y,
+                         double* x_best);
 /* For a PHR tree: per PHR-ordered triangle, its sorted position in the LBVH it was
  * built from (relates PHR leaves to the LBVH order). */
 int trace_phr_export_source(trace_ctx* ctx, const trace_bvh* phr, int32_t* h_src);

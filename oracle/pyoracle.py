@@ -228,6 +228,32 @@ class PhrTree:
         return rec, box, prims
 
 
+def omega(prims: int, pixels: int) -> float:
+    """omega (optimizer.go:88-90): (prims / PRIM_THRESH)^2 * pixels / RES_THRESH."""
+    r = float(prims) / 2000000
+    return (r * r) * (float(pixels) / 200000)
+
+
+def eval_phr(sah: float, max_sah: float, build: int, max_build: int, om: float) -> float:
+    """evalPHR (optimizer.go:92-94)."""
+    return float(build) / float(max_build) + (om * om) * (sah / max_sah)
+
+
+def bayes_objective(tree: Tree, alpha: float, delta: float, alpha_range, delta_range, width: int, height: int,
+                    branching: int = 2, max_cut: int = 2500) -> tuple:
+    """The objective bayesianOptimizer.OptimizedPHRparams hands to its search
+    (optimizer.go:132-151) at one (alpha, delta): Cost() of the tree at the range
+    minimum and the build cost at the range maximum normalise evalPHR; a zero
+    parameter scores the normalisers.  Returns (objective, Cost, build cost)."""
+    max_sah = PhrTree(tree, alpha_range[0], delta_range[0], branching, max_cut).cost()
+    max_build = PhrTree(tree, alpha_range[1], delta_range[1], branching, max_cut).build_cost()
+    sah, build = max_sah, max_build
+    if alpha != 0 and delta != 0:
+        p = PhrTree(tree, alpha, delta, branching, max_cut)
+        sah, build = p.cost(), p.build_cost()
+    return eval_phr(sah, max_sah, build, max_build, omega(tree.n, width * height)), sah, build
+
+
 def grid_optimize(tree: Tree, alphas, deltas, width: int, height: int, branching: int = 2,
                   max_cut: int = 2500):
     """gridOptimizer.OptimizedPHRparams (optimizer.go:44-94): BuildWithCost
@@ -242,11 +268,10 @@ def grid_optimize(tree: Tree, alphas, deltas, width: int, height: int, branching
             evals.append((p.cost(), p.build_cost(), float(a), float(d)))
     max_build = max(e[1] for e in evals)
     max_sah = max(e[0] for e in evals)
-    r = float(tree.n) / 2000000
-    omega = (r * r) * (float(width * height) / 200000)
+    om = omega(tree.n, width * height)
     best, best_cost = None, np.finfo(np.float64).max
     for sah, build, a, d in evals:
-        c = float(build) / float(max_build) + (omega * omega) * (sah / max_sah)
+        c = eval_phr(sah, max_sah, build, max_build, om)
         if c < best_cost:
             best_cost, best = c, (a, d)
     return best[0], best[1], [(e[0], e[1]) for e in evals]

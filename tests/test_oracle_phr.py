@@ -320,3 +320,41 @@ def test_grid_optimizer_selection(oracle):
     # every evaluation is BuildWithCost + Cost of that (alpha, delta)
     p = oracle.PhrTree(tree, alphas[1], deltas[2])
     assert ev[5] == (p.cost(), p.build_cost())
+
+
+def test_omega_and_eval_phr_closed_forms():
+    """optimizer.go:88-94 at values whose results are exact: a dropped square,
+    a swapped ratio or a lost term changes each of them."""
+    from oracle.pyoracle import eval_phr, omega
+
+    assert omega(2_000_000, 200_000) == 1.0
+    assert omega(1_000_000, 400_000) == 0.5  # (1/2)^2 * 2
+    assert omega(4_000_000, 100_000) == 2.0  # 2^2 * (1/2)
+    assert eval_phr(1.0, 4.0, 3, 6, 2.0) == 1.5  # 3/6 + 2^2 * 1/4
+    assert eval_phr(5.0, 5.0, 7, 7, 0.0) == 1.0
+    assert eval_phr(2.0, 8.0, 0, 9, 3.0) == 2.25  # 0 + 9 * 1/4
+
+
+def test_bayes_objective_closed_forms(oracle):
+    """optimizer.go:132-151: at the range minimum the SAH ratio is 1, at the
+    range maximum the build ratio is 1, and a zero parameter scores the two
+    normalisers, 1 + omega^2."""
+    from oracle.pyoracle import bayes_objective, omega
+
+    pos = scenes.random_triangles(3000, seed=22, mean_edge=0.02)
+    tree = oracle.Tree(pos)
+    ar, dr = (0.4, 0.55), (6.0, 9.0)
+    om = omega(tree.n, 640 * 480)
+    lo = oracle.PhrTree(tree, ar[0], dr[0])
+    hi = oracle.PhrTree(tree, ar[1], dr[1])
+    y, sah, build = bayes_objective(tree, ar[0], dr[0], ar, dr, 640, 480)
+    assert (sah, build) == (lo.cost(), lo.build_cost())
+    assert y == float(build) / float(hi.build_cost()) + om * om
+    y, sah, build = bayes_objective(tree, ar[1], dr[1], ar, dr, 640, 480)
+    assert y == 1.0 + (om * om) * (hi.cost() / lo.cost())
+    y, sah, build = bayes_objective(tree, 0.0, 7.0, ar, dr, 640, 480)
+    assert (sah, build) == (lo.cost(), hi.build_cost()) and y == 1.0 + om * om
+    # a point between the ends is that point's BuildWithCost + Cost
+    p = oracle.PhrTree(tree, 0.47, 7.5)
+    y, sah, build = bayes_objective(tree, 0.47, 7.5, ar, dr, 640, 480)
+    assert (sah, build) == (p.cost(), p.build_cost())

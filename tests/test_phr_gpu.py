@@ -171,6 +171,41 @@ def test_grid_optimizer_matches_oracle(ctx, oracle):
     assert (a2, d2) == (oa2, od2)
 
 
+def test_bayes_optimizer_matches_oracle(ctx, oracle):
+    """bayesianOptimizer.OptimizedPHRparams: the normaliser builds and, at every
+    point the search evaluated, the build cost, Cost() and objective equal the
+    oracle's; the result is the evaluated minimum and the same seed repeats the
+    run.  Which points the reference's third-party search (changkun.de/x/bo)
+    would try is parity unpinned."""
+    from oracle.pyoracle import bayes_objective
+    from trace_amd.device import phr_optimize_bayes
+
+    print("phr bayes optimizer", flush=True)
+    pos = scenes.random_triangles(20_000, seed=26)
+    aux = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    ar, dr = (0.4, 0.55), (6.0, 9.0)
+    a, d, tr = phr_optimize_bayes(ctx, aux, ar, dr, width=1920, height=1080, rounds=8, random_rounds=3, seed=4)
+    tree = oracle.Tree(pos)
+    lo = oracle.PhrTree(tree, ar[0], dr[0])
+    hi = oracle.PhrTree(tree, ar[1], dr[1])
+    assert tuple(tr[0, :2]) == (ar[0], dr[0]) and tuple(tr[1, :2]) == (ar[1], dr[1])
+    assert tr[0, 3] == lo.build_cost() and tr[1, 3] == hi.build_cost()
+    assert np.allclose(tr[:2, 2], [lo.cost(), hi.cost()], rtol=1e-12, atol=0)
+    assert np.all(np.isnan(tr[:2, 4]))
+    ev = tr[2:]
+    assert np.all((ev[:, 0] >= ar[0]) & (ev[:, 0] <= ar[1]) & (ev[:, 1] >= dr[0]) & (ev[:, 1] <= dr[1]))
+    assert len({(r[0], r[1]) for r in ev}) > 4  # the search moves
+    for al, de, sah, build, y in ev:
+        oy, osah, obuild = bayes_objective(tree, al, de, ar, dr, 1920, 1080)
+        assert build == obuild, (al, de)
+        assert sah == pytest.approx(osah, rel=1e-12, abs=0)
+        assert y == pytest.approx(oy, rel=1e-12, abs=0)
+    k = int(np.argmin(ev[:, 4]))
+    assert (a, d) == (ev[k, 0], ev[k, 1])
+    a2, d2, tr2 = phr_optimize_bayes(ctx, aux, ar, dr, width=1920, height=1080, rounds=8, random_rounds=3, seed=4)
+    assert (a2, d2) == (a, d) and np.array_equal(tr2[:, :4], tr[:, :4])
+
+
 def test_phr_rejects_unsupported(ctx):
     pos = scenes.random_triangles(100, seed=1)
     aux = Bvh.lbvh(ctx, Mesh(ctx, pos))

@@ -29,6 +29,8 @@ _I64 = C.c_int64
 _I32 = C.c_int32
 _DP = C.POINTER(C.c_double)
 _FP = C.POINTER(C.c_float)
+# trace_objective_fn: int (*)(void* user, const double* x, double* y)
+OBJECTIVE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, _DP, _DP)
 
 
 class TraceError(RuntimeError):
@@ -105,6 +107,8 @@ SIGNATURES = {
     "trace_mesh_export_shading": [_P, _P, _P, _P],
     "trace_phr_build_with_cost": [_P, _P, C.c_double, C.c_double, C.c_int, _I32, _P, _P],
     "trace_phr_optimize_grid": [_P, _P, _P, _I32, _P, _I32, C.c_int, _I32, _I32, _I32, _P, _P, _P, _P],
+    "trace_phr_optimize_bayes": [_P, _P, _P, _P, C.c_int, _I32, _I32, _I32, _I32, _I32, C.c_uint64, _P, _P, _P],
+    "trace_bayes_minimize": [_P, _P, _I32, _I32, _I32, C.c_uint64, OBJECTIVE_FN, _P, _P, _P, _P],
     "trace_traversal_steps": [_P, _P, _P, _I64, _P],
     "trace_traversal_steps_device": [_P, _P, _P, _I64, _P],
     "trace_render_heatmap": [_P, _P, _P, _I32, _I32, _I32, _P, _P],

@@ -419,6 +419,55 @@ def phr_optimize_grid(ctx: Context, aux: Bvh, alphas=(0.4, 0.45, 0.5, 0.55), del
     return ao.value, do.value, sah, build
 
 
+def phr_optimize_bayes(ctx: Context, aux: Bvh, alpha_range=(0.4, 0.55), delta_range=(6.0, 9.0),
+                       width: int = 1920, height: int = 1080, branching: int = 2, max_cut: int = 2500,
+                       rounds: int = 20, random_rounds: int = 5, seed: int = 1):
+    """bayesianOptimizer.OptimizedPHRparams (trace_phr_optimize_bayes): (alpha, delta,
+    trace) with trace rows (alpha, delta, Cost, build cost, objective): the two
+    normaliser builds, then the evaluations in order."""
+    ar = np.ascontiguousarray(alpha_range, np.float64)
+    dr = np.ascontiguousarray(delta_range, np.float64)
+    if ar.shape != (2,) or dr.shape != (2,):
+        raise ValueError("alpha_range and delta_range are (min, max) pairs")
+    trace = np.empty((rounds + 2, 5), np.float64)
+    ao, do = C.c_double(), C.c_double()
+    ctx.check(ctx.lib.trace_phr_optimize_bayes(ctx.h, aux.h, _ptr(ar), _ptr(dr), branching, max_cut, width, height,
+                                               rounds, random_rounds, seed, _ptr(trace), C.byref(ao), C.byref(do)))
+    return ao.value, do.value, trace
+
+
+def bayes_minimize(objective, lo, hi, rounds: int = 20, random_rounds: int = 5, seed: int = 1):
+    """trace_bayes_minimize (host only): minimises objective(x) -> float over the box
+    [lo, hi].  Returns (x_best, xs (rounds, dims), ys (rounds,)).  An exception
+    raised by the objective ends the search and is raised again here."""
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray(hi, np.float64)
+    if lo.ndim != 1 or lo.shape != hi.shape:
+        raise ValueError("lo and hi must be vectors of one length")
+    dims = len(lo)
+    xs = np.empty((rounds, dims), np.float64)
+    ys = np.empty(rounds, np.float64)
+    best = np.empty(dims, np.float64)
+    failure = []
+
+    def call(_user, x, y):
+        try:
+            y[0] = float(objective(np.array([x[k] for k in range(dims)])))
+            return _lib.TRACE_OK
+        except BaseException as e:  # carried across the C frame
+            failure.append(e)
+            return _lib.TRACE_E_INVALID
+
+    fn = _lib.OBJECTIVE_FN(call)
+    rc = _lib.load().trace_bayes_minimize(_ptr(lo), _ptr(hi), dims, rounds, random_rounds, seed, fn, None,
+                                          _ptr(xs), _ptr(ys), _ptr(best))
+    if failure:
+        raise failure[0]
+    if rc != _lib.TRACE_OK:
+        raise _lib.TraceError(rc, "trace_bayes_minimize: bad arguments or a NaN objective")
+    return best, xs, ys
+
+
 def adam_step(ctx: Context, d_param: int, d_grad: int, d_m: int, d_v: int, n: int, lr: float, step: int,
               beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, lo: float = 0.0,
               hi: float = 1.0) -> None:
