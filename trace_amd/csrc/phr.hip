@@ -4,7 +4,8 @@
 // The reference recursion is a tree of independent jobs: a job holds a cut
 // (LBVH nodes, a bounding box, a depth), splits it with SweepSAH and refines
 // both halves into child jobs.  Here the jobs of one tree level run as one
-// launch, one wave per job of a large cut:
+// launch per job list.  Jobs of more than 64 cut nodes run one wave per job,
+// with the cut staged through memory:
 //   sort      three stable sorts of the cut by box centre x, y, z (rank by
 //             counting: rank = #smaller + #equal-and-earlier), with the z sort
 //             fed from the y or x order exactly as phr.go:384-404 does
@@ -14,8 +15,10 @@
 //   refine    keep a node if it is a leaf or SA < S / 2^(alpha*depth+delta),
 //             else replace it by its children; one survivor becomes a leaf of
 //             all its triangles (phr.go:313-352)
-// Jobs whose cuts hold at most kSmallCut nodes (most of the deep levels) run
-// one per lane instead, the same steps done sequentially.
+// Smaller jobs (most of the deep levels) keep their cut in registers: 17-64
+// nodes one wave per job, 9-16 nodes 16 lanes per job, and at most 8 nodes one
+// lane per job with the same steps done sequentially.  Every form performs the
+// same FP64 operations in the same order.
 // Child jobs, output nodes and leaf lists are appended with atomics; the tree
 // *structure* and every triangle range are fixed by the parent, so the result
 // is deterministic.  A final pass lays the tree out as the binary traversal
@@ -27,13 +30,20 @@
 // std::pow so they equal the oracle's bit for bit.
 #include <cmath>
 
+#include "onesweep.hpp"
 #include "phr.hpp"
 #include "scan.hpp"
 #include "wide.hpp"
 
 namespace phr {
 
-constexpr int kSmallCut = 8;  // jobs with cuts this small run one per lane
+// Job lists by cut size: one lane per job, 16 lanes per job, one wave per job
+// held in registers, and one wave per job staged through memory.
+constexpr int kSmallCut = 8;
+constexpr int kGroupCut = 16;
+constexpr int kWaveCut = 64;
+constexpr int kLists = 4;
+constexpr int kLarge = 3;
 constexpr int kSortLds = 4096;  // cuts of 65 .. this size sort in LDS (bitonic); larger ones by ranking
 constexpr int kMaxDepth = 1024;
 
@@ -53,6 +63,16 @@ struct Task {
   int32_t level;     // depth of the output node
 };
 
+// The list a job of k cut nodes goes to.
+__device__ inline int route(uint32_t k) {
+  return k <= uint32_t(kSmallCut) ? 0 : k <= uint32_t(kGroupCut) ? 1 : k <= uint32_t(kWaveCut) ? 2 : kLarge;
+}
+
+struct Lists {  // the next level's job lists
+  Task* p[kLists];
+  int32_t cap[kLists];
+};
+
 // Level counters.  Every job of a level bumps several of them with device-scope
 // atomics; each sits on its own 128-byte line so that they do not serialise
 // behind one another.
@@ -60,7 +80,6 @@ struct Task {
   uint32_t name;              \
   uint32_t name##_pad[31]
 struct Ctl {
-  TRACE_CTL_FIELD(n_tasks);    // jobs appended for the next level
   TRACE_CTL_FIELD(pool);       // pool entries appended for the next level
   TRACE_CTL_FIELD(out);        // output nodes allocated
   TRACE_CTL_FIELD(list);       // leaf-list entries allocated
@@ -68,8 +87,8 @@ struct Ctl {
   TRACE_CTL_FIELD(n_leaves);
   TRACE_CTL_FIELD(err);        // 1 capacity, 2 depth table
   TRACE_CTL_FIELD(cut);        // initial cut size
-  TRACE_CTL_FIELD(max_m);      // largest cut among the jobs appended for the next level
-  TRACE_CTL_FIELD(n_small);    // small jobs (cuts of at most kSmallCut) appended for the next level
+  TRACE_CTL_FIELD(max_m);      // largest cut among the large jobs appended for the next level
+  uint32_t jobs[kLists][32];   // jobs appended for the next level, per list (element 0)
 };
 #undef TRACE_CTL_FIELD
 
@@ -225,7 +244,7 @@ __global__ __launch_bounds__(1024) void initial_cut_kernel(Flat f, double th0, i
     t.level = 0;
     tasks[0] = t;
     ctl->cut = uint32_t(s_cut);
-    ctl->n_tasks = 1;
+    ctl->jobs[kLarge][0] = 1;
     ctl->out = 1;  // output node 0 is the root
   }
 }
@@ -490,15 +509,15 @@ __device__ inline void write_out(OutNode* out, int32_t o, int kind, int nchild, 
 
 struct Caps {
   int64_t pool;
-  int32_t out, list, tasks;
+  int32_t out, list;
 };
 
 // Emits one refined half [a, b) of order C: a leaf when a single node
 // survives, otherwise a job of the next level.
 __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, int a, int b,
                           double th, const WBox& hb, int32_t prim_off, int32_t prim_cnt,
-                          const Task& t, int32_t o, int32_t* pool_next, Task* tasks_next,
-                          Task* small_next, OutNode* out, int32_t* list, Ctl* ctl, const Caps& caps) {
+                          const Task& t, int32_t o, int32_t* pool_next, const Lists& next, OutNode* out,
+                          int32_t* list, Ctl* ctl, const Caps& caps) {
   const int lane = threadIdx.x & 63;
   uint32_t k_total = 0;
   for (int r = a + lane; r < b; r += 64) {
@@ -522,15 +541,15 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
     }
     return;
   }
-  const bool small = k_total <= uint32_t(kSmallCut);
+  const int dst = route(k_total);
   int32_t slot = 0, poff = 0;
   if (lane == 0) {
-    slot = int32_t(atomicAdd(small ? &ctl->n_small : &ctl->n_tasks, 1u));
+    slot = int32_t(atomicAdd(&ctl->jobs[dst][0], 1u));
     poff = int32_t(atomicAdd(&ctl->pool, k_total));
   }
   slot = __shfl(slot, 0, 64);
   poff = __shfl(poff, 0, 64);
-  if (slot >= caps.tasks || int64_t(poff) + k_total > caps.pool) {
+  if (slot >= next.cap[dst] || int64_t(poff) + k_total > caps.pool) {
     if (lane == 0) atomicOr(&ctl->err, 1u);
     return;
   }
@@ -556,7 +575,7 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
     carry += tot;
   }
   if (lane == 0) {
-    if (!small) raise_max(&ctl->max_m, k_total);
+    if (dst == kLarge) raise_max(&ctl->max_m, k_total);
     Task n;
     n.off = poff;
     n.cnt = int32_t(k_total);
@@ -565,7 +584,7 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
     n.out = o;
     n.prim_off = prim_off;
     n.level = t.level + 1;
-    (small ? small_next : tasks_next)[slot] = n;
+    next.p[dst][slot] = n;
   }
 }
 
@@ -574,9 +593,7 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
                                                   const Task* __restrict__ tasks,
                                                   const int32_t* __restrict__ pool,
                                                   int32_t* __restrict__ pool_next,
-                                                  Task* __restrict__ tasks_next,
-                                                  Task* __restrict__ small_next,
-                                                  int32_t* __restrict__ ordbuf, double* gkey,
+                                                  Lists next, int32_t* __restrict__ ordbuf, double* gkey,
                                                   double* rcost_buf, OutNode* __restrict__ out,
                                                   int32_t* __restrict__ list, Ctl* ctl, Caps caps,
                                                   int lds_p) {
@@ -639,10 +656,10 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
     if (lane == 0)
       write_out(out, t.out, 0, 2, o, o + 1, -1, 0, t.prim_off, int32_t(pl + pr), t.level, t.box);
     const double thd = th[t.depth];  // refined() uses the split cut's depth
-    emit_half(f, ids, C, 0, s.at, thd, L, t.prim_off, int32_t(pl), t, o, pool_next, tasks_next,
-              small_next, out, list, ctl, caps);
-    emit_half(f, ids, C, s.at, m, thd, R, t.prim_off + int32_t(pl), int32_t(pr), t, o + 1,
-              pool_next, tasks_next, small_next, out, list, ctl, caps);
+    emit_half(f, ids, C, 0, s.at, thd, L, t.prim_off, int32_t(pl), t, o, pool_next, next, out, list, ctl,
+              caps);
+    emit_half(f, ids, C, s.at, m, thd, R, t.prim_off + int32_t(pl), int32_t(pr), t, o + 1, pool_next, next,
+              out, list, ctl, caps);
   } else {
     // no split beats the whole cut: one leaf of every triangle, in order C
     uint32_t pc = 0;
@@ -770,9 +787,7 @@ __device__ inline uint32_t wave_alloc(uint32_t* counter, uint32_t n) {
 __global__ __launch_bounds__(256) void small_task_kernel(Flat f, const double* __restrict__ th,
                                                          const Task* __restrict__ tasks, int32_t n_jobs,
                                                          const int32_t* __restrict__ pool,
-                                                         int32_t* __restrict__ pool_next,
-                                                         Task* __restrict__ tasks_next,
-                                                         Task* __restrict__ small_next,
+                                                         int32_t* __restrict__ pool_next, Lists next,
                                                          OutNode* __restrict__ out, int32_t* __restrict__ list,
                                                          Ctl* ctl, Caps caps) {
   const int32_t job = blockIdx.x * 256 + int32_t(threadIdx.x);
@@ -844,24 +859,22 @@ __global__ __launch_bounds__(256) void small_task_kernel(Flat f, const double* _
     }
   }
   // one allocation per counter per wave
-  uint32_t n_pool = 0, n_sm = 0, n_lg = 0, max_lg = 0, max_lvl = 0;
+  uint32_t n_pool = 0, n_job[kLists] = {0, 0, 0, 0}, max_lg = 0, max_lvl = 0;
   for (int h = 0; h < 2; ++h) {
     const uint32_t k = k_half[h];
     if (k >= 2) {
       n_pool += k;
-      if (k <= uint32_t(kSmallCut)) ++n_sm;
-      else {
-        ++n_lg;
-        max_lg = max(max_lg, k);
-      }
+      const int dst = route(k);
+      ++n_job[dst];
+      if (dst == kLarge) max_lg = max(max_lg, k);
     }
   }
   if (valid && (s.at == 0 || k_half[0] == 1 || k_half[1] == 1)) max_lvl = uint32_t(t.level + 1);
   const int32_t o = int32_t(wave_alloc(&ctl->out, n_out));
   const int32_t ls = int32_t(wave_alloc(&ctl->list, n_list));
   const int32_t po = int32_t(wave_alloc(&ctl->pool, n_pool));
-  const int32_t sl_small = int32_t(wave_alloc(&ctl->n_small, n_sm));
-  const int32_t sl_large = int32_t(wave_alloc(&ctl->n_tasks, n_lg));
+  int32_t slot[kLists];
+  for (int r = 0; r < kLists; ++r) slot[r] = int32_t(wave_alloc(&ctl->jobs[r][0], n_job[r]));
   max_lg = wave_max(max_lg);
   max_lvl = wave_max(max_lvl);
   if ((threadIdx.x & 63) == 0) {
@@ -869,8 +882,9 @@ __global__ __launch_bounds__(256) void small_task_kernel(Flat f, const double* _
     if (max_lvl) raise_max(&ctl->max_level, max_lvl);
   }
   if (!valid) return;
-  if (int64_t(o) + n_out > caps.out || int64_t(ls) + n_list > caps.list || int64_t(po) + n_pool > caps.pool ||
-      int64_t(sl_small) + n_sm > caps.tasks || int64_t(sl_large) + n_lg > caps.tasks) {
+  bool over = int64_t(o) + n_out > caps.out || int64_t(ls) + n_list > caps.list || int64_t(po) + n_pool > caps.pool;
+  for (int r = 0; r < kLists; ++r) over = over || int64_t(slot[r]) + n_job[r] > next.cap[r];
+  if (over) {
     atomicOr(&ctl->err, 1u);
     return;
   }
@@ -882,7 +896,7 @@ __global__ __launch_bounds__(256) void small_task_kernel(Flat f, const double* _
     return;
   }
   write_out(out, t.out, 0, 2, o, o + 1, -1, 0, t.prim_off, int32_t(pl + pr), t.level, t.box);
-  int32_t lnext = ls, pnext = po, snext = sl_small, gnext = sl_large;
+  int32_t lnext = ls, pnext = po;
   for (int h = 0; h < 2; ++h) {
     const int a = h == 0 ? 0 : s.at, b = h == 0 ? s.at : m;
     const float* hb = h == 0 ? L : R;
@@ -913,9 +927,318 @@ __global__ __launch_bounds__(256) void small_task_kernel(Flat f, const double* _
     n.out = o + h;
     n.prim_off = poff_prim;
     n.level = t.level + 1;
-    if (k <= uint32_t(kSmallCut)) small_next[snext++] = n;
-    else tasks_next[gnext++] = n;
+    const int dst = route(k);
+    next.p[dst][slot[dst]++] = n;
     pnext += int32_t(k);
+  }
+}
+
+// ---- register-resident jobs: G lanes per job -------------------------------------
+// Jobs with cuts of kSmallCut+1 .. G nodes: each lane of a group of G holds one
+// member (node id, box, subtree size, triangle count), loaded once.  The sorts
+// are bitonic networks over the group's lanes on (centre, position) pairs
+// followed by one gather of the records, the minCost sweeps are lane scans,
+// and outputs are allocated with one atomic per counter per wave -- the same
+// FP64 operations as the staged wave form, with no memory round trips between
+// the steps.
+
+struct Mem {
+  int32_t node;
+  float lo[3], hi[3];
+  uint32_t size, prims;
+};
+
+__device__ inline Mem mem_shfl(const Mem& a, int src) {
+  Mem r;
+  r.node = __shfl(a.node, src, 64);
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = __shfl(a.lo[k], src, 64);
+    r.hi[k] = __shfl(a.hi[k], src, 64);
+  }
+  r.size = __shfl(a.size, src, 64);
+  r.prims = __shfl(a.prims, src, 64);
+  return r;
+}
+
+__device__ inline WBox wb_shfl_down(const WBox& b, int o) {
+  WBox r;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = __shfl_down(b.lo[k], o, 64);
+    r.hi[k] = __shfl_down(b.hi[k], o, 64);
+  }
+  r.cnt = __shfl_down(b.cnt, o, 64);
+  return r;
+}
+
+// sort.SliceStable of the group's members by box centre on `axis` (lanes at
+// or beyond m carry +inf and stay behind the members).
+template <int G>
+__device__ inline Mem group_sort(const Mem& a, int g, int gbase, int m, int axis) {
+  double key = g < m ? (double(a.lo[axis]) + double(a.hi[axis])) * (1.0 / 2.0) : INFINITY;
+  int32_t pos = g;
+  for (int k = 2; k <= G; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const double ok = __shfl_xor(key, j, 64);
+      const int32_t op = __shfl_xor(pos, j, 64);
+      const bool other_less = ok < key || (ok == key && op < pos);
+      const bool keep_min = ((g & j) == 0) == ((g & k) == 0);
+      if (keep_min == other_less) {
+        key = ok;
+        pos = op;
+      }
+    }
+  }
+  return mem_shfl(a, gbase + pos);
+}
+
+template <int G>
+__device__ inline WBox group_reduce(WBox v) {
+  for (int o = G / 2; o > 0; o >>= 1) wb_merge(v, wb_shfl_xor(v, o));
+  return v;
+}
+
+template <int G>
+__device__ inline uint32_t group_sum(uint32_t v) {
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// minCost (phr.go:414-459) of the group's members in lane order, as min_cost
+// computes it.
+template <int G>
+__device__ inline Split group_cost(const Mem& a, int g, int gbase, int m, const float* cut_box) {
+#pragma clang fp contract(off)
+  WBox v;
+  wb_empty(v);
+  if (g < m) {
+    for (int k = 0; k < 3; ++k) {
+      v.lo[k] = a.lo[k];
+      v.hi[k] = a.hi[k];
+    }
+    v.cnt = a.size;
+  }
+  WBox R = v;  // union of [g, m)
+  for (int o = 1; o < G; o <<= 1) {
+    const WBox u = wb_shfl_down(R, o);
+    if (g + o < G) wb_merge(R, u);
+  }
+  WBox I = v;  // union of [0, g]
+  for (int o = 1; o < G; o <<= 1) {
+    const WBox u = wb_shfl_up(I, o);
+    if (g >= o) wb_merge(I, u);
+  }
+  WBox P = wb_shfl_up(I, 1);  // union of [0, g)
+  if (g == 0) wb_empty(P);
+  double best = INFINITY;
+  int32_t best_at = m;
+  if (g >= 1 && g < m) {
+    const double rc = sa_wb(R) * double(R.cnt);
+    best = rc + sa_wb(P) * double(P.cnt);
+    best_at = g;
+  }
+  for (int o = G / 2; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const int32_t oa = __shfl_xor(best_at, o, 64);
+    if (ob < best || (ob == best && oa < best_at)) {
+      best = ob;
+      best_at = oa;
+    }
+  }
+  const uint32_t total = __shfl(R.cnt, gbase, 64);
+  const double nosplit = sa_box(cut_box) * double(total);
+  if (best < nosplit) return Split{best, best_at};
+  return Split{nosplit, 0};
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void group_task_kernel(Flat f, const double* __restrict__ th,
+                                                         const Task* __restrict__ tasks, int32_t n_jobs,
+                                                         const int32_t* __restrict__ pool,
+                                                         int32_t* __restrict__ pool_next, Lists next,
+                                                         OutNode* __restrict__ out, int32_t* __restrict__ list,
+                                                         Ctl* ctl, Caps caps) {
+  static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "groups are powers of two within a wave");
+  const int lane = threadIdx.x & 63;
+  const int g = lane & (G - 1);
+  const int gbase = lane - g;
+  const bool lead = g == 0;
+  const int32_t job = int32_t((blockIdx.x * 256u + threadIdx.x) / G);
+  // every lane of a group reads the same job, so control flow stays uniform
+  // per group; all cross-lane steps below run with the whole wave active
+  bool valid = job < n_jobs;
+  Task t{};
+  int m = 0;
+  if (valid) {
+    t = tasks[job];
+    m = t.cnt;
+    if (m < 2 || m > G || t.depth >= kMaxDepth) {
+      if (lead) atomicOr(&ctl->err, 2u);
+      valid = false;
+      m = 0;
+    }
+  }
+  Mem a;
+  a.node = -1;
+  for (int k = 0; k < 3; ++k) {
+    a.lo[k] = INFINITY;
+    a.hi[k] = -INFINITY;
+  }
+  a.size = 0;
+  a.prims = 0;
+  if (g < m) {
+    a.node = pool[t.off + g];
+    const float* b = f.box + 6 * size_t(a.node);
+    for (int k = 0; k < 3; ++k) {
+      a.lo[k] = b[k];
+      a.hi[k] = b[3 + k];
+    }
+    a.size = uint32_t(f.size[a.node]);
+    a.prims = uint32_t(f.prim[2 * a.node + 1] - f.prim[2 * a.node]);
+  }
+
+  // SweepSAH (phr.go:368-405)
+  const Mem X = group_sort<G>(a, g, gbase, m, 0);
+  const Mem Y = group_sort<G>(X, g, gbase, m, 1);
+  const Split sx = group_cost<G>(X, g, gbase, m, t.box);
+  const Split sy = group_cost<G>(Y, g, gbase, m, t.box);
+  const bool x_first = sx.cost < sy.cost;
+  const Mem Z = group_sort<G>(x_first ? Y : X, g, gbase, m, 2);
+  const Split sz = group_cost<G>(Z, g, gbase, m, t.box);
+  Mem c;
+  Split s;
+  if (x_first) {
+    if (sx.cost < sz.cost) { c = X; s = sx; } else { c = Z; s = sz; }
+  } else {
+    if (sy.cost < sz.cost) { c = Y; s = sy; } else { c = Z; s = sz; }
+  }
+
+  // refined() (phr.go:313-338) with the split cut's depth
+  const bool member = g < m;
+  const bool left = member && g < s.at;
+  const bool right = member && s.at > 0 && g >= s.at;
+  uint32_t k = 0;
+  bool keep = true;
+  if (s.at > 0 && member) {
+    const double thd = th[t.depth];
+    const float bb[6] = {c.lo[0], c.lo[1], c.lo[2], c.hi[0], c.hi[1], c.hi[2]};
+    keep = f.child[2 * c.node] < 0 || sa_box(bb) < thd;
+    k = keep ? 1u : 2u;
+  }
+  const uint32_t kL = group_sum<G>(left ? k : 0u);
+  const uint32_t kR = group_sum<G>(right ? k : 0u);
+  const uint32_t pl = group_sum<G>(left ? c.prims : 0u);
+  const uint32_t pr = group_sum<G>(right ? c.prims : 0u);
+  const uint32_t pall = group_sum<G>(member ? c.prims : 0u);
+  WBox hv;
+  wb_empty(hv);
+  WBox hw = hv;
+  if (left || right) {
+    WBox& dst = left ? hv : hw;
+    for (int q = 0; q < 3; ++q) {
+      dst.lo[q] = c.lo[q];
+      dst.hi[q] = c.hi[q];
+    }
+  }
+  const WBox Lb = group_reduce<G>(hv);
+  const WBox Rb = group_reduce<G>(hw);
+  uint32_t kx = k;  // exclusive scan of k over the group
+  for (int o = 1; o < G; o <<= 1) {
+    const uint32_t y = __shfl_up(kx, o, 64);
+    if (g >= o) kx += y;
+  }
+  kx -= k;
+  const int32_t first_left = __shfl(c.node, gbase, 64);
+  const int32_t first_right = __shfl(c.node, gbase + (s.at > 0 ? s.at : 0), 64);
+
+  // what the job emits; the group leader's needs are allocated per wave
+  uint32_t n_out = 0, n_list = 0, n_pool = 0, n_job[kLists] = {0, 0, 0, 0}, max_lg = 0, max_lvl = 0;
+  if (valid) {
+    if (s.at > 0) {
+      n_out = 2;
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t kh = h ? kR : kL;
+        if (kh == 1) {
+          ++n_list;
+          max_lvl = uint32_t(t.level + 1);
+        } else {
+          n_pool += kh;
+          const int dst = route(kh);
+          ++n_job[dst];
+          if (dst == kLarge) max_lg = max(max_lg, kh);
+        }
+      }
+    } else {
+      n_out = 1;
+      n_list = uint32_t(m);
+      max_lvl = uint32_t(t.level + 1);
+    }
+  }
+  const int32_t o = __shfl(int32_t(wave_alloc(&ctl->out, lead ? n_out : 0u)), gbase, 64);
+  const int32_t ls = __shfl(int32_t(wave_alloc(&ctl->list, lead ? n_list : 0u)), gbase, 64);
+  const int32_t po = __shfl(int32_t(wave_alloc(&ctl->pool, lead ? n_pool : 0u)), gbase, 64);
+  int32_t slot[kLists];
+  for (int r = 0; r < kLists; ++r)
+    slot[r] = __shfl(int32_t(wave_alloc(&ctl->jobs[r][0], lead ? n_job[r] : 0u)), gbase, 64);
+  max_lg = wave_max(lead ? max_lg : 0u);
+  max_lvl = wave_max(lead ? max_lvl : 0u);
+  if (lane == 0) {
+    if (max_lg) raise_max(&ctl->max_m, max_lg);
+    if (max_lvl) raise_max(&ctl->max_level, max_lvl);
+  }
+  if (!valid) return;
+  bool over = int64_t(o) + n_out > caps.out || int64_t(ls) + n_list > caps.list || int64_t(po) + n_pool > caps.pool;
+  for (int r = 0; r < kLists; ++r) over = over || int64_t(slot[r]) + n_job[r] > next.cap[r];
+  if (over) {
+    if (lead) atomicOr(&ctl->err, 1u);
+    return;
+  }
+  if (s.at == 0) {
+    // no split beats the whole cut: one leaf of every triangle, in sorted order
+    if (member) list[ls + g] = c.node;
+    if (lead) {
+      write_out(out, o, 1, 0, -1, -1, ls, m, t.prim_off, int32_t(pall), t.level + 1, t.box);
+      write_out(out, t.out, 0, 1, o, -1, -1, 0, t.prim_off, int32_t(pall), t.level, t.box);
+    }
+    return;
+  }
+  if (lead) write_out(out, t.out, 0, 2, o, o + 1, -1, 0, t.prim_off, int32_t(pl + pr), t.level, t.box);
+  int32_t lnext = ls, pnext = po;
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t kh = h ? kR : kL;
+    const WBox& hb = h ? Rb : Lb;
+    const float box[6] = {hb.lo[0], hb.lo[1], hb.lo[2], hb.hi[0], hb.hi[1], hb.hi[2]};
+    const int32_t poff_prim = h ? t.prim_off + int32_t(pl) : t.prim_off;
+    const int32_t pcnt = int32_t(h ? pr : pl);
+    if (kh == 1) {  // one kept node: a leaf of all its triangles
+      if (lead) {
+        list[lnext] = h ? first_right : first_left;
+        write_out(out, o + h, 1, 0, -1, -1, lnext, 1, poff_prim, pcnt, t.level + 1, box);
+      }
+      ++lnext;
+      continue;
+    }
+    if (h ? right : left) {
+      const int32_t p = pnext + int32_t(kx - (h ? kL : 0u));
+      if (keep) {
+        pool_next[p] = c.node;
+      } else {
+        pool_next[p] = f.child[2 * c.node];
+        pool_next[p + 1] = f.child[2 * c.node + 1];
+      }
+    }
+    if (lead) {
+      Task n;
+      n.off = pnext;
+      n.cnt = int32_t(kh);
+      for (int q = 0; q < 6; ++q) n.box[q] = box[q];
+      n.depth = t.depth + 1;
+      n.out = o + h;
+      n.prim_off = poff_prim;
+      n.level = t.level + 1;
+      const int dst = route(kh);
+      next.p[dst][slot[dst]++] = n;
+    }
+    pnext += int32_t(kh);
   }
 }
 
@@ -932,8 +1255,7 @@ __global__ __launch_bounds__(256) void count_leaves_kernel(const OutNode* __rest
 }
 
 __global__ void reset_level_kernel(Ctl* ctl) {
-  ctl->n_tasks = 0;
-  ctl->n_small = 0;
+  for (int r = 0; r < kLists; ++r) ctl->jobs[r][0] = 0;
   ctl->pool = 0;
   ctl->max_m = 0;
 }
@@ -965,6 +1287,27 @@ __global__ void trav_nodes_kernel(const OutNode* __restrict__ out, int32_t n,
     for (int a = 0; a < 6; ++a) boxes[6 * c + a] = out[k].box[a];
   }
   nodes[tindex[o]] = nd;
+}
+
+// Canonical numbering of the traversal nodes.  Jobs allocate output nodes with
+// atomics, so output indices vary from run to run; the two-child branches are
+// numbered instead in (level, first triangle) order, which is unique per node
+// (the nodes of one level cover disjoint, non-empty triangle ranges) and puts
+// the root first.  The tree's layout, and every sum taken over its nodes in
+// index order, are then the same on every run.
+__global__ void canon_keys_kernel(const OutNode* __restrict__ out, int32_t n, const uint32_t* __restrict__ flag,
+                                  const uint32_t* __restrict__ tindex, int prim_bits, uint64_t* __restrict__ keys,
+                                  uint32_t* __restrict__ vals) {
+  const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n || !flag[o]) return;
+  const uint32_t i = tindex[o];
+  keys[i] = (uint64_t(uint32_t(out[o].level)) << prim_bits) | uint64_t(uint32_t(out[o].prim_off));
+  vals[i] = uint32_t(o);
+}
+
+__global__ void canon_rank_kernel(const uint32_t* __restrict__ sorted, int32_t n_trav, uint32_t* __restrict__ tindex) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_trav) tindex[sorted[r]] = uint32_t(r);
 }
 
 // One wave per output leaf: copy its LBVH triangle ranges into PHR order.
@@ -1092,7 +1435,11 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const int64_t pool_cap = F + 64;
   const int32_t out_cap = int32_t(std::min<int64_t>(3 * n + 64, (int64_t(1) << 31) - 1));
   const int32_t list_cap = int32_t(F + 64);
-  const int32_t task_cap = int32_t(n + 64);
+  // a job of list r owns at least kMinCut[r] entries of the next pool, which
+  // bounds each list by the pool's capacity
+  const int32_t kMinCut[kLists] = {2, kSmallCut + 1, kGroupCut + 1, kWaveCut + 1};
+  int32_t list_caps[kLists];
+  for (int r = 0; r < kLists; ++r) list_caps[r] = int32_t(pool_cap / kMinCut[r] + 64);
 
   ScratchPlan plan;
   const size_t o_fbox = plan.take(F * 6 * sizeof(float));
@@ -1104,10 +1451,9 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const size_t o_fb = plan.take(fcap * sizeof(int32_t));
   const size_t o_pa = plan.take(pool_cap * sizeof(int32_t));
   const size_t o_pb = plan.take(pool_cap * sizeof(int32_t));
-  const size_t o_ta = plan.take(task_cap * sizeof(Task));
-  const size_t o_tb = plan.take(task_cap * sizeof(Task));
-  const size_t o_sa = plan.take(task_cap * sizeof(Task));
-  const size_t o_sb = plan.take(task_cap * sizeof(Task));
+  size_t o_jobs[2][kLists];
+  for (int b = 0; b < 2; ++b)
+    for (int r = 0; r < kLists; ++r) o_jobs[b][r] = plan.take(list_caps[r] * sizeof(Task));
   const size_t o_ord = plan.take(3 * pool_cap * sizeof(int32_t));
   const size_t o_key = plan.take(pool_cap * sizeof(double));
   const size_t o_rc = plan.take(pool_cap * sizeof(double));
@@ -1117,6 +1463,8 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const size_t o_tidx = plan.take(out_cap * sizeof(uint32_t));
   const size_t o_scan = plan.take(scan::scratch_words(out_cap) * sizeof(uint32_t));
   const size_t o_ntr = plan.take(64);
+  // status words of the canonical-order sort (at most n - 1 branches, 8 passes)
+  const size_t o_cst = plan.take(onesweep::status_words(std::max<int64_t>(n, 1), 8) * sizeof(uint32_t));
   int rc = ctx_reserve_scratch(ctx, plan.off);
   if (rc) return rc;
   char* base = static_cast<char*>(ctx->scratch);
@@ -1129,8 +1477,9 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   auto* fa = reinterpret_cast<int32_t*>(base + o_fa);
   auto* fb = reinterpret_cast<int32_t*>(base + o_fb);
   int32_t* pools[2] = {reinterpret_cast<int32_t*>(base + o_pa), reinterpret_cast<int32_t*>(base + o_pb)};
-  Task* tasks[2] = {reinterpret_cast<Task*>(base + o_ta), reinterpret_cast<Task*>(base + o_tb)};
-  Task* small[2] = {reinterpret_cast<Task*>(base + o_sa), reinterpret_cast<Task*>(base + o_sb)};
+  Task* jobs[2][kLists];
+  for (int b = 0; b < 2; ++b)
+    for (int r = 0; r < kLists; ++r) jobs[b][r] = reinterpret_cast<Task*>(base + o_jobs[b][r]);
   auto* ord = reinterpret_cast<int32_t*>(base + o_ord);
   auto* gkey = reinterpret_cast<double*>(base + o_key);
   auto* rcost = reinterpret_cast<double*>(base + o_rc);
@@ -1150,7 +1499,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
                                                     aux->inner_range.p, aux->run_start.p,
                                                     aux->header.p, f);
   initial_cut_kernel<<<1, 1024, 0, st>>>(f, th[0], max_cut, fa, fb, int32_t(fcap), pools[0],
-                                         int32_t(pool_cap), tasks[0], ctl, aux->header.p);
+                                         int32_t(pool_cap), jobs[0][kLarge], ctl, aux->header.p);
   Ctl hc;
   rc = read_ctl(ctx, ctl, &hc);
   if (rc) return rc;
@@ -1158,9 +1507,8 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   out->phr_initial_cut = hc.cut;
   if (hc.cut <= 1) return clone_lbvh(ctx, aux, out);  // the whole LBVH is attached as is
 
-  const Caps caps{pool_cap, out_cap, list_cap, task_cap};
-  uint32_t n_tasks = 1;  // wave-per-job list (the initial cut's job is on it)
-  uint32_t n_small = 0;  // lane-per-job list
+  const Caps caps{pool_cap, out_cap, list_cap};
+  uint32_t n_jobs[kLists] = {0, 0, 0, 1};  // the initial cut's job is on the large list
   uint32_t max_m = hc.cut;  // the one job of the first level holds the initial cut
   // Every job holds a cut of at least two members and makes one split attempt
   // (branching 2), so BuildWithCost's counter (phr.go:221) gains the level's
@@ -1168,7 +1516,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   uint64_t level_pool = hc.cut;
   int cur = 0;
   int levels = 0;
-  while (n_tasks + n_small > 0) {
+  while (n_jobs[0] + n_jobs[1] + n_jobs[2] + n_jobs[3] > 0) {
     // LDS for the bitonic sort, sized to the level's largest cut
     // (cuts of up to 64 sort in registers; larger ones up to kSortLds in LDS,
     // which every workgroup of the level then reserves)
@@ -1180,19 +1528,28 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     const size_t lds_bytes = size_t(lds_p) * (sizeof(double) + sizeof(int32_t));
     out->phr_split_work += int64_t(level_pool);
     reset_level_kernel<<<1, 1, 0, st>>>(ctl);
-    if (n_tasks > 0)
-      task_kernel<<<int(n_tasks), 64, lds_bytes, st>>>(f, d_th, tasks[cur], pools[cur], pools[cur ^ 1],
-                                                      tasks[cur ^ 1], small[cur ^ 1], ord, gkey, rcost,
-                                                      outn, list, ctl, caps, lds_p);
-    if (n_small > 0)
-      small_task_kernel<<<int(ceil_div(n_small, 256)), 256, 0, st>>>(
-          f, d_th, small[cur], int32_t(n_small), pools[cur], pools[cur ^ 1], tasks[cur ^ 1], small[cur ^ 1],
-          outn, list, ctl, caps);
+    Lists nx;
+    for (int r = 0; r < kLists; ++r) {
+      nx.p[r] = jobs[cur ^ 1][r];
+      nx.cap[r] = list_caps[r];
+    }
+    if (n_jobs[kLarge] > 0)
+      task_kernel<<<int(n_jobs[kLarge]), 64, lds_bytes, st>>>(f, d_th, jobs[cur][kLarge], pools[cur],
+                                                             pools[cur ^ 1], nx, ord, gkey, rcost, outn, list,
+                                                             ctl, caps, lds_p);
+    if (n_jobs[2] > 0)
+      group_task_kernel<kWaveCut><<<int(ceil_div(int64_t(n_jobs[2]) * kWaveCut, 256)), 256, 0, st>>>(
+          f, d_th, jobs[cur][2], int32_t(n_jobs[2]), pools[cur], pools[cur ^ 1], nx, outn, list, ctl, caps);
+    if (n_jobs[1] > 0)
+      group_task_kernel<kGroupCut><<<int(ceil_div(int64_t(n_jobs[1]) * kGroupCut, 256)), 256, 0, st>>>(
+          f, d_th, jobs[cur][1], int32_t(n_jobs[1]), pools[cur], pools[cur ^ 1], nx, outn, list, ctl, caps);
+    if (n_jobs[0] > 0)
+      small_task_kernel<<<int(ceil_div(n_jobs[0], 256)), 256, 0, st>>>(
+          f, d_th, jobs[cur][0], int32_t(n_jobs[0]), pools[cur], pools[cur ^ 1], nx, outn, list, ctl, caps);
     rc = read_ctl(ctx, ctl, &hc);
     if (rc) return rc;
     if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x at level %d", hc.err, levels);
-    n_tasks = hc.n_tasks;
-    n_small = hc.n_small;
+    for (int r = 0; r < kLists; ++r) n_jobs[r] = hc.jobs[r][0];
     max_m = hc.max_m;
     level_pool = hc.pool;
     cur ^= 1;
@@ -1208,6 +1565,28 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   uint32_t h_ntrav = 0;
   HIP_TRY(ctx, hipMemcpyAsync(&h_ntrav, n_trav, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
+  if (h_ntrav > 1) {
+    // canonical order (the buffers of the refinement loop are free by now:
+    // branches number fewer than the LBVH leaves, so keys fit gkey / rcost
+    // and values the order buffer)
+    int prim_bits = 1, level_bits = 1;
+    while ((int64_t(1) << prim_bits) < n) ++prim_bits;
+    while ((int64_t(1) << level_bits) <= int64_t(hc.max_level)) ++level_bits;
+    const int passes = (prim_bits + level_bits + 7) / 8;
+    if (int64_t(h_ntrav) > pool_cap || prim_bits + level_bits > 64)
+      TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: layout buffers too small");
+    auto* keys_a = reinterpret_cast<uint64_t*>(gkey);
+    auto* keys_b = reinterpret_cast<uint64_t*>(rcost);
+    auto* vals_a = reinterpret_cast<uint32_t*>(ord);
+    auto* vals_b = reinterpret_cast<uint32_t*>(ord + pool_cap);
+    auto* status = reinterpret_cast<uint32_t*>(base + o_cst);
+    canon_keys_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, prim_bits, keys_a, vals_a);
+    const int where = onesweep::sort_pairs(st, keys_a, vals_a, keys_b, vals_b, h_ntrav, passes, status, ctx->d_err,
+                                           false);
+    canon_rank_kernel<<<int(ceil_div(h_ntrav, 256)), 256, 0, st>>>(where ? vals_b : vals_a, int32_t(h_ntrav), tidx);
+    rc = ctx_check_device_error(ctx, "trace_phr_build");
+    if (rc) return rc;
+  }
   HIP_TRY(ctx, out->nodes.alloc(std::max<uint32_t>(1, h_ntrav)));
   HIP_TRY(ctx, out->perm.alloc(n));
   HIP_TRY(ctx, out->phr_src.alloc(n));
