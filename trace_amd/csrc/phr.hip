@@ -343,41 +343,128 @@ __device__ inline void load_member(const Flat& f, int32_t node, WBox& b, bool wi
   b.cnt = with_size ? uint32_t(f.size[node]) : 1u;
 }
 
+// ---- block helpers: B threads (a multiple of 64) working on one job ----------
+// Each helper that crosses waves stages per-wave values in LDS and ends with a
+// barrier, so that calls can follow one another.
+struct BlockLds {
+  WBox box[16];
+  uint32_t u[16];
+  double d[16];
+  int32_t i[16];
+};
+
+// Inclusive and exclusive scans of boxes over thread order, and the total.
+template <int B>
+__device__ inline void block_scan_box(const WBox& x, BlockLds& s, WBox* incl, WBox* excl, WBox* total) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const WBox v = wb_scan(x);
+  WBox e = wb_shfl_up(v, 1);
+  if (lane == 0) wb_empty(e);
+  WBox pre, tot;
+  wb_empty(pre);
+  if constexpr (B == 64) {
+    tot = wb_shfl(v, 63);
+  } else {
+    if (lane == 63) s.box[w] = v;
+    __syncthreads();
+    wb_empty(tot);
+    for (int i = 0; i < B / 64; ++i) {
+      if (i < w) wb_merge(pre, s.box[i]);
+      wb_merge(tot, s.box[i]);
+    }
+    __syncthreads();
+  }
+  *incl = v;
+  wb_merge(*incl, pre);
+  *excl = e;
+  wb_merge(*excl, pre);
+  *total = tot;
+}
+
+template <int B>
+__device__ inline uint32_t block_excl_u32(uint32_t v, BlockLds& s, uint32_t* total) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  uint32_t wt;
+  const uint32_t ex = wave_excl(v, &wt);
+  if constexpr (B == 64) {
+    *total = wt;
+    return ex;
+  } else {
+    if (lane == 0) s.u[w] = wt;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int i = 0; i < B / 64; ++i) {
+      if (i < w) pre += s.u[i];
+      tot += s.u[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + ex;
+  }
+}
+
+template <int B>
+__device__ inline uint32_t block_sum_u32(uint32_t v, BlockLds& s) {
+  uint32_t t;
+  block_excl_u32<B>(v, s, &t);
+  return t;
+}
+
+template <int B>
+__device__ inline int32_t block_bcast(int32_t v, BlockLds& s) {
+  if constexpr (B == 64) {
+    return __shfl(v, 0, 64);
+  } else {
+    if (threadIdx.x == 0) s.i[0] = v;
+    __syncthreads();
+    const int32_t r = s.i[0];
+    __syncthreads();
+    return r;
+  }
+}
+
 // sort.SliceStable of the cut by box centre on `axis`, starting from order
 // `in` (identity when null).  Keys are exact FP64 centres.  Sorting the pairs
 // (centre, input position) makes every key distinct, so any correct sort gives
 // the stable order: a bitonic network in LDS (P = m rounded up to a power of
-// two, padded with +inf) for cuts that fit, rank by counting otherwise.
+// two, padded with +inf) for cuts that fit, rank by counting otherwise; a
+// single wave sorts cuts of at most 64 in registers.
+template <int B>
 __device__ void sort_axis(const Flat& f, const int32_t* ids, const int32_t* in, int32_t* out,
                           int axis, int m, double* keys, double* lkey, int32_t* lpos, int P) {
-  const int lane = threadIdx.x & 63;
-  if (m <= 64) {
-    // one pair per lane, bitonic network over lane shuffles
-    double key = INFINITY;
-    int32_t pos = lane;
-    if (lane < m) {
-      const int32_t j = in ? in[lane] : lane;
-      const float* b = f.box + 6 * size_t(ids[j]);
-      key = (double(b[axis]) + double(b[3 + axis])) * (1.0 / 2.0);
-    }
-    for (int k = 2; k <= 64; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const double ok = __shfl_xor(key, j, 64);
-        const int32_t op = __shfl_xor(pos, j, 64);
-        const bool other_less = ok < key || (ok == key && op < pos);
-        const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-        if (keep_min == other_less) {
-          key = ok;
-          pos = op;
+  const int tid = threadIdx.x;
+  if constexpr (B == 64) {
+    if (m <= 64) {
+      // one pair per lane, bitonic network over lane shuffles
+      const int lane = tid;
+      double key = INFINITY;
+      int32_t pos = lane;
+      if (lane < m) {
+        const int32_t j = in ? in[lane] : lane;
+        const float* b = f.box + 6 * size_t(ids[j]);
+        key = (double(b[axis]) + double(b[3 + axis])) * (1.0 / 2.0);
+      }
+      for (int k = 2; k <= 64; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          const double ok = __shfl_xor(key, j, 64);
+          const int32_t op = __shfl_xor(pos, j, 64);
+          const bool other_less = ok < key || (ok == key && op < pos);
+          const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+          if (keep_min == other_less) {
+            key = ok;
+            pos = op;
+          }
         }
       }
+      if (lane < m) out[lane] = in ? in[pos] : pos;
+      __syncthreads();
+      return;
     }
-    if (lane < m) out[lane] = in ? in[pos] : pos;
-    __syncthreads();
-    return;
   }
   if (P > 0) {
-    for (int r = lane; r < P; r += 64) {
+    for (int r = tid; r < P; r += B) {
       if (r < m) {
         const int32_t j = in ? in[r] : r;
         const float* b = f.box + 6 * size_t(ids[j]);
@@ -391,7 +478,7 @@ __device__ void sort_axis(const Flat& f, const int32_t* ids, const int32_t* in, 
     __syncthreads();
     for (int k = 2; k <= P; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int t = lane; t < (P >> 1); t += 64) {
+        for (int t = tid; t < (P >> 1); t += B) {
           const int i = 2 * t - (t & (j - 1));  // lower index of pair (i, i + j)
           const int l = i + j;
           const double ka = lkey[i], kb = lkey[l];
@@ -407,17 +494,17 @@ __device__ void sort_axis(const Flat& f, const int32_t* ids, const int32_t* in, 
         __syncthreads();
       }
     }
-    for (int r = lane; r < m; r += 64) out[r] = in ? in[lpos[r]] : lpos[r];
+    for (int r = tid; r < m; r += B) out[r] = in ? in[lpos[r]] : lpos[r];
     __syncthreads();
     return;
   }
-  for (int r = lane; r < m; r += 64) {
+  for (int r = tid; r < m; r += B) {
     const int32_t j = in ? in[r] : r;
     const float* b = f.box + 6 * size_t(ids[j]);
     keys[r] = (double(b[axis]) + double(b[3 + axis])) * (1.0 / 2.0);
   }
   __syncthreads();
-  for (int r = lane; r < m; r += 64) {
+  for (int r = tid; r < m; r += B) {
     const double kr = keys[r];
     int rank = 0;
     for (int q = 0; q < m; ++q) {
@@ -434,37 +521,42 @@ struct Split {
   int32_t at;  // 0 = no split
 };
 
-// minCost (phr.go:414-459) for the cut in order `ord`.
+// minCost (phr.go:414-459) for the cut in order `ord`: suffix and prefix
+// unions and subtree-size sums by block scans over chunks of B positions,
+// FP64 costs in the reference's operation order, the first minimum.
+template <int B>
 __device__ Split min_cost(const Flat& f, const int32_t* ids, const int32_t* ord, int m,
-                          const float* cut_box, double* rcost) {
+                          const float* cut_box, double* rcost, BlockLds& s) {
 #pragma clang fp contract(off)
-  const int lane = threadIdx.x & 63;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
   WBox carry;
   wb_empty(carry);
-  for (int end = m; end > 0; end -= 64) {
-    const int r = end - 1 - lane;
+  for (int end = m; end > 0; end -= B) {
+    const int r = end - 1 - tid;
     WBox v;
     wb_empty(v);
     if (r >= 0) load_member(f, ids[ord[r]], v, true);
-    v = wb_scan(v);  // lane l: positions [end - 1 - l, end)
-    WBox R = v;
+    WBox incl, excl, tot;
+    block_scan_box<B>(v, s, &incl, &excl, &tot);  // thread t: positions [end - 1 - t, end)
+    WBox R = incl;
     wb_merge(R, carry);
     if (r >= 1) rcost[r] = sa_wb(R) * double(R.cnt);
-    wb_merge(carry, wb_shfl(v, 63));
+    wb_merge(carry, tot);
   }
   __syncthreads();
   const uint32_t total = carry.cnt;
   wb_empty(carry);
   double best = INFINITY;
   int32_t best_at = m;
-  for (int start = 0; start < m; start += 64) {
-    const int r = start + lane;
+  for (int start = 0; start < m; start += B) {
+    const int r = start + tid;
     WBox v;
     wb_empty(v);
     if (r < m) load_member(f, ids[ord[r]], v, true);
-    const WBox incl = wb_scan(v);
-    WBox P = wb_shfl_up(incl, 1);
-    if (lane == 0) wb_empty(P);
+    WBox incl, excl, tot;
+    block_scan_box<B>(v, s, &incl, &excl, &tot);
+    WBox P = excl;
     wb_merge(P, carry);
     if (r >= 1 && r < m) {
       const double c = rcost[r] + sa_wb(P) * double(P.cnt);
@@ -473,7 +565,7 @@ __device__ Split min_cost(const Flat& f, const int32_t* ids, const int32_t* ord,
         best_at = r;
       }
     }
-    wb_merge(carry, wb_shfl(incl, 63));
+    wb_merge(carry, tot);
   }
   for (int o = 32; o > 0; o >>= 1) {
     const double ob = __shfl_xor(best, o, 64);
@@ -481,6 +573,22 @@ __device__ Split min_cost(const Flat& f, const int32_t* ids, const int32_t* ord,
     if (ob < best || (ob == best && oa < best_at)) {
       best = ob;
       best_at = oa;
+    }
+  }
+  if constexpr (B > 64) {
+    const int w = tid >> 6;
+    if (lane == 0) {
+      s.d[w] = best;
+      s.i[w] = best_at;
+    }
+    __syncthreads();
+    best = s.d[0];
+    best_at = s.i[0];
+    for (int i = 1; i < B / 64; ++i) {
+      if (s.d[i] < best || (s.d[i] == best && s.i[i] < best_at)) {
+        best = s.d[i];
+        best_at = s.i[i];
+      }
     }
   }
   __syncthreads();
@@ -514,23 +622,23 @@ struct Caps {
 
 // Emits one refined half [a, b) of order C: a leaf when a single node
 // survives, otherwise a job of the next level.
+template <int B>
 __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, int a, int b,
                           double th, const WBox& hb, int32_t prim_off, int32_t prim_cnt,
                           const Task& t, int32_t o, int32_t* pool_next, const Lists& next, OutNode* out,
-                          int32_t* list, Ctl* ctl, const Caps& caps) {
-  const int lane = threadIdx.x & 63;
+                          int32_t* list, Ctl* ctl, const Caps& caps, BlockLds& s) {
+  const int tid = threadIdx.x;
   uint32_t k_total = 0;
-  for (int r = a + lane; r < b; r += 64) {
+  for (int r = a + tid; r < b; r += B) {
     const int32_t node = ids[C[r]];
     const bool keep = f.child[2 * node] < 0 || sa_box(f.box + 6 * size_t(node)) < th;
     k_total += keep ? 1u : 2u;
   }
-  k_total = wave_sum(k_total);
+  k_total = block_sum_u32<B>(k_total, s);
   const float box[6] = {hb.lo[0], hb.lo[1], hb.lo[2], hb.hi[0], hb.hi[1], hb.hi[2]};
   if (k_total == 1) {  // one kept node: a leaf of all its triangles
-    int32_t ls = 0;
-    if (lane == 0) {
-      ls = int32_t(atomicAdd(&ctl->list, 1u));
+    if (tid == 0) {
+      const int32_t ls = int32_t(atomicAdd(&ctl->list, 1u));
       if (ls < caps.list) {
         list[ls] = ids[C[a]];
         write_out(out, o, 1, 0, -1, -1, ls, 1, prim_off, prim_cnt, t.level + 1, box);
@@ -543,19 +651,19 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
   }
   const int dst = route(k_total);
   int32_t slot = 0, poff = 0;
-  if (lane == 0) {
+  if (tid == 0) {
     slot = int32_t(atomicAdd(&ctl->jobs[dst][0], 1u));
     poff = int32_t(atomicAdd(&ctl->pool, k_total));
   }
-  slot = __shfl(slot, 0, 64);
-  poff = __shfl(poff, 0, 64);
+  slot = block_bcast<B>(slot, s);
+  poff = block_bcast<B>(poff, s);
   if (slot >= next.cap[dst] || int64_t(poff) + k_total > caps.pool) {
-    if (lane == 0) atomicOr(&ctl->err, 1u);
+    if (tid == 0) atomicOr(&ctl->err, 1u);
     return;
   }
   uint32_t carry = 0;
-  for (int r0 = a; r0 < b; r0 += 64) {
-    const int r = r0 + lane;
+  for (int r0 = a; r0 < b; r0 += B) {
+    const int r = r0 + tid;
     int32_t node = 0;
     uint32_t k = 0;
     if (r < b) {
@@ -564,7 +672,7 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
       k = keep ? 1u : 2u;
     }
     uint32_t tot;
-    const uint32_t ex = wave_excl(k, &tot);
+    const uint32_t ex = block_excl_u32<B>(k, s, &tot);
     const int32_t p = poff + int32_t(carry + ex);
     if (k == 1) {
       pool_next[p] = node;
@@ -574,7 +682,7 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
     }
     carry += tot;
   }
-  if (lane == 0) {
+  if (tid == 0) {
     if (dst == kLarge) raise_max(&ctl->max_m, k_total);
     Task n;
     n.off = poff;
@@ -588,17 +696,20 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
   }
 }
 
-// One wave per job of the current level (phr.go:92-152 with BranchingFactor 2).
-__global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restrict__ th,
-                                                  const Task* __restrict__ tasks,
-                                                  const int32_t* __restrict__ pool,
-                                                  int32_t* __restrict__ pool_next,
-                                                  Lists next, int32_t* __restrict__ ordbuf, double* gkey,
-                                                  double* rcost_buf, OutNode* __restrict__ out,
-                                                  int32_t* __restrict__ list, Ctl* ctl, Caps caps,
-                                                  int lds_p) {
+// One workgroup of B threads per job of the current level (phr.go:92-152
+// with BranchingFactor 2).
+template <int B>
+__global__ __launch_bounds__(B) void task_kernel(Flat f, const double* __restrict__ th,
+                                                 const Task* __restrict__ tasks,
+                                                 const int32_t* __restrict__ pool,
+                                                 int32_t* __restrict__ pool_next,
+                                                 Lists next, int32_t* __restrict__ ordbuf, double* gkey,
+                                                 double* rcost_buf, OutNode* __restrict__ out,
+                                                 int32_t* __restrict__ list, Ctl* ctl, Caps caps,
+                                                 int lds_p) {
   extern __shared__ double s_dyn[];  // P keys then P positions (P = lds_p, 0 = no LDS sort)
-  const int lane = threadIdx.x & 63;
+  __shared__ BlockLds s_blk;
+  const int tid = threadIdx.x;
   const Task t = tasks[blockIdx.x];
   const int m = t.cnt;
   const int P = m <= lds_p ? lds_p : 0;
@@ -611,17 +722,17 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
   double* keys = gkey + t.off;
   double* rc = rcost_buf + t.off;
   if (t.depth >= kMaxDepth) {
-    if (lane == 0) atomicOr(&ctl->err, 2u);
+    if (tid == 0) atomicOr(&ctl->err, 2u);
     return;
   }
 
   // SweepSAH (phr.go:368-405)
-  sort_axis(f, ids, nullptr, oX, 0, m, keys, lkey, lpos, P);
-  sort_axis(f, ids, oX, oY, 1, m, keys, lkey, lpos, P);
-  const Split x = min_cost(f, ids, oX, m, t.box, rc);
-  const Split y = min_cost(f, ids, oY, m, t.box, rc);
-  sort_axis(f, ids, x.cost < y.cost ? oY : oX, oZ, 2, m, keys, lkey, lpos, P);
-  const Split z = min_cost(f, ids, oZ, m, t.box, rc);
+  sort_axis<B>(f, ids, nullptr, oX, 0, m, keys, lkey, lpos, P);
+  sort_axis<B>(f, ids, oX, oY, 1, m, keys, lkey, lpos, P);
+  const Split x = min_cost<B>(f, ids, oX, m, t.box, rc, s_blk);
+  const Split y = min_cost<B>(f, ids, oY, m, t.box, rc, s_blk);
+  sort_axis<B>(f, ids, x.cost < y.cost ? oY : oX, oZ, 2, m, keys, lkey, lpos, P);
+  const Split z = min_cost<B>(f, ids, oZ, m, t.box, rc, s_blk);
   const int32_t* C;
   Split s;
   if (x.cost < y.cost) {
@@ -635,52 +746,53 @@ __global__ __launch_bounds__(64) void task_kernel(Flat f, const double* __restri
     wb_empty(L);
     wb_empty(R);
     uint32_t pl = 0, pr = 0;
-    for (int r = lane; r < m; r += 64) {
+    for (int r = tid; r < m; r += B) {
       const int32_t node = ids[C[r]];
       WBox v;
       load_member(f, node, v, false);
       const uint32_t pc = uint32_t(f.prim[2 * node + 1] - f.prim[2 * node]);
       if (r < s.at) { wb_merge(L, v); pl += pc; } else { wb_merge(R, v); pr += pc; }
     }
-    L = wb_reduce(L);
-    R = wb_reduce(R);
-    pl = wave_sum(pl);
-    pr = wave_sum(pr);
+    WBox i_, e_, Lt, Rt;
+    block_scan_box<B>(L, s_blk, &i_, &e_, &Lt);
+    block_scan_box<B>(R, s_blk, &i_, &e_, &Rt);
+    pl = block_sum_u32<B>(pl, s_blk);
+    pr = block_sum_u32<B>(pr, s_blk);
     int32_t o = 0;
-    if (lane == 0) o = int32_t(atomicAdd(&ctl->out, 2u));
-    o = __shfl(o, 0, 64);
+    if (tid == 0) o = int32_t(atomicAdd(&ctl->out, 2u));
+    o = block_bcast<B>(o, s_blk);
     if (o + 1 >= caps.out) {
-      if (lane == 0) atomicOr(&ctl->err, 1u);
+      if (tid == 0) atomicOr(&ctl->err, 1u);
       return;
     }
-    if (lane == 0)
+    if (tid == 0)
       write_out(out, t.out, 0, 2, o, o + 1, -1, 0, t.prim_off, int32_t(pl + pr), t.level, t.box);
     const double thd = th[t.depth];  // refined() uses the split cut's depth
-    emit_half(f, ids, C, 0, s.at, thd, L, t.prim_off, int32_t(pl), t, o, pool_next, next, out, list, ctl,
-              caps);
-    emit_half(f, ids, C, s.at, m, thd, R, t.prim_off + int32_t(pl), int32_t(pr), t, o + 1, pool_next, next,
-              out, list, ctl, caps);
+    emit_half<B>(f, ids, C, 0, s.at, thd, Lt, t.prim_off, int32_t(pl), t, o, pool_next, next, out, list, ctl,
+                 caps, s_blk);
+    emit_half<B>(f, ids, C, s.at, m, thd, Rt, t.prim_off + int32_t(pl), int32_t(pr), t, o + 1, pool_next, next,
+                 out, list, ctl, caps, s_blk);
   } else {
     // no split beats the whole cut: one leaf of every triangle, in order C
     uint32_t pc = 0;
-    for (int r = lane; r < m; r += 64) {
+    for (int r = tid; r < m; r += B) {
       const int32_t node = ids[C[r]];
       pc += uint32_t(f.prim[2 * node + 1] - f.prim[2 * node]);
     }
-    pc = wave_sum(pc);
+    pc = block_sum_u32<B>(pc, s_blk);
     int32_t o = 0, ls = 0;
-    if (lane == 0) {
+    if (tid == 0) {
       o = int32_t(atomicAdd(&ctl->out, 1u));
       ls = int32_t(atomicAdd(&ctl->list, uint32_t(m)));
     }
-    o = __shfl(o, 0, 64);
-    ls = __shfl(ls, 0, 64);
+    o = block_bcast<B>(o, s_blk);
+    ls = block_bcast<B>(ls, s_blk);
     if (o >= caps.out || ls + m > caps.list) {
-      if (lane == 0) atomicOr(&ctl->err, 1u);
+      if (tid == 0) atomicOr(&ctl->err, 1u);
       return;
     }
-    for (int r = lane; r < m; r += 64) list[ls + r] = ids[C[r]];
-    if (lane == 0) {
+    for (int r = tid; r < m; r += B) list[ls + r] = ids[C[r]];
+    if (tid == 0) {
       write_out(out, o, 1, 0, -1, -1, ls, m, t.prim_off, int32_t(pc), t.level + 1, t.box);
       write_out(out, t.out, 0, 1, o, -1, -1, 0, t.prim_off, int32_t(pc), t.level, t.box);
       raise_max(&ctl->max_level, uint32_t(t.level + 1));
@@ -1310,16 +1422,21 @@ __global__ void canon_rank_kernel(const uint32_t* __restrict__ sorted, int32_t n
   if (r < n_trav) tindex[sorted[r]] = uint32_t(r);
 }
 
-// One wave per output leaf: copy its LBVH triangle ranges into PHR order.
-__global__ __launch_bounds__(64) void leaf_copy_kernel(
-    const OutNode* __restrict__ out, const int32_t* __restrict__ list, Flat f,
+// Eight lanes per output node (most leaves hold a few triangles; branches
+// have nothing to copy): copy a leaf's LBVH triangle ranges into PHR order.
+constexpr int kCopyLanes = 8;
+__global__ __launch_bounds__(256) void leaf_copy_kernel(
+    const OutNode* __restrict__ out, int32_t n_out, const int32_t* __restrict__ list, Flat f,
     const int32_t* __restrict__ perm_in, const float* __restrict__ tris_in,
     const float* __restrict__ box_in, int32_t* __restrict__ perm_out, int32_t* __restrict__ src_out,
     float* __restrict__ tris_out, float* __restrict__ box_out, int32_t n, Ctl* ctl) {
-  const OutNode nd = out[blockIdx.x];
+  const int32_t o = int32_t((blockIdx.x * 256u + threadIdx.x) / kCopyLanes);
+  const int g = int(threadIdx.x) & (kCopyLanes - 1);
+  if (o >= n_out) return;
+  const OutNode nd = out[o];
   if (nd.kind != 1) return;
   if (nd.prim_off < 0 || nd.prim_cnt < 0 || nd.prim_cnt > n - nd.prim_off) {
-    if (threadIdx.x == 0) atomicOr(&ctl->err, 4u);
+    if (g == 0) atomicOr(&ctl->err, 4u);
     return;
   }
   int32_t dst = nd.prim_off;
@@ -1328,10 +1445,10 @@ __global__ __launch_bounds__(64) void leaf_copy_kernel(
     const int32_t node = list[nd.list_off + e];
     const int32_t a = f.prim[2 * node], b = f.prim[2 * node + 1];
     if (b - a > end - dst) {  // the list must fill exactly [prim_off, end)
-      if (threadIdx.x == 0) atomicOr(&ctl->err, 4u);
+      if (g == 0) atomicOr(&ctl->err, 4u);
       return;
     }
-    for (int32_t k = a + int32_t(threadIdx.x); k < b; k += 64) {
+    for (int32_t k = a + g; k < b; k += kCopyLanes) {
       const int32_t d = dst + (k - a);
       perm_out[d] = perm_in[k];
       src_out[d] = k;
@@ -1533,10 +1650,22 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
       nx.p[r] = jobs[cur ^ 1][r];
       nx.cap[r] = list_caps[r];
     }
-    if (n_jobs[kLarge] > 0)
-      task_kernel<<<int(n_jobs[kLarge]), 64, lds_bytes, st>>>(f, d_th, jobs[cur][kLarge], pools[cur],
-                                                             pools[cur ^ 1], nx, ord, gkey, rcost, outn, list,
-                                                             ctl, caps, lds_p);
+    if (n_jobs[kLarge] > 0) {
+      // workgroup size from the level's largest cut: the top levels hold few
+      // jobs of thousands of nodes each
+      const int nb = int(n_jobs[kLarge]);
+      int bsz = 64;  // thousands of jobs fill the chip one wave each
+      if (max_m > 64u && nb < 2048) bsz = (nb <= 128 && max_m > 256u) ? 512 : 256;
+      if (bsz == 512)
+        task_kernel<512><<<nb, 512, lds_bytes, st>>>(f, d_th, jobs[cur][kLarge], pools[cur], pools[cur ^ 1], nx,
+                                                       ord, gkey, rcost, outn, list, ctl, caps, lds_p);
+      else if (bsz == 256)
+        task_kernel<256><<<nb, 256, lds_bytes, st>>>(f, d_th, jobs[cur][kLarge], pools[cur], pools[cur ^ 1], nx,
+                                                     ord, gkey, rcost, outn, list, ctl, caps, lds_p);
+      else
+        task_kernel<64><<<nb, 64, lds_bytes, st>>>(f, d_th, jobs[cur][kLarge], pools[cur], pools[cur ^ 1], nx,
+                                                   ord, gkey, rcost, outn, list, ctl, caps, lds_p);
+    }
     if (n_jobs[2] > 0)
       group_task_kernel<kWaveCut><<<int(ceil_div(int64_t(n_jobs[2]) * kWaveCut, 256)), 256, 0, st>>>(
           f, d_th, jobs[cur][2], int32_t(n_jobs[2]), pools[cur], pools[cur ^ 1], nx, outn, list, ctl, caps);
@@ -1594,7 +1723,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   HIP_TRY(ctx, out->prim_box.alloc(6 * n));
   HIP_TRY(ctx, out->header.alloc(1));
   trav_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, out->nodes.p);
-  leaf_copy_kernel<<<n_out, 64, 0, st>>>(outn, list, f, aux->perm.p, aux->tris.p, aux->prim_box.p,
+  leaf_copy_kernel<<<int(ceil_div(int64_t(n_out) * kCopyLanes, 256)), 256, 0, st>>>(outn, n_out, list, f, aux->perm.p, aux->tris.p, aux->prim_box.p,
                                         out->perm.p, out->phr_src.p, out->tris.p, out->prim_box.p,
                                         int32_t(n), ctl);
   phr_header_kernel<<<1, 1, 0, st>>>(outn, aux->header.p, ctl, n_trav, int32_t(n), out->header.p);
