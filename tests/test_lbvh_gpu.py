@@ -112,3 +112,15 @@ def test_refit_boxes_stable_across_rebuilds(ctx, oracle):
         bvh.rebuild(mesh)
         _, box = bvh.canonical()
         assert np.array_equal(box.astype(np.float64), obox), f"rebuild {k}: node boxes differ"
+
+
+def test_empty_and_oversized_meshes_are_rejected(ctx):
+    """LBVH over zero primitives has no root (the reference's enclosing() of an
+    empty slice is undefined): the boundary rejects it before any launch, and
+    a malformed position array is refused rather than read out of bounds."""
+    from trace_amd import _lib
+
+    with pytest.raises(_lib.TraceError):
+        Mesh(ctx, np.zeros((0, 9), np.float64))
+    with pytest.raises((_lib.TraceError, ValueError)):
+        Mesh(ctx, np.zeros((5, 8), np.float64))
