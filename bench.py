@@ -351,9 +351,12 @@ def bench_large(torch, ctx, steps=2):
     W, H, spp, depth, seed = d["width"], d["height"], 64, 8, 9
     mesh = Mesh(ctx, d["pos"])
     mesh.set_shading(d["mat_idx"], d["materials"])
+    from trace_amd.device import SORT_CONFIGS, autotune_build
+    sort_ms = autotune_build(ctx, mesh)  # the key sort tuned on this scene's own build
+    lbvh = Bvh.lbvh(ctx, mesh)
     ev = _events(torch, 4)
     ev[0].record()
-    lbvh = Bvh.lbvh(ctx, mesh)
+    lbvh.rebuild(mesh)
     ev[1].record()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -401,7 +404,9 @@ def bench_large(torch, ctx, steps=2):
     ctx.synchronize()
     render_ms = ev[2].elapsed_time(ev[3])
     out = {"tris": n, "terrain_tris": nt, "image": [W, H], "spp": spp, "max_depth": depth,
-           "lbvh_build_ms": round(ev[0].elapsed_time(ev[1]), 3), "phr_build_ms": round(phr_ms, 2),
+           "lbvh_build_ms": round(ev[0].elapsed_time(ev[1]), 3),
+           "lbvh_sort_ms_by_config": {f"{c[0]}x{c[1]}": round(float(m), 3) for c, m in zip(SORT_CONFIGS, sort_ms)},
+           "phr_build_ms": round(phr_ms, 2),
            "render_ms": round(render_ms, 2), "render_rays": render_rays,
            "render_mrays_s": round(render_rays / (render_ms * 1e3), 1),
            "samples_per_s": round(W * H * spp / (render_ms * 1e-3), 1),
@@ -620,12 +625,19 @@ def main():
         per_c, _ = time_launches(torch, trace_coh(lbvh), max(5, args.steps // 2), 2)
         out["lbvh_coherent_mrays_s"] = round(m / (float(per_c.mean()) * 1e-3) / 1e6, 2)
         builds = {}
+        from trace_amd.device import SORT_CONFIGS, autotune_build
         for name, mode in (("ref63", KEY_REF63), ("m30", KEY_M30)):
+            # the autotuner times the key sort's digit width x tile size on this build
+            sort_ms = autotune_build(ctx, mesh, mode)
+            sort_cfg = SORT_CONFIGS[int(np.argmin(sort_ms))]
             b = Bvh.lbvh(ctx, mesh, key_mode=mode)
             per_b, _ = time_launches(torch, lambda: b.rebuild(mesh, key_mode=mode), max(5, args.steps // 2), 2)
             ctx.synchronize()
             builds[name] = {"ms": round(float(per_b.mean()), 4),
-                            "mtris_per_s": round(args.tris / (float(per_b.mean()) * 1e-3) / 1e6, 1)}
+                            "mtris_per_s": round(args.tris / (float(per_b.mean()) * 1e-3) / 1e6, 1),
+                            "sort": {"bits": sort_cfg[0], "keys_per_thread": sort_cfg[1],
+                                     "ms_by_config": {f"{c[0]}x{c[1]}": round(float(m), 4)
+                                                      for c, m in zip(SORT_CONFIGS, sort_ms)}}}
             b.close()
         out["lbvh_build"] = builds
         note("PHR grid optimiser (16 BuildWithCost + Cost)")

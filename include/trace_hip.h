@@ -387,6 +387,10 @@ typedef struct {
                               0 = one ray per lane */
   int32_t render_persistent; /* path segments: 1 = split (persistent trace pass over the
                                 queue, then a shade pass), 0 = fused (one segment per lane) */
+  int32_t sort_bits;            /* LBVH key sort: radix digit width, 8, 10 or 11 bits */
+  int32_t sort_keys_per_thread; /* LBVH key sort: keys per thread, 4, 8 or 16 (tiles of 256x);
+                                   valid pairs: 8-bit digits with 4, 8 or 16; 10-bit with 4 or
+                                   8; 11-bit with 4 */
   int32_t pad;
 } trace_tune;
 int trace_get_tune(trace_ctx* ctx, trace_tune* out);
@@ -402,6 +406,12 @@ int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* 
  * fastest; h_ms receives 3 + 4 + 2 times. */
 int trace_autotune_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
                           const trace_camera* cam, const trace_render_params* prm, float* h_ms);
+/* Times the LBVH build of `mesh` (key_mode as trace_lbvh_build) for each key-sort
+ * configuration (digit bits, keys per thread) = (8, 16), (8, 8), (8, 4), (10, 8),
+ * (10, 4), (11, 4) -- 8, 8, 8, 7, 7 and 6 passes of 63-bit keys, tiles of 4096 to
+ * 1024 keys -- keeps the fastest in the context and reports the times in ms (6
+ * entries, in that order).  Every configuration builds the same tree. */
+int trace_autotune_build(trace_ctx* ctx, const trace_mesh* mesh, int key_mode, float* h_ms);
 
 #ifdef __cplusplus
 }

@@ -124,3 +124,55 @@ def test_empty_and_oversized_meshes_are_rejected(ctx):
         Mesh(ctx, np.zeros((0, 9), np.float64))
     with pytest.raises((_lib.TraceError, ValueError)):
         Mesh(ctx, np.zeros((5, 8), np.float64))
+
+
+@pytest.mark.parametrize("bits,kpt", [(8, 16), (8, 8), (8, 4), (10, 8), (10, 4), (11, 4)])
+def test_every_sort_configuration_builds_the_same_tree(ctx, oracle, bits, kpt):
+    """The key sort's digit width and tile size change the passes, not the
+    result: keys, permutation and topology stay bit-exact for 63- and 30-bit
+    keys, across tile edges, with duplicates, and in a forest (whose segment
+    digits add 8-bit passes, here two of them for 300 segments)."""
+    from trace_amd.device import set_tune
+
+    print(f"sort configuration {bits} bits x {kpt} keys per thread", flush=True)
+    set_tune(ctx, sort_bits=bits, sort_keys_per_thread=kpt)
+    try:
+        for key_mode in (KEY_REF63, KEY_M30):
+            for n, seed in ((1, 1), (1023, 2), (1024, 3), (1025, 4), (70_000, 5)):
+                pos = scenes.random_triangles(n, seed=seed)
+                _, bvh = _build(ctx, pos, key_mode)
+                _assert_same_tree(bvh, oracle.Tree(pos, key_mode=key_mode), pos)
+            base = scenes.random_triangles(3000, seed=8)
+            pos = np.concatenate([base, base[:1000], base[500:700]])
+            _, bvh = _build(ctx, pos, key_mode)
+            _assert_same_tree(bvh, oracle.Tree(pos, key_mode=key_mode), pos)
+        rng = np.random.default_rng(5)
+        parts = [scenes.random_triangles(int(k), seed=2000 + i) for i, k in enumerate(rng.integers(1, 40, 300))]
+        forest = Bvh.lbvh(ctx, Mesh.segmented(ctx, parts))
+        for i in (0, 255, 256, 299):
+            v = forest.segment(i)
+            _assert_same_tree(v, oracle.Tree(parts[i]), parts[i])
+            v.close()
+    finally:
+        set_tune(ctx)
+
+
+def test_autotune_build_keeps_the_fastest_sort(ctx, oracle):
+    from trace_amd import _lib
+    from trace_amd.device import SORT_CONFIGS, autotune_build, get_tune, set_tune
+
+    print("autotune build", flush=True)
+    pos = scenes.random_triangles(200_000, seed=12)
+    mesh = Mesh(ctx, pos)
+    try:
+        ms = autotune_build(ctx, mesh, KEY_REF63)
+        assert ms.shape == (len(SORT_CONFIGS),) and np.all(ms > 0)
+        t = get_tune(ctx)
+        assert (t.sort_bits, t.sort_keys_per_thread) == SORT_CONFIGS[int(np.argmin(ms))]
+        _assert_same_tree(Bvh.lbvh(ctx, mesh), oracle.Tree(pos), pos)
+        with pytest.raises(_lib.TraceError):
+            set_tune(ctx, sort_bits=11, sort_keys_per_thread=16)  # 2048 counters per wave: tiles of 1024 only
+        with pytest.raises(_lib.TraceError):
+            set_tune(ctx, sort_bits=9)
+    finally:
+        set_tune(ctx)

@@ -97,6 +97,7 @@ SIGNATURES = {
     "trace_set_tune": [_P, _P],
     "trace_autotune_traversal": [_P, _P, _P, _I64, _P],
     "trace_autotune_render": [_P, _P, _P, _P, _P, _P],
+    "trace_autotune_build": [_P, _P, C.c_int, _P],
     "trace_mesh_bind_params": [_P, _P, _P, _P],
     "trace_render_backward": [_P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_segments": [_P, _P],
@@ -126,6 +127,8 @@ class Tune(C.Structure):
         ("tuned", C.c_int32),
         ("trav_persistent", C.c_int32),
         ("render_persistent", C.c_int32),
+        ("sort_bits", C.c_int32),
+        ("sort_keys_per_thread", C.c_int32),
         ("pad", C.c_int32),
     ]
 

@@ -25,6 +25,8 @@ struct TuneState {
   int64_t render_batch = int64_t(1) << 27;
   bool trav_persistent = true;          // closest hit: persistent lanes refilled from a counter
   bool render_persistent = false;       // path segments: the same for the render's queues
+  int sort_bits = 8;                    // LBVH key sort: radix digit width (8, 10 or 11)
+  int sort_kpt = 16;                    // LBVH key sort: keys per thread (tiles of 256 * kpt)
   bool tuned = false;
 };
 

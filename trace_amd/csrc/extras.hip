@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "onesweep.hpp"
 #include "philox.hpp"
 #include "render.hpp"
 #include "traverse.hpp"
@@ -247,6 +248,8 @@ int trace_get_tune(trace_ctx* ctx, trace_tune* out) {
   out->tuned = ctx->tune.tuned ? 1 : 0;
   out->trav_persistent = ctx->tune.trav_persistent ? 1 : 0;
   out->render_persistent = ctx->tune.render_persistent ? 1 : 0;
+  out->sort_bits = ctx->tune.sort_bits;
+  out->sort_keys_per_thread = ctx->tune.sort_kpt;
   return TRACE_OK;
 }
 
@@ -256,11 +259,16 @@ int trace_set_tune(trace_ctx* ctx, const trace_tune* t) {
   auto ok_threads = [](int v) { return v == 64 || v == 128 || v == 256; };
   if (!ok_threads(t->trav_threads) || !ok_threads(t->render_threads) || t->render_batch < 1)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_set_tune: workgroup sizes must be 64, 128 or 256");
+  if (!onesweep::valid_config(onesweep::Config{t->sort_bits, t->sort_keys_per_thread}))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_set_tune: sort configuration (%d bits, %d keys per thread) unsupported",
+               t->sort_bits, t->sort_keys_per_thread);
   ctx->tune.trav_threads = t->trav_threads;
   ctx->tune.render_threads = t->render_threads;
   ctx->tune.render_batch = t->render_batch;
   ctx->tune.trav_persistent = t->trav_persistent != 0;
   ctx->tune.render_persistent = t->render_persistent != 0;
+  ctx->tune.sort_bits = t->sort_bits;
+  ctx->tune.sort_kpt = t->sort_keys_per_thread;
   return TRACE_OK;
 }
 
@@ -325,6 +333,40 @@ int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* 
   ctx->tune.trav_persistent = best_p;
   ctx->tune.tuned = true;
   return ctx_check_device_error(ctx, "trace_autotune_traversal");
+}
+
+int trace_autotune_build(trace_ctx* ctx, const trace_mesh* mesh, int key_mode, float* h_ms) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || (key_mode != TRACE_KEY_REF63 && key_mode != TRACE_KEY_M30))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_autotune_build: bad arguments");
+  const onesweep::Config cand[6] = {{8, 16}, {8, 8}, {8, 4}, {10, 8}, {10, 4}, {11, 4}};
+  const TuneState before = ctx->tune;
+  trace_bvh* bvh = nullptr;
+  int rc = trace_lbvh_build(ctx, mesh, nullptr, key_mode, &bvh);
+  if (rc) return rc;
+  int best = -1;
+  float best_ms = INFINITY;
+  for (int c = 0; c < 6 && rc == TRACE_OK; ++c) {
+    ctx->tune.sort_bits = cand[c].bits;
+    ctx->tune.sort_kpt = cand[c].kpt;
+    float ms = 0;
+    rc = time_ms(ctx, [&] { return trace_lbvh_rebuild(ctx, mesh, nullptr, key_mode, bvh); }, 3, &ms);
+    if (rc) break;
+    if (h_ms) h_ms[c] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  trace_bvh_destroy(bvh);
+  if (rc || best < 0) {
+    ctx->tune = before;
+    return rc ? rc : TRACE_E_DEVICE;
+  }
+  ctx->tune.sort_bits = cand[best].bits;
+  ctx->tune.sort_kpt = cand[best].kpt;
+  ctx->tune.tuned = true;
+  return ctx_check_device_error(ctx, "trace_autotune_build");
 }
 
 int trace_autotune_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,

@@ -653,7 +653,8 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
           trace_bvh* bvh) {
   const int64_t n = mesh->ntri;
   hipStream_t st = ctx->stream;
-  const int passes = key_mode == TRACE_KEY_M30 ? 4 : 8;
+  const int key_bits = key_mode == TRACE_KEY_M30 ? 30 : 63;
+  const onesweep::Config sort_cfg{ctx->tune.sort_bits, ctx->tune.sort_kpt};
   const int32_t nseg = mesh->nseg;
   const bool forest = nseg > 1;
   const int seg_passes = !forest ? 0 : nseg <= 256 ? 1 : nseg <= 65536 ? 2 : 3;
@@ -688,7 +689,7 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   const size_t o_kb = plan.take(n * sizeof(uint64_t));
   const size_t o_va = plan.take(n * sizeof(uint32_t));
   const size_t o_vb = plan.take(n * sizeof(uint32_t));
-  const size_t o_st = plan.take(onesweep::status_words(n, passes + seg_passes) * sizeof(uint32_t));
+  const size_t o_st = plan.take(onesweep::status_words(n, key_bits, seg_passes, sort_cfg) * sizeof(uint32_t));
   const size_t o_uk = plan.take(n * sizeof(uint64_t));
   const size_t o_blk = plan.take(utiles * sizeof(uint32_t));
   const size_t o_cnt = plan.take(std::max<int64_t>(1, n) * sizeof(uint32_t));
@@ -728,8 +729,8 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   morton_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, qbox, tseg, key_mode,
                                                   bvh->keys.p);
 
-  const int where = onesweep::sort_pairs(st, bvh->keys.p, vals_a, keys_b, vals_b, n, passes,
-                                         status, ctx->d_err, true, tseg, seg_passes);
+  const int where = onesweep::sort_pairs(st, bvh->keys.p, vals_a, keys_b, vals_b, n, key_bits,
+                                         status, ctx->d_err, true, tseg, seg_passes, sort_cfg);
   uint64_t* sorted_keys = where ? keys_b : bvh->keys.p;
   uint32_t* sorted_vals = where ? vals_b : vals_a;
   if (where) {

@@ -1581,7 +1581,8 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const size_t o_scan = plan.take(scan::scratch_words(out_cap) * sizeof(uint32_t));
   const size_t o_ntr = plan.take(64);
   // status words of the canonical-order sort (at most n - 1 branches, 8 passes)
-  const size_t o_cst = plan.take(onesweep::status_words(std::max<int64_t>(n, 1), 8) * sizeof(uint32_t));
+  const size_t o_cst = plan.take(onesweep::status_words(std::max<int64_t>(n, 1), 64, 0, onesweep::Config{}) *
+                                 sizeof(uint32_t));
   int rc = ctx_reserve_scratch(ctx, plan.off);
   if (rc) return rc;
   char* base = static_cast<char*>(ctx->scratch);
@@ -1701,7 +1702,6 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     int prim_bits = 1, level_bits = 1;
     while ((int64_t(1) << prim_bits) < n) ++prim_bits;
     while ((int64_t(1) << level_bits) <= int64_t(hc.max_level)) ++level_bits;
-    const int passes = (prim_bits + level_bits + 7) / 8;
     if (int64_t(h_ntrav) > pool_cap || prim_bits + level_bits > 64)
       TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: layout buffers too small");
     auto* keys_a = reinterpret_cast<uint64_t*>(gkey);
@@ -1710,8 +1710,8 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     auto* vals_b = reinterpret_cast<uint32_t*>(ord + pool_cap);
     auto* status = reinterpret_cast<uint32_t*>(base + o_cst);
     canon_keys_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, prim_bits, keys_a, vals_a);
-    const int where = onesweep::sort_pairs(st, keys_a, vals_a, keys_b, vals_b, h_ntrav, passes, status, ctx->d_err,
-                                           false);
+    const int where = onesweep::sort_pairs(st, keys_a, vals_a, keys_b, vals_b, h_ntrav, prim_bits + level_bits,
+                                           status, ctx->d_err, false, nullptr, 0, onesweep::Config{});
     canon_rank_kernel<<<int(ceil_div(h_ntrav, 256)), 256, 0, st>>>(where ? vals_b : vals_a, int32_t(h_ntrav), tidx);
     rc = ctx_check_device_error(ctx, "trace_phr_build");
     if (rc) return rc;

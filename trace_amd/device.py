@@ -529,12 +529,25 @@ def get_tune(ctx: Context) -> "_lib.Tune":
 
 
 def set_tune(ctx: Context, trav_threads: int = 64, render_threads: int = 64, render_batch: int = 1 << 27,
-             trav_persistent: bool = True, render_persistent: bool = False) -> None:
+             trav_persistent: bool = True, render_persistent: bool = False, sort_bits: int = 8,
+             sort_keys_per_thread: int = 16) -> None:
     t = _lib.Tune()
     t.trav_threads, t.render_threads, t.render_batch = trav_threads, render_threads, render_batch
     t.trav_persistent = 1 if trav_persistent else 0
     t.render_persistent = 1 if render_persistent else 0
+    t.sort_bits, t.sort_keys_per_thread = sort_bits, sort_keys_per_thread
     ctx.check(ctx.lib.trace_set_tune(ctx.h, C.byref(t)))
+
+
+# the key-sort configurations trace_autotune_build times, in its order
+SORT_CONFIGS = ((8, 16), (8, 8), (8, 4), (10, 8), (10, 4), (11, 4))
+
+
+def autotune_build(ctx: Context, mesh: Mesh, key_mode: int = 0) -> np.ndarray:
+    """Times the LBVH build for each key-sort configuration (SORT_CONFIGS); keeps the best."""
+    ms = np.zeros(len(SORT_CONFIGS), np.float32)
+    ctx.check(ctx.lib.trace_autotune_build(ctx.h, mesh.h, key_mode, _ptr(ms)))
+    return ms
 
 
 def autotune_traversal(ctx: Context, bvh: Bvh, d_rays: int, n: int) -> np.ndarray:
