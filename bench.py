@@ -265,10 +265,31 @@ def bench_adjoint(torch, ctx, steps):
     per_b, _ = time_launches(torch, backward, steps, 1)
     ctx.synchronize()
     p_ms, b_ms = float(np.median(per_p)), float(np.median(per_b))
+    # the same pair with the path summaries kept by the primal (16 B per path):
+    # the backward then traces only the replay sweep
+    from trace_amd.device import render_backward_kept, render_keep
+    keep = torch.empty((spp * H * W * 4,), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    def primal_keep():
+        render_keep(ctx, mesh, bvh, d["camera"], W, H, spp, depth, _lib.MISS_DEFAULT, 7, s.data_ptr(), q.data_ptr(),
+                    keep.data_ptr())
+
+    def backward_kept():
+        render_backward_kept(ctx, mesh, bvh, d["camera"], W, H, spp, depth, _lib.MISS_DEFAULT, 7, keep.data_ptr(),
+                             g_img.data_ptr(), ga.data_ptr(), ge.data_ptr())
+
+    per_pk, _ = time_launches(torch, primal_keep, steps, 1)
+    per_bk, _ = time_launches(torch, backward_kept, steps, 1)
+    ctx.synchronize()
+    pk_ms, bk_ms = float(np.median(per_pk)), float(np.median(per_bk))
     out = {"tris": n, "image": [W, H], "spp": spp, "max_depth": depth, "primal_ms": round(p_ms, 3),
            "backward_ms": round(b_ms, 3), "adjoint_over_primal": round(b_ms / p_ms, 3), "loss": loss,
            "grad_albedo_norm": float(torch.linalg.vector_norm(ga).cpu()),
-           "grad_emission_norm": float(torch.linalg.vector_norm(ge).cpu())}
+           "grad_emission_norm": float(torch.linalg.vector_norm(ge).cpu()),
+           "kept": {"primal_keep_ms": round(pk_ms, 3), "backward_kept_ms": round(bk_ms, 3),
+                    "backward_kept_over_primal": round(bk_ms / p_ms, 3),
+                    "step_ms": round(pk_ms + bk_ms, 3), "step_ms_without_keep": round(p_ms + b_ms, 3)}}
     mesh.bind_params(None, None)
     bvh.close()
     mesh.close()
@@ -344,7 +365,8 @@ def bench_large(torch, ctx, steps=2):
     Times the builds, one primal render and `steps` optimisation steps
     (primal + backward + Adam)."""
     from trace_amd import _lib, scenes
-    from trace_amd.device import Bvh, Mesh, adam_step, render_backward, render_device, render_segments
+    from trace_amd.device import (Bvh, Mesh, adam_step, render_backward_kept, render_device, render_keep,
+                                  render_segments)
 
     d = scenes.terrain_scene()
     n, nt = len(d["pos"]), d["n_terrain"]
@@ -382,22 +404,26 @@ def bench_large(torch, ctx, steps=2):
     render_rays = render_segments(ctx)
     target = img / spp
     mesh.bind_params(alb.data_ptr(), emi.data_ptr())
+    # the primal keeps every path's summary (16 B per path, 2.1 GB here) so
+    # that the backward traces only its replay sweep
+    keep = torch.empty((spp * H * W * 4,), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
     losses, step_ms, step_rays = [], [], []
     for k in range(1, steps + 1):
         note(f"large: Adam step {k}")
         e0, e1, e2 = _events(torch, 3)
         e0.record()
-        render_device(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed, img.data_ptr(),
-                      sq.data_ptr())
+        render_keep(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed, img.data_ptr(),
+                    sq.data_ptr(), keep.data_ptr())
         e1.record()
         diff = img / spp - target
         g_img = (2.0 / diff.numel()) * diff
-        render_backward(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed, g_img.data_ptr(),
-                        ga.data_ptr(), ge.data_ptr())
+        render_backward_kept(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed, keep.data_ptr(),
+                             g_img.data_ptr(), ga.data_ptr(), ge.data_ptr())
         adam_step(ctx, alb.data_ptr(), ga.data_ptr(), m.data_ptr(), v.data_ptr(), 3 * nt, 1e-2, k)
         e2.record()
         torch.cuda.synchronize()
-        # rays of the primal render (the same paths as the target's) and of both backward sweeps
+        # rays of the primal render (the same paths as the target's) and of the replay sweep
         step_rays.append(render_rays + render_segments(ctx))
         losses.append(float((diff * diff).mean()))
         step_ms.append(e0.elapsed_time(e2))
@@ -665,6 +691,8 @@ def main():
         note("adjoint")
         out["adjoint"] = bench_adjoint(torch, ctx, max(3, args.steps // 4))
         out["adjoint_over_primal"] = out["adjoint"]["adjoint_over_primal"]
+        # the backward from the summaries its primal kept (replay sweep only)
+        out["adjoint_over_primal_kept"] = out["adjoint"]["kept"]["backward_kept_over_primal"]
         if not args.no_large:
             note("large scene (10.4M triangles) and inverse-rendering steps")
             out["large"] = bench_large(torch, ctx)

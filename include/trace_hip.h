@@ -332,6 +332,23 @@ int trace_mesh_bind_params(trace_ctx* ctx, trace_mesh* mesh, const float* d_albe
 int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
                           const trace_camera* cam, const trace_render_params* prm,
                           const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission);
+/* trace_render_device that also keeps, per path, what trace_render_backward's first
+ * sweep would compute (where the path ended, and the product of its non-zero albedos
+ * and the radiance there): d_keep = W*H*spp*4 floats of device memory (16-byte aligned),
+ * written sample-major.  The image is the same, bit for bit, as trace_render_device's.
+ * Meshes with Reflective or Refractive materials are rejected, as by the backward.
+ * Asynchronous.  (No reference counterpart: for inverse-rendering loops, which render
+ * the primal image anyway.) */
+int trace_render_keep(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
+                      const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last,
+                      float* d_keep);
+/* trace_render_backward given the d_keep of a trace_render_keep call with the same
+ * arguments and the same bound parameters: only the replay sweep is traced.  The
+ * gradient equals trace_render_backward's (up to the order of its atomic sums).
+ * Asynchronous. */
+int trace_render_backward_kept(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                               const trace_camera* cam, const trace_render_params* prm, const float* d_keep,
+                               const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission);
 
 /* One Adam step (Kingma & Ba 2015) on n device parameters, in place: m, v are the
  * caller's moment buffers (zero before step 1), step counts from 1; the result is

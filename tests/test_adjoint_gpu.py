@@ -251,3 +251,70 @@ def test_inverse_rendering_lowers_the_loss(ctx, small):
         adam_step(ctx, alb.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), 3 * n, 0.02, step)
         ctx.synchronize()
     assert losses[-1] < 0.8 * losses[0], losses
+
+
+@pytest.mark.parametrize("zeros,miss", [(False, _lib.MISS_DEFAULT), (True, _lib.MISS_SKY)])
+def test_kept_summaries_give_the_same_image_and_gradient(ctx, small, zeros, miss):
+    """trace_render_keep derives the image from the backward's summary sweep:
+    it equals trace_render_device's bit for bit (zero albedo channels exercise
+    the zero counts; the sky makes every escaping path radiant).  The backward
+    from the kept summaries traces only the replay and gives the same gradient
+    as the self-contained backward, up to the order of its atomic sums."""
+    from trace_amd.device import render_backward_kept, render_keep
+
+    print(f"adjoint: kept summaries zeros={zeros} miss={miss}", flush=True)
+    d = small.d
+    alb = d["albedo"].copy()
+    if zeros:
+        rng = np.random.default_rng(9)
+        alb[rng.random(len(alb)) < 0.3, rng.integers(0, 3)] = 0.0
+    small.bind(alb, d["emission"])
+    spp, seed = 5, 21  # 5 samples: the keep buffer's sample-major layout over an odd count
+    out = {k: torch.zeros((H, W, 3), dtype=torch.float32, device="cuda") for k in ("s", "q", "l", "s2", "q2", "l2")}
+    keep = torch.full((spp, H, W, 4), float("nan"), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    render_device(ctx, small.mesh, small.bvh, d["camera"], W, H, spp, DEPTH, miss, seed, out["s"].data_ptr(),
+                  out["q"].data_ptr(), out["l"].data_ptr())
+    render_keep(ctx, small.mesh, small.bvh, d["camera"], W, H, spp, DEPTH, miss, seed, out["s2"].data_ptr(),
+                out["q2"].data_ptr(), keep.data_ptr(), out["l2"].data_ptr())
+    ctx.synchronize()
+    for a, b in (("s", "s2"), ("q", "q2"), ("l", "l2")):
+        assert torch.equal(out[a], out[b]), a
+    assert not torch.isnan(keep).any()  # every path's summary was written
+    g_img = np.random.default_rng(3).standard_normal((H, W, 3)) * 0.1
+    ga, ge = small.grad(W, H, spp, DEPTH, miss, seed, g_img)
+    full_segments = render_segments(ctx)
+    g = _dev(g_img)
+    ka = torch.empty((small.n, 3), dtype=torch.float32, device="cuda")
+    ke = torch.empty_like(ka)
+    torch.cuda.synchronize()
+    render_backward_kept(ctx, small.mesh, small.bvh, d["camera"], W, H, spp, DEPTH, miss, seed, keep.data_ptr(),
+                         g.data_ptr(), ka.data_ptr(), ke.data_ptr())
+    ctx.synchronize()
+    kept_segments = render_segments(ctx)
+    ka, ke = ka.cpu().numpy().astype(np.float64), ke.cpu().numpy().astype(np.float64)
+    for x, y in ((ka, ga), (ke, ge)):
+        assert np.abs(y).max() > 0
+        assert np.allclose(x, y, rtol=1e-5, atol=1e-7 * np.abs(y).max())
+    assert 0 < kept_segments < full_segments
+    small.mesh.bind_params(None, None)
+
+
+def test_kept_entry_points_reject_bad_input(ctx, small):
+    from trace_amd.device import render_backward_kept, render_keep
+
+    d = small.d
+    s = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    keep = torch.zeros((2 * H * W * 4 + 1,), dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.TraceError):  # misaligned summary buffer
+        render_keep(ctx, small.mesh, small.bvh, d["camera"], W, H, 2, DEPTH, 0, 1, s.data_ptr(), s.data_ptr(),
+                    keep.data_ptr() + 4)
+    with pytest.raises(_lib.TraceError):
+        render_backward_kept(ctx, small.mesh, small.bvh, d["camera"], W, H, 2, DEPTH, 0, 1, 0, s.data_ptr(),
+                             s.data_ptr(), s.data_ptr())
+    sc = scenes.specular_box_scene(slices=8, rings=6, width=16, height=16)
+    mesh = Mesh(ctx, sc["pos"])
+    mesh.set_shading_ex(sc["mat_idx"], sc["materials"], sc["normals"])
+    bvh = Bvh.lbvh(ctx, mesh)
+    with pytest.raises(_lib.TraceError):  # Reflective / Refractive are not differentiated
+        render_keep(ctx, mesh, bvh, sc["camera"], 8, 8, 1, 2, 0, 1, s.data_ptr(), s.data_ptr(), keep.data_ptr())

@@ -100,6 +100,8 @@ SIGNATURES = {
     "trace_autotune_build": [_P, _P, C.c_int, _P],
     "trace_mesh_bind_params": [_P, _P, _P, _P],
     "trace_render_backward": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "trace_render_keep": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "trace_render_backward_kept": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_segments": [_P, _P],
     "trace_mesh_set_shading_ex": [_P, _P, _P, _P, _P, _I32],
     "trace_scene_create": [_P, _P, _P, _P, _I32, _P, _P, _I32, _P],

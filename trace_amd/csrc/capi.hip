@@ -846,6 +846,22 @@ int trace_render_device(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh*
   return render::render(ctx, mesh, bvh, cam, prm, d_sum, d_sumsq, d_last);
 }
 
+int trace_render_keep(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
+                      const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last,
+                      float* d_keep) {
+  if (!ctx) return TRACE_E_INVALID;
+  int rc = check_render_args(ctx, mesh, bvh, cam, prm, "trace_render_keep");
+  if (rc) return rc;
+  if (!d_sum || !d_sumsq || !d_keep) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_keep: null output");
+  if (reinterpret_cast<uintptr_t>(d_keep) % 16)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_keep: d_keep must be 16-byte aligned");
+  if (mesh->has_specular)
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_keep: Reflective / Refractive materials are not differentiated");
+  if (prm->spp == 0) return TRACE_OK;
+  const trace_bvh* one[1] = {bvh};
+  return render::render_scenes(ctx, mesh, one, 1, cam, prm, d_sum, d_sumsq, d_last, reinterpret_cast<float4*>(d_keep));
+}
+
 int trace_render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
                  const trace_camera* cam, const trace_render_params* prm, float* h_sum,
                  float* h_sumsq, float* h_last) {

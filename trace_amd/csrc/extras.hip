@@ -131,9 +131,9 @@ int trace_mesh_bind_params(trace_ctx* ctx, trace_mesh* mesh, const float* d_albe
   return TRACE_OK;
 }
 
-int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
-                          const trace_camera* cam, const trace_render_params* prm,
-                          const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission) {
+static int backward_impl(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
+                         const trace_render_params* prm, const float* d_keep, const float* d_grad_image,
+                         float* d_grad_albedo, float* d_grad_emission) {
   if (!ctx) return TRACE_E_INVALID;
   if (!mesh || !bvh || !cam || !prm || !d_grad_image || !d_grad_albedo || !d_grad_emission)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: null argument");
@@ -149,8 +149,24 @@ int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward: bad render parameters (spp >= 1 needed)");
   if (int64_t(prm->width) * prm->height > (int64_t(1) << 28))
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_backward: image too large");
+  if (d_keep && reinterpret_cast<uintptr_t>(d_keep) % 16)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward_kept: d_keep must be 16-byte aligned");
   const trace_bvh* one[1] = {bvh};
-  return render::backward_scenes(ctx, mesh, one, 1, cam, prm, d_grad_image, d_grad_albedo, d_grad_emission);
+  return render::backward_scenes(ctx, mesh, one, 1, cam, prm, d_grad_image, d_grad_albedo, d_grad_emission,
+                                 reinterpret_cast<const float4*>(d_keep));
+}
+
+int trace_render_backward(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                          const trace_camera* cam, const trace_render_params* prm,
+                          const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission) {
+  return backward_impl(ctx, mesh, bvh, cam, prm, nullptr, d_grad_image, d_grad_albedo, d_grad_emission);
+}
+
+int trace_render_backward_kept(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh,
+                               const trace_camera* cam, const trace_render_params* prm, const float* d_keep,
+                               const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission) {
+  if (ctx && !d_keep) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_backward_kept: null d_keep");
+  return backward_impl(ctx, mesh, bvh, cam, prm, d_keep, d_grad_image, d_grad_albedo, d_grad_emission);
 }
 
 int trace_adam_step(trace_ctx* ctx, float* d_param, const float* d_grad, float* d_m, float* d_v,

@@ -173,20 +173,26 @@ __global__ __launch_bounds__(256) void camera_kernel(const Scene* __restrict__ s
 // Replay start: re-creates the camera ray of every path whose summary can
 // carry a gradient (it ended on a light, or with non-zero radiance, and its
 // pixel's image gradient is non-zero) and queues it; rad keeps the summary.
+// With `keep`, the summaries come from a primal that kept them (at
+// keep[scene * keep_scene + keep_off + q]) and are copied into rad for the
+// replayed paths.
 __global__ __launch_bounds__(256) void replay_camera_kernel(
     const Scene* __restrict__ scenes, int32_t W, int32_t H, int64_t n_paths, int64_t pps,
     int64_t path_base, const float* __restrict__ g_img, float4* __restrict__ ro,
-    float4* __restrict__ rd, float4* __restrict__ thr, const float4* __restrict__ rad,
-    int32_t* __restrict__ queue, uint32_t* __restrict__ count) {
+    float4* __restrict__ rd, float4* __restrict__ thr, float4* __restrict__ rad,
+    int32_t* __restrict__ queue, uint32_t* __restrict__ count, const float4* __restrict__ keep,
+    int64_t keep_scene, int64_t keep_off) {
   const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   bool push = false;
   if (p < n_paths) {
     const int64_t wh = int64_t(W) * H;
     const int64_t scene = p / pps, q = p % pps;
     const float* g = g_img + 3 * (scene * wh + q % wh);
-    const uint32_t flags = __float_as_uint(rad[p].w);
+    const float4 S = keep ? keep[scene * keep_scene + keep_off + q] : rad[p];
+    const uint32_t flags = __float_as_uint(S.w);
     push = (flags & (kFlagLight | kFlagRadiant)) && (g[0] != 0.0f || g[1] != 0.0f || g[2] != 0.0f);
     if (push) {
+      if (keep) rad[p] = S;
       float4 o, d;
       camera_ray(scenes[scene], W, H, q, path_base, o, d);
       ro[p] = o;
@@ -488,9 +494,17 @@ __device__ inline void segment_epilogue(const Adj& adj, int64_t pps, int32_t p, 
 
 // One thread per (scene, pixel): the batch's samples in sample order.  Output
 // is scene-major: scene c's image starts at 3 * c * wh.
+//
+// With `keep`, rad holds path summaries (the summary sweep) instead of
+// radiance: channel c of a path's radiance is its summary's when no albedo on
+// the path was zero in that channel (the summary multiplies the non-zero
+// albedos in the primal's order, so the value is the primal's bit for bit),
+// and 0 otherwise.  The summaries themselves are stored at keep[scene *
+// keep_scene + keep_off + q] for the backward pass.
 __global__ void accumulate_kernel(const float4* __restrict__ rad, int64_t wh, int32_t nscene,
                                   int32_t nsamp, int32_t first_batch, float* __restrict__ sum,
-                                  float* __restrict__ sumsq, float* __restrict__ last) {
+                                  float* __restrict__ sumsq, float* __restrict__ last,
+                                  float4* __restrict__ keep, int64_t keep_scene, int64_t keep_off) {
   const int64_t qx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (qx >= wh * nscene) return;
   const int64_t scene = qx / wh, pix = qx % wh;
@@ -500,6 +514,13 @@ __global__ void accumulate_kernel(const float4* __restrict__ rad, int64_t wh, in
   float4 v = make_float4(0, 0, 0, 0);
   for (int32_t s = 0; s < nsamp; ++s) {
     v = base[int64_t(s) * wh];
+    if (keep) {
+      keep[scene * keep_scene + keep_off + int64_t(s) * wh + pix] = v;
+      const uint32_t f = __float_as_uint(v.w);
+      v.x = (f & 3u) == 0u ? v.x : 0.0f;
+      v.y = ((f >> 2) & 3u) == 0u ? v.y : 0.0f;
+      v.z = ((f >> 4) & 3u) == 0u ? v.z : 0.0f;
+    }
     s0 += v.x;
     s1 += v.y;
     s2 += v.z;
@@ -878,19 +899,20 @@ Adj mesh_adj(const trace_mesh* mesh, int32_t mode) {
 int render(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
            const trace_render_params* prm, float* d_sum, float* d_sumsq, float* d_last) {
   const trace_bvh* one[1] = {bvh};
-  return render_scenes(ctx, mesh, one, 1, cam, prm, d_sum, d_sumsq, d_last);
+  return render_scenes(ctx, mesh, one, 1, cam, prm, d_sum, d_sumsq, d_last, nullptr);
 }
 
 int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
                   int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
-                  float* d_sum, float* d_sumsq, float* d_last) {
+                  float* d_sum, float* d_sumsq, float* d_last, float4* d_keep) {
   Wave w;
   int rc = prepare(ctx, trees, nscene, cams, prm, w);
   if (rc) return rc;
   hipStream_t st = ctx->stream;
   fill_async<unsigned long long>(st, segment_counter(ctx), 1, 0ull);
   const int64_t wh = int64_t(prm->width) * prm->height;
-  const Adj adj = mesh_adj(mesh, kModePrimal);
+  // keeping summaries traces the summary sweep and derives the image from it
+  const Adj adj = mesh_adj(mesh, d_keep ? kModeSummary : kModePrimal);
   for (int64_t s0 = 0; s0 < prm->spp; s0 += w.spb) {
     const int64_t nsamp = std::min<int64_t>(w.spb, prm->spp - s0);
     const int64_t pps = nsamp * wh;
@@ -903,7 +925,7 @@ int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const
     const bool last_batch = s0 + nsamp >= prm->spp;
     accumulate_kernel<<<int(ceil_div(wh * nscene, 256)), 256, 0, st>>>(
         w.rad, wh, nscene, int32_t(nsamp), s0 == 0 ? 1 : 0, d_sum, d_sumsq,
-        last_batch ? d_last : nullptr);
+        last_batch ? d_last : nullptr, d_keep, int64_t(prm->spp) * wh, s0 * wh);
   }
   HIP_TRY(ctx, hipGetLastError());
   return TRACE_OK;
@@ -911,7 +933,8 @@ int render_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const
 
 int backward_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* const* trees,
                     int32_t nscene, const trace_camera* cams, const trace_render_params* prm,
-                    const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission) {
+                    const float* d_grad_image, float* d_grad_albedo, float* d_grad_emission,
+                    const float4* d_keep) {
   Wave w;
   int rc = prepare(ctx, trees, nscene, cams, prm, w);
   if (rc) return rc;
@@ -933,15 +956,18 @@ int backward_scenes(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* con
     const int64_t npaths = pps * nscene;
     const int64_t path_base = (prm->sample_offset + s0) * wh;
     const int grid = int(ceil_div(npaths, 256));
-    // sweep 1: trace every path, keep its summary
-    camera_kernel<<<grid, 256, 0, st>>>(w.scenes, prm->width, prm->height, npaths, pps, path_base,
-                                        w.ro, w.rd, w.thr, w.rad, w.q[0], w.counts);
-    bounces(ctx, mesh, trees[0], w, npaths, pps, path_base, prm, summary);
+    // sweep 1: trace every path, keep its summary (skipped when the primal
+    // kept the summaries)
+    if (!d_keep) {
+      camera_kernel<<<grid, 256, 0, st>>>(w.scenes, prm->width, prm->height, npaths, pps, path_base,
+                                          w.ro, w.rd, w.thr, w.rad, w.q[0], w.counts);
+      bounces(ctx, mesh, trees[0], w, npaths, pps, path_base, prm, summary);
+    }
     // sweep 2: replay the paths that carry a gradient and scatter it
     fill_async<uint32_t>(st, w.counts, 2, 0u);
     replay_camera_kernel<<<grid, 256, 0, st>>>(w.scenes, prm->width, prm->height, npaths, pps,
                                                path_base, d_grad_image, w.ro, w.rd, w.thr, w.rad,
-                                               w.q[0], w.counts);
+                                               w.q[0], w.counts, d_keep, int64_t(prm->spp) * wh, s0 * wh);
     bounces(ctx, mesh, trees[0], w, npaths, pps, path_base, prm, replay);
   }
   HIP_TRY(ctx, hipGetLastError());

@@ -397,6 +397,29 @@ def render_backward(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height:
                                             C.c_void_p(d_grad_albedo), C.c_void_p(d_grad_emission)))
 
 
+def render_keep(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: int, spp: int, max_depth: int,
+                miss: int, seed: int, d_sum: int, d_sumsq: int, d_keep: int, d_last: int | None = None,
+                sample_offset: int = 0) -> None:
+    """trace_render_keep (asynchronous): render_device that also keeps the path
+    summaries (d_keep: width * height * spp * 4 floats) for render_backward_kept."""
+    c = camera_struct(cam)
+    p = render_params(width, height, spp, max_depth, miss, seed, sample_offset)
+    ctx.check(ctx.lib.trace_render_keep(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), C.c_void_p(d_sum),
+                                        C.c_void_p(d_sumsq), C.c_void_p(d_last or 0), C.c_void_p(d_keep)))
+
+
+def render_backward_kept(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: int, spp: int,
+                         max_depth: int, miss: int, seed: int, d_keep: int, d_grad_image: int, d_grad_albedo: int,
+                         d_grad_emission: int, sample_offset: int = 0) -> None:
+    """trace_render_backward_kept (asynchronous): render_backward from the
+    summaries of a render_keep call with the same arguments (replay only)."""
+    c = camera_struct(cam)
+    p = render_params(width, height, spp, max_depth, miss, seed, sample_offset)
+    ctx.check(ctx.lib.trace_render_backward_kept(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), C.c_void_p(d_keep),
+                                                 C.c_void_p(d_grad_image), C.c_void_p(d_grad_albedo),
+                                                 C.c_void_p(d_grad_emission)))
+
+
 def render_segments(ctx: Context) -> int:
     """trace_render_segments: rays traced by the context's last render or
     backward call (waits for the context stream)."""
