@@ -97,9 +97,11 @@ int trace_ctx_create(int device, trace_ctx** out) {
       hipMalloc(reinterpret_cast<void**>(&ctx->d_err), sizeof(uint32_t)) != hipSuccess ||
       hipMemset(ctx->d_err, 0, sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&ctx->d_work), 64 * sizeof(uint32_t)) != hipSuccess ||
+      hipHostMalloc(&ctx->h_pinned, kPinnedBytes, hipHostMallocDefault) != hipSuccess ||
       hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     if (ctx->d_work) (void)hipFree(ctx->d_work);
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return TRACE_E_HIP;
@@ -117,6 +119,7 @@ int trace_ctx_destroy(trace_ctx* ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->d_err) (void)hipFree(ctx->d_err);
   if (ctx->d_work) (void)hipFree(ctx->d_work);
+  if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return TRACE_OK;

@@ -30,6 +30,8 @@ struct TuneState {
   bool tuned = false;
 };
 
+constexpr size_t kPinnedBytes = 4096;
+
 struct trace_ctx {
   TuneState tune;
   int device = 0;
@@ -41,6 +43,7 @@ struct trace_ctx {
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
   uint32_t* d_work = nullptr;  // work counters of persistent kernels (64 words)
+  void* h_pinned = nullptr;    // kPinnedBytes of pinned host memory for small per-step readbacks
   int n_cu = 256;              // compute units of the device
   // HIP graphs recorded from this context hold scratch addresses: while any is
   // alive (or a capture is open) the arena must not move

@@ -29,6 +29,7 @@
 // with one worker.  Thresholds per depth are computed on the host with
 // std::pow so they equal the oracle's bit for bit.
 #include <cmath>
+#include <cstring>
 
 #include "onesweep.hpp"
 #include "phr.hpp"
@@ -1479,9 +1480,13 @@ __global__ void phr_header_kernel(const OutNode* __restrict__ out, const BvhHead
 }
 
 // ---- host driver ---------------------------------------------------------------
+// The level counters, read once per level through the context's pinned
+// staging buffer (a direct device-to-host copy, no pageable bounce).
+static_assert(sizeof(Ctl) <= kPinnedBytes, "Ctl fits the pinned staging buffer");
 static int read_ctl(trace_ctx* ctx, const Ctl* d, Ctl* h) {
-  HIP_TRY(ctx, hipMemcpyAsync(h, d, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pinned, d, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(h, ctx->h_pinned, sizeof(Ctl));
   return TRACE_OK;
 }
 
