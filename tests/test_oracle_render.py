@@ -3,7 +3,6 @@ fixed by the mathematics, not by the oracle's own formulas:
 Philox known-answer vectors, a furnace test, the set of values a path can
 take under the reference's depth rule, and direct views of emitters."""
 import numpy as np
-import pytest
 
 from trace_amd import scenes
 

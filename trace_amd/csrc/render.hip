@@ -767,7 +767,6 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
                           int64_t path_base, float4* ro, float4* rd, float4* thr, float4* rad,
                           int32_t* qin, int32_t* qout, uint32_t* cin, uint32_t* cout, int4* hits,
                           const Adj& adj) {
-  hipStream_t st = ctx->stream;
   SegArgs g;
   g.tris = bvh->tris_p();
   g.perm = bvh->perm_p();
