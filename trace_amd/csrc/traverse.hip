@@ -47,8 +47,12 @@ __global__ __launch_bounds__(THREADS) void closest_hit_kernel(
 constexpr uint32_t kChunk = 256;
 constexpr int kRefillAt = 16;
 
+// The FP64 re-solve of each winner is left to refine_kernel: without it the
+// loop's register peak is 64 VGPRs, so eight waves fit on a SIMD (asked for
+// where nothing spills: the plain, triangle-only variant); hit_out receives
+// the winner's sorted position and t_out its FP32 distance.
 template <int THREADS, int STACK, bool COUNT, bool SPH>
-__global__ __launch_bounds__(THREADS) void closest_hit_persistent(
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(COUNT || SPH ? 1 : 8, 8))) void closest_hit_persistent(
     const WideNode* __restrict__ wnodes, const float* __restrict__ tris,
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
     const float4* __restrict__ rays, int64_t n, int32_t id_offset, int32_t* __restrict__ hit_out,
@@ -70,8 +74,7 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
   int64_t pool_next = 0, pool_end = 0;   // this wave's unclaimed ray indices
   int32_t nn = 0, nt = 0;
   auto finish = [&]() {
-    refine_hit_fp64(tris, ray, s.h);
-    hit_out[rid] = s.h.pos >= 0 ? perm[s.h.pos] - id_offset : -1;
+    hit_out[rid] = s.h.pos;
     t_out[rid] = s.h.t;
     if (COUNT) {
       cnt_nodes[rid] = nn;
@@ -129,6 +132,35 @@ __global__ __launch_bounds__(THREADS) void closest_hit_persistent(
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);
 }
 
+// After closest_hit_persistent: each winner re-solved in FP64
+// (refine_hit_fp64) and its sorted position mapped to the caller's index.
+__global__ __launch_bounds__(256) void refine_kernel(const float* __restrict__ tris, const int32_t* __restrict__ perm,
+                                                     int32_t n_prims, const float4* __restrict__ rays, int64_t n,
+                                                     int32_t id_offset, int32_t* __restrict__ hit_out,
+                                                     float* __restrict__ t_out, uint32_t* d_err) {
+  const int64_t i = blockIdx.x * int64_t(256) + threadIdx.x;
+  if (i >= n) return;
+  const int32_t pos = hit_out[i];
+  if (pos < 0) return;  // a miss: -1 and the ray's tmax are final
+  if (pos >= n_prims) {
+    atomicOr(d_err, TRACE_ERR_TREE);
+    hit_out[i] = -1;
+    return;
+  }
+  const float4 a = rays[2 * i];
+  const float4 b = rays[2 * i + 1];
+  const Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  Hit h;
+  h.pos = pos;
+  h.t = t_out[i];
+  h.t64 = h.t;
+  h.u = h.v = 0.0f;
+  h.front = true;
+  refine_hit_fp64(tris, ray, h);
+  hit_out[i] = perm[pos] - id_offset;
+  t_out[i] = h.t;
+}
+
 template <int THREADS, int STACK, bool COUNT, bool SPH>
 static void launch_kind(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, int64_t n,
                         int32_t id_offset, int32_t* d_hit, float* d_t, int32_t* d_nodes, int32_t* d_tris) {
@@ -141,6 +173,10 @@ static void launch_kind(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays
     closest_hit_persistent<THREADS, STACK, COUNT, SPH><<<int(blocks), THREADS, 0, ctx->stream>>>(
         bvh->wnodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n,
         id_offset, d_hit, d_t, d_nodes, d_tris, ctx->d_err, work);
+    // sorted positions index the whole triangle array (a forest's, for a segment view)
+    const int64_t n_sorted = bvh->base ? bvh->base->n_prims : bvh->n_prims;
+    refine_kernel<<<int(ceil_div(n, 256)), 256, 0, ctx->stream>>>(bvh->tris_p(), bvh->perm_p(), int32_t(n_sorted),
+                                                                 rays, n, id_offset, d_hit, d_t, ctx->d_err);
     return;
   }
   closest_hit_kernel<THREADS, STACK, COUNT, SPH><<<int(ceil_div(n, THREADS)), THREADS, 0, ctx->stream>>>(
