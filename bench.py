@@ -80,10 +80,18 @@ def cpu_baseline(pos, rays, budget_s=12.0):
     t0 = time.perf_counter()
     tree.closest_hit(rays[:n], threads)
     dt = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {
         "value": round(n / dt / 1e6, 4),
         "unit": "Mrays/s",
         "cores": threads,
+        "host_cpus": os.cpu_count(),
+        "cpu_model": model,
         "kind": "port",
         "sample": f"first {n} of the 8M incoherent rays, FP64 restatement of BVH.intersected (recursive, "
                   f"stored child order) on the oracle's PHR tree of the same 1M triangles, "
