@@ -293,9 +293,12 @@ def test_kept_summaries_give_the_same_image_and_gradient(ctx, small, zeros, miss
     ctx.synchronize()
     kept_segments = render_segments(ctx)
     ka, ke = ka.cpu().numpy().astype(np.float64), ke.cpu().numpy().astype(np.float64)
+    # both are float32 atomic sums over the replayed paths, queued in a different
+    # order: entries where contributions cancel differ by a few float32 roundings
+    # of the largest terms; a single path's share is ~1e-3 of the largest entry
     for x, y in ((ka, ga), (ke, ge)):
         assert np.abs(y).max() > 0
-        assert np.allclose(x, y, rtol=1e-5, atol=1e-7 * np.abs(y).max())
+        assert np.allclose(x, y, rtol=1e-5, atol=1e-5 * np.abs(y).max())
     assert 0 < kept_segments < full_segments
     small.mesh.bind_params(None, None)
 

@@ -381,7 +381,8 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
                                                      BvhNode* __restrict__ nodes_all,
                                                      int32_t* __restrict__ parent_inner_all,
                                                      int32_t* __restrict__ parent_leaf_all,
-                                                     int32_t* __restrict__ inner_range_all) {
+                                                     int32_t* __restrict__ inner_range_all,
+                                                     uint32_t* __restrict__ counters) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= hdr->n_leaves) return;
   int32_t b = 0, s_id = 0, u = hdr->n_leaves;
@@ -391,8 +392,12 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
     u = u_begin[s_id + 1] - b;
   }
   const int32_t i = r - b;
-  if (i >= u - 1) return;
+  if (i >= u - 1) {
+    if (u == 1) parent_leaf_all[b] = -1;  // a one-leaf tree: its leaf is the root
+    return;
+  }
   const int32_t ib = b - s_id;  // first inner-node index of this tree
+  counters[ib + i] = 0u;        // the refit's arrival counter of this inner node
   const uint64_t* k = k_all + b;
   const uint32_t* run_start = run_start_all + b;
   BvhNode* nodes = nodes_all + ib;
@@ -687,7 +692,6 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   const size_t o_enc = plan.take(6 * std::max<int64_t>(kEncBlocks, nseg) * sizeof(unsigned long long));
   const size_t o_qbox = plan.take(6 * size_t(nseg) * sizeof(double));
   const size_t o_kb = plan.take(n * sizeof(uint64_t));
-  const size_t o_va = plan.take(n * sizeof(uint32_t));
   const size_t o_vb = plan.take(n * sizeof(uint32_t));
   const size_t o_st = plan.take(onesweep::status_words(n, key_bits, seg_passes, sort_cfg) * sizeof(uint32_t));
   const size_t o_uk = plan.take(n * sizeof(uint64_t));
@@ -705,7 +709,6 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   int32_t* u_begin = forest ? reinterpret_cast<int32_t*>(base + o_ub) : nullptr;
   const int32_t* tseg = forest ? mesh->tri_seg.p : nullptr;
   auto* keys_b = reinterpret_cast<uint64_t*>(base + o_kb);
-  auto* vals_a = reinterpret_cast<uint32_t*>(base + o_va);
   auto* vals_b = reinterpret_cast<uint32_t*>(base + o_vb);
   auto* status = reinterpret_cast<uint32_t*>(base + o_st);
   auto* ukeys = reinterpret_cast<uint64_t*>(base + o_uk);
@@ -729,17 +732,19 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   morton_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, qbox, tseg, key_mode,
                                                   bvh->keys.p);
 
-  const int where = onesweep::sort_pairs(st, bvh->keys.p, vals_a, keys_b, vals_b, n, key_bits,
+  // the values ping-pong between the permutation itself and scratch, so an
+  // even number of passes leaves the sorted order in place
+  auto* perm_u = reinterpret_cast<uint32_t*>(bvh->perm.p);
+  const int where = onesweep::sort_pairs(st, bvh->keys.p, perm_u, keys_b, vals_b, n, key_bits,
                                          status, ctx->d_err, true, tseg, seg_passes, sort_cfg);
   uint64_t* sorted_keys = where ? keys_b : bvh->keys.p;
-  uint32_t* sorted_vals = where ? vals_b : vals_a;
   if (where) {
     HIP_TRY(ctx, hipMemcpyAsync(bvh->keys.p, keys_b, n * sizeof(uint64_t),
                                 hipMemcpyDeviceToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(bvh->perm.p, vals_b, n * sizeof(uint32_t),
+                                hipMemcpyDeviceToDevice, st));
     sorted_keys = bvh->keys.p;
   }
-  HIP_TRY(ctx, hipMemcpyAsync(bvh->perm.p, sorted_vals, n * sizeof(uint32_t),
-                              hipMemcpyDeviceToDevice, st));
 
   const uint32_t* sperm = reinterpret_cast<const uint32_t*>(bvh->perm.p);
   count_heads_kernel<<<int(utiles), kUThreads, 0, st>>>(sorted_keys, n, tseg, sperm, blk);
@@ -749,13 +754,12 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   if (forest) seg_begin_kernel<<<grid_for(n, 256), 256, 0, st>>>(useg, bvh->header.p, nseg, u_begin);
   int32_t* parent_inner = bvh->parent.p;
   int32_t* parent_leaf = bvh->parent.p + n;
-  fill_async<int32_t>(st, bvh->parent.p, size_t(2 * n), -1);
+  // every link the refit follows is written here: children's, and -1 at roots
   karras_kernel<<<grid_for(n, 256), 256, 0, st>>>(ukeys, bvh->run_start.p, bvh->header.p, useg,
                                                   u_begin, bvh->nodes.p, parent_inner,
-                                                  parent_leaf, bvh->inner_range.p);
+                                                  parent_leaf, bvh->inner_range.p, counters);
   reorder_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, sperm, bvh->tris.p,
                                                    bvh->prim_box.p);
-  fill_async<uint32_t>(st, counters, size_t(n), 0u);
   refit_kernel<<<grid_for(n, kRefitThreads), kRefitThreads, 0, st>>>(
       bvh->prim_box.p, bvh->run_start.p, parent_inner, parent_leaf, bvh->inner_range.p, bvh->nodes.p,
       counters, heights, bvh->header.p, ctx->d_err, useg, forest ? bvh->seg_headers.p : nullptr, nseg);
@@ -763,7 +767,7 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
     seg_finalize_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(u_begin, bvh->run_start.p, nseg,
                                                             bvh->header.p, bvh->seg_headers.p);
   HIP_TRY(ctx, hipGetLastError());
-  return wide::widen(ctx, bvh);
+  return wide::widen(ctx, bvh, parent_inner);
 }
 
 }  // namespace lbvh

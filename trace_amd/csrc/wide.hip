@@ -50,7 +50,9 @@ __global__ void parent_kernel(const BvhNode* __restrict__ nodes, const BvhHeader
     if (b.cnt[c] == 0 && b.ref[c] >= 0 && b.ref[c] < n_inner) parent[b.ref[c]] = i;
 }
 
-// flag[i] = 1 for binary inner nodes at even depth.
+// flag[i] = 1 for binary inner nodes at even depth.  SHIFT 1: parent holds
+// encoded links, (parent << 1) | slot (-1 stays -1 under the shift).
+template <int SHIFT>
 __global__ void flag_kernel(const int32_t* __restrict__ parent, const BvhHeader* __restrict__ hdr,
                             int64_t cap, uint32_t* __restrict__ flag, uint32_t* d_err) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -60,7 +62,7 @@ __global__ void flag_kernel(const int32_t* __restrict__ parent, const BvhHeader*
     return;
   }
   int d = 0;
-  for (int32_t p = parent[i]; p >= 0; p = parent[p]) {
+  for (int32_t p = parent[i] >> SHIFT; p >= 0; p = parent[p] >> SHIFT) {
     if (++d > 4096) {  // deeper than any tree built here: a cycle
       atomicOr(d_err, TRACE_ERR_TREE);
       break;
@@ -370,7 +372,7 @@ __global__ void wroot_kernel(BvhHeader* __restrict__ hdrs, int32_t count,
 
 }  // namespace
 
-int widen(trace_ctx* ctx, trace_bvh* bvh) {
+int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links) {
   const int64_t cap = std::max<int64_t>(1, int64_t(bvh->nodes.n));
   const int32_t n_roots = bvh->nseg > 1 ? bvh->nseg : 1;
   BvhHeader* roots = bvh->nseg > 1 ? bvh->seg_headers.p : bvh->header.p;
@@ -409,9 +411,13 @@ int widen(trace_ctx* ctx, trace_bvh* bvh) {
   auto* soff = reinterpret_cast<uint32_t*>(base + o_soff);
   auto* sscr = reinterpret_cast<uint32_t*>(base + o_scan);
   const int grid = int(ceil_div(cap, 256));
-  fill_async<int32_t>(st, parent, size_t(cap), -1);
-  parent_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, parent);
-  flag_kernel<<<grid, 256, 0, st>>>(parent, bvh->header.p, cap, flag, ctx->d_err);
+  if (links) {
+    flag_kernel<1><<<grid, 256, 0, st>>>(links, bvh->header.p, cap, flag, ctx->d_err);
+  } else {
+    fill_async<int32_t>(st, parent, size_t(cap), -1);
+    parent_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, parent);
+    flag_kernel<0><<<grid, 256, 0, st>>>(parent, bvh->header.p, cap, flag, ctx->d_err);
+  }
   scan::exclusive_u32(st, flag, widx, cap, total, sscr);
   split_count_kernel<<<int(ceil_div(n_entries, 256)), 256, 0, st>>>(bvh->nodes.p, bvh->header.p, cap, roots,
                                                                     n_roots, max_levels, need);

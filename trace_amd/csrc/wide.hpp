@@ -7,5 +7,8 @@ namespace wide {
 // Enqueue the widening of bvh's binary tree (LBVH, forest or PHR layout) into
 // bvh->wnodes and set wroot in its header(s).  The binary inner-node count is
 // read on the device from the header; bvh->nodes' allocation bounds it.
-int widen(trace_ctx* ctx, trace_bvh* bvh);
+// links (optional): the binary tree's parent links as the LBVH build leaves
+// them, (parent << 1) | slot per inner node and -1 at roots; without them the
+// links are derived from the nodes.
+int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links = nullptr);
 }  // namespace wide
