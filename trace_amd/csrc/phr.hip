@@ -713,7 +713,13 @@ __global__ __launch_bounds__(B) void task_kernel(Flat f, const double* __restric
   const int tid = threadIdx.x;
   const Task t = tasks[blockIdx.x];
   const int m = t.cnt;
-  const int P = m <= lds_p ? lds_p : 0;
+  // bitonic size of this job: its own cut rounded up to a power of two (the
+  // level's LDS holds the largest), or 0 to sort by ranking
+  int P = 0;
+  if (m <= lds_p) {
+    P = 128;
+    while (P < m) P <<= 1;
+  }
   double* lkey = s_dyn;
   int32_t* lpos = reinterpret_cast<int32_t*>(s_dyn + lds_p);
   const int32_t* ids = pool + t.off;
