@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from trace_amd import KEY_M30, KEY_REF63, scenes
-from trace_amd.device import Bvh, Mesh
+from trace_amd.device import Bvh, Mesh, set_tune
 
 pytestmark = pytest.mark.gpu
 
@@ -80,6 +80,29 @@ def test_tie_goes_to_reference_winner(ctx, oracle):
     hit, _ = bvh.closest_hit(ray)
     ohit, _ = oracle.Tree(pos).closest_hit(ray)
     assert hit[0] == ohit[0] == 2  # last in sorted (stable) order
+
+
+@pytest.mark.parametrize("threads", [64, 256])
+def test_ray_counts_around_the_claim_size(ctx, oracle, threads):
+    # The persistent kernel hands rays to its waves in chunks of 64 consecutive
+    # indices.  The full batch (a ragged last chunk) meets the oracle bar, and
+    # every prefix shorter than, equal to or just past one or two chunks gives
+    # bit for bit the full batch's results for its rays.
+    print("traverse ray counts around the chunk size", threads, flush=True)
+    pos = scenes.random_triangles(3000, seed=31)
+    bvh = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    rays = scenes.incoherent_rays(4097, seed=32)
+    set_tune(ctx, trav_threads=threads, trav_persistent=True)
+    try:
+        full_hit, full_t = bvh.closest_hit(rays)
+        ohit, ot = oracle.Tree(pos).closest_hit(rays)
+        _check(full_hit, full_t, ohit, ot, min_hits=0.0)
+        for n in (1, 63, 64, 65, 127, 128, 129):
+            hit, t = bvh.closest_hit(rays[:n])
+            assert np.array_equal(hit, full_hit[:n]), n
+            assert np.array_equal(t, full_t[:n]), n
+    finally:
+        set_tune(ctx)
 
 
 def test_empty_ray_batch(ctx):
