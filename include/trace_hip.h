@@ -10,6 +10,8 @@
  *   - Every call returns an int status: TRACE_OK (0) or a negative TRACE_E_*;
  *     on failure trace_last_error(ctx) holds a message.  (The reference panics
  *     on errors; the Go shim turns a non-zero status into a panic.)
+ *     Host memory exhaustion inside a call (a C++ allocation failure)
+ *     is not turned into a status: it terminates the process.
  *   - Plain pointers and sizes only.  Pointers named h_* are host memory, d_*
  *     are device memory.  Host inputs are copied during the call and never
  *     retained.  Host-pointer calls are synchronous; *_device calls are
@@ -173,7 +175,8 @@ int trace_phr_optimize_grid(trace_ctx* ctx, const trace_bvh* aux, const double* 
  * changkun.de/x/bo v0.0.1, is replaced by that function).  h_trace (optional,
  * (rounds + 2) x 5 doubles) receives per row alpha, delta, Cost, build cost, objective:
  * the two normaliser builds (objective NaN), then the evaluations in order.  Writes the
- * evaluated (alpha, delta) of the smallest objective (first minimum). */
+ * evaluated (alpha, delta) of the smallest objective (first minimum).  Returns
+ * TRACE_E_INVALID when either normaliser is zero (the reference's objective is then NaN). */
 int trace_phr_optimize_bayes(trace_ctx* ctx, const trace_bvh* aux, const double* alpha_range,
                              const double* delta_range, int branching, int32_t max_cut, int32_t frame_width,
                              int32_t frame_height, int32_t rounds, int32_t random_rounds, uint64_t seed,

@@ -305,6 +305,10 @@ int trace_phr_optimize_bayes(trace_ctx* ctx, const trace_bvh* aux, const double*
   if (rc) return rc;
   rc = phr_eval(ctx, aux, alpha_range[1], delta_range[1], branching, max_cut, &sah_hi, &o.max_build);
   if (rc) return rc;
+  // evalPHR divides by both normalisers; the reference's objective is NaN
+  // everywhere when either is zero, so the search is refused up front
+  if (o.max_build <= 0 || !(o.max_sah > 0.0))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_optimize_bayes: zero build or SAH cost at the range ends");
   o.rows.insert(o.rows.end(), {alpha_range[0], delta_range[0], o.max_sah, double(build_lo), NAN});
   o.rows.insert(o.rows.end(), {alpha_range[1], delta_range[1], sah_hi, double(o.max_build), NAN});
   const double r = double(aux->n_prims) / 2000000.0;  // omega (optimizer.go:88-90)
