@@ -70,7 +70,7 @@ def cpu_baseline(pos, rays, budget_s=12.0):
     the same rays, using the host's cores."""
     from oracle import pyoracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, os.cpu_count() or 1)
     tree = pyoracle.PhrTree(pyoracle.Tree(pos))
     probe = 20_000
     t0 = time.perf_counter()
@@ -92,8 +92,9 @@ def cpu_baseline(pos, rays, budget_s=12.0):
         "cores": threads,
         "host_cpus": os.cpu_count(),
         "cpu_model": model,
-        "kind": "port",
-        "sample": f"first {n} of the 8M incoherent rays, FP64 restatement of BVH.intersected (recursive, "
+        "kind": "restatement",
+        "sample": f"first {n} of the 8M incoherent rays, FP64 C++ restatement (not the Go path) of "
+                  f"BVH.intersected (recursive, "
                   f"stored child order) on the oracle's PHR tree of the same 1M triangles, "
                   f"{threads} threads, {dt:.1f} s",
     }

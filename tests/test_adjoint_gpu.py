@@ -253,16 +253,30 @@ def test_inverse_rendering_lowers_the_loss(ctx, small):
     assert losses[-1] < 0.8 * losses[0], losses
 
 
-@pytest.mark.parametrize("zeros,miss", [(False, _lib.MISS_DEFAULT), (True, _lib.MISS_SKY)])
-def test_kept_summaries_give_the_same_image_and_gradient(ctx, small, zeros, miss):
+@pytest.mark.parametrize("zeros,miss,batch", [(False, _lib.MISS_DEFAULT, None), (True, _lib.MISS_SKY, None),
+                                               (True, _lib.MISS_SKY, 2 * W * H)])
+def test_kept_summaries_give_the_same_image_and_gradient(ctx, small, zeros, miss, batch):
     """trace_render_keep derives the image from the backward's summary sweep:
     it equals trace_render_device's bit for bit (zero albedo channels exercise
     the zero counts; the sky makes every escaping path radiant).  The backward
     from the kept summaries traces only the replay and gives the same gradient
-    as the self-contained backward, up to the order of its atomic sums."""
+    as the self-contained backward, up to the order of its atomic sums.  With
+    a wavefront batch of 2 W H paths the 5 samples run as batches of 2 + 2 + 1,
+    so the keep buffer's sample-major offsets of the later batches are used."""
+    from trace_amd.device import set_tune
+
+    print(f"adjoint: kept summaries zeros={zeros} miss={miss} batch={batch}", flush=True)
+    if batch is not None:
+        set_tune(ctx, render_batch=batch)
+    try:
+        _kept_summaries_case(ctx, small, zeros, miss)
+    finally:
+        set_tune(ctx)
+
+
+def _kept_summaries_case(ctx, small, zeros, miss):
     from trace_amd.device import render_backward_kept, render_keep
 
-    print(f"adjoint: kept summaries zeros={zeros} miss={miss}", flush=True)
     d = small.d
     alb = d["albedo"].copy()
     if zeros:

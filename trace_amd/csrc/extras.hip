@@ -87,6 +87,12 @@ int trace_render_batched(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh
   if (prm->width < 2 || prm->height < 2 || prm->spp < 0 || prm->max_depth < 0 || prm->sample_offset < 0 ||
       prm->miss < TRACE_MISS_DEFAULT || prm->miss > TRACE_MISS_SUN)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_batched: bad render parameters");
+  if (int64_t(prm->width) * prm->height > (int64_t(1) << 28))
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_batched: image too large");
+  // one sample of every scene must fit a wavefront batch (int32 path ids)
+  if (int64_t(prm->width) * prm->height * nscene > (int64_t(1) << 30))
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_batched: %d scenes of %dx%d exceed 2^30 paths per sample",
+               nscene, prm->width, prm->height);
   std::vector<trace_bvh> views(nscene);
   std::vector<const trace_bvh*> trees(nscene);
   for (int32_t c = 0; c < nscene; ++c) {

@@ -823,6 +823,10 @@ int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const
   const int64_t per_sample = wh * nscene;
   w.spb = std::max<int64_t>(1, std::min<int64_t>(prm->spp, ctx->tune.render_batch / per_sample));
   const int64_t max_paths = w.spb * per_sample;
+  // queues hold int32 path ids and uint32 counts
+  if (max_paths > (int64_t(1) << 30))
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "render: a batch of %lld paths exceeds the 2^30 the ray queues index",
+               static_cast<long long>(max_paths));
   hipStream_t st = ctx->stream;
 
   ScratchPlan plan;

@@ -216,6 +216,10 @@ static void dispatch(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, i
 int closest_hit(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays, int64_t n,
                 int32_t* d_hit, float* d_t, int32_t* d_nodes, int32_t* d_tris) {
   if (n == 0) return TRACE_OK;
+  // persistent lanes claim rays through a 32-bit counter, and hit ids are int32
+  if (n > int64_t(INT32_MAX))
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "closest hit: %lld rays exceed the 2^31 - 1 one call takes",
+               static_cast<long long>(n));
   if (bvh->depth_bound() > 255)
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "closest hit: tree depth %d exceeds 255", bvh->depth_bound());
   if (!bvh->base && bvh->nseg > 1)
