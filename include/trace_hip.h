@@ -108,6 +108,19 @@ int trace_lbvh_build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enc
  * The per-frame rebuild of cmd/dynamic/main.go:113-117. */
 int trace_lbvh_rebuild(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing,
                        int key_mode, trace_bvh* bvh);
+/* trace_lbvh_rebuild that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Its scratch is a fixed array of
+ * 64 MiB that the library holds on each device until the process ends (about 1.97M
+ * triangles of one mesh at the default key sort); larger meshes are refused with
+ * TRACE_E_UNSUPPORTED before anything is enqueued.  The tree's own arrays must
+ * already be sized: bvh was built from a mesh of the same shape with
+ * trace_lbvh_build (else TRACE_E_INVALID).  h_enclosing is passed to the device by
+ * value (no host-memory copy).  Asynchronous on the context stream.  Every call on a
+ * device shares that device's array, which keeps the last call's contents: calls
+ * that overlap on one device (several contexts or streams) race. */
+int trace_lbvh_rebuild_noalloc(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing,
+                               int key_mode, trace_bvh* bvh);
 int trace_bvh_destroy(trace_bvh* bvh);
 
 typedef struct {

@@ -70,6 +70,7 @@ SIGNATURES = {
     "trace_mesh_destroy": [_P],
     "trace_lbvh_build": [_P, _P, _P, C.c_int, C.POINTER(_P)],
     "trace_lbvh_rebuild": [_P, _P, _P, C.c_int, _P],
+    "trace_lbvh_rebuild_noalloc": [_P, _P, _P, C.c_int, _P],
     "trace_bvh_destroy": [_P],
     "trace_bvh_get_info": [_P, _P, C.POINTER(BvhInfo)],
     "trace_bvh_export_sorted": [_P, _P, _P, _P],

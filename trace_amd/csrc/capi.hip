@@ -107,6 +107,8 @@ int trace_ctx_create(int device, trace_ctx** out) {
     return TRACE_E_HIP;
   }
   if (ctx->n_cu < 1) ctx->n_cu = 1;
+  // a failure is kept and reported by trace_lbvh_rebuild_noalloc, the only user
+  (void)noalloc_init_device(device);
   ctx->stream = ctx->own_stream;
   *out = ctx;
   return TRACE_OK;

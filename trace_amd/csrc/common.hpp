@@ -83,6 +83,7 @@ struct DevArray {
     p = nullptr;
     n = 0;
   }
+  bool fits(size_t count) const { return p && count <= n; }
   hipError_t alloc(size_t count) {
     if (count <= n && p) return hipSuccess;
     release();
@@ -105,6 +106,52 @@ struct ScratchPlan {
 };
 int ctx_reserve_scratch(trace_ctx* ctx, size_t bytes);
 int ctx_check_device_error(trace_ctx* ctx, const char* what);
+// Takes (once per device, with `device` current) the device address of the
+// fixed scratch array that trace_lbvh_rebuild_noalloc uses.
+hipError_t noalloc_init_device(int device);
+
+// Where a build takes its scratch memory from.  The default (base == nullptr)
+// is the context's grow-only arena; a fixed workspace is a caller-provided
+// device buffer of `bytes` that is never grown, and then the build also
+// refuses to (re)allocate any output array of its tree: every call that
+// would need device memory fails instead of allocating.
+struct Workspace {
+  char* base = nullptr;
+  size_t bytes = 0;
+  bool fixed() const { return base != nullptr; }
+};
+
+// Base address of `bytes` of scratch: the context arena (grown if needed) or
+// the fixed workspace (TRACE_E_UNSUPPORTED when it is too small).
+inline int workspace_base(trace_ctx* ctx, const Workspace* ws, size_t bytes, char** out, const char* what) {
+  if (ws && ws->fixed()) {
+    if (bytes > ws->bytes)
+      TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "%s: needs %zu bytes of scratch, the fixed workspace holds %zu", what,
+                 bytes, ws->bytes);
+    *out = ws->base;
+    return TRACE_OK;
+  }
+  const int rc = ctx_reserve_scratch(ctx, bytes);
+  if (rc) return rc;
+  *out = static_cast<char*>(ctx->scratch);
+  return TRACE_OK;
+}
+
+// Sizes a tree's output array: allocates (grow-only) with the default
+// workspace; with a fixed one the array must already hold `count` entries.
+#define TRACE_ENSURE(ctx, ws, arr, count)                                                              \
+  do {                                                                                                \
+    const size_t _c = size_t(count);                                                                  \
+    if ((ws) && (ws)->fixed()) {                                                                      \
+      if (!(arr).fits(_c))                                                                            \
+        TRACE_FAIL(ctx, TRACE_E_INVALID,                                                              \
+                   "no-allocation build: array %s holds %zu entries, %zu needed (build the tree once " \
+                   "with an allocating call first)",                                                  \
+                   #arr, (arr).n, _c);                                                                \
+    } else {                                                                                          \
+      HIP_TRY(ctx, (arr).alloc(_c));                                                                  \
+    }                                                                                                 \
+  } while (0)
 // Sets mesh->has_spheres from its device positions (synchronous); fails on a malformed sphere.
 int mesh_scan_prims(trace_ctx* ctx, trace_mesh* mesh, const char* what);
 

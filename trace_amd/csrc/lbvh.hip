@@ -654,70 +654,109 @@ __global__ void seg_finalize_kernel(const int32_t* __restrict__ u_begin,
 
 static int grid_for(int64_t n, int threads) { return int(std::max<int64_t>(1, ceil_div(n, threads))); }
 
+namespace {
+
+// Scratch layout of build(): enclosing words, qbox, sort ping-pong buffers,
+// sort status, unique keys, per-tile head counts, refit counters and heights.
+struct LbvhPlan {
+  size_t o_enc, o_qbox, o_kb, o_vb, o_st, o_uk, o_blk, o_cnt, o_hgt, o_useg, o_ub, total;
+  LbvhPlan(int64_t n, int32_t nseg, int key_bits, int seg_passes, const onesweep::Config& cfg) {
+    const bool forest = nseg > 1;
+    const int64_t utiles = ceil_div(n, kUTile);
+    ScratchPlan plan;
+    o_enc = plan.take(6 * std::max<int64_t>(kEncBlocks, nseg) * sizeof(unsigned long long));
+    o_qbox = plan.take(6 * size_t(nseg) * sizeof(double));
+    o_kb = plan.take(n * sizeof(uint64_t));
+    o_vb = plan.take(n * sizeof(uint32_t));
+    o_st = plan.take(onesweep::status_words(n, key_bits, seg_passes, cfg) * sizeof(uint32_t));
+    o_uk = plan.take(n * sizeof(uint64_t));
+    o_blk = plan.take(utiles * sizeof(uint32_t));
+    o_cnt = plan.take(std::max<int64_t>(1, n) * sizeof(uint32_t));
+    o_hgt = plan.take(2 * std::max<int64_t>(1, n) * sizeof(int32_t));
+    o_useg = plan.take(forest ? n * sizeof(int32_t) : 0);
+    o_ub = plan.take(forest ? (size_t(nseg) + 1) * sizeof(int32_t) : 0);
+    total = plan.off;
+  }
+};
+
+int seg_passes_for(int32_t nseg) { return nseg <= 1 ? 0 : nseg <= 256 ? 1 : nseg <= 65536 ? 2 : 3; }
+
+struct QBox {
+  double v[6];
+};
+
+// Writes a caller's enclosing box passed by value (no host-memory copy).
+__global__ void qbox_kernel(QBox b, double* __restrict__ qbox) {
+  if (threadIdx.x < 6) qbox[threadIdx.x] = b.v[threadIdx.x];
+}
+
+}  // namespace
+
+size_t scratch_bytes(const trace_ctx* ctx, const trace_mesh* mesh, int key_mode) {
+  const onesweep::Config cfg{ctx->tune.sort_bits, ctx->tune.sort_kpt};
+  const int key_bits = key_mode == TRACE_KEY_M30 ? 30 : 63;
+  return LbvhPlan(mesh->ntri, mesh->nseg, key_bits, seg_passes_for(mesh->nseg), cfg).total;
+}
+
 int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int key_mode,
-          trace_bvh* bvh) {
+          trace_bvh* bvh, const Workspace* ws) {
   const int64_t n = mesh->ntri;
   hipStream_t st = ctx->stream;
   const int key_bits = key_mode == TRACE_KEY_M30 ? 30 : 63;
   const onesweep::Config sort_cfg{ctx->tune.sort_bits, ctx->tune.sort_kpt};
   const int32_t nseg = mesh->nseg;
   const bool forest = nseg > 1;
-  const int seg_passes = !forest ? 0 : nseg <= 256 ? 1 : nseg <= 65536 ? 2 : 3;
+  const int seg_passes = seg_passes_for(nseg);
   if (forest && h_enclosing)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "LBVH: a segmented mesh takes no caller enclosing box");
   if (nseg > (1 << 24))
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "LBVH: %d segments exceed the 2^24 the sort's segment digits cover", nseg);
+  const LbvhPlan plan(n, nseg, key_bits, seg_passes, sort_cfg);
+  if (ws && ws->fixed() && plan.total > ws->bytes)  // refused before anything changes
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "LBVH: %lld triangles need %zu bytes of scratch, the fixed workspace holds %zu",
+               static_cast<long long>(n), plan.total, ws->bytes);
 
+  TRACE_ENSURE(ctx, ws, bvh->keys, n);
+  TRACE_ENSURE(ctx, ws, bvh->perm, n);
+  TRACE_ENSURE(ctx, ws, bvh->tris, 9 * n);
+  TRACE_ENSURE(ctx, ws, bvh->prim_box, 6 * n);
+  TRACE_ENSURE(ctx, ws, bvh->nodes, std::max<int64_t>(1, n - 1));
+  TRACE_ENSURE(ctx, ws, bvh->parent, 2 * n);
+  TRACE_ENSURE(ctx, ws, bvh->run_start, n + 1);
+  TRACE_ENSURE(ctx, ws, bvh->inner_range, 2 * std::max<int64_t>(1, n - 1));
+  TRACE_ENSURE(ctx, ws, bvh->header, 1);
+  if (forest) TRACE_ENSURE(ctx, ws, bvh->seg_headers, nseg);
   bvh->n_prims = n;
   bvh->has_spheres = mesh->has_spheres;
   bvh->key_mode = key_mode;
   bvh->arity = 2;
   bvh->nseg = nseg;
   bvh->seg_off_h = mesh->seg_off_h;
-  if (forest) HIP_TRY(ctx, bvh->seg_headers.alloc(nseg));
-  HIP_TRY(ctx, bvh->keys.alloc(n));
-  HIP_TRY(ctx, bvh->perm.alloc(n));
-  HIP_TRY(ctx, bvh->tris.alloc(9 * n));
-  HIP_TRY(ctx, bvh->prim_box.alloc(6 * n));
-  HIP_TRY(ctx, bvh->nodes.alloc(std::max<int64_t>(1, n - 1)));
-  HIP_TRY(ctx, bvh->parent.alloc(2 * n));
-  HIP_TRY(ctx, bvh->run_start.alloc(n + 1));
-  HIP_TRY(ctx, bvh->inner_range.alloc(2 * std::max<int64_t>(1, n - 1)));
-  HIP_TRY(ctx, bvh->header.alloc(1));
 
-  // scratch: enclosing words, qbox, sort ping-pong buffers, sort status,
-  // unique keys, per-tile head counts, refit counters and heights
-  const int64_t utiles = ceil_div(n, kUTile);
-  ScratchPlan plan;
-  const size_t o_enc = plan.take(6 * std::max<int64_t>(kEncBlocks, nseg) * sizeof(unsigned long long));
-  const size_t o_qbox = plan.take(6 * size_t(nseg) * sizeof(double));
-  const size_t o_kb = plan.take(n * sizeof(uint64_t));
-  const size_t o_vb = plan.take(n * sizeof(uint32_t));
-  const size_t o_st = plan.take(onesweep::status_words(n, key_bits, seg_passes, sort_cfg) * sizeof(uint32_t));
-  const size_t o_uk = plan.take(n * sizeof(uint64_t));
-  const size_t o_blk = plan.take(utiles * sizeof(uint32_t));
-  const size_t o_cnt = plan.take(std::max<int64_t>(1, n) * sizeof(uint32_t));
-  const size_t o_hgt = plan.take(2 * std::max<int64_t>(1, n) * sizeof(int32_t));
-  const size_t o_useg = plan.take(forest ? n * sizeof(int32_t) : 0);
-  const size_t o_ub = plan.take(forest ? (size_t(nseg) + 1) * sizeof(int32_t) : 0);
-  int rc = ctx_reserve_scratch(ctx, plan.off);
+  char* base = nullptr;
+  int rc = workspace_base(ctx, ws, plan.total, &base, "LBVH");
   if (rc) return rc;
-  char* base = static_cast<char*>(ctx->scratch);
-  auto* enc = reinterpret_cast<unsigned long long*>(base + o_enc);
-  auto* qbox = reinterpret_cast<double*>(base + o_qbox);
-  int32_t* useg = forest ? reinterpret_cast<int32_t*>(base + o_useg) : nullptr;
-  int32_t* u_begin = forest ? reinterpret_cast<int32_t*>(base + o_ub) : nullptr;
+  auto* enc = reinterpret_cast<unsigned long long*>(base + plan.o_enc);
+  auto* qbox = reinterpret_cast<double*>(base + plan.o_qbox);
+  int32_t* useg = forest ? reinterpret_cast<int32_t*>(base + plan.o_useg) : nullptr;
+  int32_t* u_begin = forest ? reinterpret_cast<int32_t*>(base + plan.o_ub) : nullptr;
   const int32_t* tseg = forest ? mesh->tri_seg.p : nullptr;
-  auto* keys_b = reinterpret_cast<uint64_t*>(base + o_kb);
-  auto* vals_b = reinterpret_cast<uint32_t*>(base + o_vb);
-  auto* status = reinterpret_cast<uint32_t*>(base + o_st);
-  auto* ukeys = reinterpret_cast<uint64_t*>(base + o_uk);
-  auto* blk = reinterpret_cast<uint32_t*>(base + o_blk);
-  auto* counters = reinterpret_cast<uint32_t*>(base + o_cnt);
-  auto* heights = reinterpret_cast<int32_t*>(base + o_hgt);
+  auto* keys_b = reinterpret_cast<uint64_t*>(base + plan.o_kb);
+  auto* vals_b = reinterpret_cast<uint32_t*>(base + plan.o_vb);
+  auto* status = reinterpret_cast<uint32_t*>(base + plan.o_st);
+  auto* ukeys = reinterpret_cast<uint64_t*>(base + plan.o_uk);
+  auto* blk = reinterpret_cast<uint32_t*>(base + plan.o_blk);
+  auto* counters = reinterpret_cast<uint32_t*>(base + plan.o_cnt);
+  auto* heights = reinterpret_cast<int32_t*>(base + plan.o_hgt);
+  const int64_t utiles = ceil_div(n, kUTile);
 
-  if (h_enclosing) HIP_TRY(ctx, hipMemcpyAsync(qbox, h_enclosing, 6 * sizeof(double),
-                                               hipMemcpyHostToDevice, st));
+  if (h_enclosing && ws && ws->fixed()) {
+    QBox b;
+    for (int k = 0; k < 6; ++k) b.v[k] = h_enclosing[k];
+    qbox_kernel<<<1, 64, 0, st>>>(b, qbox);
+  } else if (h_enclosing) {
+    HIP_TRY(ctx, hipMemcpyAsync(qbox, h_enclosing, 6 * sizeof(double), hipMemcpyHostToDevice, st));
+  }
   if (!forest) {
     const int enc_blocks = int(std::min<int64_t>(grid_for(n, 256), kEncBlocks));
     enclosing_kernel<<<enc_blocks, 256, 0, st>>>(mesh->pos.p, n, enc);
@@ -767,7 +806,7 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
     seg_finalize_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(u_begin, bvh->run_start.p, nseg,
                                                             bvh->header.p, bvh->seg_headers.p);
   HIP_TRY(ctx, hipGetLastError());
-  return wide::widen(ctx, bvh, parent_inner);
+  return wide::widen(ctx, bvh, parent_inner, ws);
 }
 
 }  // namespace lbvh

@@ -372,15 +372,46 @@ __global__ void wroot_kernel(BvhHeader* __restrict__ hdrs, int32_t count,
 
 }  // namespace
 
-int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links) {
+namespace {
+
+// Scratch layout of widen(): parent links, wide flags and indices of the
+// binary inner nodes, two totals, then per entry (inner nodes and roots) the
+// sub-tree sizes of split leaves, their offsets and the scan's partials.
+struct WidePlan {
+  size_t o_par, o_flag, o_idx, o_tot, o_need, o_soff, o_scan, total;
+  WidePlan(int64_t cap, int32_t n_roots) {
+    const int64_t n_entries = cap + n_roots;
+    ScratchPlan plan;
+    o_par = plan.take(cap * sizeof(int32_t));
+    o_flag = plan.take(cap * sizeof(uint32_t));
+    o_idx = plan.take(cap * sizeof(uint32_t));
+    o_tot = plan.take(2 * sizeof(uint32_t));
+    o_need = plan.take(n_entries * sizeof(uint32_t));
+    o_soff = plan.take(n_entries * sizeof(uint32_t));
+    o_scan = plan.take(scan::scratch_words(n_entries) * sizeof(uint32_t));
+    total = plan.off;
+  }
+};
+
+}  // namespace
+
+size_t scratch_bytes(int64_t inner_cap, int32_t n_roots) {
+  return WidePlan(std::max<int64_t>(1, inner_cap), std::max<int32_t>(1, n_roots)).total;
+}
+
+// A sub-tree of h levels has (4^h - 1) / 3 < 4 n / (3 kSplitLeaf) nodes
+// (h is the fewest levels with ceil(n / 4^h) <= kSplitLeaf, so
+// n > kSplitLeaf 4^(h-1)), and split leaves hold at most n_prims triangles.
+int64_t wide_nodes(int64_t inner_cap, int64_t n_prims) {
+  return std::max<int64_t>(1, inner_cap) + 4 * n_prims / (3 * kSplitLeaf) + 1;
+}
+
+int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links, const Workspace* ws) {
   const int64_t cap = std::max<int64_t>(1, int64_t(bvh->nodes.n));
   const int32_t n_roots = bvh->nseg > 1 ? bvh->nseg : 1;
   BvhHeader* roots = bvh->nseg > 1 ? bvh->seg_headers.p : bvh->header.p;
   const int64_t n_entries = cap + n_roots;
-  // A sub-tree of h levels has (4^h - 1) / 3 < 4 n / (3 kSplitLeaf) nodes
-  // (h is the fewest levels with ceil(n / 4^h) <= kSplitLeaf, so
-  // n > kSplitLeaf 4^(h-1)), and split leaves hold at most n_prims triangles.
-  const int64_t split_cap = 4 * bvh->n_prims / (3 * kSplitLeaf) + 1;
+  const int64_t split_cap = wide_nodes(cap, bvh->n_prims) - cap;
   if (cap + split_cap > int64_t(INT32_MAX))
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "widen: %lld wide nodes exceed the 32-bit index",
                static_cast<long long>(cap + split_cap));
@@ -391,25 +422,18 @@ int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links) {
   const int stack = depth > 63 ? 512 : 128;
   const int max_levels = std::min(15, std::max(10, stack / 3 - 1 - std::max(0, (depth - 1) / 2)));
   hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, bvh->wnodes.alloc(size_t(cap + split_cap)));
-  ScratchPlan plan;
-  const size_t o_par = plan.take(cap * sizeof(int32_t));
-  const size_t o_flag = plan.take(cap * sizeof(uint32_t));
-  const size_t o_idx = plan.take(cap * sizeof(uint32_t));
-  const size_t o_tot = plan.take(2 * sizeof(uint32_t));
-  const size_t o_need = plan.take(n_entries * sizeof(uint32_t));
-  const size_t o_soff = plan.take(n_entries * sizeof(uint32_t));
-  const size_t o_scan = plan.take(scan::scratch_words(n_entries) * sizeof(uint32_t));
-  int rc = ctx_reserve_scratch(ctx, plan.off);
+  TRACE_ENSURE(ctx, ws, bvh->wnodes, size_t(cap + split_cap));
+  const WidePlan plan(cap, n_roots);
+  char* base = nullptr;
+  int rc = workspace_base(ctx, ws, plan.total, &base, "widen");
   if (rc) return rc;
-  char* base = static_cast<char*>(ctx->scratch);
-  auto* parent = reinterpret_cast<int32_t*>(base + o_par);
-  auto* flag = reinterpret_cast<uint32_t*>(base + o_flag);
-  auto* widx = reinterpret_cast<uint32_t*>(base + o_idx);
-  auto* total = reinterpret_cast<uint32_t*>(base + o_tot);  // [0] binary-derived, [1] sub-tree nodes
-  auto* need = reinterpret_cast<uint32_t*>(base + o_need);
-  auto* soff = reinterpret_cast<uint32_t*>(base + o_soff);
-  auto* sscr = reinterpret_cast<uint32_t*>(base + o_scan);
+  auto* parent = reinterpret_cast<int32_t*>(base + plan.o_par);
+  auto* flag = reinterpret_cast<uint32_t*>(base + plan.o_flag);
+  auto* widx = reinterpret_cast<uint32_t*>(base + plan.o_idx);
+  auto* total = reinterpret_cast<uint32_t*>(base + plan.o_tot);  // [0] binary-derived, [1] sub-tree nodes
+  auto* need = reinterpret_cast<uint32_t*>(base + plan.o_need);
+  auto* soff = reinterpret_cast<uint32_t*>(base + plan.o_soff);
+  auto* sscr = reinterpret_cast<uint32_t*>(base + plan.o_scan);
   const int grid = int(ceil_div(cap, 256));
   if (links) {
     flag_kernel<1><<<grid, 256, 0, st>>>(links, bvh->header.p, cap, flag, ctx->d_err);

@@ -10,5 +10,10 @@ namespace wide {
 // links (optional): the binary tree's parent links as the LBVH build leaves
 // them, (parent << 1) | slot per inner node and -1 at roots; without them the
 // links are derived from the nodes.
-int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links = nullptr);
+// ws: as lbvh::build's (a fixed workspace allocates nothing).
+int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links = nullptr, const Workspace* ws = nullptr);
+// Scratch bytes widen() takes for a binary tree of `inner_cap` inner-node
+// slots, `n_roots` trees and `n_prims` triangles, and the wide nodes it writes.
+size_t scratch_bytes(int64_t inner_cap, int32_t n_roots);
+int64_t wide_nodes(int64_t inner_cap, int64_t n_prims);
 }  // namespace wide

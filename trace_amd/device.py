@@ -263,6 +263,15 @@ class Bvh:
         self.ctx.check(self.ctx.lib.trace_lbvh_rebuild(self.ctx.h, mesh.h, None if enc is None else _ptr(enc),
                                                        key_mode, self.h))
 
+    def rebuild_noalloc(self, mesh: Mesh, enclosing=None, key_mode: int = _lib.KEY_REF63,
+                        ctx: Context | None = None) -> None:
+        """trace_lbvh_rebuild_noalloc (asynchronous): the rebuild with the
+        library's fixed scratch array, on `ctx` (default: the tree's context)."""
+        c = ctx or self.ctx
+        enc = None if enclosing is None else np.ascontiguousarray(enclosing, dtype=np.float64)
+        self._enc = enc
+        c.check(c.lib.trace_lbvh_rebuild_noalloc(c.h, mesh.h, None if enc is None else _ptr(enc), key_mode, self.h))
+
     def close(self) -> None:
         if self.h:
             self.ctx.lib.trace_bvh_destroy(self.h)
