@@ -528,6 +528,15 @@ struct Phr {
   std::vector<int64_t> size;    // LBVH subtree sizes (bvh.go:159-167)
   int64_t split_work = 0;       // BuildWithCost's counter (phr.go:221)
   int64_t max_split = 0;        // largest cut handed to SweepSAH
+  // Optional record of every SweepSAH call, for tests: per call the ints
+  // m, split, nl, nr, the number of cuts the job holds and the index of the
+  // one split, each cut's size, then (first, last) sorted-position range of each member
+  // (-1, -1 for a PHR-made leaf), the members taken left and right (indices
+  // into the member list, in the order the split cuts hold them); the doubles
+  // hold the cut's bounding box, then each member's box.
+  bool tracing = false;
+  std::vector<int32_t> trace_i;
+  std::vector<double> trace_d;
 
   const Box& box_of(Ref r) const { return r.made ? made[r.id].box : aux->nodes[r.id].box; }
   bool is_leaf(Ref r) const { return r.made || aux->nodes[r.id].leaf; }
@@ -667,6 +676,38 @@ struct Phr {
     right.bounding = b.rbox;
   }
 
+  void record(const std::vector<Cut>& cuts, size_t mi, bool split, const Cut& l, const Cut& r) {
+    const Cut& cut = cuts[mi];
+    const int32_t m = int32_t(cut.nodes.size());
+    auto index_of = [&](const Ref& x) {
+      for (int32_t k = 0; k < m; ++k)
+        if (cut.nodes[k].id == x.id && cut.nodes[k].made == x.made) return k;
+      return int32_t(-1);
+    };
+    trace_i.push_back(m);
+    trace_i.push_back(split ? 1 : 0);
+    trace_i.push_back(split ? int32_t(l.nodes.size()) : 0);
+    trace_i.push_back(split ? int32_t(r.nodes.size()) : 0);
+    trace_i.push_back(int32_t(cuts.size()));
+    trace_i.push_back(int32_t(mi));
+    for (const Cut& c : cuts) trace_i.push_back(int32_t(c.nodes.size()));
+    for (const Ref& x : cut.nodes) {
+      trace_i.push_back(x.made ? -1 : aux->nodes[x.id].first);
+      trace_i.push_back(x.made ? -1 : aux->nodes[x.id].last);
+    }
+    if (split) {
+      for (const Ref& x : l.nodes) trace_i.push_back(index_of(x));
+      for (const Ref& x : r.nodes) trace_i.push_back(index_of(x));
+    }
+    for (int a = 0; a < 3; ++a) trace_d.push_back(cut.bounding.lo[a]);
+    for (int a = 0; a < 3; ++a) trace_d.push_back(cut.bounding.hi[a]);
+    for (const Ref& x : cut.nodes) {
+      const Box& b = box_of(x);
+      for (int a = 0; a < 3; ++a) trace_d.push_back(b.lo[a]);
+      for (int a = 0; a < 3; ++a) trace_d.push_back(b.hi[a]);
+    }
+  }
+
   int32_t attach(Ref r) {
     PNode p;
     if (r.made) {
@@ -692,7 +733,9 @@ struct Phr {
       split_work += int64_t(cuts[mi].nodes.size());
       max_split = std::max<int64_t>(max_split, int64_t(cuts[mi].nodes.size()));
       Cut l, r;
-      if (sweep_sah(cuts[mi], l, r)) {
+      const bool split = sweep_sah(cuts[mi], l, r);
+      if (tracing) record(cuts, mi, split, l, r);
+      if (split) {
         Cut rl = refined(l), rr = refined(r);
         cuts[mi] = rl;
         cuts.push_back(rr);
@@ -750,6 +793,9 @@ struct PhrTree {
   int64_t split_work;
   int64_t initial_cut;
   int64_t max_split;
+  std::vector<int32_t> trace_i;
+  std::vector<double> trace_d;
+  std::vector<int32_t> cut_ranges;  // the initial cut's members: (first, last) each
 };
 
 double phr_cost(const PhrTree& t, const Tree& aux, int32_t id);
@@ -777,9 +823,10 @@ extern "C" {
 typedef struct PhrTree oracle_phr;
 
 // PhrBuilder.BuildFromAuxilary (phr.go:47-84) on the oracle's LBVH.
-oracle_phr* oracle_phr_build(const oracle_tree* aux, double alpha, double delta, int branching,
-                             int64_t max_cut) {
+static oracle_phr* phr_build(const oracle_tree* aux, double alpha, double delta, int branching, int64_t max_cut,
+                             bool tracing) {
   Phr b;
+  b.tracing = tracing;
   b.aux = aux;
   b.alpha = alpha;
   b.delta = delta;
@@ -791,9 +838,40 @@ oracle_phr* oracle_phr_build(const oracle_tree* aux, double alpha, double delta,
   const int64_t n_cut = int64_t(cut.nodes.size());
   b.split_work = n_cut;  // BuildWithCost counts the initial cut (phr.go:160)
   // The temporary parent of phr.go:64-65 takes job.cut.bounding = root box.
+  std::vector<int32_t> ranges;
+  for (const Ref& x : cut.nodes) {
+    ranges.push_back(aux->nodes[x.id].first);
+    ranges.push_back(aux->nodes[x.id].last);
+  }
   const int32_t root = b.build(cut);
-  PhrTree* t = new PhrTree{aux, std::move(b.out), root, b.split_work, n_cut, b.max_split};
+  PhrTree* t = new PhrTree{aux,       std::move(b.out),     root, b.split_work, n_cut, b.max_split,
+                           std::move(b.trace_i), std::move(b.trace_d), std::move(ranges)};
   return t;
+}
+
+oracle_phr* oracle_phr_build(const oracle_tree* aux, double alpha, double delta, int branching,
+                             int64_t max_cut) {
+  return phr_build(aux, alpha, delta, branching, max_cut, false);
+}
+
+// oracle_phr_build that also records every SweepSAH call (Phr::record).
+oracle_phr* oracle_phr_build_traced(const oracle_tree* aux, double alpha, double delta, int branching,
+                                    int64_t max_cut) {
+  return phr_build(aux, alpha, delta, branching, max_cut, true);
+}
+
+// Copies the SweepSAH record (null outputs: sizes only).
+void oracle_phr_trace(const oracle_phr* t, int32_t* ints, double* dbls, int64_t* n_ints, int64_t* n_dbls) {
+  *n_ints = int64_t(t->trace_i.size());
+  *n_dbls = int64_t(t->trace_d.size());
+  if (ints) std::copy(t->trace_i.begin(), t->trace_i.end(), ints);
+  if (dbls) std::copy(t->trace_d.begin(), t->trace_d.end(), dbls);
+}
+
+// The initial cut in its order: (first, last) sorted-position range per member.
+int64_t oracle_phr_initial_ranges(const oracle_phr* t, int32_t* out) {
+  if (out) std::copy(t->cut_ranges.begin(), t->cut_ranges.end(), out);
+  return int64_t(t->cut_ranges.size() / 2);
 }
 
 void oracle_phr_free(oracle_phr* t) { delete t; }

@@ -46,6 +46,9 @@ def load() -> C.CDLL:
         "oracle_closest_hit_brute": ([_P, C.c_int64, _P, C.c_int64, _P, _P, C.c_int], None),
         "oracle_philox": ([C.c_uint32] * 6 + [_P], None),
         "oracle_phr_build": ([_P, C.c_double, C.c_double, C.c_int, C.c_int64], _P),
+        "oracle_phr_build_traced": ([_P, C.c_double, C.c_double, C.c_int, C.c_int64], _P),
+        "oracle_phr_trace": ([_P, _P, _P, _P, _P], None),
+        "oracle_phr_initial_ranges": ([_P, _P], C.c_int64),
         "oracle_phr_free": ([_P], None),
         "oracle_phr_cost": ([_P], C.c_double),
         "oracle_phr_split_work": ([_P], C.c_int64),
@@ -179,10 +182,50 @@ class Tree:
 class PhrTree:
     """PhrBuilder.BuildFromAuxilary on an oracle LBVH (deterministic FIFO initial cut)."""
 
-    def __init__(self, aux: Tree, alpha: float = 0.5, delta: float = 6.0, branching: int = 2, max_cut: int = 2500):
+    def __init__(self, aux: Tree, alpha: float = 0.5, delta: float = 6.0, branching: int = 2, max_cut: int = 2500,
+                 trace: bool = False):
         self.lib = load()
         self.aux = aux  # keep the LBVH alive
-        self.h = self.lib.oracle_phr_build(aux.h, alpha, delta, branching, max_cut)
+        build = self.lib.oracle_phr_build_traced if trace else self.lib.oracle_phr_build
+        self.h = build(aux.h, alpha, delta, branching, max_cut)
+
+    def split_calls(self):
+        """The SweepSAH calls of a traced build, in order: a list of dicts with
+        'ranges' (m, 2) sorted-position ranges of the members (-1 for a PHR-made
+        leaf), 'boxes' (m, 6), 'bounding' (6,), 'split' (bool), 'left' /
+        'right' member indices in the order the split cuts hold them, and the
+        sizes of the job's cuts at that moment ('cut_sizes') with the index of
+        the one split ('chosen')."""
+        ni, nd = C.c_int64(), C.c_int64()
+        self.lib.oracle_phr_trace(self.h, None, None, C.byref(ni), C.byref(nd))
+        ints = np.empty(ni.value, np.int32)
+        dbl = np.empty(nd.value, np.float64)
+        self.lib.oracle_phr_trace(self.h, _p(ints), _p(dbl), C.byref(ni), C.byref(nd))
+        out, i, d = [], 0, 0
+        while i < len(ints):
+            m, split, nl, nr, ncuts, chosen = (int(v) for v in ints[i:i + 6])
+            i += 6
+            sizes = ints[i:i + ncuts].copy()
+            i += ncuts
+            ranges = ints[i:i + 2 * m].reshape(m, 2).copy()
+            i += 2 * m
+            left = ints[i:i + nl].copy()
+            i += nl
+            right = ints[i:i + nr].copy()
+            i += nr
+            bounding = dbl[d:d + 6].copy()
+            boxes = dbl[d + 6:d + 6 + 6 * m].reshape(m, 6).copy()
+            d += 6 + 6 * m
+            out.append({"ranges": ranges, "boxes": boxes, "bounding": bounding, "split": bool(split),
+                        "left": left, "right": right, "cut_sizes": sizes, "chosen": chosen})
+        return out
+
+    def initial_ranges(self) -> np.ndarray:
+        """The initial cut in order: (first, last) sorted-position range per member."""
+        n = int(self.lib.oracle_phr_initial_ranges(self.h, None))
+        out = np.empty((n, 2), np.int32)
+        self.lib.oracle_phr_initial_ranges(self.h, _p(out))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):

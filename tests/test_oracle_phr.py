@@ -1,8 +1,12 @@
-"""Pins of the oracle's PHR refinement (CPU only).
+"""Consistency checks of the oracle's PHR refinement (CPU only).
 
-A pure-Python restatement of PhrBuilder.BuildFromAuxilary (phr.go:47-459),
-written from the reference's text and run on small inputs, must give the
-oracle's tree exactly: preorder topology, leaf triangle lists and FP64 boxes.
+The oracle's PHR is pinned by what the mathematics fixes in
+test_oracle_phr_pins.py (delta = 0 gives the LBVH, every SweepSAH call equals
+a brute-force argmin, the refinement chain, the initial cut's breadth-first
+characterisation, threshold equalities and hand-computed trees).  Here a
+second, pure-Python restatement of PhrBuilder.BuildFromAuxilary
+(phr.go:47-459), run on small inputs, must also give the oracle's tree
+exactly: preorder topology, leaf triangle lists and FP64 boxes.
 It works on the oracle's LBVH (itself pinned in test_oracle.py) and takes the
 initial cut in FIFO order from the root, the order the oracle defines (one
 worker draining the reference's channel queue).  Cost() of the PHR tree is
