@@ -366,7 +366,21 @@ def bench_scene_compile(torch, ctx, steps):
     return out
 
 
-def bench_large(torch, ctx, steps=2):
+def build_roofline(ms, ntri, key_mode_bits, sort_bits):
+    """HBM roofline of one LBVH build: the algorithmic bytes per triangle
+    36 (positions) + (kb + 4)(1 + 2P) (keys and values written once, then read
+    and written per radix pass) + 64 (two 32-byte nodes) + 4 (permutation) +
+    24 (the refit re-reads the triangle box), with kb = 8-byte keys for 63-bit
+    codes, 4-byte for 30-bit ones, and P = ceil(key bits / digit bits)."""
+    kb = 8 if key_mode_bits > 32 else 4
+    passes = -(-key_mode_bits // sort_bits)
+    per_tri = 36 + (kb + 4) * (1 + 2 * passes) + 64 + 4 + 24
+    achieved = per_tri * ntri / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "bytes_per_tri": per_tri, "passes": passes, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
+
+
+def bench_large(torch, ctx, steps=100):
     """The large configuration: 10.38M triangles (2048x2048 fBm terrain + 5000
     400-triangle spheres), PHR tree, 1920x1080, 64 spp, depth 8, sky miss;
     inverse rendering of the per-triangle terrain albedo with Adam (lr 1e-2)
@@ -419,7 +433,8 @@ def bench_large(torch, ctx, steps=2):
     torch.cuda.synchronize()
     losses, step_ms, step_rays = [], [], []
     for k in range(1, steps + 1):
-        note(f"large: Adam step {k}")
+        if k == 1 or k % 10 == 0:
+            note(f"large: Adam step {k} of {steps}")
         e0, e1, e2 = _events(torch, 3)
         e0.record()
         render_keep(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed, img.data_ptr(),
@@ -438,15 +453,20 @@ def bench_large(torch, ctx, steps=2):
         step_ms.append(e0.elapsed_time(e2))
     ctx.synchronize()
     render_ms = ev[2].elapsed_time(ev[3])
+    build_ms = ev[0].elapsed_time(ev[1])
+    best = SORT_CONFIGS[int(np.argmin(sort_ms))]
     out = {"tris": n, "terrain_tris": nt, "image": [W, H], "spp": spp, "max_depth": depth,
-           "lbvh_build_ms": round(ev[0].elapsed_time(ev[1]), 3),
+           "lbvh_build_ms": round(build_ms, 3),
+           "lbvh_build_roofline": build_roofline(build_ms, n, 63, best[0]),
            "lbvh_sort_ms_by_config": {f"{c[0]}x{c[1]}": round(float(m), 3) for c, m in zip(SORT_CONFIGS, sort_ms)},
            "phr_build_ms": round(phr_ms, 2),
            "render_ms": round(render_ms, 2), "render_rays": render_rays,
            "render_mrays_s": round(render_rays / (render_ms * 1e3), 1),
            "samples_per_s": round(W * H * spp / (render_ms * 1e-3), 1),
-           "adam_steps": steps, "step_ms": [round(x, 2) for x in step_ms], "step_rays": step_rays,
-           "step_mrays_s": [round(r / (t * 1e3), 1) for r, t in zip(step_rays, step_ms)], "loss": losses}
+           "adam_steps": steps, "loop_s": round(sum(step_ms) * 1e-3, 3),
+           "step_ms": [round(x, 2) for x in step_ms], "step_rays": step_rays,
+           "step_mrays_s": [round(r / (t * 1e3), 1) for r, t in zip(step_rays, step_ms)],
+           "loss": [float(f"{x:.6e}") for x in losses]}
     mesh.bind_params(None, None)
     phr.close()
     lbvh.close()
@@ -538,6 +558,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--quick", action="store_true", help="skip the extra measurements")
     ap.add_argument("--no-large", action="store_true", help="skip the 10.4M-triangle inverse-rendering leg")
+    ap.add_argument("--adam-steps", type=int, default=100,
+                    help="inverse-rendering steps of the 10.4M-triangle leg (the configuration's 100)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run only the counter workload (copy kernel + closest hit) and exit")
     ap.add_argument("--traffic-from", nargs="+", metavar="CSV",
@@ -677,6 +699,8 @@ def main():
             ctx.synchronize()
             builds[name] = {"ms": round(float(per_b.mean()), 4),
                             "mtris_per_s": round(args.tris / (float(per_b.mean()) * 1e-3) / 1e6, 1),
+                            "roofline": build_roofline(float(per_b.mean()), args.tris,
+                                                       63 if mode == KEY_REF63 else 30, sort_cfg[0]),
                             "sort": {"bits": sort_cfg[0], "keys_per_thread": sort_cfg[1],
                                      "ms_by_config": {f"{c[0]}x{c[1]}": round(float(m), 4)
                                                       for c, m in zip(SORT_CONFIGS, sort_ms)}}}
@@ -711,7 +735,7 @@ def main():
         out["adjoint_over_primal_kept"] = out["adjoint"]["kept"]["backward_kept_over_primal"]
         if not args.no_large:
             note("large scene (10.4M triangles) and inverse-rendering steps")
-            out["large"] = bench_large(torch, ctx)
+            out["large"] = bench_large(torch, ctx, args.adam_steps)
             note("large scene compiled on the device from its scene graph")
             out["scene_compile"] = bench_scene_compile(torch, ctx, max(3, args.steps // 4))
 
