@@ -706,6 +706,18 @@ def main():
                                                       for c, m in zip(SORT_CONFIGS, sort_ms)}}}
             b.close()
         out["lbvh_build"] = builds
+        # the reference benchmark's branching factor 4 (benchmark_test.go:22)
+        walls = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            p4 = Bvh.phr(ctx, lbvh, branching=4)
+            ctx.synchronize()
+            walls.append((time.perf_counter() - t0) * 1e3)
+        out["phr_bf4"] = {"build_ms": round(float(np.median(walls)), 2), "cost": round(p4.cost(), 4),
+                          "cost_bf2": round(phr.cost(), 4)}
+        per_4, _ = time_launches(torch, tracer(p4, d_rays, n), max(5, args.steps // 2), 2)
+        out["phr_bf4"]["incoherent_mrays_s"] = round(n / (float(per_4.mean()) * 1e-3) / 1e6, 2)
+        p4.close()
         note("PHR grid optimiser (16 BuildWithCost + Cost)")
         from trace_amd.device import phr_optimize_grid
         t0 = time.perf_counter()

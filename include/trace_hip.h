@@ -159,9 +159,11 @@ int trace_bvh_size(trace_ctx* ctx, const trace_bvh* bvh, int64_t* size);
 /* PhrBuilder{Alpha, Delta, BranchingFactor}.BuildFromAuxilary(aux) (phr.go:30-84, 92-459)
  * with MAX_CUT_SIZE = max_cut (phr.go:11, 2500 in the reference); aux is an LBVH built
  * by trace_lbvh_build.  NewDefaultBuilder is alpha 0.5, delta 6, branching 2
- * (phr.go:26-28).  The initial cut is taken in FIFO order from the root.  This build
- * supports branching 2 (TRACE_E_UNSUPPORTED otherwise).  The result is a new BVH handle
- * (its triangles in PHR leaf order); Cost/Size/export/closest-hit/render accept it. */
+ * (phr.go:26-28).  The initial cut is taken in FIFO order from the root.  Branching
+ * factors 2 to 8 are built (TRACE_E_UNSUPPORTED otherwise): each job splits its largest
+ * cut until it holds `branching` cuts (phr.go:98-127).  The result is a new BVH handle
+ * (its triangles in PHR leaf order); Cost/Size/export/closest-hit/render/traversal steps
+ * accept it, and the export reports each branch's own child count. */
 int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,
                     int branching, int32_t max_cut, trace_bvh** out);
 /* PhrBuilder.BuildWithCost (phr.go:154-255): trace_phr_build plus its build cost -- the

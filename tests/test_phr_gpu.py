@@ -12,12 +12,12 @@ from trace_amd.device import Bvh, Mesh
 pytestmark = pytest.mark.gpu
 
 
-def _compare(ctx, oracle, pos, alpha=0.5, delta=6.0, max_cut=2500):
+def _compare(ctx, oracle, pos, alpha=0.5, delta=6.0, max_cut=2500, branching=2):
     mesh = Mesh(ctx, pos)
     aux = Bvh.lbvh(ctx, mesh)
-    phr = Bvh.phr(ctx, aux, alpha, delta, 2, max_cut)
+    phr = Bvh.phr(ctx, aux, alpha, delta, branching, max_cut)
     otree = oracle.Tree(pos)
-    ophr = oracle.PhrTree(otree, alpha, delta, 2, max_cut)
+    ophr = oracle.PhrTree(otree, alpha, delta, branching, max_cut)
     rec, box = phr.canonical()
     orec, obox, oprims = ophr.canonical()
     assert rec.shape[0] == orec.shape[0], (rec.shape, orec.shape)
@@ -209,7 +209,107 @@ def test_bayes_optimizer_matches_oracle(ctx, oracle):
 def test_phr_rejects_unsupported(ctx):
     pos = scenes.random_triangles(100, seed=1)
     aux = Bvh.lbvh(ctx, Mesh(ctx, pos))
-    with pytest.raises(_lib.TraceError):
-        Bvh.phr(ctx, aux, branching=4)
+    for bf in (1, 9):  # this build takes branching factors 2 to 8
+        with pytest.raises(_lib.TraceError):
+            Bvh.phr(ctx, aux, branching=bf)
     with pytest.raises(_lib.TraceError):
         Bvh.phr(ctx, aux, max_cut=0)
+
+
+# ---- branching factors above 2 (phr.go:92-152, scene.go:32-36) ---------------------------
+
+
+@pytest.mark.parametrize("bf", [3, 4, 8])
+@pytest.mark.parametrize("n,seed", [(2, 1), (7, 2), (100, 3), (5000, 4), (50_000, 5)])
+def test_phr_nary_matches_oracle(ctx, oracle, n, seed, bf):
+    """n-ary trees: node kinds and child counts in preorder, leaf lists, FP64
+    boxes and Cost() equal the oracle's; BuildWithCost's counter too."""
+    print(f"phr n-ary parity n={n} bf={bf}", flush=True)
+    _, aux, phr, _, ophr = _compare(ctx, oracle, scenes.random_triangles(n, seed=seed), branching=bf)
+    rec, _ = phr.canonical()
+    assert rec[:, 3].max() <= bf
+    if n >= 5000:
+        assert rec[:, 3].max() == bf
+    info = phr.info()
+    assert info.arity == bf
+    cost = Bvh.phr_with_cost(ctx, aux, 0.5, 6.0, bf)[1]
+    assert cost == ophr.build_cost()
+
+
+@pytest.mark.parametrize("bf", [3, 4])
+def test_phr_nary_settings(ctx, oracle, bf):
+    """The HQ setting (the 2500 cap binds), a small cap, no refinement beyond
+    the initial cut and full refinement, each with branching 3 and 4."""
+    print(f"phr n-ary settings bf={bf}", flush=True)
+    pos = scenes.random_triangles(60_000, seed=6)
+    for alpha, delta, cap in ((0.55, 9.0, 2500), (0.5, 6.0, 37), (0.0, 2.0, 2500), (0.0, 30.0, 2500),
+                              (1.0, 0.5, 2500)):
+        _compare(ctx, oracle, pos, alpha, delta, cap, branching=bf)
+
+
+def test_phr_nary_config2_full(ctx, oracle):
+    """1M triangles with the reference benchmark's branching factor 4
+    (benchmark_test.go:22): the tree bit-exact, closest hits (the benchmark's
+    persistent 128-thread traversal) and traversalSteps equal the oracle's."""
+    from trace_amd.device import set_tune
+
+    print("phr n-ary 1M bf=4", flush=True)
+    pos = scenes.random_triangles(1_000_000, seed=1)
+    _, aux, phr, otree, ophr = _compare(ctx, oracle, pos, branching=4)
+    assert phr.cost() < otree.cost()
+    inc = scenes.incoherent_rays(200_000, seed=2)
+    coh = scenes.coherent_rays_cfg2(4096, 2048)[::41]
+    set_tune(ctx, trav_threads=128, trav_persistent=True)
+    try:
+        for rays in (inc, coh):
+            hit, t = phr.closest_hit(rays)
+            ohit, ot = ophr.closest_hit(rays)
+            assert np.mean(hit == ohit) >= 0.9999
+            both = (hit == ohit) & (hit >= 0)
+            assert np.max(np.abs(t[both] - ot[both]) / ot[both]) < 1e-5
+    finally:
+        set_tune(ctx)
+    r64 = inc[:20_000].astype(np.float64)
+    assert np.array_equal(phr.traversal_steps(r64), ophr.traversal_steps(r64))
+
+
+@pytest.mark.parametrize("bf", [3, 4, 8])
+def test_phr_nary_steps_heatmap_render(ctx, oracle, bf):
+    """traversalSteps and the heat map walk the n-ary tree's binary split but
+    count only the tree's own nodes; a render over the n-ary tree matches the
+    oracle's; the terrain's big leaves are split for traversal."""
+    from trace_amd.device import render
+
+    print(f"phr n-ary steps/heatmap/render bf={bf}", flush=True)
+    pos = scenes.random_triangles(50_000, seed=51, mean_edge=0.01)
+    lb = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    phr = Bvh.phr(ctx, lb, branching=bf)
+    ref = oracle.PhrTree(oracle.Tree(pos), branching=bf)
+    rays = scenes.incoherent_rays(20_000, seed=52).astype(np.float64)
+    got = phr.traversal_steps(rays)
+    want = ref.traversal_steps(rays)
+    assert np.array_equal(got, want)
+    assert want.max() > 10
+    sc = scenes.box_scene(seed=1, slices=40, rings=22, width=160, height=90)
+    m2 = Mesh(ctx, sc["pos"])
+    m2.set_shading(sc["mat_idx"], sc["materials"])
+    p2 = Bvh.phr(ctx, Bvh.lbvh(ctx, m2), branching=bf)
+    ot2 = oracle.Tree(sc["pos"])
+    rgb, counts = p2.heatmap(sc["camera"], 160, 90, 20)
+    orgb, ocounts = oracle.PhrTree(ot2, branching=bf).heatmap(sc["camera"], 160, 90, 20)
+    assert np.array_equal(counts, ocounts) and np.array_equal(rgb, orgb)
+    s, q, _ = render(ctx, m2, p2, sc["camera"], 160, 90, 8, 5, _lib.MISS_SKY, seed=4)
+    os_, oq, _ = oracle.render(ot2, sc["mat_idx"], sc["materials"], sc["camera"], 160, 90, 8, 5, _lib.MISS_SKY,
+                               seed=4)
+    from test_render_gpu import within_3_sigma
+
+    frac, _ = within_3_sigma(s, q, os_, oq, 8)
+    assert frac < 0.01
+    d = scenes.terrain_scene(n=256, n_spheres=100, width=128, height=72)
+    _, _, tphr, tot, _ = _compare(ctx, oracle, d["pos"], branching=bf)
+    lo, hi = d["pos"].reshape(-1, 3).min(0), d["pos"].reshape(-1, 3).max(0)
+    inc = scenes.incoherent_rays(20_000, seed=31)
+    inc[:, :3] = lo + inc[:, :3] * (hi - lo)
+    hit, t = tphr.closest_hit(inc)
+    ohit, _ = tot.closest_hit(inc)
+    assert np.mean(hit == ohit) >= 0.9999

@@ -346,8 +346,8 @@ static int export_phr(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap, int32_t
       TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_bvh_export_canonical: PHR node %d out of range", o);
     order.push_back(o);
     const phr::OutNode& nd = out[o];
-    if (nd.kind == 0)
-      for (int c = nd.nchild - 1; c >= 0; --c) st.push_back(nd.child[c]);
+    if (nd.kind == 0)  // a branch's children are consecutive output nodes
+      for (int c = nd.nchild - 1; c >= 0; --c) st.push_back(nd.child[0] + c);
   }
   *n_out = int64_t(order.size());
   if (!h_rec || !h_box) return TRACE_OK;
@@ -374,7 +374,8 @@ int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double d
   *out = nullptr;
   if (aux->key_mode < 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: the auxiliary tree must be an LBVH");
   if (aux->base || aux->nseg > 1) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: forests and segment views are not refined by this build");
-  if (branching != 2) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: branching factor %d (this build supports 2)", branching);
+  if (branching < 2 || branching > 8)
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: branching factor %d (this build supports 2 to 8)", branching);
   if (!(alpha >= 0) || !std::isfinite(delta) || max_cut < 1)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: need alpha >= 0, finite delta, max_cut >= 1");
   trace_bvh* b = new (std::nothrow) trace_bvh();

@@ -255,6 +255,9 @@ struct trace_bvh {
   DevArray<uint8_t> phr_out;
   int64_t phr_out_count = 0;
   DevArray<int32_t> phr_src;
+  // PHR trees of branching above 2: per binary traversal node, 1 when it is
+  // virtual (an inner node of a branch's binary split, not a node of the tree)
+  DevArray<uint8_t> virt;
   bool phr_clone = false;  // the PHR tree is the LBVH itself (initial cut of one node)
   double phr_alpha = 0, phr_delta = 0;
   int64_t phr_initial_cut = 0;
@@ -277,6 +280,7 @@ struct trace_bvh {
   const int32_t* perm_p() const { return base ? base->perm.p : perm.p; }
   const BvhHeader* header_p() const { return base ? base->seg_headers.p + seg : header.p; }
   int32_t depth_bound() const { return base ? base->host_depth : host_depth; }
+  const uint8_t* virt_p() const { return base ? base->virt.p : virt.p; }
   bool spheres() const { return base ? base->has_spheres : has_spheres; }
 };
 

@@ -1,5 +1,6 @@
 // Progressive Hierarchical Refinement (PhrBuilder.BuildFromAuxilary,
-// phr.go:47-459) on the GPU, branching factor 2.
+// phr.go:47-459) on the GPU: branching factor 2 by the job kernels below,
+// 3 to 8 by the group kernel (nary_kernel).
 //
 // The reference recursion is a tree of independent jobs: a job holds a cut
 // (LBVH nodes, a bounding box, a depth), splits it with SweepSAH and refines
@@ -89,9 +90,44 @@ struct Ctl {
   TRACE_CTL_FIELD(err);        // 1 capacity, 2 depth table
   TRACE_CTL_FIELD(cut);        // initial cut size
   TRACE_CTL_FIELD(max_m);      // largest cut among the large jobs appended for the next level
+  TRACE_CTL_FIELD(groups);     // n-ary build: groups appended for the next step
+  unsigned long long work;     // n-ary build: sizes of the cuts handed to SweepSAH (phr.go:221)
+  unsigned long long work_pad[15];
   uint32_t jobs[kLists][32];   // jobs appended for the next level, per list (element 0)
 };
 #undef TRACE_CTL_FIELD
+
+// ---- branching factors above 2 ---------------------------------------------------
+// buildSubTree (phr.go:92-152) keeps splitting the largest of its cuts until it
+// holds BranchingFactor of them.  A group is one such job: its cuts, and the
+// one to split next.  Every step splits one cut per group (sort, minCost,
+// refinement of both halves, as the binary job does), then either hands the
+// group on to the next step or, once it holds BranchingFactor cuts or no cut
+// has two members, writes its branch: a cut of one member is a leaf, every
+// other cut becomes a new group of one cut.  Cuts that are alive at one step
+// are disjoint LBVH subtrees, so the members of all of them fit one pool of
+// 2U - 1 entries; the members of cuts that wait are carried to the next
+// step's pool with the group.
+constexpr int kMaxBranching = 8;
+
+struct CutRec {
+  int32_t off, cnt;        // members in the pool (a PHR-made leaf: cnt 1)
+  int32_t depth;           // cut depth (phr.go:360-364)
+  int32_t prims;           // triangles under the cut
+  int32_t leaf_ls, leaf_lc;  // a PHR-made leaf: its LBVH node list (leaf_lc 0: a cut still to split)
+  float box[6];            // cut bounding
+};
+static_assert(sizeof(CutRec) == 48, "CutRec is 48 bytes");
+
+struct Group {
+  int32_t out, level, prim_off, ncuts;
+  int32_t split;   // the cut split by the next step
+  int32_t pad[3];
+  float box[6];    // the job's own cut bounding: its branch's box
+  int32_t pad2[2];
+  CutRec cuts[kMaxBranching];
+};
+static_assert(sizeof(Group) % 16 == 0, "Group is a multiple of 16 bytes");
 
 // bounding.go:45-47, FP64 from FP32 box corners (exact differences)
 __device__ inline double sa_box(const float* b) {
@@ -807,6 +843,284 @@ __global__ __launch_bounds__(B) void task_kernel(Flat f, const double* __restric
   }
 }
 
+// ---- n-ary jobs: one workgroup per group --------------------------------------------
+struct NCaps {
+  int64_t pool;
+  int32_t out, list, groups;
+};
+
+// refined() (phr.go:313-338) of one half [a, b) of order C with the split
+// cut's threshold: the single survivor becomes a leaf of all its triangles,
+// otherwise the refined members are appended to the next pool.  The record
+// is the same in every thread.
+template <int B>
+__device__ CutRec refine_half(const Flat& f, const int32_t* ids, const int32_t* C, int a, int b, double th,
+                              const WBox& hb, int32_t prims, int32_t depth, int32_t* pool_next, int32_t* list,
+                              Ctl* ctl, const NCaps& caps, BlockLds& s) {
+  const int tid = threadIdx.x;
+  uint32_t k_total = 0;
+  for (int r = a + tid; r < b; r += B) {
+    const int32_t node = ids[C[r]];
+    const bool keep = f.child[2 * node] < 0 || sa_box(f.box + 6 * size_t(node)) < th;
+    k_total += keep ? 1u : 2u;
+  }
+  k_total = block_sum_u32<B>(k_total, s);
+  CutRec rec;
+  rec.depth = depth;
+  rec.prims = prims;
+  const float box[6] = {hb.lo[0], hb.lo[1], hb.lo[2], hb.hi[0], hb.hi[1], hb.hi[2]};
+  for (int q = 0; q < 6; ++q) rec.box[q] = box[q];
+  if (k_total == 1) {
+    int32_t ls = 0;
+    if (tid == 0) {
+      ls = int32_t(atomicAdd(&ctl->list, 1u));
+      if (ls < caps.list) list[ls] = ids[C[a]];
+      else atomicOr(&ctl->err, 1u);
+    }
+    ls = block_bcast<B>(ls, s);
+    rec.off = 0;
+    rec.cnt = 1;
+    rec.leaf_ls = ls;
+    rec.leaf_lc = 1;
+    return rec;
+  }
+  int32_t poff = 0;
+  if (tid == 0) poff = int32_t(atomicAdd(&ctl->pool, k_total));
+  poff = block_bcast<B>(poff, s);
+  rec.off = poff;
+  rec.cnt = int32_t(k_total);
+  rec.leaf_ls = 0;
+  rec.leaf_lc = 0;
+  if (int64_t(poff) + k_total > caps.pool) {
+    if (tid == 0) atomicOr(&ctl->err, 1u);
+    rec.cnt = 0;
+    return rec;
+  }
+  uint32_t carry = 0;
+  for (int r0 = a; r0 < b; r0 += B) {
+    const int r = r0 + tid;
+    int32_t node = 0;
+    uint32_t k = 0;
+    if (r < b) {
+      node = ids[C[r]];
+      const bool keep = f.child[2 * node] < 0 || sa_box(f.box + 6 * size_t(node)) < th;
+      k = keep ? 1u : 2u;
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_u32<B>(k, s, &tot);
+    const int32_t p = poff + int32_t(carry + ex);
+    if (k == 1) {
+      pool_next[p] = node;
+    } else if (k == 2) {
+      pool_next[p] = f.child[2 * node];
+      pool_next[p + 1] = f.child[2 * node + 1];
+    }
+    carry += tot;
+  }
+  return rec;
+}
+
+// One step of every group: one split (phr.go:102-127 for BranchingFactor bf),
+// then the group moves on or writes its branch.
+template <int B>
+__global__ __launch_bounds__(B) void nary_kernel(Flat f, const double* __restrict__ th,
+                                                 const Group* __restrict__ groups, Group* __restrict__ groups_next,
+                                                 const int32_t* __restrict__ pool, int32_t* __restrict__ pool_next,
+                                                 int32_t* __restrict__ ordbuf, double* gkey, double* rcost_buf,
+                                                 OutNode* __restrict__ out, int32_t* __restrict__ list, Ctl* ctl,
+                                                 NCaps caps, int lds_p, int bf) {
+  extern __shared__ double s_dyn[];
+  __shared__ BlockLds s_blk;
+  __shared__ Group g;
+  const int tid = threadIdx.x;
+  {
+    const int32_t* src = reinterpret_cast<const int32_t*>(groups + blockIdx.x);
+    int32_t* dst = reinterpret_cast<int32_t*>(&g);
+    for (int w = tid; w < int(sizeof(Group) / 4); w += B) dst[w] = src[w];
+  }
+  __syncthreads();
+  const int si = g.split;
+  const CutRec c = g.cuts[si];
+  const int m = c.cnt;
+  if (c.depth >= kMaxDepth || si < 0 || si >= g.ncuts || m < 2) {
+    if (tid == 0) atomicOr(&ctl->err, 2u);
+    return;
+  }
+  int P = 0;
+  if (m <= lds_p) {
+    P = 128;
+    while (P < m) P <<= 1;
+  }
+  double* lkey = s_dyn;
+  int32_t* lpos = reinterpret_cast<int32_t*>(s_dyn + lds_p);
+  const int32_t* ids = pool + c.off;
+  int32_t* oX = ordbuf + c.off;
+  int32_t* oY = ordbuf + caps.pool + c.off;
+  int32_t* oZ = ordbuf + 2 * caps.pool + c.off;
+  double* keys = gkey + c.off;
+  double* rc = rcost_buf + c.off;
+
+  // SweepSAH (phr.go:368-405), as the binary job does it
+  sort_axis<B>(f, ids, nullptr, oX, 0, m, keys, lkey, lpos, P);
+  sort_axis<B>(f, ids, oX, oY, 1, m, keys, lkey, lpos, P);
+  const Split x = min_cost<B>(f, ids, oX, m, c.box, rc, s_blk);
+  const Split y = min_cost<B>(f, ids, oY, m, c.box, rc, s_blk);
+  sort_axis<B>(f, ids, x.cost < y.cost ? oY : oX, oZ, 2, m, keys, lkey, lpos, P);
+  const Split z = min_cost<B>(f, ids, oZ, m, c.box, rc, s_blk);
+  const int32_t* C;
+  Split sp;
+  if (x.cost < y.cost) {
+    if (x.cost < z.cost) { C = oX; sp = x; } else { C = oZ; sp = z; }
+  } else {
+    if (y.cost < z.cost) { C = oY; sp = y; } else { C = oZ; sp = z; }
+  }
+  if (tid == 0) atomicAdd(&ctl->work, (unsigned long long)m);  // buildSubTreeCost's counter
+
+  CutRec first, second;
+  bool two = false;
+  if (sp.at > 0) {
+    WBox L, R;
+    wb_empty(L);
+    wb_empty(R);
+    uint32_t pl = 0, pr = 0;
+    for (int r = tid; r < m; r += B) {
+      const int32_t node = ids[C[r]];
+      WBox v;
+      load_member(f, node, v, false);
+      const uint32_t pc = uint32_t(f.prim[2 * node + 1] - f.prim[2 * node]);
+      if (r < sp.at) { wb_merge(L, v); pl += pc; } else { wb_merge(R, v); pr += pc; }
+    }
+    WBox i_, e_, Lt, Rt;
+    block_scan_box<B>(L, s_blk, &i_, &e_, &Lt);
+    block_scan_box<B>(R, s_blk, &i_, &e_, &Rt);
+    pl = block_sum_u32<B>(pl, s_blk);
+    pr = block_sum_u32<B>(pr, s_blk);
+    const double thd = th[c.depth];  // refined() uses the split cut's depth
+    first = refine_half<B>(f, ids, C, 0, sp.at, thd, Lt, int32_t(pl), c.depth + 1, pool_next, list, ctl, caps, s_blk);
+    second = refine_half<B>(f, ids, C, sp.at, m, thd, Rt, int32_t(pr), c.depth + 1, pool_next, list, ctl, caps,
+                            s_blk);
+    two = true;
+  } else {
+    // no split beats the whole cut: a leaf of every member, in order C (phr.go:121-126)
+    int32_t ls = 0;
+    if (tid == 0) ls = int32_t(atomicAdd(&ctl->list, uint32_t(m)));
+    ls = block_bcast<B>(ls, s_blk);
+    if (ls + m > caps.list) {
+      if (tid == 0) atomicOr(&ctl->err, 1u);
+      return;
+    }
+    for (int r = tid; r < m; r += B) list[ls + r] = ids[C[r]];
+    first = c;
+    first.off = 0;
+    first.cnt = 1;
+    first.leaf_ls = ls;
+    first.leaf_lc = m;
+  }
+  // the group's cuts after this split: the split one replaced, the right half appended
+  const int n_old = g.ncuts;
+  __syncthreads();
+  if (tid == 0) {
+    g.cuts[si] = first;
+    if (two) g.cuts[g.ncuts++] = second;
+  }
+  __syncthreads();
+  // carry the waiting cuts' members into the next pool
+  for (int j = 0; j < n_old; ++j) {
+    if (j == si || g.cuts[j].leaf_lc > 0) continue;
+    const int32_t cnt = g.cuts[j].cnt;
+    int32_t poff = 0;
+    if (tid == 0) poff = int32_t(atomicAdd(&ctl->pool, uint32_t(cnt)));
+    poff = block_bcast<B>(poff, s_blk);
+    if (int64_t(poff) + cnt > caps.pool) {
+      if (tid == 0) atomicOr(&ctl->err, 1u);
+      return;
+    }
+    const int32_t* from = pool + g.cuts[j].off;
+    for (int r = tid; r < cnt; r += B) pool_next[poff + r] = from[r];
+    __syncthreads();
+    if (tid == 0) g.cuts[j].off = poff;
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  // the largest cut, the first of equal sizes (phr.go:104-111)
+  int32_t mx = 0, mi = 0;
+  for (int j = 0; j < g.ncuts; ++j)
+    if (g.cuts[j].cnt > mx) {
+      mx = g.cuts[j].cnt;
+      mi = j;
+    }
+  if (g.ncuts < bf && mx > 1) {
+    const int32_t slot = int32_t(atomicAdd(&ctl->groups, 1u));
+    if (slot >= caps.groups) {
+      atomicOr(&ctl->err, 1u);
+      return;
+    }
+    g.split = mi;
+    groups_next[slot] = g;
+    raise_max(&ctl->max_m, uint32_t(mx));
+    return;
+  }
+  // the branch (box: the job's cut bounding) and its children in cut order
+  const int32_t k = g.ncuts;
+  const int32_t o = int32_t(atomicAdd(&ctl->out, uint32_t(k)));
+  if (o + k > caps.out) {
+    atomicOr(&ctl->err, 1u);
+    return;
+  }
+  int32_t p = g.prim_off;
+  for (int j = 0; j < k; ++j) {
+    const CutRec& cj = g.cuts[j];
+    if (cj.leaf_lc > 0) {
+      write_out(out, o + j, 1, 0, -1, -1, cj.leaf_ls, cj.leaf_lc, p, cj.prims, g.level + 1, cj.box);
+      raise_max(&ctl->max_level, uint32_t(g.level + 1));
+    } else {
+      const int32_t slot = int32_t(atomicAdd(&ctl->groups, 1u));
+      if (slot >= caps.groups) {
+        atomicOr(&ctl->err, 1u);
+        return;
+      }
+      Group n;
+      n.out = o + j;
+      n.level = g.level + 1;
+      n.prim_off = p;
+      n.ncuts = 1;
+      n.split = 0;
+      for (int q = 0; q < 3; ++q) n.pad[q] = 0;
+      for (int q = 0; q < 6; ++q) n.box[q] = cj.box[q];
+      n.pad2[0] = n.pad2[1] = 0;
+      n.cuts[0] = cj;
+      groups_next[slot] = n;
+      raise_max(&ctl->max_m, uint32_t(cj.cnt));
+    }
+    p += cj.prims;
+  }
+  write_out(out, g.out, 0, k, o, k > 1 ? o + 1 : -1, -1, 0, g.prim_off, p - g.prim_off, g.level, g.box);
+}
+
+// The first group: the initial cut's job as initial_cut_kernel left it.
+__global__ void nary_root_kernel(const Task* __restrict__ t0, int32_t n, Group* __restrict__ g0) {
+  const Task t = *t0;
+  Group g;
+  g.out = t.out;
+  g.level = t.level;
+  g.prim_off = t.prim_off;
+  g.ncuts = 1;
+  g.split = 0;
+  for (int q = 0; q < 3; ++q) g.pad[q] = 0;
+  for (int q = 0; q < 6; ++q) g.box[q] = t.box[q];
+  g.pad2[0] = g.pad2[1] = 0;
+  CutRec c;
+  c.off = t.off;
+  c.cnt = t.cnt;
+  c.depth = t.depth;
+  c.prims = n;
+  c.leaf_ls = 0;
+  c.leaf_lc = 0;
+  for (int q = 0; q < 6; ++q) c.box[q] = t.box[q];
+  g.cuts[0] = c;
+  *g0 = g;
+}
+
 // ---- small jobs: one lane per job ----------------------------------------------
 // The deep levels hold hundreds of thousands of jobs whose cuts have a few
 // nodes; a wave per job spends its time on memory round trips and on
@@ -1377,12 +1691,13 @@ __global__ void reset_level_kernel(Ctl* ctl) {
   for (int r = 0; r < kLists; ++r) ctl->jobs[r][0] = 0;
   ctl->pool = 0;
   ctl->max_m = 0;
+  ctl->groups = 0;
 }
 
 // ---- layout of the traversal tree ------------------------------------------------
 __global__ void branch_flag_kernel(const OutNode* __restrict__ out, int32_t n, uint32_t* flag) {
   const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o < n) flag[o] = (out[o].kind == 0 && out[o].nchild == 2) ? 1u : 0u;
+  if (o < n) flag[o] = (out[o].kind == 0 && out[o].nchild >= 2) ? 1u : 0u;
 }
 
 __global__ void trav_nodes_kernel(const OutNode* __restrict__ out, int32_t n,
@@ -1406,6 +1721,85 @@ __global__ void trav_nodes_kernel(const OutNode* __restrict__ out, int32_t n,
     for (int a = 0; a < 6; ++a) boxes[6 * c + a] = out[k].box[a];
   }
   nodes[tindex[o]] = nd;
+}
+
+// n-ary trees: a branch of k children becomes k - 1 binary traversal nodes,
+// a balanced binary tree over its children in order (the first half, rounded
+// up, on the left), numbered in preorder from the branch's base index.  The
+// nodes after the first are virtual: they exist only for the traversal, and
+// BVH.traversalSteps does not count them.  Walked with the reference's stack
+// order (children pushed in order, the last popped first), the real nodes
+// are visited in the reference's order, and a virtual node's box, the union
+// of its children's, is missed only where all of them are.
+__device__ inline void child_slot(const OutNode* __restrict__ out, int32_t k, const uint32_t* __restrict__ tindex,
+                                  int32_t& ref, int32_t& cnt) {
+  int32_t kk = k;
+  if (out[kk].kind == 0 && out[kk].nchild == 1) kk = out[kk].child[0];  // branch over one leaf
+  if (out[kk].kind == 0) {
+    ref = int32_t(tindex[kk]);
+    cnt = 0;
+  } else {
+    ref = out[kk].prim_off;
+    cnt = out[kk].prim_cnt;
+  }
+}
+
+__global__ void nary_nodes_kernel(const OutNode* __restrict__ out, int32_t n, const uint32_t* __restrict__ flag,
+                                  const uint32_t* __restrict__ tindex, BvhNode* __restrict__ nodes,
+                                  uint8_t* __restrict__ virt) {
+  const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n || !flag[o]) return;
+  const int32_t k = out[o].nchild, c0 = out[o].child[0];
+  const uint32_t base = tindex[o];
+  struct Range {
+    int32_t lo, hi;
+    uint32_t idx;
+  };
+  Range st[kMaxBranching];
+  int sp = 0;
+  st[sp++] = Range{0, k, base};
+  while (sp > 0) {
+    const Range r = st[--sp];
+    const int32_t mid = r.lo + (r.hi - r.lo + 1) / 2;
+    BvhNode nd;
+    float* boxes = reinterpret_cast<float*>(&nd);
+    const int32_t part[2][2] = {{r.lo, mid}, {mid, r.hi}};
+    uint32_t next = r.idx + 1;
+    for (int c = 0; c < 2; ++c) {
+      const int32_t lo = part[c][0], hi = part[c][1];
+      float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+      for (int32_t j = lo; j < hi; ++j)
+        for (int a = 0; a < 3; ++a) {
+          b[a] = fminf(b[a], out[c0 + j].box[a]);
+          b[3 + a] = fmaxf(b[3 + a], out[c0 + j].box[3 + a]);
+        }
+      for (int a = 0; a < 6; ++a) boxes[6 * c + a] = b[a];
+      if (hi - lo == 1) {
+        child_slot(out, c0 + lo, tindex, nd.ref[c], nd.cnt[c]);
+      } else {
+        nd.ref[c] = int32_t(next);
+        nd.cnt[c] = 0;
+        if (sp < kMaxBranching) st[sp++] = Range{lo, hi, next};
+        next += uint32_t(hi - lo - 1);
+      }
+    }
+    nodes[r.idx] = nd;
+    virt[r.idx] = r.idx == base ? 0 : 1;
+  }
+}
+
+// Base index of each canonically ordered branch: the running sum of the
+// traversal nodes (children - 1) of the branches before it.
+__global__ void nary_weight_kernel(const OutNode* __restrict__ out, const uint32_t* __restrict__ sorted,
+                                   int32_t n_real, uint32_t* __restrict__ w) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_real) w[r] = uint32_t(out[sorted[r]].nchild - 1);
+}
+
+__global__ void nary_base_kernel(const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ base,
+                                 int32_t n_real, uint32_t* __restrict__ tindex) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_real) tindex[sorted[r]] = base[r];
 }
 
 // Canonical numbering of the traversal nodes.  Jobs allocate output nodes with
@@ -1471,7 +1865,7 @@ __global__ void phr_header_kernel(const OutNode* __restrict__ out, const BvhHead
                                   int32_t n, BvhHeader* __restrict__ hdr) {
   BvhHeader h = *aux;
   const OutNode& root = out[0];
-  if (root.kind == 0 && root.nchild == 2) {
+  if (root.kind == 0 && root.nchild >= 2) {
     h.root_ref = 0;  // the root is the first two-child branch in index order
     h.root_cnt = 0;
   } else {
@@ -1525,6 +1919,100 @@ static int clone_lbvh(trace_ctx* ctx, const trace_bvh* aux, trace_bvh* out) {
   HIP_TRY(ctx, hipStreamSynchronize(st));
   out->phr_clone = true;
   out->host_depth = h.depth;
+  return wide::widen(ctx, out);
+}
+
+// The traversal layout of the output tree: branches in canonical order as
+// binary nodes (one per two-child branch; k - 1 per branch of k children,
+// see nary_nodes_kernel), triangles copied into PHR leaf order, the header,
+// then the 4-wide tree.
+static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching, const Flat& f, OutNode* outn,
+                  const int32_t* list, Ctl* ctl, uint32_t* flag, uint32_t* tidx, uint32_t* sscr, uint32_t* n_trav,
+                  int32_t* ord, double* gkey, double* rcost, uint32_t* status, int64_t pool_cap, int levels,
+                  trace_bvh* out) {
+  hipStream_t st = ctx->stream;
+  Ctl hc;
+  int rc = read_ctl(ctx, ctl, &hc);
+  if (rc) return rc;
+  const int32_t n_out = int32_t(hc.out);
+  out->phr_out_count = n_out;
+  out->phr_levels = levels;
+
+  branch_flag_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag);
+  count_leaves_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, ctl);
+  scan::exclusive_u32(st, flag, tidx, n_out, n_trav, sscr);
+  uint32_t h_real = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&h_real, n_trav, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  // binary traversal nodes: one per real branch, or children - 1 when n-ary
+  uint32_t h_ntrav = h_real;
+  uint32_t* d_ntrav = n_trav;
+  if (h_real > 1) {
+    // canonical order (the buffers of the refinement loop are free by now:
+    // branches number fewer than the LBVH leaves, so keys fit gkey / rcost
+    // and values the order buffer)
+    int prim_bits = 1, level_bits = 1;
+    while ((int64_t(1) << prim_bits) < n) ++prim_bits;
+    while ((int64_t(1) << level_bits) <= int64_t(hc.max_level)) ++level_bits;
+    if (int64_t(h_real) > pool_cap || prim_bits + level_bits > 64)
+      TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: layout buffers too small");
+    auto* keys_a = reinterpret_cast<uint64_t*>(gkey);
+    auto* keys_b = reinterpret_cast<uint64_t*>(rcost);
+    auto* vals_a = reinterpret_cast<uint32_t*>(ord);
+    auto* vals_b = reinterpret_cast<uint32_t*>(ord + pool_cap);
+    canon_keys_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, prim_bits, keys_a, vals_a);
+    const int where = onesweep::sort_pairs(st, keys_a, vals_a, keys_b, vals_b, h_real, prim_bits + level_bits,
+                                           status, ctx->d_err, false, nullptr, 0, onesweep::Config{});
+    const uint32_t* sorted = where ? vals_b : vals_a;
+    if (branching > 2) {
+      // base index of each branch in canonical order (the key buffers are free again)
+      auto* w = reinterpret_cast<uint32_t*>(gkey);
+      auto* b = w + pool_cap;
+      d_ntrav = n_trav + 1;
+      nary_weight_kernel<<<int(ceil_div(h_real, 256)), 256, 0, st>>>(outn, sorted, int32_t(h_real), w);
+      scan::exclusive_u32(st, w, b, h_real, d_ntrav, sscr);
+      nary_base_kernel<<<int(ceil_div(h_real, 256)), 256, 0, st>>>(sorted, b, int32_t(h_real), tidx);
+      HIP_TRY(ctx, hipMemcpyAsync(&h_ntrav, d_ntrav, 4, hipMemcpyDeviceToHost, st));
+    } else {
+      canon_rank_kernel<<<int(ceil_div(h_real, 256)), 256, 0, st>>>(sorted, int32_t(h_real), tidx);
+    }
+    rc = ctx_check_device_error(ctx, "trace_phr_build");
+    if (rc) return rc;
+  } else if (h_real == 1 && branching > 2) {
+    // the root alone: its children - 1 nodes from index 0
+    OutNode root;
+    HIP_TRY(ctx, hipMemcpyAsync(&root, outn, sizeof(OutNode), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    h_ntrav = uint32_t(root.nchild - 1);
+    d_ntrav = n_trav + 1;
+    HIP_TRY(ctx, hipMemcpyAsync(d_ntrav, &h_ntrav, 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+  }
+  HIP_TRY(ctx, out->nodes.alloc(std::max<uint32_t>(1, h_ntrav)));
+  HIP_TRY(ctx, out->perm.alloc(n));
+  HIP_TRY(ctx, out->phr_src.alloc(n));
+  HIP_TRY(ctx, out->tris.alloc(9 * n));
+  HIP_TRY(ctx, out->prim_box.alloc(6 * n));
+  HIP_TRY(ctx, out->header.alloc(1));
+  if (branching > 2) {
+    HIP_TRY(ctx, out->virt.alloc(std::max<uint32_t>(1, h_ntrav)));
+    nary_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, out->nodes.p,
+                                                                 out->virt.p);
+  } else {
+    trav_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, out->nodes.p);
+  }
+  leaf_copy_kernel<<<int(ceil_div(int64_t(n_out) * kCopyLanes, 256)), 256, 0, st>>>(outn, n_out, list, f, aux->perm.p, aux->tris.p, aux->prim_box.p,
+                                        out->perm.p, out->phr_src.p, out->tris.p, out->prim_box.p,
+                                        int32_t(n), ctl);
+  phr_header_kernel<<<1, 1, 0, st>>>(outn, aux->header.p, ctl, d_ntrav, int32_t(n), out->header.p);
+  HIP_TRY(ctx, hipGetLastError());
+  rc = read_ctl(ctx, ctl, &hc);
+  if (rc) return rc;
+  if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: layout error 0x%x", hc.err);
+  // binary depth: each output level is at most ceil(log2(branching)) binary levels
+  int per_level = 0;
+  while ((1 << per_level) < branching) ++per_level;
+  out->host_depth = int32_t(hc.max_level) * per_level;
   return wide::widen(ctx, out);
 }
 
@@ -1591,6 +2079,11 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const size_t o_tidx = plan.take(out_cap * sizeof(uint32_t));
   const size_t o_scan = plan.take(scan::scratch_words(out_cap) * sizeof(uint32_t));
   const size_t o_ntr = plan.take(64);
+  // n-ary build: the groups of the current and the next step (every group's
+  // cut to split holds two or more members of the pool)
+  const int32_t group_cap = int32_t(pool_cap / 2 + 64);
+  const size_t o_ga = plan.take(branching > 2 ? group_cap * sizeof(Group) : 0);
+  const size_t o_gb = plan.take(branching > 2 ? group_cap * sizeof(Group) : 0);
   // status words of the canonical-order sort (at most n - 1 branches, 8 passes)
   const size_t o_cst = plan.take(onesweep::status_words(std::max<int64_t>(n, 1), 64, 0, onesweep::Config{}) *
                                  sizeof(uint32_t));
@@ -1618,6 +2111,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   auto* tidx = reinterpret_cast<uint32_t*>(base + o_tidx);
   auto* sscr = reinterpret_cast<uint32_t*>(base + o_scan);
   auto* n_trav = reinterpret_cast<uint32_t*>(base + o_ntr);
+  Group* gbuf[2] = {reinterpret_cast<Group*>(base + o_ga), reinterpret_cast<Group*>(base + o_gb)};
 
   HIP_TRY(ctx, out->phr_out.alloc(size_t(out_cap) * sizeof(OutNode)));
   OutNode* outn = reinterpret_cast<OutNode*>(out->phr_out.p);
@@ -1637,14 +2131,51 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   if (hc.cut <= 1) return clone_lbvh(ctx, aux, out);  // the whole LBVH is attached as is
 
   const Caps caps{pool_cap, out_cap, list_cap};
+  int cur = 0;
+  int levels = 0;
+  if (branching > 2) {
+    // one step splits one cut of every group (see nary_kernel)
+    const NCaps ncaps{pool_cap, out_cap, list_cap, group_cap};
+    nary_root_kernel<<<1, 1, 0, st>>>(jobs[0][kLarge], int32_t(n), gbuf[0]);
+    uint32_t n_groups = 1, max_m = hc.cut;
+    while (n_groups > 0) {
+      int lds_p = 0;
+      if (max_m > 64u && max_m <= uint32_t(kSortLds)) {
+        lds_p = 128;
+        while (uint32_t(lds_p) < max_m) lds_p <<= 1;
+      }
+      const size_t lds_bytes = size_t(lds_p) * (sizeof(double) + sizeof(int32_t));
+      reset_level_kernel<<<1, 1, 0, st>>>(ctl);
+      const int nb = int(n_groups);
+      int bsz = 64;
+      if (max_m > 64u && nb < 2048) bsz = (nb <= 128 && max_m > 256u) ? 512 : 256;
+      if (bsz == 512)
+        nary_kernel<512><<<nb, 512, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
+                                                       ord, gkey, rcost, outn, list, ctl, ncaps, lds_p, branching);
+      else if (bsz == 256)
+        nary_kernel<256><<<nb, 256, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
+                                                       ord, gkey, rcost, outn, list, ctl, ncaps, lds_p, branching);
+      else
+        nary_kernel<64><<<nb, 64, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
+                                                     ord, gkey, rcost, outn, list, ctl, ncaps, lds_p, branching);
+      rc = read_ctl(ctx, ctl, &hc);
+      if (rc) return rc;
+      if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x at step %d", hc.err, levels);
+      n_groups = hc.groups;
+      max_m = hc.max_m;
+      cur ^= 1;
+      ++levels;
+    }
+    out->phr_split_work = int64_t(hc.work);
+    return layout(ctx, aux, n, branching, f, outn, list, ctl, flag, tidx, sscr, n_trav, ord, gkey, rcost,
+                  reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, levels, out);
+  }
   uint32_t n_jobs[kLists] = {0, 0, 0, 1};  // the initial cut's job is on the large list
   uint32_t max_m = hc.cut;  // the one job of the first level holds the initial cut
   // Every job holds a cut of at least two members and makes one split attempt
   // (branching 2), so BuildWithCost's counter (phr.go:221) gains the level's
   // pool size: the initial cut, then what each level appends for the next.
   uint64_t level_pool = hc.cut;
-  int cur = 0;
-  int levels = 0;
   while (n_jobs[0] + n_jobs[1] + n_jobs[2] + n_jobs[3] > 0) {
     // LDS for the bitonic sort, sized to the level's largest cut
     // (cuts of up to 64 sort in registers; larger ones up to kSortLds in LDS,
@@ -1696,54 +2227,8 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     cur ^= 1;
     ++levels;
   }
-  const int32_t n_out = int32_t(hc.out);
-  out->phr_out_count = n_out;
-  out->phr_levels = levels;
-
-  branch_flag_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag);
-  count_leaves_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, ctl);
-  scan::exclusive_u32(st, flag, tidx, n_out, n_trav, sscr);
-  uint32_t h_ntrav = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(&h_ntrav, n_trav, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  if (h_ntrav > 1) {
-    // canonical order (the buffers of the refinement loop are free by now:
-    // branches number fewer than the LBVH leaves, so keys fit gkey / rcost
-    // and values the order buffer)
-    int prim_bits = 1, level_bits = 1;
-    while ((int64_t(1) << prim_bits) < n) ++prim_bits;
-    while ((int64_t(1) << level_bits) <= int64_t(hc.max_level)) ++level_bits;
-    if (int64_t(h_ntrav) > pool_cap || prim_bits + level_bits > 64)
-      TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: layout buffers too small");
-    auto* keys_a = reinterpret_cast<uint64_t*>(gkey);
-    auto* keys_b = reinterpret_cast<uint64_t*>(rcost);
-    auto* vals_a = reinterpret_cast<uint32_t*>(ord);
-    auto* vals_b = reinterpret_cast<uint32_t*>(ord + pool_cap);
-    auto* status = reinterpret_cast<uint32_t*>(base + o_cst);
-    canon_keys_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, prim_bits, keys_a, vals_a);
-    const int where = onesweep::sort_pairs(st, keys_a, vals_a, keys_b, vals_b, h_ntrav, prim_bits + level_bits,
-                                           status, ctx->d_err, false, nullptr, 0, onesweep::Config{});
-    canon_rank_kernel<<<int(ceil_div(h_ntrav, 256)), 256, 0, st>>>(where ? vals_b : vals_a, int32_t(h_ntrav), tidx);
-    rc = ctx_check_device_error(ctx, "trace_phr_build");
-    if (rc) return rc;
-  }
-  HIP_TRY(ctx, out->nodes.alloc(std::max<uint32_t>(1, h_ntrav)));
-  HIP_TRY(ctx, out->perm.alloc(n));
-  HIP_TRY(ctx, out->phr_src.alloc(n));
-  HIP_TRY(ctx, out->tris.alloc(9 * n));
-  HIP_TRY(ctx, out->prim_box.alloc(6 * n));
-  HIP_TRY(ctx, out->header.alloc(1));
-  trav_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, out->nodes.p);
-  leaf_copy_kernel<<<int(ceil_div(int64_t(n_out) * kCopyLanes, 256)), 256, 0, st>>>(outn, n_out, list, f, aux->perm.p, aux->tris.p, aux->prim_box.p,
-                                        out->perm.p, out->phr_src.p, out->tris.p, out->prim_box.p,
-                                        int32_t(n), ctl);
-  phr_header_kernel<<<1, 1, 0, st>>>(outn, aux->header.p, ctl, n_trav, int32_t(n), out->header.p);
-  HIP_TRY(ctx, hipGetLastError());
-  rc = read_ctl(ctx, ctl, &hc);
-  if (rc) return rc;
-  if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: layout error 0x%x", hc.err);
-  out->host_depth = int32_t(hc.max_level);
-  return wide::widen(ctx, out);
+  return layout(ctx, aux, n, branching, f, outn, list, ctl, flag, tidx, sscr, n_trav, ord, gkey, rcost,
+                reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, levels, out);
 }
 
 }  // namespace phr

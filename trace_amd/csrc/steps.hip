@@ -112,10 +112,12 @@ __device__ inline void sphere_hit(const float* __restrict__ p, const Ray64& r, d
 // Stack entries: -1 = the root (header), else (binary node << 1) | child slot.
 // The reference's stack holds at most one pending sibling per level plus the
 // two children just pushed: depth + 2 entries.
+// virt (optional): binary nodes that are not nodes of the tree (the binary
+// split of an n-ary branch) are walked but not counted.
 template <int STACK>
-__device__ int32_t walk(const BvhNode* __restrict__ nodes, int64_t n_nodes, const float* __restrict__ tris,
-                        const BvhHeader& hdr, const Ray64& r, double tmin, double tmax, bool& overflow,
-                        bool& bad) {
+__device__ int32_t walk(const BvhNode* __restrict__ nodes, int64_t n_nodes, const uint8_t* __restrict__ virt,
+                        const float* __restrict__ tris, const BvhHeader& hdr, const Ray64& r, double tmin,
+                        double tmax, bool& overflow, bool& bad) {
   int32_t st[STACK];
   int sp = 0;
   st[sp++] = -1;
@@ -164,7 +166,7 @@ __device__ int32_t walk(const BvhNode* __restrict__ nodes, int64_t n_nodes, cons
       }
       st[sp++] = ref << 1;
       st[sp++] = (ref << 1) | 1;
-      ++count;
+      if (!virt || !virt[ref]) ++count;
     }
   }
   return count;
@@ -172,6 +174,7 @@ __device__ int32_t walk(const BvhNode* __restrict__ nodes, int64_t n_nodes, cons
 
 template <int STACK>
 __global__ __launch_bounds__(256) void steps_kernel(const BvhNode* __restrict__ nodes, int64_t n_nodes,
+                                                    const uint8_t* __restrict__ virt,
                                                     const float* __restrict__ tris,
                                                     const BvhHeader* __restrict__ hdr_p,
                                                     const double* __restrict__ rays, int64_t n,
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(256) void steps_kernel(const BvhNode* __restrict__ 
   const double* q = rays + 8 * i;
   const Ray64 r = make_ray(q[0], q[1], q[2], q[4], q[5], q[6]);
   bool overflow = false, bad = false;
-  counts[i] = walk<STACK>(nodes, n_nodes, tris, hdr, r, q[3], q[7], overflow, bad);
+  counts[i] = walk<STACK>(nodes, n_nodes, virt, tris, hdr, r, q[3], q[7], overflow, bad);
   if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
   if (bad) atomicOr(d_err, TRACE_ERR_TREE);
 }
@@ -196,6 +199,7 @@ struct Cam {
 // colour by DefaultTraversalCountShader; pixel index y * W + x.
 template <int STACK>
 __global__ __launch_bounds__(256) void heatmap_kernel(const BvhNode* __restrict__ nodes, int64_t n_nodes,
+                                                      const uint8_t* __restrict__ virt,
                                                       const float* __restrict__ tris,
                                                       const BvhHeader* __restrict__ hdr_p, const Cam cam,
                                                       int32_t W, int32_t H, int32_t threshold,
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(256) void heatmap_kernel(const BvhNode* __restrict_
                            cam.ll[1] + cam.h[1] * s + cam.v[1] * t - cam.o[1],
                            cam.ll[2] + cam.h[2] * s + cam.v[2] * t - cam.o[2]);
   bool overflow = false, bad = false;
-  const int32_t c = walk<STACK>(nodes, n_nodes, tris, hdr, r, 0.001, DBL_MAX, overflow, bad);
+  const int32_t c = walk<STACK>(nodes, n_nodes, virt, tris, hdr, r, 0.001, DBL_MAX, overflow, bad);
   if (counts) counts[pix] = c;
   float* out = rgb + 3 * pix;
   if (c > threshold) {
@@ -250,10 +254,12 @@ int trace_traversal_steps_device(trace_ctx* ctx, const trace_bvh* bvh, const dou
   if (n == 0) return TRACE_OK;
   const int grid = int(ceil_div(n, 256));
   if (steps::deep(bvh))
-    steps::steps_kernel<264><<<grid, 256, 0, ctx->stream>>>(bvh->nodes_p(), steps::node_count(bvh), bvh->tris_p(),
+    steps::steps_kernel<264><<<grid, 256, 0, ctx->stream>>>(bvh->nodes_p(), steps::node_count(bvh), bvh->virt_p(),
+                                                            bvh->tris_p(),
                                                             bvh->header_p(), d_rays, n, d_counts, ctx->d_err);
   else
-    steps::steps_kernel<72><<<grid, 256, 0, ctx->stream>>>(bvh->nodes_p(), steps::node_count(bvh), bvh->tris_p(),
+    steps::steps_kernel<72><<<grid, 256, 0, ctx->stream>>>(bvh->nodes_p(), steps::node_count(bvh), bvh->virt_p(),
+                                                           bvh->tris_p(),
                                                            bvh->header_p(), d_rays, n, d_counts, ctx->d_err);
   HIP_TRY(ctx, hipGetLastError());
   return TRACE_OK;
@@ -295,11 +301,11 @@ int trace_render_heatmap_device(trace_ctx* ctx, const trace_bvh* bvh, const trac
   const int grid = int(ceil_div(int64_t(width) * height, 256));
   if (steps::deep(bvh))
     steps::heatmap_kernel<264><<<grid, 256, 0, ctx->stream>>>(bvh->nodes_p(), steps::node_count(bvh),
-                                                              bvh->tris_p(), bvh->header_p(), c, width, height,
+                                                              bvh->virt_p(), bvh->tris_p(), bvh->header_p(), c, width, height,
                                                               threshold, d_rgb, d_counts, ctx->d_err);
   else
     steps::heatmap_kernel<72><<<grid, 256, 0, ctx->stream>>>(bvh->nodes_p(), steps::node_count(bvh),
-                                                             bvh->tris_p(), bvh->header_p(), c, width, height,
+                                                             bvh->virt_p(), bvh->tris_p(), bvh->header_p(), c, width, height,
                                                              threshold, d_rgb, d_counts, ctx->d_err);
   HIP_TRY(ctx, hipGetLastError());
   return TRACE_OK;
