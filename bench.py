@@ -425,6 +425,13 @@ def bench_large(torch, ctx, steps=100):
     ev[3].record()
     torch.cuda.synchronize()
     render_rays = render_segments(ctx)
+    # byte accounting of the same render: B_seg = 32 (ray in) + 8 (hit out) +
+    # 64 per 4-wide node visited + 36 per primitive tested + 120 of path state
+    # (hit record, shading data, throughput, radiance, new ray, queue index)
+    from trace_amd.device import render_counts
+    segs, visits, tests = render_counts(ctx, mesh, phr, d["camera"], W, H, spp, depth, _lib.MISS_SKY, seed)
+    render_roof = {"bound": "hbm", "segments": segs, "nodes_per_segment": round(visits / segs, 2),
+                   "tris_per_segment": round(tests / segs, 2)}
     target = img / spp
     mesh.bind_params(alb.data_ptr(), emi.data_ptr())
     # the primal keeps every path's summary (16 B per path, 2.1 GB here) so
@@ -454,13 +461,17 @@ def bench_large(torch, ctx, steps=100):
     ctx.synchronize()
     render_ms = ev[2].elapsed_time(ev[3])
     build_ms = ev[0].elapsed_time(ev[1])
+    b_seg = 32 + 8 + 64 * visits / segs + 36 * tests / segs + 120
+    ach = b_seg * segs / (render_ms * 1e-3) / 1e9
+    render_roof.update({"bytes_per_segment": round(b_seg, 1), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)})
     best = SORT_CONFIGS[int(np.argmin(sort_ms))]
     out = {"tris": n, "terrain_tris": nt, "image": [W, H], "spp": spp, "max_depth": depth,
            "lbvh_build_ms": round(build_ms, 3),
            "lbvh_build_roofline": build_roofline(build_ms, n, 63, best[0]),
            "lbvh_sort_ms_by_config": {f"{c[0]}x{c[1]}": round(float(m), 3) for c, m in zip(SORT_CONFIGS, sort_ms)},
            "phr_build_ms": round(phr_ms, 2),
-           "render_ms": round(render_ms, 2), "render_rays": render_rays,
+           "render_ms": round(render_ms, 2), "render_rays": render_rays, "render_roofline": render_roof,
            "render_mrays_s": round(render_rays / (render_ms * 1e3), 1),
            "samples_per_s": round(W * H * spp / (render_ms * 1e-3), 1),
            "adam_steps": steps, "loop_s": round(sum(step_ms) * 1e-3, 3),

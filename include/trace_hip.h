@@ -332,6 +332,12 @@ int trace_render_heatmap_device(trace_ctx* ctx, const trace_bvh* bvh, const trac
  * sweeps).  Waits for the context stream.  No reference counterpart: the reference's
  * renderer keeps no ray count; this gives the rays-per-second figure of a render. */
 int trace_render_segments(trace_ctx* ctx, int64_t* out);
+/* Counting mode of trace_render (the image is discarded): h_counts[0] = path segments
+ * traced, [1] = nodes of the 4-wide traversal tree visited, [2] = primitives tested, over
+ * all segments.  Synchronous.  No reference counterpart: the byte count behind the
+ * render's bandwidth figure, as trace_closest_hit_counts is for closest hit. */
+int trace_render_counts(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
+                        const trace_render_params* prm, int64_t* h_counts);
 
 /* ---- differentiation (no reference counterpart) ----------------------------- */
 /* Bind per-triangle parameters: d_albedo replaces the colour of Diffuse triangles

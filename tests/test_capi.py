@@ -58,7 +58,11 @@ def test_no_device_here_is_reported():
 
     lib = _lib.load()
     h = C.c_void_p()
-    assert lib.trace_ctx_create(0, C.byref(h)) == _lib.TRACE_E_NODEVICE
+    rc = lib.trace_ctx_create(0, C.byref(h))
+    if rc == _lib.TRACE_OK:  # the library found a device that torch does not report
+        lib.trace_ctx_destroy(h)
+        pytest.skip("a device is present")
+    assert rc == _lib.TRACE_E_NODEVICE
 
 
 @pytest.mark.gpu

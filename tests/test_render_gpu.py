@@ -69,6 +69,32 @@ def test_segment_count_of_furnace_sphere(ctx):
     assert render_segments(ctx) == n_miss + 2 * n_hit
 
 
+def test_render_counts(ctx):
+    """trace_render_counts traces the render's own segments (the same count as
+    trace_render_segments) and sums their node visits and triangle tests: a
+    convex sphere under a white sky has one segment per miss and two per hit,
+    and a miss pixel's ray visits no triangle."""
+    from trace_amd.device import render_counts, render_segments
+
+    print("render counting mode", flush=True)
+    pos = scenes.uv_sphere((0, 0, 0), 1.0, 24, 14).astype(np.float32).astype(np.float64)
+    mi = np.zeros(len(pos), np.int32)
+    mats = np.array([[scenes.MAT_DIFFUSE, 0.6, 0.6, 0.6]], np.float32)
+    cam = scenes.Camera(2.0, 60.0, (0, 0, 3.0), (0, 0, 0), (0, 1, 0))
+    mesh, bvh = _setup(ctx, pos, mi, mats)
+    _, _, last = render(ctx, mesh, bvh, cam, 128, 64, 1, 5, _lib.MISS_WHITE, seed=7)
+    want = render_segments(ctx)
+    segs, nodes, tris = render_counts(ctx, mesh, bvh, cam, 128, 64, 1, 5, _lib.MISS_WHITE, 7)
+    assert segs == want
+    n_hit = int(np.count_nonzero(np.all(last == np.float32(0.6), axis=-1)))
+    assert tris >= n_hit and nodes >= n_hit
+    s2, n2, t2 = render_counts(ctx, mesh, bvh, cam, 128, 64, 2, 5, _lib.MISS_WHITE, 7)
+    assert s2 > segs and n2 > nodes and t2 > tris
+    # the counting run leaves ordinary renders unchanged
+    _, _, last2 = render(ctx, mesh, bvh, cam, 128, 64, 1, 5, _lib.MISS_WHITE, seed=7)
+    assert np.array_equal(last, last2)
+
+
 def test_depth_rule_value_set(ctx, oracle):
     print("render depth rule", flush=True)
     sc = scenes.box_scene(seed=3, slices=16, rings=10, width=96, height=54)

@@ -104,6 +104,7 @@ SIGNATURES = {
     "trace_render_keep": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_backward_kept": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
     "trace_render_segments": [_P, _P],
+    "trace_render_counts": [_P, _P, _P, _P, _P, _P],
     "trace_mesh_set_shading_ex": [_P, _P, _P, _P, _P, _I32],
     "trace_scene_create": [_P, _P, _P, _P, _I32, _P, _P, _I32, _P],
     "trace_scene_compile": [_P, _P, _P, _P, _P, _I32, _P],

@@ -49,6 +49,9 @@ struct trace_ctx {
   // alive (or a capture is open) the arena must not move
   int live_graphs = 0;
   bool capturing = false;
+  // set during trace_render_counts: the path-segment kernels sum their
+  // wide-node visits and triangle tests here (two 64-bit words)
+  unsigned long long* count_visits = nullptr;
 };
 
 #define TRACE_FAIL(ctx, code, ...)                                \

@@ -201,6 +201,39 @@ int trace_render_segments(trace_ctx* ctx, int64_t* out) {
   return TRACE_OK;
 }
 
+int trace_render_counts(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const trace_camera* cam,
+                        const trace_render_params* prm, int64_t* h_counts) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!mesh || !bvh || !cam || !prm || !h_counts)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_counts: null argument");
+  if (!mesh->has_shading) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_counts: mesh has no shading data");
+  if (!bvh->base && bvh->nseg > 1)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_counts: render a forest through its segment views");
+  const int64_t tree_tris = bvh->base ? bvh->base->n_prims : bvh->n_prims;
+  if (tree_tris != mesh->ntri) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_counts: bvh and mesh differ");
+  if (prm->width < 2 || prm->height < 2 || prm->spp < 1 || prm->max_depth < 0 || prm->sample_offset < 0 ||
+      prm->miss < TRACE_MISS_DEFAULT || prm->miss > TRACE_MISS_SUN ||
+      int64_t(prm->width) * prm->height > (int64_t(1) << 28))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_counts: bad render parameters");
+  const size_t px = size_t(prm->width) * prm->height * 3;
+  DevArray<float> img;
+  DevArray<unsigned long long> cnt;
+  HIP_TRY(ctx, img.alloc(2 * px));
+  HIP_TRY(ctx, cnt.alloc(2));
+  fill_async<unsigned long long>(ctx->stream, cnt.p, 2, 0ull);
+  ctx->count_visits = cnt.p;
+  int rc = render::render(ctx, mesh, bvh, cam, prm, img.p, img.p + px, nullptr);
+  ctx->count_visits = nullptr;
+  if (rc) return rc;
+  unsigned long long v[3] = {0, 0, 0};
+  HIP_TRY(ctx, hipMemcpyAsync(v, segment_counter(ctx), sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(v + 1, cnt.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 3; ++k) h_counts[k] = int64_t(v[k]);
+  return ctx_check_device_error(ctx, "trace_render_counts");
+}
+
 int trace_mesh_export(trace_ctx* ctx, const trace_mesh* mesh, double* h_pos) {
   if (!ctx) return TRACE_E_INVALID;
   if (!mesh || !h_pos) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_mesh_export: null argument");

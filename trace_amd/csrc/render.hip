@@ -259,6 +259,7 @@ struct SegArgs {
   int64_t path_base;
   float4 *ro, *rd, *thr, *rad;
   unsigned long long* seg_total;  // segments traced since the call began
+  unsigned long long* visits;     // counting mode: wide-node visits, triangle tests (else null)
 };
 
 // Shades segment k of path p after its closest hit h (primitive.go:145-150,
@@ -587,7 +588,9 @@ int refresh_geo_normals(trace_ctx* ctx, trace_mesh* mesh) {
 // Eight waves per SIMD (64 VGPRs, a few spilled): the render's late bounces
 // are latency-bound, and the extra waves hide more of it than the spills cost
 // (measured on the 10.4M-triangle frame: 125 ms at 6 waves, 118 at 7, 113 at 8).
-template <int THREADS, int STACK, bool SPH, bool SPEC>
+// CNT: the counting form, which also sums the wide-node visits and triangle
+// tests of its segments (for the byte accounting of the render's roofline).
+template <int THREADS, int STACK, bool SPH, bool SPEC, bool CNT = false>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void bounce_kernel(
     const WideNode* __restrict__ wnodes, const SegArgs g,
     const int32_t* __restrict__ q_in, int32_t* __restrict__ q_out, const uint32_t* __restrict__ count_in,
@@ -600,20 +603,30 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
   int32_t p = 0;
   int32_t al_tri = -1, em_tri = -1;  // replay: triangles that receive a gradient
   float4 al_w = make_float4(0.0f, 0.0f, 0.0f, 0.0f), em_w = al_w;
+  int32_t nn = 0, nt = 0;
   if (i < count) {
     p = q_in[i];
     const BvhHeader hdr = *g.scenes[p / g.pps].hdr;
     const float4 o4 = g.ro[p];
     const float4 d4 = g.rd[p];
     traverse::Ray ray{o4.x, o4.y, o4.z, g.k == 0 ? 0.001f : 0.0001f, d4.x, d4.y, d4.z, INFINITY};
-    int32_t nn = 0, nt = 0;
     bool overflow = false, bad = false;
-    traverse::Hit h = traverse::closest_hit_wide<STACK, false, SPH>(
+    traverse::Hit h = traverse::closest_hit_wide<STACK, CNT, SPH>(
         wnodes, g.tris, hdr, ray, nn, nt, overflow, bad, s_stack + threadIdx.x, THREADS);
     if (overflow) atomicOr(d_err, TRACE_ERR_STACK);
     if (bad) atomicOr(d_err, TRACE_ERR_TREE);
     traverse::refine_hit_fp64(g.tris, ray, h);
     shade_segment<SPH, SPEC>(g, adj, p, o4, d4, h, push, al_tri, al_w, em_tri, em_w);
+  }
+  if constexpr (CNT) {
+    for (int o = 32; o > 0; o >>= 1) {
+      nn += __shfl_xor(nn, o, 64);
+      nt += __shfl_xor(nt, o, 64);
+    }
+    if (__lane_id() == 0 && (nn | nt)) {
+      atomicAdd(g.visits, static_cast<unsigned long long>(nn));
+      atomicAdd(g.visits + 1, static_cast<unsigned long long>(nt));
+    }
   }
   segment_epilogue(adj, g.pps, p, push, al_tri, al_w, em_tri, em_w, q_out, count_out);
 }
@@ -742,6 +755,17 @@ static void launch_segments(trace_ctx* ctx, const trace_bvh* bvh, int grid, cons
                             int32_t* qout, uint32_t* cin, uint32_t* cout, int4* hits, const Adj& adj) {
   hipStream_t st = ctx->stream;
   const bool deep = bvh->depth_bound() > 63;
+  if constexpr (THREADS == 64) {
+    if (g.visits) {  // counting mode (64-thread workgroups only)
+      if (deep)
+        bounce_kernel<64, 512, SPH, SPEC, true><<<grid, 64, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin, cout,
+                                                                     ctx->d_err, adj);
+      else
+        bounce_kernel<64, 128, SPH, SPEC, true><<<grid, 64, 0, st>>>(bvh->wnodes_p(), g, qin, qout, cin, cout,
+                                                                     ctx->d_err, adj);
+      return;
+    }
+  }
   if (ctx->tune.render_persistent) {
     // resident grid (up to 32 waves per CU); queue slots are claimed from a counter
     const int pgrid = int(std::min<int64_t>(grid, int64_t(ctx->n_cu) * 32 * 64 / THREADS));
@@ -788,6 +812,7 @@ static void launch_bounce(trace_ctx* ctx, const trace_mesh* mesh, const trace_bv
   g.thr = thr;
   g.rad = rad;
   g.seg_total = segment_counter(ctx);
+  g.visits = ctx->count_visits;
   // the shader's sphere and specular branches are compiled in only for
   // meshes that need them
   const bool spec = mesh->has_specular;
@@ -878,7 +903,7 @@ void bounces(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const
   for (int32_t k = 0; k <= prm->max_depth + 1; ++k) {
     const int in = k & 1, out = in ^ 1;
     fill_async<uint32_t>(st, w.counts + out, 1, 0u);
-    const int threads = ctx->tune.render_threads;
+    const int threads = ctx->count_visits ? 64 : ctx->tune.render_threads;
     const int grid = int(ceil_div(npaths, threads));
     if (threads == 64)
       launch_bounce<64>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,

@@ -429,6 +429,16 @@ def render_backward_kept(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, he
                                                  C.c_void_p(d_grad_emission)))
 
 
+def render_counts(ctx: Context, mesh: Mesh, bvh: Bvh, cam, width: int, height: int, spp: int, max_depth: int,
+                  miss: int, seed: int):
+    """trace_render_counts: (segments, wide-node visits, primitive tests) of a render."""
+    c = camera_struct(cam)
+    p = render_params(width, height, spp, max_depth, miss, seed)
+    out = np.zeros(3, np.int64)
+    ctx.check(ctx.lib.trace_render_counts(ctx.h, mesh.h, bvh.h, C.byref(c), C.byref(p), _ptr(out)))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
 def render_segments(ctx: Context) -> int:
     """trace_render_segments: rays traced by the context's last render or
     backward call (waits for the context stream)."""
