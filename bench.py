@@ -380,6 +380,54 @@ def build_roofline(ms, ntri, key_mode_bits, sort_bits):
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
 
 
+def bench_instancing(torch, ctx, steps):
+    """The large configuration's 5,000 sphere placements as instances of one
+    400-triangle tree under a top level over the instances (plus the terrain
+    as one more instance), against the flattened 10.38M-triangle scene: build
+    time, closest hit on the 1920x1080 camera rays, device memory."""
+    from trace_amd import scenes
+    from trace_amd.device import Bvh, Mesh
+
+    meshes, _, _, node_mesh, mats = scenes.terrain_scene_graph()
+    terrain = meshes[0].astype(np.float32).astype(np.float64)
+    unit = meshes[1].astype(np.float32).astype(np.float64)
+    xforms = mats.reshape(-1, 4, 4)[:, :3, :].reshape(-1, 12)
+    base = np.concatenate([[0], len(terrain) + len(unit) * np.arange(len(node_mesh) - 1)]).astype(np.int64)
+    flat = np.concatenate([terrain] + [(unit.reshape(-1, 3) @ x.reshape(3, 4)[:, :3].T + x.reshape(3, 4)[:, 3])
+                                       .reshape(-1, 9) for x in xforms[1:]])
+    d = scenes.terrain_scene()
+    rays = torch.from_numpy(scenes.camera_rays(d["camera"], d["width"], d["height"]).astype(np.float32)).cuda()
+    n = rays.shape[0]
+    hit = torch.empty(n, dtype=torch.int32, device="cuda")
+    t = torch.empty(n, dtype=torch.float32, device="cuda")
+    fmesh = Mesh(ctx, flat)
+    fl = Bvh.lbvh(ctx, fmesh)
+    per_fb, _ = time_launches(torch, lambda: fl.rebuild(fmesh), 3, 1)
+    blas = [Bvh.lbvh(ctx, Mesh(ctx, terrain)), Bvh.lbvh(ctx, Mesh(ctx, unit))]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    inst = Bvh.instances(ctx, blas, node_mesh, xforms, base)
+    tlas_ms = (time.perf_counter() - t0) * 1e3
+    tmesh = Mesh(ctx, terrain)
+    per_tb, _ = time_launches(torch, lambda: blas[0].rebuild(tmesh), 3, 1)
+    per_f, _ = time_launches(torch, lambda: fl.closest_hit_device(rays.data_ptr(), n, hit.data_ptr(), t.data_ptr()),
+                             steps, 2)
+    per_i, _ = time_launches(torch, lambda: inst.closest_hit_device(rays.data_ptr(), n, hit.data_ptr(), t.data_ptr()),
+                             steps, 2)
+    ctx.synchronize()
+    fb = fl.device_bytes()
+    ib = inst.device_bytes() + sum(b.device_bytes() for b in blas)
+    out = {"instances": int(len(node_mesh)), "flattened_tris": int(len(flat)), "rays": int(n),
+           "flattened_build_ms": round(float(np.median(per_fb)), 3),
+           "terrain_build_ms": round(float(np.median(per_tb)), 3), "tlas_build_wall_ms": round(tlas_ms, 2),
+           "flattened_mrays_s": round(n / (float(np.median(per_f)) * 1e3), 1),
+           "instanced_mrays_s": round(n / (float(np.median(per_i)) * 1e3), 1),
+           "flattened_tree_bytes": fb, "instanced_tree_bytes": ib, "bytes_saved": fb - ib}
+    for b in (inst, fl, *blas):
+        b.close()
+    return out
+
+
 def bench_large(torch, ctx, steps=100):
     """The large configuration: 10.38M triangles (2048x2048 fBm terrain + 5000
     400-triangle spheres), PHR tree, 1920x1080, 64 spp, depth 8, sky miss;
@@ -759,6 +807,8 @@ def main():
         if not args.no_large:
             note("large scene (10.4M triangles) and inverse-rendering steps")
             out["large"] = bench_large(torch, ctx, args.adam_steps)
+            note("large scene instanced: one sphere tree placed 5000 times")
+            out["instancing"] = bench_instancing(torch, ctx, max(3, args.steps // 4))
             note("large scene compiled on the device from its scene graph")
             out["scene_compile"] = bench_scene_compile(torch, ctx, max(3, args.steps // 4))
 

@@ -216,6 +216,26 @@ int trace_bayes_minimize(const double* lo, const double* hi, int32_t dims, int32
  * built from (relates PHR leaves to the LBVH order). */
 int trace_phr_export_source(trace_ctx* ctx, const trace_bvh* phr, int32_t* h_src);
 
+/* ---- instancing (no reference counterpart) ------------------------------------ */
+/* A two-level tree for scenes that place meshes many times.  The reference flattens every
+ * placement into one triangle list (Scene.collectTracables, scene.go:100-114); here
+ * instance i places bottom-level tree h_blas[h_inst_blas[i]] (the LBVH or PHR tree of a
+ * triangle mesh in its own coordinates, or a segment view of a forest) with the affine
+ * object-to-world transform h_xform[12 i .. 12 i + 11] (rows (A | t): world = A object + t,
+ * A invertible), and its triangles take the flattened indices h_tri_base[i] + (the mesh's
+ * own index).  The top level is an LBVH over the instances' world boxes.  trace_closest_hit
+ * and trace_closest_hit_device accept the result and report flattened indices; rays are
+ * moved into each instance's space in FP64 and the winner is re-solved there in FP64, so a
+ * hit agrees with the flattened scene's up to rounding at triangle edges (ties go to the
+ * larger flattened index).  Export, Cost and Size describe the top level; renders,
+ * traversal steps and PHR refinement refuse it (TRACE_E_UNSUPPORTED).  The bottom-level
+ * trees must outlive the result and must not hold spheres. */
+int trace_instances_build(trace_ctx* ctx, const trace_bvh* const* h_blas, int32_t n_blas, const int32_t* h_inst_blas,
+                          const double* h_xform, const int64_t* h_tri_base, int32_t n_inst, trace_bvh** out);
+/* Device memory a tree holds, in bytes (a segment view holds none; an instanced tree's
+ * count leaves out its bottom-level trees). */
+int trace_bvh_device_bytes(trace_ctx* ctx, const trace_bvh* bvh, int64_t* bytes);
+
 /* ---- closest hit -------------------------------------------------------- */
 /* Bulk BVH.intersected (bvh.go:74-77, 179-199 + primitive.go:172-208): per ray the
  * closest triangle with t in [tmin, tmax]; hit = triangle index or -1, t = hit

@@ -81,6 +81,8 @@ SIGNATURES = {
     "trace_phr_export_source": [_P, _P, _P],
     "trace_closest_hit": [_P, _P, _P, _I64, _P, _P],
     "trace_closest_hit_device": [_P, _P, _P, _I64, _P, _P],
+    "trace_instances_build": [_P, _P, C.c_int32, _P, _P, _P, C.c_int32, _P],
+    "trace_bvh_device_bytes": [_P, _P, _P],
     "trace_closest_hit_counts": [_P, _P, _P, _I64, _P, _P],
     "trace_mesh_set_shading": [_P, _P, _P, _P, _P, _I32],
     "trace_render": [_P, _P, _P, _P, _P, _P, _P, _P],

@@ -373,6 +373,7 @@ int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double d
   if (!aux || !out) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: null argument");
   *out = nullptr;
   if (aux->key_mode < 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: the auxiliary tree must be an LBVH");
+  if (aux->instanced) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: instanced trees are not refined");
   if (aux->base || aux->nseg > 1) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: forests and segment views are not refined by this build");
   if (branching < 2 || branching > 8)
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_build: branching factor %d (this build supports 2 to 8)", branching);

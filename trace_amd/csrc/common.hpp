@@ -261,6 +261,14 @@ struct trace_bvh {
   // PHR trees of branching above 2: per binary traversal node, 1 when it is
   // virtual (an inner node of a branch's binary split, not a node of the tree)
   DevArray<uint8_t> virt;
+  // instanced scenes (trace_instances_build): this tree is the top level over
+  // the instances' world boxes, its sorted primitives are instances
+  bool instanced = false;
+  int32_t n_inst = 0;
+  int32_t blas_depth = 0;  // deepest bottom-level tree (sizes the second stack)
+  int64_t flat_prims = 0;  // triangles of the flattened scene
+  DevArray<uint8_t> inst;  // per instance: transform, tree, first flattened index
+  DevArray<uint8_t> blas;  // per bottom-level tree: its arrays and header
   bool phr_clone = false;  // the PHR tree is the LBVH itself (initial cut of one node)
   double phr_alpha = 0, phr_delta = 0;
   int64_t phr_initial_cut = 0;

@@ -842,6 +842,7 @@ struct Wave {
 int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const trace_camera* cams,
             const trace_render_params* prm, Wave& w) {
   const trace_bvh* bvh = trees[0];  // every tree shares bvh's node and triangle arrays
+  if (bvh->instanced) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "render: instanced trees are traced by closest hit only");
   if (bvh->depth_bound() > 255)
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "render: tree depth %d exceeds 255", bvh->depth_bound());
   const int64_t wh = int64_t(prm->width) * prm->height;

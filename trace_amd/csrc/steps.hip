@@ -249,6 +249,7 @@ int trace_traversal_steps_device(trace_ctx* ctx, const trace_bvh* bvh, const dou
   if (!ctx) return TRACE_E_INVALID;
   if (!bvh || n < 0 || (n > 0 && (!d_rays || !d_counts)))
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_traversal_steps: bad arguments");
+  if (bvh->instanced) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_traversal_steps: instanced trees are not walked");
   if (bvh->depth_bound() > 255)
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_traversal_steps: tree depth %d exceeds 255", bvh->depth_bound());
   if (n == 0) return TRACE_OK;
@@ -289,6 +290,7 @@ int trace_render_heatmap_device(trace_ctx* ctx, const trace_bvh* bvh, const trac
   if (!ctx) return TRACE_E_INVALID;
   if (!bvh || !cam || !d_rgb || width < 2 || height < 2 || threshold < 1)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_render_heatmap: bad arguments (width, height >= 2, threshold >= 1)");
+  if (bvh->instanced) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_heatmap: instanced trees are not walked");
   if (bvh->depth_bound() > 255)
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_render_heatmap: tree depth %d exceeds 255", bvh->depth_bound());
   steps::Cam c;

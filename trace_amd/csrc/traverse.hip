@@ -1,6 +1,7 @@
 // Bulk closest-hit kernel: one ray per lane, rays streamed from HBM.
 #include <algorithm>
 
+#include "instances.hpp"
 #include "traverse_dev.hpp"
 
 namespace traverse {
@@ -220,6 +221,10 @@ int closest_hit(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays, int64
   if (n > int64_t(INT32_MAX))
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "closest hit: %lld rays exceed the 2^31 - 1 one call takes",
                static_cast<long long>(n));
+  if (bvh->instanced) {
+    if (d_nodes || d_tris) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "closest hit: no counting mode on instanced trees");
+    return instances::closest_hit(ctx, bvh, d_rays, n, d_hit, d_t);
+  }
   if (bvh->depth_bound() > 255)
     TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "closest hit: tree depth %d exceeds 255", bvh->depth_bound());
   if (!bvh->base && bvh->nseg > 1)

@@ -259,11 +259,11 @@ __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeade
 // false when the ray is done.
 // n_bound = the tree's wide-node count, n_prims = its triangle count:
 // bounds for every reference followed.
-template <int STACK, bool COUNT, bool SPH>
-__device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
-                                 const float* __restrict__ tris, uint32_t n_bound, int32_t n_prims,
-                                 WideState& s, TravStack<STACK>& stack, int32_t& n_nodes,
-                                 int32_t& n_tris, bool& overflow, bool& bad) {
+// leaf(first, count) tests the primitives of a hit leaf against s.h.
+template <int STACK, bool COUNT, class Leaf>
+__device__ inline bool wide_step_with(const WideNode* __restrict__ wnodes, uint32_t n_bound, WideState& s,
+                                      TravStack<STACK>& stack, int32_t& n_nodes, bool& overflow, bool& bad,
+                                      Leaf&& leaf) {
   if (uint32_t(s.node) >= n_bound || ++s.visits > n_bound) {
     bad = true;
     return false;
@@ -320,7 +320,7 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
     leaves &= leaves - 1u;
     const int32_t first = k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3];
     const int32_t count = int32_t(k == 0 ? cnt[0] : k == 1 ? cnt[1] : k == 2 ? cnt[2] : cnt[3]);
-    leaf_test<COUNT, SPH>(r, tris, first, count, n_prims, s.h, n_tris, bad);
+    leaf(first, count);
   }
   // inner children still in reach, sorted by entry distance
   float key[4];
@@ -360,6 +360,17 @@ __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
   }
   s.node = id[0];
   return true;
+}
+
+template <int STACK, bool COUNT, bool SPH>
+__device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
+                                 const float* __restrict__ tris, uint32_t n_bound, int32_t n_prims,
+                                 WideState& s, TravStack<STACK>& stack, int32_t& n_nodes,
+                                 int32_t& n_tris, bool& overflow, bool& bad) {
+  return wide_step_with<STACK, COUNT>(wnodes, n_bound, s, stack, n_nodes, overflow, bad,
+                                      [&](int32_t first, int32_t count) {
+                                        leaf_test<COUNT, SPH>(s.r, tris, first, count, n_prims, s.h, n_tris, bad);
+                                      });
 }
 
 // Whole-ray wide traversal (used where one lane keeps one ray).

@@ -242,6 +242,28 @@ class Bvh:
         b._aux = aux
         return b, int(cost.value)
 
+    @classmethod
+    def instances(cls, ctx: Context, blas, inst_blas, xforms, tri_base) -> "Bvh":
+        """trace_instances_build: instance i places blas[inst_blas[i]] with the affine
+        object-to-world transform xforms[i] (3 x 4 rows (A | t)); its triangles are
+        numbered from tri_base[i] in the flattened order."""
+        arr = (C.c_void_p * len(blas))(*[b.h.value for b in blas])
+        ib = np.ascontiguousarray(inst_blas, np.int32)
+        xf = np.ascontiguousarray(xforms, np.float64).reshape(-1, 12)
+        tb = np.ascontiguousarray(tri_base, np.int64)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.trace_instances_build(ctx.h, arr, len(blas), _ptr(ib), _ptr(xf), _ptr(tb), len(ib),
+                                                C.byref(h)))
+        b = cls(ctx, h, int(max(int(tb[i]) + blas[int(ib[i])].n_prims for i in range(len(ib)))))
+        b._blas = list(blas)  # the bottom-level trees must outlive the instanced one
+        return b
+
+    def device_bytes(self) -> int:
+        """trace_bvh_device_bytes."""
+        v = C.c_int64()
+        self.ctx.check(self.ctx.lib.trace_bvh_device_bytes(self.ctx.h, self.h, C.byref(v)))
+        return int(v.value)
+
     def segment(self, seg: int) -> "Bvh":
         """View of tree `seg` of a forest (trace_bvh_segment)."""
         h = C.c_void_p()
