@@ -300,16 +300,20 @@ __device__ inline bool is_head(const uint64_t* __restrict__ keys, int64_t i,
   return tseg && tseg[perm[i]] != tseg[perm[i - 1]];
 }
 
+// A tile of kUTile sorted positions is read as kUPerThread rows of kUThreads
+// consecutive positions (row j, thread t: position j * kUThreads + t), so every
+// load of a row is one coalesced sweep.
 __global__ __launch_bounds__(kUThreads) void count_heads_kernel(const uint64_t* __restrict__ keys,
                                                                 int64_t n,
                                                                 const int32_t* __restrict__ tseg,
                                                                 const uint32_t* __restrict__ perm,
                                                                 uint32_t* blk_cnt) {
   __shared__ uint32_t s_wave[4];
-  const int64_t base = blockIdx.x * int64_t(kUTile) + threadIdx.x * int64_t(kUPerThread);
+  const int64_t base = blockIdx.x * int64_t(kUTile) + threadIdx.x;
   uint32_t c = 0;
+#pragma unroll 4
   for (int j = 0; j < kUPerThread; ++j) {
-    const int64_t i = base + j;
+    const int64_t i = base + int64_t(j) * kUThreads;
     if (i < n) c += is_head(keys, i, tseg, perm) ? 1u : 0u;
   }
   uint32_t tot;
@@ -337,25 +341,55 @@ __global__ __launch_bounds__(256) void scan_blocks_kernel(uint32_t* blk, int64_t
   }
 }
 
+// Same row layout as count_heads_kernel.  Heads keep sorted order: the rank of
+// the head at (row j, wave w, lane l) is the tile's offset, plus the heads of
+// every earlier (row, wave) pair, plus the heads of lower lanes in its wave.
 __global__ __launch_bounds__(kUThreads) void write_unique_kernel(
     const uint64_t* __restrict__ keys, int64_t n, const uint32_t* __restrict__ blk_off,
     const int32_t* __restrict__ tseg, const uint32_t* __restrict__ perm,
     uint64_t* __restrict__ ukeys, uint32_t* __restrict__ run_start, int32_t* __restrict__ useg,
     BvhHeader* hdr) {
-  __shared__ uint32_t s_wave[4];
-  const int64_t base = blockIdx.x * int64_t(kUTile) + threadIdx.x * int64_t(kUPerThread);
+  constexpr int kWaves = kUThreads / 64;
+  constexpr int kPairs = kUPerThread * kWaves;
+  static_assert(kPairs == 64, "one wave scans the (row, wave) head counts");
+  __shared__ uint32_t s_off[kPairs];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t base = blockIdx.x * int64_t(kUTile) + threadIdx.x;
   uint32_t heads = 0;
+#pragma unroll 4
   for (int j = 0; j < kUPerThread; ++j) {
-    const int64_t i = base + j;
-    if (i < n && is_head(keys, i, tseg, perm)) heads |= 1u << j;
+    const int64_t i = base + int64_t(j) * kUThreads;
+    const bool hd = i < n && is_head(keys, i, tseg, perm);
+    if (hd) heads |= 1u << j;
+    const uint64_t m = __ballot(hd);
+    if (lane == 0) s_off[j * kWaves + w] = uint32_t(__popcll(m));
   }
-  uint32_t r = blk_off[blockIdx.x] + block_excl_scan256(__popc(heads), s_wave, nullptr);
+  __syncthreads();
+  if (w == 0) {
+    // exclusive scan over the 64 (row, wave) counts, in row-major order
+    const uint32_t v = s_off[lane];
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    s_off[lane] = x - v;
+  }
+  __syncthreads();
+  const uint32_t tile0 = blk_off[blockIdx.x];
+  const uint64_t below = (uint64_t(1) << lane) - 1u;
+#pragma unroll 4
   for (int j = 0; j < kUPerThread; ++j) {
-    if (heads & (1u << j)) {
-      ukeys[r] = keys[base + j];
-      run_start[r] = uint32_t(base + j);
-      if (useg) useg[r] = tseg[perm[base + j]];
-      ++r;
+    const bool hd = (heads >> j) & 1u;
+    const uint64_t m = __ballot(hd);
+    if (hd) {
+      const int64_t i = base + int64_t(j) * kUThreads;
+      const uint32_t r = tile0 + s_off[j * kWaves + w] + uint32_t(__popcll(m & below));
+      ukeys[r] = keys[i];
+      run_start[r] = uint32_t(i);
+      if (useg) useg[r] = tseg[perm[i]];
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) run_start[hdr->n_leaves] = uint32_t(n);
@@ -507,7 +541,9 @@ __device__ inline float ld_f32(const float* p) {
 // LDS, indexed by the node's own key index (a Karras node's index is one end
 // of its range).  Nodes whose range crosses the span are met by two
 // workgroups: their box words cross through agent-scope atomics whose
-// completion is awaited before the global counter is bumped.
+// completion is awaited before the global counter is bumped.  A climb is a
+// chain of dependent loads, so each node's range and parent link are loaded
+// one level ahead, while the arrival below them is still in flight.
 constexpr int kRefitThreads = 256;
 
 __global__ __launch_bounds__(kRefitThreads) void refit_kernel(
@@ -538,6 +574,13 @@ __global__ __launch_bounds__(kRefitThreads) void refit_kernel(
     }
   }
   const int32_t n_inner = u - nseg;  // every tree of the forest has one leaf more than inner nodes
+  int32_t link = parent_leaf[leaf];
+  int32_t f = 0, l = 0, par = -1;
+  if (link >= 0 && (link >> 1) < n_inner) {
+    f = inner_range[2 * (link >> 1)];
+    l = inner_range[2 * (link >> 1) + 1];
+    par = parent_inner[link >> 1];
+  }
   float lo[3], hi[3];
   const uint32_t s = run_start[leaf], e = run_start[leaf + 1];
 #pragma unroll
@@ -553,8 +596,7 @@ __global__ __launch_bounds__(kRefitThreads) void refit_kernel(
     }
   }
   int32_t h = 0;
-  int32_t link = parent_leaf[leaf];
-  // A radix tree over 64-bit keys is at most 64 levels deep; the bound and
+  // A radix tree over 64-bit keys is at most 65 levels deep; the bound and
   // the index check only guard against a malformed hierarchy.
   for (int step = 0; link >= 0; ++step) {
     const int32_t node = link >> 1;
@@ -563,9 +605,14 @@ __global__ __launch_bounds__(kRefitThreads) void refit_kernel(
       atomicOr(d_err, TRACE_ERR_TREE);
       return;
     }
+    int32_t nf = 0, nl = 0, npar = -1;
+    if (par >= 0 && (par >> 1) < n_inner) {
+      nf = inner_range[2 * (par >> 1)];
+      nl = inner_range[2 * (par >> 1) + 1];
+      npar = parent_inner[par >> 1];
+    }
     float* nb = reinterpret_cast<float*>(&nodes[node]);
     float* my = nb + 6 * slot;
-    const int32_t f = inner_range[2 * node], l = inner_range[2 * node + 1];
     if (f >= span0 && l < span0 + kRefitThreads) {
       // local node: key index = node index (+ segment number in a forest)
       const int32_t key = node + (useg ? useg[f] : 0);
@@ -593,35 +640,38 @@ __global__ __launch_bounds__(kRefitThreads) void refit_kernel(
       }
       const int32_t ho = l_h[li][slot ^ 1];
       h = 1 + (h > ho ? h : ho);
-      link = parent_inner[node];
-      continue;
-    }
+    } else {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      st_f32(my + a, lo[a]);
-      st_f32(my + 3 + a, hi[a]);
-    }
-    __hip_atomic_store(heights + 2 * node + slot, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // The box and height words are agent-scope atomic stores (written through
-    // to the device coherence point); waiting for their completion before the
-    // arrival counter is bumped makes them visible to whichever workgroup, on
-    // whichever XCD, arrives second.  A workgroup-scope fence alone emits no
-    // wait for global memory, and the counter could overtake the boxes.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(0);
-    const uint32_t prev = atomicAdd(counters + node, 1u);
-    if (prev == 0) return;  // sibling not done yet; it will continue
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const float* other = nb + 6 * (slot ^ 1);
+      for (int a = 0; a < 3; ++a) {
+        st_f32(my + a, lo[a]);
+        st_f32(my + 3 + a, hi[a]);
+      }
+      __hip_atomic_store(heights + 2 * node + slot, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // The box and height words are agent-scope atomic stores (written
+      // through to the device coherence point); waiting for their completion
+      // before the arrival counter is bumped makes them visible to whichever
+      // workgroup, on whichever XCD, arrives second.  A workgroup-scope fence
+      // alone emits no wait for global memory, and the counter could overtake
+      // the boxes.
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_s_waitcnt(0);
+      const uint32_t prev = atomicAdd(counters + node, 1u);
+      if (prev == 0) return;  // sibling not done yet; it will continue
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const float* other = nb + 6 * (slot ^ 1);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = fminf(lo[a], ld_f32(other + a));
-      hi[a] = fmaxf(hi[a], ld_f32(other + 3 + a));
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fminf(lo[a], ld_f32(other + a));
+        hi[a] = fmaxf(hi[a], ld_f32(other + 3 + a));
+      }
+      const int32_t ho = __hip_atomic_load(heights + 2 * node + (slot ^ 1), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      h = 1 + (h > ho ? h : ho);
     }
-    const int32_t ho = __hip_atomic_load(heights + 2 * node + (slot ^ 1), __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-    h = 1 + (h > ho ? h : ho);
-    link = parent_inner[node];
+    link = par;
+    f = nf;
+    l = nl;
+    par = npar;
   }
   // reached a root (inner node 0, or a forest tree's first inner node)
   if (useg) seg_hdr[useg[leaf]].depth = h;

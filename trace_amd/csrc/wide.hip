@@ -50,25 +50,52 @@ __global__ void parent_kernel(const BvhNode* __restrict__ nodes, const BvhHeader
     if (b.cnt[c] == 0 && b.ref[c] >= 0 && b.ref[c] < n_inner) parent[b.ref[c]] = i;
 }
 
-// flag[i] = 1 for binary inner nodes at even depth.  SHIFT 1: parent holds
-// encoded links, (parent << 1) | slot (-1 stays -1 under the shift).
+// Depth parity by pointer jumping.  jump[i] packs (ancestor << 32 | steps):
+// some ancestor of inner node i (-1 once past the root) and how many parent
+// links lead there.  Every value ever stored is such a pair, so a thread may
+// read another node's word at any moment (one 64-bit load: never torn) and
+// jump by it; each jump moves at least one level up and the jumps double as
+// other threads publish theirs, so a node reaches the root in about
+// log2(depth) dependent loads instead of depth of them.
+__device__ inline uint64_t jump_word(int32_t anc, uint32_t steps) {
+  return (uint64_t(uint32_t(anc)) << 32) | steps;
+}
+
+// SHIFT 1: parent holds encoded links, (parent << 1) | slot (-1 stays -1
+// under the shift).
 template <int SHIFT>
-__global__ void flag_kernel(const int32_t* __restrict__ parent, const BvhHeader* __restrict__ hdr,
-                            int64_t cap, uint32_t* __restrict__ flag, uint32_t* d_err) {
+__global__ void jump_init_kernel(const int32_t* __restrict__ parent, const BvhHeader* __restrict__ hdr,
+                                 int64_t cap, uint64_t* __restrict__ jump) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= cap || i >= hdr->n_inner) return;
+  const int32_t p = parent[i] >> SHIFT;
+  jump[i] = p >= 0 ? jump_word(p, 1u) : jump_word(-1, 0u);
+}
+
+// flag[i] = 1 for binary inner nodes at even depth.
+__global__ void flag_kernel(uint64_t* jump, const BvhHeader* __restrict__ hdr, int64_t cap,
+                            uint32_t* __restrict__ flag, uint32_t* d_err) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= cap) return;
-  if (i >= hdr->n_inner) {
+  const int32_t n_inner = hdr->n_inner;
+  if (i >= n_inner) {
     flag[i] = 0u;
     return;
   }
-  int d = 0;
-  for (int32_t p = parent[i] >> SHIFT; p >= 0; p = parent[p] >> SHIFT) {
-    if (++d > 4096) {  // deeper than any tree built here: a cycle
+  uint64_t w = jump[i];
+  int32_t anc = int32_t(uint32_t(w >> 32));
+  uint32_t steps = uint32_t(w);
+  for (int it = 0; anc >= 0; ++it) {
+    if (it > 4096 || anc >= n_inner || steps > 4096u) {  // deeper than any tree built here: a cycle
       atomicOr(d_err, TRACE_ERR_TREE);
       break;
     }
+    const uint64_t v = __hip_atomic_load(jump + anc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    anc = int32_t(uint32_t(v >> 32));
+    steps += uint32_t(v);
+    __hip_atomic_store(jump + i, jump_word(anc, steps), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  flag[i] = (d & 1) == 0 ? 1u : 0u;
+  flag[i] = (steps & 1u) == 0u ? 1u : 0u;
 }
 
 constexpr int64_t kSplitAbove = 4;  // leaves of more triangles get a sub-tree
@@ -374,15 +401,16 @@ __global__ void wroot_kernel(BvhHeader* __restrict__ hdrs, int32_t count,
 
 namespace {
 
-// Scratch layout of widen(): parent links, wide flags and indices of the
+// Scratch layout of widen(): parent links, depth jumps, wide flags and indices of the
 // binary inner nodes, two totals, then per entry (inner nodes and roots) the
 // sub-tree sizes of split leaves, their offsets and the scan's partials.
 struct WidePlan {
-  size_t o_par, o_flag, o_idx, o_tot, o_need, o_soff, o_scan, total;
+  size_t o_par, o_jump, o_flag, o_idx, o_tot, o_need, o_soff, o_scan, total;
   WidePlan(int64_t cap, int32_t n_roots) {
     const int64_t n_entries = cap + n_roots;
     ScratchPlan plan;
     o_par = plan.take(cap * sizeof(int32_t));
+    o_jump = plan.take(cap * sizeof(uint64_t));
     o_flag = plan.take(cap * sizeof(uint32_t));
     o_idx = plan.take(cap * sizeof(uint32_t));
     o_tot = plan.take(2 * sizeof(uint32_t));
@@ -435,13 +463,15 @@ int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links, const Workspace*
   auto* soff = reinterpret_cast<uint32_t*>(base + plan.o_soff);
   auto* sscr = reinterpret_cast<uint32_t*>(base + plan.o_scan);
   const int grid = int(ceil_div(cap, 256));
+  auto* jump = reinterpret_cast<uint64_t*>(base + plan.o_jump);
   if (links) {
-    flag_kernel<1><<<grid, 256, 0, st>>>(links, bvh->header.p, cap, flag, ctx->d_err);
+    jump_init_kernel<1><<<grid, 256, 0, st>>>(links, bvh->header.p, cap, jump);
   } else {
     fill_async<int32_t>(st, parent, size_t(cap), -1);
     parent_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, parent);
-    flag_kernel<0><<<grid, 256, 0, st>>>(parent, bvh->header.p, cap, flag, ctx->d_err);
+    jump_init_kernel<0><<<grid, 256, 0, st>>>(parent, bvh->header.p, cap, jump);
   }
+  flag_kernel<<<grid, 256, 0, st>>>(jump, bvh->header.p, cap, flag, ctx->d_err);
   scan::exclusive_u32(st, flag, widx, cap, total, sscr);
   split_count_kernel<<<int(ceil_div(n_entries, 256)), 256, 0, st>>>(bvh->nodes.p, bvh->header.p, cap, roots,
                                                                     n_roots, max_levels, need);
