@@ -856,7 +856,7 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
     seg_finalize_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(u_begin, bvh->run_start.p, nseg,
                                                             bvh->header.p, bvh->seg_headers.p);
   HIP_TRY(ctx, hipGetLastError());
-  return wide::widen(ctx, bvh, parent_inner, ws);
+  return wide::widen(ctx, bvh, parent_inner, ws, parent_leaf);
 }
 
 }  // namespace lbvh

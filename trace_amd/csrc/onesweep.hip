@@ -3,6 +3,9 @@
 
 namespace onesweep {
 
+// Status words a look-back step loads at once.
+constexpr int kLookBatch = 8;
+
 // Words exchanged between workgroups go through agent-scope atomics, which
 // bypass the per-XCD L2 on gfx950; workgroup-scope fences only order them.
 __device__ inline uint32_t ld_coherent(const uint32_t* p) {
@@ -49,14 +52,24 @@ __global__ __launch_bounds__(256) void histogram_kernel(const uint64_t* __restri
   extern __shared__ uint32_t sh[];  // bins words
   for (int i = threadIdx.x; i < bins; i += blockDim.x) sh[i] = 0;
   __syncthreads();
+  const int lane = threadIdx.x & 63;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     const uint64_t k = keys[i];
     const uint32_t s = seg ? uint32_t(seg[i]) : 0u;
+    const uint64_t act = __ballot(1);
+    const int first = __builtin_amdgcn_readfirstlane(lane);
     for (int p = 0; p < pd.n; ++p) {
       const uint32_t mask = (1u << pd.bits[p]) - 1u;
       const uint32_t d = pd.seg[p] ? (s >> pd.shift[p]) & mask : uint32_t(k >> pd.shift[p]) & mask;
-      atomicAdd(&sh[pd.off[p] + d], 1u);
+      // spatially coherent keys share their high digits across a wave: one
+      // add of the wave's count then replaces 64 adds to one counter
+      const uint32_t d_first = __builtin_amdgcn_readfirstlane(d);
+      if (__ballot(d == d_first) == act) {
+        if (lane == first) atomicAdd(&sh[pd.off[p] + d_first], uint32_t(__popcll(act)));
+      } else {
+        atomicAdd(&sh[pd.off[p] + d], 1u);
+      }
     }
   }
   __syncthreads();
@@ -190,8 +203,10 @@ __global__ __launch_bounds__(kThreads) void onesweep_pass_kernel(
   } else {
 #pragma unroll
     for (int k = 0; k < kDpt; ++k) st_coherent(mine + k, kFlagAgg | tot[k]);
-    // every owned digit walks back over the preceding tiles; the walks of
-    // a thread's digits advance together so that their loads overlap
+    // every owned digit walks back over the preceding tiles, kLookBatch
+    // status words per step: the batch's loads are in flight together, and
+    // its words are then taken in tile order up to the first inclusive prefix
+    // (done) or the first tile that has published nothing yet (wait there)
     int64_t at[kDpt];
     uint32_t pending = (1u << kDpt) - 1u;
 #pragma unroll
@@ -205,15 +220,28 @@ __global__ __launch_bounds__(kThreads) void onesweep_pass_kernel(
 #pragma unroll
       for (int k = 0; k < kDpt; ++k) {
         if (!((pending >> k) & 1u)) continue;
-        const uint32_t v = ld_coherent(status + size_t(at[k]) * kRadix + d0 + k);
-        const uint32_t f = v & ~kValueMask;
-        if (f == 0) {
-          waited = true;
-          continue;
+        uint32_t v[kLookBatch];
+#pragma unroll
+        for (int b = 0; b < kLookBatch; ++b)
+          v[b] = at[k] - b >= 0 ? ld_coherent(status + size_t(at[k] - b) * kRadix + d0 + k) : kFlagPrefix;
+        bool stop = false;
+#pragma unroll
+        for (int b = 0; b < kLookBatch; ++b) {
+          if (stop) continue;
+          const uint32_t f = v[b] & ~kValueMask;
+          if (f == 0) {  // tile at[k] has published nothing yet
+            waited = true;
+            stop = true;
+            continue;
+          }
+          excl[k] += v[b] & kValueMask;
+          if (f == kFlagPrefix || at[k] == 0) {
+            pending &= ~(1u << k);
+            stop = true;
+          } else {
+            --at[k];
+          }
         }
-        excl[k] += v & kValueMask;
-        if (f == kFlagPrefix || at[k] == 0) pending &= ~(1u << k);
-        else --at[k];
       }
       if (waited && pending) {
         if (++spins > kSpinLimit) {

@@ -289,6 +289,24 @@ __global__ void split_count_kernel(const BvhNode* __restrict__ nodes, const BvhH
   need[e] = w;
 }
 
+// need[] from the LBVH leaves: zero for every inner node except the parents of
+// leaves over kSplitAbove triangles (rare: a run of equal keys), which add
+// their sub-tree sizes; root entries as split_count_kernel.  Reads one run
+// start per leaf instead of every node.
+__global__ void leaf_need_kernel(const uint32_t* __restrict__ run_start, const int32_t* __restrict__ leaf_links,
+                                 const BvhHeader* __restrict__ hdr, int64_t cap, const BvhHeader* __restrict__ roots,
+                                 int32_t n_roots, int max_levels, uint32_t* __restrict__ need) {
+  const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (r < n_roots) {
+    const int32_t rc = roots[r].root_cnt;
+    need[cap + r] = rc > 0 ? split_nodes(rc, max_levels) : 0u;
+  }
+  if (r >= hdr->n_leaves) return;
+  const int64_t c = int64_t(run_start[r + 1]) - int64_t(run_start[r]);
+  const int32_t link = leaf_links[r];
+  if (c > kSplitAbove && link >= 0 && (link >> 1) < cap) atomicAdd(&need[link >> 1], split_nodes(c, max_levels));
+}
+
 // One sub-tree node per thread (grid-stride over the device-side total):
 // finds its leaf by binary search over the offsets, then its level and
 // index, and writes its four children (sub-tree nodes, or leaves of the
@@ -434,7 +452,7 @@ int64_t wide_nodes(int64_t inner_cap, int64_t n_prims) {
   return std::max<int64_t>(1, inner_cap) + 4 * n_prims / (3 * kSplitLeaf) + 1;
 }
 
-int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links, const Workspace* ws) {
+int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links, const Workspace* ws, const int32_t* leaf_links) {
   const int64_t cap = std::max<int64_t>(1, int64_t(bvh->nodes.n));
   const int32_t n_roots = bvh->nseg > 1 ? bvh->nseg : 1;
   BvhHeader* roots = bvh->nseg > 1 ? bvh->seg_headers.p : bvh->header.p;
@@ -473,8 +491,14 @@ int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links, const Workspace*
   }
   flag_kernel<<<grid, 256, 0, st>>>(jump, bvh->header.p, cap, flag, ctx->d_err);
   scan::exclusive_u32(st, flag, widx, cap, total, sscr);
-  split_count_kernel<<<int(ceil_div(n_entries, 256)), 256, 0, st>>>(bvh->nodes.p, bvh->header.p, cap, roots,
-                                                                    n_roots, max_levels, need);
+  if (links && leaf_links) {
+    fill_async<uint32_t>(st, need, size_t(cap), 0u);
+    leaf_need_kernel<<<int(ceil_div(std::max<int64_t>(bvh->n_prims, n_roots), 256)), 256, 0, st>>>(
+        bvh->run_start.p, leaf_links, bvh->header.p, cap, roots, n_roots, max_levels, need);
+  } else {
+    split_count_kernel<<<int(ceil_div(n_entries, 256)), 256, 0, st>>>(bvh->nodes.p, bvh->header.p, cap, roots,
+                                                                      n_roots, max_levels, need);
+  }
   scan::exclusive_u32(st, need, soff, n_entries, total + 1, sscr);
   const SplitMap sm{soff, total, max_levels};
   widen_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, flag, widx, sm, bvh->wnodes.p, ctx->d_err);
