@@ -456,6 +456,10 @@ def bench_large(torch, ctx, steps=100):
     phr = Bvh.phr(ctx, lbvh)
     ctx.synchronize()
     phr_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    phr.phr_rebuild(lbvh)
+    ctx.synchronize()
+    phr_rebuild_ms = (time.perf_counter() - t0) * 1e3
     true_alb = torch.from_numpy(d["albedo"]).cuda()
     emi = torch.from_numpy(d["emission"]).cuda()
     alb = true_alb.clone()
@@ -519,6 +523,7 @@ def bench_large(torch, ctx, steps=100):
            "lbvh_build_roofline": build_roofline(build_ms, n, 63, best[0]),
            "lbvh_sort_ms_by_config": {f"{c[0]}x{c[1]}": round(float(m), 3) for c, m in zip(SORT_CONFIGS, sort_ms)},
            "phr_build_ms": round(phr_ms, 2),
+           "phr_rebuild_ms": round(phr_rebuild_ms, 2),
            "render_ms": round(render_ms, 2), "render_rays": render_rays, "render_roofline": render_roof,
            "render_mrays_s": round(render_rays / (render_ms * 1e3), 1),
            "samples_per_s": round(W * H * spp / (render_ms * 1e-3), 1),
@@ -651,6 +656,12 @@ def main():
     t0 = time.perf_counter()
     phr = Bvh.phr(ctx, lbvh)
     phr_ms = (time.perf_counter() - t0) * 1e3
+    # steady state: rebuilt into the same storage (no allocation)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        phr.phr_rebuild(lbvh)
+    ctx.synchronize()
+    phr_rebuild_ms = (time.perf_counter() - t0) * 1e3 / 3
 
     d_rays = torch.from_numpy(rays).cuda()
     n = rays.shape[0]
@@ -723,6 +734,7 @@ def main():
                      "ms_one_ray_per_lane": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms[:3])},
                      "ms_persistent": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms[3:])}},
         "phr_build_ms": round(phr_ms, 2),
+        "phr_rebuild_ms": round(phr_rebuild_ms, 2),
     }
 
     if not args.quick:

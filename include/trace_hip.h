@@ -166,6 +166,14 @@ int trace_bvh_size(trace_ctx* ctx, const trace_bvh* bvh, int64_t* size);
  * accept it, and the export reports each branch's own child count. */
 int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,
                     int branching, int32_t max_cut, trace_bvh** out);
+/* trace_phr_build into the storage of an existing PHR tree `phr` (a handle that
+ * trace_phr_build returned; any aux, alpha, delta, branching and max_cut): the same
+ * tree a fresh build gives, without allocating when the storage is large enough --
+ * the reference builds a new tree per BuildFromAuxilary call (phr.go:47-84), which a
+ * caller that rebuilds per frame or per optimiser candidate replaces with this.  On an
+ * error the tree's contents are undefined until a rebuild succeeds. */
+int trace_phr_rebuild(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int branching,
+                      int32_t max_cut, trace_bvh* phr);
 /* PhrBuilder.BuildWithCost (phr.go:154-255): trace_phr_build plus its build cost -- the
  * initial cut's size plus the size of every cut handed to SweepSAH. */
 int trace_phr_build_with_cost(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,

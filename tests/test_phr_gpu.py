@@ -216,6 +216,63 @@ def test_phr_rejects_unsupported(ctx):
         Bvh.phr(ctx, aux, max_cut=0)
 
 
+def _same_tree(a, b):
+    ra, ba = a.canonical()
+    rb, bb = b.canonical()
+    assert np.array_equal(ra, rb) and np.array_equal(ba, bb)
+    assert np.array_equal(a.phr_source(), b.phr_source())
+    assert np.array_equal(a.perm(), b.perm())
+    assert a.cost() == b.cost() and a.size() == b.size()
+
+
+def test_phr_rebuild_in_place(ctx, oracle):
+    """trace_phr_rebuild gives the tree a fresh trace_phr_build gives, whatever the
+    tree held before: other settings, other branching factors (binary after n-ary
+    and back), a one-node initial cut (the LBVH itself) and another mesh."""
+    print("phr rebuild in place", flush=True)
+    pos = scenes.random_triangles(30_000, seed=31)
+    aux = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    tree = Bvh.phr(ctx, aux)
+    cases = [(0.55, 9.0, 2, 2500), (0.5, 6.0, 4, 2500), (0.4, 7.0, 2, 100), (0.5, 6.0, 8, 2500),
+             (0.5, 6.0, 2, 1), (0.45, 8.0, 3, 2500), (0.5, 6.0, 2, 2500)]
+    for alpha, delta, bf, cap in cases:
+        tree.phr_rebuild(aux, alpha, delta, bf, cap)
+        fresh = Bvh.phr(ctx, aux, alpha, delta, bf, cap)
+        _same_tree(tree, fresh)
+        assert tree.info().arity == fresh.info().arity
+        fresh.close()
+    # the last case against the oracle, and the traversal of the rebuilt tree
+    rec, box = tree.canonical()
+    orec, obox, _ = oracle.PhrTree(oracle.Tree(pos), 0.5, 6.0, 2, 2500).canonical()
+    assert np.array_equal(rec[:, 0], orec[:, 0]) and np.array_equal(box.astype(np.float64), obox)
+    rays = scenes.incoherent_rays(20_000, seed=32)
+    hit, t = tree.closest_hit(rays)
+    ohit, ot = oracle.Tree(pos).closest_hit(rays)
+    assert np.mean(hit == ohit) >= 0.9999
+    # another, larger mesh into the same storage
+    pos2 = scenes.random_triangles(60_000, seed=33)
+    aux2 = Bvh.lbvh(ctx, Mesh(ctx, pos2))
+    tree.phr_rebuild(aux2)
+    fresh = Bvh.phr(ctx, aux2)
+    _same_tree(tree, fresh)
+
+
+def test_phr_rebuild_refusals(ctx):
+    pos = scenes.random_triangles(1000, seed=34)
+    aux = Bvh.lbvh(ctx, Mesh(ctx, pos))
+    tree = Bvh.phr(ctx, aux)
+    with pytest.raises(_lib.TraceError):  # the target is an LBVH
+        aux.phr_rebuild(aux)
+    with pytest.raises(_lib.TraceError):  # the auxiliary tree is a PHR tree
+        tree.phr_rebuild(tree)
+    for bf in (1, 9):
+        with pytest.raises(_lib.TraceError):
+            tree.phr_rebuild(aux, branching=bf)
+    # a refused call leaves the tree as it was
+    fresh = Bvh.phr(ctx, aux)
+    _same_tree(tree, fresh)
+
+
 # ---- branching factors above 2 (phr.go:92-152, scene.go:32-36) ---------------------------
 
 

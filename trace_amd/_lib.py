@@ -78,6 +78,7 @@ SIGNATURES = {
     "trace_bvh_cost": [_P, _P, _DP],
     "trace_bvh_size": [_P, _P, C.POINTER(_I64)],
     "trace_phr_build": [_P, _P, C.c_double, C.c_double, C.c_int, _I32, C.POINTER(_P)],
+    "trace_phr_rebuild": [_P, _P, C.c_double, C.c_double, C.c_int, _I32, _P],
     "trace_phr_export_source": [_P, _P, _P],
     "trace_closest_hit": [_P, _P, _P, _I64, _P, _P],
     "trace_closest_hit_device": [_P, _P, _P, _I64, _P, _P],

@@ -219,16 +219,18 @@ struct PhrObjective {
   int64_t max_build;
   double omega;
   std::vector<double> rows;  // alpha, delta, Cost, build cost, objective
+  trace_bvh* tree = nullptr;  // every evaluation is rebuilt into the first one's storage
+  ~PhrObjective() {
+    if (tree) trace_bvh_destroy(tree);
+  }
 };
 
-int phr_eval(trace_ctx* ctx, const trace_bvh* aux, double a, double d, int branching, int32_t max_cut,
-             double* sah, int64_t* build) {
-  trace_bvh* t = nullptr;
-  int rc = trace_phr_build_with_cost(ctx, aux, a, d, branching, max_cut, &t, build);
+int phr_eval(PhrObjective* o, double a, double d, double* sah, int64_t* build) {
+  int rc = o->tree ? trace_phr_rebuild(o->ctx, o->aux, a, d, o->branching, o->max_cut, o->tree)
+                   : trace_phr_build(o->ctx, o->aux, a, d, o->branching, o->max_cut, &o->tree);
   if (rc) return rc;
-  rc = trace_bvh_cost(ctx, t, sah);
-  trace_bvh_destroy(t);
-  return rc;
+  *build = o->tree->phr_initial_cut + o->tree->phr_split_work;
+  return trace_bvh_cost(o->ctx, o->tree, sah);
 }
 
 // evalPHR (optimizer.go:92-94) of one (alpha, delta), with the reference's
@@ -239,7 +241,7 @@ int phr_objective(void* user, const double* x, double* y) {
   double sah = o->max_sah;
   int64_t build = o->max_build;
   if (a != 0.0 && d != 0.0) {
-    const int rc = phr_eval(o->ctx, o->aux, a, d, o->branching, o->max_cut, &sah, &build);
+    const int rc = phr_eval(o, a, d, &sah, &build);
     if (rc) return rc;
   }
   *y = double(build) / double(o->max_build) + (o->omega * o->omega) * (sah / o->max_sah);
@@ -296,14 +298,14 @@ int trace_phr_optimize_bayes(trace_ctx* ctx, const trace_bvh* aux, const double*
       random_rounds < 0 || !alpha_out || !delta_out || !(alpha_range[0] <= alpha_range[1]) ||
       !(delta_range[0] <= delta_range[1]))
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_optimize_bayes: bad arguments");
-  PhrObjective o{ctx, aux, branching, max_cut, 0.0, 0, 0.0, {}};
+  PhrObjective o{ctx, aux, branching, max_cut, 0.0, 0, 0.0, {}, nullptr};
   // normalisers (optimizer.go:136-140): Cost() at the range minimum, the build
   // cost at the range maximum
   int64_t build_lo = 0;
   double sah_hi = 0.0;
-  int rc = phr_eval(ctx, aux, alpha_range[0], delta_range[0], branching, max_cut, &o.max_sah, &build_lo);
+  int rc = phr_eval(&o, alpha_range[0], delta_range[0], &o.max_sah, &build_lo);
   if (rc) return rc;
-  rc = phr_eval(ctx, aux, alpha_range[1], delta_range[1], branching, max_cut, &sah_hi, &o.max_build);
+  rc = phr_eval(&o, alpha_range[1], delta_range[1], &sah_hi, &o.max_build);
   if (rc) return rc;
   // evalPHR divides by both normalisers; the reference's objective is NaN
   // everywhere when either is zero, so the search is refused up front

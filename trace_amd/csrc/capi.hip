@@ -391,6 +391,26 @@ int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double d
   return TRACE_OK;
 }
 
+int trace_phr_rebuild(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int branching,
+                      int32_t max_cut, trace_bvh* phr) {
+  if (!ctx) return TRACE_E_INVALID;
+  if (!aux || !phr) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: null argument");
+  if (phr == aux || phr->key_mode >= 0 || phr->instanced || phr->base || phr->nseg > 1)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: the target must be a PHR tree made by trace_phr_build");
+  if (aux->key_mode < 0) TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: the auxiliary tree must be an LBVH");
+  if (aux->instanced) TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_rebuild: instanced trees are not refined");
+  if (aux->base || aux->nseg > 1)
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_rebuild: forests and segment views are not refined by this build");
+  if (branching < 2 || branching > 8)
+    TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_rebuild: branching factor %d (this build supports 2 to 8)",
+               branching);
+  if (!(alpha >= 0) || !std::isfinite(delta) || max_cut < 1)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: need alpha >= 0, finite delta, max_cut >= 1");
+  int rc = phr::build(ctx, aux, alpha, delta, branching, max_cut, phr);
+  if (rc == TRACE_OK) rc = ctx_check_device_error(ctx, "trace_phr_rebuild");
+  return rc;
+}
+
 int trace_phr_build_with_cost(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta,
                               int branching, int32_t max_cut, trace_bvh** out, int64_t* build_cost) {
   if (!ctx) return TRACE_E_INVALID;
@@ -412,18 +432,25 @@ int trace_phr_optimize_grid(trace_ctx* ctx, const trace_bvh* aux, const double* 
   const int64_t n_eval = int64_t(n_alpha) * n_delta;
   std::vector<double> sah(n_eval);
   std::vector<int64_t> build(n_eval);
-  // every (alpha, delta), alphas outer (optimizer.go:47-61)
+  // every (alpha, delta), alphas outer (optimizer.go:47-61), each rebuilt
+  // into the storage of the first
+  trace_bvh* t = nullptr;
   for (int32_t i = 0; i < n_alpha; ++i) {
     for (int32_t j = 0; j < n_delta; ++j) {
       const int64_t k = int64_t(i) * n_delta + j;
-      trace_bvh* t = nullptr;
-      int rc = trace_phr_build_with_cost(ctx, aux, h_alphas[i], h_deltas[j], branching, max_cut, &t, &build[k]);
-      if (rc) return rc;
-      rc = trace_bvh_cost(ctx, t, &sah[k]);
-      trace_bvh_destroy(t);
-      if (rc) return rc;
+      int rc = t ? trace_phr_rebuild(ctx, aux, h_alphas[i], h_deltas[j], branching, max_cut, t)
+                 : trace_phr_build(ctx, aux, h_alphas[i], h_deltas[j], branching, max_cut, &t);
+      if (rc == TRACE_OK) {
+        build[k] = t->phr_initial_cut + t->phr_split_work;
+        rc = trace_bvh_cost(ctx, t, &sah[k]);
+      }
+      if (rc) {
+        if (t) trace_bvh_destroy(t);
+        return rc;
+      }
     }
   }
+  if (t) trace_bvh_destroy(t);
   // scores (optimizer.go:63-94): build / max build + omega^2 * SAH / max SAH,
   // omega = (prims / PRIM_THRESH)^2 * pixels / RES_THRESH; the first minimum wins
   int64_t max_build = 0;

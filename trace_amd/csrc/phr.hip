@@ -5,8 +5,11 @@
 // The reference recursion is a tree of independent jobs: a job holds a cut
 // (LBVH nodes, a bounding box, a depth), splits it with SweepSAH and refines
 // both halves into child jobs.  Here the jobs of one tree level run as one
-// launch per job list.  Jobs of more than 64 cut nodes run one wave per job,
-// with the cut staged through memory:
+// launch per job list, and the levels are driven from the device: each
+// level's launches read its job counts from the control block, so the host
+// enqueues levels in batches and reads the counters once per batch.  Jobs of
+// more than 64 cut nodes run one workgroup per job, with the cut staged
+// through memory:
 //   sort      three stable sorts of the cut by box centre x, y, z (rank by
 //             counting: rank = #smaller + #equal-and-earlier), with the z sort
 //             fed from the y or x order exactly as phr.go:384-404 does
@@ -47,6 +50,7 @@ constexpr int kWaveCut = 64;
 constexpr int kLists = 4;
 constexpr int kLarge = 3;
 constexpr int kSortLds = 4096;  // cuts of 65 .. this size sort in LDS (bitonic); larger ones by ranking
+constexpr int kSmallLds = 1024;  // bitonic size of the device-driven launches without the full sort LDS
 constexpr int kMaxDepth = 1024;
 
 struct Flat {  // LBVH as flat ids: 0..U-2 inner nodes, U-1..2U-2 leaves
@@ -91,9 +95,16 @@ struct Ctl {
   TRACE_CTL_FIELD(cut);        // initial cut size
   TRACE_CTL_FIELD(max_m);      // largest cut among the large jobs appended for the next level
   TRACE_CTL_FIELD(groups);     // n-ary build: groups appended for the next step
+  TRACE_CTL_FIELD(done);       // device-driven levels: workgroups of the level's last launch that finished
   unsigned long long work;     // n-ary build: sizes of the cuts handed to SweepSAH (phr.go:221)
   unsigned long long work_pad[15];
   uint32_t jobs[kLists][32];   // jobs appended for the next level, per list (element 0)
+  // device-driven levels (advance_level_kernel): the current level's job
+  // counts, largest cut, bitonic LDS size and workgroup size of its large
+  // jobs, and the levels that held jobs
+  uint32_t cur_jobs[kLists];
+  uint32_t cur_max_m, cur_lds, sel_b, levels;
+  uint32_t cur_pad[24];
 };
 #undef TRACE_CTL_FIELD
 
@@ -735,19 +746,14 @@ __device__ void emit_half(const Flat& f, const int32_t* ids, const int32_t* C, i
 
 // One workgroup of B threads per job of the current level (phr.go:92-152
 // with BranchingFactor 2).
+// One job of the large list, by the whole workgroup (s_dyn: lds_p keys then
+// lds_p positions of dynamic LDS, 0 = no LDS sort).
 template <int B>
-__global__ __launch_bounds__(B) void task_kernel(Flat f, const double* __restrict__ th,
-                                                 const Task* __restrict__ tasks,
-                                                 const int32_t* __restrict__ pool,
-                                                 int32_t* __restrict__ pool_next,
-                                                 Lists next, int32_t* __restrict__ ordbuf, double* gkey,
-                                                 double* rcost_buf, OutNode* __restrict__ out,
-                                                 int32_t* __restrict__ list, Ctl* ctl, Caps caps,
-                                                 int lds_p) {
-  extern __shared__ double s_dyn[];  // P keys then P positions (P = lds_p, 0 = no LDS sort)
-  __shared__ BlockLds s_blk;
+__device__ void task_job(const Task t, Flat f, const double* __restrict__ th, const int32_t* __restrict__ pool,
+                         int32_t* __restrict__ pool_next, Lists next, int32_t* __restrict__ ordbuf, double* gkey,
+                         double* rcost_buf, OutNode* __restrict__ out, int32_t* __restrict__ list, Ctl* ctl,
+                         Caps caps, int lds_p, double* s_dyn, BlockLds& s_blk) {
   const int tid = threadIdx.x;
-  const Task t = tasks[blockIdx.x];
   const int m = t.cnt;
   // bitonic size of this job: its own cut rounded up to a power of two (the
   // level's LDS holds the largest), or 0 to sort by ranking
@@ -840,6 +846,31 @@ __global__ __launch_bounds__(B) void task_kernel(Flat f, const double* __restric
       write_out(out, t.out, 0, 1, o, -1, -1, 0, t.prim_off, int32_t(pc), t.level, t.box);
       raise_max(&ctl->max_level, uint32_t(t.level + 1));
     }
+  }
+}
+
+// Device-driven levels: the workgroup size advance_level_kernel chose for
+// this level's large jobs runs them, grid-stride; lds_cap bounds the bitonic
+// size this launch reserved LDS for (larger cuts sort by ranking, which
+// gives the same order).
+template <int B>
+__global__ __launch_bounds__(B) void task_dev_kernel(Flat f, const double* __restrict__ th,
+                                                     const Task* __restrict__ tasks,
+                                                     const int32_t* __restrict__ pool,
+                                                     int32_t* __restrict__ pool_next, Lists next,
+                                                     int32_t* __restrict__ ordbuf, double* gkey,
+                                                     double* rcost_buf, OutNode* __restrict__ out,
+                                                     int32_t* __restrict__ list, Ctl* ctl, Caps caps,
+                                                     int lds_cap) {
+  extern __shared__ double s_dyn[];
+  __shared__ BlockLds s_blk;
+  if (ctl->sel_b != uint32_t(B)) return;
+  const uint32_t n = ctl->cur_jobs[kLarge];
+  const int lds_p = min(int(ctl->cur_lds), lds_cap);
+  for (uint32_t j = blockIdx.x; j < n; j += gridDim.x) {
+    task_job<B>(tasks[j], f, th, pool, pool_next, next, ordbuf, gkey, rcost_buf, out, list, ctl, caps, lds_p,
+                s_dyn, s_blk);
+    __syncthreads();  // the next job reuses the LDS
   }
 }
 
@@ -1217,13 +1248,11 @@ __device__ inline uint32_t wave_alloc(uint32_t* counter, uint32_t n) {
   return __shfl(base, 0, 64) + ex;
 }
 
-__global__ __launch_bounds__(256) void small_task_kernel(Flat f, const double* __restrict__ th,
-                                                         const Task* __restrict__ tasks, int32_t n_jobs,
-                                                         const int32_t* __restrict__ pool,
-                                                         int32_t* __restrict__ pool_next, Lists next,
-                                                         OutNode* __restrict__ out, int32_t* __restrict__ list,
-                                                         Ctl* ctl, Caps caps) {
-  const int32_t job = blockIdx.x * 256 + int32_t(threadIdx.x);
+// One job per lane; every lane of the wave calls it (the allocations are
+// wave-wide).
+__device__ void small_job(int32_t job, Flat f, const double* __restrict__ th, const Task* __restrict__ tasks,
+                          int32_t n_jobs, const int32_t* __restrict__ pool, int32_t* __restrict__ pool_next,
+                          Lists next, OutNode* __restrict__ out, int32_t* __restrict__ list, Ctl* ctl, Caps caps) {
   bool valid = job < n_jobs;
   Task t{};
   int m = 0;
@@ -1483,19 +1512,16 @@ __device__ inline Split group_cost(const Mem& a, int g, int gbase, int m, const 
   return Split{nosplit, 0};
 }
 
+// One job per G lanes; every lane of the wave calls it.
 template <int G>
-__global__ __launch_bounds__(256) void group_task_kernel(Flat f, const double* __restrict__ th,
-                                                         const Task* __restrict__ tasks, int32_t n_jobs,
-                                                         const int32_t* __restrict__ pool,
-                                                         int32_t* __restrict__ pool_next, Lists next,
-                                                         OutNode* __restrict__ out, int32_t* __restrict__ list,
-                                                         Ctl* ctl, Caps caps) {
+__device__ void group_job(int32_t job, Flat f, const double* __restrict__ th, const Task* __restrict__ tasks,
+                          int32_t n_jobs, const int32_t* __restrict__ pool, int32_t* __restrict__ pool_next,
+                          Lists next, OutNode* __restrict__ out, int32_t* __restrict__ list, Ctl* ctl, Caps caps) {
   static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "groups are powers of two within a wave");
   const int lane = threadIdx.x & 63;
   const int g = lane & (G - 1);
   const int gbase = lane - g;
   const bool lead = g == 0;
-  const int32_t job = int32_t((blockIdx.x * 256u + threadIdx.x) / G);
   // every lane of a group reads the same job, so control flow stays uniform
   // per group; all cross-lane steps below run with the whole wave active
   bool valid = job < n_jobs;
@@ -1673,6 +1699,105 @@ __global__ __launch_bounds__(256) void group_task_kernel(Flat f, const double* _
     }
     pnext += int32_t(kh);
   }
+}
+
+// Device-driven levels: at the start of a level, the counts its jobs appended
+// become the current ones (the kernels of the level read those) and the
+// counters restart for the next level; BuildWithCost's counter (phr.go:221)
+// gains the level's pool.  Also picks the large jobs' workgroup size and LDS
+// sort size: 512 threads for a few jobs of big cuts and for cuts that need
+// more than kSmallLds of bitonic sort, 64 for thousands of jobs, else 256.
+// The words are read and written with device-scope atomics: this runs at the
+// end of a level's last launch, whose other workgroups updated them.
+__device__ inline uint32_t ctl_ld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void ctl_st(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void advance_level(Ctl* ctl) {
+  uint32_t total = 0;
+  for (int r = 0; r < kLists; ++r) {
+    const uint32_t c = ctl_ld(&ctl->jobs[r][0]);
+    ctl_st(&ctl->cur_jobs[r], c);
+    total += c;
+    ctl_st(&ctl->jobs[r][0], 0u);
+  }
+  const uint32_t mm = ctl_ld(&ctl->max_m);
+  const uint32_t nb = ctl_ld(&ctl->cur_jobs[kLarge]);
+  uint32_t lds_p = 0;
+  if (mm > 64u && mm <= uint32_t(kSortLds)) {
+    lds_p = 128;
+    while (lds_p < mm) lds_p <<= 1;
+  }
+  uint32_t bsz = 64;
+  if (mm > 64u && nb < 2048u) bsz = ((nb <= 128u && mm > 256u) || lds_p > uint32_t(kSmallLds)) ? 512u : 256u;
+  ctl_st(&ctl->cur_max_m, mm);
+  ctl_st(&ctl->cur_lds, lds_p);
+  ctl_st(&ctl->sel_b, bsz);
+  if (total > 0) {
+    const uint32_t pool = ctl_ld(&ctl->pool);
+    __hip_atomic_fetch_add(&ctl->work, (unsigned long long)pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ctl_st(&ctl->levels, ctl_ld(&ctl->levels) + 1u);
+  }
+  ctl_st(&ctl->pool, 0u);
+  ctl_st(&ctl->max_m, 0u);
+  ctl_st(&ctl->groups, 0u);
+}
+
+__global__ void advance_level_kernel(Ctl* ctl) { advance_level(ctl); }
+
+// Small and group jobs of the current level, grid-stride over its count.
+// The small jobs run last in a level: the workgroup that finishes last
+// (arrival counter ctl->done) advances the counters for the next level.
+__global__ __launch_bounds__(256) void small_task_kernel(Flat f, const double* __restrict__ th,
+                                                         const Task* __restrict__ tasks,
+                                                         const int32_t* __restrict__ pool,
+                                                         int32_t* __restrict__ pool_next, Lists next,
+                                                         OutNode* __restrict__ out, int32_t* __restrict__ list,
+                                                         Ctl* ctl, Caps caps) {
+  __shared__ uint32_t s_last;
+  const int32_t n_jobs = int32_t(ctl->cur_jobs[0]);
+  for (int64_t base = blockIdx.x * int64_t(256); base < n_jobs; base += int64_t(gridDim.x) * 256)
+    small_job(int32_t(base) + int32_t(threadIdx.x), f, th, tasks, n_jobs, pool, pool_next, next, out, list, ctl,
+              caps);
+  // this workgroup's updates of the control block are device-scope atomics;
+  // waiting for their completion orders them before its arrival, and the
+  // last arrival reads the block with device-scope loads
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    s_last = atomicAdd(&ctl->done, 1u) == gridDim.x - 1 ? 1u : 0u;
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    ctl_st(&ctl->done, 0u);
+    advance_level(ctl);
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void group_task_kernel(Flat f, const double* __restrict__ th,
+                                                         const Task* __restrict__ tasks,
+                                                         const int32_t* __restrict__ pool,
+                                                         int32_t* __restrict__ pool_next, Lists next,
+                                                         OutNode* __restrict__ out, int32_t* __restrict__ list,
+                                                         Ctl* ctl, Caps caps) {
+  const int32_t n_jobs = int32_t(ctl->cur_jobs[G == kWaveCut ? 2 : 1]);
+  constexpr int kPerBlock = 256 / G;
+  for (int64_t base = blockIdx.x * int64_t(kPerBlock); base < n_jobs; base += int64_t(gridDim.x) * kPerBlock)
+    group_job<G>(int32_t(base) + int32_t(threadIdx.x / G), f, th, tasks, n_jobs, pool, pool_next, next, out, list,
+                 ctl, caps);
+}
+
+// The first device-driven level: the initial cut's job (on the large list)
+// and its pool.
+__global__ void seed_levels_kernel(Ctl* ctl) {
+  for (int r = 0; r < kLists; ++r) ctl->jobs[r][0] = r == kLarge ? 1u : 0u;
+  ctl->max_m = ctl->cut;
+  ctl->pool = ctl->cut;
+  ctl->work = 0;
+  ctl->levels = 0;
 }
 
 // Leaves of the output tree: one atomic per workgroup.
@@ -1919,6 +2044,7 @@ static int clone_lbvh(trace_ctx* ctx, const trace_bvh* aux, trace_bvh* out) {
   HIP_TRY(ctx, hipStreamSynchronize(st));
   out->phr_clone = true;
   out->host_depth = h.depth;
+  out->virt.release();
   return wide::widen(ctx, out);
 }
 
@@ -1999,6 +2125,7 @@ static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching
     nary_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, out->nodes.p,
                                                                  out->virt.p);
   } else {
+    out->virt.release();  // binary: no virtual nodes (a rebuild of a tree that was n-ary)
     trav_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, out->nodes.p);
   }
   leaf_copy_kernel<<<int(ceil_div(int64_t(n_out) * kCopyLanes, 256)), 256, 0, st>>>(outn, n_out, list, f, aux->perm.p, aux->tris.p, aux->prim_box.p,
@@ -2025,6 +2152,10 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   out->key_mode = -1;
   out->has_spheres = aux->has_spheres;
   out->arity = branching;
+  // a rebuild starts from the state of a fresh tree
+  out->phr_clone = false;
+  out->phr_levels = 0;
+  out->phr_out_count = 0;
   out->phr_alpha = alpha;
   out->phr_delta = delta;
 
@@ -2170,63 +2301,53 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     return layout(ctx, aux, n, branching, f, outn, list, ctl, flag, tidx, sscr, n_trav, ord, gkey, rcost,
                   reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, levels, out);
   }
-  uint32_t n_jobs[kLists] = {0, 0, 0, 1};  // the initial cut's job is on the large list
-  uint32_t max_m = hc.cut;  // the one job of the first level holds the initial cut
-  // Every job holds a cut of at least two members and makes one split attempt
-  // (branching 2), so BuildWithCost's counter (phr.go:221) gains the level's
-  // pool size: the initial cut, then what each level appends for the next.
-  uint64_t level_pool = hc.cut;
-  while (n_jobs[0] + n_jobs[1] + n_jobs[2] + n_jobs[3] > 0) {
-    // LDS for the bitonic sort, sized to the level's largest cut
-    // (cuts of up to 64 sort in registers; larger ones up to kSortLds in LDS,
-    // which every workgroup of the level then reserves)
-    int lds_p = 0;
-    if (max_m > 64u && max_m <= uint32_t(kSortLds)) {
-      lds_p = 128;
-      while (uint32_t(lds_p) < max_m) lds_p <<= 1;
+  // Device-driven levels, kLevelBatch per read of the counters: each level's
+  // kernels take its job counts from the device (advance_level_kernel), so
+  // the host only learns after a batch whether jobs remain; levels past the
+  // last one find no jobs.  Every job holds a cut of at least two members and
+  // makes one split attempt (branching 2), so BuildWithCost's counter
+  // (phr.go:221) gains the level's pool size: the initial cut, then what each
+  // level appends for the next.
+  constexpr int kLevelBatch = 8;
+  const size_t lds_full = size_t(kSortLds) * (sizeof(double) + sizeof(int32_t));
+  const size_t lds_small = size_t(kSmallLds) * (sizeof(double) + sizeof(int32_t));
+  const int n_cu = std::max(1, ctx->n_cu);
+  seed_levels_kernel<<<1, 1, 0, st>>>(ctl);
+  advance_level_kernel<<<1, 1, 0, st>>>(ctl);  // later levels are advanced by the small jobs' launch
+  for (;;) {
+    for (int k = 0; k < kLevelBatch; ++k) {
+      Lists nx;
+      for (int r = 0; r < kLists; ++r) {
+        nx.p[r] = jobs[cur ^ 1][r];
+        nx.cap[r] = list_caps[r];
+      }
+      const Task* lg = jobs[cur][kLarge];
+      // the large jobs: one of the three launches is the level's (sel_b)
+      task_dev_kernel<512><<<2048, 512, lds_full, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
+                                                        rcost, outn, list, ctl, caps, kSortLds);
+      task_dev_kernel<256><<<2048, 256, lds_small, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
+                                                         rcost, outn, list, ctl, caps, kSmallLds);
+      task_dev_kernel<64><<<n_cu * 16, 64, lds_small, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
+                                                            rcost, outn, list, ctl, caps, kSmallLds);
+      group_task_kernel<kWaveCut><<<n_cu * 8, 256, 0, st>>>(f, d_th, jobs[cur][2], pools[cur], pools[cur ^ 1], nx,
+                                                            outn, list, ctl, caps);
+      group_task_kernel<kGroupCut><<<n_cu * 4, 256, 0, st>>>(f, d_th, jobs[cur][1], pools[cur], pools[cur ^ 1], nx,
+                                                             outn, list, ctl, caps);
+      small_task_kernel<<<n_cu * 2, 256, 0, st>>>(f, d_th, jobs[cur][0], pools[cur], pools[cur ^ 1], nx, outn, list,
+                                                  ctl, caps);
+      cur ^= 1;
     }
-    const size_t lds_bytes = size_t(lds_p) * (sizeof(double) + sizeof(int32_t));
-    out->phr_split_work += int64_t(level_pool);
-    reset_level_kernel<<<1, 1, 0, st>>>(ctl);
-    Lists nx;
-    for (int r = 0; r < kLists; ++r) {
-      nx.p[r] = jobs[cur ^ 1][r];
-      nx.cap[r] = list_caps[r];
-    }
-    if (n_jobs[kLarge] > 0) {
-      // workgroup size from the level's largest cut: the top levels hold few
-      // jobs of thousands of nodes each
-      const int nb = int(n_jobs[kLarge]);
-      int bsz = 64;  // thousands of jobs fill the chip one wave each
-      if (max_m > 64u && nb < 2048) bsz = (nb <= 128 && max_m > 256u) ? 512 : 256;
-      if (bsz == 512)
-        task_kernel<512><<<nb, 512, lds_bytes, st>>>(f, d_th, jobs[cur][kLarge], pools[cur], pools[cur ^ 1], nx,
-                                                       ord, gkey, rcost, outn, list, ctl, caps, lds_p);
-      else if (bsz == 256)
-        task_kernel<256><<<nb, 256, lds_bytes, st>>>(f, d_th, jobs[cur][kLarge], pools[cur], pools[cur ^ 1], nx,
-                                                     ord, gkey, rcost, outn, list, ctl, caps, lds_p);
-      else
-        task_kernel<64><<<nb, 64, lds_bytes, st>>>(f, d_th, jobs[cur][kLarge], pools[cur], pools[cur ^ 1], nx,
-                                                   ord, gkey, rcost, outn, list, ctl, caps, lds_p);
-    }
-    if (n_jobs[2] > 0)
-      group_task_kernel<kWaveCut><<<int(ceil_div(int64_t(n_jobs[2]) * kWaveCut, 256)), 256, 0, st>>>(
-          f, d_th, jobs[cur][2], int32_t(n_jobs[2]), pools[cur], pools[cur ^ 1], nx, outn, list, ctl, caps);
-    if (n_jobs[1] > 0)
-      group_task_kernel<kGroupCut><<<int(ceil_div(int64_t(n_jobs[1]) * kGroupCut, 256)), 256, 0, st>>>(
-          f, d_th, jobs[cur][1], int32_t(n_jobs[1]), pools[cur], pools[cur ^ 1], nx, outn, list, ctl, caps);
-    if (n_jobs[0] > 0)
-      small_task_kernel<<<int(ceil_div(n_jobs[0], 256)), 256, 0, st>>>(
-          f, d_th, jobs[cur][0], int32_t(n_jobs[0]), pools[cur], pools[cur ^ 1], nx, outn, list, ctl, caps);
     rc = read_ctl(ctx, ctl, &hc);
     if (rc) return rc;
-    if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x at level %d", hc.err, levels);
-    for (int r = 0; r < kLists; ++r) n_jobs[r] = hc.jobs[r][0];
-    max_m = hc.max_m;
-    level_pool = hc.pool;
-    cur ^= 1;
-    ++levels;
+    if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x by level %u", hc.err, hc.levels);
+    // the last level of the batch has already advanced the counters: the
+    // next level's jobs are the current ones now
+    uint32_t pending = 0;
+    for (int r = 0; r < kLists; ++r) pending += hc.cur_jobs[r];
+    if (pending == 0) break;
   }
+  levels = int(hc.levels);
+  out->phr_split_work = int64_t(hc.work);
   return layout(ctx, aux, n, branching, f, outn, list, ctl, flag, tidx, sscr, n_trav, ord, gkey, rcost,
                 reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, levels, out);
 }

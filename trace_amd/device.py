@@ -230,6 +230,13 @@ class Bvh:
         b._aux = aux
         return b
 
+    def phr_rebuild(self, aux: "Bvh", alpha: float = 0.5, delta: float = 6.0, branching: int = 2,
+                    max_cut: int = 2500) -> None:
+        """Rebuilds this PHR tree from aux in place (trace_phr_rebuild)."""
+        self.ctx.check(self.ctx.lib.trace_phr_rebuild(self.ctx.h, aux.h, alpha, delta, branching, max_cut, self.h))
+        self.n_prims = aux.n_prims
+        self._aux = aux
+
     @classmethod
     def phr_with_cost(cls, ctx: Context, aux: "Bvh", alpha: float = 0.5, delta: float = 6.0, branching: int = 2,
                       max_cut: int = 2500):
