@@ -191,14 +191,16 @@ def bench_batched(torch, ctx, steps):
     return out
 
 
-def bench_animation(torch, ctx, frames=120):
-    """120 frames of a 200k-triangle jittered sphere under seeded sinusoidal
-    vertex displacement (amplitude 0.02), rebuilt every frame: all frames
-    recorded in one HIP graph, against the same calls issued eagerly."""
+def bench_animation(torch, ctx, frames=120, tris=200_000):
+    """120 frames of a jittered sphere (200k triangles; 4k for the
+    launch-bound case, where a frame's ~20 kernels each run for a few
+    microseconds) under seeded sinusoidal vertex displacement (amplitude
+    0.02), rebuilt every frame: all frames recorded in one HIP graph, against
+    the same calls issued eagerly."""
     from trace_amd import scenes
     from trace_amd.device import Bvh, Graph, Mesh
 
-    pos = scenes.displaced_sphere(200_000, seed=21, radius=1.0)
+    pos = scenes.displaced_sphere(tris, seed=21, radius=1.0)
     rest = Mesh(ctx, pos)
     mesh = Mesh(ctx, pos)
     bvh = Bvh.lbvh(ctx, mesh)
@@ -811,6 +813,7 @@ def main():
         out["batched"] = bench_batched(torch, ctx, max(3, args.steps // 4))
         note("animation")
         out["animation"] = bench_animation(torch, ctx)
+        out["animation_small"] = bench_animation(torch, ctx, tris=4_000)
         note("adjoint")
         out["adjoint"] = bench_adjoint(torch, ctx, max(3, args.steps // 4))
         out["adjoint_over_primal"] = out["adjoint"]["adjoint_over_primal"]
