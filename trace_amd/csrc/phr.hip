@@ -94,7 +94,8 @@ struct Ctl {
   TRACE_CTL_FIELD(err);        // 1 capacity, 2 depth table
   TRACE_CTL_FIELD(cut);        // initial cut size
   TRACE_CTL_FIELD(max_m);      // largest cut among the large jobs appended for the next level
-  TRACE_CTL_FIELD(groups);     // n-ary build: groups appended for the next step
+  TRACE_CTL_FIELD(groups);     // n-ary build: groups appended for the next step (large list)
+  TRACE_CTL_FIELD(groups_small);  // n-ary build: groups of the next step whose cut to split has <= kNaryLaneCut members
   TRACE_CTL_FIELD(done);       // device-driven levels: workgroups of the level's last launch that finished
   unsigned long long work;     // n-ary build: sizes of the cuts handed to SweepSAH (phr.go:221)
   unsigned long long work_pad[15];
@@ -120,6 +121,7 @@ struct Ctl {
 // 2U - 1 entries; the members of cuts that wait are carried to the next
 // step's pool with the group.
 constexpr int kMaxBranching = 8;
+constexpr int kNaryLaneCut = 16;  // groups whose cut to split is this small take one lane (nary_lane_kernel)
 
 struct CutRec {
   int32_t off, cnt;        // members in the pool (a PHR-made leaf: cnt 1)
@@ -951,6 +953,23 @@ __device__ CutRec refine_half(const Flat& f, const int32_t* ids, const int32_t* 
   return rec;
 }
 
+// The groups of a step sit in one buffer of caps.groups records: those whose
+// cut to split has more than kNaryLaneCut members from the front (ctl->groups),
+// the others from the back (ctl->groups_small; entry s at caps.groups - 1 - s).
+// A group's cuts are disjoint sets of at least two pool entries, so the two
+// lists together hold at most caps.groups records.
+__device__ inline Group* group_slot(Group* next, const NCaps& caps, Ctl* ctl, uint32_t split_cnt) {
+  if (split_cnt <= uint32_t(kNaryLaneCut)) {
+    const int32_t s = int32_t(atomicAdd(&ctl->groups_small, 1u));
+    if (s >= caps.groups) return nullptr;
+    return next + (caps.groups - 1 - s);
+  }
+  const int32_t s = int32_t(atomicAdd(&ctl->groups, 1u));
+  if (s >= caps.groups) return nullptr;
+  raise_max(&ctl->max_m, split_cnt);
+  return next + s;
+}
+
 // One step of every group: one split (phr.go:102-127 for BranchingFactor bf),
 // then the group moves on or writes its branch.
 template <int B>
@@ -1081,14 +1100,13 @@ __global__ __launch_bounds__(B) void nary_kernel(Flat f, const double* __restric
       mi = j;
     }
   if (g.ncuts < bf && mx > 1) {
-    const int32_t slot = int32_t(atomicAdd(&ctl->groups, 1u));
-    if (slot >= caps.groups) {
+    Group* dst = group_slot(groups_next, caps, ctl, uint32_t(mx));
+    if (!dst) {
       atomicOr(&ctl->err, 1u);
       return;
     }
     g.split = mi;
-    groups_next[slot] = g;
-    raise_max(&ctl->max_m, uint32_t(mx));
+    *dst = g;
     return;
   }
   // the branch (box: the job's cut bounding) and its children in cut order
@@ -1105,8 +1123,8 @@ __global__ __launch_bounds__(B) void nary_kernel(Flat f, const double* __restric
       write_out(out, o + j, 1, 0, -1, -1, cj.leaf_ls, cj.leaf_lc, p, cj.prims, g.level + 1, cj.box);
       raise_max(&ctl->max_level, uint32_t(g.level + 1));
     } else {
-      const int32_t slot = int32_t(atomicAdd(&ctl->groups, 1u));
-      if (slot >= caps.groups) {
+      Group* dst = group_slot(groups_next, caps, ctl, uint32_t(cj.cnt));
+      if (!dst) {
         atomicOr(&ctl->err, 1u);
         return;
       }
@@ -1120,8 +1138,7 @@ __global__ __launch_bounds__(B) void nary_kernel(Flat f, const double* __restric
       for (int q = 0; q < 6; ++q) n.box[q] = cj.box[q];
       n.pad2[0] = n.pad2[1] = 0;
       n.cuts[0] = cj;
-      groups_next[slot] = n;
-      raise_max(&ctl->max_m, uint32_t(cj.cnt));
+      *dst = n;
     }
     p += cj.prims;
   }
@@ -1168,9 +1185,10 @@ __device__ inline double centre(const Flat& f, int32_t node, int axis) {
 
 // Stable sort by box centre on `axis` of the members in order `in` (indices
 // into ids): insertion sort on (centre, input position).
+template <int N = kSmallCut>
 __device__ inline void small_sort(const Flat& f, const int32_t* ids, const int* in, int* o, int m, int axis) {
-  double key[kSmallCut];
-  for (int i = 0; i < kSmallCut; ++i) {
+  double key[N];
+  for (int i = 0; i < N; ++i) {
     if (i >= m) break;
     o[i] = in ? in[i] : i;
     key[i] = centre(f, ids[o[i]], axis);
@@ -1199,9 +1217,10 @@ __device__ inline double sa_lohi(const float* lo, const float* hi) {
 }
 
 // minCost (phr.go:414-459) of order o, as min_cost computes it.
+template <int N = kSmallCut>
 __device__ inline Split small_cost(const Flat& f, const int32_t* ids, const int* o, int m, const float* cut_box) {
 #pragma clang fp contract(off)
-  double rc[kSmallCut];
+  double rc[N];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   uint32_t cnt = 0;
   for (int r = m - 1; r >= 1; --r) {
@@ -1748,6 +1767,270 @@ __device__ void advance_level(Ctl* ctl) {
 
 __global__ void advance_level_kernel(Ctl* ctl) { advance_level(ctl); }
 
+// ---- n-ary groups whose cuts are small: one lane per group ----------------------------
+
+// A group whose cut to split has at most kNaryLaneCut members (it is the group's
+// largest cut, so every cut of the group is that small) takes one lane: the
+// split is the lane job's (same sorts, minCost and refinement, the same FP64
+// operations), then the group's bookkeeping as nary_kernel does it -- the
+// split cut replaced by its first half, the second appended, the waiting cuts'
+// members carried into the next pool, then either the group moves on (fewer
+// than bf cuts and one of two or more members) or its branch is written.
+// Every lane of the wave calls it: the allocations are wave-wide.
+__device__ void nary_lane_job(int32_t gi, int32_t n_groups, Flat f, const double* __restrict__ th,
+                              const Group* __restrict__ groups, Group* __restrict__ groups_next,
+                              const int32_t* __restrict__ pool, int32_t* __restrict__ pool_next,
+                              OutNode* __restrict__ out, int32_t* __restrict__ list, Ctl* ctl, NCaps caps, int bf) {
+  bool valid = gi < n_groups;
+  const Group* gp = groups + (caps.groups - 1 - (valid ? gi : 0));  // the small list runs from the back
+  int32_t ncuts = 0, si = 0;
+  CutRec c{};
+  int m = 0;
+  if (valid) {
+    ncuts = gp->ncuts;
+    si = gp->split;
+    if (si >= 0 && si < ncuts && ncuts <= kMaxBranching) c = gp->cuts[si];
+    m = c.cnt;
+    if (c.depth >= kMaxDepth || si < 0 || si >= ncuts || ncuts > kMaxBranching || m < 2 || m > kNaryLaneCut ||
+        c.leaf_lc > 0) {
+      atomicOr(&ctl->err, 2u);
+      valid = false;
+      m = 0;
+    }
+  }
+  int32_t ids[kNaryLaneCut];
+  int oX[kNaryLaneCut], oY[kNaryLaneCut], oZ[kNaryLaneCut];
+  const int* C = oX;
+  Split sp{0.0, 0};
+  uint32_t k0 = 0, k1 = 0, pl = 0, pr = 0;
+  float L[6], R[6];
+  double thd = 0.0;
+  if (valid) {
+    for (int k = 0; k < kNaryLaneCut; ++k)
+      if (k < m) ids[k] = pool[c.off + k];
+    // SweepSAH (phr.go:368-405)
+    small_sort<kNaryLaneCut>(f, ids, nullptr, oX, m, 0);
+    small_sort<kNaryLaneCut>(f, ids, oX, oY, m, 1);
+    const Split x = small_cost<kNaryLaneCut>(f, ids, oX, m, c.box);
+    const Split y = small_cost<kNaryLaneCut>(f, ids, oY, m, c.box);
+    small_sort<kNaryLaneCut>(f, ids, x.cost < y.cost ? oY : oX, oZ, m, 2);
+    const Split z = small_cost<kNaryLaneCut>(f, ids, oZ, m, c.box);
+    if (x.cost < y.cost) {
+      if (x.cost < z.cost) { C = oX; sp = x; } else { C = oZ; sp = z; }
+    } else {
+      if (y.cost < z.cost) { C = oY; sp = y; } else { C = oZ; sp = z; }
+    }
+    if (sp.at > 0) {
+      for (int a = 0; a < 3; ++a) {
+        L[a] = R[a] = INFINITY;
+        L[3 + a] = R[3 + a] = -INFINITY;
+      }
+      thd = th[c.depth];  // refined() uses the split cut's depth
+      for (int r = 0; r < m; ++r) {
+        const int32_t node = ids[C[r]];
+        const uint32_t pc = uint32_t(f.prim[2 * node + 1] - f.prim[2 * node]);
+        const uint32_t k = keep_node(f, node, thd) ? 1u : 2u;
+        if (r < sp.at) {
+          box_merge(L, L + 3, f, node);
+          pl += pc;
+          k0 += k;
+        } else {
+          box_merge(R, R + 3, f, node);
+          pr += pc;
+          k1 += k;
+        }
+      }
+    }
+  }
+  const bool two = valid && sp.at > 0;
+  // the group's cuts after the split: member counts (a leaf counts 1) and leaf flags
+  const int n_new = valid ? ncuts + (two ? 1 : 0) : 0;
+  uint32_t cnt[kMaxBranching];
+  bool leaf[kMaxBranching];
+  uint32_t carried = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxBranching; ++j) {
+    cnt[j] = 0;
+    leaf[j] = false;
+    if (j >= n_new) continue;
+    if (j == si) {
+      leaf[j] = !two || k0 == 1;
+      cnt[j] = leaf[j] ? 1u : k0;
+    } else if (j == ncuts) {  // the appended second half
+      leaf[j] = k1 == 1;
+      cnt[j] = leaf[j] ? 1u : k1;
+    } else {
+      const CutRec cj = gp->cuts[j];
+      leaf[j] = cj.leaf_lc > 0;
+      cnt[j] = uint32_t(cj.cnt);
+      if (!leaf[j]) carried += cnt[j];
+    }
+  }
+  uint32_t mx = 0;
+  int mi = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxBranching; ++j)
+    if (j < n_new && cnt[j] > mx) {
+      mx = cnt[j];
+      mi = j;
+    }
+  const bool cont = valid && n_new < bf && mx > 1;
+  const bool branch = valid && !cont;
+  // what the lane emits
+  const uint32_t n_list = !valid ? 0u : !two ? uint32_t(m) : (k0 == 1 ? 1u : 0u) + (k1 == 1 ? 1u : 0u);
+  const uint32_t n_pool = !valid ? 0u : (two && k0 > 1 ? k0 : 0u) + (two && k1 > 1 ? k1 : 0u) + carried;
+  const uint32_t n_out = branch ? uint32_t(n_new) : 0u;
+  uint32_t g_big = 0, g_small = 0, max_big = 0, max_lvl = 0;
+  if (cont) {
+    if (mx <= uint32_t(kNaryLaneCut)) g_small = 1;
+    else {
+      g_big = 1;
+      max_big = mx;
+    }
+  }
+  if (branch) {
+#pragma unroll
+    for (int j = 0; j < kMaxBranching; ++j) {
+      if (j >= n_new) continue;
+      if (leaf[j]) {
+        max_lvl = 1;
+      } else if (cnt[j] <= uint32_t(kNaryLaneCut)) {
+        ++g_small;
+      } else {
+        ++g_big;
+        max_big = max(max_big, cnt[j]);
+      }
+    }
+  }
+  const int32_t ls = int32_t(wave_alloc(&ctl->list, n_list));
+  const int32_t po = int32_t(wave_alloc(&ctl->pool, n_pool));
+  const int32_t o = int32_t(wave_alloc(&ctl->out, n_out));
+  const int32_t gb = int32_t(wave_alloc(&ctl->groups, g_big));
+  const int32_t gs = int32_t(wave_alloc(&ctl->groups_small, g_small));
+  // buildSubTreeCost's counter (phr.go:221) and the level bounds
+  uint32_t wsum;
+  wave_excl(uint32_t(m), &wsum);
+  const uint32_t lvl = max_lvl ? uint32_t(gp->level + 1) : 0u;
+  const uint32_t w_big = wave_max(max_big), w_lvl = wave_max(lvl);
+  if ((threadIdx.x & 63) == 0) {
+    if (wsum) atomicAdd(&ctl->work, (unsigned long long)wsum);
+    if (w_big) raise_max(&ctl->max_m, w_big);
+    if (w_lvl) raise_max(&ctl->max_level, w_lvl);
+  }
+  if (!valid) return;
+  if (int64_t(ls) + n_list > caps.list || int64_t(po) + n_pool > caps.pool || int64_t(o) + n_out > caps.out ||
+      int64_t(gb) + g_big > caps.groups || int64_t(gs) + g_small > caps.groups) {
+    atomicOr(&ctl->err, 1u);
+    return;
+  }
+  // the two halves (or the leaf of the whole cut) and their pool entries
+  CutRec first = c, second = c;
+  int32_t lnext = ls, pnext = po;
+  if (!two) {
+    for (int r = 0; r < m; ++r) list[ls + r] = ids[C[r]];
+    first.off = 0;
+    first.cnt = 1;
+    first.leaf_ls = ls;
+    first.leaf_lc = m;
+  } else {
+    for (int h = 0; h < 2; ++h) {
+      CutRec& rec = h ? second : first;
+      const int a = h ? sp.at : 0, b = h ? m : sp.at;
+      const uint32_t k = h ? k1 : k0;
+      rec.depth = c.depth + 1;
+      rec.prims = int32_t(h ? pr : pl);
+      for (int q = 0; q < 6; ++q) rec.box[q] = h ? R[q] : L[q];
+      if (k == 1) {
+        list[lnext] = ids[C[a]];
+        rec.off = 0;
+        rec.cnt = 1;
+        rec.leaf_ls = lnext;
+        rec.leaf_lc = 1;
+        ++lnext;
+        continue;
+      }
+      rec.off = pnext;
+      rec.cnt = int32_t(k);
+      rec.leaf_ls = 0;
+      rec.leaf_lc = 0;
+      for (int r = a; r < b; ++r) {
+        const int32_t node = ids[C[r]];
+        if (keep_node(f, node, thd)) {
+          pool_next[pnext++] = node;
+        } else {
+          pool_next[pnext++] = f.child[2 * node];
+          pool_next[pnext++] = f.child[2 * node + 1];
+        }
+      }
+    }
+  }
+  const int32_t g_out = gp->out, g_level = gp->level, g_prim = gp->prim_off;
+  float g_box[6];
+  for (int q = 0; q < 6; ++q) g_box[q] = gp->box[q];
+  int32_t big_next = gb, small_next = gs;
+  Group* dst = nullptr;
+  if (cont) {
+    dst = mx <= uint32_t(kNaryLaneCut) ? groups_next + (caps.groups - 1 - small_next++) : groups_next + big_next++;
+    dst->out = g_out;
+    dst->level = g_level;
+    dst->prim_off = g_prim;
+    dst->ncuts = n_new;
+    dst->split = mi;
+    for (int q = 0; q < 3; ++q) dst->pad[q] = 0;
+    for (int q = 0; q < 6; ++q) dst->box[q] = g_box[q];
+    dst->pad2[0] = dst->pad2[1] = 0;
+  }
+  int32_t p = g_prim;
+  for (int j = 0; j < n_new; ++j) {
+    CutRec cj;
+    if (j == si) {
+      cj = first;
+    } else if (j == ncuts) {
+      cj = second;
+    } else {
+      cj = gp->cuts[j];
+      if (cj.leaf_lc == 0) {  // a waiting cut: its members move to the next pool
+        for (int r = 0; r < cj.cnt; ++r) pool_next[pnext + r] = pool[cj.off + r];
+        cj.off = pnext;
+        pnext += cj.cnt;
+      }
+    }
+    if (cont) {
+      dst->cuts[j] = cj;
+      continue;
+    }
+    // the branch's child j
+    if (cj.leaf_lc > 0) {
+      write_out(out, o + j, 1, 0, -1, -1, cj.leaf_ls, cj.leaf_lc, p, cj.prims, g_level + 1, cj.box);
+    } else {
+      Group* ng = uint32_t(cj.cnt) <= uint32_t(kNaryLaneCut) ? groups_next + (caps.groups - 1 - small_next++)
+                                                          : groups_next + big_next++;
+      ng->out = o + j;
+      ng->level = g_level + 1;
+      ng->prim_off = p;
+      ng->ncuts = 1;
+      ng->split = 0;
+      for (int q = 0; q < 3; ++q) ng->pad[q] = 0;
+      for (int q = 0; q < 6; ++q) ng->box[q] = cj.box[q];
+      ng->pad2[0] = ng->pad2[1] = 0;
+      ng->cuts[0] = cj;
+    }
+    p += cj.prims;
+  }
+  if (branch)
+    write_out(out, g_out, 0, n_new, o, n_new > 1 ? o + 1 : -1, -1, 0, g_prim, p - g_prim, g_level, g_box);
+}
+
+__global__ __launch_bounds__(256) void nary_lane_kernel(Flat f, const double* __restrict__ th,
+                                                        const Group* __restrict__ groups, int32_t n_groups,
+                                                        Group* __restrict__ groups_next,
+                                                        const int32_t* __restrict__ pool,
+                                                        int32_t* __restrict__ pool_next, OutNode* __restrict__ out,
+                                                        int32_t* __restrict__ list, Ctl* ctl, NCaps caps, int bf) {
+  nary_lane_job(int32_t(blockIdx.x * 256 + threadIdx.x), n_groups, f, th, groups, groups_next, pool, pool_next, out,
+                list, ctl, caps, bf);
+}
+
 // Small and group jobs of the current level, grid-stride over its count.
 // The small jobs run last in a level: the workgroup that finishes last
 // (arrival counter ctl->done) advances the counters for the next level.
@@ -1817,6 +2100,7 @@ __global__ void reset_level_kernel(Ctl* ctl) {
   ctl->pool = 0;
   ctl->max_m = 0;
   ctl->groups = 0;
+  ctl->groups_small = 0;
 }
 
 // ---- layout of the traversal tree ------------------------------------------------
@@ -2268,8 +2552,8 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     // one step splits one cut of every group (see nary_kernel)
     const NCaps ncaps{pool_cap, out_cap, list_cap, group_cap};
     nary_root_kernel<<<1, 1, 0, st>>>(jobs[0][kLarge], int32_t(n), gbuf[0]);
-    uint32_t n_groups = 1, max_m = hc.cut;
-    while (n_groups > 0) {
+    uint32_t n_groups = 1, n_small = 0, max_m = hc.cut;  // the root's group is on the large list
+    while (n_groups + n_small > 0) {
       int lds_p = 0;
       if (max_m > 64u && max_m <= uint32_t(kSortLds)) {
         lds_p = 128;
@@ -2278,21 +2562,31 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
       const size_t lds_bytes = size_t(lds_p) * (sizeof(double) + sizeof(int32_t));
       reset_level_kernel<<<1, 1, 0, st>>>(ctl);
       const int nb = int(n_groups);
-      int bsz = 64;
-      if (max_m > 64u && nb < 2048) bsz = (nb <= 128 && max_m > 256u) ? 512 : 256;
-      if (bsz == 512)
-        nary_kernel<512><<<nb, 512, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
+      if (nb > 0) {
+        int bsz = 64;
+        if (max_m > 64u && nb < 2048) bsz = (nb <= 128 && max_m > 256u) ? 512 : 256;
+        if (bsz == 512)
+          nary_kernel<512><<<nb, 512, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
+                                                         ord, gkey, rcost, outn, list, ctl, ncaps, lds_p, branching);
+        else if (bsz == 256)
+          nary_kernel<256><<<nb, 256, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
+                                                         ord, gkey, rcost, outn, list, ctl, ncaps, lds_p, branching);
+        else
+          nary_kernel<64><<<nb, 64, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
                                                        ord, gkey, rcost, outn, list, ctl, ncaps, lds_p, branching);
-      else if (bsz == 256)
-        nary_kernel<256><<<nb, 256, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
-                                                       ord, gkey, rcost, outn, list, ctl, ncaps, lds_p, branching);
-      else
-        nary_kernel<64><<<nb, 64, lds_bytes, st>>>(f, d_th, gbuf[cur], gbuf[cur ^ 1], pools[cur], pools[cur ^ 1],
-                                                     ord, gkey, rcost, outn, list, ctl, ncaps, lds_p, branching);
+      }
+      if (n_small > 0)
+        nary_lane_kernel<<<int(ceil_div(n_small, 256)), 256, 0, st>>>(f, d_th, gbuf[cur], int32_t(n_small),
+                                                                      gbuf[cur ^ 1], pools[cur], pools[cur ^ 1], outn,
+                                                                      list, ctl, ncaps, branching);
       rc = read_ctl(ctx, ctl, &hc);
       if (rc) return rc;
       if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x at step %d", hc.err, levels);
+      if (int64_t(hc.groups) + hc.groups_small > group_cap)
+        TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: %u + %u groups exceed the %d the step buffer holds",
+                   hc.groups, hc.groups_small, group_cap);
       n_groups = hc.groups;
+      n_small = hc.groups_small;
       max_m = hc.max_m;
       cur ^= 1;
       ++levels;
