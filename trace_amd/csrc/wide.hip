@@ -54,9 +54,10 @@ __global__ void parent_kernel(const BvhNode* __restrict__ nodes, const BvhHeader
 // some ancestor of inner node i (-1 once past the root) and how many parent
 // links lead there.  Every value ever stored is such a pair, so a thread may
 // read another node's word at any moment (one 64-bit load: never torn) and
-// jump by it; each jump moves at least one level up and the jumps double as
-// other threads publish theirs, so a node reaches the root in about
-// log2(depth) dependent loads instead of depth of them.
+// jump by it; each jump moves at least one level up.  A thread publishes its
+// node's word once, when it has reached the root (the stores write through
+// to memory): a node whose ancestor has finished then ends in one more load,
+// and the upper levels, whose chains are short, finish first.
 __device__ inline uint64_t jump_word(int32_t anc, uint32_t steps) {
   return (uint64_t(uint32_t(anc)) << 32) | steps;
 }
@@ -93,8 +94,8 @@ __global__ void flag_kernel(uint64_t* jump, const BvhHeader* __restrict__ hdr, i
     const uint64_t v = __hip_atomic_load(jump + anc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     anc = int32_t(uint32_t(v >> 32));
     steps += uint32_t(v);
-    __hip_atomic_store(jump + i, jump_word(anc, steps), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  __hip_atomic_store(jump + i, jump_word(-1, steps), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   flag[i] = (steps & 1u) == 0u ? 1u : 0u;
 }
 
@@ -239,12 +240,26 @@ __device__ __attribute__((always_inline)) inline void write_node(const Slot (&s)
   *dst = w;
 }
 
+// inv[widx[i]] = i for the binary nodes that become wide nodes.
+__global__ void compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ widx, int64_t cap,
+                               const BvhHeader* __restrict__ hdr, uint32_t* __restrict__ inv) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < cap && i < hdr->n_inner && flag[i]) inv[widx[i]] = uint32_t(i);
+}
+
+// One thread per wide node derived from a binary node (w < *n_derived).
 __global__ void widen_kernel(const BvhNode* __restrict__ nodes, const BvhHeader* __restrict__ hdr,
-                             const uint32_t* __restrict__ flag, const uint32_t* __restrict__ widx,
-                             const SplitMap sm, WideNode* __restrict__ out, uint32_t* d_err) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                             const uint32_t* __restrict__ inv, const uint32_t* __restrict__ n_derived,
+                             const uint32_t* __restrict__ widx, const SplitMap sm, WideNode* __restrict__ out,
+                             uint32_t* d_err) {
+  const int64_t w = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (w >= *n_derived) return;
+  const int32_t i = int32_t(inv[w]);
   const int32_t n_inner = hdr->n_inner;
-  if (i >= n_inner || !flag[i]) return;
+  if (i < 0 || i >= n_inner) {
+    atomicOr(d_err, TRACE_ERR_TREE);
+    return;
+  }
   bool ok = true;
   Slot s[4];
 #pragma unroll
@@ -266,7 +281,7 @@ __global__ void widen_kernel(const BvhNode* __restrict__ nodes, const BvhHeader*
     atomicOr(d_err, TRACE_ERR_TREE);
     return;
   }
-  write_node(s, out + widx[i], d_err);
+  write_node(s, out + w, d_err);
 }
 
 // need[e]: sub-tree nodes of entry e (see SplitMap).
@@ -419,16 +434,17 @@ __global__ void wroot_kernel(BvhHeader* __restrict__ hdrs, int32_t count,
 
 namespace {
 
-// Scratch layout of widen(): parent links, depth jumps, wide flags and indices of the
+// Scratch layout of widen(): parent links, depth jumps, the wide-to-binary map, wide flags and indices of the
 // binary inner nodes, two totals, then per entry (inner nodes and roots) the
 // sub-tree sizes of split leaves, their offsets and the scan's partials.
 struct WidePlan {
-  size_t o_par, o_jump, o_flag, o_idx, o_tot, o_need, o_soff, o_scan, total;
+  size_t o_par, o_jump, o_inv, o_flag, o_idx, o_tot, o_need, o_soff, o_scan, total;
   WidePlan(int64_t cap, int32_t n_roots) {
     const int64_t n_entries = cap + n_roots;
     ScratchPlan plan;
     o_par = plan.take(cap * sizeof(int32_t));
     o_jump = plan.take(cap * sizeof(uint64_t));
+    o_inv = plan.take(cap * sizeof(uint32_t));
     o_flag = plan.take(cap * sizeof(uint32_t));
     o_idx = plan.take(cap * sizeof(uint32_t));
     o_tot = plan.take(2 * sizeof(uint32_t));
@@ -501,7 +517,9 @@ int widen(trace_ctx* ctx, trace_bvh* bvh, const int32_t* links, const Workspace*
   }
   scan::exclusive_u32(st, need, soff, n_entries, total + 1, sscr);
   const SplitMap sm{soff, total, max_levels};
-  widen_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, flag, widx, sm, bvh->wnodes.p, ctx->d_err);
+  auto* inv = reinterpret_cast<uint32_t*>(base + plan.o_inv);
+  compact_kernel<<<grid, 256, 0, st>>>(flag, widx, cap, bvh->header.p, inv);
+  widen_kernel<<<grid, 256, 0, st>>>(bvh->nodes.p, bvh->header.p, inv, total, widx, sm, bvh->wnodes.p, ctx->d_err);
   const int sgrid = int(std::min<int64_t>(ceil_div(split_cap, 256), int64_t(ctx->n_cu) * 16));
   split_kernel<<<std::max(1, sgrid), 256, 0, st>>>(bvh->prim_box.p, bvh->n_prims, bvh->nodes.p, cap, roots,
                                                    n_roots, sm, total + 1, bvh->wnodes.p, cap + split_cap,
