@@ -33,28 +33,48 @@ __device__ inline double double_of(uint64_t o) {
   return __longlong_as_double(b);
 }
 
-constexpr int kEncBlocks = 512;
+constexpr int kEncBlocks = 2048;
 
 // Stage 1 of the scene box: each workgroup reduces a grid-stride share of the
 // triangles (min/max are exact, so any order gives the reference's box) and
 // writes one partial box as order-preserving u64 words; no atomics.
+// A tile of 256 triangles' positions (256 x 72 bytes) staged through LDS with
+// 16-byte loads: each wave reads whole lines instead of nine 8-byte loads per
+// lane at a 72-byte stride.  Returns the tile's triangle count.
+constexpr int kPosTile = 256;
+__device__ inline int stage_positions(const double* __restrict__ pos, int64_t n, int64_t base, double* s_pos) {
+  const int64_t left = n - base;
+  const int cnt = left < kPosTile ? int(left) : kPosTile;
+  const int words = 9 * cnt;  // doubles; base is even, so the tile starts on a 16-byte boundary
+  const double2* src = reinterpret_cast<const double2*>(pos + 9 * base);
+  double2* dst = reinterpret_cast<double2*>(s_pos);
+  for (int k = threadIdx.x; k < words / 2; k += blockDim.x) dst[k] = src[k];
+  if ((words & 1) && threadIdx.x == 0) s_pos[words - 1] = pos[9 * base + words - 1];
+  __syncthreads();
+  return cnt;
+}
+
 __global__ __launch_bounds__(256) void enclosing_kernel(const double* __restrict__ pos, int64_t n,
                                                         unsigned long long* __restrict__ partial) {
   __shared__ uint64_t s_part[6][4];
+  __shared__ double s_pos[9 * kPosTile];
   uint64_t lo[3] = {~0ull, ~0ull, ~0ull};
   uint64_t hi[3] = {0ull, 0ull, 0ull};
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
-    const double* p = pos + 9 * i;
+  for (int64_t base = blockIdx.x * int64_t(kPosTile); base < n; base += int64_t(gridDim.x) * kPosTile) {
+    const int cnt = stage_positions(pos, n, base, s_pos);
+    if (int(threadIdx.x) < cnt) {
+      const double* p = s_pos + 9 * threadIdx.x;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double l, h;
-      prim_bounds(p, a, l, h);
-      const uint64_t mn = ord_of(l);
-      const uint64_t mx = ord_of(h);
-      lo[a] = mn < lo[a] ? mn : lo[a];
-      hi[a] = mx > hi[a] ? mx : hi[a];
+      for (int a = 0; a < 3; ++a) {
+        double l, h;
+        prim_bounds(p, a, l, h);
+        const uint64_t mn = ord_of(l);
+        const uint64_t mx = ord_of(h);
+        lo[a] = mn < lo[a] ? mn : lo[a];
+        hi[a] = mx > hi[a] ? mx : hi[a];
+      }
     }
+    __syncthreads();  // the next tile reuses s_pos
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -243,15 +263,18 @@ __device__ inline uint64_t quantise(double enc_lo, double extent, double c) {
 }
 
 // tseg (segmented meshes): each triangle quantises in its own segment's box.
-__global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ pos, int64_t n,
-                                                     const double* __restrict__ qbox_all,
-                                                     const int32_t* __restrict__ tseg, int key_mode,
-                                                     uint64_t* __restrict__ keys) {
+__global__ __launch_bounds__(kPosTile) void morton_kernel(const double* __restrict__ pos, int64_t n,
+                                                          const double* __restrict__ qbox_all,
+                                                          const int32_t* __restrict__ tseg, int key_mode,
+                                                          uint64_t* __restrict__ keys) {
 #pragma clang fp contract(off)
-  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  __shared__ double s_pos[9 * kPosTile];
+  const int64_t base = blockIdx.x * int64_t(kPosTile);
+  stage_positions(pos, n, base, s_pos);
+  const int64_t i = base + threadIdx.x;
   if (i >= n) return;
   const double* qbox = tseg ? qbox_all + 6 * size_t(tseg[i]) : qbox_all;
-  const double* p = pos + 9 * i;
+  const double* p = s_pos + 9 * threadIdx.x;
   uint64_t q[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -496,29 +519,45 @@ __global__ void seg_begin_kernel(const int32_t* __restrict__ useg, const BvhHead
 // FP32 triangles (v0, v1, v2) and outward-rounded FP32 boxes, in sorted
 // order.  Vertices rather than edges are kept so that the winning hit of a
 // ray can be re-evaluated in FP64 from the same vertices the reference uses.
-__global__ __launch_bounds__(256) void reorder_kernel(const double* __restrict__ pos, int64_t n,
-                                                      const uint32_t* __restrict__ perm,
-                                                      float* __restrict__ tris,
-                                                      float* __restrict__ boxes) {
-  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t src = perm[i] < uint64_t(n) ? perm[i] : 0u;  // perm is a bijection on [0, n)
-  const double* p = pos + 9 * int64_t(src);
-  double v[9];
+// Gathers each sorted position's triangle; the FP32 triangles and boxes of a
+// tile of kPosTile are staged in LDS and written with 16-byte stores.
+__device__ inline void store_tile(const float* s, float* __restrict__ dst, int words) {
+  float4* d4 = reinterpret_cast<float4*>(dst);  // tiles start on 16-byte boundaries
+  const float4* s4 = reinterpret_cast<const float4*>(s);
+  for (int k = threadIdx.x; k < words / 4; k += blockDim.x) d4[k] = s4[k];
+  for (int k = (words & ~3) + int(threadIdx.x); k < words; k += blockDim.x) dst[k] = s[k];
+}
+
+__global__ __launch_bounds__(kPosTile) void reorder_kernel(const double* __restrict__ pos, int64_t n,
+                                                           const uint32_t* __restrict__ perm,
+                                                           float* __restrict__ tris, float* __restrict__ boxes) {
+  __shared__ __align__(16) float s_tri[9 * kPosTile];
+  __shared__ __align__(16) float s_box[6 * kPosTile];
+  const int64_t base = blockIdx.x * int64_t(kPosTile);
+  const int64_t i = base + threadIdx.x;
+  if (i < n) {
+    const uint32_t src = perm[i] < uint64_t(n) ? perm[i] : 0u;  // perm is a bijection on [0, n)
+    const double* p = pos + 9 * int64_t(src);
+    double v[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) v[k] = p[k];
-  float* t = tris + 9 * i;
-  float* b = boxes + 6 * i;
+    for (int k = 0; k < 9; ++k) v[k] = p[k];
+    float* t = s_tri + 9 * threadIdx.x;
+    float* b = s_box + 6 * threadIdx.x;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    t[a] = float(v[a]);
-    t[3 + a] = float(v[3 + a]);
-    t[6 + a] = float(v[6 + a]);
-    double lo, hi;
-    prim_bounds(v, a, lo, hi);
-    b[a] = f32_down(lo);
-    b[3 + a] = f32_up(hi);
+    for (int a = 0; a < 3; ++a) {
+      t[a] = float(v[a]);
+      t[3 + a] = float(v[3 + a]);
+      t[6 + a] = float(v[6 + a]);
+      double lo, hi;
+      prim_bounds(v, a, lo, hi);
+      b[a] = f32_down(lo);
+      b[3 + a] = f32_up(hi);
+    }
   }
+  __syncthreads();
+  const int cnt = n - base < kPosTile ? int(n - base) : kPosTile;
+  store_tile(s_tri, tris + 9 * base, 9 * cnt);
+  store_tile(s_box, boxes + 6 * base, 6 * cnt);
 }
 
 __device__ inline void st_f32(float* p, float v) {
@@ -818,8 +857,7 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
     seg_header_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(enc, nseg, bvh->seg_headers.p, qbox,
                                                           bvh->header.p, int32_t(n));
   }
-  morton_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, qbox, tseg, key_mode,
-                                                  bvh->keys.p);
+  morton_kernel<<<grid_for(n, kPosTile), kPosTile, 0, st>>>(mesh->pos.p, n, qbox, tseg, key_mode, bvh->keys.p);
 
   // the values ping-pong between the permutation itself and scratch, so an
   // even number of passes leaves the sorted order in place
@@ -847,8 +885,7 @@ int build(trace_ctx* ctx, const trace_mesh* mesh, const double* h_enclosing, int
   karras_kernel<<<grid_for(n, 256), 256, 0, st>>>(ukeys, bvh->run_start.p, bvh->header.p, useg,
                                                   u_begin, bvh->nodes.p, parent_inner,
                                                   parent_leaf, bvh->inner_range.p, counters);
-  reorder_kernel<<<grid_for(n, 256), 256, 0, st>>>(mesh->pos.p, n, sperm, bvh->tris.p,
-                                                   bvh->prim_box.p);
+  reorder_kernel<<<grid_for(n, kPosTile), kPosTile, 0, st>>>(mesh->pos.p, n, sperm, bvh->tris.p, bvh->prim_box.p);
   refit_kernel<<<grid_for(n, kRefitThreads), kRefitThreads, 0, st>>>(
       bvh->prim_box.p, bvh->run_start.p, parent_inner, parent_leaf, bvh->inner_range.p, bvh->nodes.p,
       counters, heights, bvh->header.p, ctx->d_err, useg, forest ? bvh->seg_headers.p : nullptr, nseg);
