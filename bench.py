@@ -560,6 +560,27 @@ def pmc_probe(torch, ctx, phr, d_rays, n, d_hit, d_t):
     ctx.synchronize()
 
 
+def pmc_probe_build(torch, ctx):
+    """Workload for the build's counter runs: the same copy of COPY_BYTES, then
+    the 10.38M-triangle terrain LBVH (63-bit keys, default key sort) built once
+    and rebuilt twice; the last rebuild is the one a summary reads."""
+    from trace_amd import scenes
+    from trace_amd.device import Bvh, Mesh
+
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    del src, dst
+    mesh = Mesh(ctx, scenes.terrain_scene()["pos"])
+    bvh = Bvh.lbvh(ctx, mesh)
+    for _ in range(2):
+        bvh.rebuild(mesh)
+    ctx.synchronize()
+    bvh.close()
+    mesh.close()
+
+
 def _counter_rows(path):
     import csv
 
@@ -628,6 +649,8 @@ def main():
                     help="inverse-rendering steps of the 10.4M-triangle leg (the configuration's 100)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="run only the counter workload (copy kernel + closest hit) and exit")
+    ap.add_argument("--pmc-probe-build", action="store_true",
+                    help="run only the build's counter workload (copy kernel + 10.4M-triangle LBVH) and exit")
     ap.add_argument("--traffic-from", nargs="+", metavar="CSV",
                     help="FETCH_CSV WRITE_CSV [HITMISS_CSV]: turn rocprofv3 counter CSVs into the traffic "
                          "file and exit")
@@ -644,8 +667,6 @@ def main():
     from trace_amd import KEY_M30, KEY_REF63, scenes
     from trace_amd.device import Bvh, Context, Mesh, autotune_traversal, get_tune
 
-    pos = scenes.random_triangles(args.tris, seed=1)
-    rays = scenes.incoherent_rays(args.rays, seed=2)
     ctx = Context(0)
     # A dedicated (default-priority) stream: torch's default stream has handle
     # 0, which the C-ABI reads as "use the context's own stream".  Events are
@@ -653,6 +674,12 @@ def main():
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     ctx.set_stream(stream.cuda_stream)
+    if args.pmc_probe_build:
+        pmc_probe_build(torch, ctx)
+        ctx.close()
+        return 0
+    pos = scenes.random_triangles(args.tris, seed=1)
+    rays = scenes.incoherent_rays(args.rays, seed=2)
     mesh = Mesh(ctx, pos)
     lbvh = Bvh.lbvh(ctx, mesh, key_mode=KEY_REF63)
     t0 = time.perf_counter()
