@@ -191,12 +191,14 @@ def bench_batched(torch, ctx, steps):
     return out
 
 
-def bench_animation(torch, ctx, frames=120, tris=200_000):
+def bench_animation(torch, ctx, frames=120, tris=200_000, phr=False):
     """120 frames of a jittered sphere (200k triangles; 4k for the
     launch-bound case, where a frame's ~20 kernels each run for a few
     microseconds) under seeded sinusoidal vertex displacement (amplitude
     0.02), rebuilt every frame: all frames recorded in one HIP graph, against
-    the same calls issued eagerly."""
+    the same calls issued eagerly.  phr: every frame also refines its LBVH
+    into one PHR tree's storage (alpha 0.5, delta 6, branching 2), whose
+    eager form reads the level counters back to the host."""
     from trace_amd import scenes
     from trace_amd.device import Bvh, Graph, Mesh
 
@@ -204,12 +206,18 @@ def bench_animation(torch, ctx, frames=120, tris=200_000):
     rest = Mesh(ctx, pos)
     mesh = Mesh(ctx, pos)
     bvh = Bvh.lbvh(ctx, mesh)
+    tree = None
+    if phr:
+        tree = Bvh.phr(ctx, bvh)
+        tree.phr_rebuild(bvh)  # sizes the tree for the captured rebuilds
     frame = torch.zeros(1, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
 
     def one():
         mesh.animate(rest, 0.02, 60.0, 5, frame.data_ptr())
         bvh.rebuild(mesh)
+        if tree is not None:
+            tree.phr_rebuild(bvh)
 
     def eager():
         for _ in range(frames):
@@ -225,6 +233,9 @@ def bench_animation(torch, ctx, frames=120, tris=200_000):
     out = {"tris": len(pos), "frames": frames, "graph_ms_per_frame": round(g_ms / frames, 4),
            "eager_ms_per_frame": round(e_ms / frames, 4), "graph_speedup": round(e_ms / g_ms, 2),
            "frames_done": int(frame.cpu()[0])}
+    if tree is not None:
+        out["phr_levels"] = tree.info().phr_levels
+        tree.close()
     bvh.close()
     mesh.close()
     rest.close()
@@ -841,6 +852,8 @@ def main():
         note("animation")
         out["animation"] = bench_animation(torch, ctx)
         out["animation_small"] = bench_animation(torch, ctx, tris=4_000)
+        out["animation_phr"] = bench_animation(torch, ctx, frames=30, phr=True)
+        out["animation_phr_small"] = bench_animation(torch, ctx, frames=30, tris=4_000, phr=True)
         note("adjoint")
         out["adjoint"] = bench_adjoint(torch, ctx, max(3, args.steps // 4))
         out["adjoint_over_primal"] = out["adjoint"]["adjoint_over_primal"]

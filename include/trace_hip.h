@@ -171,7 +171,17 @@ int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double d
  * tree a fresh build gives, without allocating when the storage is large enough --
  * the reference builds a new tree per BuildFromAuxilary call (phr.go:47-84), which a
  * caller that rebuilds per frame or per optimiser candidate replaces with this.  On an
- * error the tree's contents are undefined until a rebuild succeeds. */
+ * error the tree's contents are undefined until a rebuild succeeds.
+ * A binary rebuild (branching 2) sizes the tree for any input of the same triangle
+ * count, and may then be recorded in a graph (trace_graph_begin): a rebuild issued
+ * while capturing takes the same alpha, delta and branching 2 as the eager one before
+ * it (TRACE_E_INVALID otherwise) and records, with no host round trip, the eager
+ * rebuild's job levels plus 4.  A replay whose input needs more levels, makes a
+ * deeper tree than the eager one's depth class (63 or 255), or has an initial cut of
+ * one node sets the device error word (reported by the next trace_ctx_synchronize)
+ * and leaves the previous tree in place.  Build statistics (initial cut, build cost)
+ * stay the eager rebuild's; depth and levels (trace_bvh_get_info) and the export
+ * follow the replays. */
 int trace_phr_rebuild(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int branching,
                       int32_t max_cut, trace_bvh* phr);
 /* PhrBuilder.BuildWithCost (phr.go:154-255): trace_phr_build plus its build cost -- the
@@ -434,8 +444,9 @@ int trace_mesh_animate(trace_ctx* ctx, trace_mesh* mesh, const trace_mesh* rest,
 int trace_mesh_export(trace_ctx* ctx, const trace_mesh* mesh, double* h_pos);
 
 /* HIP graph capture of the context stream: asynchronous calls issued between begin
- * and end (trace_mesh_animate, trace_lbvh_rebuild, trace_closest_hit_device,
- * trace_render_device) are recorded once and replayed by trace_graph_launch.  Device
+ * and end (trace_mesh_animate, trace_lbvh_rebuild, trace_phr_rebuild with branching 2,
+ * trace_closest_hit_device, trace_render_device) are recorded once and replayed by
+ * trace_graph_launch.  Device
  * buffers must already be sized (run the work once eagerly first); no call that
  * synchronises may be issued while capturing.  While a graph is alive the context's
  * scratch arena cannot grow (a call that needs more fails with TRACE_E_INVALID);

@@ -238,11 +238,19 @@ def test_animation_graph_matches_eager(ctx, oracle):
         # animated vertices are not FP32-exact: the FP32 boxes are the FP64 boxes
         # rounded outwards
         assert np.array_equal(box, _outward_f32(obox))
-        # the scratch arena cannot grow while the graph is alive
-        big = torch.empty((256, 256, 3), dtype=torch.float32, device="cuda")
-        with pytest.raises(_lib.TraceError):
-            render_device(ctx, g["mesh"], g["bvh"], cam, 256, 256, 64, 3, _lib.MISS_SKY, 5, big.data_ptr(),
-                          big.data_ptr())
+        # the scratch arena cannot grow while the graph is alive: a render whose
+        # 2^29 paths of state no earlier call of the session has needed
+        old = get_tune(ctx)
+        set_tune(ctx, old.trav_threads, old.render_threads, 1 << 29, bool(old.trav_persistent),
+                 bool(old.render_persistent), old.sort_bits, old.sort_keys_per_thread)
+        big = torch.empty((2048, 2048, 3), dtype=torch.float32, device="cuda")
+        try:
+            with pytest.raises(_lib.TraceError, match="scratch arena"):
+                render_device(ctx, g["mesh"], g["bvh"], cam, 2048, 2048, 128, 3, _lib.MISS_SKY, 5, big.data_ptr(),
+                              big.data_ptr())
+        finally:
+            set_tune(ctx, old.trav_threads, old.render_threads, old.render_batch, bool(old.trav_persistent),
+                     bool(old.render_persistent), old.sort_bits, old.sort_keys_per_thread)
     finally:
         graph.close()
 

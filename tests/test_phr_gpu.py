@@ -273,6 +273,130 @@ def test_phr_rebuild_refusals(ctx):
     _same_tree(tree, fresh)
 
 
+def _outward_f32(box64):
+    """FP64 boxes rounded outwards to FP32 (the device's boxes of animated,
+    not FP32-exact, vertices)."""
+    lo, hi = box64[:, :3], box64[:, 3:]
+    flo, fhi = lo.astype(np.float32), hi.astype(np.float32)
+    flo = np.where(flo.astype(np.float64) > lo, np.nextafter(flo, np.float32(-np.inf)), flo)
+    fhi = np.where(fhi.astype(np.float64) < hi, np.nextafter(fhi, np.float32(np.inf)), fhi)
+    return np.concatenate([flo, fhi], axis=1)
+
+
+def test_phr_rebuild_in_a_graph(ctx, oracle):
+    """A frame recorded once in a HIP graph -- animate the mesh, rebuild its LBVH,
+    refine it into the PHR tree's storage, trace rays -- replays to the trees and
+    hits the same calls give eagerly, frame by frame; the last replayed frame's
+    PHR tree is the oracle's for the exported positions, and the header's levels
+    and output nodes follow the replays."""
+    import torch
+
+    from trace_amd.device import Graph
+
+    print("phr rebuild in a graph", flush=True)
+    ntri, frames, amp, period, aseed = 20_000, 4, 0.02, 30.0, 9
+    pos = scenes.displaced_sphere(ntri, seed=71, radius=0.5)
+    rays = torch.from_numpy(scenes.incoherent_rays(30_000, seed=74)).cuda()
+
+    def setup():
+        rest, mesh = Mesh(ctx, pos), Mesh(ctx, pos)
+        aux = Bvh.lbvh(ctx, mesh)
+        tree = Bvh.phr(ctx, aux)
+        tree.phr_rebuild(aux)  # the eager rebuild a captured one repeats
+        st = {"rest": rest, "mesh": mesh, "aux": aux, "tree": tree,
+              "frame": torch.zeros(1, dtype=torch.int32, device="cuda"),
+              "hit": torch.empty(len(rays), dtype=torch.int32, device="cuda"),
+              "t": torch.empty(len(rays), dtype=torch.float32, device="cuda")}
+        torch.cuda.synchronize()
+        return st
+
+    def step(st):
+        st["mesh"].animate(st["rest"], amp, period, aseed, st["frame"].data_ptr())
+        st["aux"].rebuild(st["mesh"])
+        st["tree"].phr_rebuild(st["aux"])
+        st["tree"].closest_hit_device(rays.data_ptr(), len(rays), st["hit"].data_ptr(), st["t"].data_ptr())
+
+    def snap(st):
+        ctx.synchronize()
+        rec, box = st["tree"].canonical()
+        return [st["hit"].cpu().numpy().copy(), st["t"].cpu().numpy().copy(), rec, box, st["tree"].perm(),
+                st["tree"].phr_source(), st["tree"].cost(), st["tree"].info().phr_levels]
+
+    eager = setup()
+    want = []
+    for _ in range(frames):
+        step(eager)
+        want.append(snap(eager))
+
+    g = setup()
+    # other settings than the eager rebuild's, and branching 4, are refused while capturing
+    for kw in ({"alpha": 0.55}, {"branching": 4}):
+        with pytest.raises(_lib.TraceError):
+            Graph.capture(ctx, lambda: g["tree"].phr_rebuild(g["aux"], **kw))
+    graph = Graph.capture(ctx, lambda: step(g))
+    try:
+        for f in range(frames):
+            graph.launch()
+            got = snap(g)
+            for k, (a, b) in enumerate(zip(got, want[f])):
+                assert np.array_equal(np.asarray(a), np.asarray(b)), f"frame {f}: item {k} differs"
+        assert int(g["frame"].cpu()[0]) == frames
+        # the replayed frame's tree against the oracle's PHR of the exported positions
+        opos = g["mesh"].export()
+        rec, box = g["tree"].canonical()
+        orec, obox, oprims = oracle.PhrTree(oracle.Tree(opos), 0.5, 6.0, 2, 2500).canonical()
+        assert rec.shape[0] == orec.shape[0]
+        assert np.array_equal(rec[:, 0], orec[:, 0]) and np.array_equal(rec[:, 3], orec[:, 2])
+        leaf = rec[:, 0] == 1
+        assert np.array_equal(rec[leaf, 2] - rec[leaf, 1] + 1, orec[leaf, 1])
+        assert np.array_equal(box, _outward_f32(obox))
+        src = g["tree"].phr_source()
+        assert np.array_equal(np.concatenate([src[a:b + 1] for a, b in rec[leaf, 1:3]]), oprims)
+        ohit, _ = oracle.Tree(opos).closest_hit(rays.cpu().numpy())
+        assert np.mean(g["hit"].cpu().numpy() == ohit) >= 0.9999
+    finally:
+        graph.close()
+
+
+def test_phr_rebuild_graph_out_of_bounds(ctx):
+    """A replay whose frame leaves the recorded bounds -- here every triangle the
+    same, so the LBVH is one leaf and the initial cut one node, which an eager
+    rebuild answers by attaching the LBVH as is -- reports TRACE_ERR_BOUND at the
+    next synchronisation and leaves the previous frame's tree in place; the next
+    frame inside the bounds replays normally."""
+    import torch
+
+    from trace_amd.device import Graph
+
+    print("phr rebuild graph out of bounds", flush=True)
+    n = 5000
+    pos = scenes.random_triangles(n, seed=75)
+    same = np.repeat(pos[:1], n, axis=0)
+    mesh = Mesh(ctx, pos)
+    aux = Bvh.lbvh(ctx, mesh)
+    tree = Bvh.phr(ctx, aux)
+    tree.phr_rebuild(aux)
+    fresh = Bvh.phr(ctx, aux)
+    ctx.synchronize()
+    graph = Graph.capture(ctx, lambda: (aux.rebuild(mesh), tree.phr_rebuild(aux)))
+    try:
+        graph.launch()
+        ctx.synchronize()
+        _same_tree(tree, fresh)
+        mesh.update(same)
+        graph.launch()
+        with pytest.raises(_lib.TraceError, match="bounds"):
+            ctx.synchronize()
+        _same_tree(tree, fresh)  # the previous frame's tree
+        mesh.update(pos)
+        graph.launch()
+        ctx.synchronize()
+        _same_tree(tree, fresh)
+    finally:
+        graph.close()
+    torch.cuda.synchronize()
+
+
 # ---- branching factors above 2 (phr.go:92-152, scene.go:32-36) ---------------------------
 
 

@@ -43,10 +43,11 @@ int ctx_check_device_error(trace_ctx* ctx, const char* what) {
   if (e) {
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    TRACE_FAIL(ctx, TRACE_E_DEVICE, "%s: device reported error word 0x%x (%s%s%s)", what, e,
+    TRACE_FAIL(ctx, TRACE_E_DEVICE, "%s: device reported error word 0x%x (%s%s%s%s)", what, e,
                (e & TRACE_ERR_SPIN) ? "bounded wait expired; " : "",
                (e & TRACE_ERR_STACK) ? "traversal stack overflow; " : "",
-               (e & TRACE_ERR_TREE) ? "malformed tree reference" : "");
+               (e & TRACE_ERR_TREE) ? "malformed tree reference; " : "",
+               (e & TRACE_ERR_BOUND) ? "a replayed graph's build exceeded the bounds it was recorded with" : "");
   }
   return TRACE_OK;
 }
@@ -318,9 +319,13 @@ int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out
     out->root_box[3 + a] = h.root_hi[a];
   }
   if (bvh->key_mode < 0) {
-    out->depth = bvh->host_depth;
+    // the tree's own levels from its header (a graph replay updates them);
+    // the build statistics are the last eager build's
+    int per_level = 0;
+    while ((1 << per_level) < bvh->arity) ++per_level;
+    out->depth = bvh->phr_clone ? h.depth : h.depth * per_level;
     out->phr_initial_cut = bvh->phr_initial_cut;
-    out->phr_levels = bvh->phr_levels;
+    out->phr_levels = bvh->phr_clone ? 0 : bvh->arity == 2 ? h.phr_levels : bvh->phr_levels;
     out->phr_build_cost = bvh->phr_initial_cut + bvh->phr_split_work;
   }
   return TRACE_OK;
@@ -331,7 +336,15 @@ int trace_bvh_get_info(trace_ctx* ctx, const trace_bvh* bvh, trace_bvh_info* out
 // (a cut that no split improved, phr.go:121-126) are kept, as Cost() sees them.
 static int export_phr(trace_ctx* ctx, const trace_bvh* bvh, int64_t cap, int32_t* h_rec,
                       float* h_box, int64_t* n_out) {
-  std::vector<phr::OutNode> out(bvh->phr_out_count);
+  BvhHeader h;
+  int rc = read_header(ctx, bvh, &h);
+  if (rc) return rc;
+  // the header's count is the last build's, a graph replay's included
+  const int64_t cap_out = int64_t(bvh->phr_out.n / sizeof(phr::OutNode));
+  if (h.phr_out < 1 || h.phr_out > cap_out)
+    TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_bvh_export_canonical: PHR output count %d outside [1, %lld]", h.phr_out,
+               static_cast<long long>(cap_out));
+  std::vector<phr::OutNode> out(h.phr_out);
   HIP_TRY(ctx, hipMemcpyAsync(out.data(), bvh->phr_out.p, out.size() * sizeof(phr::OutNode),
                               hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -381,7 +394,7 @@ int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double d
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_build: need alpha >= 0, finite delta, max_cut >= 1");
   trace_bvh* b = new (std::nothrow) trace_bvh();
   if (!b) TRACE_FAIL(ctx, TRACE_E_OOM, "trace_phr_build: host allocation failed");
-  int rc = phr::build(ctx, aux, alpha, delta, branching, max_cut, b);
+  int rc = phr::build(ctx, aux, alpha, delta, branching, max_cut, b, phr::kFirstBuild);
   if (rc == TRACE_OK) rc = ctx_check_device_error(ctx, "trace_phr_build");
   if (rc != TRACE_OK) {
     delete b;
@@ -406,7 +419,21 @@ int trace_phr_rebuild(trace_ctx* ctx, const trace_bvh* aux, double alpha, double
                branching);
   if (!(alpha >= 0) || !std::isfinite(delta) || max_cut < 1)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: need alpha >= 0, finite delta, max_cut >= 1");
-  int rc = phr::build(ctx, aux, alpha, delta, branching, max_cut, phr);
+  if (ctx->capturing) {
+    // recorded in a graph: the launches of an eager binary rebuild of the same
+    // settings, with every count read on the device (no host round trip)
+    if (branching != 2)
+      TRACE_FAIL(ctx, TRACE_E_UNSUPPORTED, "trace_phr_rebuild: a rebuild under graph capture takes branching 2, not %d",
+                 branching);
+    if (phr->arity != 2 || phr->phr_clone || !phr->phr_pw_valid || phr->phr_pw_alpha != alpha ||
+        phr->phr_pw_delta != delta || phr->n_prims != aux->n_prims)
+      TRACE_FAIL(ctx, TRACE_E_INVALID,
+                 "trace_phr_rebuild: run this rebuild once eagerly (branching 2, the same alpha and delta, an "
+                 "auxiliary tree of %lld triangles, an initial cut of two or more nodes) before capturing it",
+                 static_cast<long long>(aux->n_prims));
+    return phr::build(ctx, aux, alpha, delta, branching, max_cut, phr, phr::kGraphRebuild);
+  }
+  int rc = phr::build(ctx, aux, alpha, delta, branching, max_cut, phr, phr::kRebuild);
   if (rc == TRACE_OK) rc = ctx_check_device_error(ctx, "trace_phr_rebuild");
   return rc;
 }

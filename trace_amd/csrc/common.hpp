@@ -214,7 +214,8 @@ struct BvhHeader {
   double enc_hi[3];
   int32_t wroot;      // root of the 4-wide traversal tree (-1: the root is a leaf kept whole)
   int32_t n_wide;     // wide nodes of the traversal tree (a bound for every wide index)
-  int32_t wpad[2];
+  int32_t phr_out;    // PHR trees: output nodes (n-ary structure) of the last build, else unset
+  int32_t phr_levels; // PHR trees: levels of the last build that held jobs, else unset
 };
 
 // Node of the 4-wide traversal tree (64 bytes, one load per visit).  Child
@@ -274,6 +275,12 @@ struct trace_bvh {
   int64_t phr_initial_cut = 0;
   int64_t phr_split_work = 0;  // sizes of the cuts handed to SweepSAH (BuildWithCost's counter)
   int32_t phr_levels = 0;
+  // PHR trees rebuilt in place: 2^(alpha * d + delta) for every depth d, as the
+  // host's pow computes it, for the (alpha, delta) of the last eager rebuild,
+  // so that a captured rebuild can form its thresholds on the device
+  DevArray<double> phr_pw;
+  bool phr_pw_valid = false;
+  double phr_pw_alpha = 0, phr_pw_delta = 0;
   // upper bound on inner nodes along any root-to-leaf path (sizes the
   // traversal stack); an LBVH over 64-bit keys never exceeds 63
   int32_t host_depth = 63;
@@ -301,6 +308,9 @@ struct trace_bvh {
 #define TRACE_ERR_SPIN 1u
 #define TRACE_ERR_STACK 2u
 #define TRACE_ERR_TREE 4u  // a kernel met an out-of-range node or triangle reference
+// a build replayed from a graph met input beyond the bounds it was recorded
+// with (more PHR levels, a deeper tree, an initial cut of one node)
+#define TRACE_ERR_BOUND 8u
 
 __device__ inline uint64_t lanemask_lt() {
   const uint32_t lane = __lane_id();

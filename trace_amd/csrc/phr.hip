@@ -105,7 +105,10 @@ struct Ctl {
   // jobs, and the levels that held jobs
   uint32_t cur_jobs[kLists];
   uint32_t cur_max_m, cur_lds, sel_b, levels;
-  uint32_t cur_pad[24];
+  // a rebuild recorded in a graph met a frame beyond its recorded bounds: the
+  // layout leaves the tree's arrays as the previous frame left them
+  uint32_t stop;
+  uint32_t cur_pad[23];
 };
 #undef TRACE_CTL_FIELD
 
@@ -214,7 +217,7 @@ __device__ inline uint32_t block_scan_1024(uint32_t v, uint32_t* s_wave, uint32_
 }
 
 // phr.go:257-311 in FIFO order (one workgroup of 1024 threads).
-__global__ __launch_bounds__(1024) void initial_cut_kernel(Flat f, double th0, int32_t max_cut,
+__global__ __launch_bounds__(1024) void initial_cut_kernel(Flat f, const double* __restrict__ th, int32_t max_cut,
                                                            int32_t* fa, int32_t* fb, int32_t fcap,
                                                            int32_t* pool, int32_t pool_cap,
                                                            Task* tasks, Ctl* ctl,
@@ -223,6 +226,7 @@ __global__ __launch_bounds__(1024) void initial_cut_kernel(Flat f, double th0, i
   __shared__ int32_t s_m, s_cut, s_next;
   __shared__ long long s_counter;
   const int tid = threadIdx.x;
+  const double th0 = th[0];
   if (tid == 0) {
     fa[0] = 0;  // the LBVH root
     s_m = 1;
@@ -2075,19 +2079,36 @@ __global__ __launch_bounds__(256) void group_task_kernel(Flat f, const double* _
 
 // The first device-driven level: the initial cut's job (on the large list)
 // and its pool.
-__global__ void seed_levels_kernel(Ctl* ctl) {
-  for (int r = 0; r < kLists; ++r) ctl->jobs[r][0] = r == kLarge ? 1u : 0u;
+// d_err (a rebuild recorded in a graph): an initial cut of one node, which an
+// eager build answers by attaching the LBVH as is, is out of the recorded
+// bounds; no job starts and the error word says so.
+__global__ void seed_levels_kernel(Ctl* ctl, uint32_t* d_err) {
+  const bool run = !d_err || ctl->cut >= 2u;
+  if (!run) {
+    atomicOr(d_err, TRACE_ERR_BOUND);
+    ctl->stop = 1u;
+  }
+  for (int r = 0; r < kLists; ++r) ctl->jobs[r][0] = r == kLarge && run ? 1u : 0u;
   ctl->max_m = ctl->cut;
   ctl->pool = ctl->cut;
   ctl->work = 0;
   ctl->levels = 0;
 }
 
+// Element count of a layout launch: n, or (d_n given: a launch sized to the
+// buffers' capacity n, recorded in a graph) the count the device holds.
+__device__ inline int32_t live_count(int32_t n, const uint32_t* d_n) {
+  if (!d_n) return n;
+  const uint32_t c = *d_n;
+  return c < uint32_t(n) ? int32_t(c) : n;
+}
+
 // Leaves of the output tree: one atomic per workgroup.
 __global__ __launch_bounds__(256) void count_leaves_kernel(const OutNode* __restrict__ out, int32_t n,
-                                                           Ctl* ctl) {
+                                                           const uint32_t* d_n, Ctl* ctl) {
   __shared__ uint32_t s_n;
   if (threadIdx.x == 0) s_n = 0;
+  n = live_count(n, d_n);
   __syncthreads();
   const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o < n && out[o].kind == 1) atomicAdd(&s_n, 1u);
@@ -2104,16 +2125,20 @@ __global__ void reset_level_kernel(Ctl* ctl) {
 }
 
 // ---- layout of the traversal tree ------------------------------------------------
-__global__ void branch_flag_kernel(const OutNode* __restrict__ out, int32_t n, uint32_t* flag) {
+// Flags over all n slots (0 past the live count, so a scan over the
+// capacity gives the live total).
+__global__ void branch_flag_kernel(const OutNode* __restrict__ out, int32_t n, const uint32_t* d_n, uint32_t* flag) {
   const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o < n) flag[o] = (out[o].kind == 0 && out[o].nchild >= 2) ? 1u : 0u;
+  const int32_t live = live_count(n, d_n);
+  if (o < n) flag[o] = (o < live && out[o].kind == 0 && out[o].nchild >= 2) ? 1u : 0u;
 }
 
-__global__ void trav_nodes_kernel(const OutNode* __restrict__ out, int32_t n,
+__global__ void trav_nodes_kernel(const OutNode* __restrict__ out, int32_t n, const uint32_t* d_n,
                                   const uint32_t* __restrict__ flag,
-                                  const uint32_t* __restrict__ tindex, BvhNode* __restrict__ nodes) {
+                                  const uint32_t* __restrict__ tindex, BvhNode* __restrict__ nodes,
+                                  const uint32_t* stop) {
   const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n || !flag[o]) return;
+  if ((stop && *stop) || o >= live_count(n, d_n) || !flag[o]) return;
   BvhNode nd;
   float* boxes = reinterpret_cast<float*>(&nd);
   for (int c = 0; c < 2; ++c) {
@@ -2217,19 +2242,32 @@ __global__ void nary_base_kernel(const uint32_t* __restrict__ sorted, const uint
 // (the nodes of one level cover disjoint, non-empty triangle ranges) and puts
 // the root first.  The tree's layout, and every sum taken over its nodes in
 // index order, are then the same on every run.
-__global__ void canon_keys_kernel(const OutNode* __restrict__ out, int32_t n, const uint32_t* __restrict__ flag,
-                                  const uint32_t* __restrict__ tindex, int prim_bits, uint64_t* __restrict__ keys,
-                                  uint32_t* __restrict__ vals) {
+__global__ void canon_keys_kernel(const OutNode* __restrict__ out, int32_t n, const uint32_t* d_n,
+                                  const uint32_t* __restrict__ flag, const uint32_t* __restrict__ tindex,
+                                  int prim_bits, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n || !flag[o]) return;
+  if (o >= live_count(n, d_n) || !flag[o]) return;
   const uint32_t i = tindex[o];
   keys[i] = (uint64_t(uint32_t(out[o].level)) << prim_bits) | uint64_t(uint32_t(out[o].prim_off));
   vals[i] = uint32_t(o);
 }
 
-__global__ void canon_rank_kernel(const uint32_t* __restrict__ sorted, int32_t n_trav, uint32_t* __restrict__ tindex) {
+__global__ void canon_rank_kernel(const uint32_t* __restrict__ sorted, int32_t n_trav, const uint32_t* d_n,
+                                  uint32_t* __restrict__ tindex) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < n_trav) tindex[sorted[r]] = uint32_t(r);
+  if (r < live_count(n_trav, d_n)) tindex[sorted[r]] = uint32_t(r);
+}
+
+// A sort over the capacity n (recorded in a graph): the slots past the live
+// branch count hold keys of all ones, which order after every real key
+// (level and first triangle both below their fields' maxima).
+__global__ void canon_pad_kernel(const uint32_t* __restrict__ d_n, int32_t n, uint64_t mask,
+                                 uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && i >= live_count(n, d_n)) {
+    keys[i] = mask;
+    vals[i] = 0xffffffffu;
+  }
 }
 
 // Eight lanes per output node (most leaves hold a few triangles; branches
@@ -2239,10 +2277,11 @@ __global__ __launch_bounds__(256) void leaf_copy_kernel(
     const OutNode* __restrict__ out, int32_t n_out, const int32_t* __restrict__ list, Flat f,
     const int32_t* __restrict__ perm_in, const float* __restrict__ tris_in,
     const float* __restrict__ box_in, int32_t* __restrict__ perm_out, int32_t* __restrict__ src_out,
-    float* __restrict__ tris_out, float* __restrict__ box_out, int32_t n, Ctl* ctl) {
+    float* __restrict__ tris_out, float* __restrict__ box_out, int32_t n, Ctl* ctl, const uint32_t* d_n,
+    const uint32_t* stop) {
   const int32_t o = int32_t((blockIdx.x * 256u + threadIdx.x) / kCopyLanes);
   const int g = int(threadIdx.x) & (kCopyLanes - 1);
-  if (o >= n_out) return;
+  if ((stop && *stop) || o >= live_count(n_out, d_n)) return;
   const OutNode nd = out[o];
   if (nd.kind != 1) return;
   if (nd.prim_off < 0 || nd.prim_cnt < 0 || nd.prim_cnt > n - nd.prim_off) {
@@ -2272,6 +2311,7 @@ __global__ __launch_bounds__(256) void leaf_copy_kernel(
 __global__ void phr_header_kernel(const OutNode* __restrict__ out, const BvhHeader* __restrict__ aux,
                                   const Ctl* __restrict__ ctl, const uint32_t* __restrict__ n_trav,
                                   int32_t n, BvhHeader* __restrict__ hdr) {
+  if (ctl->stop) return;  // a replay beyond its recorded bounds keeps the previous tree
   BvhHeader h = *aux;
   const OutNode& root = out[0];
   if (root.kind == 0 && root.nchild >= 2) {
@@ -2285,7 +2325,48 @@ __global__ void phr_header_kernel(const OutNode* __restrict__ out, const BvhHead
   h.n_leaves = int32_t(ctl->n_leaves);
   h.depth = int32_t(ctl->max_level);
   h.n_prims = n;
+  h.phr_out = int32_t(ctl->out);
+  h.phr_levels = int32_t(ctl->levels);
   *hdr = h;
+}
+
+// ---- a rebuild recorded in a graph ------------------------------------------------
+// Thresholds S / 2^(alpha*d + delta) (phr.go:356-358) from the LBVH root box
+// on the device: S in FP64 with the host's operation order, divided by the
+// powers the host computed (one IEEE division each, so every threshold equals
+// the host's).
+__global__ void thresholds_kernel(const BvhHeader* __restrict__ aux, const double* __restrict__ pw,
+                                  double* __restrict__ th) {
+#pragma clang fp contract(off)
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= kMaxDepth) return;
+  const double w = double(aux->root_hi[0]) - double(aux->root_lo[0]);
+  const double hgt = double(aux->root_hi[1]) - double(aux->root_lo[1]);
+  const double dep = double(aux->root_hi[2]) - double(aux->root_lo[2]);
+  const double S = 2 * w * hgt + 2 * w * dep + 2 * hgt * dep;
+  th[d] = S / pw[d];
+}
+
+// After the recorded levels: jobs still pending mean the frame needed more
+// levels than were recorded; a counter error of the levels is a tree error.
+__global__ void levels_check_kernel(Ctl* ctl, uint32_t* d_err) {
+  uint32_t pending = 0;
+  for (int r = 0; r < kLists; ++r) pending += ctl->cur_jobs[r];
+  if (pending) {
+    atomicOr(d_err, TRACE_ERR_BOUND);
+    ctl->stop = 1u;
+  }
+  if (ctl->err) {
+    atomicOr(d_err, TRACE_ERR_TREE);
+    ctl->stop = 1u;
+  }
+}
+
+// After the layout: the traversal kernels and the widening pass were recorded
+// for trees of at most max_depth binary levels.
+__global__ void layout_check_kernel(const Ctl* __restrict__ ctl, int32_t max_depth, uint32_t* d_err) {
+  if (ctl->err) atomicOr(d_err, TRACE_ERR_TREE);
+  if (!ctl->stop && int64_t(ctl->max_level) > int64_t(max_depth)) atomicOr(d_err, TRACE_ERR_BOUND);
 }
 
 // ---- host driver ---------------------------------------------------------------
@@ -2339,7 +2420,7 @@ static int clone_lbvh(trace_ctx* ctx, const trace_bvh* aux, trace_bvh* out) {
 static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching, const Flat& f, OutNode* outn,
                   const int32_t* list, Ctl* ctl, uint32_t* flag, uint32_t* tidx, uint32_t* sscr, uint32_t* n_trav,
                   int32_t* ord, double* gkey, double* rcost, uint32_t* status, int64_t pool_cap, int levels,
-                  trace_bvh* out) {
+                  bool fixed_caps, trace_bvh* out) {
   hipStream_t st = ctx->stream;
   Ctl hc;
   int rc = read_ctl(ctx, ctl, &hc);
@@ -2348,8 +2429,8 @@ static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching
   out->phr_out_count = n_out;
   out->phr_levels = levels;
 
-  branch_flag_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag);
-  count_leaves_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, ctl);
+  branch_flag_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, nullptr, flag);
+  count_leaves_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, nullptr, ctl);
   scan::exclusive_u32(st, flag, tidx, n_out, n_trav, sscr);
   uint32_t h_real = 0;
   HIP_TRY(ctx, hipMemcpyAsync(&h_real, n_trav, 4, hipMemcpyDeviceToHost, st));
@@ -2370,7 +2451,8 @@ static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching
     auto* keys_b = reinterpret_cast<uint64_t*>(rcost);
     auto* vals_a = reinterpret_cast<uint32_t*>(ord);
     auto* vals_b = reinterpret_cast<uint32_t*>(ord + pool_cap);
-    canon_keys_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, prim_bits, keys_a, vals_a);
+    canon_keys_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, nullptr, flag, tidx, prim_bits, keys_a,
+                                                                 vals_a);
     const int where = onesweep::sort_pairs(st, keys_a, vals_a, keys_b, vals_b, h_real, prim_bits + level_bits,
                                            status, ctx->d_err, false, nullptr, 0, onesweep::Config{});
     const uint32_t* sorted = where ? vals_b : vals_a;
@@ -2384,7 +2466,7 @@ static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching
       nary_base_kernel<<<int(ceil_div(h_real, 256)), 256, 0, st>>>(sorted, b, int32_t(h_real), tidx);
       HIP_TRY(ctx, hipMemcpyAsync(&h_ntrav, d_ntrav, 4, hipMemcpyDeviceToHost, st));
     } else {
-      canon_rank_kernel<<<int(ceil_div(h_real, 256)), 256, 0, st>>>(sorted, int32_t(h_real), tidx);
+      canon_rank_kernel<<<int(ceil_div(h_real, 256)), 256, 0, st>>>(sorted, int32_t(h_real), nullptr, tidx);
     }
     rc = ctx_check_device_error(ctx, "trace_phr_build");
     if (rc) return rc;
@@ -2398,7 +2480,8 @@ static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching
     HIP_TRY(ctx, hipMemcpyAsync(d_ntrav, &h_ntrav, 4, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipStreamSynchronize(st));
   }
-  HIP_TRY(ctx, out->nodes.alloc(std::max<uint32_t>(1, h_ntrav)));
+  // fixed capacities: the most branches a binary tree of n triangles holds
+  HIP_TRY(ctx, out->nodes.alloc(std::max<int64_t>({int64_t(1), int64_t(h_ntrav), fixed_caps ? n - 1 : 0})));
   HIP_TRY(ctx, out->perm.alloc(n));
   HIP_TRY(ctx, out->phr_src.alloc(n));
   HIP_TRY(ctx, out->tris.alloc(9 * n));
@@ -2410,11 +2493,12 @@ static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching
                                                                  out->virt.p);
   } else {
     out->virt.release();  // binary: no virtual nodes (a rebuild of a tree that was n-ary)
-    trav_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, flag, tidx, out->nodes.p);
+    trav_nodes_kernel<<<int(ceil_div(n_out, 256)), 256, 0, st>>>(outn, n_out, nullptr, flag, tidx, out->nodes.p,
+                                                                 nullptr);
   }
   leaf_copy_kernel<<<int(ceil_div(int64_t(n_out) * kCopyLanes, 256)), 256, 0, st>>>(outn, n_out, list, f, aux->perm.p, aux->tris.p, aux->prim_box.p,
                                         out->perm.p, out->phr_src.p, out->tris.p, out->prim_box.p,
-                                        int32_t(n), ctl);
+                                        int32_t(n), ctl, nullptr, nullptr);
   phr_header_kernel<<<1, 1, 0, st>>>(outn, aux->header.p, ctl, d_ntrav, int32_t(n), out->header.p);
   HIP_TRY(ctx, hipGetLastError());
   rc = read_ctl(ctx, ctl, &hc);
@@ -2424,43 +2508,103 @@ static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching
   int per_level = 0;
   while ((1 << per_level) < branching) ++per_level;
   out->host_depth = int32_t(hc.max_level) * per_level;
+  // fixed capacities: the traversal's depth class (stack of 128 entries up to
+  // depth 63, 512 up to 255), which a captured rebuild keeps for every frame
+  if (fixed_caps && out->host_depth <= 255) out->host_depth = out->host_depth <= 63 ? 63 : 255;
+  return wide::widen(ctx, out);
+}
+
+// A binary rebuild recorded in a graph: the layout with every launch sized by
+// the buffers' capacities, the live counts read on the device (output nodes
+// from the counters, branches from the scan), the canonical order sorted over
+// the most branches n triangles make with padding keys after the real ones.
+static int layout_graph(trace_ctx* ctx, const trace_bvh* aux, int64_t n, const Flat& f, OutNode* outn,
+                        const int32_t* list, Ctl* ctl, uint32_t* flag, uint32_t* tidx, uint32_t* sscr,
+                        uint32_t* n_trav, int32_t* ord, double* gkey, double* rcost, uint32_t* status,
+                        int64_t pool_cap, int32_t out_cap, int rec_levels, trace_bvh* out) {
+  hipStream_t st = ctx->stream;
+  const uint32_t* d_out = &ctl->out;
+  const int gb = int(ceil_div(out_cap, 256));
+  branch_flag_kernel<<<gb, 256, 0, st>>>(outn, out_cap, d_out, flag);
+  count_leaves_kernel<<<gb, 256, 0, st>>>(outn, out_cap, d_out, ctl);
+  scan::exclusive_u32(st, flag, tidx, out_cap, n_trav, sscr);
+  const int32_t scap = int32_t(std::max<int64_t>(1, n - 1));
+  // output levels stay below the recorded job levels + 1
+  int prim_bits = 1, level_bits = 1;
+  while ((int64_t(1) << prim_bits) < n) ++prim_bits;
+  while ((int64_t(1) << level_bits) <= int64_t(rec_levels) + 1) ++level_bits;
+  const int bits = prim_bits + level_bits;
+  if (int64_t(scap) > pool_cap || bits > 64)
+    TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_rebuild: layout buffers too small");
+  auto* keys_a = reinterpret_cast<uint64_t*>(gkey);
+  auto* keys_b = reinterpret_cast<uint64_t*>(rcost);
+  auto* vals_a = reinterpret_cast<uint32_t*>(ord);
+  auto* vals_b = reinterpret_cast<uint32_t*>(ord + pool_cap);
+  canon_keys_kernel<<<gb, 256, 0, st>>>(outn, out_cap, d_out, flag, tidx, prim_bits, keys_a, vals_a);
+  const uint64_t mask = bits == 64 ? ~0ull : (uint64_t(1) << bits) - 1;
+  canon_pad_kernel<<<int(ceil_div(scap, 256)), 256, 0, st>>>(n_trav, scap, mask, keys_a, vals_a);
+  const int where = onesweep::sort_pairs(st, keys_a, vals_a, keys_b, vals_b, scap, bits, status, ctx->d_err, false,
+                                         nullptr, 0, onesweep::Config{});
+  const uint32_t* sorted = where ? vals_b : vals_a;
+  canon_rank_kernel<<<int(ceil_div(scap, 256)), 256, 0, st>>>(sorted, scap, n_trav, tidx);
+  trav_nodes_kernel<<<gb, 256, 0, st>>>(outn, out_cap, d_out, flag, tidx, out->nodes.p, &ctl->stop);
+  leaf_copy_kernel<<<int(ceil_div(int64_t(out_cap) * kCopyLanes, 256)), 256, 0, st>>>(
+      outn, out_cap, list, f, aux->perm.p, aux->tris.p, aux->prim_box.p, out->perm.p, out->phr_src.p, out->tris.p,
+      out->prim_box.p, int32_t(n), ctl, d_out, &ctl->stop);
+  phr_header_kernel<<<1, 1, 0, st>>>(outn, aux->header.p, ctl, n_trav, int32_t(n), out->header.p);
+  layout_check_kernel<<<1, 1, 0, st>>>(ctl, out->host_depth, ctx->d_err);
+  HIP_TRY(ctx, hipGetLastError());
   return wide::widen(ctx, out);
 }
 
 int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int branching,
-          int32_t max_cut, trace_bvh* out) {
+          int32_t max_cut, trace_bvh* out, Mode mode) {
 #pragma clang fp contract(off)
   const int64_t n = aux->n_prims;
   hipStream_t st = ctx->stream;
-  out->n_prims = n;
-  out->key_mode = -1;
-  out->has_spheres = aux->has_spheres;
-  out->arity = branching;
-  // a rebuild starts from the state of a fresh tree
-  out->phr_clone = false;
-  out->phr_levels = 0;
-  out->phr_out_count = 0;
-  out->phr_alpha = alpha;
-  out->phr_delta = delta;
+  const bool graph = mode == kGraphRebuild;
+  // a binary rebuild sizes its arrays and scratch for any frame of n
+  // triangles (n distinct keys, n - 1 branches), so that the same rebuild
+  // recorded in a graph later finds everything in place
+  const bool fixed_caps = mode != kFirstBuild && branching == 2;
+  if (graph && (branching != 2 || out->phr_clone || !out->phr_pw_valid || !out->phr_pw.fits(kMaxDepth)))
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: a captured rebuild needs an eager binary rebuild first");
 
-  BvhHeader ah;
-  HIP_TRY(ctx, hipMemcpyAsync(&ah, aux->header.p, sizeof(ah), hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  out->phr_split_work = 0;
-  if (ah.root_cnt > 0) {  // a one-leaf LBVH: the initial cut is that leaf
-    out->phr_initial_cut = 1;
-    return clone_lbvh(ctx, aux, out);
+  BvhHeader ah{};
+  std::vector<double> th, pw;
+  if (!graph) {
+    out->n_prims = n;
+    out->key_mode = -1;
+    out->has_spheres = aux->has_spheres;
+    out->arity = branching;
+    // a rebuild starts from the state of a fresh tree
+    out->phr_clone = false;
+    out->phr_levels = 0;
+    out->phr_out_count = 0;
+    out->phr_alpha = alpha;
+    out->phr_delta = delta;
+    HIP_TRY(ctx, hipMemcpyAsync(&ah, aux->header.p, sizeof(ah), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    out->phr_split_work = 0;
+    if (ah.root_cnt > 0) {  // a one-leaf LBVH: the initial cut is that leaf
+      out->phr_initial_cut = 1;
+      return clone_lbvh(ctx, aux, out);
+    }
+    // thresholds S / 2^(alpha*depth + delta) (phr.go:356-358), FP64 on the host
+    const float rb[6] = {ah.root_lo[0], ah.root_lo[1], ah.root_lo[2], ah.root_hi[0], ah.root_hi[1], ah.root_hi[2]};
+    const double w = double(rb[3]) - double(rb[0]), hgt = double(rb[4]) - double(rb[1]),
+                 dep = double(rb[5]) - double(rb[2]);
+    const double S = 2 * w * hgt + 2 * w * dep + 2 * hgt * dep;
+    th.resize(kMaxDepth);
+    pw.resize(kMaxDepth);
+    for (int d = 0; d < kMaxDepth; ++d) {
+      pw[d] = std::pow(2.0, alpha * double(d) + delta);
+      th[d] = S / pw[d];
+    }
   }
 
-  const int32_t U = ah.n_leaves;
+  const int32_t U = fixed_caps ? int32_t(n) : ah.n_leaves;
   const int64_t F = 2 * int64_t(U) - 1;
-  // thresholds S / 2^(alpha*depth + delta) (phr.go:356-358), FP64 on the host
-  const float rb[6] = {ah.root_lo[0], ah.root_lo[1], ah.root_lo[2], ah.root_hi[0], ah.root_hi[1], ah.root_hi[2]};
-  const double w = double(rb[3]) - double(rb[0]), hgt = double(rb[4]) - double(rb[1]),
-               dep = double(rb[5]) - double(rb[2]);
-  const double S = 2 * w * hgt + 2 * w * dep + 2 * hgt * dep;
-  std::vector<double> th(kMaxDepth);
-  for (int d = 0; d < kMaxDepth; ++d) th[d] = S / std::pow(2.0, alpha * double(d) + delta);
 
   const int64_t fcap = std::min<int64_t>(2 * int64_t(std::max<int32_t>(max_cut, 1)) + 64, 2 * F + 64);
   const int64_t pool_cap = F + 64;
@@ -2528,22 +2672,41 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   auto* n_trav = reinterpret_cast<uint32_t*>(base + o_ntr);
   Group* gbuf[2] = {reinterpret_cast<Group*>(base + o_ga), reinterpret_cast<Group*>(base + o_gb)};
 
-  HIP_TRY(ctx, out->phr_out.alloc(size_t(out_cap) * sizeof(OutNode)));
+  if (graph) {
+    if (!out->phr_out.fits(size_t(out_cap) * sizeof(OutNode)) || !out->nodes.fits(size_t(std::max<int64_t>(1, n - 1))))
+      TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: a captured rebuild needs an eager binary rebuild first");
+  } else {
+    HIP_TRY(ctx, out->phr_out.alloc(size_t(out_cap) * sizeof(OutNode)));
+  }
   OutNode* outn = reinterpret_cast<OutNode*>(out->phr_out.p);
 
-  HIP_TRY(ctx, hipMemcpyAsync(d_th, th.data(), kMaxDepth * sizeof(double), hipMemcpyHostToDevice, st));
+  if (graph) {
+    thresholds_kernel<<<int(ceil_div(kMaxDepth, 256)), 256, 0, st>>>(aux->header.p, out->phr_pw.p, d_th);
+  } else {
+    HIP_TRY(ctx, hipMemcpyAsync(d_th, th.data(), kMaxDepth * sizeof(double), hipMemcpyHostToDevice, st));
+    out->phr_pw_valid = false;
+    if (fixed_caps) {
+      HIP_TRY(ctx, out->phr_pw.alloc(kMaxDepth));
+      HIP_TRY(ctx, hipMemcpyAsync(out->phr_pw.p, pw.data(), kMaxDepth * sizeof(double), hipMemcpyHostToDevice, st));
+      out->phr_pw_valid = true;
+      out->phr_pw_alpha = alpha;
+      out->phr_pw_delta = delta;
+    }
+  }
   fill_async<uint32_t>(st, reinterpret_cast<uint32_t*>(ctl), sizeof(Ctl) / 4, 0u);
   flat_kernel<<<int(ceil_div(F, 256)), 256, 0, st>>>(aux->nodes.p, aux->parent.p, aux->parent.p + n,
                                                     aux->inner_range.p, aux->run_start.p,
                                                     aux->header.p, f);
-  initial_cut_kernel<<<1, 1024, 0, st>>>(f, th[0], max_cut, fa, fb, int32_t(fcap), pools[0],
+  initial_cut_kernel<<<1, 1024, 0, st>>>(f, d_th, max_cut, fa, fb, int32_t(fcap), pools[0],
                                          int32_t(pool_cap), jobs[0][kLarge], ctl, aux->header.p);
-  Ctl hc;
-  rc = read_ctl(ctx, ctl, &hc);
-  if (rc) return rc;
-  if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: initial cut exceeded its buffers");
-  out->phr_initial_cut = hc.cut;
-  if (hc.cut <= 1) return clone_lbvh(ctx, aux, out);  // the whole LBVH is attached as is
+  Ctl hc{};
+  if (!graph) {
+    rc = read_ctl(ctx, ctl, &hc);
+    if (rc) return rc;
+    if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: initial cut exceeded its buffers");
+    out->phr_initial_cut = hc.cut;
+    if (hc.cut <= 1) return clone_lbvh(ctx, aux, out);  // the whole LBVH is attached as is
+  }
 
   const Caps caps{pool_cap, out_cap, list_cap};
   int cur = 0;
@@ -2593,7 +2756,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
     }
     out->phr_split_work = int64_t(hc.work);
     return layout(ctx, aux, n, branching, f, outn, list, ctl, flag, tidx, sscr, n_trav, ord, gkey, rcost,
-                  reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, levels, out);
+                  reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, levels, false, out);
   }
   // Device-driven levels, kLevelBatch per read of the counters: each level's
   // kernels take its job counts from the device (advance_level_kernel), so
@@ -2606,31 +2769,42 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const size_t lds_full = size_t(kSortLds) * (sizeof(double) + sizeof(int32_t));
   const size_t lds_small = size_t(kSmallLds) * (sizeof(double) + sizeof(int32_t));
   const int n_cu = std::max(1, ctx->n_cu);
-  seed_levels_kernel<<<1, 1, 0, st>>>(ctl);
-  advance_level_kernel<<<1, 1, 0, st>>>(ctl);  // later levels are advanced by the small jobs' launch
-  for (;;) {
-    for (int k = 0; k < kLevelBatch; ++k) {
-      Lists nx;
-      for (int r = 0; r < kLists; ++r) {
-        nx.p[r] = jobs[cur ^ 1][r];
-        nx.cap[r] = list_caps[r];
-      }
-      const Task* lg = jobs[cur][kLarge];
-      // the large jobs: one of the three launches is the level's (sel_b)
-      task_dev_kernel<512><<<2048, 512, lds_full, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
-                                                        rcost, outn, list, ctl, caps, kSortLds);
-      task_dev_kernel<256><<<2048, 256, lds_small, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
-                                                         rcost, outn, list, ctl, caps, kSmallLds);
-      task_dev_kernel<64><<<n_cu * 16, 64, lds_small, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
-                                                            rcost, outn, list, ctl, caps, kSmallLds);
-      group_task_kernel<kWaveCut><<<n_cu * 8, 256, 0, st>>>(f, d_th, jobs[cur][2], pools[cur], pools[cur ^ 1], nx,
-                                                            outn, list, ctl, caps);
-      group_task_kernel<kGroupCut><<<n_cu * 4, 256, 0, st>>>(f, d_th, jobs[cur][1], pools[cur], pools[cur ^ 1], nx,
-                                                             outn, list, ctl, caps);
-      small_task_kernel<<<n_cu * 2, 256, 0, st>>>(f, d_th, jobs[cur][0], pools[cur], pools[cur ^ 1], nx, outn, list,
-                                                  ctl, caps);
-      cur ^= 1;
+  auto level = [&]() {
+    Lists nx;
+    for (int r = 0; r < kLists; ++r) {
+      nx.p[r] = jobs[cur ^ 1][r];
+      nx.cap[r] = list_caps[r];
     }
+    const Task* lg = jobs[cur][kLarge];
+    // the large jobs: one of the three launches is the level's (sel_b)
+    task_dev_kernel<512><<<2048, 512, lds_full, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
+                                                      rcost, outn, list, ctl, caps, kSortLds);
+    task_dev_kernel<256><<<2048, 256, lds_small, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
+                                                       rcost, outn, list, ctl, caps, kSmallLds);
+    task_dev_kernel<64><<<n_cu * 16, 64, lds_small, st>>>(f, d_th, lg, pools[cur], pools[cur ^ 1], nx, ord, gkey,
+                                                          rcost, outn, list, ctl, caps, kSmallLds);
+    group_task_kernel<kWaveCut><<<n_cu * 8, 256, 0, st>>>(f, d_th, jobs[cur][2], pools[cur], pools[cur ^ 1], nx,
+                                                          outn, list, ctl, caps);
+    group_task_kernel<kGroupCut><<<n_cu * 4, 256, 0, st>>>(f, d_th, jobs[cur][1], pools[cur], pools[cur ^ 1], nx,
+                                                           outn, list, ctl, caps);
+    small_task_kernel<<<n_cu * 2, 256, 0, st>>>(f, d_th, jobs[cur][0], pools[cur], pools[cur ^ 1], nx, outn, list,
+                                                ctl, caps);
+    cur ^= 1;
+  };
+  seed_levels_kernel<<<1, 1, 0, st>>>(ctl, graph ? ctx->d_err : nullptr);
+  advance_level_kernel<<<1, 1, 0, st>>>(ctl);  // later levels are advanced by the small jobs' launch
+  if (graph) {
+    // the levels of the last eager rebuild and a few more; levels past the
+    // frame's last one find no jobs
+    const int rec = std::max(1, out->phr_levels) + kGraphSpareLevels;
+    for (int k = 0; k < rec; ++k) level();
+    levels_check_kernel<<<1, 1, 0, st>>>(ctl, ctx->d_err);
+    HIP_TRY(ctx, hipGetLastError());
+    return layout_graph(ctx, aux, n, f, outn, list, ctl, flag, tidx, sscr, n_trav, ord, gkey, rcost,
+                        reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, out_cap, rec, out);
+  }
+  for (;;) {
+    for (int k = 0; k < kLevelBatch; ++k) level();
     rc = read_ctl(ctx, ctl, &hc);
     if (rc) return rc;
     if (hc.err) TRACE_FAIL(ctx, TRACE_E_DEVICE, "trace_phr_build: device error 0x%x by level %u", hc.err, hc.levels);
@@ -2643,7 +2817,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   levels = int(hc.levels);
   out->phr_split_work = int64_t(hc.work);
   return layout(ctx, aux, n, branching, f, outn, list, ctl, flag, tidx, sscr, n_trav, ord, gkey, rcost,
-                reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, levels, out);
+                reinterpret_cast<uint32_t*>(base + o_cst), pool_cap, levels, fixed_caps, out);
 }
 
 }  // namespace phr
