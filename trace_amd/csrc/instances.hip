@@ -109,9 +109,11 @@ __global__ __launch_bounds__(THREADS) void instanced_hit_kernel(
   const float4 b = rays[2 * i + 1];
   const traverse::Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   traverse::WideState s;
+  traverse::TravSpill<STACK> spill;
   traverse::TravStack<STACK> stack;
-  stack.lds = s_top + threadIdx.x;
+  stack.lds = traverse::lds_column(s_top + threadIdx.x);
   stack.stride = THREADS;
+  stack.spill = spill.v;
   bool overflow = false, bad = false;
   int32_t nn = 0, nt = 0;
   // s.h.t: the best distance so far; s.h.pos: the winner's flattened index

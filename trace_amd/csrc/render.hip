@@ -657,9 +657,11 @@ __global__ __launch_bounds__(THREADS) void trace_persistent(
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
   const float tmin = g.k == 0 ? 0.001f : 0.0001f;
+  traverse::TravSpill<STACK> spill;
   traverse::TravStack<STACK> stack;
-  stack.lds = s_stack + threadIdx.x;
+  stack.lds = traverse::lds_column(s_stack + threadIdx.x);
   stack.stride = THREADS;
+  stack.spill = spill.v;
   traverse::WideState s;
   int64_t slot = 0;
   uint32_t n_bound = 0;

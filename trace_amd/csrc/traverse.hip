@@ -67,9 +67,11 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(COUNT |
   const uint32_t n_bound = uint32_t(hdr.n_wide);
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
+  TravSpill<STACK> spill;
   TravStack<STACK> stack;
-  stack.lds = s_stack + threadIdx.x;
+  stack.lds = lds_column(s_stack + threadIdx.x);
   stack.stride = THREADS;
+  stack.spill = spill.v;
   WideState s;
   Ray ray{};
   int64_t rid = -1;

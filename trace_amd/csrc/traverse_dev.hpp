@@ -201,19 +201,40 @@ __device__ inline void leaf_test(const RayPre& r, const float* __restrict__ tris
 // to a per-lane private array.
 constexpr int kLdsStack = 16;
 
+// The deeper entries' private array is held apart from the stack's pointers:
+// an array indexed by a lane's stack depth lives in scratch memory, and a
+// struct that embedded it would keep the LDS column pointer and the stride
+// there too, so that every push and pop would first load them from scratch
+// (and address LDS through flat instructions).
+template <int STACK>
+struct TravSpill {
+  int32_t v[STACK > kLdsStack ? STACK - kLdsStack : 1];
+};
+
+// An LDS pointer typed as such: a load that may come from LDS or from the
+// private array is then two loads of different pointer types, which the
+// compiler cannot fold into one flat load of a selected address.
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+
 template <int STACK>
 struct TravStack {
-  int32_t* lds;  // this lane's column: entry e at lds[e * stride]
+  lds_i32* lds;    // this lane's column: entry e at lds[e * stride]
   int stride;
-  int32_t spill[STACK > kLdsStack ? STACK - kLdsStack : 1];
+  int32_t* spill;  // TravSpill<STACK>::v: entry e >= kLdsStack at spill[e - kLdsStack]
   __device__ inline void put(int sp, int32_t v) {
     if (sp < kLdsStack) lds[sp * stride] = v;
     else spill[sp - kLdsStack] = v;
   }
   __device__ inline int32_t get(int sp) const {
-    return sp < kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+    int32_t v;
+    if (sp < kLdsStack) v = lds[sp * stride];
+    else v = spill[sp - kLdsStack];
+    return v;
   }
 };
+
+// The LDS column of this lane in a __shared__ stack array.
+__device__ inline lds_i32* lds_column(int32_t* shared_base) { return (lds_i32*)(shared_base); }
 
 // ---- wide tree (WideNode: 4 children, quantised boxes) -----------------------
 //
@@ -380,9 +401,11 @@ __device__ inline Hit closest_hit_wide(const WideNode* __restrict__ wnodes,
                                        const Ray& ray, int32_t& n_nodes, int32_t& n_tris,
                                        bool& overflow, bool& bad, int32_t* lds_col, int stride) {
   WideState s;
+  TravSpill<STACK> spill;
   TravStack<STACK> stack;
-  stack.lds = lds_col;
+  stack.lds = lds_column(lds_col);
   stack.stride = stride;
+  stack.spill = spill.v;
   if (wide_begin<COUNT, SPH>(tris, hdr, ray, s, n_tris, bad))
     while (wide_step<STACK, COUNT, SPH>(wnodes, tris, uint32_t(hdr.n_wide), hdr.n_prims, s, stack,
                                    n_nodes, n_tris, overflow, bad)) {
