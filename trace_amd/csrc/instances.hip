@@ -118,27 +118,22 @@ __global__ __launch_bounds__(THREADS) void instanced_hit_kernel(
   int32_t nn = 0, nt = 0;
   // s.h.t: the best distance so far; s.h.pos: the winner's flattened index
   int32_t best_inst = -1, best_local = -1;
-  auto leaf = [&](int32_t first, int32_t count) {
-    if (first < 0 || count > th.n_prims - first) {
-      bad = true;
-      return;
-    }
-    for (int32_t k = first; k < first + count; ++k) {
-      const int32_t q = tlas_perm[k];
-      const InstRec& I = inst[q];
-      const BlasDesc B = blas[I.blas];
-      const BvhHeader bh = *B.hdr;
-      const traverse::Ray ro = to_object(I, ray, s.h.t);
-      const traverse::Hit h = traverse::closest_hit_wide<STACK, false, false>(
-          B.wnodes, B.tris, bh, ro, nn, nt, overflow, bad, s_bot + threadIdx.x, THREADS);
-      if (h.pos < 0) continue;
-      const int32_t gid = I.tri_base + (B.perm[h.pos] - B.id_offset);
-      if (h.t < s.h.t || (h.t == s.h.t && gid > s.h.pos)) {
-        s.h.t = h.t;
-        s.h.pos = gid;
-        best_inst = q;
-        best_local = h.pos;
-      }
+  // an instance leaf's primitive k: the walk of its bottom-level tree
+  auto prim = [&](int32_t k) {
+    const int32_t q = tlas_perm[k];
+    const InstRec& I = inst[q];
+    const BlasDesc B = blas[I.blas];
+    const BvhHeader bh = *B.hdr;
+    const traverse::Ray ro = to_object(I, ray, s.h.t);
+    const traverse::Hit h = traverse::closest_hit_wide<STACK, false, false>(
+        B.wnodes, B.tris, bh, ro, nn, nt, overflow, bad, s_bot + threadIdx.x, THREADS);
+    if (h.pos < 0) return;
+    const int32_t gid = I.tri_base + (B.perm[h.pos] - B.id_offset);
+    if (h.t < s.h.t || (h.t == s.h.t && gid > s.h.pos)) {
+      s.h.t = h.t;
+      s.h.pos = gid;
+      best_inst = q;
+      best_local = h.pos;
     }
   };
   s.r = traverse::ray_pre(ray);
@@ -154,9 +149,13 @@ __global__ __launch_bounds__(THREADS) void instanced_hit_kernel(
   if (traverse::slab(s.r, th.root_lo[0], th.root_lo[1], th.root_lo[2], th.root_hi[0], th.root_hi[1], th.root_hi[2],
                      s.h.t, tn)) {
     if (th.root_cnt > 0 && th.wroot < 0) {
-      leaf(th.root_ref, th.root_cnt);  // a top level of one leaf
+      // a top level of one leaf
+      if (th.root_ref < 0 || th.root_cnt > th.n_prims - th.root_ref) bad = true;
+      else
+        for (int32_t k = th.root_ref; k < th.root_ref + th.root_cnt; ++k) prim(k);
     } else {
-      while (traverse::wide_step_with<STACK, false>(tlas, uint32_t(th.n_wide), s, stack, nn, overflow, bad, leaf)) {
+      while (traverse::wide_step_with<STACK, false>(tlas, uint32_t(th.n_wide), th.n_prims, s, stack, nn, overflow,
+                                                    bad, prim)) {
       }
     }
   }

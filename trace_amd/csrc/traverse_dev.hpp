@@ -272,19 +272,36 @@ __device__ inline bool wide_begin(const float* __restrict__ tris, const BvhHeade
   return true;
 }
 
-// One wide-node visit.  All children's boxes are decoded relative to the ray
-// origin and slab-tested with the near / far planes picked by the direction's
-// signs; hit leaves are tested at
-// once, then the surviving inner children are ordered by entry distance: the
-// nearest is visited next and the others are pushed farthest first.  Returns
-// false when the ray is done.
-// n_bound = the tree's wide-node count, n_prims = its triangle count:
+// One wide-node visit, in three parts:
+//   wide_visit   -- all children's boxes are decoded relative to the ray
+//                   origin and slab-tested with the near / far planes picked
+//                   by the direction's signs; the hit leaf and inner children;
+//   leaf tests   -- the primitives of the hit leaves (they can only shorten
+//                   the ray, so they run before the inner children are
+//                   ordered);
+//   wide_advance -- the inner children still in reach are ordered by entry
+//                   distance: the nearest is visited next and the others are
+//                   pushed farthest first.  Returns false when the ray is done.
+// n_bound = the tree's wide-node count, n_prims = its primitive count:
 // bounds for every reference followed.
-// leaf(first, count) tests the primitives of a hit leaf against s.h.
-template <int STACK, bool COUNT, class Leaf>
-__device__ inline bool wide_step_with(const WideNode* __restrict__ wnodes, uint32_t n_bound, WideState& s,
-                                      TravStack<STACK>& stack, int32_t& n_nodes, bool& overflow, bool& bad,
-                                      Leaf&& leaf) {
+struct NodeVisit {
+  float tk[4];      // entry distance per child
+  int32_t ref[4];   // child reference: wide node, or first primitive of a leaf
+  uint32_t cnt[2];  // leaf sizes, child c in bits 16 (c & 1) .. of cnt[c >> 1]
+  uint32_t leaves;  // hit leaf children (bit c)
+  uint32_t inner;   // hit inner children (bit c)
+  __device__ inline int32_t count(int k) const {
+    const uint32_t w = k < 2 ? cnt[0] : cnt[1];
+    return int32_t((k & 1) ? w >> 16 : w & 0xffffu);
+  }
+  __device__ inline int32_t first(int k) const {
+    return k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3];
+  }
+};
+
+template <bool COUNT>
+__device__ inline bool wide_visit(const WideNode* __restrict__ wnodes, uint32_t n_bound, WideState& s,
+                                  int32_t& n_nodes, bool& bad, NodeVisit& v) {
   if (uint32_t(s.node) >= n_bound || ++s.visits > n_bound) {
     bad = true;
     return false;
@@ -308,11 +325,15 @@ __device__ inline bool wide_step_with(const WideNode* __restrict__ wnodes, uint3
   const uint32_t nqx = px ? b.x : b.w, fqx = px ? b.w : b.x;
   const uint32_t nqy = py ? b.y : c.x, fqy = py ? c.x : b.y;
   const uint32_t nqz = pz ? b.z : c.y, fqz = pz ? c.y : b.z;
-  const int32_t ref[4] = {int32_t(c.z), int32_t(c.w), int32_t(d.x), int32_t(d.y)};
-  const uint32_t cnt[4] = {d.z & 0xffffu, d.z >> 16, d.w & 0xffffu, d.w >> 16};
+  v.ref[0] = int32_t(c.z);
+  v.ref[1] = int32_t(c.w);
+  v.ref[2] = int32_t(d.x);
+  v.ref[3] = int32_t(d.y);
+  v.cnt[0] = d.z;
+  v.cnt[1] = d.w;
   const float tlim = s.h.t;
-  float tk[4];
-  bool hk[4];
+  v.leaves = 0u;
+  v.inner = 0u;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int sh = 8 * k;
@@ -324,33 +345,27 @@ __device__ inline bool wide_step_with(const WideNode* __restrict__ wnodes, uint3
     const float fz = fmaf(float((fqz >> sh) & 0xffu), sz, rz) * r.iz;
     const float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
     const float tf = fminf(fminf(fx, fy), fminf(fz, tlim));
-    tk[k] = tn;
-    hk[k] = ((used >> k) & 1u) && tn <= tf * kSlabSlack;
-  }
-  // leaves first (they can only shorten the ray), in one loop over the hit ones
-  uint32_t leaves = 0u;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (hk[k] && cnt[k] != 0u) {
-      leaves |= 1u << k;
-      hk[k] = false;
+    v.tk[k] = tn;
+    const bool hit = ((used >> k) & 1u) && tn <= tf * kSlabSlack;
+    const bool leaf = ((k < 2 ? d.z : d.w) >> (16 * (k & 1))) & 0xffffu;
+    if (hit) {
+      if (leaf) v.leaves |= 1u << k;
+      else v.inner |= 1u << k;
     }
   }
-  while (leaves) {
-    const int k = __builtin_ctz(leaves);
-    leaves &= leaves - 1u;
-    const int32_t first = k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3];
-    const int32_t count = int32_t(k == 0 ? cnt[0] : k == 1 ? cnt[1] : k == 2 ? cnt[2] : cnt[3]);
-    leaf(first, count);
-  }
+  return true;
+}
+
+template <int STACK>
+__device__ inline bool wide_advance(WideState& s, TravStack<STACK>& stack, bool& overflow, const NodeVisit& v) {
   // inner children still in reach, sorted by entry distance
   float key[4];
   int32_t id[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const bool live = hk[k] && tk[k] <= s.h.t * kSlabSlack;
-    key[k] = live ? tk[k] : INFINITY;
-    id[k] = live ? ref[k] : -1;
+    const bool live = ((v.inner >> k) & 1u) && v.tk[k] <= s.h.t * kSlabSlack;
+    key[k] = live ? v.tk[k] : INFINITY;
+    id[k] = live ? v.ref[k] : -1;
   }
 #define TRACE_CSWAP(i, j)                      \
   if (key[j] < key[i]) {                       \
@@ -383,14 +398,52 @@ __device__ inline bool wide_step_with(const WideNode* __restrict__ wnodes, uint3
   return true;
 }
 
+// The primitives of a lane's hit leaves back to back in one loop, so that the
+// wave iterates as often as its busiest lane has primitives, not once per leaf
+// slot times the largest leaf in that slot.  prim(k) tests primitive k.
+template <class Prim>
+__device__ inline void leaf_prims(const NodeVisit& v, int32_t n_prims, bool& bad, Prim&& prim) {
+  uint32_t leaves = v.leaves;
+  int32_t cur = 0, end = 0;
+  for (;;) {
+    if (cur == end && leaves) {
+      const int k = __builtin_ctz(leaves);
+      leaves &= leaves - 1u;
+      const int32_t first = v.first(k), count = v.count(k);
+      if (first < 0 || count > n_prims - first) {  // a malformed leaf is reported, not read
+        bad = true;
+        leaves = 0u;
+      } else {
+        cur = first;
+        end = first + count;
+      }
+    }
+    if (cur == end) break;
+    prim(cur);
+    ++cur;
+  }
+}
+
+// A whole visit with the leaf tests in the visiting lane.
+template <int STACK, bool COUNT, class Prim>
+__device__ inline bool wide_step_with(const WideNode* __restrict__ wnodes, uint32_t n_bound, int32_t n_prims,
+                                      WideState& s, TravStack<STACK>& stack, int32_t& n_nodes, bool& overflow,
+                                      bool& bad, Prim&& prim) {
+  NodeVisit v;
+  if (!wide_visit<COUNT>(wnodes, n_bound, s, n_nodes, bad, v)) return false;
+  leaf_prims(v, n_prims, bad, prim);
+  return wide_advance<STACK>(s, stack, overflow, v);
+}
+
 template <int STACK, bool COUNT, bool SPH>
 __device__ inline bool wide_step(const WideNode* __restrict__ wnodes,
                                  const float* __restrict__ tris, uint32_t n_bound, int32_t n_prims,
                                  WideState& s, TravStack<STACK>& stack, int32_t& n_nodes,
                                  int32_t& n_tris, bool& overflow, bool& bad) {
-  return wide_step_with<STACK, COUNT>(wnodes, n_bound, s, stack, n_nodes, overflow, bad,
-                                      [&](int32_t first, int32_t count) {
-                                        leaf_test<COUNT, SPH>(s.r, tris, first, count, n_prims, s.h, n_tris, bad);
+  return wide_step_with<STACK, COUNT>(wnodes, n_bound, n_prims, s, stack, n_nodes, overflow, bad,
+                                      [&](int32_t k) {
+                                        if (COUNT) ++n_tris;
+                                        prim_test<SPH>(s.r, tris, k, s.h);
                                       });
 }
 
