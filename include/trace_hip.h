@@ -172,13 +172,13 @@ int trace_phr_build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double d
  * the reference builds a new tree per BuildFromAuxilary call (phr.go:47-84), which a
  * caller that rebuilds per frame or per optimiser candidate replaces with this.  On an
  * error the tree's contents are undefined until a rebuild succeeds.
- * A binary rebuild (branching 2) sizes the tree for any input of the same triangle
- * count, and may then be recorded in a graph (trace_graph_begin): a rebuild issued
+ * A binary rebuild (branching 2) sizes the tree for inputs of the same triangle count,
+ * and may then be recorded in a graph (trace_graph_begin): a rebuild issued
  * while capturing takes the same alpha, delta and branching 2 as the eager one before
  * it (TRACE_E_INVALID otherwise) and records, with no host round trip, the eager
- * rebuild's job levels plus 4.  A replay whose input needs more levels, makes a
- * deeper tree than the eager one's depth class (63 or 255), or has an initial cut of
- * one node sets the device error word (reported by the next trace_ctx_synchronize)
+ * rebuild's job levels plus 4.  A replay whose input needs more levels or more than an
+ * eighth more branches, makes a deeper tree than the eager one's depth class (63 or
+ * 255), or has an initial cut of one node sets the device error word (reported by the next trace_ctx_synchronize)
  * and leaves the previous tree in place.  Build statistics (initial cut, build cost)
  * stay the eager rebuild's; depth and levels (trace_bvh_get_info) and the export
  * follow the replays. */

@@ -7,7 +7,9 @@ import numpy as np
 import pytest
 
 from trace_amd import _lib, scenes
-from trace_amd.device import Bvh, Mesh
+from trace_amd.device import Bvh, Graph, Mesh
+
+torch = pytest.importorskip("torch")
 
 pytestmark = pytest.mark.gpu
 
@@ -289,10 +291,6 @@ def test_phr_rebuild_in_a_graph(ctx, oracle):
     hits the same calls give eagerly, frame by frame; the last replayed frame's
     PHR tree is the oracle's for the exported positions, and the header's levels
     and output nodes follow the replays."""
-    import torch
-
-    from trace_amd.device import Graph
-
     print("phr rebuild in a graph", flush=True)
     ntri, frames, amp, period, aseed = 20_000, 4, 0.02, 30.0, 9
     pos = scenes.displaced_sphere(ntri, seed=71, radius=0.5)
@@ -364,10 +362,6 @@ def test_phr_rebuild_graph_out_of_bounds(ctx):
     rebuild answers by attaching the LBVH as is -- reports TRACE_ERR_BOUND at the
     next synchronisation and leaves the previous frame's tree in place; the next
     frame inside the bounds replays normally."""
-    import torch
-
-    from trace_amd.device import Graph
-
     print("phr rebuild graph out of bounds", flush=True)
     n = 5000
     pos = scenes.random_triangles(n, seed=75)

@@ -2362,6 +2362,15 @@ __global__ void levels_check_kernel(Ctl* ctl, uint32_t* d_err) {
   }
 }
 
+// After the scan of the branches: more than the tree's node array holds is
+// out of the recorded bounds (the layout then leaves the previous tree).
+__global__ void node_cap_check_kernel(const uint32_t* __restrict__ n_trav, uint32_t cap, Ctl* ctl, uint32_t* d_err) {
+  if (*n_trav > cap) {
+    ctl->stop = 1u;
+    atomicOr(d_err, TRACE_ERR_BOUND);
+  }
+}
+
 // After the layout: the traversal kernels and the widening pass were recorded
 // for trees of at most max_depth binary levels.
 __global__ void layout_check_kernel(const Ctl* __restrict__ ctl, int32_t max_depth, uint32_t* d_err) {
@@ -2480,8 +2489,10 @@ static int layout(trace_ctx* ctx, const trace_bvh* aux, int64_t n, int branching
     HIP_TRY(ctx, hipMemcpyAsync(d_ntrav, &h_ntrav, 4, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipStreamSynchronize(st));
   }
-  // fixed capacities: the most branches a binary tree of n triangles holds
-  HIP_TRY(ctx, out->nodes.alloc(std::max<int64_t>({int64_t(1), int64_t(h_ntrav), fixed_caps ? n - 1 : 0})));
+  // fixed capacities: this tree's branches and an eighth more, for the frames
+  // a captured rebuild replays (the 4-wide conversion runs over the capacity)
+  HIP_TRY(ctx, out->nodes.alloc(std::max<int64_t>(
+                   {int64_t(1), int64_t(h_ntrav), fixed_caps ? int64_t(h_ntrav) + int64_t(h_ntrav) / 8 + 64 : 0})));
   HIP_TRY(ctx, out->perm.alloc(n));
   HIP_TRY(ctx, out->phr_src.alloc(n));
   HIP_TRY(ctx, out->tris.alloc(9 * n));
@@ -2528,7 +2539,10 @@ static int layout_graph(trace_ctx* ctx, const trace_bvh* aux, int64_t n, const F
   branch_flag_kernel<<<gb, 256, 0, st>>>(outn, out_cap, d_out, flag);
   count_leaves_kernel<<<gb, 256, 0, st>>>(outn, out_cap, d_out, ctl);
   scan::exclusive_u32(st, flag, tidx, out_cap, n_trav, sscr);
-  const int32_t scap = int32_t(std::max<int64_t>(1, n - 1));
+  // branches beyond the node array's capacity stop the layout
+  const int64_t node_cap = std::max<int64_t>(1, std::min<int64_t>(int64_t(out->nodes.n), INT32_MAX));
+  node_cap_check_kernel<<<1, 1, 0, st>>>(n_trav, uint32_t(node_cap), ctl, ctx->d_err);
+  const int32_t scap = int32_t(std::min<int64_t>(node_cap, std::max<int64_t>(1, n - 1)));
   // output levels stay below the recorded job levels + 1
   int prim_bits = 1, level_bits = 1;
   while ((int64_t(1) << prim_bits) < n) ++prim_bits;
@@ -2563,9 +2577,10 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   const int64_t n = aux->n_prims;
   hipStream_t st = ctx->stream;
   const bool graph = mode == kGraphRebuild;
-  // a binary rebuild sizes its arrays and scratch for any frame of n
-  // triangles (n distinct keys, n - 1 branches), so that the same rebuild
-  // recorded in a graph later finds everything in place
+  // a binary rebuild sizes its scratch for any frame of n triangles (n
+  // distinct keys) and its node array for an eighth more branches than it
+  // makes, so that the same rebuild recorded in a graph later finds
+  // everything in place
   const bool fixed_caps = mode != kFirstBuild && branching == 2;
   if (graph && (branching != 2 || out->phr_clone || !out->phr_pw_valid || !out->phr_pw.fits(kMaxDepth)))
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: a captured rebuild needs an eager binary rebuild first");
@@ -2673,7 +2688,7 @@ int build(trace_ctx* ctx, const trace_bvh* aux, double alpha, double delta, int 
   Group* gbuf[2] = {reinterpret_cast<Group*>(base + o_ga), reinterpret_cast<Group*>(base + o_gb)};
 
   if (graph) {
-    if (!out->phr_out.fits(size_t(out_cap) * sizeof(OutNode)) || !out->nodes.fits(size_t(std::max<int64_t>(1, n - 1))))
+    if (!out->phr_out.fits(size_t(out_cap) * sizeof(OutNode)) || !out->nodes.fits(1))
       TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_phr_rebuild: a captured rebuild needs an eager binary rebuild first");
   } else {
     HIP_TRY(ctx, out->phr_out.alloc(size_t(out_cap) * sizeof(OutNode)));

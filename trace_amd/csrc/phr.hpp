@@ -26,13 +26,15 @@ static_assert(sizeof(OutNode) == 64, "OutNode is 64 bytes");
 // How build() runs:
 //  kFirstBuild   -- a new tree, sized to what it holds (synchronous);
 //  kRebuild      -- into an existing tree (synchronous); a binary rebuild sizes
-//                   its buffers and scratch for any frame of the same triangle
-//                   count and keeps the threshold powers on the device;
+//                   its scratch for any frame of the same triangle count, its
+//                   node array for an eighth more branches than it makes, and
+//                   keeps the threshold powers on the device;
 //  kGraphRebuild -- a binary rebuild while the context stream is captured: no
 //                   host reads, every launch sized by capacities, the job
 //                   levels of the last eager rebuild plus kGraphSpareLevels
-//                   recorded; a replay whose frame needs more, or a deeper
-//                   tree, or an initial cut of one node, sets TRACE_ERR_BOUND.
+//                   recorded; a replay whose frame needs more levels or
+//                   branches, or makes a deeper tree, or has an initial cut of
+//                   one node, sets TRACE_ERR_BOUND.
 enum Mode { kFirstBuild = 0, kRebuild = 1, kGraphRebuild = 2 };
 constexpr int kGraphSpareLevels = 4;
 
