@@ -716,6 +716,7 @@ def main():
     tune = get_tune(ctx)
     trav_threads = int(tune.trav_threads)
     trav_persistent = bool(tune.trav_persistent)
+    trav_order = trav_persistent and bool(tune.trav_order)
     if args.pmc_probe:
         pmc_probe(torch, ctx, phr, d_rays, n, d_hit, d_t)
         ctx.close()
@@ -763,16 +764,19 @@ def main():
             "traffic": load_traffic(),
             # the persistent traversal hands its winners to a short FP64 re-solve pass;
             # kernel_ms brackets both (HIP events around the call)
-            "kernel": "closest_hit_persistent + refine_kernel" if trav_persistent else "closest_hit_kernel",
+            "kernel": ("cell_count + cell_scan + cell_scatter + " if trav_order else "")
+                      + ("closest_hit_persistent + refine_kernel" if trav_persistent else "closest_hit_kernel"),
             "kernel_ms": round(launch_ms, 4),
             "bytes_per_ray": round(bytes_launch / n, 1),
             "inner_nodes_per_ray": round(float(nodes.mean()), 2),
             "tris_per_ray": round(float(tris.mean()), 2),
         },
         "hit_fraction": round(float((hit >= 0).mean()), 4),
-        "autotune": {"trav_threads": trav_threads, "trav_persistent": trav_persistent,
+        "autotune": {"trav_threads": trav_threads, "trav_persistent": trav_persistent, "trav_order": trav_order,
                      "ms_one_ray_per_lane": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms[:3])},
-                     "ms_persistent": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms[3:])}},
+                     "ms_persistent": {str(t): round(float(m), 4) for t, m in zip((64, 128, 256), tune_ms[3:6])},
+                     "ms_persistent_origin_order": {str(t): round(float(m), 4)
+                                                    for t, m in zip((64, 128, 256), tune_ms[6:9])}},
         "phr_build_ms": round(phr_ms, 2),
         "phr_rebuild_ms": round(phr_rebuild_ms, 2),
     }

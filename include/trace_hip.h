@@ -471,13 +471,16 @@ typedef struct {
   int32_t sort_keys_per_thread; /* LBVH key sort: keys per thread, 4, 8 or 16 (tiles of 256x);
                                    valid pairs: 8-bit digits with 4, 8 or 16; 10-bit with 4 or
                                    8; 11-bit with 4 */
-  int32_t pad;
+  int32_t trav_order; /* persistent closest hit: 1 = batches of 2^18 rays or more are traced
+                         grouped by origin (512 cells of the tree's root box, one counting
+                         pass; results still at each ray's index), 0 = in input order */
 } trace_tune;
 int trace_get_tune(trace_ctx* ctx, trace_tune* out);
 int trace_set_tune(trace_ctx* ctx, const trace_tune* tune);
 /* Times closest hit on d_rays (device, n rays) for each workgroup size (64, 128, 256)
- * with one ray per lane and then with persistent lanes, keeps the fastest in the
- * context, and reports the per-candidate times in ms (6 entries, in that order). */
+ * with one ray per lane, then with persistent lanes in input order, then with
+ * persistent lanes and rays grouped by origin, keeps the fastest in the context, and
+ * reports the per-candidate times in ms (9 entries, in that order). */
 int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* d_rays,
                              int64_t n, float* h_ms);
 /* Times a render for each workgroup size, wavefront batch size (candidate batches

@@ -291,13 +291,14 @@ def test_autotune_keeps_results(ctx):
     rays = torch.from_numpy(rays_h).cuda()
     torch.cuda.synchronize()
     ms = autotune_traversal(ctx, bvh, rays.data_ptr(), len(rays))
-    assert ms.shape == (6,) and np.all(ms > 0)
+    assert ms.shape == (9,) and np.all(ms > 0)
     t = get_tune(ctx)
     assert t.tuned == 1 and t.trav_threads in (64, 128, 256) and t.trav_persistent in (0, 1)
+    assert t.trav_order in (0, 1) and (t.trav_persistent or not t.trav_order)
     ref = None
-    for persistent in (False, True):
+    for persistent, order in ((False, False), (True, False), (True, True)):
         for threads in (64, 128, 256):
-            set_tune(ctx, trav_threads=threads, trav_persistent=persistent)
+            set_tune(ctx, trav_threads=threads, trav_persistent=persistent, trav_order=order)
             hit, tt = bvh.closest_hit(rays_h)
             if ref is None:
                 ref = (hit, tt)

@@ -105,6 +105,40 @@ def test_ray_counts_around_the_claim_size(ctx, oracle, threads):
         set_tune(ctx)
 
 
+def test_origin_order_matches_input_order(ctx, oracle):
+    # Batches of 2^18 rays or more are traced grouped by origin cell (512 cells
+    # of the root box, filled 4096 rays per workgroup).  Results are each ray's
+    # own, bit for bit the input-order trace's: on the LBVH and the PHR tree, for
+    # a ragged batch with origins far outside the root box and on its faces, and
+    # for prefixes just below, at and above the size where grouping starts.
+    print("traverse origin order vs input order", flush=True)
+    pos = scenes.random_triangles(100_000, seed=41, mean_edge=0.01)
+    rays = scenes.incoherent_rays(300_001, seed=42)
+    rays[:2000, 0:3] = rays[:2000, 0:3] * 20.0 - 10.0
+    lo, hi = pos.reshape(-1, 3).min(0), pos.reshape(-1, 3).max(0)
+    rays[2000:2100, 0:3] = hi.astype(np.float32)
+    rays[2100:2200, 0:3] = lo.astype(np.float32)
+    mesh = Mesh(ctx, pos)
+    lbvh = Bvh.lbvh(ctx, mesh)
+    phr = Bvh.phr(ctx, lbvh)
+    olbvh = oracle.Tree(pos)
+    trees = ((lbvh, olbvh), (phr, oracle.PhrTree(olbvh)))
+    try:
+        for bvh, otree in trees:
+            set_tune(ctx, trav_threads=128, trav_persistent=True, trav_order=False)
+            ref_hit, ref_t = bvh.closest_hit(rays)
+            set_tune(ctx, trav_threads=128, trav_persistent=True, trav_order=True)
+            hit, t = bvh.closest_hit(rays)
+            assert np.array_equal(hit, ref_hit) and np.array_equal(t, ref_t)
+            ohit, ot = otree.closest_hit(rays)
+            _check(hit, t, ohit, ot, min_hits=0.5)
+            for n in (2**18 - 1, 2**18, 2**18 + 1, 2**18 + 4095):
+                h, tt = bvh.closest_hit(rays[:n])
+                assert np.array_equal(h, ref_hit[:n]) and np.array_equal(tt, ref_t[:n]), n
+    finally:
+        set_tune(ctx)
+
+
 def test_empty_ray_batch(ctx):
     pos = scenes.random_triangles(10, seed=1)
     bvh = Bvh.lbvh(ctx, Mesh(ctx, pos))

@@ -136,7 +136,7 @@ class Tune(C.Structure):
         ("render_persistent", C.c_int32),
         ("sort_bits", C.c_int32),
         ("sort_keys_per_thread", C.c_int32),
-        ("pad", C.c_int32),
+        ("trav_order", C.c_int32),
     ]
 
 

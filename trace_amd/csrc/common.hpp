@@ -24,6 +24,7 @@ struct TuneState {
   // batches keep the late bounces, where few paths remain, filling the GPU.
   int64_t render_batch = int64_t(1) << 27;
   bool trav_persistent = true;          // closest hit: persistent lanes refilled from a counter
+  bool trav_order = true;               // persistent closest hit: rays grouped by origin cell first
   bool render_persistent = false;       // path segments: the same for the render's queues
   int sort_bits = 8;                    // LBVH key sort: radix digit width (8, 10 or 11)
   int sort_kpt = 16;                    // LBVH key sort: keys per thread (tiles of 256 * kpt)

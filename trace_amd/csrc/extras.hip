@@ -305,6 +305,7 @@ int trace_get_tune(trace_ctx* ctx, trace_tune* out) {
   out->render_persistent = ctx->tune.render_persistent ? 1 : 0;
   out->sort_bits = ctx->tune.sort_bits;
   out->sort_keys_per_thread = ctx->tune.sort_kpt;
+  out->trav_order = ctx->tune.trav_order ? 1 : 0;
   return TRACE_OK;
 }
 
@@ -324,6 +325,7 @@ int trace_set_tune(trace_ctx* ctx, const trace_tune* t) {
   ctx->tune.render_persistent = t->render_persistent != 0;
   ctx->tune.sort_bits = t->sort_bits;
   ctx->tune.sort_kpt = t->sort_keys_per_thread;
+  ctx->tune.trav_order = t->trav_order != 0;
   return TRACE_OK;
 }
 
@@ -367,25 +369,34 @@ int trace_autotune_traversal(trace_ctx* ctx, const trace_bvh* bvh, const float* 
   float* d_t = reinterpret_cast<float*>(out.p + size_t(n) * 4);
   const int cand[3] = {64, 128, 256};
   int best = ctx->tune.trav_threads;
-  bool best_p = ctx->tune.trav_persistent;
+  bool best_p = ctx->tune.trav_persistent, best_o = ctx->tune.trav_order;
   float best_ms = INFINITY;
-  for (int m = 0; m < 2; ++m) {
+  // modes: one ray per lane; persistent lanes in input order; persistent lanes, rays grouped by origin
+  for (int m = 0; m < 3; ++m) {
     for (int c = 0; c < 3; ++c) {
       ctx->tune.trav_threads = cand[c];
-      ctx->tune.trav_persistent = m == 1;
+      ctx->tune.trav_persistent = m >= 1;
+      ctx->tune.trav_order = m == 2;
       float ms = 0;
       int rc = time_ms(ctx, [&] { return traverse::closest_hit(ctx, bvh, d_rays, n, d_hit, d_t, nullptr, nullptr); }, 3, &ms);
-      if (rc) return rc;
+      if (rc) {
+        ctx->tune.trav_threads = best;
+        ctx->tune.trav_persistent = best_p;
+        ctx->tune.trav_order = best_o;
+        return rc;
+      }
       if (h_ms) h_ms[3 * m + c] = ms;
       if (ms < best_ms) {
         best_ms = ms;
         best = cand[c];
-        best_p = m == 1;
+        best_p = m >= 1;
+        best_o = m == 2;
       }
     }
   }
   ctx->tune.trav_threads = best;
   ctx->tune.trav_persistent = best_p;
+  ctx->tune.trav_order = best_o;
   ctx->tune.tuned = true;
   return ctx_check_device_error(ctx, "trace_autotune_traversal");
 }

@@ -601,12 +601,13 @@ def get_tune(ctx: Context) -> "_lib.Tune":
 
 def set_tune(ctx: Context, trav_threads: int = 64, render_threads: int = 64, render_batch: int = 1 << 27,
              trav_persistent: bool = True, render_persistent: bool = False, sort_bits: int = 8,
-             sort_keys_per_thread: int = 16) -> None:
+             sort_keys_per_thread: int = 16, trav_order: bool = True) -> None:
     t = _lib.Tune()
     t.trav_threads, t.render_threads, t.render_batch = trav_threads, render_threads, render_batch
     t.trav_persistent = 1 if trav_persistent else 0
     t.render_persistent = 1 if render_persistent else 0
     t.sort_bits, t.sort_keys_per_thread = sort_bits, sort_keys_per_thread
+    t.trav_order = 1 if trav_order else 0
     ctx.check(ctx.lib.trace_set_tune(ctx.h, C.byref(t)))
 
 
@@ -622,8 +623,9 @@ def autotune_build(ctx: Context, mesh: Mesh, key_mode: int = 0) -> np.ndarray:
 
 
 def autotune_traversal(ctx: Context, bvh: Bvh, d_rays: int, n: int) -> np.ndarray:
-    """Times (64, 128, 256 threads) x (one ray per lane, persistent); keeps the best."""
-    ms = np.zeros(6, np.float32)
+    """Times (64, 128, 256 threads) x (one ray per lane, persistent in input order,
+    persistent with rays grouped by origin); keeps the best."""
+    ms = np.zeros(9, np.float32)
     ctx.check(ctx.lib.trace_autotune_traversal(ctx.h, bvh.h, C.c_void_p(d_rays), n, _ptr(ms)))
     return ms
 
