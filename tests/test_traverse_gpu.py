@@ -106,11 +106,13 @@ def test_ray_counts_around_the_claim_size(ctx, oracle, threads):
 
 
 def test_origin_order_matches_input_order(ctx, oracle):
-    # Batches of 2^18 rays or more are traced grouped by origin cell (512 cells
-    # of the root box, filled 4096 rays per workgroup).  Results are each ray's
-    # own, bit for bit the input-order trace's: on the LBVH and the PHR tree, for
-    # a ragged batch with origins far outside the root box and on its faces, and
-    # for prefixes just below, at and above the size where grouping starts.
+    # Batches of 2^18 rays or more are traced grouped by origin cell (64 cells
+    # of the root box, copied 1024 rays per workgroup), unless one cell holds
+    # more than half of them (a camera's rays: traced in input order).  Results
+    # are each ray's own, bit for bit the input-order trace's: on the LBVH and
+    # the PHR tree, for a ragged batch with origins far outside the root box and
+    # on its faces, for prefixes just below, at and above the size where
+    # grouping starts, for a camera batch and for a batch that is 60 % one cell.
     print("traverse origin order vs input order", flush=True)
     pos = scenes.random_triangles(100_000, seed=41, mean_edge=0.01)
     rays = scenes.incoherent_rays(300_001, seed=42)
@@ -123,6 +125,7 @@ def test_origin_order_matches_input_order(ctx, oracle):
     phr = Bvh.phr(ctx, lbvh)
     olbvh = oracle.Tree(pos)
     trees = ((lbvh, olbvh), (phr, oracle.PhrTree(olbvh)))
+    cam = scenes.camera_rays(scenes.Camera(4 / 3, 50.0, (0.5, 0.5, 2.5), (0.5, 0.5, 0.5), (0, 1, 0)), 640, 480)
     try:
         for bvh, otree in trees:
             set_tune(ctx, trav_threads=128, trav_persistent=True, trav_order=False)
@@ -135,6 +138,16 @@ def test_origin_order_matches_input_order(ctx, oracle):
             for n in (2**18 - 1, 2**18, 2**18 + 1, 2**18 + 4095):
                 h, tt = bvh.closest_hit(rays[:n])
                 assert np.array_equal(h, ref_hit[:n]) and np.array_equal(tt, ref_t[:n]), n
+            crowded = rays.copy()
+            crowded[: int(0.6 * len(rays)), 0:3] = np.float32(0.05)
+            for batch in (cam, crowded):
+                set_tune(ctx, trav_threads=128, trav_persistent=True, trav_order=False)
+                ref_hit, ref_t = bvh.closest_hit(batch)
+                set_tune(ctx, trav_threads=128, trav_persistent=True, trav_order=True)
+                hit, t = bvh.closest_hit(batch)
+                assert np.array_equal(hit, ref_hit) and np.array_equal(t, ref_t)
+                ohit, ot = otree.closest_hit(batch)
+                _check(hit, t, ohit, ot, min_hits=0.05)
     finally:
         set_tune(ctx)
 

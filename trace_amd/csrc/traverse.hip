@@ -62,8 +62,14 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(COUNT |
     const int32_t* __restrict__ perm, const BvhHeader* __restrict__ hdr_p,
     const float4* __restrict__ rays, int64_t n, int32_t id_offset, int32_t* __restrict__ hit_out,
     float* __restrict__ t_out, int32_t* __restrict__ cnt_nodes, int32_t* __restrict__ cnt_tris,
-    uint32_t* d_err, uint32_t* work) {
+    uint32_t* d_err, uint32_t* work, const uint32_t* __restrict__ ordered, const float4* __restrict__ orays,
+    int32_t* __restrict__ ohit, float* __restrict__ ot) {
   __shared__ int32_t s_stack[kLdsStack * THREADS];
+  if (ordered && *ordered) {  // trace the cell-major copy (ray_order) into its own result arrays
+    rays = orays;
+    hit_out = ohit;
+    t_out = ot;
+  }
   const BvhHeader hdr = *hdr_p;
   const uint32_t n_bound = uint32_t(hdr.n_wide);
   const int lane = __lane_id();
@@ -141,31 +147,35 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(COUNT |
 
 // After closest_hit_persistent: each winner re-solved in FP64
 // (refine_hit_fp64) and its sorted position mapped to the caller's index.
+// ordered (optional): set when the rays were traced as a cell-major copy,
+// whose re-solve is refine_traced_kernel's.
 __global__ __launch_bounds__(256) void refine_kernel(const float* __restrict__ tris, const int32_t* __restrict__ perm,
                                                      int32_t n_prims, const float4* __restrict__ rays, int64_t n,
                                                      int32_t id_offset, int32_t* __restrict__ hit_out,
-                                                     float* __restrict__ t_out, uint32_t* d_err) {
-  const int64_t i = blockIdx.x * int64_t(256) + threadIdx.x;
-  if (i >= n) return;
-  const int32_t pos = hit_out[i];
-  if (pos < 0) return;  // a miss: -1 and the ray's tmax are final
-  if (pos >= n_prims) {
-    atomicOr(d_err, TRACE_ERR_TREE);
-    hit_out[i] = -1;
-    return;
+                                                     float* __restrict__ t_out, uint32_t* d_err,
+                                                     const uint32_t* __restrict__ ordered) {
+  if (ordered && *ordered) return;
+  for (int64_t i = blockIdx.x * int64_t(256) + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+    const int32_t pos = hit_out[i];
+    if (pos < 0) continue;  // a miss: -1 and the ray's tmax are final
+    if (pos >= n_prims) {
+      atomicOr(d_err, TRACE_ERR_TREE);
+      hit_out[i] = -1;
+      continue;
+    }
+    const float4 a = rays[2 * i];
+    const float4 b = rays[2 * i + 1];
+    const Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    Hit h;
+    h.pos = pos;
+    h.t = t_out[i];
+    h.t64 = h.t;
+    h.u = h.v = 0.0f;
+    h.front = true;
+    refine_hit_fp64(tris, ray, h);
+    hit_out[i] = perm[pos] - id_offset;
+    t_out[i] = h.t;
   }
-  const float4 a = rays[2 * i];
-  const float4 b = rays[2 * i + 1];
-  const Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-  Hit h;
-  h.pos = pos;
-  h.t = t_out[i];
-  h.t64 = h.t;
-  h.u = h.v = 0.0f;
-  h.front = true;
-  refine_hit_fp64(tris, ray, h);
-  hit_out[i] = perm[pos] - id_offset;
-  t_out[i] = h.t;
 }
 
 // ---- ray order ---------------------------------------------------------------
@@ -195,15 +205,15 @@ constexpr int kOrderPerThread = 4;
 constexpr int kOrderTile = kOrderThreads * kOrderPerThread;  // rays per workgroup
 constexpr int64_t kOrderMin = int64_t(1) << 18;             // smaller batches are traced in input order
 
+// box: the root box's low corner, then cells per unit length on each axis
+// (load_root_box).
 __device__ inline uint32_t origin_cell(const float* box, float4 o) {
   const float oc[3] = {o.x, o.y, o.z};
   uint32_t key = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float ext = box[3 + a] - box[a];
-    const float s = ext > 0.0f ? float(1 << kCellBits) / ext : 0.0f;
     // fmaxf/fminf return the non-NaN operand: a NaN coordinate lands in cell 0
-    const uint32_t q = uint32_t(fminf(fmaxf((oc[a] - box[a]) * s, 0.0f), float((1 << kCellBits) - 1)));
+    const uint32_t q = uint32_t(fminf(fmaxf((oc[a] - box[a]) * box[3 + a], 0.0f), float((1 << kCellBits) - 1)));
 #pragma unroll
     for (int b = 0; b < kCellBits; ++b) key |= ((q >> b) & 1u) << (3 * b + a);
   }
@@ -212,16 +222,45 @@ __device__ inline uint32_t origin_cell(const float* box, float4 o) {
 
 __device__ inline void load_root_box(const BvhHeader* __restrict__ hdr, float* box) {
   if (threadIdx.x < 3) {
-    box[threadIdx.x] = hdr->root_lo[threadIdx.x];
-    box[3 + threadIdx.x] = hdr->root_hi[threadIdx.x];
+    const float lo = hdr->root_lo[threadIdx.x], ext = hdr->root_hi[threadIdx.x] - lo;
+    box[threadIdx.x] = lo;
+    box[3 + threadIdx.x] = ext > 0.0f ? float(1 << kCellBits) / ext : 0.0f;
   }
+}
+
+// The copy pays only when the origins are spread out: *ordered = 0 (trace in
+// input order) when one cell holds more than half of an evenly strided sample
+// of kProbe rays, as a camera's rays, which share one origin, all do.  One
+// workgroup.
+constexpr int kProbe = 1024;
+
+__global__ __launch_bounds__(kOrderThreads) void cell_probe_kernel(const BvhHeader* __restrict__ hdr,
+                                                                   const float4* __restrict__ rays, int64_t n,
+                                                                   uint32_t* __restrict__ ordered) {
+  __shared__ uint32_t h[kCells];
+  __shared__ float box[6];
+  for (int c = threadIdx.x; c < kCells; c += kOrderThreads) h[c] = 0u;
+  load_root_box(hdr, box);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kProbe / kOrderThreads; ++k) {
+    const int64_t i = int64_t(k * kOrderThreads + threadIdx.x) * n / kProbe;
+    atomicAdd(&h[origin_cell(box, rays[2 * i])], 1u);
+  }
+  __syncthreads();
+  bool crowded = false;
+  for (int c = threadIdx.x; c < kCells; c += kOrderThreads) crowded |= 2 * h[c] > uint32_t(kProbe);
+  crowded = __syncthreads_or(crowded);
+  if (threadIdx.x == 0) *ordered = crowded ? 0u : 1u;
 }
 
 // counts[c * nblocks + b] = the rays of tile b in cell c (cell-major, so that
 // one exclusive scan gives every tile's run start in every cell).
 __global__ __launch_bounds__(kOrderThreads) void cell_count_kernel(const BvhHeader* __restrict__ hdr,
                                                                    const float4* __restrict__ rays, int64_t n,
-                                                                   int32_t nblocks, uint32_t* __restrict__ counts) {
+                                                                   int32_t nblocks, uint32_t* __restrict__ counts,
+                                                                   const uint32_t* __restrict__ ordered) {
+  if (!*ordered) return;
   __shared__ uint32_t h[kCells];
   __shared__ float box[6];
   for (int c = threadIdx.x; c < kCells; c += kOrderThreads) h[c] = 0u;
@@ -246,7 +285,9 @@ __global__ __launch_bounds__(kOrderThreads) void cell_scatter_kernel(const BvhHe
                                                                      int32_t nblocks,
                                                                      const uint32_t* __restrict__ offsets,
                                                                      float4* __restrict__ sorted,
-                                                                     uint32_t* __restrict__ inv) {
+                                                                     uint32_t* __restrict__ inv,
+                                                                     const uint32_t* __restrict__ ordered) {
+  if (!*ordered) return;
   __shared__ float4 sray[2 * kOrderTile];
   __shared__ uint8_t scell[kOrderTile];
   __shared__ uint32_t slot[kOrderPerThread * kOrderWaves][kCells];  // per (k, wave): count, then prefix
@@ -259,18 +300,12 @@ __global__ __launch_bounds__(kOrderThreads) void cell_scatter_kernel(const BvhHe
   __syncthreads();
   const int64_t base = int64_t(blockIdx.x) * kOrderTile;
   const uint64_t lt = lanemask_lt();
-  float4 ra[kOrderPerThread], rb[kOrderPerThread];
   uint32_t c[kOrderPerThread], r[kOrderPerThread];
 #pragma unroll
   for (int k = 0; k < kOrderPerThread; ++k) {
     const int64_t i = base + k * kOrderThreads + tid;
     const bool valid = i < n;
-    c[k] = 0u;
-    if (valid) {
-      ra[k] = rays[2 * i];
-      rb[k] = rays[2 * i + 1];
-      c[k] = origin_cell(box, ra[k]);
-    }
+    c[k] = valid ? origin_cell(box, rays[2 * i]) : 0u;
     // lanes of this wave with the same cell
     uint64_t m = ballot64(valid);
 #pragma unroll
@@ -311,8 +346,10 @@ __global__ __launch_bounds__(kOrderThreads) void cell_scatter_kernel(const BvhHe
     if (i < n) {
       const uint32_t within = slot[k * kOrderWaves + wave][c[k]] + r[k];
       const uint32_t p = cstart[c[k]] + within;
-      sray[2 * p] = ra[k];
-      sray[2 * p + 1] = rb[k];
+      // the ray is read again (its origin from L2): holding both halves of
+      // every ray across the barriers would take 32 more VGPRs
+      sray[2 * p] = rays[2 * i];
+      sray[2 * p + 1] = rays[2 * i + 1];
       scell[p] = uint8_t(c[k]);
       inv[i] = coff[c[k]] + within;
     }
@@ -335,42 +372,45 @@ __global__ __launch_bounds__(256) void refine_traced_kernel(const float* __restr
                                                             const float4* __restrict__ rays, int64_t n,
                                                             int32_t id_offset, const int32_t* __restrict__ hit,
                                                             const float* __restrict__ t, int2* __restrict__ res,
-                                                            uint32_t* d_err) {
-  const int64_t j = blockIdx.x * int64_t(256) + threadIdx.x;
-  if (j >= n) return;
-  const int32_t pos = hit[j];
-  if (pos < 0 || pos >= n_prims) {  // a miss: -1 and the ray's tmax are final
-    if (pos >= n_prims) atomicOr(d_err, TRACE_ERR_TREE);
-    res[j] = make_int2(-1, __float_as_int(t[j]));
-    return;
+                                                            uint32_t* d_err, const uint32_t* __restrict__ ordered) {
+  if (!*ordered) return;
+  for (int64_t j = blockIdx.x * int64_t(256) + threadIdx.x; j < n; j += int64_t(gridDim.x) * 256) {
+    const int32_t pos = hit[j];
+    if (pos < 0 || pos >= n_prims) {  // a miss: -1 and the ray's tmax are final
+      if (pos >= n_prims) atomicOr(d_err, TRACE_ERR_TREE);
+      res[j] = make_int2(-1, __float_as_int(t[j]));
+      continue;
+    }
+    const float4 a = rays[2 * j];
+    const float4 b = rays[2 * j + 1];
+    const Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    Hit h;
+    h.pos = pos;
+    h.t = t[j];
+    h.t64 = h.t;
+    h.u = h.v = 0.0f;
+    h.front = true;
+    refine_hit_fp64(tris, ray, h);
+    res[j] = make_int2(perm[pos] - id_offset, __float_as_int(h.t));
   }
-  const float4 a = rays[2 * j];
-  const float4 b = rays[2 * j + 1];
-  const Ray ray{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-  Hit h;
-  h.pos = pos;
-  h.t = t[j];
-  h.t64 = h.t;
-  h.u = h.v = 0.0f;
-  h.front = true;
-  refine_hit_fp64(tris, ray, h);
-  res[j] = make_int2(perm[pos] - id_offset, __float_as_int(h.t));
 }
 
 // Caller ray i's result from where it was traced (one 8-byte gather).
 __global__ __launch_bounds__(256) void unpermute_kernel(const uint32_t* __restrict__ inv, const int2* __restrict__ res,
                                                         int64_t n, int32_t* __restrict__ hit_out,
-                                                        float* __restrict__ t_out) {
-  const int64_t i = blockIdx.x * int64_t(256) + threadIdx.x;
-  if (i >= n) return;
-  const int2 r = res[inv[i]];
-  hit_out[i] = r.x;
-  t_out[i] = __int_as_float(r.y);
+                                                        float* __restrict__ t_out, const uint32_t* __restrict__ ordered) {
+  if (!*ordered) return;
+  for (int64_t i = blockIdx.x * int64_t(256) + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+    const int2 r = res[inv[i]];
+    hit_out[i] = r.x;
+    t_out[i] = __int_as_float(r.y);
+  }
 }
 
 // A cell-major batch in scratch: the rays, where each caller ray went, and room
 // for the traced order's hits, distances and re-solved results.
 struct RayOrder {
+  const uint32_t* ordered = nullptr;  // device word: 1 = the copy below is traced
   const float4* rays = nullptr;
   const uint32_t* inv = nullptr;
   int32_t* hit = nullptr;
@@ -390,16 +430,18 @@ static void order_plan(int64_t n, ScratchPlan& plan, size_t* o) {
   o[6] = plan.take(sizeof(uint32_t));                         // scan total
   o[7] = plan.take(scan::scratch_words(ncount) * sizeof(uint32_t));
   o[8] = plan.take(size_t(n) * sizeof(int2));                 // re-solved results
+  o[9] = plan.take(sizeof(uint32_t));                         // ordered flag
 }
 
-// Enqueues the cell-major copy of n rays into scratch; out->rays stays null
-// (trace in input order) for batches below kOrderMin and for calls that would
-// have to grow the scratch arena while graphs hold it.
+// Enqueues the cell-major copy of n rays into scratch (made or skipped on the
+// device, see cell_probe_kernel); out stays empty (trace in input order) for
+// batches below kOrderMin and for calls that would have to grow the scratch
+// arena while graphs hold it.
 static int ray_order(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, int64_t n, RayOrder* out) {
   *out = RayOrder{};
   if (!ctx->tune.trav_order || n < kOrderMin) return TRACE_OK;
   ScratchPlan plan;
-  size_t o[9];
+  size_t o[10];
   order_plan(n, plan, o);
   if (plan.off > ctx->scratch_bytes && (ctx->capturing || ctx->live_graphs > 0)) return TRACE_OK;
   const int rc = ctx_reserve_scratch(ctx, plan.off);
@@ -409,13 +451,17 @@ static int ray_order(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, i
   auto* inv = reinterpret_cast<uint32_t*>(base + o[1]);
   auto* counts = reinterpret_cast<uint32_t*>(base + o[4]);
   auto* offsets = reinterpret_cast<uint32_t*>(base + o[5]);
+  auto* total = reinterpret_cast<uint32_t*>(base + o[6]);
+  auto* ordered = reinterpret_cast<uint32_t*>(base + o[9]);
   const int32_t nblocks = int32_t(ceil_div(n, int64_t(kOrderTile)));
-  cell_count_kernel<<<nblocks, kOrderThreads, 0, ctx->stream>>>(bvh->header_p(), rays, n, nblocks, counts);
-  scan::exclusive_u32(ctx->stream, counts, offsets, int64_t(nblocks) * kCells, reinterpret_cast<uint32_t*>(base + o[6]),
+  cell_probe_kernel<<<1, kOrderThreads, 0, ctx->stream>>>(bvh->header_p(), rays, n, ordered);
+  cell_count_kernel<<<nblocks, kOrderThreads, 0, ctx->stream>>>(bvh->header_p(), rays, n, nblocks, counts, ordered);
+  // (with the copy skipped, this scans stale counts that nothing reads)
+  scan::exclusive_u32(ctx->stream, counts, offsets, int64_t(nblocks) * kCells, total,
                       reinterpret_cast<uint32_t*>(base + o[7]));
   cell_scatter_kernel<<<nblocks, kOrderThreads, 0, ctx->stream>>>(bvh->header_p(), rays, n, nblocks, offsets, sorted,
-                                                                  inv);
-  *out = RayOrder{sorted, inv, reinterpret_cast<int32_t*>(base + o[2]), reinterpret_cast<float*>(base + o[3]),
+                                                                  inv, ordered);
+  *out = RayOrder{ordered, sorted, inv, reinterpret_cast<int32_t*>(base + o[2]), reinterpret_cast<float*>(base + o[3]),
                   reinterpret_cast<int2*>(base + o[8])};
   return TRACE_OK;
 }
@@ -430,27 +476,26 @@ static int launch_kind(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays,
       const int rc = ray_order(ctx, bvh, rays, n, &ro);
       if (rc) return rc;
     }
-    const float4* trays = ro.rays ? ro.rays : rays;
-    int32_t* thit = ro.rays ? ro.hit : d_hit;
-    float* tt = ro.rays ? ro.t : d_t;
     // up to 32 resident waves per CU; workgroups beyond what fits start as
     // others leave and find the counter exhausted
     const int64_t blocks = std::min<int64_t>(ceil_div(n, THREADS), int64_t(ctx->n_cu) * 32 * 64 / THREADS);
     uint32_t* work = ctx->d_work;
     fill_async<uint32_t>(ctx->stream, work, 1, 0u);
+    // with a copy made, ro.ordered decides on the device which arrays are traced
     closest_hit_persistent<THREADS, STACK, COUNT, SPH><<<int(blocks), THREADS, 0, ctx->stream>>>(
-        bvh->wnodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), trays, n,
-        id_offset, thit, tt, d_nodes, d_tris, ctx->d_err, work);
+        bvh->wnodes_p(), bvh->tris_p(), bvh->perm_p(), bvh->header_p(), rays, n,
+        id_offset, d_hit, d_t, d_nodes, d_tris, ctx->d_err, work, ro.ordered, ro.rays, ro.hit, ro.t);
     // sorted positions index the whole triangle array (a forest's, for a segment view)
     const int64_t n_sorted = bvh->base ? bvh->base->n_prims : bvh->n_prims;
-    const int grid = int(ceil_div(n, 256));
-    if (ro.rays) {
-      refine_traced_kernel<<<grid, 256, 0, ctx->stream>>>(bvh->tris_p(), bvh->perm_p(), int32_t(n_sorted), trays, n,
-                                                          id_offset, thit, tt, ro.res, ctx->d_err);
-      unpermute_kernel<<<grid, 256, 0, ctx->stream>>>(ro.inv, ro.res, n, d_hit, d_t);
-    } else {
-      refine_kernel<<<grid, 256, 0, ctx->stream>>>(bvh->tris_p(), bvh->perm_p(), int32_t(n_sorted), rays, n,
-                                                   id_offset, d_hit, d_t, ctx->d_err);
+    // grid-stride passes: the one of these that the device-side decision
+    // leaves idle costs a few workgroups, not one per 256 rays
+    const int grid = int(std::min<int64_t>(ceil_div(n, 256), int64_t(ctx->n_cu) * 32));
+    refine_kernel<<<grid, 256, 0, ctx->stream>>>(bvh->tris_p(), bvh->perm_p(), int32_t(n_sorted), rays, n, id_offset,
+                                                 d_hit, d_t, ctx->d_err, ro.ordered);
+    if (ro.ordered) {
+      refine_traced_kernel<<<grid, 256, 0, ctx->stream>>>(bvh->tris_p(), bvh->perm_p(), int32_t(n_sorted), ro.rays,
+                                                          n, id_offset, ro.hit, ro.t, ro.res, ctx->d_err, ro.ordered);
+      unpermute_kernel<<<grid, 256, 0, ctx->stream>>>(ro.inv, ro.res, n, d_hit, d_t, ro.ordered);
     }
     return TRACE_OK;
   }
