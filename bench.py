@@ -561,12 +561,19 @@ def load_traffic():
 def pmc_probe(torch, ctx, phr, d_rays, n, d_hit, d_t):
     """Workload for the two counter runs (FETCH_SIZE, WRITE_SIZE): a plain
     copy of COPY_BYTES (torch.mul by 1 into a second buffer, read once and
-    written once), then three closest-hit launches of the timed workload."""
+    written once), then PROBE_CALLS closest-hit calls of the timed workload."""
+    from trace_amd.device import get_tune, set_tune
+
+    # counter collection serialises and slows every dispatch, so the
+    # autotuner's pick in this process may differ from the timed run's: the
+    # probe uses the mode the timed run picks (persistent lanes, rays grouped
+    # by origin) with the workgroup size tuned here
+    set_tune(ctx, trav_threads=int(get_tune(ctx).trav_threads), trav_persistent=True, trav_order=True)
     src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
-    for _ in range(3):
+    for _ in range(PROBE_CALLS):
         phr.closest_hit_device(d_rays.data_ptr(), n, d_hit.data_ptr(), d_t.data_ptr())
     ctx.synchronize()
 
@@ -609,35 +616,55 @@ def _counter_rows(path):
     return [(k, c, v) for _, k, c, v in rows]
 
 
+# The kernels of one closest-hit call (traverse.hip): the work-counter fill,
+# the origin-order passes (probe, cell counts, their scan, the cell-major
+# copy), the persistent traversal, then the FP64 re-solve in input order or,
+# for an ordered batch, in traced order followed by the gather back.
+PROBE_CALLS = 3
+CALL_KERNELS = ("fill_kernel", "traverse::cell_probe_kernel", "traverse::cell_count_kernel", "scan::tile_sums",
+                "scan::scan_sums", "scan::tile_scan", "traverse::cell_scatter_kernel", "traverse::closest_hit",
+                "traverse::refine_kernel", "traverse::refine_traced_kernel", "traverse::unpermute_kernel")
+
+
 def traffic_from(fetch_csv, write_csv, hitmiss_csv=None):
     """HBM bytes per closest-hit call from the two counter runs, each scaled
-    so that the copy kernel reads / writes exactly COPY_BYTES: the traversal
-    kernel plus, after the persistent one, its FP64 re-solve pass; with a
-    third run of TCC_HIT_sum and TCC_MISS_sum, the traversal kernel's L2 hit
-    rate."""
+    so that the copy kernel reads / writes exactly COPY_BYTES: every dispatch
+    after the copy (the probe's PROBE_CALLS calls, CALL_KERNELS) summed and
+    divided by PROBE_CALLS; with a third run of TCC_HIT_sum and TCC_MISS_sum,
+    the traversal kernel's L2 hit rate."""
     out = {}
     for label, path in (("fetch", fetch_csv), ("write", write_csv)):
         rows = _counter_rows(path)
-        copy = [v for k, c, v in rows if "elementwise" in k.lower() or "mul" in k.lower()]
-        trav = [v for k, c, v in rows if "traverse::closest_hit" in k][-3:]  # the probe's launches
-        refine = [v for k, c, v in rows if "traverse::refine_kernel" in k][-3:]
-        if not copy or not trav:
-            raise SystemExit(f"{path}: copy or closest_hit_kernel dispatches not found")
-        copy_kib = max(copy)
+        copies = [i for i, (k, c, v) in enumerate(rows) if "elementwise" in k.lower() or "mul" in k.lower()]
+        if not copies:
+            raise SystemExit(f"{path}: copy kernel dispatch not found")
+        copy_kib = rows[copies[-1]][2]
+        per = {}
+        for k, c, v in rows[copies[-1] + 1:]:
+            if k.startswith("__amd_rocclr"):  # the runtime's own copies (error word, results)
+                continue
+            name = next((m for m in CALL_KERNELS if m in k), None)
+            if name is None:
+                raise SystemExit(f"{path}: unexpected dispatch {k} after the copy")
+            per[name] = per.get(name, 0.0) + v / PROBE_CALLS
+        if "traverse::closest_hit" not in per:
+            raise SystemExit(f"{path}: closest_hit dispatches not found")
         scale = (COPY_BYTES / 1024.0) / copy_kib
-        kib = float(np.mean(trav)) + (float(np.mean(refine)) if refine else 0.0)
-        out[label] = {"copy_kib": copy_kib, "scale": scale, "kernel_kib_raw": float(np.mean(trav)),
-                      "refine_kib_raw": float(np.mean(refine)) if refine else 0.0,
-                      "kernel_bytes": kib * 1024.0 * scale}
+        kib = sum(per.values())
+        out[label] = {"copy_kib": copy_kib, "scale": scale, "kib_raw_by_kernel": per,
+                      "kernel_bytes": kib * 1024.0 * scale,
+                      "traversal_bytes": per["traverse::closest_hit"] * 1024.0 * scale}
     total = out["fetch"]["kernel_bytes"] + out["write"]["kernel_bytes"]
     res = {"hbm_bytes_per_launch": round(total), "fetch_bytes": round(out["fetch"]["kernel_bytes"]),
-           "write_bytes": round(out["write"]["kernel_bytes"]), "fetch_scale": round(out["fetch"]["scale"], 4),
+           "write_bytes": round(out["write"]["kernel_bytes"]),
+           "traversal_kernel_bytes": round(out["fetch"]["traversal_bytes"] + out["write"]["traversal_bytes"]),
+           "fetch_scale": round(out["fetch"]["scale"], 4),
            "write_scale": round(out["write"]["scale"], 4), "copy_bytes": COPY_BYTES,
            "raw": out}
     if hitmiss_csv:
         rows = [(k, c, v) for k, c, v in _counter_rows(hitmiss_csv) if "traverse::closest_hit" in k]
-        hits = [v for k, c, v in rows if c.startswith("TCC_HIT")][-3:]
-        miss = [v for k, c, v in rows if c.startswith("TCC_MISS")][-3:]
+        hits = [v for k, c, v in rows if c.startswith("TCC_HIT")][-PROBE_CALLS:]
+        miss = [v for k, c, v in rows if c.startswith("TCC_MISS")][-PROBE_CALLS:]
         if hits and miss:
             res["l2_hit_rate"] = round(float(np.sum(hits) / (np.sum(hits) + np.sum(miss))), 4)
     with open(TRAFFIC_JSON, "w") as f:
@@ -762,10 +789,12 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic(),
-            # the persistent traversal hands its winners to a short FP64 re-solve pass;
-            # kernel_ms brackets both (HIP events around the call)
-            "kernel": ("cell_count + cell_scan + cell_scatter + " if trav_order else "")
-                      + ("closest_hit_persistent + refine_kernel" if trav_persistent else "closest_hit_kernel"),
+            # the persistent traversal hands its winners to a short FP64 re-solve pass,
+            # and an origin-ordered batch is copied cell-major first and its results
+            # gathered back after; kernel_ms brackets all of it (HIP events around the call)
+            "kernel": ("cell_probe + cell_count + scan + cell_scatter + closest_hit_persistent + refine_traced"
+                       " + unpermute" if trav_order else
+                       "closest_hit_persistent + refine_kernel" if trav_persistent else "closest_hit_kernel"),
             "kernel_ms": round(launch_ms, 4),
             "bytes_per_ray": round(bytes_launch / n, 1),
             "inner_nodes_per_ray": round(float(nodes.mean()), 2),
