@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -117,6 +118,8 @@ int trace_ctx_create(int device, trace_ctx** out) {
 
 int trace_ctx_destroy(trace_ctx* ctx) {
   if (!ctx) return TRACE_E_INVALID;
+  for (trace_ctx* w : ctx->workers) (void)trace_ctx_destroy(w);
+  ctx->workers.clear();
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
@@ -448,6 +451,22 @@ int trace_phr_build_with_cost(trace_ctx* ctx, const trace_bvh* aux, double alpha
   return TRACE_OK;
 }
 
+// Worker contexts for concurrent evaluations (at most kGridWorkers, one
+// hardware queue each on the default four-queue configuration).
+constexpr int kGridWorkers = 4;
+
+static int ensure_workers(trace_ctx* ctx, int n) {
+  while (int(ctx->workers.size()) < n) {
+    trace_ctx* w = nullptr;
+    const int rc = trace_ctx_create(ctx->device, &w);
+    if (rc) TRACE_FAIL(ctx, rc, "trace_phr_optimize_grid: could not create a worker context (%d)", rc);
+    w->tune = ctx->tune;
+    ctx->workers.push_back(w);
+  }
+  for (int i = 0; i < n; ++i) ctx->workers[i]->tune = ctx->tune;
+  return TRACE_OK;
+}
+
 int trace_phr_optimize_grid(trace_ctx* ctx, const trace_bvh* aux, const double* h_alphas, int32_t n_alpha,
                             const double* h_deltas, int32_t n_delta, int branching, int32_t max_cut,
                             int32_t frame_width, int32_t frame_height, double* h_sah, int64_t* h_build,
@@ -459,25 +478,54 @@ int trace_phr_optimize_grid(trace_ctx* ctx, const trace_bvh* aux, const double* 
   const int64_t n_eval = int64_t(n_alpha) * n_delta;
   std::vector<double> sah(n_eval);
   std::vector<int64_t> build(n_eval);
-  // every (alpha, delta), alphas outer (optimizer.go:47-61), each rebuilt
-  // into the storage of the first
-  trace_bvh* t = nullptr;
-  for (int32_t i = 0; i < n_alpha; ++i) {
-    for (int32_t j = 0; j < n_delta; ++j) {
-      const int64_t k = int64_t(i) * n_delta + j;
-      int rc = t ? trace_phr_rebuild(ctx, aux, h_alphas[i], h_deltas[j], branching, max_cut, t)
-                 : trace_phr_build(ctx, aux, h_alphas[i], h_deltas[j], branching, max_cut, &t);
-      if (rc == TRACE_OK) {
+  // every (alpha, delta), alphas outer (optimizer.go:47-61).  Evaluation k
+  // goes to worker k mod W, which rebuilds each of its candidates into the
+  // storage of its first: a PHR build's upper levels hold few jobs and its
+  // host reads counters between level groups, so W builds in flight on W
+  // streams fill the GPU where one does not.  Each result is the same as a
+  // sequential evaluation's (every build is deterministic).
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // aux is complete before other streams read it
+  const int nw = int(std::min<int64_t>(kGridWorkers, n_eval));
+  int rc = ensure_workers(ctx, nw);
+  if (rc) return rc;
+  std::vector<int> wrc(nw, TRACE_OK);
+  auto evaluate = [&](int w) {
+    trace_ctx* c = ctx->workers[w];
+    if (hipSetDevice(c->device) != hipSuccess) {
+      c->err = "trace_phr_optimize_grid: hipSetDevice failed in a worker thread";
+      wrc[w] = TRACE_E_HIP;
+      return;
+    }
+    trace_bvh* t = nullptr;
+    for (int64_t k = w; k < n_eval; k += nw) {
+      const double a = h_alphas[k / n_delta], d = h_deltas[k % n_delta];
+      int r = t ? trace_phr_rebuild(c, aux, a, d, branching, max_cut, t)
+                : trace_phr_build(c, aux, a, d, branching, max_cut, &t);
+      if (r == TRACE_OK) {
         build[k] = t->phr_initial_cut + t->phr_split_work;
-        rc = trace_bvh_cost(ctx, t, &sah[k]);
+        r = trace_bvh_cost(c, t, &sah[k]);
       }
-      if (rc) {
-        if (t) trace_bvh_destroy(t);
-        return rc;
+      if (r) {
+        wrc[w] = r;
+        break;
       }
     }
+    if (t) trace_bvh_destroy(t);
+  };
+  std::vector<std::thread> threads;
+  int started = 0;
+  try {
+    for (; started < nw; ++started) threads.emplace_back(evaluate, started);
+  } catch (...) {
+    // no thread to spare: the rest run on this one
   }
-  if (t) trace_bvh_destroy(t);
+  for (int w = started; w < nw; ++w) evaluate(w);
+  for (auto& th : threads) th.join();
+  for (int w = 0; w < nw; ++w)
+    if (wrc[w]) {
+      ctx->err = ctx->workers[w]->err;
+      return wrc[w];
+    }
   // scores (optimizer.go:63-94): build / max build + omega^2 * SAH / max SAH,
   // omega = (prims / PRIM_THRESH)^2 * pixels / RES_THRESH; the first minimum wins
   int64_t max_build = 0;

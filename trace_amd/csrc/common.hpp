@@ -53,6 +53,10 @@ struct trace_ctx {
   // set during trace_render_counts: the path-segment kernels sum their
   // wide-node visits and triangle tests here (two 64-bit words)
   unsigned long long* count_visits = nullptr;
+  // contexts of the same device (own stream, scratch and error word) on which
+  // the PHR grid search evaluates candidates concurrently; made on first use,
+  // destroyed with this context
+  std::vector<trace_ctx*> workers;
 };
 
 #define TRACE_FAIL(ctx, code, ...)                                \
