@@ -193,7 +193,9 @@ int trace_phr_build_with_cost(trace_ctx* ctx, const trace_bvh* aux, double alpha
  * (alpha, delta), alphas outer, then the (alpha, delta) of the first minimum of
  * build / max build + omega^2 * Cost / max Cost with omega = (prims / 2e6)^2 *
  * (frame_width * frame_height / 2e5).  h_sah and h_build (optional, n_alpha * n_delta)
- * receive every evaluation.  Synchronous. */
+ * receive every evaluation.  Synchronous: waits for ctx's stream, then evaluates the
+ * candidates on up to four worker contexts of the same device (made on first use, kept
+ * and destroyed with ctx), each on its own stream and host thread. */
 int trace_phr_optimize_grid(trace_ctx* ctx, const trace_bvh* aux, const double* h_alphas, int32_t n_alpha,
                             const double* h_deltas, int32_t n_delta, int branching, int32_t max_cut,
                             int32_t frame_width, int32_t frame_height, double* h_sah, int64_t* h_build,
