@@ -435,8 +435,8 @@ static void order_plan(int64_t n, ScratchPlan& plan, size_t* o) {
 
 // Enqueues the cell-major copy of n rays into scratch (made or skipped on the
 // device, see cell_probe_kernel); out stays empty (trace in input order) for
-// batches below kOrderMin and for calls that would have to grow the scratch
-// arena while graphs hold it.
+// batches below kOrderMin, for calls that would have to grow the scratch arena
+// while graphs hold it, and when the arena cannot grow.
 static int ray_order(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, int64_t n, RayOrder* out) {
   *out = RayOrder{};
   if (!ctx->tune.trav_order || n < kOrderMin) return TRACE_OK;
@@ -444,8 +444,12 @@ static int ray_order(trace_ctx* ctx, const trace_bvh* bvh, const float4* rays, i
   size_t o[10];
   order_plan(n, plan, o);
   if (plan.off > ctx->scratch_bytes && (ctx->capturing || ctx->live_graphs > 0)) return TRACE_OK;
-  const int rc = ctx_reserve_scratch(ctx, plan.off);
-  if (rc) return rc;
+  if (ctx_reserve_scratch(ctx, plan.off) != TRACE_OK) {
+    // no room for the copy (46 bytes a ray): trace in input order; the failed
+    // allocation is not this call's error
+    (void)hipGetLastError();
+    return TRACE_OK;
+  }
   char* base = static_cast<char*>(ctx->scratch);
   auto* sorted = reinterpret_cast<float4*>(base + o[0]);
   auto* inv = reinterpret_cast<uint32_t*>(base + o[1]);
