@@ -253,21 +253,25 @@ def test_inverse_rendering_lowers_the_loss(ctx, small):
     assert losses[-1] < 0.8 * losses[0], losses
 
 
-@pytest.mark.parametrize("zeros,miss,batch", [(False, _lib.MISS_DEFAULT, None), (True, _lib.MISS_SKY, None),
-                                               (True, _lib.MISS_SKY, 2 * W * H)])
-def test_kept_summaries_give_the_same_image_and_gradient(ctx, small, zeros, miss, batch):
+@pytest.mark.parametrize("zeros,miss,batch,order", [(False, _lib.MISS_DEFAULT, None, 1),
+                                                     (True, _lib.MISS_SKY, None, 1),
+                                                     (True, _lib.MISS_SKY, 2 * W * H, 1),
+                                                     (True, _lib.MISS_SKY, 2 * W * H, 2)])
+def test_kept_summaries_give_the_same_image_and_gradient(ctx, small, zeros, miss, batch, order):
     """trace_render_keep derives the image from the backward's summary sweep:
     it equals trace_render_device's bit for bit (zero albedo channels exercise
     the zero counts; the sky makes every escaping path radiant).  The backward
     from the kept summaries traces only the replay and gives the same gradient
     as the self-contained backward, up to the order of its atomic sums.  With
     a wavefront batch of 2 W H paths the 5 samples run as batches of 2 + 2 + 1,
-    so the keep buffer's sample-major offsets of the later batches are used."""
+    so the keep buffer's sample-major offsets of the later batches are used.
+    render_order 2 groups every queue after the camera's by origin cell, which
+    the default leaves to batches of 2^24 paths or more: the same bar holds."""
     from trace_amd.device import set_tune
 
-    print(f"adjoint: kept summaries zeros={zeros} miss={miss} batch={batch}", flush=True)
-    if batch is not None:
-        set_tune(ctx, render_batch=batch)
+    print(f"adjoint: kept summaries zeros={zeros} miss={miss} batch={batch} order={order}", flush=True)
+    if batch is not None or order != 1:
+        set_tune(ctx, render_batch=batch or (1 << 27), render_order=order)
     try:
         _kept_summaries_case(ctx, small, zeros, miss)
     finally:

@@ -152,6 +152,36 @@ def test_deterministic_and_progressive(ctx):
     assert np.array_equal(p2[2], a[2])
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_origin_ordered_queues_give_the_same_image(ctx, persistent):
+    """Grouping the queue of every segment after the camera's by ray-origin
+    cell (render_order 2; the default does it for single-scene batches of 2^24
+    paths or more) changes the order paths are traced in, not any path: the
+    image, its squares and the segment count equal the input-order render's
+    bit for bit, across several wavefront batches.  The persistent render has
+    no queue to group and takes the setting as a no-op."""
+    from trace_amd.device import render_counts, set_tune
+
+    print(f"render origin order persistent={persistent}", flush=True)
+    sc = scenes.box_scene(seed=5, slices=40, rings=24, width=96, height=64)
+    mesh, bvh = _setup(ctx, sc["pos"], sc["mat_idx"], sc["materials"])
+    out = []
+    try:
+        for order in (0, 2):
+            set_tune(ctx, render_batch=3 * 96 * 64, render_persistent=persistent, render_order=order)
+            img = render(ctx, mesh, bvh, sc["camera"], 96, 64, 7, 6, _lib.MISS_SKY, seed=11)
+            segs = render_counts(ctx, mesh, bvh, sc["camera"], 96, 64, 7, 6, _lib.MISS_SKY, 11)
+            out.append((img, segs))
+    finally:
+        set_tune(ctx)
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][1], out[1][1])
+    with pytest.raises(_lib.TraceError):
+        set_tune(ctx, render_order=3)
+    set_tune(ctx)
+
+
 def test_render_rejects_bad_arguments(ctx):
     sc = scenes.box_scene(seed=2, slices=8, rings=6, width=8, height=8)
     mesh = Mesh(ctx, sc["pos"])

@@ -137,6 +137,8 @@ class Tune(C.Structure):
         ("sort_bits", C.c_int32),
         ("sort_keys_per_thread", C.c_int32),
         ("trav_order", C.c_int32),
+        ("render_order", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 

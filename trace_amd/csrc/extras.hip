@@ -306,6 +306,7 @@ int trace_get_tune(trace_ctx* ctx, trace_tune* out) {
   out->sort_bits = ctx->tune.sort_bits;
   out->sort_keys_per_thread = ctx->tune.sort_kpt;
   out->trav_order = ctx->tune.trav_order ? 1 : 0;
+  out->render_order = ctx->tune.render_order;
   return TRACE_OK;
 }
 
@@ -315,6 +316,8 @@ int trace_set_tune(trace_ctx* ctx, const trace_tune* t) {
   auto ok_threads = [](int v) { return v == 64 || v == 128 || v == 256; };
   if (!ok_threads(t->trav_threads) || !ok_threads(t->render_threads) || t->render_batch < 1)
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_set_tune: workgroup sizes must be 64, 128 or 256");
+  if (t->render_order < 0 || t->render_order > 2)
+    TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_set_tune: render_order must be 0, 1 or 2");
   if (!onesweep::valid_config(onesweep::Config{t->sort_bits, t->sort_keys_per_thread}))
     TRACE_FAIL(ctx, TRACE_E_INVALID, "trace_set_tune: sort configuration (%d bits, %d keys per thread) unsupported",
                t->sort_bits, t->sort_keys_per_thread);
@@ -326,6 +329,7 @@ int trace_set_tune(trace_ctx* ctx, const trace_tune* t) {
   ctx->tune.sort_bits = t->sort_bits;
   ctx->tune.sort_kpt = t->sort_keys_per_thread;
   ctx->tune.trav_order = t->trav_order != 0;
+  ctx->tune.render_order = t->render_order;
   return TRACE_OK;
 }
 

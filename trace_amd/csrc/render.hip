@@ -20,6 +20,7 @@
 
 #include "philox.hpp"
 #include "render.hpp"
+#include "scan.hpp"
 #include "traverse_dev.hpp"
 
 namespace render {
@@ -638,6 +639,15 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 // shade_kernel shades slot i in thread i.  The traversal lanes never wait on
 // shading; the records cost 32 bytes of traffic per segment.
 constexpr uint32_t kSegChunk = 256;
+// Queue order (see qcell_count_kernel): 64 origin cells, tiles of 4096 slots,
+// by default for batches of at least kQOrderMin paths.
+constexpr int64_t kQOrderMin = int64_t(1) << 24;
+constexpr int kQCellBits = 2;
+constexpr int kQCells = 1 << (3 * kQCellBits);
+constexpr int kQThreads = 256;
+constexpr int kQWaves = kQThreads / 64;
+constexpr int kQPer = 16;
+constexpr int kQTile = kQThreads * kQPer;
 constexpr int kSegRefillAt = 16;
 
 // Hit record: triangle position (-1: miss), t with the sign of the facing
@@ -839,6 +849,11 @@ struct Wave {
   uint32_t* counts;
   int4* hits;   // hit record per queue slot (split segments only)
   int64_t spb;  // samples per batch
+  // queue order (render_order): the cell-major queue, slot cells, per-tile
+  // counts and their scan
+  int32_t* q_sort = nullptr;
+  uint8_t* q_cell = nullptr;
+  uint32_t *q_counts = nullptr, *q_offsets = nullptr, *q_total = nullptr, *q_scan = nullptr;
 };
 
 int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const trace_camera* cams,
@@ -867,6 +882,17 @@ int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const
   const size_t o_qb = plan.take(max_paths * sizeof(int32_t));
   const size_t o_cnt = plan.take(64 * sizeof(uint32_t));
   const size_t o_hit = ctx->tune.render_persistent ? plan.take(max_paths * sizeof(int4)) : 0;
+  // queue order: worth its three passes per segment where the tree outgrows
+  // the caches and the paths are many (one scene: the cells are one tree's)
+  const bool qorder = !ctx->tune.render_persistent && nscene == 1 &&
+                      (ctx->tune.render_order == 2 || (ctx->tune.render_order == 1 && max_paths >= kQOrderMin));
+  const int64_t qblocks = ceil_div(max_paths, int64_t(kQTile));
+  const size_t o_qs = qorder ? plan.take(max_paths * sizeof(int32_t)) : 0;
+  const size_t o_qc = qorder ? plan.take(max_paths * sizeof(uint8_t)) : 0;
+  const size_t o_qn = qorder ? plan.take(size_t(qblocks) * kQCells * sizeof(uint32_t)) : 0;
+  const size_t o_qo = qorder ? plan.take(size_t(qblocks) * kQCells * sizeof(uint32_t)) : 0;
+  const size_t o_qt = qorder ? plan.take(sizeof(uint32_t)) : 0;
+  const size_t o_qx = qorder ? plan.take(scan::scratch_words(qblocks * kQCells) * sizeof(uint32_t)) : 0;
   int rc = ctx_reserve_scratch(ctx, plan.off);
   if (rc) return rc;
   char* base = static_cast<char*>(ctx->scratch);
@@ -879,6 +905,14 @@ int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const
   w.counts = reinterpret_cast<uint32_t*>(base + o_cnt);
   w.hits = ctx->tune.render_persistent ? reinterpret_cast<int4*>(base + o_hit) : nullptr;
   w.scenes = reinterpret_cast<Scene*>(base + o_scn);
+  if (qorder) {
+    w.q_sort = reinterpret_cast<int32_t*>(base + o_qs);
+    w.q_cell = reinterpret_cast<uint8_t*>(base + o_qc);
+    w.q_counts = reinterpret_cast<uint32_t*>(base + o_qn);
+    w.q_offsets = reinterpret_cast<uint32_t*>(base + o_qo);
+    w.q_total = reinterpret_cast<uint32_t*>(base + o_qt);
+    w.q_scan = reinterpret_cast<uint32_t*>(base + o_qx);
+  }
 
   // Scene table: written with kernels (arguments by value), so a render stays
   // capturable in a HIP graph with no host-memory copy node.
@@ -898,25 +932,172 @@ int prepare(trace_ctx* ctx, const trace_bvh* const* trees, int32_t nscene, const
   return TRACE_OK;
 }
 
+// ---- queue order ---------------------------------------------------------------
+//
+// Before each segment after the camera's, the queued paths can be regrouped by
+// the cell of their ray origin (64 cells of the scene's root box, stable: a
+// cell's paths keep their queue order), so that the resident waves trace paths
+// that start near each other and share the nodes near them in the caches.  The
+// path state stays where it is (indexed by path); neighbouring pixels' hit
+// points mostly share a cell, so a cell's run of the queue keeps runs of
+// consecutive paths.  On the large configuration (10.38M triangles, a tree
+// larger than the Infinity Cache, 1920x1080x64 spp, depth 8) the render takes
+// 102.9 ms against 107.1 ms: the scattered segments are 6-11 % faster, for
+// 0.1-0.5 ms of ordering per segment.  Each path's result is the same in any
+// order.
+
+__device__ inline void q_box(const Scene* __restrict__ scenes, float* box) {
+  if (threadIdx.x < 3) {
+    const BvhHeader* h = scenes[0].hdr;
+    const float lo = h->root_lo[threadIdx.x], ext = h->root_hi[threadIdx.x] - lo;
+    box[threadIdx.x] = lo;
+    box[3 + threadIdx.x] = ext > 0.0f ? float(1 << kQCellBits) / ext : 0.0f;
+  }
+}
+
+__device__ inline uint32_t q_cell(const float* box, float4 o) {
+  const float oc[3] = {o.x, o.y, o.z};
+  uint32_t key = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const uint32_t q = uint32_t(fminf(fmaxf((oc[a] - box[a]) * box[3 + a], 0.0f), float((1 << kQCellBits) - 1)));
+#pragma unroll
+    for (int b = 0; b < kQCellBits; ++b) key |= ((q >> b) & 1u) << (3 * b + a);
+  }
+  return key;
+}
+
+// Per tile of kQTile queue slots: each slot's cell (cell[i]) and the tile's
+// count per cell (cell-major, tiles beyond the queue write zeros).
+__global__ __launch_bounds__(kQThreads) void qcell_count_kernel(const Scene* __restrict__ scenes,
+                                                               const int32_t* __restrict__ q,
+                                                               const uint32_t* __restrict__ count_in,
+                                                               const float4* __restrict__ ro, int32_t nblocks,
+                                                               uint8_t* __restrict__ cell,
+                                                               uint32_t* __restrict__ counts) {
+  __shared__ uint32_t h[kQCells];
+  __shared__ float box[6];
+  for (int c = threadIdx.x; c < kQCells; c += kQThreads) h[c] = 0u;
+  q_box(scenes, box);
+  __syncthreads();
+  const int64_t n = *count_in;
+  const int64_t base = int64_t(blockIdx.x) * kQTile;
+#pragma unroll
+  for (int k = 0; k < kQPer; ++k) {
+    const int64_t i = base + k * kQThreads + threadIdx.x;
+    if (i < n) {
+      const uint32_t c = q_cell(box, ro[q[i]]);
+      cell[i] = uint8_t(c);
+      atomicAdd(&h[c], 1u);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < kQCells; c += kQThreads) counts[int64_t(c) * nblocks + blockIdx.x] = h[c];
+}
+
+// The cell-major queue: a tile's slots ranked stably by cell (one ballot per
+// cell bit), staged in LDS in cell order and written as one run per cell.
+__global__ __launch_bounds__(kQThreads) void qcell_scatter_kernel(const int32_t* __restrict__ q,
+                                                                 const uint32_t* __restrict__ count_in,
+                                                                 const uint8_t* __restrict__ cell, int32_t nblocks,
+                                                                 const uint32_t* __restrict__ offsets,
+                                                                 int32_t* __restrict__ q_out) {
+  __shared__ int32_t sval[kQTile];
+  __shared__ uint8_t scell[kQTile];
+  __shared__ uint32_t slot[kQPer * kQWaves][kQCells];
+  __shared__ uint32_t cstart[kQCells];
+  __shared__ uint32_t coff[kQCells];
+  const int tid = threadIdx.x, wave = tid / 64;
+  const int64_t n = *count_in;
+  const int64_t base = int64_t(blockIdx.x) * kQTile;
+  if (base >= n) return;
+  for (int e = tid; e < kQPer * kQWaves * kQCells; e += kQThreads) (&slot[0][0])[e] = 0u;
+  __syncthreads();
+  const uint64_t lt = lanemask_lt();
+  uint32_t c[kQPer], r[kQPer];
+#pragma unroll
+  for (int k = 0; k < kQPer; ++k) {
+    const int64_t i = base + k * kQThreads + tid;
+    const bool valid = i < n;
+    c[k] = valid ? uint32_t(cell[i]) : 0u;
+    uint64_t m = ballot64(valid);
+#pragma unroll
+    for (int bit = 0; bit < 3 * kQCellBits; ++bit) {
+      const bool set = (c[k] >> bit) & 1u;
+      const uint64_t bb = ballot64(set);
+      m &= set ? bb : ~bb;
+    }
+    r[k] = uint32_t(__popcll(m & lt));
+    if (valid && (m & lt) == 0u) slot[k * kQWaves + wave][c[k]] = uint32_t(__popcll(m));
+  }
+  __syncthreads();
+  if (tid < kQCells) {
+    uint32_t run = 0;
+    for (int sl = 0; sl < kQPer * kQWaves; ++sl) {
+      const uint32_t v = slot[sl][tid];
+      slot[sl][tid] = run;
+      run += v;
+    }
+    cstart[tid] = run;
+    coff[tid] = offsets[int64_t(tid) * nblocks + blockIdx.x];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (int cc = 0; cc < kQCells; ++cc) {
+      const uint32_t v = cstart[cc];
+      cstart[cc] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kQPer; ++k) {
+    const int64_t i = base + k * kQThreads + tid;
+    if (i < n) {
+      const uint32_t p = cstart[c[k]] + slot[k * kQWaves + wave][c[k]] + r[k];
+      sval[p] = q[i];
+      scell[p] = uint8_t(c[k]);
+    }
+  }
+  __syncthreads();
+  const int valid_n = int(std::min<int64_t>(kQTile, n - base));
+  for (int p = tid; p < valid_n; p += kQThreads) {
+    const uint32_t cc = scell[p];
+    q_out[int64_t(coff[cc]) + (uint32_t(p) - cstart[cc])] = sval[p];
+  }
+}
+
 // Segments k = 0 .. max_depth + 1 of the paths queued in q[0] (count in counts[0]).
 void bounces(trace_ctx* ctx, const trace_mesh* mesh, const trace_bvh* bvh, const Wave& w,
              int64_t npaths, int64_t pps, int64_t path_base, const trace_render_params* prm,
              const Adj& adj) {
   hipStream_t st = ctx->stream;
+  const int32_t qblocks = int32_t(ceil_div(npaths, int64_t(kQTile)));
   for (int32_t k = 0; k <= prm->max_depth + 1; ++k) {
     const int in = k & 1, out = in ^ 1;
     fill_async<uint32_t>(st, w.counts + out, 1, 0u);
+    int32_t* qin = w.q[in];
+    if (w.q_sort && k >= 1) {
+      // the scattered rays of this segment, grouped by origin cell
+      qcell_count_kernel<<<qblocks, kQThreads, 0, st>>>(w.scenes, qin, w.counts + in, w.ro, qblocks, w.q_cell,
+                                                        w.q_counts);
+      scan::exclusive_u32(st, w.q_counts, w.q_offsets, int64_t(qblocks) * kQCells, w.q_total, w.q_scan);
+      qcell_scatter_kernel<<<qblocks, kQThreads, 0, st>>>(qin, w.counts + in, w.q_cell, qblocks, w.q_offsets,
+                                                          w.q_sort);
+      qin = w.q_sort;
+    }
     const int threads = ctx->count_visits ? 64 : ctx->tune.render_threads;
     const int grid = int(ceil_div(npaths, threads));
     if (threads == 64)
       launch_bounce<64>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
-                        w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, w.hits, adj);
+                        w.rad, qin, w.q[out], w.counts + in, w.counts + out, w.hits, adj);
     else if (threads == 256)
       launch_bounce<256>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
-                         w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, w.hits, adj);
+                         w.rad, qin, w.q[out], w.counts + in, w.counts + out, w.hits, adj);
     else
       launch_bounce<128>(ctx, mesh, bvh, grid, w.scenes, pps, k, prm, path_base, w.ro, w.rd, w.thr,
-                         w.rad, w.q[in], w.q[out], w.counts + in, w.counts + out, w.hits, adj);
+                         w.rad, qin, w.q[out], w.counts + in, w.counts + out, w.hits, adj);
   }
 }
 

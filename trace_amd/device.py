@@ -601,13 +601,14 @@ def get_tune(ctx: Context) -> "_lib.Tune":
 
 def set_tune(ctx: Context, trav_threads: int = 64, render_threads: int = 64, render_batch: int = 1 << 27,
              trav_persistent: bool = True, render_persistent: bool = False, sort_bits: int = 8,
-             sort_keys_per_thread: int = 16, trav_order: bool = True) -> None:
+             sort_keys_per_thread: int = 16, trav_order: bool = True, render_order: int = 1) -> None:
     t = _lib.Tune()
     t.trav_threads, t.render_threads, t.render_batch = trav_threads, render_threads, render_batch
     t.trav_persistent = 1 if trav_persistent else 0
     t.render_persistent = 1 if render_persistent else 0
     t.sort_bits, t.sort_keys_per_thread = sort_bits, sort_keys_per_thread
     t.trav_order = 1 if trav_order else 0
+    t.render_order = render_order
     ctx.check(ctx.lib.trace_set_tune(ctx.h, C.byref(t)))
 
 
