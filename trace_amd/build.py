@@ -1,8 +1,11 @@
-"""Builds the gfx950 shared library ``trace_amd/libtrace_hip.so`` with hipcc.
+"""Builds the gfx950 shared library ``trace_amd/libtrace_hip.so`` with hipcc,
+and the C++ host API over it, ``trace_amd/libtrace_pt.so``, with g++.
 
 Every ``.hip`` file under ``trace_amd/csrc`` is compiled to an object in
 parallel and linked into one shared library exporting the C-ABI of
-``include/trace_hip.h``.
+``include/trace_hip.h``.  ``trace_amd/pt/pt.cpp`` (the ``pkg/pt`` API of
+``include/trace_pt.hpp``) is host code only; it links against
+``libtrace_hip.so`` and finds it beside itself.
 """
 from __future__ import annotations
 
@@ -17,6 +20,9 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD_DIR = os.path.join(PKG, "_build")
 LIB_PATH = os.path.join(PKG, "libtrace_hip.so")
+PT_SRC = os.path.join(PKG, "pt", "pt.cpp")
+PT_LIB_PATH = os.path.join(PKG, "libtrace_pt.so")
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -63,7 +69,23 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB_PATH)
     if verbose:
         print(f"built {LIB_PATH} from {len(srcs)} sources", flush=True)
+    build_pt(verbose)
     return LIB_PATH
+
+
+def build_pt(verbose: bool = True) -> str:
+    """Compile the C++ host API into ``libtrace_pt.so`` (needs ``libtrace_hip.so``).
+    FP64 host arithmetic is kept uncontracted, as the reference's Go code is."""
+    tmp = PT_LIB_PATH + ".tmp"
+    cmd = [CXX, "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", "-ffp-contract=off", f"-I{INCLUDE}",
+           PT_SRC, "-o", tmp, f"-L{PKG}", "-ltrace_hip", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for {PT_SRC}:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, PT_LIB_PATH)
+    if verbose:
+        print(f"built {PT_LIB_PATH}", flush=True)
+    return PT_LIB_PATH
 
 
 if __name__ == "__main__":
