@@ -5,7 +5,8 @@ camera against closed forms, the buffers' merge rules, the OBJ reader on the
 reference's cube fixture, the no-device error) run on the CPU; its device cases
 (LBVH, PHR at branching 2 and 4, the optimisers, a scene graph rendered with the
 benchmark and default renderers, the heat map, error paths) run on the GPU, each
-checked against the FP64 oracle."""
+checked against the FP64 oracle.  A CPU check also reads the header's function
+declarations and finds each defined in libtrace_pt.so."""
 import os
 import subprocess
 
@@ -46,3 +47,50 @@ def test_pt_device_cases():
     res = _run("--gpu", 180)
     assert res.returncode == 0, res.stdout
     assert "FAIL" not in res.stdout
+
+
+def _declared_functions(header: str):
+    """(scope, name) of every function trace_pt.hpp declares without a body:
+    namespace-level functions (scope "") and members of its classes."""
+    import re
+
+    text = re.sub(r"//[^\n]*", "", open(header).read())
+    out, scopes, depth, stmt = set(), [], 0, ""
+    i = 0
+    while i < len(text):
+        ch = text[i]
+        if ch == "{":
+            m = re.search(r"\b(?:class|struct)\s+(\w+)[^;()]*$", stmt)
+            scopes.append(m.group(1) if m else None)
+            depth += 1
+            stmt = ""
+        elif ch == "}":
+            depth -= 1
+            scopes.pop()
+            stmt = ""
+        elif ch == ";":
+            s = " ".join(stmt.split())
+            stmt = ""
+            if "(" in s and not s.startswith(("using", "constexpr", "return")) and "=" not in s.split("(")[0]:
+                if not re.search(r"=\s*(default|0|delete)\s*$", s):
+                    name = re.findall(r"(~?\w+)\s*\(", s)[0]
+                    cls = [c for c in scopes if c and c != "pt"]
+                    if name != "operator" and (len(scopes) == 1 or (cls and len(scopes) == 2)):
+                        out.add(("" if len(scopes) == 1 else cls[-1], name))
+        else:
+            stmt += ch
+        i += 1
+    return out
+
+
+def test_pt_library_defines_every_declared_function():
+    lib = os.path.join(ROOT, "trace_amd", "libtrace_pt.so")
+    if not os.path.exists(lib):
+        pytest.skip("libtrace_pt.so is not built (run __graft_entry__.build())")
+    nm = subprocess.run(["nm", "-D", "--defined-only", "-C", lib], capture_output=True, text=True, check=True).stdout
+    declared = _declared_functions(os.path.join(ROOT, "include", "trace_pt.hpp"))
+    assert ("", "DefaultLBVH") in declared and ("PhrBuilder", "BuildFromAuxilary") in declared
+    assert ("ImageRenderer", "RenderToBuffer") in declared and len(declared) > 50
+    missing = [f"pt::{c + '::' if c else ''}{n}" for c, n in sorted(declared)
+               if f" pt::{c + '::' if c else ''}{n}(" not in nm]
+    assert not missing, missing
