@@ -477,7 +477,7 @@ typedef struct {
                          grouped by origin (512 cells of the tree's root box, one counting
                          pass; results still at each ray's index), 0 = in input order */
   int32_t render_order; /* path segments after the camera's, their queue grouped by ray-origin
-                           cell first: 0 never, 1 single-scene batches of 2^24 paths or more
+                           cell first: 0 never, 1 single-scene batches of 2^22 paths or more
                            (the default), 2 every single-scene batch; images are the same */
   int32_t pad;
 } trace_tune;

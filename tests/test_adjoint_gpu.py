@@ -266,7 +266,7 @@ def test_kept_summaries_give_the_same_image_and_gradient(ctx, small, zeros, miss
     a wavefront batch of 2 W H paths the 5 samples run as batches of 2 + 2 + 1,
     so the keep buffer's sample-major offsets of the later batches are used.
     render_order 2 groups every queue after the camera's by origin cell, which
-    the default leaves to batches of 2^24 paths or more: the same bar holds."""
+    the default leaves to batches of 2^22 paths or more: the same bar holds."""
     from trace_amd.device import set_tune
 
     print(f"adjoint: kept summaries zeros={zeros} miss={miss} batch={batch} order={order}", flush=True)

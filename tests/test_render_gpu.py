@@ -155,7 +155,7 @@ def test_deterministic_and_progressive(ctx):
 @pytest.mark.parametrize("persistent", [False, True])
 def test_origin_ordered_queues_give_the_same_image(ctx, persistent):
     """Grouping the queue of every segment after the camera's by ray-origin
-    cell (render_order 2; the default does it for single-scene batches of 2^24
+    cell (render_order 2; the default does it for single-scene batches of 2^22
     paths or more) changes the order paths are traced in, not any path: the
     image, its squares and the segment count equal the input-order render's
     bit for bit, across several wavefront batches.  The persistent render has

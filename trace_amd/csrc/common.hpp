@@ -27,7 +27,7 @@ struct TuneState {
   bool trav_order = true;               // persistent closest hit: rays grouped by origin cell first
   bool render_persistent = false;       // path segments: the same for the render's queues
   // path segments after the camera's, queue grouped by ray-origin cell: 0 never,
-  // 1 single-scene batches of at least 2^24 paths, 2 every single-scene batch
+  // 1 single-scene batches of at least 2^22 paths, 2 every single-scene batch
   int render_order = 1;
   int sort_bits = 8;                    // LBVH key sort: radix digit width (8, 10 or 11)
   int sort_kpt = 16;                    // LBVH key sort: keys per thread (tiles of 256 * kpt)
