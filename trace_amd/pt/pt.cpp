@@ -23,8 +23,10 @@ namespace pt {
 namespace {
 
 // One context per process; recursive so that a handle released while a call holds
-// the lock (an exception unwinding through it) can still take it.
-std::recursive_mutex g_mu;
+// the lock (an exception unwinding through it) can still take it.  Never destroyed,
+// so that handles held by the caller's own static objects can still be released
+// after this file's statics would have been.
+std::recursive_mutex& g_mu = *new std::recursive_mutex;
 trace_ctx* g_ctx = nullptr;
 int g_device = 0;
 
