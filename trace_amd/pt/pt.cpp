@@ -17,6 +17,7 @@
 #include <fstream>
 #include <mutex>
 #include <sstream>
+#include <unordered_map>
 
 namespace pt {
 
@@ -765,6 +766,7 @@ struct FlatGraph {
   std::vector<int32_t> parent, node_mesh;
   std::vector<double> matrix;
   std::vector<const Mesh*> meshes;
+  std::unordered_map<const Mesh*, int32_t> mesh_index;
 };
 
 void flatten(const SceneNode& n, int32_t parent, FlatGraph& g) {
@@ -773,12 +775,9 @@ void flatten(const SceneNode& n, int32_t parent, FlatGraph& g) {
   int32_t mi = -1;
   if (n.MeshOf() && !n.MeshOf()->geometry.empty()) {
     const Mesh* m = n.MeshOf().get();
-    for (size_t k = 0; k < g.meshes.size() && mi < 0; ++k)
-      if (g.meshes[k] == m) mi = int32_t(k);
-    if (mi < 0) {
-      mi = int32_t(g.meshes.size());
-      g.meshes.push_back(m);
-    }
+    const auto it = g.mesh_index.emplace(m, int32_t(g.meshes.size())).first;
+    mi = it->second;
+    if (mi == int32_t(g.meshes.size())) g.meshes.push_back(m);
   }
   g.node_mesh.push_back(mi);
   g.matrix.insert(g.matrix.end(), n.Transformation().begin(), n.Transformation().end());
