@@ -53,6 +53,7 @@ void oracle_render_ex(const oracle_tree* t, const double* nrm, const int32_t* ma
                       int32_t miss, uint64_t seed, int threads, double* sum, double* sumsq, double* last);
 void oracle_heatmap(const oracle_tree* t, const double* cam, int32_t W, int32_t H, int32_t threshold, float* rgb,
                     int32_t* counts, int threads);
+void oracle_traversal_steps(const oracle_tree* t, const double* rays, int64_t nrays, int32_t* counts, int threads);
 }
 
 namespace {
@@ -363,6 +364,19 @@ void case_lbvh() {
   std::vector<double> otv(rays.size());
   oracle_closest_hit(ot, rf.data(), int64_t(rays.size()), ohit.data(), otv.data(), kThreads);
   check_hits(bvh.ClosestHit(rays), ohit, otv);
+  // traversalSteps (bvh.go:21-44) of the first 4000 rays, counted as the reference
+  // counts them, with the reference's heat-map bounds (0.001, MaxFloat64)
+  const std::vector<pt::Ray> sub(rays.begin(), rays.begin() + 4000);
+  const double tmax = std::numeric_limits<double>::max();
+  std::vector<double> r64;
+  for (const auto& q : sub)
+    r64.insert(r64.end(), {q.Origin.X, q.Origin.Y, q.Origin.Z, 0.001, q.Direction.X, q.Direction.Y, q.Direction.Z,
+                           tmax});
+  std::vector<int32_t> osteps(sub.size());
+  oracle_traversal_steps(ot, r64.data(), int64_t(sub.size()), osteps.data(), kThreads);
+  const std::vector<int32_t> steps = bvh.TraversalSteps(sub, 0.001, tmax);
+  EXPECT(steps == osteps);
+  EXPECT(*std::max_element(steps.begin(), steps.end()) > 10);
   oracle_tree_free(ot);
 }
 
