@@ -550,11 +550,15 @@ double frac_beyond_3_sigma(const std::vector<double>& m_gpu, const std::vector<d
 }
 
 void case_scene_and_render(const std::string& golden) {
-  for (int specular = 0; specular < 2; ++specular) {
-    std::printf("    cube material %s\n", specular ? "Reflective" : "Diffuse");
+  const char* names[3] = {"Diffuse", "Reflective", "Refractive"};
+  for (int specular = 0; specular < 3; ++specular) {
+    std::printf("    cube material %s\n", names[specular]);
     std::fflush(stdout);
-    const pt::Material cube_mat = specular ? pt::Material(pt::Reflective{pt::NewColor(0.9, 0.8, 0.6), 0.2})
-                                           : pt::Material(pt::Diffuse{pt::NewColor(0.7, 0.3, 0.3)});
+    const pt::Material cube_mat = specular == 2   ? pt::Material(pt::Refractive{pt::NewColor(0.95, 0.95, 1.0), 1.5})
+                                  : specular == 1 ? pt::Material(pt::Reflective{pt::NewColor(0.9, 0.8, 0.6), 0.2})
+                                                  : pt::Material(pt::Diffuse{pt::NewColor(0.7, 0.3, 0.3)});
+    // the sun miss shader with the glass cube, the sky otherwise
+    const pt::MissShader miss = specular == 2 ? pt::SunMissShader : pt::SkyMissShader;
     Room room = make_room(golden, cube_mat);
     OracleScene o;
     flatten_oracle(room, o);
@@ -610,13 +614,13 @@ void case_scene_and_render(const std::string& golden) {
     auto r2 = pt::NewDefaultRenderer(bvh, camera);
     EXPECT(r2->Spp == 300 && r2->MaxDepth == 5 && r2->Miss == pt::DefaultMissShader);
     r2->Spp = 8;
-    r2->Miss = pt::SkyMissShader;
+    r2->Miss = miss;
     r2->Seed = 9;
     r2->RenderToBuffer(*pb);
     r2->RenderToBuffer(*pb);
     EXPECT(r2->SamplesDrawn() == 16 && pb->At(0, 0).Samples == 16 && pb->At(W - 1, H - 1).Samples == 16);
-    oracle_render_ex(o.tree, o.nrm.data(), o.mat.data(), room.mats8.data(), 3, cp, W, H, 16, 5, TRACE_MISS_SKY,
-                     9, kThreads, s.data(), q.data(), last.data());
+    oracle_render_ex(o.tree, o.nrm.data(), o.mat.data(), room.mats8.data(), 3, cp, W, H, 16, 5, int32_t(miss), 9,
+                     kThreads, s.data(), q.data(), last.data());
     std::vector<double> m(3 * W * H);
     for (int y = 0; y < H; ++y)
       for (int x = 0; x < W; ++x) {
